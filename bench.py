@@ -1,0 +1,168 @@
+"""Benchmark: GDOF/s of Formulation::assemble() on one MI355X.
+
+Workload (default): 3-D scalar Helmholtz, unit cube, 50^3 cubes x 6 jittered
+P2 tetrahedra (750,000 tets, 1,030,301 DOFs) + 30,000 P2 Robin boundary
+triangles, k = 10 pi, Gaussian source (sigma 0.05), complex128.  One step =
+one full assembly (geometry, source evaluation, CSR matrix values and RHS)
+with the mesh, pattern and maps resident in HBM.
+
+value = element-DOFs assembled per second (sum over elements of DOFs per
+element / assembly time) of the FP64 fused path; the mixed-precision path and
+the plain reference-order GPU assembler are reported beside it.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (os.path.join(ROOT, "fem-g_amd"), ROOT):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def timed_steps(F, steps, warmup, precision, strategy):
+    for _ in range(warmup):
+        F.assemble_async(precision, strategy)
+    F.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        F.assemble_async(precision, strategy)
+    F.sync()
+    return (time.perf_counter() - t0) / steps
+
+
+def kernel_profile(F, precision, strategy, reps=5):
+    """average duration of each kernel over `reps` assemblies (HIP events)."""
+    F.set_profiling(True)
+    acc = {}
+    for _ in range(reps):
+        F.assemble_async(precision, strategy)
+        F.sync()
+        for name, ms in F.kernel_times():
+            acc.setdefault(name, []).append(ms)
+    F.set_profiling(False)
+    return {k: sum(v) / len(v) * (len(v) / reps) / (len(v) / reps) for k, v in acc.items()}, acc
+
+
+def cpu_baseline(args):
+    """The oracle (single-threaded restatement of the reference CPU assembler)
+    on a bounded sample of the same workload: a smaller cube, same order,
+    same terms."""
+    from gfem import workloads as W
+    from oracle import oracle as O
+    n = args.cpu_n
+    spec = W.helmholtz3d(n=n, order=args.order)
+    P = O.from_spec(spec)
+    P.pre()
+    t0 = time.perf_counter()
+    P.assemble()
+    dt = time.perf_counter() - t0
+    edofs = spec.element_dofs()
+    return {"value": edofs / dt / 1e9, "unit": "GDOF/s", "cores": 1, "kind": "port",
+            "sample": f"oracle assemble() of the same Helmholtz P{args.order} workload on a {n}^3x6-tet cube "
+                      f"({edofs} element-DOFs, {dt:.2f} s, 1 thread)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=50)
+    ap.add_argument("--order", type=int, default=2)
+    ap.add_argument("--cpu-n", type=int, default=14)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the mixed / reference-strategy lines")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (this build runs on a single "
+              f"MI355X; multi-GPU sharding is a design note only)", file=sys.stderr)
+        sys.exit(2)
+
+    import math
+    from gfem import workloads as W
+
+    t0 = time.perf_counter()
+    spec = W.helmholtz3d(n=args.n, order=args.order)
+    t_mesh = time.perf_counter() - t0
+    log(f"mesh {spec.name}: {t_mesh:.2f} s")
+    F = W.build(spec)
+    t_pre = F.pre()
+    log(f"pre: {t_pre:.2f} s  n={F.n_unknown} nnz={F.nnz}")
+    edofs = spec.element_dofs()
+    t0 = time.perf_counter()
+    F.assemble(download=False)  # first call uploads the system to the device
+    log(f"first assemble (upload + maps): {time.perf_counter() - t0:.2f} s")
+
+    sec = timed_steps(F, args.steps, args.warmup, "fp64", "fused")
+    value = edofs / sec / 1e9
+    log(f"fused fp64: {sec * 1e3:.3f} ms/step  {value:.2f} GDOF/s")
+
+    prof, raw = kernel_profile(F, "fp64", "fused")
+    gather_ms = prof.get("gather", float("nan"))
+    gbytes = F.gather_bytes("fp64")
+    achieved = gbytes / (gather_ms * 1e-3) / 1e9
+    log("kernel ms (fp64 fused): " + ", ".join(f"{k}={v:.4f}" for k, v in prof.items()))
+
+    extra = {}
+    if not args.no_extra:
+        sec_m = timed_steps(F, args.steps, args.warmup, "mixed", "fused")
+        prof_m, _ = kernel_profile(F, "mixed", "fused")
+        gb_m = F.gather_bytes("mixed")
+        extra["mixed"] = {"value": edofs / sec_m / 1e9, "ms_per_step": sec_m * 1e3, "dtype": "c64 values, fp32 "
+                          "contraction, fp64 geometry",
+                          "gather_ms": prof_m.get("gather"),
+                          "gather_GBs": gb_m / (prof_m.get("gather", float("nan")) * 1e-3) / 1e9}
+        log(f"fused mixed: {sec_m * 1e3:.3f} ms/step  {edofs / sec_m / 1e9:.2f} GDOF/s")
+        ref_steps = max(2, args.steps // 4)
+        sec_r = timed_steps(F, ref_steps, 1, "fp64", "reference")
+        extra["reference_gpu_fp64"] = {"value": edofs / sec_r / 1e9, "ms_per_step": sec_r * 1e3,
+                                       "what": "per-Gauss-point dense element matrices + atomic CSR scatter"}
+        log(f"reference-order fp64: {sec_r * 1e3:.3f} ms/step  {edofs / sec_r / 1e9:.2f} GDOF/s")
+
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_baseline(args)
+            log(f"cpu baseline: {cpu['value']:.4f} GDOF/s")
+        except Exception as e:  # the oracle is optional at run time
+            log(f"cpu baseline skipped: {e}")
+
+    line = {
+        "metric": "GDOF/s assembled (FP64 vs mixed) at 1e-8 rel err; p50/p99 solve latency",
+        "value": value,
+        "unit": "GDOF/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": sec * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c128",
+        "data": "synthetic",
+        "config": {"workload": f"helmholtz3d unit cube {args.n}^3x6 jittered P{args.order} tets + Robin "
+                               f"P{args.order} boundary triangles, k=10pi, Gaussian source, complex128",
+                   "elements": sum(b["conn"].shape[0] for b in spec.model.blocks() if b["dim"] >= 2),
+                   "element_dofs": edofs, "n_unknown": int(F.n_unknown), "nnz": int(F.nnz),
+                   "pre_host_s": t_pre},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_gather<complex,fp64>", "kernel_ms": gather_ms,
+                     "algorithmic_bytes": gbytes},
+        "cpu_baseline": cpu,
+    }
+    line.update(extra)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,770 @@
+// C-ABI of the gfem device library: context, system, element groups, term
+// descriptors, map construction (finalize) and the assembly driver.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "gfem.h"
+#include "internal.h"
+#include "refelem.h"
+
+using namespace gfem;
+
+namespace {
+
+thread_local std::string g_err;
+
+gfem_status fail(gfem_ctx *ctx, gfem_status s, const std::string &msg)
+{
+  if(ctx) ctx->err = msg;
+  g_err = msg;
+  return s;
+}
+
+gfem_status hip_fail(gfem_ctx *ctx, hipError_t e, const char *what)
+{
+  const gfem_status s = (e == hipErrorOutOfMemory) ? GFEM_ERR_OUT_OF_MEMORY : GFEM_ERR_HIP;
+  return fail(ctx, s, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define GFEM_HIP(ctx, call)                                                                                           \
+  do {                                                                                                                \
+    hipError_t e_ = (call);                                                                                           \
+    if(e_ != hipSuccess) return hip_fail(ctx, e_, #call);                                                             \
+  } while(0)
+
+template <typename T>
+hipError_t dalloc(T **p, size_t count)
+{
+  *p = nullptr;
+  if(count == 0) count = 1;
+  return hipMalloc(reinterpret_cast<void **>(p), count * sizeof(T));
+}
+
+template <typename T>
+hipError_t dupload(T **p, const T *h, size_t count)
+{
+  hipError_t e = dalloc(p, count);
+  if(e != hipSuccess) return e;
+  if(count) e = hipMemcpy(*p, h, count * sizeof(T), hipMemcpyHostToDevice);
+  return e;
+}
+
+template <typename T>
+void dfree(T *&p)
+{
+  if(p) (void)hipFree(p);
+  p = nullptr;
+}
+
+void free_group(Group &g)
+{
+  dfree(g.d_vtx);
+  dfree(g.d_nodes);
+  dfree(g.d_terms);
+  dfree(g.d_F64);
+  dfree(g.d_F32);
+  dfree(g.d_T64);
+  dfree(g.d_T32);
+  dfree(g.d_dense);
+  dfree(g.d_be);
+  dfree(g.d_e2n);
+  dfree(g.d_num);
+  dfree(g.d_type);
+  for(auto &r : g.rules) {
+    dfree(r.d_w);
+    dfree(r.d_pts);
+    dfree(r.d_bf);
+    dfree(r.d_gbf);
+  }
+}
+
+bool is_row_type(int32_t t) { return t == GFEM_AT_UNKNOWN || t == GFEM_AT_UNKNOWN_BUBBLE; }
+
+// stable counting sort of (key, payload) records by key, keys in [0, nkeys)
+template <typename P>
+void bucket_by_key(int64_t nkeys, const std::vector<int64_t> &keys, const std::vector<P> &payload,
+                   std::vector<int64_t> &ptr, std::vector<P> &out)
+{
+  ptr.assign(nkeys + 1, 0);
+  for(int64_t k : keys) ptr[k + 1]++;
+  for(int64_t i = 0; i < nkeys; ++i) ptr[i + 1] += ptr[i];
+  std::vector<int64_t> pos(ptr.begin(), ptr.end() - 1);
+  out.resize(keys.size());
+  for(size_t i = 0; i < keys.size(); ++i) out[pos[keys[i]]++] = payload[i];
+}
+
+struct Prof {
+  gfem_system *sys;
+  hipStream_t s;
+  void begin(const char *name)
+  {
+    if(!sys->profiling) return;
+    hipEvent_t e;
+    (void)hipEventCreate(&e);
+    (void)hipEventRecord(e, s);
+    sys->prof_names.push_back(name);
+    sys->prof_ev.push_back(e);
+  }
+  void end()
+  {
+    if(!sys->profiling) return;
+    hipEvent_t e;
+    (void)hipEventCreate(&e);
+    (void)hipEventRecord(e, s);
+    sys->prof_ev.push_back(e);
+  }
+};
+
+void clear_prof(gfem_system *sys)
+{
+  for(auto e : sys->prof_ev) (void)hipEventDestroy(e);
+  sys->prof_ev.clear();
+  sys->prof_names.clear();
+  sys->prof_ms.clear();
+}
+
+size_t value_bytes(const gfem_system *sys, int prec) { return (prec ? 4 : 8) * (sys->cplx ? 2 : 1); }
+
+GroupTable make_table(const gfem_system *sys)
+{
+  GroupTable gt{};
+  gt.ng = (int)sys->groups.size();
+  gt.logP = sys->logP;
+  for(int g = 0; g < gt.ng; ++g) {
+    const Group &G = sys->groups[g];
+    gt.e_off[g] = G.e_off;
+    gt.N[g] = G.N;
+    gt.F64[g] = G.d_F64;
+    gt.F32[g] = G.d_F32;
+    gt.T64[g] = G.d_T64;
+    gt.T32[g] = G.d_T32;
+    gt.dense[g] = G.has_dense ? G.d_dense : nullptr;
+    gt.be[g] = G.has_linear ? G.d_be : nullptr;
+    gt.ne[g] = G.ne;
+  }
+  gt.e_off[gt.ng] = gt.ng ? sys->groups.back().e_off + sys->groups.back().ne : 0;
+  return gt;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *gfem_status_string(gfem_status s)
+{
+  switch(s) {
+  case GFEM_OK: return "ok";
+  case GFEM_ERR_INVALID_ARG: return "invalid argument";
+  case GFEM_ERR_NO_DEVICE: return "no HIP device";
+  case GFEM_ERR_HIP: return "HIP runtime error";
+  case GFEM_ERR_OUT_OF_MEMORY: return "device out of memory";
+  case GFEM_ERR_UNSUPPORTED: return "unsupported";
+  case GFEM_ERR_STATE: return "invalid call order";
+  case GFEM_ERR_NO_UNKNOWNS: return "no unknown dofs";
+  case GFEM_ERR_TOO_LARGE: return "index too large for 32-bit device maps";
+  case GFEM_ERR_NONAFFINE: return "non-affine element";
+  }
+  return "unknown status";
+}
+
+gfem_status gfem_ctx_create(int device, gfem_ctx **out)
+{
+  if(!out) return fail(nullptr, GFEM_ERR_INVALID_ARG, "out is null");
+  *out = nullptr;
+  int n = 0;
+  if(hipGetDeviceCount(&n) != hipSuccess || n == 0) return fail(nullptr, GFEM_ERR_NO_DEVICE, "no HIP device");
+  if(device < 0 || device >= n) return fail(nullptr, GFEM_ERR_NO_DEVICE, "device index out of range");
+  if(hipSetDevice(device) != hipSuccess) return fail(nullptr, GFEM_ERR_NO_DEVICE, "hipSetDevice failed");
+  gfem_ctx *ctx = new(std::nothrow) gfem_ctx();
+  if(!ctx) return fail(nullptr, GFEM_ERR_OUT_OF_MEMORY, "host allocation failed");
+  ctx->device = device;
+  *out = ctx;
+  return GFEM_OK;
+}
+
+gfem_status gfem_ctx_destroy(gfem_ctx *ctx)
+{
+  if(!ctx) return GFEM_OK;
+  (void)hipSetDevice(ctx->device);
+  dfree(ctx->d_xyz);
+  delete ctx;
+  return GFEM_OK;
+}
+
+const char *gfem_ctx_last_error(const gfem_ctx *ctx) { return ctx ? ctx->err.c_str() : g_err.c_str(); }
+
+gfem_status gfem_nodes_upload(gfem_ctx *ctx, int64_t n, const double *xyz)
+{
+  if(!ctx || n < 0 || (n > 0 && !xyz)) return fail(ctx, GFEM_ERR_INVALID_ARG, "nodes_upload: bad arguments");
+  GFEM_HIP(ctx, hipSetDevice(ctx->device));
+  dfree(ctx->d_xyz);
+  ctx->h_xyz.assign(xyz, xyz + 3 * n);
+  ctx->n_nodes = n;
+  GFEM_HIP(ctx, dupload(&ctx->d_xyz, xyz, (size_t)(3 * n)));
+  return GFEM_OK;
+}
+
+gfem_status gfem_system_create(gfem_ctx *ctx, gfem_scalar scalar, int64_t n, const int64_t *ai, const int64_t *aj,
+                               int symmetric, int64_t n_fixed, const double *values_dc, int n_rhs, double frequency,
+                               gfem_system **out)
+{
+  if(!ctx || !out || n < 0 || !ai || n_rhs < 1 || n_fixed < 0 || (n_fixed > 0 && !values_dc))
+    return fail(ctx, GFEM_ERR_INVALID_ARG, "system_create: bad arguments");
+  if(scalar < GFEM_SCALAR_F64 || scalar > GFEM_SCALAR_C64) return fail(ctx, GFEM_ERR_INVALID_ARG, "bad scalar");
+  *out = nullptr;
+  if(n == 0) return fail(ctx, GFEM_ERR_NO_UNKNOWNS, "there are no unknown dofs to assemble");
+  const int64_t nnz = ai[n];
+  if(nnz > 0 && !aj) return fail(ctx, GFEM_ERR_INVALID_ARG, "aj is null");
+  if(nnz >= (int64_t(1) << 31) || n >= (int64_t(1) << 31))
+    return fail(ctx, GFEM_ERR_TOO_LARGE, "pattern exceeds 2^31 entries");
+  GFEM_HIP(ctx, hipSetDevice(ctx->device));
+  gfem_system *sys = new(std::nothrow) gfem_system();
+  if(!sys) return fail(ctx, GFEM_ERR_OUT_OF_MEMORY, "host allocation failed");
+  sys->ctx = ctx;
+  sys->scalar = scalar;
+  sys->cplx = (scalar == GFEM_SCALAR_C128 || scalar == GFEM_SCALAR_C64);
+  sys->n = n;
+  sys->nnz = nnz;
+  sys->symmetric = symmetric;
+  sys->n_fixed = n_fixed;
+  sys->nrhs = n_rhs;
+  sys->frequency = frequency;
+  sys->h_dc.assign(values_dc ? values_dc : nullptr, values_dc ? values_dc + 2 * n_fixed : nullptr);
+  if(sys->h_dc.empty()) sys->h_dc.assign(2, 0.);
+  hipError_t e = dupload(&sys->d_dc, reinterpret_cast<const double2 *>(sys->h_dc.data()), sys->h_dc.size() / 2);
+  if(e != hipSuccess) {
+    delete sys;
+    return hip_fail(ctx, e, "upload values_dc");
+  }
+  *out = sys;
+  return GFEM_OK;
+}
+
+gfem_status gfem_system_destroy(gfem_system *sys)
+{
+  if(!sys) return GFEM_OK;
+  (void)hipSetDevice(sys->ctx->device);
+  clear_prof(sys);
+  for(auto &g : sys->groups) free_group(g);
+  dfree(sys->d_dc);
+  dfree(sys->d_words);
+  dfree(sys->d_chunk_c);
+  dfree(sys->d_chunk_n);
+  dfree(sys->d_lin_words);
+  dfree(sys->d_dc_words);
+  dfree(sys->d_dc_fix);
+  dfree(sys->d_lin_ptr);
+  dfree(sys->d_dc_ptr);
+  for(int p = 0; p < 2; ++p) {
+    if(sys->d_vals[p]) (void)hipFree(sys->d_vals[p]);
+    if(sys->d_rhs[p]) (void)hipFree(sys->d_rhs[p]);
+  }
+  delete sys;
+  return GFEM_OK;
+}
+
+gfem_status gfem_group_add(gfem_system *sys, int dim, int type, int64_t ne, const int64_t *conn,
+                           const int64_t *dof_num, const int32_t *dof_type, const int64_t *elem2nnz, int *group_out)
+{
+  if(!sys) return fail(nullptr, GFEM_ERR_INVALID_ARG, "system is null");
+  gfem_ctx *ctx = sys->ctx;
+  if(sys->finalized) return fail(ctx, GFEM_ERR_STATE, "group_add after finalize");
+  ElemType et;
+  if(!elem_type_info(type, et) || et.dim != dim || dim < 1)
+    return fail(ctx, GFEM_ERR_UNSUPPORTED, "element type " + std::to_string(type) + " not supported");
+  if(ne < 0 || (ne > 0 && (!conn || !dof_num || !dof_type)))
+    return fail(ctx, GFEM_ERR_INVALID_ARG, "group_add: bad arguments");
+  if((int)sys->groups.size() >= kMaxGroups)
+    return fail(ctx, GFEM_ERR_UNSUPPORTED, "more than " + std::to_string(kMaxGroups) + " element groups");
+  if(et.npe > kMaxN) return fail(ctx, GFEM_ERR_UNSUPPORTED, "too many dofs by element");
+  Group g;
+  g.dim = dim;
+  g.type = type;
+  g.order = et.order;
+  g.N = et.npe;
+  g.nv = dim + 1;
+  g.ne = ne;
+  g.e_off = sys->groups.empty() ? 0 : sys->groups.back().e_off + sys->groups.back().ne;
+  const int N = g.N;
+  // vertices + affinity check of the high-order nodes
+  const std::vector<Lat> lat = lattice_nodes(dim, et.order);
+  g.h_vtx.resize((size_t)(ne * g.nv));
+  const std::vector<double> &X = ctx->h_xyz;
+  for(int64_t e = 0; e < ne; ++e) {
+    for(int k = 0; k < N; ++k) {
+      const int64_t tag = conn[e * N + k];
+      if(tag < 1 || tag > ctx->n_nodes) return fail(ctx, GFEM_ERR_INVALID_ARG, "node tag out of range");
+    }
+    for(int v = 0; v < g.nv; ++v) g.h_vtx[e * g.nv + v] = (int32_t)(conn[e * N + v] - 1);
+    double h = 0.;
+    for(int v = 1; v < g.nv; ++v)
+      for(int i = 0; i < 3; ++i) h = std::max(h, std::fabs(X[3 * (conn[e * N + v] - 1) + i] - X[3 * (conn[e * N] - 1) + i]));
+    for(int k = g.nv; k < N; ++k) {
+      for(int i = 0; i < 3; ++i) {
+        double x = 0.;
+        for(int v = 0; v < g.nv; ++v) x += lat[k][v] * X[3 * (conn[e * N + v] - 1) + i];
+        x /= et.order;
+        if(std::fabs(x - X[3 * (conn[e * N + k] - 1) + i]) > 1e-9 * (h + 1e-300))
+          return fail(ctx, GFEM_ERR_NONAFFINE, "element " + std::to_string(e) + " has a curved high-order node");
+      }
+    }
+  }
+  g.h_num.assign(dof_num, dof_num + ne * N);
+  g.h_type.assign(dof_type, dof_type + ne * N);
+  if(elem2nnz) g.h_e2n.assign(elem2nnz, elem2nnz + ne * N * N);
+  sys->groups.push_back(std::move(g));
+  if(group_out) *group_out = (int)sys->groups.size() - 1;
+  return GFEM_OK;
+}
+
+gfem_status gfem_group_add_term(gfem_system *sys, int group, const gfem_term_desc *t)
+{
+  if(!sys || !t) return fail(sys ? sys->ctx : nullptr, GFEM_ERR_INVALID_ARG, "add_term: null argument");
+  gfem_ctx *ctx = sys->ctx;
+  if(sys->finalized) return fail(ctx, GFEM_ERR_STATE, "add_term after finalize");
+  if(group < 0 || group >= (int)sys->groups.size()) return fail(ctx, GFEM_ERR_INVALID_ARG, "bad group index");
+  Group &g = sys->groups[group];
+  if(t->kind != 0 && t->kind != 1) return fail(ctx, GFEM_ERR_INVALID_ARG, "term kind");
+  if(t->kind == 0 && !((t->form_dof == 0 && t->form_tf == 0) || (t->form_dof == 1 && t->form_tf == 1)))
+    return fail(ctx, GFEM_ERR_UNSUPPORTED, "bilinear forms must be (dof, tf) or (grad dof, grad tf)");
+  if(t->kind == 1 && t->form_tf != 0) return fail(ctx, GFEM_ERR_UNSUPPORTED, "linear terms take tf(u)");
+  if(t->kind == 1 && (t->rhs_idx < 0 || t->rhs_idx >= sys->nrhs)) return fail(ctx, GFEM_ERR_INVALID_ARG, "rhs_idx");
+  if(t->dt_order < 0 || t->dt_order > 2) return fail(ctx, GFEM_ERR_INVALID_ARG, "dt_order");
+  if(t->n_ops < 0 || t->n_ops > GFEM_MAX_OPS || t->n_consts < 0 || t->n_consts > GFEM_MAX_CONSTS ||
+     (t->n_ops > 0 && !t->ops) || (t->n_consts > 0 && !t->consts))
+    return fail(ctx, GFEM_ERR_INVALID_ARG, "coefficient program");
+  if(t->gauss < 0 || t->gauss > 2 * 16 - 1) return fail(ctx, GFEM_ERR_UNSUPPORTED, "gauss degree");
+  // validate the program: stack depth, opcodes, constant indices
+  int sp = 0;
+  for(int k = 0; k < t->n_ops; ++k) {
+    const int op = t->ops[2 * k], arg = t->ops[2 * k + 1];
+    if(op == GFEM_OP_CONST) {
+      if(arg < 0 || arg >= t->n_consts) return fail(ctx, GFEM_ERR_INVALID_ARG, "constant index");
+      ++sp;
+    }
+    else if(op >= GFEM_OP_X && op <= GFEM_OP_Z)
+      ++sp;
+    else if((op >= GFEM_OP_ADD && op <= GFEM_OP_DIV) || op == GFEM_OP_POW) {
+      if(sp < 2) return fail(ctx, GFEM_ERR_INVALID_ARG, "stack underflow");
+      --sp;
+    }
+    else if((op >= GFEM_OP_NEG && op <= GFEM_OP_COS) || op == GFEM_OP_ABS) {
+      if(sp < 1) return fail(ctx, GFEM_ERR_INVALID_ARG, "stack underflow");
+    }
+    else
+      return fail(ctx, GFEM_ERR_INVALID_ARG, "unknown opcode");
+    if(sp > GFEM_MAX_STACK) return fail(ctx, GFEM_ERR_INVALID_ARG, "stack overflow");
+  }
+  if(t->n_ops > 0 && sp != 1) return fail(ctx, GFEM_ERR_INVALID_ARG, "program must leave one value");
+  if(!sys->cplx && (t->coef_im != 0. || t->tf_coef_im != 0.))
+    return fail(ctx, GFEM_ERR_INVALID_ARG, "complex coefficient in a real formulation");
+  g.terms.push_back(*t);
+  g.term_ops.emplace_back(t->ops, t->ops + 2 * t->n_ops);
+  g.term_consts.emplace_back(t->consts, t->consts + 2 * t->n_consts);
+  if(t->kind == 0) g.has_bilinear = true;
+  if(t->kind == 1) g.has_linear = true;
+  return GFEM_OK;
+}
+
+gfem_status gfem_system_finalize(gfem_system *sys)
+{
+  if(!sys) return fail(nullptr, GFEM_ERR_INVALID_ARG, "system is null");
+  gfem_ctx *ctx = sys->ctx;
+  if(sys->finalized) return fail(ctx, GFEM_ERR_STATE, "finalize called twice");
+  if(!ctx->d_xyz) return fail(ctx, GFEM_ERR_STATE, "nodes not uploaded");
+  GFEM_HIP(ctx, hipSetDevice(ctx->device));
+  // module: AFrequency when a bilinear term carries a time derivative
+  // (problem/Formulation.cpp:1095-1149)
+  sys->freq_module = false;
+  for(auto &g : sys->groups)
+    for(auto &t : g.terms)
+      if(t.kind == 0 && t.dt_order != 0) sys->freq_module = true;
+  if(sys->freq_module && !sys->cplx)
+    return fail(ctx, GFEM_ERR_UNSUPPORTED,
+                "only a complex formulation can be assembled into 'Ax = b' with mass or damping terms");
+  int maxNN = 1;
+  for(auto &g : sys->groups) maxNN = std::max(maxNN, g.N * g.N);
+  int logP = 0;
+  while((1 << logP) < maxNN) ++logP;
+  sys->logP = logP;
+  const int64_t n_elem = sys->groups.empty() ? 0 : sys->groups.back().e_off + sys->groups.back().ne;
+  if((n_elem << logP) >= (int64_t(1) << 31))
+    return fail(ctx, GFEM_ERR_TOO_LARGE, "elements x local pairs exceed the 31-bit contribution word");
+  const double w = sys->frequency;
+  for(auto &g : sys->groups) {
+    if(g.terms.size() > 16) return fail(ctx, GFEM_ERR_UNSUPPORTED, "more than 16 terms on one element group");
+    // rules: one per distinct Gauss degree
+    g.dterms.clear();
+    for(size_t k = 0; k < g.terms.size(); ++k) {
+      const gfem_term_desc &t = g.terms[k];
+      int r = -1;
+      for(size_t q = 0; q < g.rules.size(); ++q)
+        if(g.rules[q].gauss == t.gauss) r = (int)q;
+      if(r < 0) {
+        if(g.rules.size() >= 8) return fail(ctx, GFEM_ERR_UNSUPPORTED, "more than 8 Gauss rules in a group");
+        Rule R;
+        R.gauss = t.gauss;
+        R.Q = gauss_rule(g.dim, t.gauss, R.pts, R.w);
+        g.rules.push_back(std::move(R));
+        r = (int)g.rules.size() - 1;
+      }
+      DevTerm d{};
+      d.kind = t.kind;
+      d.rule = r;
+      d.form_dof = t.form_dof;
+      d.form_tf = t.form_tf;
+      d.rhs_idx = t.rhs_idx;
+      d.per_point = t.n_ops > 0 ? 1 : 0;
+      d.c_re = t.coef_re;
+      d.c_im = t.coef_im;
+      // tf-side factor: conj(c_tf) for Hermitian products (BilinearTerm.cpp:324,
+      // LinearTerm.cpp:189), times -w^2 / -i w for dt^2 / dt terms
+      // in the AFrequency module (Assembler.cpp:285-296)
+      double fr = t.tf_coef_re, fi = t.hermitian ? -t.tf_coef_im : t.tf_coef_im;
+      if(t.kind == 0 && sys->freq_module && t.dt_order == 2) {
+        fr *= -w * w;
+        fi *= -w * w;
+      }
+      else if(t.kind == 0 && sys->freq_module && t.dt_order == 1) {
+        const double nr = fi * w, ni = -fr * w;  // (fr + i fi) * (-i w)
+        fr = nr;
+        fi = ni;
+      }
+      d.fac_re = fr;
+      d.fac_im = fi;
+      d.expr.n_ops = t.n_ops;
+      d.expr.n_consts = t.n_consts;
+      for(int q = 0; q < 2 * t.n_ops; ++q) d.expr.ops[q] = g.term_ops[k][q];
+      for(int q = 0; q < 2 * t.n_consts; ++q) d.expr.consts[q] = g.term_consts[k][q];
+      if(t.kind == 0 && d.per_point) g.has_dense = true;
+      g.dterms.push_back(d);
+    }
+    const std::vector<Lat> lat = lattice_nodes(g.dim, g.order);
+    std::vector<int> nodes(4 * lat.size());
+    for(size_t a = 0; a < lat.size(); ++a)
+      for(int k = 0; k < 4; ++k) nodes[4 * a + k] = lat[a][k];
+    GFEM_HIP(ctx, dupload(&g.d_nodes, nodes.data(), nodes.size()));
+    GFEM_HIP(ctx, dupload(&g.d_vtx, g.h_vtx.data(), g.h_vtx.size()));
+    GFEM_HIP(ctx, dupload(&g.d_terms, g.dterms.data(), g.dterms.size()));
+    GFEM_HIP(ctx, dalloc(&g.d_F64, (size_t)(g.ne * 8)));
+    GFEM_HIP(ctx, dalloc(&g.d_F32, (size_t)(g.ne * 8)));
+    GFEM_HIP(ctx, dalloc(&g.d_T64, (size_t)(g.N * g.N * 8)));
+    GFEM_HIP(ctx, dalloc(&g.d_T32, (size_t)(g.N * g.N * 8)));
+    if(g.has_dense) GFEM_HIP(ctx, dalloc(&g.d_dense, (size_t)(g.ne * g.N * g.N)));
+    if(g.has_linear) GFEM_HIP(ctx, dalloc(&g.d_be, (size_t)(sys->nrhs * g.ne * g.N)));
+    for(auto &R : g.rules) {
+      GFEM_HIP(ctx, dupload(&R.d_w, R.w.data(), R.w.size()));
+      GFEM_HIP(ctx, dupload(&R.d_pts, R.pts.data(), R.pts.size()));
+      GFEM_HIP(ctx, dalloc(&R.d_bf, (size_t)(R.Q * g.N)));
+      GFEM_HIP(ctx, dalloc(&R.d_gbf, (size_t)(R.Q * g.N * 3)));
+      GFEM_HIP(ctx, launch_basis(g, R, nullptr));
+    }
+    GFEM_HIP(ctx, launch_tensor(g, nullptr));
+  }
+
+  // nonzero-sorted element map (contribution words), traversal order kept
+  // inside each nonzero: group, element, local pair
+  {
+    std::vector<int64_t> cnt(sys->nnz + 1, 0);
+    int64_t total = 0;
+    for(auto &g : sys->groups) {
+      if(!g.has_bilinear) continue;
+      if(g.h_e2n.size() != (size_t)(g.ne * g.N * g.N))
+        return fail(ctx, GFEM_ERR_INVALID_ARG, "bilinear group without element->nonzero map");
+      for(int64_t v : g.h_e2n) {
+        if(v < -1 || v >= sys->nnz) return fail(ctx, GFEM_ERR_INVALID_ARG, "element->nonzero map out of range");
+        if(v >= 0) {
+          cnt[v + 1]++;
+          ++total;
+        }
+      }
+    }
+    for(int64_t i = 0; i < sys->nnz; ++i) {
+      if(cnt[i + 1] == 0) return fail(ctx, GFEM_ERR_INVALID_ARG, "a stored nonzero has no element contribution");
+      if(cnt[i + 1] > kChunk) return fail(ctx, GFEM_ERR_UNSUPPORTED, "a nonzero has more than 256 contributions");
+      cnt[i + 1] += cnt[i];
+    }
+    if(total >= (int64_t(1) << 32)) return fail(ctx, GFEM_ERR_TOO_LARGE, "too many contributions");
+    std::vector<uint32_t> words((size_t)total);
+    std::vector<int64_t> pos(cnt.begin(), cnt.end() - 1);
+    for(auto &g : sys->groups) {
+      if(!g.has_bilinear) continue;
+      const int NN = g.N * g.N;
+      for(int64_t e = 0; e < g.ne; ++e)
+        for(int p = 0; p < NN; ++p) {
+          const int64_t v = g.h_e2n[e * NN + p];
+          if(v < 0) continue;
+          const uint32_t wd = (uint32_t)(((g.e_off + e) << logP) | p);
+          const int64_t at = pos[v]++;
+          words[at] = wd | (at == cnt[v] ? 0x80000000u : 0u);
+        }
+    }
+    // chunks of whole nonzeros, at most kChunk contributions each
+    std::vector<uint32_t> cc, cn;
+    int64_t nz = 0;
+    while(nz < sys->nnz) {
+      cc.push_back((uint32_t)cnt[nz]);
+      cn.push_back((uint32_t)nz);
+      const int64_t start = cnt[nz];
+      while(nz < sys->nnz && cnt[nz + 1] - start <= kChunk) ++nz;
+    }
+    cc.push_back((uint32_t)total);
+    sys->n_chunks = (int64_t)cn.size();
+    sys->n_words = total;
+    GFEM_HIP(ctx, dupload(&sys->d_words, words.data(), words.size()));
+    GFEM_HIP(ctx, dupload(&sys->d_chunk_c, cc.data(), cc.size()));
+    GFEM_HIP(ctx, dupload(&sys->d_chunk_n, cn.data(), cn.size()));
+  }
+
+  // RHS maps: linear contributions (row <- element key) and Dirichlet lifting
+  // contributions (row <- element, transposed pair, fixed dof)
+  {
+    std::vector<int64_t> lin_key, dc_key;
+    std::vector<uint32_t> lin_pay, dc_pay;
+    std::vector<int32_t> dc_fixv;
+    std::vector<int64_t> dc_fix_key;
+    for(auto &g : sys->groups) {
+      const int N = g.N;
+      for(int64_t e = 0; e < g.ne; ++e) {
+        const int64_t eg = g.e_off + e;
+        for(int i = 0; i < N; ++i) {
+          const int32_t ti = g.h_type[e * N + i];
+          if(!is_row_type(ti)) continue;
+          const int64_t row = g.h_num[e * N + i];
+          if(row < 0 || row >= sys->n) return fail(ctx, GFEM_ERR_INVALID_ARG, "dof index out of range");
+          if(g.has_bilinear)
+            for(int j = 0; j < N; ++j) {
+              if(g.h_type[e * N + j] != GFEM_AT_FIXED) continue;
+              const int64_t fx = g.h_num[e * N + j];
+              if(fx < 0 || fx >= sys->n_fixed) return fail(ctx, GFEM_ERR_INVALID_ARG, "fixed dof index out of range");
+              dc_key.push_back(row);
+              dc_pay.push_back((uint32_t)((eg << logP) | (j * N + i)));
+              dc_fixv.push_back((int32_t)fx);
+            }
+          if(g.has_linear) {
+            if(eg >= (int64_t(1) << 26)) return fail(ctx, GFEM_ERR_TOO_LARGE, "too many elements for linear map");
+            lin_key.push_back(row);
+            lin_pay.push_back((uint32_t)((eg << 6) | i));
+          }
+        }
+      }
+    }
+    std::vector<int64_t> lptr, dptr;
+    std::vector<uint32_t> lw, dw;
+    std::vector<int32_t> df;
+    bucket_by_key(sys->n, lin_key, lin_pay, lptr, lw);
+    bucket_by_key(sys->n, dc_key, dc_pay, dptr, dw);
+    bucket_by_key(sys->n, dc_key, dc_fixv, dptr, df);
+    sys->n_lin = (int64_t)lw.size();
+    sys->n_dcw = (int64_t)dw.size();
+    GFEM_HIP(ctx, dupload(&sys->d_lin_ptr, lptr.data(), lptr.size()));
+    GFEM_HIP(ctx, dupload(&sys->d_lin_words, lw.data(), lw.size()));
+    GFEM_HIP(ctx, dupload(&sys->d_dc_ptr, dptr.data(), dptr.size()));
+    GFEM_HIP(ctx, dupload(&sys->d_dc_words, dw.data(), dw.size()));
+    GFEM_HIP(ctx, dupload(&sys->d_dc_fix, df.data(), df.size()));
+  }
+  GFEM_HIP(ctx, hipDeviceSynchronize());
+  sys->finalized = true;
+  return GFEM_OK;
+}
+
+static gfem_status ensure_outputs(gfem_system *sys, int prec)
+{
+  gfem_ctx *ctx = sys->ctx;
+  const size_t vb = value_bytes(sys, prec);
+  if(!sys->d_vals[prec]) GFEM_HIP(ctx, hipMalloc(&sys->d_vals[prec], vb * (size_t)std::max<int64_t>(sys->nnz, 1)));
+  if(!sys->d_rhs[prec]) GFEM_HIP(ctx, hipMalloc(&sys->d_rhs[prec], vb * (size_t)(sys->n * sys->nrhs)));
+  return GFEM_OK;
+}
+
+static gfem_status ensure_reference_maps(gfem_system *sys)
+{
+  if(sys->built_ref) return GFEM_OK;
+  gfem_ctx *ctx = sys->ctx;
+  for(auto &g : sys->groups) {
+    const int64_t NN = (int64_t)g.N * g.N;
+    std::vector<int32_t> num(g.h_num.size());
+    for(size_t i = 0; i < num.size(); ++i) num[i] = (int32_t)g.h_num[i];
+    GFEM_HIP(ctx, dupload(&g.d_num, num.data(), num.size()));
+    GFEM_HIP(ctx, dupload(&g.d_type, g.h_type.data(), g.h_type.size()));
+    if(g.has_bilinear) {
+      std::vector<int32_t> e2n(g.h_e2n.size());
+      for(size_t i = 0; i < e2n.size(); ++i) e2n[i] = (int32_t)g.h_e2n[i];
+      GFEM_HIP(ctx, dupload(&g.d_e2n, e2n.data(), e2n.size()));
+      if(!g.d_dense) GFEM_HIP(ctx, dalloc(&g.d_dense, (size_t)(g.ne * NN)));
+    }
+  }
+  sys->built_ref = true;
+  return GFEM_OK;
+}
+
+gfem_status gfem_assemble(gfem_system *sys, gfem_precision prec, gfem_strategy strategy, void *stream)
+{
+  if(!sys) return fail(nullptr, GFEM_ERR_INVALID_ARG, "system is null");
+  gfem_ctx *ctx = sys->ctx;
+  if(!sys->finalized) return fail(ctx, GFEM_ERR_STATE, "assemble before finalize");
+  if(prec != GFEM_PREC_FP64 && prec != GFEM_PREC_MIXED) return fail(ctx, GFEM_ERR_INVALID_ARG, "precision");
+  if(strategy != GFEM_STRATEGY_FUSED && strategy != GFEM_STRATEGY_REFERENCE)
+    return fail(ctx, GFEM_ERR_INVALID_ARG, "strategy");
+  if(strategy == GFEM_STRATEGY_REFERENCE && prec != GFEM_PREC_FP64)
+    return fail(ctx, GFEM_ERR_UNSUPPORTED, "the reference strategy is FP64 only");
+  GFEM_HIP(ctx, hipSetDevice(ctx->device));
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  const int p = (int)prec;
+  gfem_status st = ensure_outputs(sys, p);
+  if(st != GFEM_OK) return st;
+  clear_prof(sys);
+  Prof prof{sys, s};
+  if(strategy == GFEM_STRATEGY_FUSED) {
+    for(auto &g : sys->groups) {
+      prof.begin("geometry");
+      GFEM_HIP(ctx, launch_geometry(g, ctx->d_xyz, s));
+      prof.end();
+      if(g.has_linear) {
+        prof.begin("linear");
+        GFEM_HIP(ctx, launch_linear(g, ctx->d_xyz, sys->nrhs, s));
+        prof.end();
+      }
+      if(g.has_dense) {
+        prof.begin("dense");
+        GFEM_HIP(ctx, launch_dense(g, ctx->d_xyz, false, sys->freq_module, sys->frequency, sys->cplx, s));
+        prof.end();
+      }
+    }
+    const GroupTable gt = make_table(sys);
+    prof.begin("gather");
+    GFEM_HIP(ctx, launch_gather(*sys, gt, p, s));
+    prof.end();
+    prof.begin("rhs");
+    GFEM_HIP(ctx, launch_rhs(*sys, gt, p, s));
+    prof.end();
+    return GFEM_OK;
+  }
+  st = ensure_reference_maps(sys);
+  if(st != GFEM_OK) return st;
+  GFEM_HIP(ctx, hipMemsetAsync(sys->d_vals[0], 0, value_bytes(sys, 0) * (size_t)sys->nnz, s));
+  GFEM_HIP(ctx, hipMemsetAsync(sys->d_rhs[0], 0, value_bytes(sys, 0) * (size_t)(sys->n * sys->nrhs), s));
+  for(auto &g : sys->groups) {
+    prof.begin("geometry");
+    GFEM_HIP(ctx, launch_geometry(g, ctx->d_xyz, s));
+    prof.end();
+    if(g.has_linear) {
+      prof.begin("linear");
+      GFEM_HIP(ctx, launch_linear(g, ctx->d_xyz, sys->nrhs, s));
+      prof.end();
+    }
+    if(g.has_bilinear) {
+      prof.begin("ref_dense");
+      GFEM_HIP(ctx, launch_dense(g, ctx->d_xyz, true, sys->freq_module, sys->frequency, sys->cplx, s));
+      prof.end();
+      prof.begin("ref_scatter");
+      GFEM_HIP(ctx, launch_ref_scatter(*sys, g, s));
+      prof.end();
+    }
+    prof.begin("ref_rhs");
+    GFEM_HIP(ctx, launch_ref_rhs(*sys, g, s));
+    prof.end();
+  }
+  return GFEM_OK;
+}
+
+gfem_status gfem_stream_sync(gfem_ctx *ctx, void *stream)
+{
+  if(!ctx) return fail(nullptr, GFEM_ERR_INVALID_ARG, "ctx is null");
+  GFEM_HIP(ctx, hipStreamSynchronize(reinterpret_cast<hipStream_t>(stream)));
+  return GFEM_OK;
+}
+
+gfem_status gfem_system_info(const gfem_system *sys, gfem_precision prec, int64_t *nnz, int64_t *n, int *vb)
+{
+  if(!sys) return GFEM_ERR_INVALID_ARG;
+  if(nnz) *nnz = sys->nnz;
+  if(n) *n = sys->n;
+  if(vb) *vb = (int)value_bytes(sys, prec == GFEM_PREC_MIXED ? 1 : 0);
+  return GFEM_OK;
+}
+
+gfem_status gfem_values_download(gfem_system *sys, gfem_precision prec, void *host, size_t bytes)
+{
+  if(!sys || !host) return fail(sys ? sys->ctx : nullptr, GFEM_ERR_INVALID_ARG, "values_download: null");
+  const int p = prec == GFEM_PREC_MIXED ? 1 : 0;
+  if(!sys->d_vals[p]) return fail(sys->ctx, GFEM_ERR_STATE, "nothing assembled in this precision");
+  const size_t need = value_bytes(sys, p) * (size_t)sys->nnz;
+  if(bytes < need) return fail(sys->ctx, GFEM_ERR_INVALID_ARG, "host buffer too small");
+  GFEM_HIP(sys->ctx, hipSetDevice(sys->ctx->device));
+  GFEM_HIP(sys->ctx, hipMemcpy(host, sys->d_vals[p], need, hipMemcpyDeviceToHost));
+  return GFEM_OK;
+}
+
+gfem_status gfem_rhs_download(gfem_system *sys, gfem_precision prec, int idx, void *host, size_t bytes)
+{
+  if(!sys || !host) return fail(sys ? sys->ctx : nullptr, GFEM_ERR_INVALID_ARG, "rhs_download: null");
+  const int p = prec == GFEM_PREC_MIXED ? 1 : 0;
+  if(idx < 0 || idx >= sys->nrhs) return fail(sys->ctx, GFEM_ERR_INVALID_ARG, "rhs index");
+  if(!sys->d_rhs[p]) return fail(sys->ctx, GFEM_ERR_STATE, "nothing assembled in this precision");
+  const size_t vb = value_bytes(sys, p), need = vb * (size_t)sys->n;
+  if(bytes < need) return fail(sys->ctx, GFEM_ERR_INVALID_ARG, "host buffer too small");
+  GFEM_HIP(sys->ctx, hipSetDevice(sys->ctx->device));
+  GFEM_HIP(sys->ctx, hipMemcpy(host, static_cast<char *>(sys->d_rhs[p]) + need * idx, need, hipMemcpyDeviceToHost));
+  return GFEM_OK;
+}
+
+gfem_status gfem_values_device(gfem_system *sys, gfem_precision prec, void **dv, void **dr)
+{
+  if(!sys) return GFEM_ERR_INVALID_ARG;
+  const int p = prec == GFEM_PREC_MIXED ? 1 : 0;
+  if(dv) *dv = sys->d_vals[p];
+  if(dr) *dr = sys->d_rhs[p];
+  return GFEM_OK;
+}
+
+gfem_status gfem_set_profiling(gfem_system *sys, int on)
+{
+  if(!sys) return GFEM_ERR_INVALID_ARG;
+  sys->profiling = on != 0;
+  return GFEM_OK;
+}
+
+gfem_status gfem_last_kernel_times(const gfem_system *csys, int max, const char **names, double *ms, int *count)
+{
+  if(!csys || !count) return GFEM_ERR_INVALID_ARG;
+  gfem_system *sys = const_cast<gfem_system *>(csys);
+  const int n = (int)sys->prof_names.size();
+  if(sys->prof_ms.size() != (size_t)n) {
+    sys->prof_ms.assign(n, 0.);
+    for(int k = 0; k < n; ++k) {
+      if(hipEventSynchronize(sys->prof_ev[2 * k + 1]) != hipSuccess) return GFEM_ERR_HIP;
+      float t = 0.f;
+      if(hipEventElapsedTime(&t, sys->prof_ev[2 * k], sys->prof_ev[2 * k + 1]) != hipSuccess) return GFEM_ERR_HIP;
+      sys->prof_ms[k] = t;
+    }
+  }
+  *count = n;
+  for(int k = 0; k < n && k < max; ++k) {
+    if(names) names[k] = sys->prof_names[k];
+    if(ms) ms[k] = sys->prof_ms[k];
+  }
+  return GFEM_OK;
+}
+
+gfem_status gfem_gather_bytes(const gfem_system *sys, gfem_precision prec, double *bytes)
+{
+  if(!sys || !bytes) return GFEM_ERR_INVALID_ARG;
+  const int p = prec == GFEM_PREC_MIXED ? 1 : 0;
+  double b = 4.0 * (double)sys->n_words + 8.0 * (double)(sys->n_chunks + 1);
+  int64_t ne = 0;
+  for(auto &g : sys->groups)
+    if(g.has_bilinear) ne += g.ne;
+  b += (double)ne * 8.0 * (p ? 4.0 : 8.0);
+  b += (double)value_bytes(sys, p) * (double)sys->nnz;
+  *bytes = b;
+  return GFEM_OK;
+}
+
+}  // extern "C"

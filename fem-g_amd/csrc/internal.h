@@ -1,0 +1,160 @@
+// Internal types shared by the C-ABI layer (gfem_api.cpp) and the kernels
+// (kernels.hip).  Device layouts:
+//   node coordinates        xyz[n_nodes][3]                        double
+//   element vertices        vtx[ne][dim+1]                         int32 (0-based node index)
+//   element factors         F[ne][8] = det*G00,G11,G22,G01,G02,G12, det, 0
+//                           with G = J^-1 J^-T (FP64, and an FP32 copy)
+//   reference tensor        T[N*N][8] complex, pair = i_tf*N + j_dof, the
+//                           coefficient-weighted sum over the group's
+//                           constant-coefficient bilinear terms
+//   contribution words      uint32, sorted by CSR position: bit31 = first
+//                           contribution of its nonzero, bits 0..30 =
+//                           (global element slot << logP) | pair
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "gfem.h"
+
+namespace gfem {
+
+constexpr int kMaxGroups = 16;
+constexpr int kMaxN = 64;
+constexpr int kChunk = 256;  // contributions per gather block
+
+// Coefficient program as read by the kernels.
+struct DevExpr {
+  int n_ops;
+  int n_consts;
+  int ops[2 * GFEM_MAX_OPS];
+  double consts[2 * GFEM_MAX_CONSTS];
+};
+
+// One term as read by the kernels.
+struct DevTerm {
+  int kind;     // 0 bilinear, 1 linear
+  int rule;     // index into the group's rules
+  int form_dof;
+  int form_tf;
+  int rhs_idx;
+  int per_point;  // coefficient is an expression of x, y, z
+  double fac_re, fac_im;  // constant factor: tf-side coefficient (conj if Hermitian) * dt factor
+  double c_re, c_im;      // constant coefficient (dof side / source) when !per_point
+  DevExpr expr;
+};
+
+// Per-group tables passed by value to the gather / rhs kernels.
+struct GroupTable {
+  int ng;
+  int logP;
+  int64_t e_off[kMaxGroups + 1];  // global element slot offsets
+  int N[kMaxGroups];
+  const double *F64[kMaxGroups];   // [ne][8]
+  const float *F32[kMaxGroups];    // [ne][8]
+  const double2 *T64[kMaxGroups];  // [N*N][8]
+  const float2 *T32[kMaxGroups];   // [N*N][8]
+  const double2 *dense[kMaxGroups];  // [ne][N*N] per-point bilinear terms, or null
+  const double2 *be[kMaxGroups];     // [nrhs][ne][N] linear terms, or null
+  int64_t ne[kMaxGroups];
+};
+
+struct Rule {
+  int gauss = 0;
+  int Q = 0;
+  std::vector<double> w, pts;  // host
+  double *d_w = nullptr, *d_pts = nullptr;
+  double *d_bf = nullptr;   // [Q][N]
+  double *d_gbf = nullptr;  // [Q][N][3]
+};
+
+struct Group {
+  int dim = 0, type = 0, order = 0, N = 0, nv = 0;
+  int64_t ne = 0, e_off = 0;
+  std::vector<int32_t> h_vtx;
+  std::vector<int64_t> h_num;
+  std::vector<int32_t> h_type;
+  std::vector<int64_t> h_e2n;
+  std::vector<gfem_term_desc> terms;
+  std::vector<std::vector<int32_t>> term_ops;
+  std::vector<std::vector<double>> term_consts;
+  std::vector<Rule> rules;
+  std::vector<DevTerm> dterms;
+  bool has_bilinear = false, has_linear = false, has_dense = false;
+  int32_t *d_vtx = nullptr;
+  int *d_nodes = nullptr;  // reference lattice multi-indices [N][4]
+  DevTerm *d_terms = nullptr;
+  double *d_F64 = nullptr;
+  float *d_F32 = nullptr;
+  double2 *d_T64 = nullptr;
+  float2 *d_T32 = nullptr;
+  double2 *d_dense = nullptr;
+  double2 *d_be = nullptr;
+  // plain reference strategy
+  int32_t *d_e2n = nullptr;
+  int32_t *d_num = nullptr;
+  int32_t *d_type = nullptr;
+};
+
+}  // namespace gfem
+
+struct gfem_ctx {
+  int device = 0;
+  double *d_xyz = nullptr;
+  int64_t n_nodes = 0;
+  std::vector<double> h_xyz;
+  std::string err;
+};
+
+struct gfem_system {
+  gfem_ctx *ctx = nullptr;
+  gfem_scalar scalar = GFEM_SCALAR_F64;
+  bool cplx = false;
+  int64_t n = 0, nnz = 0, n_fixed = 0;
+  int symmetric = 0;
+  int nrhs = 1;
+  double frequency = 0.;
+  bool freq_module = false;
+  bool finalized = false;
+  int logP = 0;
+  std::vector<gfem::Group> groups;
+  std::vector<double> h_dc;
+  double2 *d_dc = nullptr;
+  // fused maps
+  uint32_t *d_words = nullptr;
+  int64_t n_words = 0;
+  uint32_t *d_chunk_c = nullptr, *d_chunk_n = nullptr;
+  int64_t n_chunks = 0;
+  uint32_t *d_lin_words = nullptr, *d_dc_words = nullptr;
+  int32_t *d_dc_fix = nullptr;
+  int64_t *d_lin_ptr = nullptr, *d_dc_ptr = nullptr;
+  int64_t n_lin = 0, n_dcw = 0;
+  // outputs per precision (0 FP64, 1 MIXED)
+  void *d_vals[2] = {nullptr, nullptr};
+  void *d_rhs[2] = {nullptr, nullptr};
+  bool built_ref = false;
+  // profiling
+  bool profiling = false;
+  std::vector<const char *> prof_names;
+  std::vector<hipEvent_t> prof_ev;
+  std::vector<double> prof_ms;
+};
+
+namespace gfem {
+
+// Launchers implemented in kernels.hip.  All are asynchronous on `s`.
+hipError_t launch_basis(const Group &g, Rule &r, hipStream_t s);
+hipError_t launch_tensor(const Group &g, hipStream_t s);
+hipError_t launch_geometry(const Group &g, const double *xyz, hipStream_t s);
+hipError_t launch_linear(const Group &g, const double *xyz, int nrhs, hipStream_t s);
+hipError_t launch_dense(const Group &g, const double *xyz, bool all_terms, bool freq_module, double freq,
+                        bool cplx, hipStream_t s);
+hipError_t launch_gather(const gfem_system &sys, const GroupTable &gt, int prec, hipStream_t s);
+hipError_t launch_rhs(const gfem_system &sys, const GroupTable &gt, int prec, hipStream_t s);
+hipError_t launch_ref_scatter(const gfem_system &sys, const Group &g, hipStream_t s);
+hipError_t launch_ref_rhs(const gfem_system &sys, const Group &g, hipStream_t s);
+
+}  // namespace gfem

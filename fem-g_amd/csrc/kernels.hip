@@ -1,0 +1,887 @@
+// CDNA4 (gfx950) kernels of the assembly hot path.
+//
+//   k_basis      Lagrange basis / reference gradients at Gauss points
+//                (gmsh getBasisFunctions, FunctionSpaceLagrange.cpp:114-137)
+//   k_tensor     coefficient-weighted reference tensors of the group's
+//                constant-coefficient bilinear terms (the quadrature sum of
+//                BilinearTerm::evaluate, BilinearTerm.cpp:315-325, with the
+//                affine J^-1 J^-T pulled out of the sum)
+//   k_geometry   per-element J, det, J^-1 J^-T in FP64 (gmsh getJacobians +
+//                FieldEvaluator<Form1>, FieldEvaluator.h:69)
+//   k_linear     per-element load vectors, per-Gauss-point source evaluation
+//                (LinearTerm::evaluate, LinearTerm.cpp:176-217)
+//   k_dense      per-Gauss-point element matrices (BilinearTerm::evaluate)
+//   k_gather     atomic-free CSR assembly: one block per chunk of the
+//                nonzero-sorted element map, segmented sum in element order
+//                (replaces the binary search + omp atomic of
+//                MatrixFactory::addValues, MatrixFactory.cpp:232-309)
+//   k_rhs        RHS rows: linear terms + Dirichlet lifting
+//                (VectorFactory::addValues / addValuesDC, VectorFactory.cpp:121-235)
+//   k_ref_*      the plain reference-order GPU assembler (dense element
+//                matrices + atomic scatter) used as the FP64 baseline.
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+
+#include "internal.h"
+
+namespace gfem {
+
+namespace {
+
+// ------------------------------------------------------------------ complex helpers
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 cmul(double2 a, double2 b)
+{
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cdiv(double2 a, double2 b)
+{
+  const double d = b.x * b.x + b.y * b.y;
+  return make_double2((a.x * b.x + a.y * b.y) / d, (a.y * b.x - a.x * b.y) / d);
+}
+__device__ __forceinline__ double2 cexp_(double2 a)
+{
+  const double m = exp(a.x);
+  double s, c;
+  sincos(a.y, &s, &c);
+  return make_double2(m * c, m * s);
+}
+__device__ __forceinline__ double2 clog_(double2 a) { return make_double2(log(hypot(a.x, a.y)), atan2(a.y, a.x)); }
+__device__ __forceinline__ double2 csqrt_(double2 a)
+{
+  const double r = hypot(a.x, a.y);
+  if(r == 0.) return make_double2(0., 0.);
+  const double re = sqrt(0.5 * (r + fabs(a.x)));
+  const double im = 0.5 * a.y / re;
+  return (a.x >= 0.) ? make_double2(re, im) : make_double2(fabs(im), a.y >= 0. ? re : -re);
+}
+
+// Coefficient program: complex stack machine (GFEM_OP_*).
+__device__ double2 eval_expr(const DevExpr &ex, double x, double y, double z)
+{
+  double2 st[GFEM_MAX_STACK];
+  int sp = 0;
+  for(int k = 0; k < ex.n_ops; ++k) {
+    const int op = ex.ops[2 * k], arg = ex.ops[2 * k + 1];
+    switch(op) {
+    case GFEM_OP_CONST: st[sp++] = make_double2(ex.consts[2 * arg], ex.consts[2 * arg + 1]); break;
+    case GFEM_OP_X: st[sp++] = make_double2(x, 0.); break;
+    case GFEM_OP_Y: st[sp++] = make_double2(y, 0.); break;
+    case GFEM_OP_Z: st[sp++] = make_double2(z, 0.); break;
+    case GFEM_OP_ADD: --sp; st[sp - 1] = cadd(st[sp - 1], st[sp]); break;
+    case GFEM_OP_SUB: --sp; st[sp - 1] = csub(st[sp - 1], st[sp]); break;
+    case GFEM_OP_MUL: --sp; st[sp - 1] = cmul(st[sp - 1], st[sp]); break;
+    case GFEM_OP_DIV: --sp; st[sp - 1] = cdiv(st[sp - 1], st[sp]); break;
+    case GFEM_OP_NEG: st[sp - 1] = make_double2(-st[sp - 1].x, -st[sp - 1].y); break;
+    case GFEM_OP_EXP: st[sp - 1] = cexp_(st[sp - 1]); break;
+    case GFEM_OP_SQRT: st[sp - 1] = csqrt_(st[sp - 1]); break;
+    case GFEM_OP_SIN: {
+      const double2 a = st[sp - 1];
+      double s, c;
+      sincos(a.x, &s, &c);
+      st[sp - 1] = make_double2(s * cosh(a.y), c * sinh(a.y));
+      break;
+    }
+    case GFEM_OP_COS: {
+      const double2 a = st[sp - 1];
+      double s, c;
+      sincos(a.x, &s, &c);
+      st[sp - 1] = make_double2(c * cosh(a.y), -s * sinh(a.y));
+      break;
+    }
+    case GFEM_OP_POW: {
+      --sp;
+      const double2 a = st[sp - 1], b = st[sp];
+      if(a.y == 0. && b.y == 0. && (a.x > 0. || b.x == floor(b.x)))
+        st[sp - 1] = make_double2(pow(a.x, b.x), 0.);
+      else
+        st[sp - 1] = (a.x == 0. && a.y == 0.) ? make_double2(0., 0.) : cexp_(cmul(b, clog_(a)));
+      break;
+    }
+    case GFEM_OP_ABS: st[sp - 1] = make_double2(hypot(st[sp - 1].x, st[sp - 1].y), 0.); break;
+    default: break;
+    }
+  }
+  return sp > 0 ? st[sp - 1] : make_double2(0., 0.);
+}
+
+// Affine element geometry from its vertices: J columns (col-major
+// J(i,j) = dx_i/du_j on the gmsh reference element), det, Kinv = J^-T
+// (row-major).  1-D/2-D elements get unit normals as missing columns and
+// det = length/area element, as gmsh getJacobians does.
+struct Geo {
+  double c[3][3];  // c[j] = column j of J
+  double det;
+  double Ki[3][3];
+};
+
+__device__ __forceinline__ void cross(const double *a, const double *b, double *r)
+{
+  r[0] = a[1] * b[2] - a[2] * b[1];
+  r[1] = a[2] * b[0] - a[0] * b[2];
+  r[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+__device__ void element_geometry(int dim, const double X[4][3], Geo &g)
+{
+  for(int j = 0; j < 3; ++j)
+    for(int i = 0; i < 3; ++i) g.c[j][i] = 0.;
+  if(dim == 3) {
+    for(int j = 0; j < 3; ++j)
+      for(int i = 0; i < 3; ++i) g.c[j][i] = X[j + 1][i] - X[0][i];
+    double n[3];
+    cross(g.c[1], g.c[2], n);
+    g.det = g.c[0][0] * n[0] + g.c[0][1] * n[1] + g.c[0][2] * n[2];
+  }
+  else if(dim == 2) {
+    for(int j = 0; j < 2; ++j)
+      for(int i = 0; i < 3; ++i) g.c[j][i] = X[j + 1][i] - X[0][i];
+    double n[3];
+    cross(g.c[0], g.c[1], n);
+    const double a = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+    for(int i = 0; i < 3; ++i) g.c[2][i] = n[i] / a;
+    g.det = a;
+  }
+  else {
+    for(int i = 0; i < 3; ++i) g.c[0][i] = 0.5 * (X[1][i] - X[0][i]);
+    const double a = sqrt(g.c[0][0] * g.c[0][0] + g.c[0][1] * g.c[0][1] + g.c[0][2] * g.c[0][2]);
+    double t[3] = {g.c[0][0] / a, g.c[0][1] / a, g.c[0][2] / a};
+    double z[3] = {0., 0., 1.};
+    if(fabs(t[2]) > 0.9) {
+      z[0] = 1.;
+      z[2] = 0.;
+    }
+    double n1[3], n2[3];
+    cross(z, t, n1);
+    const double l1 = sqrt(n1[0] * n1[0] + n1[1] * n1[1] + n1[2] * n1[2]);
+    for(int i = 0; i < 3; ++i) n1[i] /= l1;
+    cross(t, n1, n2);
+    for(int i = 0; i < 3; ++i) {
+      g.c[1][i] = n1[i];
+      g.c[2][i] = n2[i];
+    }
+    g.det = a;
+  }
+  // A = J^T: A[i][j] = c[i][j]; Ki = A^-1 by cofactors
+  const double(*A)[3] = g.c;
+  const double c00 = A[1][1] * A[2][2] - A[1][2] * A[2][1];
+  const double c01 = A[1][2] * A[2][0] - A[1][0] * A[2][2];
+  const double c02 = A[1][0] * A[2][1] - A[1][1] * A[2][0];
+  const double d = A[0][0] * c00 + A[0][1] * c01 + A[0][2] * c02;
+  const double id = 1. / d;
+  g.Ki[0][0] = c00 * id;
+  g.Ki[1][0] = c01 * id;
+  g.Ki[2][0] = c02 * id;
+  g.Ki[0][1] = (A[0][2] * A[2][1] - A[0][1] * A[2][2]) * id;
+  g.Ki[1][1] = (A[0][0] * A[2][2] - A[0][2] * A[2][0]) * id;
+  g.Ki[2][1] = (A[0][1] * A[2][0] - A[0][0] * A[2][1]) * id;
+  g.Ki[0][2] = (A[0][1] * A[1][2] - A[0][2] * A[1][1]) * id;
+  g.Ki[1][2] = (A[0][2] * A[1][0] - A[0][0] * A[1][2]) * id;
+  g.Ki[2][2] = (A[0][0] * A[1][1] - A[0][1] * A[1][0]) * id;
+}
+
+__device__ __forceinline__ void load_vertices(const int32_t *__restrict__ vtx, int nv, int64_t e,
+                                              const double *__restrict__ xyz, double X[4][3])
+{
+  for(int k = 0; k < 4; ++k) {
+    if(k < nv) {
+      const int64_t n = vtx[e * nv + k];
+      X[k][0] = xyz[3 * n];
+      X[k][1] = xyz[3 * n + 1];
+      X[k][2] = xyz[3 * n + 2];
+    }
+    else {
+      X[k][0] = X[k][1] = X[k][2] = 0.;
+    }
+  }
+}
+
+// physical point of reference point u (affine map)
+__device__ __forceinline__ void phys_point(int dim, const double X[4][3], const double *u, double *x)
+{
+  if(dim == 1) {
+    const double t = 0.5 * (u[0] + 1.);
+    for(int i = 0; i < 3; ++i) x[i] = X[0][i] + t * (X[1][i] - X[0][i]);
+    return;
+  }
+  for(int i = 0; i < 3; ++i) {
+    double s = X[0][i];
+    for(int j = 0; j < dim; ++j) s += u[j] * (X[j + 1][i] - X[0][i]);
+    x[i] = s;
+  }
+}
+
+// ------------------------------------------------------------------ k_basis
+// Lagrange basis on the gmsh reference simplex, lattice node a with
+// barycentric multi-index m: phi = prod_k prod_{l<m_k} (p lam_k - l)/(l+1).
+__global__ void k_basis(int dim, int p, int N, int Q, const int *__restrict__ nodes, const double *__restrict__ pts,
+                        double *__restrict__ bf, double *__restrict__ gbf)
+{
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if(t >= N * Q) return;
+  const int q = t / N, a = t % N;
+  const double *u = &pts[3 * q];
+  double lam[4] = {0., 0., 0., 0.}, dlam[4][3] = {{0.}};
+  if(dim == 1) {
+    lam[0] = 0.5 * (1. - u[0]);
+    lam[1] = 0.5 * (1. + u[0]);
+    dlam[0][0] = -0.5;
+    dlam[1][0] = 0.5;
+  }
+  else {
+    double s = 0.;
+    for(int j = 0; j < dim; ++j) {
+      lam[j + 1] = u[j];
+      dlam[j + 1][j] = 1.;
+      dlam[0][j] = -1.;
+      s += u[j];
+    }
+    lam[0] = 1. - s;
+  }
+  double f = 1., g[3] = {0., 0., 0.};
+  for(int k = 0; k <= dim; ++k) {
+    const int mk = nodes[4 * a + k];
+    for(int l = 0; l < mk; ++l) {
+      const double fac = (p * lam[k] - l) / (l + 1);
+      const double dfac = (double)p / (l + 1);
+      for(int j = 0; j < 3; ++j) g[j] = g[j] * fac + f * dfac * dlam[k][j];
+      f *= fac;
+    }
+  }
+  bf[q * N + a] = f;
+  for(int j = 0; j < 3; ++j) gbf[(q * N + a) * 3 + j] = g[j];
+}
+
+// ------------------------------------------------------------------ k_tensor
+struct TensorArgs {
+  int N;
+  int nterms;
+  int form_dof[16], form_tf[16], rule[16], active[16];
+  double2 fac[16];  // full constant factor of the term
+  int Q[8];
+  const double *w[8];
+  const double *bf[8];
+  const double *gbf[8];
+};
+
+// T[pair][c], pair = i_tf*N + j_dof:
+//   c=0..2  sum_q w dphi_j/du_c dphi_i/du_c      c=3..5 symmetric cross terms
+//   c=6     sum_q w phi_j phi_i
+// weighted by each term's constant factor and summed over terms.
+__global__ void k_tensor(TensorArgs a, double2 *__restrict__ T64, float2 *__restrict__ T32)
+{
+  const int pair = blockIdx.x * blockDim.x + threadIdx.x;
+  if(pair >= a.N * a.N) return;
+  const int i = pair / a.N, j = pair % a.N;
+  double2 acc[8];
+  for(int c = 0; c < 8; ++c) acc[c] = make_double2(0., 0.);
+  for(int t = 0; t < a.nterms; ++t) {
+    if(!a.active[t]) continue;
+    const int r = a.rule[t];
+    double s[7] = {0., 0., 0., 0., 0., 0., 0.};
+    for(int q = 0; q < a.Q[r]; ++q) {
+      const double w = a.w[r][q];
+      if(a.form_dof[t] == 1) {
+        const double *gj = &a.gbf[r][(q * a.N + j) * 3];
+        const double *gi = &a.gbf[r][(q * a.N + i) * 3];
+        s[0] += w * gj[0] * gi[0];
+        s[1] += w * gj[1] * gi[1];
+        s[2] += w * gj[2] * gi[2];
+        s[3] += w * (gj[0] * gi[1] + gj[1] * gi[0]);
+        s[4] += w * (gj[0] * gi[2] + gj[2] * gi[0]);
+        s[5] += w * (gj[1] * gi[2] + gj[2] * gi[1]);
+      }
+      else {
+        s[6] += w * a.bf[r][q * a.N + j] * a.bf[r][q * a.N + i];
+      }
+    }
+    for(int c = 0; c < 7; ++c) acc[c] = cadd(acc[c], make_double2(a.fac[t].x * s[c], a.fac[t].y * s[c]));
+  }
+  for(int c = 0; c < 8; ++c) {
+    T64[pair * 8 + c] = acc[c];
+    T32[pair * 8 + c] = make_float2((float)acc[c].x, (float)acc[c].y);
+  }
+}
+
+// ------------------------------------------------------------------ k_geometry
+__global__ void __launch_bounds__(256) k_geometry(int dim, int nv, int64_t ne, const int32_t *__restrict__ vtx,
+                                                  const double *__restrict__ xyz, double *__restrict__ F64,
+                                                  float *__restrict__ F32)
+{
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(e >= ne) return;
+  double X[4][3];
+  load_vertices(vtx, nv, e, xyz, X);
+  Geo g;
+  element_geometry(dim, X, g);
+  double G[3][3];
+  for(int a = 0; a < 3; ++a)
+    for(int b = 0; b < 3; ++b) G[a][b] = g.Ki[0][a] * g.Ki[0][b] + g.Ki[1][a] * g.Ki[1][b] + g.Ki[2][a] * g.Ki[2][b];
+  const double d = g.det;
+  const double f[8] = {d * G[0][0], d * G[1][1], d * G[2][2], d * G[0][1], d * G[0][2], d * G[1][2], d, 0.};
+  double2 *o2 = reinterpret_cast<double2 *>(&F64[e * 8]);
+  float4 *o4 = reinterpret_cast<float4 *>(&F32[e * 8]);
+  for(int k = 0; k < 4; ++k) o2[k] = make_double2(f[2 * k], f[2 * k + 1]);
+  o4[0] = make_float4((float)f[0], (float)f[1], (float)f[2], (float)f[3]);
+  o4[1] = make_float4((float)f[4], (float)f[5], (float)f[6], 0.f);
+}
+
+// ------------------------------------------------------------------ k_linear
+struct LinearArgs {
+  int dim, nv, N, nterms, rhs;
+  const DevTerm *terms;
+  int Q[8];
+  const double *w[8];
+  const double *pts[8];
+  const double *bf[8];
+};
+
+template <int N>
+__global__ void __launch_bounds__(128) k_linear(LinearArgs a, int64_t ne, const int32_t *__restrict__ vtx,
+                                                const double *__restrict__ xyz, const double *__restrict__ F64,
+                                                double2 *__restrict__ be)
+{
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(e >= ne) return;
+  double X[4][3];
+  load_vertices(vtx, a.nv, e, xyz, X);
+  const double det = F64[e * 8 + 6];
+  double2 acc[N];
+  for(int i = 0; i < N; ++i) acc[i] = make_double2(0., 0.);
+  for(int t = 0; t < a.nterms; ++t) {
+    const DevTerm &dt = a.terms[t];
+    if(dt.kind != 1 || dt.rhs_idx != a.rhs) continue;
+    const int r = dt.rule;
+    for(int q = 0; q < a.Q[r]; ++q) {
+      double2 f;
+      if(dt.per_point) {
+        double x[3];
+        phys_point(a.dim, X, &a.pts[r][3 * q], x);
+        f = eval_expr(dt.expr, x[0], x[1], x[2]);
+      }
+      else {
+        f = make_double2(dt.c_re, dt.c_im);
+      }
+      const double wd = a.w[r][q] * det;
+      const double2 cf = cmul(make_double2(dt.fac_re * wd, dt.fac_im * wd), f);
+#pragma unroll
+      for(int i = 0; i < N; ++i) {
+        const double b = a.bf[r][q * N + i];
+        acc[i].x += cf.x * b;
+        acc[i].y += cf.y * b;
+      }
+    }
+  }
+#pragma unroll
+  for(int i = 0; i < N; ++i) be[e * N + i] = acc[i];
+}
+
+// ------------------------------------------------------------------ k_dense
+struct DenseArgs {
+  int dim, nv, N, nterms, all_terms, freq_module, cplx;
+  double freq;
+  const DevTerm *terms;
+  int Q[8];
+  const double *w[8];
+  const double *pts[8];
+  const double *bf[8];
+  const double *gbf[8];
+};
+
+// one thread per (element, pair): the reference's per-Gauss-point loop
+// A_e(j,i) += w det (c E_j) . conj(c_tf E_i)
+__global__ void __launch_bounds__(256) k_dense(DenseArgs a, int64_t ne, const int32_t *__restrict__ vtx,
+                                               const double *__restrict__ xyz, double2 *__restrict__ Ae)
+{
+  const int NN = a.N * a.N;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(t >= ne * NN) return;
+  const int64_t e = t / NN;
+  const int pair = (int)(t % NN);
+  const int i = pair / a.N, j = pair % a.N;
+  double X[4][3];
+  load_vertices(vtx, a.nv, e, xyz, X);
+  Geo g;
+  element_geometry(a.dim, X, g);
+  double2 acc = make_double2(0., 0.);
+  for(int k = 0; k < a.nterms; ++k) {
+    const DevTerm &dt = a.terms[k];
+    if(dt.kind != 0) continue;
+    if(!a.all_terms && !dt.per_point) continue;
+    const int r = dt.rule;
+    double2 sum = make_double2(0., 0.);
+    for(int q = 0; q < a.Q[r]; ++q) {
+      double s;
+      if(dt.form_dof == 1) {
+        const double *gj = &a.gbf[r][(q * a.N + j) * 3];
+        const double *gi = &a.gbf[r][(q * a.N + i) * 3];
+        s = 0.;
+        for(int c = 0; c < 3; ++c) {
+          const double ej = g.Ki[c][0] * gj[0] + g.Ki[c][1] * gj[1] + g.Ki[c][2] * gj[2];
+          const double ei = g.Ki[c][0] * gi[0] + g.Ki[c][1] * gi[1] + g.Ki[c][2] * gi[2];
+          s += ej * ei;
+        }
+      }
+      else {
+        s = a.bf[r][q * a.N + j] * a.bf[r][q * a.N + i];
+      }
+      double2 c;
+      if(dt.per_point) {
+        double x[3];
+        phys_point(a.dim, X, &a.pts[r][3 * q], x);
+        c = eval_expr(dt.expr, x[0], x[1], x[2]);
+      }
+      else {
+        c = make_double2(dt.c_re, dt.c_im);
+      }
+      const double wd = a.w[r][q] * g.det * s;
+      sum.x += wd * c.x;
+      sum.y += wd * c.y;
+    }
+    acc = cadd(acc, cmul(make_double2(dt.fac_re, dt.fac_im), sum));
+  }
+  Ae[e * NN + pair] = acc;
+}
+
+// ------------------------------------------------------------------ element value
+__device__ __forceinline__ int find_group(const GroupTable &gt, int64_t eg)
+{
+  int g = 0;
+  while(g + 1 < gt.ng && eg >= gt.e_off[g + 1]) ++g;
+  return g;
+}
+
+// value of local entry `pair` of global element slot `eg`
+template <bool CPLX>
+__device__ __forceinline__ double2 elem_value64(const GroupTable &gt, int64_t eg, int pair)
+{
+  const int g = find_group(gt, eg);
+  const int64_t el = eg - gt.e_off[g];
+  const double2 *F = reinterpret_cast<const double2 *>(gt.F64[g] + el * 8);
+  const double2 *T = gt.T64[g] + (int64_t)pair * 8;
+  double2 v = make_double2(0., 0.);
+#pragma unroll
+  for(int k = 0; k < 4; ++k) {
+    const double2 f = F[k];
+    const double2 t0 = T[2 * k], t1 = T[2 * k + 1];
+    v.x += f.x * t0.x + f.y * t1.x;
+    if(CPLX) v.y += f.x * t0.y + f.y * t1.y;
+  }
+  if(gt.dense[g]) {
+    const int N = gt.N[g];
+    const double2 d = gt.dense[g][el * N * N + pair];
+    v.x += d.x;
+    v.y += d.y;
+  }
+  return v;
+}
+
+template <bool CPLX>
+__device__ __forceinline__ float2 elem_value32(const GroupTable &gt, int64_t eg, int pair)
+{
+  const int g = find_group(gt, eg);
+  const int64_t el = eg - gt.e_off[g];
+  const float4 *F = reinterpret_cast<const float4 *>(gt.F32[g] + el * 8);
+  const float4 *T = reinterpret_cast<const float4 *>(gt.T32[g] + (int64_t)pair * 8);
+  float2 v = make_float2(0.f, 0.f);
+#pragma unroll
+  for(int k = 0; k < 2; ++k) {
+    const float4 f = F[k];
+    const float4 ta = T[2 * k], tb = T[2 * k + 1];
+    v.x += f.x * ta.x + f.y * ta.z + f.z * tb.x + f.w * tb.z;
+    if(CPLX) v.y += f.x * ta.y + f.y * ta.w + f.z * tb.y + f.w * tb.w;
+  }
+  if(gt.dense[g]) {
+    const int N = gt.N[g];
+    const double2 d = gt.dense[g][el * N * N + pair];
+    v.x += (float)d.x;
+    v.y += (float)d.y;
+  }
+  return v;
+}
+
+template <typename V>
+struct ValTraits;
+template <>
+struct ValTraits<double> {
+  static constexpr bool cplx = false;
+  __device__ static double make(double2 v) { return v.x; }
+};
+template <>
+struct ValTraits<double2> {
+  static constexpr bool cplx = true;
+  __device__ static double2 make(double2 v) { return v; }
+};
+
+// ------------------------------------------------------------------ k_gather
+// One 256-thread block per chunk: contributions [cs, ce) covering whole
+// nonzeros [ns, ...).  Thread t evaluates contribution cs+t; the first
+// thread of each nonzero sums its segment in element order and writes it.
+template <bool CPLX, bool MIXED>
+__global__ void __launch_bounds__(kChunk) k_gather(GroupTable gt, const uint32_t *__restrict__ words,
+                                                   const uint32_t *__restrict__ chunk_c,
+                                                   const uint32_t *__restrict__ chunk_n, void *__restrict__ out)
+{
+  using R = typename std::conditional<MIXED, float, double>::type;
+  __shared__ R s_re[kChunk];
+  __shared__ R s_im[CPLX ? kChunk : 1];
+  __shared__ uint8_t s_head[kChunk + 1];
+  __shared__ int s_wave[kChunk / 64];
+  const int64_t c = blockIdx.x;
+  const uint32_t cs = chunk_c[c], ce = chunk_c[c + 1], ns = chunk_n[c];
+  const int t = threadIdx.x;
+  const int len = (int)(ce - cs);
+  const bool valid = t < len;
+  bool head = false;
+  R vre = 0, vim = 0;
+  if(valid) {
+    const uint32_t w = words[cs + t];
+    head = (w >> 31) != 0;
+    const uint32_t flat = w & 0x7fffffffu;
+    const int64_t eg = flat >> gt.logP;
+    const int pair = (int)(flat & ((1u << gt.logP) - 1u));
+    if(MIXED) {
+      const float2 v = elem_value32<CPLX>(gt, eg, pair);
+      vre = v.x;
+      vim = v.y;
+    }
+    else {
+      const double2 v = elem_value64<CPLX>(gt, eg, pair);
+      vre = v.x;
+      vim = v.y;
+    }
+  }
+  s_re[t] = vre;
+  if(CPLX) s_im[t] = vim;
+  s_head[t] = head ? 1 : 0;
+  if(t == 0) s_head[kChunk] = 1;
+  const unsigned long long m = __ballot(head);
+  const int lane = t & 63, wv = t >> 6;
+  const int before = __popcll(m & ((1ull << lane) - 1ull));
+  if(lane == 0) s_wave[wv] = __popcll(m);
+  __syncthreads();
+  if(!head) return;
+  int off = before;
+  for(int k = 0; k < wv; ++k) off += s_wave[k];
+  R sre = vre, sim = vim;
+  for(int k = t + 1; k < len && !s_head[k]; ++k) {
+    sre += s_re[k];
+    if(CPLX) sim += s_im[k];
+  }
+  const int64_t pos = (int64_t)ns + off;
+  if(CPLX) {
+    if(MIXED)
+      reinterpret_cast<float2 *>(out)[pos] = make_float2((float)sre, (float)sim);
+    else
+      reinterpret_cast<double2 *>(out)[pos] = make_double2((double)sre, (double)sim);
+  }
+  else {
+    reinterpret_cast<R *>(out)[pos] = sre;
+  }
+}
+
+// ------------------------------------------------------------------ k_rhs
+// One thread per row: Dirichlet lifting A_e(i_dof, j_tf) u_D for fixed
+// columns (transposed local index, VectorFactory.cpp:184), then the element
+// load vectors, each in element order.  Sums in FP64.
+template <bool CPLX, bool MIXED>
+__global__ void __launch_bounds__(256) k_rhs(GroupTable gt, int64_t n, int nrhs, const int64_t *__restrict__ dc_ptr,
+                                             const uint32_t *__restrict__ dc_words, const int32_t *__restrict__ dc_fix,
+                                             const double2 *__restrict__ uD, const int64_t *__restrict__ lin_ptr,
+                                             const uint32_t *__restrict__ lin_words, void *__restrict__ out)
+{
+  const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(row >= n) return;
+  double2 lift = make_double2(0., 0.);
+  for(int64_t k = dc_ptr[row]; k < dc_ptr[row + 1]; ++k) {
+    const uint32_t w = dc_words[k];
+    const int64_t eg = w >> gt.logP;
+    const int pair = (int)(w & ((1u << gt.logP) - 1u));
+    double2 a;
+    if(MIXED) {
+      const float2 v = elem_value32<CPLX>(gt, eg, pair);
+      a = make_double2(v.x, v.y);
+    }
+    else {
+      a = elem_value64<CPLX>(gt, eg, pair);
+    }
+    lift = cadd(lift, cmul(a, uD[dc_fix[k]]));
+  }
+  for(int r = 0; r < nrhs; ++r) {
+    double2 acc = lift;
+    for(int64_t k = lin_ptr[row]; k < lin_ptr[row + 1]; ++k) {
+      const uint32_t w = lin_words[k];
+      const int64_t eg = w >> 6;
+      const int il = (int)(w & 63u);
+      const int g = find_group(gt, eg);
+      if(!gt.be[g]) continue;
+      const int64_t el = eg - gt.e_off[g];
+      const double2 b = gt.be[g][((int64_t)r * gt.ne[g] + el) * gt.N[g] + il];
+      acc.x += b.x;
+      acc.y += b.y;
+    }
+    const int64_t pos = (int64_t)r * n + row;
+    if(MIXED) {
+      if(CPLX)
+        reinterpret_cast<float2 *>(out)[pos] = make_float2((float)acc.x, (float)acc.y);
+      else
+        reinterpret_cast<float *>(out)[pos] = (float)acc.x;
+    }
+    else {
+      if(CPLX)
+        reinterpret_cast<double2 *>(out)[pos] = acc;
+      else
+        reinterpret_cast<double *>(out)[pos] = acc.x;
+    }
+  }
+}
+
+// ------------------------------------------------------------------ reference strategy
+template <bool CPLX>
+__global__ void __launch_bounds__(256) k_ref_scatter(int64_t total, const double2 *__restrict__ Ae,
+                                                     const int32_t *__restrict__ e2n, void *__restrict__ out)
+{
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(t >= total) return;
+  const int32_t nz = e2n[t];
+  if(nz < 0) return;
+  const double2 v = Ae[t];
+  if(CPLX) {
+    double *o = reinterpret_cast<double *>(out) + 2 * (int64_t)nz;
+    unsafeAtomicAdd(o, v.x);
+    unsafeAtomicAdd(o + 1, v.y);
+  }
+  else {
+    unsafeAtomicAdd(reinterpret_cast<double *>(out) + nz, v.x);
+  }
+}
+
+template <bool CPLX>
+__global__ void __launch_bounds__(256) k_ref_rhs(int64_t ne, int N, int nrhs, int64_t n,
+                                                 const int32_t *__restrict__ num, const int32_t *__restrict__ type,
+                                                 const double2 *__restrict__ Ae, const double2 *__restrict__ be,
+                                                 const double2 *__restrict__ uD, void *__restrict__ out)
+{
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(t >= ne * N) return;
+  const int64_t e = t / N;
+  const int i = (int)(t % N);
+  const int ti = type[t];
+  if(ti != GFEM_AT_UNKNOWN && ti != GFEM_AT_UNKNOWN_BUBBLE) return;
+  const int64_t row = num[t];
+  double2 lift = make_double2(0., 0.);
+  if(Ae) {
+    for(int j = 0; j < N; ++j) {
+      if(type[e * N + j] != GFEM_AT_FIXED) continue;
+      const double2 a = Ae[e * N * N + (int64_t)j * N + i];
+      lift = cadd(lift, cmul(a, uD[num[e * N + j]]));
+    }
+  }
+  for(int r = 0; r < nrhs; ++r) {
+    double2 v = lift;
+    if(be) {
+      const double2 b = be[((int64_t)r * ne + e) * N + i];
+      v.x += b.x;
+      v.y += b.y;
+    }
+    double *o = reinterpret_cast<double *>(out);
+    if(CPLX) {
+      unsafeAtomicAdd(o + 2 * ((int64_t)r * n + row), v.x);
+      unsafeAtomicAdd(o + 2 * ((int64_t)r * n + row) + 1, v.y);
+    }
+    else {
+      unsafeAtomicAdd(o + (int64_t)r * n + row, v.x);
+    }
+  }
+}
+
+inline unsigned blocks_for(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+}  // namespace
+
+// ====================================================================== launchers
+
+hipError_t launch_basis(const Group &g, Rule &r, hipStream_t s)
+{
+  const int n = g.N * r.Q;
+  hipLaunchKernelGGL(k_basis, dim3(blocks_for(n, 128)), dim3(128), 0, s, g.dim, g.order, g.N, r.Q, g.d_nodes,
+                     r.d_pts, r.d_bf, r.d_gbf);
+  return hipGetLastError();
+}
+
+hipError_t launch_tensor(const Group &g, hipStream_t s)
+{
+  TensorArgs a{};
+  a.N = g.N;
+  a.nterms = 0;
+  for(size_t k = 0; k < g.dterms.size() && a.nterms < 16; ++k) {
+    const DevTerm &t = g.dterms[k];
+    const int i = a.nterms++;
+    a.form_dof[i] = t.form_dof;
+    a.form_tf[i] = t.form_tf;
+    a.rule[i] = t.rule;
+    a.active[i] = (t.kind == 0 && !t.per_point) ? 1 : 0;
+    // constant coefficient * tf factor (conjugated tf coefficient and dt factor)
+    const double cr = t.c_re * t.fac_re - t.c_im * t.fac_im;
+    const double ci = t.c_re * t.fac_im + t.c_im * t.fac_re;
+    a.fac[i] = make_double2(cr, ci);
+  }
+  for(size_t r = 0; r < g.rules.size() && r < 8; ++r) {
+    a.Q[r] = g.rules[r].Q;
+    a.w[r] = g.rules[r].d_w;
+    a.bf[r] = g.rules[r].d_bf;
+    a.gbf[r] = g.rules[r].d_gbf;
+  }
+  hipLaunchKernelGGL(k_tensor, dim3(blocks_for(g.N * g.N, 64)), dim3(64), 0, s, a, g.d_T64, g.d_T32);
+  return hipGetLastError();
+}
+
+hipError_t launch_geometry(const Group &g, const double *xyz, hipStream_t s)
+{
+  if(g.ne == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_geometry, dim3(blocks_for(g.ne, 256)), dim3(256), 0, s, g.dim, g.nv, g.ne, g.d_vtx, xyz,
+                     g.d_F64, g.d_F32);
+  return hipGetLastError();
+}
+
+template <int N>
+static void linear_n(const LinearArgs &a, const Group &g, const double *xyz, double2 *be, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_linear<N>, dim3(blocks_for(g.ne, 128)), dim3(128), 0, s, a, g.ne, g.d_vtx, xyz, g.d_F64,
+                     be);
+}
+
+hipError_t launch_linear(const Group &g, const double *xyz, int nrhs, hipStream_t s)
+{
+  if(g.ne == 0 || !g.has_linear) return hipSuccess;
+  LinearArgs a{};
+  a.dim = g.dim;
+  a.nv = g.nv;
+  a.N = g.N;
+  a.nterms = (int)g.dterms.size();
+  a.terms = g.d_terms;
+  for(size_t r = 0; r < g.rules.size() && r < 8; ++r) {
+    a.Q[r] = g.rules[r].Q;
+    a.w[r] = g.rules[r].d_w;
+    a.pts[r] = g.rules[r].d_pts;
+    a.bf[r] = g.rules[r].d_bf;
+  }
+  for(int r = 0; r < nrhs; ++r) {
+    a.rhs = r;
+    double2 *be = g.d_be + (int64_t)r * g.ne * g.N;
+    switch(g.N) {
+    case 1: linear_n<1>(a, g, xyz, be, s); break;
+    case 2: linear_n<2>(a, g, xyz, be, s); break;
+    case 3: linear_n<3>(a, g, xyz, be, s); break;
+    case 4: linear_n<4>(a, g, xyz, be, s); break;
+    case 5: linear_n<5>(a, g, xyz, be, s); break;
+    case 6: linear_n<6>(a, g, xyz, be, s); break;
+    case 10: linear_n<10>(a, g, xyz, be, s); break;
+    case 15: linear_n<15>(a, g, xyz, be, s); break;
+    case 20: linear_n<20>(a, g, xyz, be, s); break;
+    case 35: linear_n<35>(a, g, xyz, be, s); break;
+    default: return hipErrorInvalidValue;
+    }
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_dense(const Group &g, const double *xyz, bool all_terms, bool freq_module, double freq, bool cplx,
+                        hipStream_t s)
+{
+  if(g.ne == 0) return hipSuccess;
+  DenseArgs a{};
+  a.dim = g.dim;
+  a.nv = g.nv;
+  a.N = g.N;
+  a.nterms = (int)g.dterms.size();
+  a.all_terms = all_terms ? 1 : 0;
+  a.freq_module = freq_module ? 1 : 0;
+  a.cplx = cplx ? 1 : 0;
+  a.freq = freq;
+  a.terms = g.d_terms;
+  for(size_t r = 0; r < g.rules.size() && r < 8; ++r) {
+    a.Q[r] = g.rules[r].Q;
+    a.w[r] = g.rules[r].d_w;
+    a.pts[r] = g.rules[r].d_pts;
+    a.bf[r] = g.rules[r].d_bf;
+    a.gbf[r] = g.rules[r].d_gbf;
+  }
+  const int64_t total = g.ne * (int64_t)g.N * g.N;
+  hipLaunchKernelGGL(k_dense, dim3(blocks_for(total, 256)), dim3(256), 0, s, a, g.ne, g.d_vtx, xyz, g.d_dense);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather(const gfem_system &sys, const GroupTable &gt, int prec, hipStream_t s)
+{
+  if(sys.n_chunks == 0) return hipSuccess;
+  void *out = sys.d_vals[prec];
+  const dim3 grid((unsigned)sys.n_chunks), block(kChunk);
+  if(sys.cplx) {
+    if(prec)
+      hipLaunchKernelGGL((k_gather<true, true>), grid, block, 0, s, gt, sys.d_words, sys.d_chunk_c, sys.d_chunk_n, out);
+    else
+      hipLaunchKernelGGL((k_gather<true, false>), grid, block, 0, s, gt, sys.d_words, sys.d_chunk_c, sys.d_chunk_n, out);
+  }
+  else {
+    if(prec)
+      hipLaunchKernelGGL((k_gather<false, true>), grid, block, 0, s, gt, sys.d_words, sys.d_chunk_c, sys.d_chunk_n, out);
+    else
+      hipLaunchKernelGGL((k_gather<false, false>), grid, block, 0, s, gt, sys.d_words, sys.d_chunk_c, sys.d_chunk_n, out);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_rhs(const gfem_system &sys, const GroupTable &gt, int prec, hipStream_t s)
+{
+  if(sys.n == 0) return hipSuccess;
+  void *out = sys.d_rhs[prec];
+  const dim3 grid(blocks_for(sys.n, 256)), block(256);
+#define GFEM_RHS_ARGS gt, sys.n, sys.nrhs, sys.d_dc_ptr, sys.d_dc_words, sys.d_dc_fix, sys.d_dc, sys.d_lin_ptr, sys.d_lin_words, out
+  if(sys.cplx) {
+    if(prec)
+      hipLaunchKernelGGL((k_rhs<true, true>), grid, block, 0, s, GFEM_RHS_ARGS);
+    else
+      hipLaunchKernelGGL((k_rhs<true, false>), grid, block, 0, s, GFEM_RHS_ARGS);
+  }
+  else {
+    if(prec)
+      hipLaunchKernelGGL((k_rhs<false, true>), grid, block, 0, s, GFEM_RHS_ARGS);
+    else
+      hipLaunchKernelGGL((k_rhs<false, false>), grid, block, 0, s, GFEM_RHS_ARGS);
+  }
+#undef GFEM_RHS_ARGS
+  return hipGetLastError();
+}
+
+hipError_t launch_ref_scatter(const gfem_system &sys, const Group &g, hipStream_t s)
+{
+  if(g.ne == 0 || !g.has_bilinear) return hipSuccess;
+  const int64_t total = g.ne * (int64_t)g.N * g.N;
+  if(sys.cplx)
+    hipLaunchKernelGGL(k_ref_scatter<true>, dim3(blocks_for(total, 256)), dim3(256), 0, s, total, g.d_dense, g.d_e2n,
+                       sys.d_vals[0]);
+  else
+    hipLaunchKernelGGL(k_ref_scatter<false>, dim3(blocks_for(total, 256)), dim3(256), 0, s, total, g.d_dense, g.d_e2n,
+                       sys.d_vals[0]);
+  return hipGetLastError();
+}
+
+hipError_t launch_ref_rhs(const gfem_system &sys, const Group &g, hipStream_t s)
+{
+  if(g.ne == 0) return hipSuccess;
+  const int64_t total = g.ne * (int64_t)g.N;
+  const double2 *Ae = g.has_bilinear ? g.d_dense : nullptr;
+  const double2 *be = g.has_linear ? g.d_be : nullptr;
+  if(sys.cplx)
+    hipLaunchKernelGGL(k_ref_rhs<true>, dim3(blocks_for(total, 256)), dim3(256), 0, s, g.ne, g.N, sys.nrhs, sys.n,
+                       g.d_num, g.d_type, Ae, be, sys.d_dc, sys.d_rhs[0]);
+  else
+    hipLaunchKernelGGL(k_ref_rhs<false>, dim3(blocks_for(total, 256)), dim3(256), 0, s, g.ne, g.N, sys.nrhs, sys.n,
+                       g.d_num, g.d_type, Ae, be, sys.d_dc, sys.d_rhs[0]);
+  return hipGetLastError();
+}
+
+}  // namespace gfem

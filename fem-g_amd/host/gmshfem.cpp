@@ -1,0 +1,832 @@
+#include "gmshfem.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+
+#include "../csrc/refelem.h"
+
+namespace gmshfem {
+
+// ====================================================================== common
+
+namespace common {
+
+static double now()
+{
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+void Timer::tick() { _start = now(); }
+void Timer::tock() { _t = now() - _start; }
+
+namespace {
+std::mutex g_ctx_mutex;
+gfem_ctx *g_ctx = nullptr;
+int g_device = 0;
+uint64_t g_nodes_uid = 0;
+}  // namespace
+
+void setDevice(int device)
+{
+  std::lock_guard<std::mutex> lk(g_ctx_mutex);
+  if(g_ctx && device != g_device) {
+    gfem_ctx_destroy(g_ctx);
+    g_ctx = nullptr;
+    g_nodes_uid = 0;
+  }
+  g_device = device;
+}
+
+gfem_ctx *deviceContext()
+{
+  std::lock_guard<std::mutex> lk(g_ctx_mutex);
+  if(!g_ctx) {
+    const gfem_status s = gfem_ctx_create(g_device, &g_ctx);
+    if(s != GFEM_OK) throw Exception(std::string("gfem_ctx_create: ") + gfem_ctx_last_error(nullptr));
+  }
+  return g_ctx;
+}
+
+static void uploadNodes(const model::Model &m)
+{
+  gfem_ctx *ctx = deviceContext();
+  std::lock_guard<std::mutex> lk(g_ctx_mutex);
+  if(g_nodes_uid == m.uid) return;
+  const gfem_status s = gfem_nodes_upload(ctx, m.numNodes(), m.xyz.data());
+  if(s != GFEM_OK) throw Exception(std::string("gfem_nodes_upload: ") + gfem_ctx_last_error(ctx));
+  g_nodes_uid = m.uid;
+}
+
+}  // namespace common
+
+// ====================================================================== domain
+
+namespace domain {
+Domain Domain::operator|(const Domain &o) const
+{
+  Domain d = *this;
+  d._ents.insert(o._ents.begin(), o._ents.end());
+  return d;
+}
+int Domain::maxDim() const
+{
+  int m = 0;
+  for(auto &e : _ents) m = std::max(m, e.first);
+  return m;
+}
+}  // namespace domain
+
+// ====================================================================== function
+
+namespace function {
+
+ScalarFunction::ScalarFunction(std::complex<double> v) : _constant(true), _value(v)
+{
+  _consts = {v.real(), v.imag()};
+  _ops = {GFEM_OP_CONST, 0};
+}
+
+ScalarFunction ScalarFunction::coordinate(int axis)
+{
+  ScalarFunction f;
+  f._constant = false;
+  f._consts.clear();
+  f._ops = {GFEM_OP_X + axis, 0};
+  return f;
+}
+
+// append the program of b to a, shifting b's constant indices
+static void append(std::vector<int32_t> &ops, std::vector<double> &consts, const ScalarFunction &b)
+{
+  const int shift = (int)consts.size() / 2;
+  for(size_t k = 0; k < b.ops().size(); k += 2) {
+    ops.push_back(b.ops()[k]);
+    ops.push_back(b.ops()[k] == GFEM_OP_CONST ? b.ops()[k + 1] + shift : b.ops()[k + 1]);
+  }
+  consts.insert(consts.end(), b.consts().begin(), b.consts().end());
+}
+
+static std::complex<double> apply1(int op, std::complex<double> a)
+{
+  switch(op) {
+  case GFEM_OP_NEG: return -a;
+  case GFEM_OP_EXP: return std::exp(a);
+  case GFEM_OP_SQRT: return std::sqrt(a);
+  case GFEM_OP_SIN: return std::sin(a);
+  case GFEM_OP_COS: return std::cos(a);
+  case GFEM_OP_ABS: return std::abs(a);
+  }
+  throw common::Exception("unknown unary operation");
+}
+
+static std::complex<double> apply2(int op, std::complex<double> a, std::complex<double> b)
+{
+  switch(op) {
+  case GFEM_OP_ADD: return a + b;
+  case GFEM_OP_SUB: return a - b;
+  case GFEM_OP_MUL: return a * b;
+  case GFEM_OP_DIV: return a / b;
+  case GFEM_OP_POW:
+    if(a.imag() == 0. && b.imag() == 0. && (a.real() > 0. || b.real() == std::floor(b.real())))
+      return std::pow(a.real(), b.real());
+    return std::pow(a, b);
+  }
+  throw common::Exception("unknown binary operation");
+}
+
+ScalarFunction ScalarFunction::unary(int op, const ScalarFunction &a)
+{
+  if(a._constant) return ScalarFunction(apply1(op, a._value));
+  ScalarFunction f;
+  f._constant = false;
+  f._ops.clear();
+  f._consts.clear();
+  append(f._ops, f._consts, a);
+  f._ops.push_back(op);
+  f._ops.push_back(0);
+  return f;
+}
+
+ScalarFunction ScalarFunction::binary(int op, const ScalarFunction &a, const ScalarFunction &b)
+{
+  if(a._constant && b._constant) return ScalarFunction(apply2(op, a._value, b._value));
+  ScalarFunction f;
+  f._constant = false;
+  f._ops.clear();
+  f._consts.clear();
+  append(f._ops, f._consts, a);
+  append(f._ops, f._consts, b);
+  f._ops.push_back(op);
+  f._ops.push_back(0);
+  return f;
+}
+
+std::complex<double> ScalarFunction::evaluate(double x, double y, double z) const
+{
+  if(_constant) return _value;
+  std::vector<std::complex<double>> st;
+  for(size_t k = 0; k < _ops.size(); k += 2) {
+    const int op = _ops[k], arg = _ops[k + 1];
+    if(op == GFEM_OP_CONST)
+      st.emplace_back(_consts[2 * arg], _consts[2 * arg + 1]);
+    else if(op == GFEM_OP_X)
+      st.emplace_back(x);
+    else if(op == GFEM_OP_Y)
+      st.emplace_back(y);
+    else if(op == GFEM_OP_Z)
+      st.emplace_back(z);
+    else if((op >= GFEM_OP_ADD && op <= GFEM_OP_DIV) || op == GFEM_OP_POW) {
+      const auto b = st.back();
+      st.pop_back();
+      st.back() = apply2(op, st.back(), b);
+    }
+    else
+      st.back() = apply1(op, st.back());
+  }
+  return st.back();
+}
+
+ScalarFunction operator+(const ScalarFunction &a, const ScalarFunction &b) { return ScalarFunction::binary(GFEM_OP_ADD, a, b); }
+ScalarFunction operator-(const ScalarFunction &a, const ScalarFunction &b) { return ScalarFunction::binary(GFEM_OP_SUB, a, b); }
+ScalarFunction operator*(const ScalarFunction &a, const ScalarFunction &b) { return ScalarFunction::binary(GFEM_OP_MUL, a, b); }
+ScalarFunction operator/(const ScalarFunction &a, const ScalarFunction &b) { return ScalarFunction::binary(GFEM_OP_DIV, a, b); }
+ScalarFunction operator-(const ScalarFunction &a) { return ScalarFunction::unary(GFEM_OP_NEG, a); }
+ScalarFunction x() { return ScalarFunction::coordinate(0); }
+ScalarFunction y() { return ScalarFunction::coordinate(1); }
+ScalarFunction z() { return ScalarFunction::coordinate(2); }
+ScalarFunction exp(const ScalarFunction &a) { return ScalarFunction::unary(GFEM_OP_EXP, a); }
+ScalarFunction sqrt(const ScalarFunction &a) { return ScalarFunction::unary(GFEM_OP_SQRT, a); }
+ScalarFunction sin(const ScalarFunction &a) { return ScalarFunction::unary(GFEM_OP_SIN, a); }
+ScalarFunction cos(const ScalarFunction &a) { return ScalarFunction::unary(GFEM_OP_COS, a); }
+ScalarFunction abs(const ScalarFunction &a) { return ScalarFunction::unary(GFEM_OP_ABS, a); }
+ScalarFunction pow(const ScalarFunction &a, const ScalarFunction &b) { return ScalarFunction::binary(GFEM_OP_POW, a, b); }
+
+}  // namespace function
+
+// ====================================================================== field / equation
+
+namespace field {
+static int g_field_tag = 0;
+Field::Field(const std::string &name, const domain::Domain &dom, int order)
+  : _name(name), _domain(dom), _order(order), _tag(g_field_tag++)
+{
+  if(order < 1 || order > 4) throw common::Exception("Lagrange order must be in 1..4");
+}
+void Field::addConstraint(const domain::Domain &dom, const function::ScalarFunction &value)
+{
+  _constraints.emplace_back(dom, value);
+}
+}  // namespace field
+
+namespace equation {
+Unknown dof(const field::Field &u)
+{
+  Unknown r;
+  r.field = &u;
+  return r;
+}
+Unknown tf(const field::Field &u)
+{
+  Unknown r = dof(u);
+  r.tf = true;
+  return r;
+}
+Unknown dt_dof(const field::Field &u)
+{
+  Unknown r = dof(u);
+  r.dt = 1;
+  return r;
+}
+Unknown dt2_dof(const field::Field &u)
+{
+  Unknown r = dof(u);
+  r.dt = 2;
+  return r;
+}
+Unknown grad(const Unknown &u)
+{
+  if(u.form != 0) throw common::Exception("grad of a gradient");
+  Unknown r = u;
+  r.form = 1;
+  return r;
+}
+Unknown operator*(const function::ScalarFunction &c, const Unknown &u)
+{
+  Unknown r = u;
+  r.coef = c * u.coef;
+  return r;
+}
+}  // namespace equation
+
+// ====================================================================== reorder
+
+namespace reorder {
+
+namespace {
+// Compact Hilbert index tables (state = direction << dim | entry point,
+// digit = coordinate bits of one level), built from the gray-code entry /
+// direction sequences of fem-master/src/reorder/Hilbert.cpp:17-128.
+struct Curve {
+  int dim = 0;
+  uint8_t nextState[32][8] = {};
+  uint8_t digitOut[32][8] = {};
+
+  explicit Curve(int d) : dim(d)
+  {
+    const int N = 1 << d;
+    const unsigned mask = (1u << d) - 1u;
+    auto gray = [](unsigned i) { return i ^ (i >> 1); };
+    auto grayInv = [d](unsigned g) {
+      unsigned v = g;
+      for(int j = 1; j < d; ++j) v ^= g >> j;
+      return v;
+    };
+    auto rotr = [d, mask](unsigned k, int r) { return ((k >> r) | (k << (d - r))) & mask; };
+    auto rotl = [d, mask](unsigned k, int r) { return ((k << r) | (k >> (d - r))) & mask; };
+    unsigned entry[8], dir[8];
+    entry[0] = 0;
+    for(int i = 1; i < N; ++i) entry[i] = gray(2u * ((i - 1) / 2));
+    dir[0] = 0;
+    for(int i = 1; i < N / 2; ++i) dir[i] = ((i % 2 == 0) ? gray(i - 1) : gray(i)) % d;
+    for(int i = N / 2; i < N; ++i) dir[i] = dir[N - i - 1];
+    for(int dd = 0; dd < d; ++dd)
+      for(int e = 0; e < N; ++e) {
+        const int s = (dd << d) | e;
+        for(int l = 0; l < N; ++l) {
+          const unsigned w = grayInv(rotr((unsigned)l ^ (unsigned)e, dd + 1));
+          digitOut[s][l] = (uint8_t)w;
+          const unsigned ne = (unsigned)e ^ rotl(entry[w], dd + 1);
+          const unsigned nd = (dd + dir[w] + 1) % d;
+          nextState[s][l] = (uint8_t)((nd << d) | ne);
+        }
+      }
+  }
+
+  uint64_t index(const uint32_t *u, int top) const
+  {
+    uint64_t h = 0;
+    unsigned s = 0;
+    for(int m = top; m >= 0; --m) {
+      unsigned l = 0;
+      for(int k = 0; k < dim; ++k) l |= ((u[k] >> m) & 1u) << k;
+      h = (h << dim) ^ digitOut[s][l];
+      s = nextState[s][l];
+    }
+    return h;
+  }
+};
+
+// float quantisation of Hilbert<dim>::apply (Hilbert.cpp:228-271): scale by
+// 2^(bits-1)/(max-min), no fused multiply-add
+struct Quant {
+  int bits;
+  float scale[3], shift[3];
+  Quant(int dim, const float *mn, const float *mx)
+  {
+    bits = 64 / dim;
+    const unsigned two = 1u << (bits - 1);
+    for(int i = 0; i < dim; ++i) {
+      scale[i] = (float)two / (mx[i] - mn[i]);
+      shift[i] = -mn[i] * scale[i];
+    }
+  }
+  uint32_t operator()(int i, float f) const
+  {
+    volatile float t = f * scale[i];
+    return (uint32_t)(t + shift[i]);
+  }
+};
+}  // namespace
+
+uint64_t hilbertKey(int dim, const float *f, const float *min, const float *max)
+{
+  const Curve c(dim);
+  const Quant q(dim, min, max);
+  uint32_t u[3] = {0, 0, 0};
+  for(int i = 0; i < dim; ++i) u[i] = q(i, f[i]);
+  return c.index(u, q.bits - 1);
+}
+
+std::vector<int64_t> hilbertOrder(int dim, const std::vector<float> &coords, const float *min, const float *max)
+{
+  const Curve c(dim);
+  const Quant q(dim, min, max);
+  const int64_t n = (int64_t)coords.size() / 3;
+  std::vector<std::pair<uint64_t, int64_t>> keys(n);
+  for(int64_t i = 0; i < n; ++i) {
+    uint32_t u[3] = {0, 0, 0};
+    for(int k = 0; k < dim; ++k) u[k] = q(k, coords[3 * i + k]);
+    keys[i] = {c.index(u, q.bits - 1), i};
+  }
+  std::stable_sort(keys.begin(), keys.end(),
+                   [](const std::pair<uint64_t, int64_t> &a, const std::pair<uint64_t, int64_t> &b) {
+                     return a.first < b.first;
+                   });
+  std::vector<int64_t> order(n);
+  for(int64_t i = 0; i < n; ++i) order[i] = keys[i].second;
+  return order;
+}
+
+}  // namespace reorder
+
+// ====================================================================== problem
+
+namespace problem {
+
+template <class T>
+struct ScalarKind;
+template <>
+struct ScalarKind<double> {
+  static constexpr gfem_scalar value = GFEM_SCALAR_F64;
+  static constexpr bool complex = false;
+};
+template <>
+struct ScalarKind<float> {
+  static constexpr gfem_scalar value = GFEM_SCALAR_F32;
+  static constexpr bool complex = false;
+};
+template <>
+struct ScalarKind<std::complex<double>> {
+  static constexpr gfem_scalar value = GFEM_SCALAR_C128;
+  static constexpr bool complex = true;
+};
+template <>
+struct ScalarKind<std::complex<float>> {
+  static constexpr gfem_scalar value = GFEM_SCALAR_C64;
+  static constexpr bool complex = true;
+};
+
+static int parseGauss(const std::string &s)
+{
+  if(s.rfind("Gauss", 0) != 0) throw common::Exception("unknown integration type '" + s + "'");
+  try {
+    return std::stoi(s.substr(5));
+  }
+  catch(...) {
+    throw common::Exception("unknown integration type '" + s + "'");
+  }
+}
+
+template <class T>
+Formulation<T>::Formulation(const std::string &name) : _name(name)
+{
+}
+
+template <class T>
+Formulation<T>::~Formulation()
+{
+  releaseSystem();
+}
+
+template <class T>
+void Formulation<T>::releaseSystem()
+{
+  if(_sys) gfem_system_destroy(_sys);
+  _sys = nullptr;
+}
+
+template <class T>
+unsigned int Formulation<T>::integral(const equation::Unknown &lhs, const equation::Unknown &rhs,
+                                      const domain::Domain &dom, const std::string &integrationType,
+                                      ProductType product)
+{
+  if(lhs.tf || !rhs.tf) throw common::Exception("integral(lhs, rhs): lhs must be a dof, rhs a test function");
+  if(lhs.field != rhs.field) throw common::Exception("one unknown field per formulation");
+  if(lhs.form != rhs.form) throw common::Exception("bilinear forms must be (dof, tf) or (grad dof, grad tf)");
+  if(rhs.dt) throw common::Exception("time derivative on a test function");
+  if(_field && _field != lhs.field) throw common::Exception("one unknown field per formulation");
+  _field = lhs.field;
+  Term t;
+  t.bilinear = true;
+  t.lhs = lhs;
+  t.rhs = rhs;
+  t.dom = dom;
+  t.gauss = parseGauss(integrationType);
+  t.product = product;
+  _terms.push_back(t);
+  return (unsigned)_terms.size() - 1;
+}
+
+template <class T>
+unsigned int Formulation<T>::integral(const function::ScalarFunction &f, const equation::Unknown &rhs,
+                                      const domain::Domain &dom, const std::string &integrationType,
+                                      ProductType product, unsigned int rhsIdx)
+{
+  if(!rhs.tf) throw common::Exception("integral(f, rhs): rhs must be a test function");
+  if(rhs.form != 0) throw common::Exception("linear terms take tf(u)");
+  if(_field && _field != rhs.field) throw common::Exception("one unknown field per formulation");
+  _field = rhs.field;
+  Term t;
+  t.bilinear = false;
+  t.rhs = rhs;
+  t.source = f;
+  t.dom = dom;
+  t.gauss = parseGauss(integrationType);
+  t.product = product;
+  t.rhsIdx = rhsIdx;
+  _numRHS = std::max(_numRHS, rhsIdx + 1);
+  _terms.push_back(t);
+  return (unsigned)_terms.size() - 1;
+}
+
+template <class T>
+std::pair<int64_t, int> Formulation<T>::nodeDof(int64_t tag) const
+{
+  if(tag < 1 || tag >= (int64_t)_dofNum.size()) return {0, 0};
+  return {_dofNum[tag], _dofType[tag]};
+}
+
+template <class T>
+const typename Formulation<T>::Pass *Formulation<T>::findPass(int dim, int tag, int type) const
+{
+  for(const auto &p : _passes)
+    if(p.dim == dim && p.tag == tag && p.type == type) return &p;
+  return nullptr;
+}
+
+template <class T>
+const std::vector<int64_t> *Formulation<T>::blockDofs(int dim, int tag, int type) const
+{
+  const Pass *p = findPass(dim, tag, type);
+  return p ? &p->num : nullptr;
+}
+
+template <class T>
+const std::vector<int32_t> *Formulation<T>::blockTypes(int dim, int tag, int type) const
+{
+  const Pass *p = findPass(dim, tag, type);
+  return p ? &p->atype : nullptr;
+}
+
+template <class T>
+const std::vector<int64_t> *Formulation<T>::blockNonzeros(int dim, int tag, int type) const
+{
+  const Pass *p = findPass(dim, tag, type);
+  return p ? &p->e2n : nullptr;
+}
+
+// Formulation::pre, fem-master/src/problem/Formulation.cpp:897-1039
+template <class T>
+common::Timer Formulation<T>::pre(DofsSort sort)
+{
+  common::Timer time;
+  time.tick();
+  releaseSystem();
+  _ai.clear();
+  _aj.clear();
+  _passes.clear();
+  _values.clear();
+  _rhs.clear();
+  _model = model::current();
+  if(!_model) throw common::Exception("no current model");
+  if(!_field) throw common::Exception("formulation '" + _name + "' has no term");
+  const model::Model &M = *_model;
+  const int64_t nn = M.numNodes();
+  _dofNum.assign(nn + 1, 0);
+  _dofType.assign(nn + 1, 0);
+  _fixedValues.clear();
+  _nFixed = _nUnknown = 0;
+
+  // fixed dofs: constraint order, entity order, element type order, key order
+  // (DofsFactory.cpp:39-74); Lagrange keys are the element node tags
+  for(const auto &c : _field->constraints()) {
+    for(const auto &ent : c.first.entities()) {
+      for(int type : M.elementTypes(ent.first, ent.second)) {
+        const model::Block *b = M.block(ent.first, ent.second, type);
+        for(int64_t tag : b->conn) {
+          if(_dofType[tag]) continue;
+          _dofType[tag] = 2;
+          _dofNum[tag] = (int64_t)++_nFixed;
+          const double *X = &M.xyz[3 * (tag - 1)];
+          _fixedValues.push_back(c.second.evaluate(X[0], X[1], X[2]));
+        }
+      }
+    }
+  }
+  // unknown dofs (DofsFactory.cpp:406-460); Lagrange keys are never bubbles
+  std::vector<int64_t> unknownTags;
+  for(const auto &ent : _field->domain().entities()) {
+    for(int type : M.elementTypes(ent.first, ent.second)) {
+      const model::Block *b = M.block(ent.first, ent.second, type);
+      if(b->order != _field->order() && b->dim > 0)
+        throw common::Exception("Lagrange order differs from the mesh order");
+      for(int64_t tag : b->conn) {
+        if(_dofType[tag]) continue;
+        _dofType[tag] = 1;
+        _dofNum[tag] = (int64_t)++_nUnknown;
+        unknownTags.push_back(tag);
+      }
+    }
+  }
+  if(_nUnknown == 0) {
+    _lastError = "no unknown dofs";
+    time.tock();
+    return time;
+  }
+  // Hilbert renumbering (DofsManager.cpp:118-239)
+  if(sort != DofsSort::None && M.dim >= 2) {
+    double mn[3], mx[3];
+    M.boundingBox(mn, mx);
+    float fmn[3], fmx[3];
+    for(int j = 0; j < 3; ++j) {
+      fmn[j] = (float)(mn[j] - 0.000001 * std::fabs(mn[j]));
+      fmx[j] = (float)(mx[j] + 0.000001 * std::fabs(mx[j]));
+    }
+    std::vector<float> coords(3 * unknownTags.size());
+    for(size_t i = 0; i < unknownTags.size(); ++i)
+      for(int j = 0; j < 3; ++j) coords[3 * i + j] = (float)M.xyz[3 * (unknownTags[i] - 1) + j];
+    const std::vector<int64_t> order = reorder::hilbertOrder(M.dim, coords, fmn, fmx);
+    for(size_t i = 0; i < order.size(); ++i) _dofNum[unknownTags[order[i]]] = (int64_t)i + 1;
+  }
+
+  // element maps of every (entity, type) carrying a term (FieldInterface::fillIndices)
+  _passes.clear();
+  for(const auto &ent : M.entities()) {
+    std::vector<unsigned> terms;
+    for(unsigned t = 0; t < _terms.size(); ++t)
+      if(_terms[t].activated && _terms[t].dom.have(ent)) terms.push_back(t);
+    if(terms.empty()) continue;
+    for(int type : M.elementTypes(ent.first, ent.second)) {
+      const model::Block *b = M.block(ent.first, ent.second, type);
+      if(b->dim == 0) continue;
+      Pass p;
+      p.dim = ent.first;
+      p.tag = ent.second;
+      p.type = type;
+      p.terms = terms;
+      p.num.resize(b->conn.size());
+      p.atype.resize(b->conn.size());
+      for(size_t k = 0; k < b->conn.size(); ++k) {
+        const int64_t tag = b->conn[k];
+        const int t = _dofType[tag];
+        p.num[k] = t ? _dofNum[tag] - 1 : 0;
+        p.atype[k] = t == 1 ? GFEM_AT_UNKNOWN : (t == 2 ? GFEM_AT_FIXED : GFEM_AT_UNDEFINED);
+      }
+      _passes.push_back(std::move(p));
+    }
+  }
+
+  // sparsity pattern (Pattern::build + MatrixFactory::addPattern/finalizePattern):
+  // row = tf dof, col = dof; unknown x unknown; upper triangle if symmetric
+  const int64_t n = (int64_t)_nUnknown;
+  std::vector<int64_t> cnt(n + 1, 0);
+  auto hasBilinear = [&](const Pass &p) {
+    for(unsigned t : p.terms)
+      if(_terms[t].bilinear) return true;
+    return false;
+  };
+  for(const auto &p : _passes) {
+    if(!hasBilinear(p)) continue;
+    const int N = M.block(p.dim, p.tag, p.type)->npe;
+    const int64_t ne = (int64_t)p.num.size() / N;
+    for(int64_t e = 0; e < ne; ++e) {
+      int nu = 0;
+      for(int j = 0; j < N; ++j) nu += p.atype[e * N + j] == GFEM_AT_UNKNOWN;
+      for(int i = 0; i < N; ++i)
+        if(p.atype[e * N + i] == GFEM_AT_UNKNOWN) cnt[p.num[e * N + i] + 1] += nu;
+    }
+  }
+  for(int64_t i = 0; i < n; ++i) cnt[i + 1] += cnt[i];
+  std::vector<int64_t> cols((size_t)cnt[n]);
+  {
+    std::vector<int64_t> fill(cnt.begin(), cnt.end() - 1);
+    for(const auto &p : _passes) {
+      if(!hasBilinear(p)) continue;
+      const int N = M.block(p.dim, p.tag, p.type)->npe;
+      const int64_t ne = (int64_t)p.num.size() / N;
+      for(int64_t e = 0; e < ne; ++e)
+        for(int i = 0; i < N; ++i) {
+          if(p.atype[e * N + i] != GFEM_AT_UNKNOWN) continue;
+          const int64_t r = p.num[e * N + i];
+          for(int j = 0; j < N; ++j)
+            if(p.atype[e * N + j] == GFEM_AT_UNKNOWN) cols[fill[r]++] = p.num[e * N + j];
+        }
+    }
+  }
+  _ai.assign(n + 1, 0);
+  _aj.clear();
+  _aj.reserve(cols.size() / 2);
+  for(int64_t r = 0; r < n; ++r) {
+    auto b = cols.begin() + cnt[r], e = cols.begin() + cnt[r + 1];
+    std::sort(b, e);
+    e = std::unique(b, e);
+    for(auto it = b; it != e; ++it)
+      if(!_symmetric || *it >= r) _aj.push_back(*it);
+    _ai[r + 1] = (int64_t)_aj.size();
+  }
+  std::vector<int64_t>().swap(cols);
+  // element -> nonzero map
+  for(auto &p : _passes) {
+    if(!hasBilinear(p)) continue;
+    const int N = M.block(p.dim, p.tag, p.type)->npe;
+    const int64_t ne = (int64_t)p.num.size() / N;
+    p.e2n.assign((size_t)(ne * N * N), -1);
+    for(int64_t e = 0; e < ne; ++e)
+      for(int i = 0; i < N; ++i) {
+        if(p.atype[e * N + i] != GFEM_AT_UNKNOWN) continue;
+        const int64_t r = p.num[e * N + i];
+        const int64_t *rb = _aj.data() + _ai[r], *re = _aj.data() + _ai[r + 1];
+        for(int j = 0; j < N; ++j) {
+          if(p.atype[e * N + j] != GFEM_AT_UNKNOWN) continue;
+          const int64_t c = p.num[e * N + j];
+          if(_symmetric && c < r) continue;
+          p.e2n[(e * N + i) * N + j] = std::lower_bound(rb, re, c) - _aj.data();
+        }
+      }
+  }
+  time.tock();
+  _preSeconds = time.time();
+  return time;
+}
+
+// Hand the pattern, the Dirichlet values and one group per assembly pass to
+// the device, in the traversal order of Formulation::assemble (dimension
+// 3..0, element type ascending, entity tag ascending; Formulation.cpp:1417-1493).
+// Runs once, at the first assemble() after pre().
+template <class T>
+void Formulation<T>::upload()
+{
+  const model::Model &M = *_model;
+  const int64_t n = (int64_t)_nUnknown;
+  common::uploadNodes(M);
+  gfem_ctx *ctx = common::deviceContext();
+  std::vector<double> dc(2 * std::max<uint64_t>(_nFixed, 1), 0.);
+  for(uint64_t i = 0; i < _nFixed; ++i) {
+    dc[2 * i] = _fixedValues[i].real();
+    dc[2 * i + 1] = _fixedValues[i].imag();
+  }
+  gfem_status st = gfem_system_create(ctx, ScalarKind<T>::value, n, _ai.data(), _aj.data(), _symmetric ? 1 : 0,
+                                      (int64_t)_nFixed, dc.data(), (int)_numRHS, _frequency, &_sys);
+  if(st != GFEM_OK) throw common::Exception(std::string("gfem_system_create: ") + gfem_ctx_last_error(ctx));
+  std::vector<const Pass *> order;
+  for(int dim = 3; dim >= 0; --dim)
+    for(int type : M.elementTypes(dim))
+      for(const auto &ent : M.entities(dim))
+        if(const Pass *p = findPass(dim, ent.second, type)) order.push_back(p);
+  for(const Pass *p : order) {
+    const model::Block *b = M.block(p->dim, p->tag, p->type);
+    int g = -1;
+    st = gfem_group_add(_sys, p->dim, p->type, b->size(), b->conn.data(), p->num.data(), p->atype.data(),
+                        p->e2n.empty() ? nullptr : p->e2n.data(), &g);
+    if(st != GFEM_OK) throw common::Exception(std::string("gfem_group_add: ") + gfem_ctx_last_error(ctx));
+    for(unsigned ti : p->terms) {
+      const Term &t = _terms[ti];
+      gfem_term_desc d{};
+      d.kind = t.bilinear ? 0 : 1;
+      d.gauss = t.gauss;
+      d.form_dof = t.bilinear ? t.lhs.form : 0;
+      d.form_tf = t.rhs.form;
+      d.hermitian = t.product == ProductType::Hermitian ? 1 : 0;
+      d.dt_order = t.bilinear ? t.lhs.dt : 0;
+      d.rhs_idx = t.bilinear ? 0 : (int)t.rhsIdx;
+      const function::ScalarFunction &c = t.bilinear ? t.lhs.coef : t.source;
+      if(!t.rhs.coef.isConstant()) throw common::Exception("test-function coefficients must be constant");
+      d.tf_coef_re = t.rhs.coef.value().real();
+      d.tf_coef_im = t.rhs.coef.value().imag();
+      if(c.isConstant()) {
+        d.coef_re = c.value().real();
+        d.coef_im = c.value().imag();
+      }
+      else {
+        d.n_ops = (int)c.ops().size() / 2;
+        d.ops = c.ops().data();
+        d.n_consts = (int)c.consts().size() / 2;
+        d.consts = c.consts().data();
+      }
+      st = gfem_group_add_term(_sys, g, &d);
+      if(st != GFEM_OK) throw common::Exception(std::string("gfem_group_add_term: ") + gfem_ctx_last_error(ctx));
+    }
+  }
+  st = gfem_system_finalize(_sys);
+  if(st != GFEM_OK) {
+    const std::string msg = gfem_ctx_last_error(ctx);
+    releaseSystem();
+    throw common::Exception("gfem_system_finalize: " + msg);
+  }
+}
+
+template <class T>
+static void convert(const std::vector<char> &raw, bool mixed, bool cplx, std::vector<T> &out, int64_t count)
+{
+  out.resize(count);
+  for(int64_t i = 0; i < count; ++i) {
+    double re, im = 0.;
+    if(mixed) {
+      const float *f = reinterpret_cast<const float *>(raw.data());
+      re = f[cplx ? 2 * i : i];
+      if(cplx) im = f[2 * i + 1];
+    }
+    else {
+      const double *f = reinterpret_cast<const double *>(raw.data());
+      re = f[cplx ? 2 * i : i];
+      if(cplx) im = f[2 * i + 1];
+    }
+    if constexpr(ScalarKind<T>::complex)
+      out[i] = T(re, im);
+    else
+      out[i] = (T)re;
+  }
+}
+
+// Formulation::assemble, fem-master/src/problem/Formulation.cpp:1151-1215:
+// on failure the matrix and every right-hand side are zeroed and the error
+// is logged; a Timer is still returned.
+template <class T>
+common::Timer Formulation<T>::assemble(const AssembleOptions &opt)
+{
+  common::Timer time;
+  time.tick();
+  _lastError.clear();
+  try {
+    if(_ai.empty()) throw common::Exception("there are no unknown dofs to assemble; did you forget to pre-process?");
+    if(!_sys) {
+      try {
+        upload();
+      }
+      catch(...) {
+        releaseSystem();
+        throw;
+      }
+    }
+    gfem_ctx *ctx = common::deviceContext();
+    gfem_status st = gfem_assemble(_sys, opt.precision, opt.strategy, opt.stream);
+    if(st != GFEM_OK) throw common::Exception(std::string("gfem_assemble: ") + gfem_ctx_last_error(ctx));
+    if(opt.download) {
+      int64_t nnz = 0, n = 0;
+      int vb = 0;
+      gfem_system_info(_sys, opt.precision, &nnz, &n, &vb);
+      std::vector<char> raw((size_t)(vb * std::max(nnz, n)));
+      st = gfem_values_download(_sys, opt.precision, raw.data(), raw.size());
+      if(st != GFEM_OK) throw common::Exception(std::string("gfem_values_download: ") + gfem_ctx_last_error(ctx));
+      const bool mixed = opt.precision == GFEM_PREC_MIXED;
+      convert(raw, mixed, ScalarKind<T>::complex, _values, nnz);
+      _rhs.resize(_numRHS);
+      for(unsigned r = 0; r < _numRHS; ++r) {
+        st = gfem_rhs_download(_sys, opt.precision, (int)r, raw.data(), raw.size());
+        if(st != GFEM_OK) throw common::Exception(std::string("gfem_rhs_download: ") + gfem_ctx_last_error(ctx));
+        convert(raw, mixed, ScalarKind<T>::complex, _rhs[r], n);
+      }
+    }
+  }
+  catch(const std::exception &e) {
+    _lastError = e.what();
+    std::fprintf(stderr, "Error   : Unable to assemble the system\nError   : %s\n", e.what());
+    std::fill(_values.begin(), _values.end(), T(0));
+    for(auto &b : _rhs) std::fill(b.begin(), b.end(), T(0));
+  }
+  time.tock();
+  return time;
+}
+
+template class Formulation<double>;
+template class Formulation<float>;
+template class Formulation<std::complex<double>>;
+template class Formulation<std::complex<float>>;
+
+}  // namespace problem
+
+}  // namespace gmshfem

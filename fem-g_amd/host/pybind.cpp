@@ -1,0 +1,263 @@
+// Python binding of the host API (model, domain, functions, fields,
+// equation DSL, Formulation<T>).  Used by the tests and bench.py; the C++
+// API in gmshfem.hpp is the primary interface.
+#include <pybind11/complex.h>
+#include <pybind11/numpy.h>
+#include <pybind11/operators.h>
+#include <pybind11/pybind11.h>
+#include <pybind11/stl.h>
+
+#include <cstring>
+
+#include "../csrc/refelem.h"
+#include "gmshfem.hpp"
+
+namespace py = pybind11;
+using namespace gmshfem;
+using function::ScalarFunction;
+
+namespace {
+
+template <class V>
+py::array_t<V> to_numpy(const std::vector<V> &v)
+{
+  py::array_t<V> a((py::ssize_t)v.size());
+  if(!v.empty()) std::memcpy(a.mutable_data(), v.data(), v.size() * sizeof(V));
+  return a;
+}
+
+gfem_precision parse_precision(const std::string &s)
+{
+  if(s == "fp64") return GFEM_PREC_FP64;
+  if(s == "mixed") return GFEM_PREC_MIXED;
+  throw std::invalid_argument("precision must be 'fp64' or 'mixed'");
+}
+
+gfem_strategy parse_strategy(const std::string &s)
+{
+  if(s == "fused") return GFEM_STRATEGY_FUSED;
+  if(s == "reference") return GFEM_STRATEGY_REFERENCE;
+  throw std::invalid_argument("strategy must be 'fused' or 'reference'");
+}
+
+problem::DofsSort parse_sort(const std::string &s)
+{
+  if(s == "default") return problem::DofsSort::Default;
+  if(s == "hilbert") return problem::DofsSort::Hilbert;
+  if(s == "none") return problem::DofsSort::None;
+  throw std::invalid_argument("sort must be 'default', 'hilbert' or 'none'");
+}
+
+void check(gfem_status s, const char *what)
+{
+  if(s != GFEM_OK)
+    throw std::runtime_error(std::string(what) + ": " + gfem_status_string(s) + ": " +
+                             gfem_ctx_last_error(common::deviceContext()));
+}
+
+template <class T>
+void bind_formulation(py::module_ &m, const char *name)
+{
+  using F = problem::Formulation<T>;
+  py::class_<F>(m, name, py::dynamic_attr())
+    .def(py::init<const std::string &>())
+    .def(
+      "integral",
+      [](F &f, const equation::Unknown &l, const equation::Unknown &r, const domain::Domain &d, const std::string &g,
+         bool herm) { return f.integral(l, r, d, g, herm ? problem::ProductType::Hermitian : problem::ProductType::Scalar); },
+      py::arg("lhs"), py::arg("rhs"), py::arg("domain"), py::arg("integration"), py::arg("hermitian") = true)
+    .def(
+      "integral",
+      [](F &f, const ScalarFunction &src, const equation::Unknown &r, const domain::Domain &d, const std::string &g,
+         bool herm, unsigned idx) {
+        return f.integral(src, r, d, g, herm ? problem::ProductType::Hermitian : problem::ProductType::Scalar, idx);
+      },
+      py::arg("source"), py::arg("rhs"), py::arg("domain"), py::arg("integration"), py::arg("hermitian") = true,
+      py::arg("rhs_idx") = 0)
+    .def("set_symmetric", &F::setSymmetric)
+    .def("set_angular_frequency", &F::setAngularFrequency)
+    .def("num_rhs", &F::numRHS)
+    .def(
+      "pre", [](F &f, const std::string &s) { return f.pre(parse_sort(s)).time(); }, py::arg("sort") = "default")
+    .def(
+      "assemble",
+      [](F &f, const std::string &prec, const std::string &strat, bool download) {
+        problem::AssembleOptions o;
+        o.precision = parse_precision(prec);
+        o.strategy = parse_strategy(strat);
+        o.download = download;
+        double t;
+        {
+          py::gil_scoped_release rel;
+          t = f.assemble(o).time();
+        }
+        if(!f.lastError().empty()) throw std::runtime_error(f.lastError());
+        return t;
+      },
+      py::arg("precision") = "fp64", py::arg("strategy") = "fused", py::arg("download") = true)
+    .def(
+      "assemble_async",
+      [](F &f, const std::string &prec, const std::string &strat) {
+        check(gfem_assemble(f.system(), parse_precision(prec), parse_strategy(strat), nullptr), "gfem_assemble");
+      },
+      py::arg("precision") = "fp64", py::arg("strategy") = "fused")
+    .def("sync", [](F &) { check(gfem_stream_sync(common::deviceContext(), nullptr), "gfem_stream_sync"); })
+    .def("set_profiling", [](F &f, bool on) { check(gfem_set_profiling(f.system(), on ? 1 : 0), "set_profiling"); })
+    .def("kernel_times",
+         [](F &f) {
+           const char *names[256];
+           double ms[256];
+           int n = 0;
+           check(gfem_last_kernel_times(f.system(), 256, names, ms, &n), "kernel_times");
+           py::list out;
+           for(int k = 0; k < n && k < 256; ++k) out.append(py::make_tuple(std::string(names[k]), ms[k]));
+           return out;
+         })
+    .def("gather_bytes",
+         [](F &f, const std::string &prec) {
+           double b = 0.;
+           check(gfem_gather_bytes(f.system(), parse_precision(prec), &b), "gather_bytes");
+           return b;
+         })
+    .def_property_readonly("n_unknown", &F::getNumberOfUnknownDof)
+    .def_property_readonly("n_fixed", &F::getNumberOfFixedDof)
+    .def_property_readonly("nnz", &F::getNumberOfNonZeros)
+    .def_property_readonly("last_error", &F::lastError)
+    .def_property_readonly("pre_seconds", &F::preHostSeconds)
+    .def("ai", [](const F &f) { return to_numpy(f.ai()); })
+    .def("aj", [](const F &f) { return to_numpy(f.aj()); })
+    .def("values", [](const F &f) { return to_numpy(f.getLHS()); })
+    .def(
+      "rhs", [](const F &f, unsigned i) { return to_numpy(f.getRHS(i)); }, py::arg("idx") = 0)
+    .def("node_dof", &F::nodeDof)
+    .def("block_maps", [](const F &f, int dim, int tag, int type) -> py::object {
+      const auto *num = f.blockDofs(dim, tag, type);
+      if(!num) return py::none();
+      return py::make_tuple(to_numpy(*num), to_numpy(*f.blockTypes(dim, tag, type)),
+                            to_numpy(*f.blockNonzeros(dim, tag, type)));
+    });
+}
+
+}  // namespace
+
+PYBIND11_MODULE(_gfem_host, m)
+{
+  m.doc() = "gfem host API: GmshFEM Formulation over the MI355X assembly engine";
+  m.def("set_device", &common::setDevice);
+
+  py::class_<model::Model, std::shared_ptr<model::Model>>(m, "Model")
+    .def_readonly("name", &model::Model::name)
+    .def_readonly("dim", &model::Model::dim)
+    .def_property_readonly("num_nodes", &model::Model::numNodes)
+    .def("xyz",
+         [](const model::Model &mo) {
+           py::array_t<double> a({(py::ssize_t)mo.numNodes(), (py::ssize_t)3});
+           std::memcpy(a.mutable_data(), mo.xyz.data(), mo.xyz.size() * sizeof(double));
+           return a;
+         })
+    .def("entities", &model::Model::entities, py::arg("dim") = -1)
+    .def("blocks", [](const model::Model &mo) {
+      py::list out;
+      for(const auto &b : mo.blocks) {
+        py::array_t<int64_t> c({(py::ssize_t)b.size(), (py::ssize_t)b.npe});
+        if(!b.conn.empty()) std::memcpy(c.mutable_data(), b.conn.data(), b.conn.size() * sizeof(int64_t));
+        out.append(py::dict(py::arg("dim") = b.dim, py::arg("tag") = b.tag, py::arg("type") = b.type,
+                            py::arg("order") = b.order, py::arg("conn") = c));
+      }
+      return out;
+    });
+  m.def("unit_square", &model::unitSquare, py::arg("n"), py::arg("order") = 1, py::arg("jitter") = 0.,
+        py::arg("seed") = 0);
+  m.def("unit_cube", &model::unitCube, py::arg("n"), py::arg("order") = 1, py::arg("jitter") = 0.,
+        py::arg("seed") = 0);
+  m.def("set_current_model", &model::setCurrent);
+  m.def("from_arrays", [](const std::string &name, int dim, py::array_t<double> xyz, py::list blocks) {
+    std::vector<double> x(xyz.data(), xyz.data() + xyz.size());
+    std::vector<model::Block> bl;
+    for(auto item : blocks) {
+      py::dict d = item.cast<py::dict>();
+      model::Block b;
+      b.tag = d["tag"].cast<int>();
+      b.type = d["type"].cast<int>();
+      py::array_t<int64_t> c = d["conn"].cast<py::array_t<int64_t>>();
+      b.conn.assign(c.data(), c.data() + c.size());
+      bl.push_back(std::move(b));
+    }
+    return model::fromArrays(name, dim, x, bl);
+  });
+
+  py::class_<domain::Domain>(m, "Domain")
+    .def(py::init<>())
+    .def(py::init<int, int>())
+    .def(py::self | py::self)
+    .def("entities", [](const domain::Domain &d) {
+      return std::vector<std::pair<int, int>>(d.entities().begin(), d.entities().end());
+    });
+
+  py::class_<ScalarFunction>(m, "ScalarFunction")
+    .def(py::init<double>())
+    .def(py::init<std::complex<double>>())
+    .def_property_readonly("is_constant", &ScalarFunction::isConstant)
+    .def_property_readonly("value", &ScalarFunction::value)
+    .def_property_readonly("ops", &ScalarFunction::ops)
+    .def_property_readonly("consts", &ScalarFunction::consts)
+    .def("evaluate", &ScalarFunction::evaluate)
+    .def(py::self + py::self)
+    .def(py::self - py::self)
+    .def(py::self * py::self)
+    .def(py::self / py::self)
+    .def(-py::self)
+    .def("__add__", [](const ScalarFunction &a, std::complex<double> b) { return a + ScalarFunction(b); })
+    .def("__radd__", [](const ScalarFunction &a, std::complex<double> b) { return ScalarFunction(b) + a; })
+    .def("__sub__", [](const ScalarFunction &a, std::complex<double> b) { return a - ScalarFunction(b); })
+    .def("__rsub__", [](const ScalarFunction &a, std::complex<double> b) { return ScalarFunction(b) - a; })
+    .def("__mul__", [](const ScalarFunction &a, std::complex<double> b) { return a * ScalarFunction(b); })
+    .def("__rmul__", [](const ScalarFunction &a, std::complex<double> b) { return ScalarFunction(b) * a; })
+    .def("__truediv__", [](const ScalarFunction &a, std::complex<double> b) { return a / ScalarFunction(b); })
+    .def("__rtruediv__", [](const ScalarFunction &a, std::complex<double> b) { return ScalarFunction(b) / a; });
+  py::implicitly_convertible<double, ScalarFunction>();
+  py::implicitly_convertible<std::complex<double>, ScalarFunction>();
+  m.def("x", &function::x);
+  m.def("y", &function::y);
+  m.def("z", &function::z);
+  m.def("exp", &function::exp);
+  m.def("sqrt", &function::sqrt);
+  m.def("sin", &function::sin);
+  m.def("cos", &function::cos);
+  m.def("abs", &function::abs);
+  m.def("pow", &function::pow);
+
+  py::class_<field::Field>(m, "Field")
+    .def(py::init<const std::string &, const domain::Domain &, int>(), py::arg("name"), py::arg("domain"),
+         py::arg("order") = 1)
+    .def("add_constraint", &field::Field::addConstraint)
+    .def_property_readonly("order", &field::Field::order);
+
+  py::class_<equation::Unknown>(m, "Unknown")
+    .def("__rmul__", [](const equation::Unknown &u, const ScalarFunction &c) { return c * u; })
+    .def("__rmul__", [](const equation::Unknown &u, std::complex<double> c) { return ScalarFunction(c) * u; });
+  m.def("dof", &equation::dof, py::keep_alive<0, 1>());
+  m.def("tf", &equation::tf, py::keep_alive<0, 1>());
+  m.def("dt_dof", &equation::dt_dof, py::keep_alive<0, 1>());
+  m.def("dt2_dof", &equation::dt2_dof, py::keep_alive<0, 1>());
+  m.def("grad", &equation::grad);
+
+  bind_formulation<double>(m, "Formulation_f64");
+  bind_formulation<float>(m, "Formulation_f32");
+  bind_formulation<std::complex<double>>(m, "Formulation_c128");
+  bind_formulation<std::complex<float>>(m, "Formulation_c64");
+
+  m.def("hilbert_key", [](int dim, std::vector<float> f, std::vector<float> mn, std::vector<float> mx) {
+    return reorder::hilbertKey(dim, f.data(), mn.data(), mx.data());
+  });
+  m.def("lattice_nodes", [](int dim, int order) {
+    std::vector<std::vector<int>> out;
+    for(auto &l : gfem::lattice_nodes(dim, order)) out.push_back({l[0], l[1], l[2], l[3]});
+    return out;
+  });
+  m.def("gauss_rule", [](int dim, int deg) {
+    std::vector<double> p, w;
+    gfem::gauss_rule(dim, deg, p, w);
+    return py::make_tuple(p, w);
+  });
+}

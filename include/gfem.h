@@ -1,0 +1,227 @@
+/*
+ * gfem -- MI355X (gfx950) finite-element assembly engine, C ABI.
+ *
+ * This is the device-side seam of GmshFEM's assembly hot path.  The host
+ * library (fem-g_amd/host, C++ gmshfem::problem::Formulation<T>) computes the
+ * DOF numbering, the CSR pattern and the element maps in pre(), hands them
+ * over once through this ABI, and calls gfem_assemble() from assemble().
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, returns a
+ * gfem_status and never throws.  The caller owns all host buffers; the
+ * library owns all device buffers it allocates.
+ *
+ * Interfaces replaced (reference = fem-master/src):
+ *   gfem_ctx_create / destroy      omp::/numa:: runtime set-up of GmshFem,
+ *                                  common/GmshFem.cpp:42-123
+ *   gfem_nodes_upload              gmsh model node storage read by
+ *                                  gmsh::model::mesh::getJacobians,
+ *                                  term/Assembler.cpp:130-145
+ *   gfem_system_create             MatrixFactory::init + finalizePattern +
+ *                                  VectorFactory::init + setValuesDC,
+ *                                  system/MatrixFactory.cpp:99-112, 187-213,
+ *                                  problem/Formulation.cpp:1041-1054, 1159-1165
+ *   gfem_group_add                 IndiceBucket / ElementBucket of one
+ *                                  (entity, element type) pass,
+ *                                  problem/IndiceBucket.h:21, problem/ElementBucket.h:24-25,
+ *                                  term/Assembler.cpp:68-155 (assemblyInitialization)
+ *   gfem_group_add_term            Assembler::addBilinearTerm / addLinearTerm,
+ *                                  term/Assembler.cpp:38-50; term evaluators
+ *                                  term/BilinearTerm.cpp:155-273, term/LinearTerm.cpp:87-159
+ *   gfem_system_finalize           Pattern::build element maps,
+ *                                  term/Pattern.cpp:54-72 (element -> nonzero positions)
+ *   gfem_assemble                  Formulation::assemble -> _assembleDim ->
+ *                                  Assembler::assemble, problem/Formulation.cpp:1151-1215,
+ *                                  1417-1493; term/Assembler.cpp:157-494;
+ *                                  MatrixFactory::addValues (MatrixFactory.cpp:232-448);
+ *                                  VectorFactory::addValues[DC] (VectorFactory.cpp:121-260)
+ *   gfem_values_download /         MatrixFactory::getModule()->data / VectorFactory::getVector,
+ *   gfem_rhs_download              algebra/MatrixCRS.cpp (extract)
+ *
+ * Conventions kept from the reference:
+ *   - CSR rows = test-function DOFs, columns = unknown DOFs, both numDof-1,
+ *     columns ascending per row (MatrixFactory.cpp:238-283).
+ *   - the assembled system is  A x = -b  (Solver.cpp:119-123): b holds the
+ *     sum of the linear terms plus the Dirichlet lifting A_{i,D} u_D.
+ *   - Dirichlet lifting reads the local matrix at the transposed local index
+ *     values[j*n + i] (VectorFactory.cpp:177-235).
+ *   - AFrequency module: dt^2 terms scaled by -w^2, dt terms by -i w
+ *     (Assembler.cpp:263-305).
+ */
+#ifndef GFEM_H
+#define GFEM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  They map the reference's failure cases (exceptions caught in
+ * Formulation::assemble, problem/Formulation.cpp:1200-1208). */
+typedef enum {
+  GFEM_OK = 0,
+  GFEM_ERR_INVALID_ARG = 1,    /* null pointer, bad size, unknown enum       */
+  GFEM_ERR_NO_DEVICE = 2,      /* no HIP device / device index out of range  */
+  GFEM_ERR_HIP = 3,            /* a HIP runtime call failed                   */
+  GFEM_ERR_OUT_OF_MEMORY = 4,  /* device allocation failed                    */
+  GFEM_ERR_UNSUPPORTED = 5,    /* element type / form / option not supported  */
+  GFEM_ERR_STATE = 6,          /* call order violated (e.g. assemble before finalize) */
+  GFEM_ERR_NO_UNKNOWNS = 7,    /* system has no unknown DOF (Formulation.cpp:1179-1184) */
+  GFEM_ERR_TOO_LARGE = 8,      /* index does not fit the 32-bit device maps   */
+  GFEM_ERR_NONAFFINE = 9       /* element is not affine (curved high-order node) */
+} gfem_status;
+
+/* dofs::AssembleType of an element key (fem-master/src/dofs/Dof.h:27-35) */
+enum {
+  GFEM_AT_UNDEFINED = 0,
+  GFEM_AT_FIXED = 1,
+  GFEM_AT_UNKNOWN = 2,
+  GFEM_AT_UNKNOWN_BUBBLE = 3,
+  GFEM_AT_UNKNOWN_GLOBAL = 4,
+  GFEM_AT_LINKED = 5,
+  GFEM_AT_LINKED_BUBBLE = 6
+};
+
+/* Formulation<T> scalar type (problem/Formulation.cpp:1968) */
+typedef enum {
+  GFEM_SCALAR_F64 = 0,
+  GFEM_SCALAR_F32 = 1,
+  GFEM_SCALAR_C128 = 2,
+  GFEM_SCALAR_C64 = 3
+} gfem_scalar;
+
+/* Arithmetic of an assembly:
+ *   FP64  : geometry, contraction and accumulation in FP64; values stored in
+ *           the system scalar type (double / complex<double> for F64 / C128).
+ *   MIXED : geometry and Jacobian inverses in FP64, local contraction and
+ *           accumulation in FP32; values stored as float / complex<float>. */
+typedef enum {
+  GFEM_PREC_FP64 = 0,
+  GFEM_PREC_MIXED = 1
+} gfem_precision;
+
+/* Assembly strategy:
+ *   FUSED     : per-element FP64 geometry factors + atomic-free deterministic
+ *               gather through the nonzero-sorted element map (default).
+ *   REFERENCE : per-Gauss-point dense element matrices (the reference's
+ *               element loop) + atomic scatter; the plain FP64 GPU assembler. */
+typedef enum {
+  GFEM_STRATEGY_FUSED = 0,
+  GFEM_STRATEGY_REFERENCE = 1
+} gfem_strategy;
+
+/* Coefficient bytecode: pairs (op, arg), evaluated on a complex stack.
+ * Mirrors the expression-tree nodes used by the hot path
+ * (function/executionTree/ExecutionTree.h:58-153; X/Y/Z nodes
+ * nullaryOperations.h:1399-1500; Exp unaryOperations.h:959-985). */
+enum {
+  GFEM_OP_CONST = 0, /* push consts[arg] = (re, im)        */
+  GFEM_OP_X = 1,     /* push x of the physical Gauss point */
+  GFEM_OP_Y = 2,
+  GFEM_OP_Z = 3,
+  GFEM_OP_ADD = 4,
+  GFEM_OP_SUB = 5,
+  GFEM_OP_MUL = 6,
+  GFEM_OP_DIV = 7,
+  GFEM_OP_NEG = 8,
+  GFEM_OP_EXP = 9,
+  GFEM_OP_SQRT = 10,
+  GFEM_OP_SIN = 11,
+  GFEM_OP_COS = 12,
+  GFEM_OP_POW = 13,
+  GFEM_OP_ABS = 14
+};
+
+#define GFEM_MAX_OPS 64
+#define GFEM_MAX_CONSTS 32
+#define GFEM_MAX_STACK 16
+
+/* One integral() term (problem/Formulation.h:103-180).  Bilinear terms read
+ * the dof side (form_dof, coef) and the test-function side (form_tf,
+ * tf_coef); linear terms read the source (coef) and the test-function side. */
+typedef struct {
+  int kind;        /* 0 bilinear, 1 linear                                  */
+  int gauss;       /* "GaussN": quadrature exact to total degree N          */
+  int form_dof;    /* 0: dof(u), 1: grad(dof(u))   (bilinear only)         */
+  int form_tf;     /* 0: tf(u),  1: grad(tf(u))                             */
+  int hermitian;   /* ProductType::Hermitian (1) or Scalar (0)              */
+  int dt_order;    /* UnknownFieldType NotDt (0), Dt (1), DtDt (2)          */
+  int rhs_idx;     /* linear terms: index of the right-hand side            */
+  double coef_re, coef_im;       /* constant coefficient when n_ops == 0   */
+  int n_ops;                     /* bytecode length in (op, arg) pairs      */
+  const int32_t *ops;            /* 2*n_ops ints                            */
+  int n_consts;
+  const double *consts;          /* 2*n_consts doubles (re, im)             */
+  double tf_coef_re, tf_coef_im; /* constant coefficient on the tf side     */
+} gfem_term_desc;
+
+typedef struct gfem_ctx gfem_ctx;
+typedef struct gfem_system gfem_system;
+
+const char *gfem_status_string(gfem_status s);
+
+/* Device context: one per GPU the caller uses. */
+gfem_status gfem_ctx_create(int device, gfem_ctx **out);
+gfem_status gfem_ctx_destroy(gfem_ctx *ctx);
+/* last error message of this context (never NULL) */
+const char *gfem_ctx_last_error(const gfem_ctx *ctx);
+
+/* Mesh nodes, node tag t <-> xyz[3*(t-1) .. 3*(t-1)+2]. */
+gfem_status gfem_nodes_upload(gfem_ctx *ctx, int64_t n_nodes, const double *xyz);
+
+/* System = CSR pattern (unknown rows x unknown cols) + RHS vectors.
+ * ai[n+1], aj[ai[n]] are the finalized pattern; values_dc[2*n_fixed] are the
+ * Dirichlet values (re, im) indexed by fixed numDof-1. */
+gfem_status gfem_system_create(gfem_ctx *ctx, gfem_scalar scalar, int64_t n_unknown, const int64_t *ai,
+                               const int64_t *aj, int symmetric, int64_t n_fixed, const double *values_dc,
+                               int n_rhs, double frequency, gfem_system **out);
+gfem_status gfem_system_destroy(gfem_system *sys);
+
+/* One (entity, element type) pass, in the assembly traversal order
+ * (dimension 3..0, element type ascending, entity tag ascending).
+ *   conn[ne*npe]        node tags (1-based) of each element, gmsh local order
+ *   dof_num[ne*npe]     numDof-1 of each key (IndiceBucket.first)
+ *   dof_type[ne*npe]    AssembleType of each key (IndiceBucket.second)
+ *   elem2nnz[ne*npe*npe] CSR position of (tf row i, dof col j) at
+ *                       [e*npe*npe + i*npe + j], -1 when not stored */
+gfem_status gfem_group_add(gfem_system *sys, int dim, int gmsh_type, int64_t ne, const int64_t *conn,
+                           const int64_t *dof_num, const int32_t *dof_type, const int64_t *elem2nnz,
+                           int *group_out);
+gfem_status gfem_group_add_term(gfem_system *sys, int group, const gfem_term_desc *term);
+
+/* Build the device maps (nonzero-sorted element map, RHS maps, reference
+ * tensors).  Must follow the last gfem_group_add[_term]. */
+gfem_status gfem_system_finalize(gfem_system *sys);
+
+/* Assemble A and every b on `stream` (a hipStream_t, NULL = null stream).
+ * Asynchronous: values are ready when the stream reaches this point. */
+gfem_status gfem_assemble(gfem_system *sys, gfem_precision prec, gfem_strategy strategy, void *stream);
+
+/* Wait for the work queued on `stream`. */
+gfem_status gfem_stream_sync(gfem_ctx *ctx, void *stream);
+
+/* Sizes: nnz, n_unknown, bytes of one stored value for `prec`. */
+gfem_status gfem_system_info(const gfem_system *sys, gfem_precision prec, int64_t *nnz, int64_t *n_unknown,
+                             int *value_bytes);
+/* Copy the last assembled values of `prec` to the host (value_bytes each,
+ * std::complex layout for complex scalars). */
+gfem_status gfem_values_download(gfem_system *sys, gfem_precision prec, void *host, size_t bytes);
+gfem_status gfem_rhs_download(gfem_system *sys, gfem_precision prec, int rhs_idx, void *host, size_t bytes);
+/* Device pointers of the last assembled values / RHS (owned by the system). */
+gfem_status gfem_values_device(gfem_system *sys, gfem_precision prec, void **dev_values, void **dev_rhs0);
+
+/* Per-kernel timing of the last gfem_assemble on its stream, from HIP events
+ * recorded around each launch when profiling is on.  names[i] points to
+ * static strings.  Returns the number of kernels timed in *count. */
+gfem_status gfem_set_profiling(gfem_system *sys, int on);
+gfem_status gfem_last_kernel_times(const gfem_system *sys, int max, const char **names, double *ms, int *count);
+
+/* Algorithmic HBM bytes of the fused matrix gather kernel for `prec`
+ * (contribution map + element factors + CSR values written). */
+gfem_status gfem_gather_bytes(const gfem_system *sys, gfem_precision prec, double *bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GFEM_H */

@@ -1,0 +1,144 @@
+"""HIP engine vs the CPU oracle (same seeded inputs, same CSR pattern).
+
+Bars: DOF numbering, CSR pattern and element maps bit-exact; FP64 matrix and
+RHS entries within 1e-12 relative (tests/_helpers.assert_close); mixed
+precision (FP32 contraction) within 2e-5 relative; the fused path is
+deterministic (bitwise equal on repeated runs)."""
+import math
+
+import numpy as np
+import pytest
+
+from _helpers import assert_close
+
+pytestmark = pytest.mark.gpu
+
+FP64_RTOL = 1e-12
+MIXED_RTOL = 2e-5
+
+
+def _pair(spec, scalar=None):
+    from gfem import workloads as W
+    from oracle import oracle as O
+    F = W.build(spec, scalar)
+    F.pre()
+    P = O.from_spec(spec)
+    P.pre()
+    P.assemble()
+    ai, aj = P.csr()
+    assert np.array_equal(F.ai(), ai) and np.array_equal(F.aj(), aj)
+    return F, P
+
+
+def _cases():
+    from gfem import workloads as W
+    return {
+        "poisson2d_p1": lambda: W.poisson2d(n=12, order=1),
+        "poisson2d_p2": lambda: W.poisson2d(n=7, order=2),
+        "poisson2d_p3": lambda: W.poisson2d(n=5, order=3),
+        "helmholtz3d_p1": lambda: W.helmholtz3d(n=5, order=1),
+        "helmholtz3d_p2": lambda: W.helmholtz3d(n=4, order=2),
+        "helmholtz3d_p3": lambda: W.helmholtz3d(n=2, order=3),
+        "highorder3d_p4": lambda: W.highorder3d(n=2, order=4, gauss=8),
+    }
+
+
+CASES = ["poisson2d_p1", "poisson2d_p2", "poisson2d_p3", "helmholtz3d_p1", "helmholtz3d_p2", "helmholtz3d_p3",
+         "highorder3d_p4"]
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_fused_fp64_matches_oracle(case):
+    print(f"case {case} fused fp64", flush=True)
+    F, P = _pair(_cases()[case]())
+    F.assemble(precision="fp64", strategy="fused")
+    assert_close(F.values(), P.values(), FP64_RTOL, case + " A")
+    assert_close(F.rhs(0), P.rhs(0), FP64_RTOL, case + " b")
+
+
+@pytest.mark.parametrize("case", CASES)
+def test_reference_strategy_matches_oracle(case):
+    print(f"case {case} reference fp64", flush=True)
+    F, P = _pair(_cases()[case]())
+    F.assemble(precision="fp64", strategy="reference")
+    assert_close(F.values(), P.values(), FP64_RTOL, case + " A")
+    assert_close(F.rhs(0), P.rhs(0), FP64_RTOL, case + " b")
+
+
+@pytest.mark.parametrize("case", ["poisson2d_p2", "helmholtz3d_p2", "highorder3d_p4"])
+def test_mixed_matches_oracle(case):
+    print(f"case {case} fused mixed", flush=True)
+    F, P = _pair(_cases()[case]())
+    F.assemble(precision="mixed", strategy="fused")
+    assert_close(F.values(), P.values(), MIXED_RTOL, case + " A (mixed)")
+    assert_close(F.rhs(0), P.rhs(0), MIXED_RTOL, case + " b (mixed)")
+
+
+def test_fused_is_deterministic():
+    from gfem import workloads as W
+    print("case determinism helmholtz3d n=6 p2", flush=True)
+    F = W.build(W.helmholtz3d(n=6, order=2))
+    F.pre()
+    F.assemble()
+    a1, b1 = F.values().copy(), F.rhs(0).copy()
+    for _ in range(3):
+        F.assemble()
+        assert np.array_equal(F.values(), a1) and np.array_equal(F.rhs(0), b1)
+
+
+def test_dirichlet_lifting_nonzero_value():
+    """u = 1 + 2i on the boundary: b carries A_{i,D} u_D (VectorFactory::addValuesDC)."""
+    from gfem import workloads as W
+    print("case dirichlet lifting", flush=True)
+    spec = W.poisson2d(n=9, order=2)
+    spec.dirichlet = [(spec.dirichlet[0][0], 1.0 + 2.0j)]
+    spec.scalar = "c128"
+    F, P = _pair(spec)
+    F.assemble()
+    assert np.abs(P.rhs(0)).max() > 0
+    assert_close(F.values(), P.values(), FP64_RTOL, "A")
+    assert_close(F.rhs(0), P.rhs(0), FP64_RTOL, "b")
+
+
+def test_multiple_rhs_scalar_product_expression_coefficient():
+    from gfem import workloads as W, x, y, sin
+    print("case multi-rhs / scalar product / per-point coefficient", flush=True)
+    spec = W.poisson2d(n=8, order=2)
+    spec.scalar = "c128"
+    omega = spec.field_domain
+    spec.terms = [
+        dict(kind="bilinear", domain=omega, gauss=4, form=1, coef=1.0 + x() * y()),
+        dict(kind="bilinear", domain=omega, gauss=4, form=0, coef=2.0 - 1.0j, tf_coef=0.5 + 0.25j, hermitian=False),
+        dict(kind="linear", domain=omega, gauss=4, source=sin(x() * 3.0) + 1j, rhs_idx=0),
+        dict(kind="linear", domain=omega, gauss=3, source=-2.0, tf_coef=1j, rhs_idx=1),
+    ]
+    F, P = _pair(spec)
+    for strat in ("fused", "reference"):
+        F.assemble(strategy=strat)
+        assert_close(F.values(), P.values(), FP64_RTOL, strat + " A")
+        for r in range(2):
+            assert_close(F.rhs(r), P.rhs(r), FP64_RTOL, f"{strat} b{r}")
+
+
+def test_frequency_module_dt_terms():
+    from gfem import workloads as W
+    print("case AFrequency module", flush=True)
+    spec = W.helmholtz3d(n=3, order=2)
+    spec.frequency = 3.0
+    spec.terms.append(dict(kind="bilinear", domain=[(3, 1)], gauss=4, form=0, coef=0.7, dt=2))
+    spec.terms.append(dict(kind="bilinear", domain=[(2, 1), (2, 2)], gauss=4, form=0, coef=0.3, dt=1))
+    F, P = _pair(spec)
+    F.assemble()
+    assert_close(F.values(), P.values(), FP64_RTOL, "A")
+    assert_close(F.rhs(0), P.rhs(0), FP64_RTOL, "b")
+
+
+def test_symmetric_storage():
+    from gfem import workloads as W
+    print("case symmetric storage", flush=True)
+    spec = W.poisson2d(n=10, order=2)
+    spec.symmetric = True
+    F, P = _pair(spec)
+    F.assemble()
+    assert_close(F.values(), P.values(), FP64_RTOL, "A")
+    assert_close(F.rhs(0), P.rhs(0), FP64_RTOL, "b")
