@@ -342,6 +342,8 @@ struct Quant {
 };
 }  // namespace
 
+uint64_t hilbertIndex(int dim, const uint32_t *u, int top) { return Curve(dim).index(u, top); }
+
 uint64_t hilbertKey(int dim, const float *f, const float *min, const float *max)
 {
   const Curve c(dim);

@@ -245,6 +245,8 @@ class Formulation {
 namespace reorder {
 // Hilbert key of a point of a 2-D / 3-D box (fem-master/src/reorder/Hilbert.cpp:17-271)
 uint64_t hilbertKey(int dim, const float *f, const float *min, const float *max);
+// Hilbert index of integer cell coordinates u, levels top..0
+uint64_t hilbertIndex(int dim, const uint32_t *u, int top);
 // stable ordering of points along the Hilbert curve
 std::vector<int64_t> hilbertOrder(int dim, const std::vector<float> &coords, const float *min, const float *max);
 }  // namespace reorder

@@ -250,6 +250,9 @@ PYBIND11_MODULE(_gfem_host, m)
   m.def("hilbert_key", [](int dim, std::vector<float> f, std::vector<float> mn, std::vector<float> mx) {
     return reorder::hilbertKey(dim, f.data(), mn.data(), mx.data());
   });
+  m.def("hilbert_index", [](int dim, std::vector<uint32_t> u, int top) {
+    return reorder::hilbertIndex(dim, u.data(), top);
+  });
   m.def("lattice_nodes", [](int dim, int order) {
     std::vector<std::vector<int>> out;
     for(auto &l : gfem::lattice_nodes(dim, order)) out.push_back({l[0], l[1], l[2], l[3]});

@@ -258,6 +258,15 @@ static int ref_nodes(int dim, int p, int out[][4])
   return nodes_tet(p, (const int(*)[4])P, out, 0);
 }
 
+int or_ref_nodes(int dim, int p, int *out)
+{
+  int nodes[64][4];
+  const int n = ref_nodes(dim, p, nodes);
+  for(int a = 0; a < n; ++a)
+    for(int k = 0; k < 4; ++k) out[4 * a + k] = nodes[a][k];
+  return n;
+}
+
 /* barycentric coordinates of a point of the gmsh reference element */
 static void bary(int dim, const double *u, double *lam, double dlam[4][3])
 {
@@ -508,6 +517,13 @@ static void hilbert_scale(int dim, const float *min, const float *max, float *sc
     scale[i] = (float)two / (max[i] - min[i]);
     smin[i] = -min[i] * scale[i];
   }
+}
+
+uint64_t or_hilbert_index(int dim, const uint32_t *u, int begbit)
+{
+  hilbert_tab H;
+  hilbert_init(&H, dim);
+  return hilbert_index(&H, u, begbit);
 }
 
 uint64_t or_hilbert_key(int dim, const float *f, const float *min, const float *max)

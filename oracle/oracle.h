@@ -91,9 +91,12 @@ int or_get_rhs(const or_problem *p, int idx, double *re, double *im);
 
 /* helpers exposed for the oracle's own pinning tests */
 uint64_t or_hilbert_key(int dim, const float *f, const float *min, const float *max);
+uint64_t or_hilbert_index(int dim, const uint32_t *u, int begbit);
 int or_gauss_rule(int dim, int degree, double *pts /* 3*maxq */, double *w, int maxq);
 int or_lagrange(int dim, int order, const double *uvw, double *val, double *grad /* 3*N */);
 int or_elem_nodes(int gmsh_type, int *dim, int *order);
+/* lattice multi-indices (4 ints each) of the nodes of an order-p element */
+int or_ref_nodes(int dim, int p, int *out);
 
 #ifdef __cplusplus
 }

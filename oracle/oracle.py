@@ -64,6 +64,9 @@ def lib():
         L.or_gauss_rule.argtypes = [ctypes.c_int, ctypes.c_int, _c_dp, _c_dp, ctypes.c_int]
         L.or_lagrange.argtypes = [ctypes.c_int, ctypes.c_int, _c_dp, _c_dp, _c_dp]
         L.or_elem_nodes.argtypes = [ctypes.c_int, _c_ip, _c_ip]
+        L.or_ref_nodes.argtypes = [ctypes.c_int, ctypes.c_int, _c_ip]
+        L.or_hilbert_index.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_uint32), ctypes.c_int]
+        L.or_hilbert_index.restype = ctypes.c_uint64
         _lib = L
     return _lib
 
@@ -212,6 +215,19 @@ def hilbert_key(dim, f, mn, mx):
     b = (ctypes.c_float * 3)(*mn)
     c = (ctypes.c_float * 3)(*mx)
     return L.or_hilbert_key(dim, a, b, c)
+
+
+def hilbert_index(dim, u, top):
+    L = lib()
+    a = (ctypes.c_uint32 * 3)(*(list(u) + [0] * (3 - len(u))))
+    return L.or_hilbert_index(dim, a, top)
+
+
+def ref_nodes(dim, order):
+    L = lib()
+    out = (ctypes.c_int * (4 * 64))()
+    n = L.or_ref_nodes(dim, order, out)
+    return [list(out[4 * a: 4 * a + 4]) for a in range(n)]
 
 
 def gauss_rule(dim, degree, maxq=4096):
