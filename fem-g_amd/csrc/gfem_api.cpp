@@ -67,6 +67,7 @@ void free_group(Group &g)
   dfree(g.d_vtx);
   dfree(g.d_nodes);
   dfree(g.d_terms);
+  dfree(g.d_dev);
   dfree(g.d_F64);
   dfree(g.d_F32);
   dfree(g.d_T64);
@@ -253,6 +254,7 @@ gfem_status gfem_system_destroy(gfem_system *sys)
   clear_prof(sys);
   for(auto &g : sys->groups) free_group(g);
   dfree(sys->d_dc);
+  dfree(sys->d_table);
   dfree(sys->d_words);
   dfree(sys->d_chunk_c);
   dfree(sys->d_chunk_n);
@@ -466,6 +468,19 @@ gfem_status gfem_system_finalize(gfem_system *sys)
       GFEM_HIP(ctx, launch_basis(g, R, nullptr));
     }
     GFEM_HIP(ctx, launch_tensor(g, nullptr));
+    GroupDev gd{};
+    gd.dim = g.dim;
+    gd.nv = g.nv;
+    gd.N = g.N;
+    gd.nterms = (int)g.dterms.size();
+    gd.terms = g.d_terms;
+    for(size_t r = 0; r < g.rules.size(); ++r)
+      gd.rules[r] = RuleDev{g.rules[r].Q, g.rules[r].d_w, g.rules[r].d_pts, g.rules[r].d_bf, g.rules[r].d_gbf};
+    GFEM_HIP(ctx, dupload(&g.d_dev, &gd, 1));
+  }
+  {
+    const GroupTable gt = make_table(sys);
+    GFEM_HIP(ctx, dupload(&sys->d_table, &gt, 1));
   }
 
   // nonzero-sorted element map (contribution words), traversal order kept
@@ -633,16 +648,15 @@ gfem_status gfem_assemble(gfem_system *sys, gfem_precision prec, gfem_strategy s
       }
       if(g.has_dense) {
         prof.begin("dense");
-        GFEM_HIP(ctx, launch_dense(g, ctx->d_xyz, false, sys->freq_module, sys->frequency, sys->cplx, s));
+        GFEM_HIP(ctx, launch_dense(g, ctx->d_xyz, false, s));
         prof.end();
       }
     }
-    const GroupTable gt = make_table(sys);
     prof.begin("gather");
-    GFEM_HIP(ctx, launch_gather(*sys, gt, p, s));
+    GFEM_HIP(ctx, launch_gather(*sys, p, s));
     prof.end();
     prof.begin("rhs");
-    GFEM_HIP(ctx, launch_rhs(*sys, gt, p, s));
+    GFEM_HIP(ctx, launch_rhs(*sys, p, s));
     prof.end();
     return GFEM_OK;
   }
@@ -661,7 +675,7 @@ gfem_status gfem_assemble(gfem_system *sys, gfem_precision prec, gfem_strategy s
     }
     if(g.has_bilinear) {
       prof.begin("ref_dense");
-      GFEM_HIP(ctx, launch_dense(g, ctx->d_xyz, true, sys->freq_module, sys->frequency, sys->cplx, s));
+      GFEM_HIP(ctx, launch_dense(g, ctx->d_xyz, true, s));
       prof.end();
       prof.begin("ref_scatter");
       GFEM_HIP(ctx, launch_ref_scatter(*sys, g, s));

@@ -47,7 +47,7 @@ struct DevTerm {
   DevExpr expr;
 };
 
-// Per-group tables passed by value to the gather / rhs kernels.
+// Per-group tables of a system, device-resident, read by the gather / rhs kernels.
 struct GroupTable {
   int ng;
   int logP;
@@ -60,6 +60,17 @@ struct GroupTable {
   const double2 *dense[kMaxGroups];  // [ne][N*N] per-point bilinear terms, or null
   const double2 *be[kMaxGroups];     // [nrhs][ne][N] linear terms, or null
   int64_t ne[kMaxGroups];
+};
+
+// Device-resident descriptor of one group's rules and terms.
+struct RuleDev {
+  int Q;
+  const double *w, *pts, *bf, *gbf;
+};
+struct GroupDev {
+  int dim, nv, N, nterms;
+  const DevTerm *terms;
+  RuleDev rules[8];
 };
 
 struct Rule {
@@ -87,6 +98,7 @@ struct Group {
   int32_t *d_vtx = nullptr;
   int *d_nodes = nullptr;  // reference lattice multi-indices [N][4]
   DevTerm *d_terms = nullptr;
+  GroupDev *d_dev = nullptr;
   double *d_F64 = nullptr;
   float *d_F32 = nullptr;
   double2 *d_T64 = nullptr;
@@ -123,6 +135,7 @@ struct gfem_system {
   std::vector<gfem::Group> groups;
   std::vector<double> h_dc;
   double2 *d_dc = nullptr;
+  gfem::GroupTable *d_table = nullptr;
   // fused maps
   uint32_t *d_words = nullptr;
   int64_t n_words = 0;
@@ -150,10 +163,9 @@ hipError_t launch_basis(const Group &g, Rule &r, hipStream_t s);
 hipError_t launch_tensor(const Group &g, hipStream_t s);
 hipError_t launch_geometry(const Group &g, const double *xyz, hipStream_t s);
 hipError_t launch_linear(const Group &g, const double *xyz, int nrhs, hipStream_t s);
-hipError_t launch_dense(const Group &g, const double *xyz, bool all_terms, bool freq_module, double freq,
-                        bool cplx, hipStream_t s);
-hipError_t launch_gather(const gfem_system &sys, const GroupTable &gt, int prec, hipStream_t s);
-hipError_t launch_rhs(const gfem_system &sys, const GroupTable &gt, int prec, hipStream_t s);
+hipError_t launch_dense(const Group &g, const double *xyz, bool all_terms, hipStream_t s);
+hipError_t launch_gather(const gfem_system &sys, int prec, hipStream_t s);
+hipError_t launch_rhs(const gfem_system &sys, int prec, hipStream_t s);
 hipError_t launch_ref_scatter(const gfem_system &sys, const Group &g, hipStream_t s);
 hipError_t launch_ref_rhs(const gfem_system &sys, const Group &g, hipStream_t s);
 

@@ -58,53 +58,70 @@ __device__ __forceinline__ double2 csqrt_(double2 a)
   return (a.x >= 0.) ? make_double2(re, im) : make_double2(fabs(im), a.y >= 0. ? re : -re);
 }
 
-// Coefficient program: complex stack machine (GFEM_OP_*).
-__device__ double2 eval_expr(const DevExpr &ex, double x, double y, double z)
+__device__ __forceinline__ double2 cunary(int op, double2 a)
 {
-  double2 st[GFEM_MAX_STACK];
-  int sp = 0;
-  for(int k = 0; k < ex.n_ops; ++k) {
-    const int op = ex.ops[2 * k], arg = ex.ops[2 * k + 1];
-    switch(op) {
-    case GFEM_OP_CONST: st[sp++] = make_double2(ex.consts[2 * arg], ex.consts[2 * arg + 1]); break;
-    case GFEM_OP_X: st[sp++] = make_double2(x, 0.); break;
-    case GFEM_OP_Y: st[sp++] = make_double2(y, 0.); break;
-    case GFEM_OP_Z: st[sp++] = make_double2(z, 0.); break;
-    case GFEM_OP_ADD: --sp; st[sp - 1] = cadd(st[sp - 1], st[sp]); break;
-    case GFEM_OP_SUB: --sp; st[sp - 1] = csub(st[sp - 1], st[sp]); break;
-    case GFEM_OP_MUL: --sp; st[sp - 1] = cmul(st[sp - 1], st[sp]); break;
-    case GFEM_OP_DIV: --sp; st[sp - 1] = cdiv(st[sp - 1], st[sp]); break;
-    case GFEM_OP_NEG: st[sp - 1] = make_double2(-st[sp - 1].x, -st[sp - 1].y); break;
-    case GFEM_OP_EXP: st[sp - 1] = cexp_(st[sp - 1]); break;
-    case GFEM_OP_SQRT: st[sp - 1] = csqrt_(st[sp - 1]); break;
-    case GFEM_OP_SIN: {
-      const double2 a = st[sp - 1];
-      double s, c;
-      sincos(a.x, &s, &c);
-      st[sp - 1] = make_double2(s * cosh(a.y), c * sinh(a.y));
-      break;
-    }
-    case GFEM_OP_COS: {
-      const double2 a = st[sp - 1];
-      double s, c;
-      sincos(a.x, &s, &c);
-      st[sp - 1] = make_double2(c * cosh(a.y), -s * sinh(a.y));
-      break;
-    }
-    case GFEM_OP_POW: {
-      --sp;
-      const double2 a = st[sp - 1], b = st[sp];
-      if(a.y == 0. && b.y == 0. && (a.x > 0. || b.x == floor(b.x)))
-        st[sp - 1] = make_double2(pow(a.x, b.x), 0.);
+  switch(op) {
+  case GFEM_OP_NEG: return make_double2(-a.x, -a.y);
+  case GFEM_OP_EXP:
+    if(a.y == 0.) return make_double2(exp(a.x), 0.);
+    return cexp_(a);
+  case GFEM_OP_SQRT: return csqrt_(a);
+  case GFEM_OP_SIN: {
+    double s, c;
+    sincos(a.x, &s, &c);
+    return a.y == 0. ? make_double2(s, 0.) : make_double2(s * cosh(a.y), c * sinh(a.y));
+  }
+  case GFEM_OP_COS: {
+    double s, c;
+    sincos(a.x, &s, &c);
+    return a.y == 0. ? make_double2(c, 0.) : make_double2(c * cosh(a.y), -s * sinh(a.y));
+  }
+  case GFEM_OP_ABS: return make_double2(hypot(a.x, a.y), 0.);
+  default: return a;
+  }
+}
+
+__device__ __forceinline__ double2 cbinary(int op, double2 a, double2 b)
+{
+  switch(op) {
+  case GFEM_OP_ADD: return cadd(a, b);
+  case GFEM_OP_SUB: return csub(a, b);
+  case GFEM_OP_MUL: return cmul(a, b);
+  case GFEM_OP_DIV: return cdiv(a, b);
+  case GFEM_OP_POW:
+    if(a.y == 0. && b.y == 0. && (a.x > 0. || b.x == floor(b.x))) return make_double2(pow(a.x, b.x), 0.);
+    return (a.x == 0. && a.y == 0.) ? make_double2(0., 0.) : cexp_(cmul(b, clog_(a)));
+  default: return a;
+  }
+}
+
+// Coefficient program: complex stack machine (GFEM_OP_*) with the stack in
+// eight registers (push shifts down, binary ops shift up).  The program is
+// the same for every lane, so the switch is uniform and nothing spills to
+// scratch memory.
+__device__ double2 eval_expr(const DevExpr *__restrict__ ex, double x, double y, double z)
+{
+  double2 s0 = make_double2(0., 0.), s1 = s0, s2 = s0, s3 = s0, s4 = s0, s5 = s0, s6 = s0, s7 = s0;
+  const int n = ex->n_ops;
+  for(int k = 0; k < n; ++k) {
+    const int op = ex->ops[2 * k], arg = ex->ops[2 * k + 1];
+    if(op <= GFEM_OP_Z) {
+      double2 v;
+      if(op == GFEM_OP_CONST)
+        v = make_double2(ex->consts[2 * arg], ex->consts[2 * arg + 1]);
       else
-        st[sp - 1] = (a.x == 0. && a.y == 0.) ? make_double2(0., 0.) : cexp_(cmul(b, clog_(a)));
-      break;
+        v = make_double2(op == GFEM_OP_X ? x : (op == GFEM_OP_Y ? y : z), 0.);
+      s7 = s6; s6 = s5; s5 = s4; s4 = s3; s3 = s2; s2 = s1; s1 = s0; s0 = v;
     }
-    case GFEM_OP_ABS: st[sp - 1] = make_double2(hypot(st[sp - 1].x, st[sp - 1].y), 0.); break;
-    default: break;
+    else if((op >= GFEM_OP_ADD && op <= GFEM_OP_DIV) || op == GFEM_OP_POW) {
+      s0 = cbinary(op, s1, s0);
+      s1 = s2; s2 = s3; s3 = s4; s4 = s5; s5 = s6; s6 = s7;
+    }
+    else {
+      s0 = cunary(op, s0);
     }
   }
-  return sp > 0 ? st[sp - 1] : make_double2(0., 0.);
+  return s0;
 }
 
 // Affine element geometry from its vertices: J columns (col-major
@@ -329,46 +346,42 @@ __global__ void __launch_bounds__(256) k_geometry(int dim, int nv, int64_t ne, c
 }
 
 // ------------------------------------------------------------------ k_linear
-struct LinearArgs {
-  int dim, nv, N, nterms, rhs;
-  const DevTerm *terms;
-  int Q[8];
-  const double *w[8];
-  const double *pts[8];
-  const double *bf[8];
-};
-
+// One thread per element: b_e(i) += w det conj(c_tf) phi_i f(x_q) over the
+// group's linear terms of right-hand side `rhs` (tables read through the
+// device-resident group descriptor).
 template <int N>
-__global__ void __launch_bounds__(128) k_linear(LinearArgs a, int64_t ne, const int32_t *__restrict__ vtx,
-                                                const double *__restrict__ xyz, const double *__restrict__ F64,
-                                                double2 *__restrict__ be)
+__global__ void __launch_bounds__(128) k_linear(const GroupDev *__restrict__ gd, int rhs, int64_t ne,
+                                                const int32_t *__restrict__ vtx, const double *__restrict__ xyz,
+                                                const double *__restrict__ F64, double2 *__restrict__ be)
 {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if(e >= ne) return;
+  const int dim = gd->dim;
   double X[4][3];
-  load_vertices(vtx, a.nv, e, xyz, X);
+  load_vertices(vtx, gd->nv, e, xyz, X);
   const double det = F64[e * 8 + 6];
   double2 acc[N];
   for(int i = 0; i < N; ++i) acc[i] = make_double2(0., 0.);
-  for(int t = 0; t < a.nterms; ++t) {
-    const DevTerm &dt = a.terms[t];
-    if(dt.kind != 1 || dt.rhs_idx != a.rhs) continue;
-    const int r = dt.rule;
-    for(int q = 0; q < a.Q[r]; ++q) {
-      double2 f;
-      if(dt.per_point) {
+  const int nterms = gd->nterms;
+  for(int t = 0; t < nterms; ++t) {
+    const DevTerm *dt = &gd->terms[t];
+    if(dt->kind != 1 || dt->rhs_idx != rhs) continue;
+    const RuleDev &R = gd->rules[dt->rule];
+    const int Q = R.Q;
+    const bool pp = dt->per_point != 0;
+    const double2 fac = make_double2(dt->fac_re, dt->fac_im), cst = make_double2(dt->c_re, dt->c_im);
+    for(int q = 0; q < Q; ++q) {
+      double2 f = cst;
+      if(pp) {
         double x[3];
-        phys_point(a.dim, X, &a.pts[r][3 * q], x);
-        f = eval_expr(dt.expr, x[0], x[1], x[2]);
+        phys_point(dim, X, &R.pts[3 * q], x);
+        f = eval_expr(&dt->expr, x[0], x[1], x[2]);
       }
-      else {
-        f = make_double2(dt.c_re, dt.c_im);
-      }
-      const double wd = a.w[r][q] * det;
-      const double2 cf = cmul(make_double2(dt.fac_re * wd, dt.fac_im * wd), f);
+      const double wd = R.w[q] * det;
+      const double2 cf = cmul(make_double2(fac.x * wd, fac.y * wd), f);
 #pragma unroll
       for(int i = 0; i < N; ++i) {
-        const double b = a.bf[r][q * N + i];
+        const double b = R.bf[q * N + i];
         acc[i].x += cf.x * b;
         acc[i].y += cf.y * b;
       }
@@ -379,44 +392,39 @@ __global__ void __launch_bounds__(128) k_linear(LinearArgs a, int64_t ne, const 
 }
 
 // ------------------------------------------------------------------ k_dense
-struct DenseArgs {
-  int dim, nv, N, nterms, all_terms, freq_module, cplx;
-  double freq;
-  const DevTerm *terms;
-  int Q[8];
-  const double *w[8];
-  const double *pts[8];
-  const double *bf[8];
-  const double *gbf[8];
-};
-
 // one thread per (element, pair): the reference's per-Gauss-point loop
 // A_e(j,i) += w det (c E_j) . conj(c_tf E_i)
-__global__ void __launch_bounds__(256) k_dense(DenseArgs a, int64_t ne, const int32_t *__restrict__ vtx,
-                                               const double *__restrict__ xyz, double2 *__restrict__ Ae)
+__global__ void __launch_bounds__(256) k_dense(const GroupDev *__restrict__ gd, int all_terms, int64_t ne,
+                                               const int32_t *__restrict__ vtx, const double *__restrict__ xyz,
+                                               double2 *__restrict__ Ae)
 {
-  const int NN = a.N * a.N;
+  const int N = gd->N;
+  const int NN = N * N;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if(t >= ne * NN) return;
   const int64_t e = t / NN;
   const int pair = (int)(t % NN);
-  const int i = pair / a.N, j = pair % a.N;
+  const int i = pair / N, j = pair % N;
+  const int dim = gd->dim;
   double X[4][3];
-  load_vertices(vtx, a.nv, e, xyz, X);
+  load_vertices(vtx, gd->nv, e, xyz, X);
   Geo g;
-  element_geometry(a.dim, X, g);
+  element_geometry(dim, X, g);
   double2 acc = make_double2(0., 0.);
-  for(int k = 0; k < a.nterms; ++k) {
-    const DevTerm &dt = a.terms[k];
-    if(dt.kind != 0) continue;
-    if(!a.all_terms && !dt.per_point) continue;
-    const int r = dt.rule;
+  const int nterms = gd->nterms;
+  for(int k = 0; k < nterms; ++k) {
+    const DevTerm *dt = &gd->terms[k];
+    if(dt->kind != 0) continue;
+    if(!all_terms && !dt->per_point) continue;
+    const RuleDev &R = gd->rules[dt->rule];
+    const int Q = R.Q;
+    const bool pp = dt->per_point != 0;
     double2 sum = make_double2(0., 0.);
-    for(int q = 0; q < a.Q[r]; ++q) {
+    for(int q = 0; q < Q; ++q) {
       double s;
-      if(dt.form_dof == 1) {
-        const double *gj = &a.gbf[r][(q * a.N + j) * 3];
-        const double *gi = &a.gbf[r][(q * a.N + i) * 3];
+      if(dt->form_dof == 1) {
+        const double *gj = &R.gbf[(q * N + j) * 3];
+        const double *gi = &R.gbf[(q * N + i) * 3];
         s = 0.;
         for(int c = 0; c < 3; ++c) {
           const double ej = g.Ki[c][0] * gj[0] + g.Ki[c][1] * gj[1] + g.Ki[c][2] * gj[2];
@@ -425,22 +433,19 @@ __global__ void __launch_bounds__(256) k_dense(DenseArgs a, int64_t ne, const in
         }
       }
       else {
-        s = a.bf[r][q * a.N + j] * a.bf[r][q * a.N + i];
+        s = R.bf[q * N + j] * R.bf[q * N + i];
       }
-      double2 c;
-      if(dt.per_point) {
+      double2 c = make_double2(dt->c_re, dt->c_im);
+      if(pp) {
         double x[3];
-        phys_point(a.dim, X, &a.pts[r][3 * q], x);
-        c = eval_expr(dt.expr, x[0], x[1], x[2]);
+        phys_point(dim, X, &R.pts[3 * q], x);
+        c = eval_expr(&dt->expr, x[0], x[1], x[2]);
       }
-      else {
-        c = make_double2(dt.c_re, dt.c_im);
-      }
-      const double wd = a.w[r][q] * g.det * s;
+      const double wd = R.w[q] * g.det * s;
       sum.x += wd * c.x;
       sum.y += wd * c.y;
     }
-    acc = cadd(acc, cmul(make_double2(dt.fac_re, dt.fac_im), sum));
+    acc = cadd(acc, cmul(make_double2(dt->fac_re, dt->fac_im), sum));
   }
   Ae[e * NN + pair] = acc;
 }
@@ -520,11 +525,12 @@ struct ValTraits<double2> {
 // nonzeros [ns, ...).  Thread t evaluates contribution cs+t; the first
 // thread of each nonzero sums its segment in element order and writes it.
 template <bool CPLX, bool MIXED>
-__global__ void __launch_bounds__(kChunk) k_gather(GroupTable gt, const uint32_t *__restrict__ words,
+__global__ void __launch_bounds__(kChunk) k_gather(const GroupTable *__restrict__ gtp, const uint32_t *__restrict__ words,
                                                    const uint32_t *__restrict__ chunk_c,
                                                    const uint32_t *__restrict__ chunk_n, void *__restrict__ out)
 {
   using R = typename std::conditional<MIXED, float, double>::type;
+  const GroupTable &gt = *gtp;
   __shared__ R s_re[kChunk];
   __shared__ R s_im[CPLX ? kChunk : 1];
   __shared__ uint8_t s_head[kChunk + 1];
@@ -587,11 +593,12 @@ __global__ void __launch_bounds__(kChunk) k_gather(GroupTable gt, const uint32_t
 // columns (transposed local index, VectorFactory.cpp:184), then the element
 // load vectors, each in element order.  Sums in FP64.
 template <bool CPLX, bool MIXED>
-__global__ void __launch_bounds__(256) k_rhs(GroupTable gt, int64_t n, int nrhs, const int64_t *__restrict__ dc_ptr,
+__global__ void __launch_bounds__(256) k_rhs(const GroupTable *__restrict__ gtp, int64_t n, int nrhs, const int64_t *__restrict__ dc_ptr,
                                              const uint32_t *__restrict__ dc_words, const int32_t *__restrict__ dc_fix,
                                              const double2 *__restrict__ uD, const int64_t *__restrict__ lin_ptr,
                                              const uint32_t *__restrict__ lin_words, void *__restrict__ out)
 {
+  const GroupTable &gt = *gtp;
   const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if(row >= n) return;
   double2 lift = make_double2(0., 0.);
@@ -747,77 +754,48 @@ hipError_t launch_geometry(const Group &g, const double *xyz, hipStream_t s)
 }
 
 template <int N>
-static void linear_n(const LinearArgs &a, const Group &g, const double *xyz, double2 *be, hipStream_t s)
+static void linear_n(const Group &g, int rhs, const double *xyz, double2 *be, hipStream_t s)
 {
-  hipLaunchKernelGGL(k_linear<N>, dim3(blocks_for(g.ne, 128)), dim3(128), 0, s, a, g.ne, g.d_vtx, xyz, g.d_F64,
-                     be);
+  hipLaunchKernelGGL(k_linear<N>, dim3(blocks_for(g.ne, 128)), dim3(128), 0, s, g.d_dev, rhs, g.ne, g.d_vtx, xyz,
+                     g.d_F64, be);
 }
 
 hipError_t launch_linear(const Group &g, const double *xyz, int nrhs, hipStream_t s)
 {
   if(g.ne == 0 || !g.has_linear) return hipSuccess;
-  LinearArgs a{};
-  a.dim = g.dim;
-  a.nv = g.nv;
-  a.N = g.N;
-  a.nterms = (int)g.dterms.size();
-  a.terms = g.d_terms;
-  for(size_t r = 0; r < g.rules.size() && r < 8; ++r) {
-    a.Q[r] = g.rules[r].Q;
-    a.w[r] = g.rules[r].d_w;
-    a.pts[r] = g.rules[r].d_pts;
-    a.bf[r] = g.rules[r].d_bf;
-  }
   for(int r = 0; r < nrhs; ++r) {
-    a.rhs = r;
     double2 *be = g.d_be + (int64_t)r * g.ne * g.N;
     switch(g.N) {
-    case 1: linear_n<1>(a, g, xyz, be, s); break;
-    case 2: linear_n<2>(a, g, xyz, be, s); break;
-    case 3: linear_n<3>(a, g, xyz, be, s); break;
-    case 4: linear_n<4>(a, g, xyz, be, s); break;
-    case 5: linear_n<5>(a, g, xyz, be, s); break;
-    case 6: linear_n<6>(a, g, xyz, be, s); break;
-    case 10: linear_n<10>(a, g, xyz, be, s); break;
-    case 15: linear_n<15>(a, g, xyz, be, s); break;
-    case 20: linear_n<20>(a, g, xyz, be, s); break;
-    case 35: linear_n<35>(a, g, xyz, be, s); break;
+    case 1: linear_n<1>(g, r, xyz, be, s); break;
+    case 2: linear_n<2>(g, r, xyz, be, s); break;
+    case 3: linear_n<3>(g, r, xyz, be, s); break;
+    case 4: linear_n<4>(g, r, xyz, be, s); break;
+    case 5: linear_n<5>(g, r, xyz, be, s); break;
+    case 6: linear_n<6>(g, r, xyz, be, s); break;
+    case 10: linear_n<10>(g, r, xyz, be, s); break;
+    case 15: linear_n<15>(g, r, xyz, be, s); break;
+    case 20: linear_n<20>(g, r, xyz, be, s); break;
+    case 35: linear_n<35>(g, r, xyz, be, s); break;
     default: return hipErrorInvalidValue;
     }
   }
   return hipGetLastError();
 }
 
-hipError_t launch_dense(const Group &g, const double *xyz, bool all_terms, bool freq_module, double freq, bool cplx,
-                        hipStream_t s)
+hipError_t launch_dense(const Group &g, const double *xyz, bool all_terms, hipStream_t s)
 {
   if(g.ne == 0) return hipSuccess;
-  DenseArgs a{};
-  a.dim = g.dim;
-  a.nv = g.nv;
-  a.N = g.N;
-  a.nterms = (int)g.dterms.size();
-  a.all_terms = all_terms ? 1 : 0;
-  a.freq_module = freq_module ? 1 : 0;
-  a.cplx = cplx ? 1 : 0;
-  a.freq = freq;
-  a.terms = g.d_terms;
-  for(size_t r = 0; r < g.rules.size() && r < 8; ++r) {
-    a.Q[r] = g.rules[r].Q;
-    a.w[r] = g.rules[r].d_w;
-    a.pts[r] = g.rules[r].d_pts;
-    a.bf[r] = g.rules[r].d_bf;
-    a.gbf[r] = g.rules[r].d_gbf;
-  }
   const int64_t total = g.ne * (int64_t)g.N * g.N;
-  hipLaunchKernelGGL(k_dense, dim3(blocks_for(total, 256)), dim3(256), 0, s, a, g.ne, g.d_vtx, xyz, g.d_dense);
+  hipLaunchKernelGGL(k_dense, dim3(blocks_for(total, 256)), dim3(256), 0, s, g.d_dev, all_terms ? 1 : 0, g.ne,
+                     g.d_vtx, xyz, g.d_dense);
   return hipGetLastError();
 }
 
-hipError_t launch_gather(const gfem_system &sys, const GroupTable &gt, int prec, hipStream_t s)
+hipError_t launch_gather(const gfem_system &sys, int prec, hipStream_t s)
 {
   if(sys.n_chunks == 0) return hipSuccess;
   void *out = sys.d_vals[prec];
+  const GroupTable *gt = sys.d_table;
   const dim3 grid((unsigned)sys.n_chunks), block(kChunk);
   if(sys.cplx) {
     if(prec)
@@ -834,10 +812,11 @@ hipError_t launch_gather(const gfem_system &sys, const GroupTable &gt, int prec,
   return hipGetLastError();
 }
 
-hipError_t launch_rhs(const gfem_system &sys, const GroupTable &gt, int prec, hipStream_t s)
+hipError_t launch_rhs(const gfem_system &sys, int prec, hipStream_t s)
 {
   if(sys.n == 0) return hipSuccess;
   void *out = sys.d_rhs[prec];
+  const GroupTable *gt = sys.d_table;
   const dim3 grid(blocks_for(sys.n, 256)), block(256);
 #define GFEM_RHS_ARGS gt, sys.n, sys.nrhs, sys.d_dc_ptr, sys.d_dc_words, sys.d_dc_fix, sys.d_dc, sys.d_lin_ptr, sys.d_lin_words, out
   if(sys.cplx) {

@@ -135,7 +135,7 @@ enum {
 
 #define GFEM_MAX_OPS 64
 #define GFEM_MAX_CONSTS 32
-#define GFEM_MAX_STACK 16
+#define GFEM_MAX_STACK 8 /* evaluation stack depth (held in registers on the device) */
 
 /* One integral() term (problem/Formulation.h:103-180).  Bilinear terms read
  * the dof side (form_dof, coef) and the test-function side (form_tf,
