@@ -258,6 +258,8 @@ gfem_status gfem_system_destroy(gfem_system *sys)
   dfree(sys->d_words);
   dfree(sys->d_chunk_c);
   dfree(sys->d_chunk_n);
+  dfree(sys->d_chunk_e);
+  dfree(sys->d_elist);
   dfree(sys->d_lin_words);
   dfree(sys->d_dc_words);
   dfree(sys->d_dc_fix);
@@ -396,8 +398,6 @@ gfem_status gfem_system_finalize(gfem_system *sys)
   while((1 << logP) < maxNN) ++logP;
   sys->logP = logP;
   const int64_t n_elem = sys->groups.empty() ? 0 : sys->groups.back().e_off + sys->groups.back().ne;
-  if((n_elem << logP) >= (int64_t(1) << 31))
-    return fail(ctx, GFEM_ERR_TOO_LARGE, "elements x local pairs exceed the 31-bit contribution word");
   const double w = sys->frequency;
   for(auto &g : sys->groups) {
     if(g.terms.size() > 16) return fail(ctx, GFEM_ERR_UNSUPPORTED, "more than 16 terms on one element group");
@@ -444,6 +444,13 @@ gfem_status gfem_system_finalize(gfem_system *sys)
       d.expr.n_consts = t.n_consts;
       for(int q = 0; q < 2 * t.n_ops; ++q) d.expr.ops[q] = g.term_ops[k][q];
       for(int q = 0; q < 2 * t.n_consts; ++q) d.expr.consts[q] = g.term_consts[k][q];
+      // real-only programs (no imaginary constant, no sqrt / pow that may
+      // leave the real line) are evaluated in real arithmetic on the device
+      d.expr.real_only = 1;
+      for(int q = 0; q < t.n_consts; ++q)
+        if(g.term_consts[k][2 * q + 1] != 0.) d.expr.real_only = 0;
+      for(int q = 0; q < t.n_ops; ++q)
+        if(g.term_ops[k][2 * q] == GFEM_OP_SQRT || g.term_ops[k][2 * q] == GFEM_OP_POW) d.expr.real_only = 0;
       if(t.kind == 0 && d.per_point) g.has_dense = true;
       g.dterms.push_back(d);
     }
@@ -483,9 +490,12 @@ gfem_status gfem_system_finalize(gfem_system *sys)
     GFEM_HIP(ctx, dupload(&sys->d_table, &gt, 1));
   }
 
-  // nonzero-sorted element map (contribution words), traversal order kept
-  // inside each nonzero: group, element, local pair
+  // Element -> nonzero map, transposed: the contributions of every nonzero in
+  // traversal order (group, element, local pair), cut into gather chunks of
+  // whole nonzeros with <= kChunk contributions and <= kChunkElems distinct
+  // elements; inside a chunk the words are grouped by element.
   {
+    if(logP > 13) return fail(ctx, GFEM_ERR_UNSUPPORTED, "more than 90 dofs by element in the gather words");
     std::vector<int64_t> cnt(sys->nnz + 1, 0);
     int64_t total = 0;
     for(auto &g : sys->groups) {
@@ -502,39 +512,81 @@ gfem_status gfem_system_finalize(gfem_system *sys)
     }
     for(int64_t i = 0; i < sys->nnz; ++i) {
       if(cnt[i + 1] == 0) return fail(ctx, GFEM_ERR_INVALID_ARG, "a stored nonzero has no element contribution");
-      if(cnt[i + 1] > kChunk) return fail(ctx, GFEM_ERR_UNSUPPORTED, "a nonzero has more than 256 contributions");
+      if(cnt[i + 1] > kChunkElems)
+        return fail(ctx, GFEM_ERR_UNSUPPORTED, "a nonzero has more than 256 element contributions");
       cnt[i + 1] += cnt[i];
     }
     if(total >= (int64_t(1) << 32)) return fail(ctx, GFEM_ERR_TOO_LARGE, "too many contributions");
-    std::vector<uint32_t> words((size_t)total);
-    std::vector<int64_t> pos(cnt.begin(), cnt.end() - 1);
-    for(auto &g : sys->groups) {
-      if(!g.has_bilinear) continue;
-      const int NN = g.N * g.N;
-      for(int64_t e = 0; e < g.ne; ++e)
-        for(int p = 0; p < NN; ++p) {
-          const int64_t v = g.h_e2n[e * NN + p];
-          if(v < 0) continue;
-          const uint32_t wd = (uint32_t)(((g.e_off + e) << logP) | p);
-          const int64_t at = pos[v]++;
-          words[at] = wd | (at == cnt[v] ? 0x80000000u : 0u);
-        }
+    if(n_elem >= (int64_t(1) << 32)) return fail(ctx, GFEM_ERR_TOO_LARGE, "too many elements");
+    std::vector<uint32_t> c_elem((size_t)total);
+    std::vector<uint16_t> c_pair((size_t)total);
+    {
+      std::vector<int64_t> pos(cnt.begin(), cnt.end() - 1);
+      for(auto &g : sys->groups) {
+        if(!g.has_bilinear) continue;
+        const int NN = g.N * g.N;
+        for(int64_t e = 0; e < g.ne; ++e)
+          for(int p = 0; p < NN; ++p) {
+            const int64_t v = g.h_e2n[e * NN + p];
+            if(v < 0) continue;
+            const int64_t at = pos[v]++;
+            c_elem[at] = (uint32_t)(g.e_off + e);
+            c_pair[at] = (uint16_t)p;
+          }
+      }
     }
-    // chunks of whole nonzeros, at most kChunk contributions each
-    std::vector<uint32_t> cc, cn;
+    std::vector<uint32_t> words((size_t)total), cc, cn, ce, elist;
+    std::vector<int32_t> mark((size_t)std::max<int64_t>(n_elem, 1), -1), local((size_t)std::max<int64_t>(n_elem, 1), 0);
+    std::vector<std::pair<uint32_t, uint32_t>> tmp;
     int64_t nz = 0;
+    int32_t chunk = 0;
     while(nz < sys->nnz) {
-      cc.push_back((uint32_t)cnt[nz]);
+      const int64_t c0 = cnt[nz];
+      cc.push_back((uint32_t)c0);
       cn.push_back((uint32_t)nz);
-      const int64_t start = cnt[nz];
-      while(nz < sys->nnz && cnt[nz + 1] - start <= kChunk) ++nz;
+      ce.push_back((uint32_t)elist.size());
+      int nel = 0;
+      while(nz < sys->nnz && cnt[nz + 1] - c0 <= kChunk) {
+        int fresh = 0;
+        for(int64_t k = cnt[nz]; k < cnt[nz + 1]; ++k)
+          if(mark[c_elem[k]] != chunk) ++fresh;
+        if(nel + fresh > kChunkElems) break;
+        for(int64_t k = cnt[nz]; k < cnt[nz + 1]; ++k)
+          if(mark[c_elem[k]] != chunk) {
+            mark[c_elem[k]] = chunk;
+            local[c_elem[k]] = nel++;
+            elist.push_back(c_elem[k]);
+          }
+        ++nz;
+      }
+      const int64_t c1 = cnt[nz];
+      tmp.clear();
+      for(int64_t k = c0; k < c1; ++k) {
+        const uint32_t slot = (uint32_t)(k - c0);
+        const uint32_t key = ((uint32_t)local[c_elem[k]] << logP) | c_pair[k];
+        tmp.emplace_back(key, slot);
+      }
+      // head flags per slot
+      std::vector<uint8_t> heads((size_t)(c1 - c0), 0);
+      for(int64_t z = cn.back(); z < nz; ++z) heads[cnt[z] - c0] = 1;
+      std::stable_sort(tmp.begin(), tmp.end(),
+                       [](const std::pair<uint32_t, uint32_t> &a, const std::pair<uint32_t, uint32_t> &b) {
+                         return a.first < b.first;
+                       });
+      for(size_t k = 0; k < tmp.size(); ++k)
+        words[c0 + k] = tmp[k].first | (tmp[k].second << (logP + 8)) | (heads[tmp[k].second] ? 0x80000000u : 0u);
+      ++chunk;
     }
     cc.push_back((uint32_t)total);
+    ce.push_back((uint32_t)elist.size());
     sys->n_chunks = (int64_t)cn.size();
     sys->n_words = total;
+    sys->n_elist = (int64_t)elist.size();
     GFEM_HIP(ctx, dupload(&sys->d_words, words.data(), words.size()));
     GFEM_HIP(ctx, dupload(&sys->d_chunk_c, cc.data(), cc.size()));
     GFEM_HIP(ctx, dupload(&sys->d_chunk_n, cn.data(), cn.size()));
+    GFEM_HIP(ctx, dupload(&sys->d_chunk_e, ce.data(), ce.size()));
+    GFEM_HIP(ctx, dupload(&sys->d_elist, elist.data(), elist.size()));
   }
 
   // RHS maps: linear contributions (row <- element key) and Dirichlet lifting
@@ -558,6 +610,8 @@ gfem_status gfem_system_finalize(gfem_system *sys)
               if(g.h_type[e * N + j] != GFEM_AT_FIXED) continue;
               const int64_t fx = g.h_num[e * N + j];
               if(fx < 0 || fx >= sys->n_fixed) return fail(ctx, GFEM_ERR_INVALID_ARG, "fixed dof index out of range");
+              if((eg << logP) >= (int64_t(1) << 32))
+                return fail(ctx, GFEM_ERR_TOO_LARGE, "too many elements for the Dirichlet lifting map");
               dc_key.push_back(row);
               dc_pay.push_back((uint32_t)((eg << logP) | (j * N + i)));
               dc_fixv.push_back((int32_t)fx);
@@ -771,7 +825,8 @@ gfem_status gfem_gather_bytes(const gfem_system *sys, gfem_precision prec, doubl
 {
   if(!sys || !bytes) return GFEM_ERR_INVALID_ARG;
   const int p = prec == GFEM_PREC_MIXED ? 1 : 0;
-  double b = 4.0 * (double)sys->n_words + 8.0 * (double)(sys->n_chunks + 1);
+  // contribution words + chunk headers (3 x uint32) + chunk element lists
+  double b = 4.0 * (double)sys->n_words + 12.0 * (double)(sys->n_chunks + 1) + 4.0 * (double)sys->n_elist;
   int64_t ne = 0;
   for(auto &g : sys->groups)
     if(g.has_bilinear) ne += g.ne;

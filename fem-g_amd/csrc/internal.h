@@ -7,9 +7,14 @@
 //   reference tensor        T[N*N][8] complex, pair = i_tf*N + j_dof, the
 //                           coefficient-weighted sum over the group's
 //                           constant-coefficient bilinear terms
-//   contribution words      uint32, sorted by CSR position: bit31 = first
-//                           contribution of its nonzero, bits 0..30 =
-//                           (global element slot << logP) | pair
+//   gather chunks           runs of whole nonzeros, <= kChunk contributions
+//                           and <= kChunkElems distinct elements each, with
+//                           the chunk's element list (global element slots)
+//   contribution words      uint32, grouped by chunk element then pair:
+//                           bits [0,logP) local pair i_tf*N + j_dof,
+//                           [logP,logP+8) chunk element, [logP+8,logP+18)
+//                           slot = position in the chunk's nonzero order,
+//                           bit 31 = slot is the first term of its nonzero
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -24,12 +29,14 @@ namespace gfem {
 
 constexpr int kMaxGroups = 16;
 constexpr int kMaxN = 64;
-constexpr int kChunk = 256;  // contributions per gather block
+constexpr int kChunk = 1024;      // contributions per gather block (whole nonzeros)
+constexpr int kChunkElems = 256;  // distinct elements per gather block
 
 // Coefficient program as read by the kernels.
 struct DevExpr {
   int n_ops;
   int n_consts;
+  int real_only;  // real constants and operations that keep real values real
   int ops[2 * GFEM_MAX_OPS];
   double consts[2 * GFEM_MAX_CONSTS];
 };
@@ -139,7 +146,8 @@ struct gfem_system {
   // fused maps
   uint32_t *d_words = nullptr;
   int64_t n_words = 0;
-  uint32_t *d_chunk_c = nullptr, *d_chunk_n = nullptr;
+  uint32_t *d_chunk_c = nullptr, *d_chunk_n = nullptr, *d_chunk_e = nullptr, *d_elist = nullptr;
+  int64_t n_elist = 0;
   int64_t n_chunks = 0;
   uint32_t *d_lin_words = nullptr, *d_dc_words = nullptr;
   int32_t *d_dc_fix = nullptr;

@@ -99,8 +99,54 @@ __device__ __forceinline__ double2 cbinary(int op, double2 a, double2 b)
 // eight registers (push shifts down, binary ops shift up).  The program is
 // the same for every lane, so the switch is uniform and nothing spills to
 // scratch memory.
-__device__ double2 eval_expr(const DevExpr *__restrict__ ex, double x, double y, double z)
+__device__ __forceinline__ double runary(int op, double a)
 {
+  switch(op) {
+  case GFEM_OP_NEG: return -a;
+  case GFEM_OP_EXP: return exp(a);
+  case GFEM_OP_SIN: return sin(a);
+  case GFEM_OP_COS: return cos(a);
+  case GFEM_OP_ABS: return fabs(a);
+  default: return a;
+  }
+}
+
+__device__ __forceinline__ double rbinary(int op, double a, double b)
+{
+  switch(op) {
+  case GFEM_OP_ADD: return a + b;
+  case GFEM_OP_SUB: return a - b;
+  case GFEM_OP_MUL: return a * b;
+  case GFEM_OP_DIV: return a / b;
+  default: return a;
+  }
+}
+
+// real-arithmetic evaluation of a real-only program (same stack discipline)
+__device__ __forceinline__ double eval_expr_real(const DevExpr *__restrict__ ex, double x, double y, double z)
+{
+  double s0 = 0., s1 = 0., s2 = 0., s3 = 0., s4 = 0., s5 = 0., s6 = 0., s7 = 0.;
+  const int n = ex->n_ops;
+  for(int k = 0; k < n; ++k) {
+    const int op = ex->ops[2 * k], arg = ex->ops[2 * k + 1];
+    if(op <= GFEM_OP_Z) {
+      const double v = op == GFEM_OP_CONST ? ex->consts[2 * arg] : (op == GFEM_OP_X ? x : (op == GFEM_OP_Y ? y : z));
+      s7 = s6; s6 = s5; s5 = s4; s4 = s3; s3 = s2; s2 = s1; s1 = s0; s0 = v;
+    }
+    else if(op >= GFEM_OP_ADD && op <= GFEM_OP_DIV) {
+      s0 = rbinary(op, s1, s0);
+      s1 = s2; s2 = s3; s3 = s4; s4 = s5; s5 = s6; s6 = s7;
+    }
+    else {
+      s0 = runary(op, s0);
+    }
+  }
+  return s0;
+}
+
+__device__ __forceinline__ double2 eval_expr(const DevExpr *__restrict__ ex, double x, double y, double z)
+{
+  if(ex->real_only) return make_double2(eval_expr_real(ex, x, y, z), 0.);
   double2 s0 = make_double2(0., 0.), s1 = s0, s2 = s0, s3 = s0, s4 = s0, s5 = s0, s6 = s0, s7 = s0;
   const int n = ex->n_ops;
   for(int k = 0; k < n; ++k) {
@@ -521,70 +567,134 @@ struct ValTraits<double2> {
 };
 
 // ------------------------------------------------------------------ k_gather
-// One 256-thread block per chunk: contributions [cs, ce) covering whole
-// nonzeros [ns, ...).  Thread t evaluates contribution cs+t; the first
-// thread of each nonzero sums its segment in element order and writes it.
+// Atomic-free CSR assembly.  One 256-thread block per chunk of whole
+// nonzeros (<= kChunk contributions, <= kChunkElems distinct elements):
+//   1. the chunk's element factors F (FP64 geometry) are staged in LDS;
+//   2. every contribution word (grouped by element, then local pair) is
+//      evaluated as F . T[pair] and parked in LDS at its nonzero-order slot;
+//   3. the first slot of each nonzero sums its run in slot order, i.e. in the
+//      element traversal order of the reference's sequential loop, and writes
+//      the value once.  No atomics, bitwise reproducible.
 template <bool CPLX, bool MIXED>
-__global__ void __launch_bounds__(kChunk) k_gather(const GroupTable *__restrict__ gtp, const uint32_t *__restrict__ words,
-                                                   const uint32_t *__restrict__ chunk_c,
-                                                   const uint32_t *__restrict__ chunk_n, void *__restrict__ out)
+__global__ void __launch_bounds__(256) k_gather(const GroupTable *__restrict__ gtp, const uint32_t *__restrict__ words,
+                                                const uint32_t *__restrict__ chunk_c,
+                                                const uint32_t *__restrict__ chunk_n,
+                                                const uint32_t *__restrict__ chunk_e,
+                                                const uint32_t *__restrict__ elist, void *__restrict__ out)
 {
   using R = typename std::conditional<MIXED, float, double>::type;
   const GroupTable &gt = *gtp;
+  __shared__ R s_F[kChunkElems][8];
+  __shared__ uint8_t s_g[kChunkElems];
+  __shared__ int32_t s_el[kChunkElems];
   __shared__ R s_re[kChunk];
   __shared__ R s_im[CPLX ? kChunk : 1];
-  __shared__ uint8_t s_head[kChunk + 1];
-  __shared__ int s_wave[kChunk / 64];
+  __shared__ uint8_t s_head[kChunk];
+  __shared__ int s_wsum[4];
   const int64_t c = blockIdx.x;
-  const uint32_t cs = chunk_c[c], ce = chunk_c[c + 1], ns = chunk_n[c];
+  const uint32_t cs = chunk_c[c], len = chunk_c[c + 1] - cs, ns = chunk_n[c];
+  const uint32_t es = chunk_e[c], nel = chunk_e[c + 1] - es;
   const int t = threadIdx.x;
-  const int len = (int)(ce - cs);
-  const bool valid = t < len;
-  bool head = false;
-  R vre = 0, vim = 0;
-  if(valid) {
-    const uint32_t w = words[cs + t];
-    head = (w >> 31) != 0;
-    const uint32_t flat = w & 0x7fffffffu;
-    const int64_t eg = flat >> gt.logP;
-    const int pair = (int)(flat & ((1u << gt.logP) - 1u));
+  const int logP = gt.logP;
+  const uint32_t pmask = (1u << logP) - 1u;
+
+  for(uint32_t k = t; k < nel; k += 256) {
+    const int64_t eg = elist[es + k];
+    const int g = find_group(gt, eg);
+    const int64_t el = eg - gt.e_off[g];
+    s_g[k] = (uint8_t)g;
+    s_el[k] = (int32_t)el;
     if(MIXED) {
-      const float2 v = elem_value32<CPLX>(gt, eg, pair);
-      vre = v.x;
-      vim = v.y;
+      const float4 *F = reinterpret_cast<const float4 *>(gt.F32[g] + el * 8);
+      const float4 a = F[0], b = F[1];
+      s_F[k][0] = a.x; s_F[k][1] = a.y; s_F[k][2] = a.z; s_F[k][3] = a.w;
+      s_F[k][4] = b.x; s_F[k][5] = b.y; s_F[k][6] = b.z; s_F[k][7] = b.w;
     }
     else {
-      const double2 v = elem_value64<CPLX>(gt, eg, pair);
-      vre = v.x;
-      vim = v.y;
+      const double2 *F = reinterpret_cast<const double2 *>(gt.F64[g] + el * 8);
+#pragma unroll
+      for(int q = 0; q < 4; ++q) {
+        const double2 f = F[q];
+        s_F[k][2 * q] = (R)f.x;
+        s_F[k][2 * q + 1] = (R)f.y;
+      }
     }
   }
-  s_re[t] = vre;
-  if(CPLX) s_im[t] = vim;
-  s_head[t] = head ? 1 : 0;
-  if(t == 0) s_head[kChunk] = 1;
-  const unsigned long long m = __ballot(head);
-  const int lane = t & 63, wv = t >> 6;
-  const int before = __popcll(m & ((1ull << lane) - 1ull));
-  if(lane == 0) s_wave[wv] = __popcll(m);
   __syncthreads();
-  if(!head) return;
-  int off = before;
-  for(int k = 0; k < wv; ++k) off += s_wave[k];
-  R sre = vre, sim = vim;
-  for(int k = t + 1; k < len && !s_head[k]; ++k) {
-    sre += s_re[k];
-    if(CPLX) sim += s_im[k];
+
+  for(uint32_t k = t; k < len; k += 256) {
+    const uint32_t w = words[cs + k];
+    const int pair = (int)(w & pmask);
+    const int le = (int)((w >> logP) & 0xFFu);
+    const int slot = (int)((w >> (logP + 8)) & 0x3FFu);
+    const int g = s_g[le];
+    R vr = 0, vi = 0;
+    if(MIXED) {
+      const float4 *T = reinterpret_cast<const float4 *>(gt.T32[g] + (int64_t)pair * 8);
+#pragma unroll
+      for(int q = 0; q < 4; ++q) {
+        const float4 tq = T[q];  // components 2q (x,y) and 2q+1 (z,w)
+        vr += s_F[le][2 * q] * tq.x + s_F[le][2 * q + 1] * tq.z;
+        if(CPLX) vi += s_F[le][2 * q] * tq.y + s_F[le][2 * q + 1] * tq.w;
+      }
+    }
+    else {
+      const double2 *T = gt.T64[g] + (int64_t)pair * 8;
+#pragma unroll
+      for(int q = 0; q < 7; ++q) {
+        const double2 tq = T[q];
+        vr += s_F[le][q] * tq.x;
+        if(CPLX) vi += s_F[le][q] * tq.y;
+      }
+    }
+    if(gt.dense[g]) {
+      const int N = gt.N[g];
+      const double2 d = gt.dense[g][(int64_t)s_el[le] * N * N + pair];
+      vr += (R)d.x;
+      vi += (R)d.y;
+    }
+    s_re[slot] = vr;
+    if(CPLX) s_im[slot] = vi;
+    s_head[slot] = (uint8_t)(w >> 31);
   }
-  const int64_t pos = (int64_t)ns + off;
-  if(CPLX) {
-    if(MIXED)
-      reinterpret_cast<float2 *>(out)[pos] = make_float2((float)sre, (float)sim);
-    else
-      reinterpret_cast<double2 *>(out)[pos] = make_double2((double)sre, (double)sim);
+  __syncthreads();
+
+  // thread t owns slots [4t, 4t+4): rank of its first head = exclusive scan
+  // of head counts
+  const int s0 = 4 * t;
+  int cnt = 0;
+#pragma unroll
+  for(int q = 0; q < 4; ++q)
+    if(s0 + q < (int)len) cnt += s_head[s0 + q];
+  const int lane = t & 63, wv = t >> 6;
+  int incl = cnt;
+#pragma unroll
+  for(int d = 1; d < 64; d <<= 1) {
+    const int v = __shfl_up(incl, d, 64);
+    if(lane >= d) incl += v;
   }
-  else {
-    reinterpret_cast<R *>(out)[pos] = sre;
+  if(lane == 63) s_wsum[wv] = incl;
+  __syncthreads();
+  int rank = incl - cnt;
+  for(int k = 0; k < wv; ++k) rank += s_wsum[k];
+  for(int q = 0; q < 4; ++q) {
+    const int sl = s0 + q;
+    if(sl >= (int)len || !s_head[sl]) continue;
+    R sre = s_re[sl], sim = CPLX ? s_im[sl] : (R)0;
+    for(int k = sl + 1; k < (int)len && !s_head[k]; ++k) {
+      sre += s_re[k];
+      if(CPLX) sim += s_im[k];
+    }
+    const int64_t pos = (int64_t)ns + rank++;
+    if(CPLX) {
+      if(MIXED)
+        reinterpret_cast<float2 *>(out)[pos] = make_float2((float)sre, (float)sim);
+      else
+        reinterpret_cast<double2 *>(out)[pos] = make_double2((double)sre, (double)sim);
+    }
+    else {
+      reinterpret_cast<R *>(out)[pos] = sre;
+    }
   }
 }
 
@@ -796,19 +906,21 @@ hipError_t launch_gather(const gfem_system &sys, int prec, hipStream_t s)
   if(sys.n_chunks == 0) return hipSuccess;
   void *out = sys.d_vals[prec];
   const GroupTable *gt = sys.d_table;
-  const dim3 grid((unsigned)sys.n_chunks), block(kChunk);
+  const dim3 grid((unsigned)sys.n_chunks), block(256);
+#define GFEM_GATHER_ARGS gt, sys.d_words, sys.d_chunk_c, sys.d_chunk_n, sys.d_chunk_e, sys.d_elist, out
   if(sys.cplx) {
     if(prec)
-      hipLaunchKernelGGL((k_gather<true, true>), grid, block, 0, s, gt, sys.d_words, sys.d_chunk_c, sys.d_chunk_n, out);
+      hipLaunchKernelGGL((k_gather<true, true>), grid, block, 0, s, GFEM_GATHER_ARGS);
     else
-      hipLaunchKernelGGL((k_gather<true, false>), grid, block, 0, s, gt, sys.d_words, sys.d_chunk_c, sys.d_chunk_n, out);
+      hipLaunchKernelGGL((k_gather<true, false>), grid, block, 0, s, GFEM_GATHER_ARGS);
   }
   else {
     if(prec)
-      hipLaunchKernelGGL((k_gather<false, true>), grid, block, 0, s, gt, sys.d_words, sys.d_chunk_c, sys.d_chunk_n, out);
+      hipLaunchKernelGGL((k_gather<false, true>), grid, block, 0, s, GFEM_GATHER_ARGS);
     else
-      hipLaunchKernelGGL((k_gather<false, false>), grid, block, 0, s, gt, sys.d_words, sys.d_chunk_c, sys.d_chunk_n, out);
+      hipLaunchKernelGGL((k_gather<false, false>), grid, block, 0, s, GFEM_GATHER_ARGS);
   }
+#undef GFEM_GATHER_ARGS
   return hipGetLastError();
 }
 
