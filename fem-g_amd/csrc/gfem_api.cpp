@@ -72,6 +72,8 @@ void free_group(Group &g)
   dfree(g.d_F32);
   dfree(g.d_T64);
   dfree(g.d_T32);
+  dfree(g.d_T64r);
+  dfree(g.d_T32r);
   dfree(g.d_dense);
   dfree(g.d_be);
   dfree(g.d_e2n);
@@ -145,6 +147,9 @@ GroupTable make_table(const gfem_system *sys)
     gt.F32[g] = G.d_F32;
     gt.T64[g] = G.d_T64;
     gt.T32[g] = G.d_T32;
+    gt.T64r[g] = G.d_T64r;
+    gt.T32r[g] = G.d_T32r;
+    gt.treal[g] = G.treal ? 1 : 0;
     gt.dense[g] = G.has_dense ? G.d_dense : nullptr;
     gt.be[g] = G.has_linear ? G.d_be : nullptr;
     gt.ne[g] = G.ne;
@@ -465,6 +470,12 @@ gfem_status gfem_system_finalize(gfem_system *sys)
     GFEM_HIP(ctx, dalloc(&g.d_F32, (size_t)(g.ne * 8)));
     GFEM_HIP(ctx, dalloc(&g.d_T64, (size_t)(g.N * g.N * 8)));
     GFEM_HIP(ctx, dalloc(&g.d_T32, (size_t)(g.N * g.N * 8)));
+    GFEM_HIP(ctx, dalloc(&g.d_T64r, (size_t)(g.N * g.N * 8)));
+    GFEM_HIP(ctx, dalloc(&g.d_T32r, (size_t)(g.N * g.N * 8)));
+    // the tensor is real when every constant-coefficient bilinear factor is
+    g.treal = true;
+    for(const DevTerm &d : g.dterms)
+      if(d.kind == 0 && !d.per_point && (d.c_re * d.fac_im + d.c_im * d.fac_re) != 0.) g.treal = false;
     if(g.has_dense) GFEM_HIP(ctx, dalloc(&g.d_dense, (size_t)(g.ne * g.N * g.N)));
     if(g.has_linear) GFEM_HIP(ctx, dalloc(&g.d_be, (size_t)(sys->nrhs * g.ne * g.N)));
     for(auto &R : g.rules) {

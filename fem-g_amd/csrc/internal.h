@@ -64,6 +64,9 @@ struct GroupTable {
   const float *F32[kMaxGroups];    // [ne][8]
   const double2 *T64[kMaxGroups];  // [N*N][8]
   const float2 *T32[kMaxGroups];   // [N*N][8]
+  const double *T64r[kMaxGroups];  // [N*N][8] real parts
+  const float *T32r[kMaxGroups];   // [N*N][8] real parts
+  int treal[kMaxGroups];           // the group's tensor has no imaginary part
   const double2 *dense[kMaxGroups];  // [ne][N*N] per-point bilinear terms, or null
   const double2 *be[kMaxGroups];     // [nrhs][ne][N] linear terms, or null
   int64_t ne[kMaxGroups];
@@ -110,6 +113,9 @@ struct Group {
   float *d_F32 = nullptr;
   double2 *d_T64 = nullptr;
   float2 *d_T32 = nullptr;
+  double *d_T64r = nullptr;
+  float *d_T32r = nullptr;
+  bool treal = true;
   double2 *d_dense = nullptr;
   double2 *d_be = nullptr;
   // plain reference strategy

@@ -126,9 +126,11 @@ __device__ __forceinline__ double rbinary(int op, double a, double b)
 __device__ __forceinline__ double eval_expr_real(const DevExpr *__restrict__ ex, double x, double y, double z)
 {
   double s0 = 0., s1 = 0., s2 = 0., s3 = 0., s4 = 0., s5 = 0., s6 = 0., s7 = 0.;
-  const int n = ex->n_ops;
+  const int n = __builtin_amdgcn_readfirstlane(ex->n_ops);
   for(int k = 0; k < n; ++k) {
-    const int op = ex->ops[2 * k], arg = ex->ops[2 * k + 1];
+    // the program is the same for every lane: keep op / arg in scalar registers
+    const int op = __builtin_amdgcn_readfirstlane(ex->ops[2 * k]);
+    const int arg = __builtin_amdgcn_readfirstlane(ex->ops[2 * k + 1]);
     if(op <= GFEM_OP_Z) {
       const double v = op == GFEM_OP_CONST ? ex->consts[2 * arg] : (op == GFEM_OP_X ? x : (op == GFEM_OP_Y ? y : z));
       s7 = s6; s6 = s5; s5 = s4; s4 = s3; s3 = s2; s2 = s1; s1 = s0; s0 = v;
@@ -146,11 +148,12 @@ __device__ __forceinline__ double eval_expr_real(const DevExpr *__restrict__ ex,
 
 __device__ __forceinline__ double2 eval_expr(const DevExpr *__restrict__ ex, double x, double y, double z)
 {
-  if(ex->real_only) return make_double2(eval_expr_real(ex, x, y, z), 0.);
+  if(__builtin_amdgcn_readfirstlane(ex->real_only)) return make_double2(eval_expr_real(ex, x, y, z), 0.);
   double2 s0 = make_double2(0., 0.), s1 = s0, s2 = s0, s3 = s0, s4 = s0, s5 = s0, s6 = s0, s7 = s0;
-  const int n = ex->n_ops;
+  const int n = __builtin_amdgcn_readfirstlane(ex->n_ops);
   for(int k = 0; k < n; ++k) {
-    const int op = ex->ops[2 * k], arg = ex->ops[2 * k + 1];
+    const int op = __builtin_amdgcn_readfirstlane(ex->ops[2 * k]);
+    const int arg = __builtin_amdgcn_readfirstlane(ex->ops[2 * k + 1]);
     if(op <= GFEM_OP_Z) {
       double2 v;
       if(op == GFEM_OP_CONST)
@@ -333,7 +336,8 @@ struct TensorArgs {
 //   c=0..2  sum_q w dphi_j/du_c dphi_i/du_c      c=3..5 symmetric cross terms
 //   c=6     sum_q w phi_j phi_i
 // weighted by each term's constant factor and summed over terms.
-__global__ void k_tensor(TensorArgs a, double2 *__restrict__ T64, float2 *__restrict__ T32)
+__global__ void k_tensor(TensorArgs a, double2 *__restrict__ T64, float2 *__restrict__ T32, double *__restrict__ T64r,
+                         float *__restrict__ T32r)
 {
   const int pair = blockIdx.x * blockDim.x + threadIdx.x;
   if(pair >= a.N * a.N) return;
@@ -365,6 +369,8 @@ __global__ void k_tensor(TensorArgs a, double2 *__restrict__ T64, float2 *__rest
   for(int c = 0; c < 8; ++c) {
     T64[pair * 8 + c] = acc[c];
     T32[pair * 8 + c] = make_float2((float)acc[c].x, (float)acc[c].y);
+    T64r[pair * 8 + c] = acc[c].x;
+    T32r[pair * 8 + c] = (float)acc[c].x;
   }
 }
 
@@ -408,13 +414,13 @@ __global__ void __launch_bounds__(128) k_linear(const GroupDev *__restrict__ gd,
   const double det = F64[e * 8 + 6];
   double2 acc[N];
   for(int i = 0; i < N; ++i) acc[i] = make_double2(0., 0.);
-  const int nterms = gd->nterms;
+  const int nterms = __builtin_amdgcn_readfirstlane(gd->nterms);
   for(int t = 0; t < nterms; ++t) {
     const DevTerm *dt = &gd->terms[t];
-    if(dt->kind != 1 || dt->rhs_idx != rhs) continue;
-    const RuleDev &R = gd->rules[dt->rule];
-    const int Q = R.Q;
-    const bool pp = dt->per_point != 0;
+    if(__builtin_amdgcn_readfirstlane(dt->kind) != 1 || __builtin_amdgcn_readfirstlane(dt->rhs_idx) != rhs) continue;
+    const RuleDev &R = gd->rules[__builtin_amdgcn_readfirstlane(dt->rule)];
+    const int Q = __builtin_amdgcn_readfirstlane(R.Q);
+    const bool pp = __builtin_amdgcn_readfirstlane(dt->per_point) != 0;
     const double2 fac = make_double2(dt->fac_re, dt->fac_im), cst = make_double2(dt->c_re, dt->c_im);
     for(int q = 0; q < Q; ++q) {
       double2 f = cst;
@@ -457,18 +463,19 @@ __global__ void __launch_bounds__(256) k_dense(const GroupDev *__restrict__ gd, 
   Geo g;
   element_geometry(dim, X, g);
   double2 acc = make_double2(0., 0.);
-  const int nterms = gd->nterms;
+  const int nterms = __builtin_amdgcn_readfirstlane(gd->nterms);
   for(int k = 0; k < nterms; ++k) {
     const DevTerm *dt = &gd->terms[k];
-    if(dt->kind != 0) continue;
-    if(!all_terms && !dt->per_point) continue;
-    const RuleDev &R = gd->rules[dt->rule];
-    const int Q = R.Q;
-    const bool pp = dt->per_point != 0;
+    if(__builtin_amdgcn_readfirstlane(dt->kind) != 0) continue;
+    const bool pp = __builtin_amdgcn_readfirstlane(dt->per_point) != 0;
+    if(!all_terms && !pp) continue;
+    const RuleDev &R = gd->rules[__builtin_amdgcn_readfirstlane(dt->rule)];
+    const int Q = __builtin_amdgcn_readfirstlane(R.Q);
+    const int fd = __builtin_amdgcn_readfirstlane(dt->form_dof);
     double2 sum = make_double2(0., 0.);
     for(int q = 0; q < Q; ++q) {
       double s;
-      if(dt->form_dof == 1) {
+      if(fd == 1) {
         const double *gj = &R.gbf[(q * N + j) * 3];
         const double *gi = &R.gbf[(q * N + i) * 3];
         s = 0.;
@@ -567,105 +574,137 @@ struct ValTraits<double2> {
 };
 
 // ------------------------------------------------------------------ k_gather
-// Atomic-free CSR assembly.  One 256-thread block per chunk of whole
+// Atomic-free CSR assembly.  One 512-thread block per chunk of whole
 // nonzeros (<= kChunk contributions, <= kChunkElems distinct elements):
-//   1. the chunk's element factors F (FP64 geometry) are staged in LDS;
-//   2. every contribution word (grouped by element, then local pair) is
-//      evaluated as F . T[pair] and parked in LDS at its nonzero-order slot;
+//   1. each thread prefetches its two contribution words; the chunk's element
+//      factors F (FP64 geometry) and table pointers are staged in LDS;
+//   2. every word (grouped by element, then local pair) is evaluated as
+//      F . T[pair] and parked in LDS at its nonzero-order slot;
 //   3. the first slot of each nonzero sums its run in slot order, i.e. in the
 //      element traversal order of the reference's sequential loop, and writes
 //      the value once.  No atomics, bitwise reproducible.
+constexpr int kGatherThreads = 512;
+
 template <bool CPLX, bool MIXED>
-__global__ void __launch_bounds__(256) k_gather(const GroupTable *__restrict__ gtp, const uint32_t *__restrict__ words,
-                                                const uint32_t *__restrict__ chunk_c,
-                                                const uint32_t *__restrict__ chunk_n,
-                                                const uint32_t *__restrict__ chunk_e,
-                                                const uint32_t *__restrict__ elist, void *__restrict__ out)
+__device__ __forceinline__ void gather_eval(uint32_t w, int logP, uint32_t pmask,
+                                            const typename std::conditional<MIXED, float, double>::type (*s_F)[8],
+                                            const void *const *s_T, const uint8_t *s_treal,
+                                            const double2 *const *s_D,
+                                            typename std::conditional<MIXED, float, double>::type *s_re,
+                                            typename std::conditional<MIXED, float, double>::type *s_im,
+                                            uint8_t *s_head)
 {
   using R = typename std::conditional<MIXED, float, double>::type;
+  const int pair = (int)(w & pmask);
+  const int le = (int)((w >> logP) & 0xFFu);
+  const int slot = (int)((w >> (logP + 8)) & 0x3FFu);
+  const R *F = s_F[le];
+  R vr = 0, vi = 0;
+  if(!CPLX || s_treal[le]) {
+    if(MIXED) {
+      const float4 *T = reinterpret_cast<const float4 *>(s_T[le]) + (int64_t)pair * 2;
+      const float4 a = T[0], b = T[1];
+      vr = F[0] * a.x + F[1] * a.y + F[2] * a.z + F[3] * a.w + F[4] * b.x + F[5] * b.y + F[6] * b.z;
+    }
+    else {
+      const double2 *T = reinterpret_cast<const double2 *>(s_T[le]) + (int64_t)pair * 4;
+      const double2 a = T[0], b = T[1], c = T[2], d = T[3];
+      vr = F[0] * a.x + F[1] * a.y + F[2] * b.x + F[3] * b.y + F[4] * c.x + F[5] * c.y + F[6] * d.x;
+    }
+  }
+  else {
+    if(MIXED) {
+      const float4 *T = reinterpret_cast<const float4 *>(s_T[le]) + (int64_t)pair * 4;
+#pragma unroll
+      for(int q = 0; q < 4; ++q) {
+        const float4 tq = T[q];  // components 2q (x,y) and 2q+1 (z,w)
+        vr += F[2 * q] * tq.x + F[2 * q + 1] * tq.z;
+        vi += F[2 * q] * tq.y + F[2 * q + 1] * tq.w;
+      }
+    }
+    else {
+      const double2 *T = reinterpret_cast<const double2 *>(s_T[le]) + (int64_t)pair * 8;
+#pragma unroll
+      for(int q = 0; q < 7; ++q) {
+        const double2 tq = T[q];
+        vr += F[q] * tq.x;
+        vi += F[q] * tq.y;
+      }
+    }
+  }
+  if(s_D[le]) {
+    const double2 d = s_D[le][pair];
+    vr += (R)d.x;
+    vi += (R)d.y;
+  }
+  s_re[slot] = vr;
+  if(CPLX) s_im[slot] = vi;
+  s_head[slot] = (uint8_t)(w >> 31);
+}
+
+template <bool CPLX, bool MIXED>
+__global__ void __launch_bounds__(kGatherThreads)
+  k_gather(const GroupTable *__restrict__ gtp, const uint32_t *__restrict__ words, const uint32_t *__restrict__ chunk_c,
+           const uint32_t *__restrict__ chunk_n, const uint32_t *__restrict__ chunk_e,
+           const uint32_t *__restrict__ elist, void *__restrict__ out)
+{
+  using R = typename std::conditional<MIXED, float, double>::type;
+  static_assert(kChunk == 2 * kGatherThreads, "two contributions per thread");
   const GroupTable &gt = *gtp;
   __shared__ R s_F[kChunkElems][8];
-  __shared__ uint8_t s_g[kChunkElems];
-  __shared__ int32_t s_el[kChunkElems];
+  __shared__ const void *s_T[kChunkElems];
+  __shared__ const double2 *s_D[kChunkElems];
+  __shared__ uint8_t s_treal[kChunkElems];
   __shared__ R s_re[kChunk];
   __shared__ R s_im[CPLX ? kChunk : 1];
   __shared__ uint8_t s_head[kChunk];
-  __shared__ int s_wsum[4];
+  __shared__ int s_wsum[kGatherThreads / 64];
   const int64_t c = blockIdx.x;
   const uint32_t cs = chunk_c[c], len = chunk_c[c + 1] - cs, ns = chunk_n[c];
   const uint32_t es = chunk_e[c], nel = chunk_e[c + 1] - es;
   const int t = threadIdx.x;
   const int logP = gt.logP;
   const uint32_t pmask = (1u << logP) - 1u;
+  const bool v0 = (uint32_t)t < len, v1 = (uint32_t)(t + kGatherThreads) < len;
+  const uint32_t w0 = v0 ? words[cs + t] : 0u;
+  const uint32_t w1 = v1 ? words[cs + t + kGatherThreads] : 0u;
 
-  for(uint32_t k = t; k < nel; k += 256) {
-    const int64_t eg = elist[es + k];
+  if((uint32_t)t < nel) {
+    const int64_t eg = elist[es + t];
     const int g = find_group(gt, eg);
     const int64_t el = eg - gt.e_off[g];
-    s_g[k] = (uint8_t)g;
-    s_el[k] = (int32_t)el;
+    const bool treal = !CPLX || gt.treal[g];
+    s_treal[t] = treal ? 1 : 0;
+    s_T[t] = MIXED ? (treal ? (const void *)gt.T32r[g] : (const void *)gt.T32[g])
+                   : (treal ? (const void *)gt.T64r[g] : (const void *)gt.T64[g]);
+    const int N = gt.N[g];
+    s_D[t] = gt.dense[g] ? gt.dense[g] + el * N * N : nullptr;
     if(MIXED) {
       const float4 *F = reinterpret_cast<const float4 *>(gt.F32[g] + el * 8);
       const float4 a = F[0], b = F[1];
-      s_F[k][0] = a.x; s_F[k][1] = a.y; s_F[k][2] = a.z; s_F[k][3] = a.w;
-      s_F[k][4] = b.x; s_F[k][5] = b.y; s_F[k][6] = b.z; s_F[k][7] = b.w;
+      s_F[t][0] = a.x; s_F[t][1] = a.y; s_F[t][2] = a.z; s_F[t][3] = a.w;
+      s_F[t][4] = b.x; s_F[t][5] = b.y; s_F[t][6] = b.z; s_F[t][7] = b.w;
     }
     else {
       const double2 *F = reinterpret_cast<const double2 *>(gt.F64[g] + el * 8);
 #pragma unroll
       for(int q = 0; q < 4; ++q) {
         const double2 f = F[q];
-        s_F[k][2 * q] = (R)f.x;
-        s_F[k][2 * q + 1] = (R)f.y;
+        s_F[t][2 * q] = (R)f.x;
+        s_F[t][2 * q + 1] = (R)f.y;
       }
     }
   }
   __syncthreads();
-
-  for(uint32_t k = t; k < len; k += 256) {
-    const uint32_t w = words[cs + k];
-    const int pair = (int)(w & pmask);
-    const int le = (int)((w >> logP) & 0xFFu);
-    const int slot = (int)((w >> (logP + 8)) & 0x3FFu);
-    const int g = s_g[le];
-    R vr = 0, vi = 0;
-    if(MIXED) {
-      const float4 *T = reinterpret_cast<const float4 *>(gt.T32[g] + (int64_t)pair * 8);
-#pragma unroll
-      for(int q = 0; q < 4; ++q) {
-        const float4 tq = T[q];  // components 2q (x,y) and 2q+1 (z,w)
-        vr += s_F[le][2 * q] * tq.x + s_F[le][2 * q + 1] * tq.z;
-        if(CPLX) vi += s_F[le][2 * q] * tq.y + s_F[le][2 * q + 1] * tq.w;
-      }
-    }
-    else {
-      const double2 *T = gt.T64[g] + (int64_t)pair * 8;
-#pragma unroll
-      for(int q = 0; q < 7; ++q) {
-        const double2 tq = T[q];
-        vr += s_F[le][q] * tq.x;
-        if(CPLX) vi += s_F[le][q] * tq.y;
-      }
-    }
-    if(gt.dense[g]) {
-      const int N = gt.N[g];
-      const double2 d = gt.dense[g][(int64_t)s_el[le] * N * N + pair];
-      vr += (R)d.x;
-      vi += (R)d.y;
-    }
-    s_re[slot] = vr;
-    if(CPLX) s_im[slot] = vi;
-    s_head[slot] = (uint8_t)(w >> 31);
-  }
+  if(v0) gather_eval<CPLX, MIXED>(w0, logP, pmask, s_F, s_T, s_treal, s_D, s_re, s_im, s_head);
+  if(v1) gather_eval<CPLX, MIXED>(w1, logP, pmask, s_F, s_T, s_treal, s_D, s_re, s_im, s_head);
   __syncthreads();
 
-  // thread t owns slots [4t, 4t+4): rank of its first head = exclusive scan
-  // of head counts
-  const int s0 = 4 * t;
-  int cnt = 0;
-#pragma unroll
-  for(int q = 0; q < 4; ++q)
-    if(s0 + q < (int)len) cnt += s_head[s0 + q];
+  // thread t owns slots 2t, 2t+1: rank of its first head = exclusive scan of
+  // the head counts
+  const int s0 = 2 * t;
+  const int h0 = (s0 < (int)len) ? s_head[s0] : 0, h1 = (s0 + 1 < (int)len) ? s_head[s0 + 1] : 0;
+  const int cnt = h0 + h1;
   const int lane = t & 63, wv = t >> 6;
   int incl = cnt;
 #pragma unroll
@@ -677,9 +716,10 @@ __global__ void __launch_bounds__(256) k_gather(const GroupTable *__restrict__ g
   __syncthreads();
   int rank = incl - cnt;
   for(int k = 0; k < wv; ++k) rank += s_wsum[k];
-  for(int q = 0; q < 4; ++q) {
+#pragma unroll
+  for(int q = 0; q < 2; ++q) {
     const int sl = s0 + q;
-    if(sl >= (int)len || !s_head[sl]) continue;
+    if(!(q == 0 ? h0 : h1)) continue;
     R sre = s_re[sl], sim = CPLX ? s_im[sl] : (R)0;
     for(int k = sl + 1; k < (int)len && !s_head[k]; ++k) {
       sre += s_re[k];
@@ -851,7 +891,8 @@ hipError_t launch_tensor(const Group &g, hipStream_t s)
     a.bf[r] = g.rules[r].d_bf;
     a.gbf[r] = g.rules[r].d_gbf;
   }
-  hipLaunchKernelGGL(k_tensor, dim3(blocks_for(g.N * g.N, 64)), dim3(64), 0, s, a, g.d_T64, g.d_T32);
+  hipLaunchKernelGGL(k_tensor, dim3(blocks_for(g.N * g.N, 64)), dim3(64), 0, s, a, g.d_T64, g.d_T32, g.d_T64r,
+                     g.d_T32r);
   return hipGetLastError();
 }
 
@@ -906,7 +947,7 @@ hipError_t launch_gather(const gfem_system &sys, int prec, hipStream_t s)
   if(sys.n_chunks == 0) return hipSuccess;
   void *out = sys.d_vals[prec];
   const GroupTable *gt = sys.d_table;
-  const dim3 grid((unsigned)sys.n_chunks), block(256);
+  const dim3 grid((unsigned)sys.n_chunks), block(kGatherThreads);
 #define GFEM_GATHER_ARGS gt, sys.d_words, sys.d_chunk_c, sys.d_chunk_n, sys.d_chunk_e, sys.d_elist, out
   if(sys.cplx) {
     if(prec)
