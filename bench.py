@@ -52,6 +52,24 @@ def kernel_profile(F, precision, strategy, reps=5):
     return {k: sum(v) / len(v) * (len(v) / reps) / (len(v) / reps) for k, v in acc.items()}, acc
 
 
+def copy_calibration(nbytes):
+    """Device-to-device copy of nbytes (the runtime's copy kernel), used to
+    scale the FETCH_SIZE / WRITE_SIZE counters: a plain copy moves exactly
+    nbytes in each direction."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(nbytes)) == 0
+    for _ in range(3):
+        assert hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"copy calibration: 3 x {nbytes} bytes device-to-device")
+
+
 def cpu_baseline(args):
     """The oracle (single-threaded restatement of the reference CPU assembler)
     on a bounded sample of the same workload: a smaller cube, same order,
@@ -81,14 +99,18 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=14)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the mixed / reference-strategy lines")
+    ap.add_argument("--calib-copy", type=int, default=0, metavar="BYTES",
+                    help="run a device-to-device copy of BYTES first (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (this build runs on a single "
               f"MI355X; multi-GPU sharding is a design note only)", file=sys.stderr)
         sys.exit(2)
 
-    import math
     from gfem import workloads as W
+
+    if args.calib_copy:
+        copy_calibration(args.calib_copy)
 
     t0 = time.perf_counter()
     spec = W.helmholtz3d(n=args.n, order=args.order)

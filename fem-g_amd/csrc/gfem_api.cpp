@@ -477,7 +477,11 @@ gfem_status gfem_system_finalize(gfem_system *sys)
     for(const DevTerm &d : g.dterms)
       if(d.kind == 0 && !d.per_point && (d.c_re * d.fac_im + d.c_im * d.fac_re) != 0.) g.treal = false;
     if(g.has_dense) GFEM_HIP(ctx, dalloc(&g.d_dense, (size_t)(g.ne * g.N * g.N)));
-    if(g.has_linear) GFEM_HIP(ctx, dalloc(&g.d_be, (size_t)(sys->nrhs * g.ne * g.N)));
+    if(g.has_linear) {
+      // right-hand sides without a linear term on this group keep zeros
+      GFEM_HIP(ctx, dalloc(&g.d_be, (size_t)(sys->nrhs * g.ne * g.N)));
+      GFEM_HIP(ctx, hipMemset(g.d_be, 0, sizeof(double2) * (size_t)(sys->nrhs * g.ne * g.N)));
+    }
     for(auto &R : g.rules) {
       GFEM_HIP(ctx, dupload(&R.d_w, R.w.data(), R.w.size()));
       GFEM_HIP(ctx, dupload(&R.d_pts, R.pts.data(), R.pts.size()));

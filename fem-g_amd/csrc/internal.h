@@ -29,8 +29,8 @@ namespace gfem {
 
 constexpr int kMaxGroups = 16;
 constexpr int kMaxN = 64;
-constexpr int kChunk = 1024;      // contributions per gather block (whole nonzeros)
-constexpr int kChunkElems = 256;  // distinct elements per gather block
+constexpr int kChunk = 512;       // contributions per gather block (whole nonzeros)
+constexpr int kChunkElems = 128;  // distinct elements per gather block
 
 // Coefficient program as read by the kernels.
 struct DevExpr {

@@ -398,45 +398,59 @@ __global__ void __launch_bounds__(256) k_geometry(int dim, int nv, int64_t ne, c
 }
 
 // ------------------------------------------------------------------ k_linear
-// One thread per element: b_e(i) += w det conj(c_tf) phi_i f(x_q) over the
-// group's linear terms of right-hand side `rhs` (tables read through the
-// device-resident group descriptor).
-template <int N>
-__global__ void __launch_bounds__(128) k_linear(const GroupDev *__restrict__ gd, int rhs, int64_t ne,
+// One linear term on one group, one thread per element:
+//   b_e(i) (+)= sum_q w_q det conj(c_tf) phi_i(q) f(x_q)
+// The coefficient program travels in the kernel arguments (scalar loads,
+// uniform across the wave); the rule's weights, points and basis values are
+// staged in LDS once per block.
+struct LinTerm {
+  DevExpr expr;
+  double fac_re, fac_im, c_re, c_im;
+  int per_point, Q, dim, nv, accumulate;
+};
+
+template <int N, bool STAGED>
+__global__ void __launch_bounds__(128) k_linear(LinTerm a, const double *__restrict__ w, const double *__restrict__ pts,
+                                                const double *__restrict__ bf, int64_t ne,
                                                 const int32_t *__restrict__ vtx, const double *__restrict__ xyz,
                                                 const double *__restrict__ F64, double2 *__restrict__ be)
 {
+  extern __shared__ double s_tab[];
+  const int Q = a.Q;
+  const double *tw = w, *tp = pts, *tb = bf;
+  if(STAGED) {
+    double *lw = s_tab, *lp = s_tab + Q, *lb = s_tab + 4 * Q;
+    for(int k = threadIdx.x; k < Q; k += blockDim.x) lw[k] = w[k];
+    for(int k = threadIdx.x; k < 3 * Q; k += blockDim.x) lp[k] = pts[k];
+    for(int k = threadIdx.x; k < Q * N; k += blockDim.x) lb[k] = bf[k];
+    __syncthreads();
+    tw = lw;
+    tp = lp;
+    tb = lb;
+  }
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if(e >= ne) return;
-  const int dim = gd->dim;
   double X[4][3];
-  load_vertices(vtx, gd->nv, e, xyz, X);
+  load_vertices(vtx, a.nv, e, xyz, X);
   const double det = F64[e * 8 + 6];
   double2 acc[N];
-  for(int i = 0; i < N; ++i) acc[i] = make_double2(0., 0.);
-  const int nterms = __builtin_amdgcn_readfirstlane(gd->nterms);
-  for(int t = 0; t < nterms; ++t) {
-    const DevTerm *dt = &gd->terms[t];
-    if(__builtin_amdgcn_readfirstlane(dt->kind) != 1 || __builtin_amdgcn_readfirstlane(dt->rhs_idx) != rhs) continue;
-    const RuleDev &R = gd->rules[__builtin_amdgcn_readfirstlane(dt->rule)];
-    const int Q = __builtin_amdgcn_readfirstlane(R.Q);
-    const bool pp = __builtin_amdgcn_readfirstlane(dt->per_point) != 0;
-    const double2 fac = make_double2(dt->fac_re, dt->fac_im), cst = make_double2(dt->c_re, dt->c_im);
-    for(int q = 0; q < Q; ++q) {
-      double2 f = cst;
-      if(pp) {
-        double x[3];
-        phys_point(dim, X, &R.pts[3 * q], x);
-        f = eval_expr(&dt->expr, x[0], x[1], x[2]);
-      }
-      const double wd = R.w[q] * det;
-      const double2 cf = cmul(make_double2(fac.x * wd, fac.y * wd), f);
 #pragma unroll
-      for(int i = 0; i < N; ++i) {
-        const double b = R.bf[q * N + i];
-        acc[i].x += cf.x * b;
-        acc[i].y += cf.y * b;
-      }
+  for(int i = 0; i < N; ++i) acc[i] = a.accumulate ? be[e * N + i] : make_double2(0., 0.);
+  const double2 fac = make_double2(a.fac_re, a.fac_im), cst = make_double2(a.c_re, a.c_im);
+  for(int q = 0; q < Q; ++q) {
+    double2 f = cst;
+    if(a.per_point) {
+      double x[3];
+      phys_point(a.dim, X, &tp[3 * q], x);
+      f = eval_expr(&a.expr, x[0], x[1], x[2]);
+    }
+    const double wd = tw[q] * det;
+    const double2 cf = cmul(make_double2(fac.x * wd, fac.y * wd), f);
+#pragma unroll
+    for(int i = 0; i < N; ++i) {
+      const double b = tb[q * N + i];
+      acc[i].x += cf.x * b;
+      acc[i].y += cf.y * b;
     }
   }
 #pragma unroll
@@ -574,7 +588,7 @@ struct ValTraits<double2> {
 };
 
 // ------------------------------------------------------------------ k_gather
-// Atomic-free CSR assembly.  One 512-thread block per chunk of whole
+// Atomic-free CSR assembly.  One 256-thread block per chunk of whole
 // nonzeros (<= kChunk contributions, <= kChunkElems distinct elements):
 //   1. each thread prefetches its two contribution words; the chunk's element
 //      factors F (FP64 geometry) and table pointers are staged in LDS;
@@ -583,7 +597,7 @@ struct ValTraits<double2> {
 //   3. the first slot of each nonzero sums its run in slot order, i.e. in the
 //      element traversal order of the reference's sequential loop, and writes
 //      the value once.  No atomics, bitwise reproducible.
-constexpr int kGatherThreads = 512;
+constexpr int kGatherThreads = 256;
 
 template <bool CPLX, bool MIXED>
 __device__ __forceinline__ void gather_eval(uint32_t w, int logP, uint32_t pmask,
@@ -905,10 +919,15 @@ hipError_t launch_geometry(const Group &g, const double *xyz, hipStream_t s)
 }
 
 template <int N>
-static void linear_n(const Group &g, int rhs, const double *xyz, double2 *be, hipStream_t s)
+static void linear_n(const LinTerm &a, const Rule &R, size_t lds, const Group &g, const double *xyz, double2 *be,
+                     hipStream_t s)
 {
-  hipLaunchKernelGGL(k_linear<N>, dim3(blocks_for(g.ne, 128)), dim3(128), 0, s, g.d_dev, rhs, g.ne, g.d_vtx, xyz,
-                     g.d_F64, be);
+  if(lds)
+    hipLaunchKernelGGL((k_linear<N, true>), dim3(blocks_for(g.ne, 128)), dim3(128), lds, s, a, R.d_w, R.d_pts, R.d_bf,
+                       g.ne, g.d_vtx, xyz, g.d_F64, be);
+  else
+    hipLaunchKernelGGL((k_linear<N, false>), dim3(blocks_for(g.ne, 128)), dim3(128), 0, s, a, R.d_w, R.d_pts, R.d_bf,
+                       g.ne, g.d_vtx, xyz, g.d_F64, be);
 }
 
 hipError_t launch_linear(const Group &g, const double *xyz, int nrhs, hipStream_t s)
@@ -916,18 +935,37 @@ hipError_t launch_linear(const Group &g, const double *xyz, int nrhs, hipStream_
   if(g.ne == 0 || !g.has_linear) return hipSuccess;
   for(int r = 0; r < nrhs; ++r) {
     double2 *be = g.d_be + (int64_t)r * g.ne * g.N;
-    switch(g.N) {
-    case 1: linear_n<1>(g, r, xyz, be, s); break;
-    case 2: linear_n<2>(g, r, xyz, be, s); break;
-    case 3: linear_n<3>(g, r, xyz, be, s); break;
-    case 4: linear_n<4>(g, r, xyz, be, s); break;
-    case 5: linear_n<5>(g, r, xyz, be, s); break;
-    case 6: linear_n<6>(g, r, xyz, be, s); break;
-    case 10: linear_n<10>(g, r, xyz, be, s); break;
-    case 15: linear_n<15>(g, r, xyz, be, s); break;
-    case 20: linear_n<20>(g, r, xyz, be, s); break;
-    case 35: linear_n<35>(g, r, xyz, be, s); break;
-    default: return hipErrorInvalidValue;
+    bool first = true;
+    for(const DevTerm &t : g.dterms) {
+      if(t.kind != 1 || t.rhs_idx != r) continue;
+      const Rule &R = g.rules[t.rule];
+      LinTerm a{};
+      a.expr = t.expr;
+      a.fac_re = t.fac_re;
+      a.fac_im = t.fac_im;
+      a.c_re = t.c_re;
+      a.c_im = t.c_im;
+      a.per_point = t.per_point;
+      a.Q = R.Q;
+      a.dim = g.dim;
+      a.nv = g.nv;
+      a.accumulate = first ? 0 : 1;
+      const size_t lds = sizeof(double) * (size_t)R.Q * (4 + g.N);
+      const size_t dyn = lds <= 48 * 1024 ? lds : 0;  // stage the rule in LDS when it fits
+      switch(g.N) {
+      case 1: linear_n<1>(a, R, dyn, g, xyz, be, s); break;
+      case 2: linear_n<2>(a, R, dyn, g, xyz, be, s); break;
+      case 3: linear_n<3>(a, R, dyn, g, xyz, be, s); break;
+      case 4: linear_n<4>(a, R, dyn, g, xyz, be, s); break;
+      case 5: linear_n<5>(a, R, dyn, g, xyz, be, s); break;
+      case 6: linear_n<6>(a, R, dyn, g, xyz, be, s); break;
+      case 10: linear_n<10>(a, R, dyn, g, xyz, be, s); break;
+      case 15: linear_n<15>(a, R, dyn, g, xyz, be, s); break;
+      case 20: linear_n<20>(a, R, dyn, g, xyz, be, s); break;
+      case 35: linear_n<35>(a, R, dyn, g, xyz, be, s); break;
+      default: return hipErrorInvalidValue;
+      }
+      first = false;
     }
   }
   return hipGetLastError();
