@@ -533,6 +533,13 @@ gfem_status gfem_system_finalize(gfem_system *sys)
     }
     if(total >= (int64_t(1) << 32)) return fail(ctx, GFEM_ERR_TOO_LARGE, "too many contributions");
     if(n_elem >= (int64_t(1) << 32)) return fail(ctx, GFEM_ERR_TOO_LARGE, "too many elements");
+    for(auto &g : sys->groups)
+      if(g.ne >= (int64_t(1) << 27)) return fail(ctx, GFEM_ERR_TOO_LARGE, "more than 2^27 elements in a group");
+    {
+      int cus = 0;
+      if(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device) == hipSuccess && cus > 0)
+        sys->gather_blocks = (int64_t)cus * 8;
+    }
     std::vector<uint32_t> c_elem((size_t)total);
     std::vector<uint16_t> c_pair((size_t)total);
     {
@@ -570,7 +577,10 @@ gfem_status gfem_system_finalize(gfem_system *sys)
           if(mark[c_elem[k]] != chunk) {
             mark[c_elem[k]] = chunk;
             local[c_elem[k]] = nel++;
-            elist.push_back(c_elem[k]);
+            // (group << 27) | element index within its group
+            int g = 0;
+            while(g + 1 < (int)sys->groups.size() && (int64_t)c_elem[k] >= sys->groups[g + 1].e_off) ++g;
+            elist.push_back(((uint32_t)g << 27) | (uint32_t)(c_elem[k] - sys->groups[g].e_off));
           }
         ++nz;
       }

@@ -9,7 +9,8 @@
 //                           constant-coefficient bilinear terms
 //   gather chunks           runs of whole nonzeros, <= kChunk contributions
 //                           and <= kChunkElems distinct elements each, with
-//                           the chunk's element list (global element slots)
+//                           the chunk's element list, entries (group << 27) |
+//                           element index within the group
 //   contribution words      uint32, grouped by chunk element then pair:
 //                           bits [0,logP) local pair i_tf*N + j_dof,
 //                           [logP,logP+8) chunk element, [logP+8,logP+18)
@@ -155,6 +156,7 @@ struct gfem_system {
   uint32_t *d_chunk_c = nullptr, *d_chunk_n = nullptr, *d_chunk_e = nullptr, *d_elist = nullptr;
   int64_t n_elist = 0;
   int64_t n_chunks = 0;
+  int64_t gather_blocks = 2048;  // persistent gather grid (CUs x resident blocks)
   uint32_t *d_lin_words = nullptr, *d_dc_words = nullptr;
   int32_t *d_dc_fix = nullptr;
   int64_t *d_lin_ptr = nullptr, *d_dc_ptr = nullptr;

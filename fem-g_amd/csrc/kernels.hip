@@ -21,6 +21,7 @@
 //                matrices + atomic scatter) used as the FP64 baseline.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <type_traits>
 
 #include "internal.h"
@@ -457,6 +458,53 @@ __global__ void __launch_bounds__(128) k_linear(LinTerm a, const double *__restr
   for(int i = 0; i < N; ++i) be[e * N + i] = acc[i];
 }
 
+// Point-parallel form of the same term: one thread per (element, Gauss
+// point) evaluates c_q = w_q det conj(c_tf) f(x_q) into LDS, then one thread
+// per (element, local dof) forms b_e(i) (+)= sum_q phi_i(q) c_q.  Used when
+// the rule has at most 256 points; keeps registers low and every lane busy.
+__global__ void __launch_bounds__(256) k_linear_pp(LinTerm a, int N, const double *__restrict__ w,
+                                                   const double *__restrict__ pts, const double *__restrict__ bf,
+                                                   int64_t ne, const int32_t *__restrict__ vtx,
+                                                   const double *__restrict__ xyz, const double *__restrict__ F64,
+                                                   double2 *__restrict__ be)
+{
+  extern __shared__ double s_dyn[];
+  const int Q = a.Q;
+  const int EB = 256 / Q;  // elements per block
+  double *s_bf = s_dyn;                    // [Q][N]
+  double2 *s_c = reinterpret_cast<double2 *>(s_dyn + Q * N);  // [EB][Q]
+  for(int k = threadIdx.x; k < Q * N; k += 256) s_bf[k] = bf[k];
+  const int t = threadIdx.x;
+  const int el = t / Q, q = t - el * Q;
+  const int64_t e = (int64_t)blockIdx.x * EB + el;
+  if(el < EB && e < ne) {
+    double X[4][3];
+    load_vertices(vtx, a.nv, e, xyz, X);
+    double2 f = make_double2(a.c_re, a.c_im);
+    if(a.per_point) {
+      double x[3];
+      phys_point(a.dim, X, &pts[3 * q], x);
+      f = eval_expr(&a.expr, x[0], x[1], x[2]);
+    }
+    const double wd = w[q] * F64[e * 8 + 6];
+    s_c[el * Q + q] = cmul(make_double2(a.fac_re * wd, a.fac_im * wd), f);
+  }
+  __syncthreads();
+  for(int k = t; k < EB * N; k += 256) {
+    const int le = k / N, i = k - le * N;
+    const int64_t ee = (int64_t)blockIdx.x * EB + le;
+    if(ee >= ne) continue;
+    double2 acc = a.accumulate ? be[ee * N + i] : make_double2(0., 0.);
+    const double2 *c = &s_c[le * Q];
+    for(int qq = 0; qq < Q; ++qq) {
+      const double b = s_bf[qq * N + i];
+      acc.x += c[qq].x * b;
+      acc.y += c[qq].y * b;
+    }
+    be[ee * N + i] = acc;
+  }
+}
+
 // ------------------------------------------------------------------ k_dense
 // one thread per (element, pair): the reference's per-Gauss-point loop
 // A_e(j,i) += w det (c E_j) . conj(c_tf E_i)
@@ -588,10 +636,11 @@ struct ValTraits<double2> {
 };
 
 // ------------------------------------------------------------------ k_gather
-// Atomic-free CSR assembly.  One 256-thread block per chunk of whole
-// nonzeros (<= kChunk contributions, <= kChunkElems distinct elements):
-//   1. each thread prefetches its two contribution words; the chunk's element
-//      factors F (FP64 geometry) and table pointers are staged in LDS;
+// Atomic-free CSR assembly.  Persistent 256-thread blocks walk the chunks of
+// whole nonzeros (<= kChunk contributions, <= kChunkElems distinct elements
+// each); while a chunk is processed the next chunk's header, words and
+// element list are already in flight.  Per chunk:
+//   1. the chunk's element factors F (FP64 geometry) are staged in LDS;
 //   2. every word (grouped by element, then local pair) is evaluated as
 //      F . T[pair] and parked in LDS at its nonzero-order slot;
 //   3. the first slot of each nonzero sums its run in slot order, i.e. in the
@@ -660,11 +709,15 @@ template <bool CPLX, bool MIXED>
 __global__ void __launch_bounds__(kGatherThreads)
   k_gather(const GroupTable *__restrict__ gtp, const uint32_t *__restrict__ words, const uint32_t *__restrict__ chunk_c,
            const uint32_t *__restrict__ chunk_n, const uint32_t *__restrict__ chunk_e,
-           const uint32_t *__restrict__ elist, void *__restrict__ out)
+           const uint32_t *__restrict__ elist, int64_t nchunks, void *__restrict__ out)
 {
   using R = typename std::conditional<MIXED, float, double>::type;
   static_assert(kChunk == 2 * kGatherThreads, "two contributions per thread");
-  const GroupTable &gt = *gtp;
+  __shared__ const void *s_gF[kMaxGroups];
+  __shared__ const void *s_gT[kMaxGroups];
+  __shared__ const double2 *s_gD[kMaxGroups];
+  __shared__ int s_gN[kMaxGroups];
+  __shared__ uint8_t s_gR[kMaxGroups];
   __shared__ R s_F[kChunkElems][8];
   __shared__ const void *s_T[kChunkElems];
   __shared__ const double2 *s_D[kChunkElems];
@@ -673,82 +726,113 @@ __global__ void __launch_bounds__(kGatherThreads)
   __shared__ R s_im[CPLX ? kChunk : 1];
   __shared__ uint8_t s_head[kChunk];
   __shared__ int s_wsum[kGatherThreads / 64];
-  const int64_t c = blockIdx.x;
-  const uint32_t cs = chunk_c[c], len = chunk_c[c + 1] - cs, ns = chunk_n[c];
-  const uint32_t es = chunk_e[c], nel = chunk_e[c + 1] - es;
+  const GroupTable &gt = *gtp;
   const int t = threadIdx.x;
   const int logP = gt.logP;
   const uint32_t pmask = (1u << logP) - 1u;
-  const bool v0 = (uint32_t)t < len, v1 = (uint32_t)(t + kGatherThreads) < len;
-  const uint32_t w0 = v0 ? words[cs + t] : 0u;
-  const uint32_t w1 = v1 ? words[cs + t + kGatherThreads] : 0u;
-
-  if((uint32_t)t < nel) {
-    const int64_t eg = elist[es + t];
-    const int g = find_group(gt, eg);
-    const int64_t el = eg - gt.e_off[g];
-    const bool treal = !CPLX || gt.treal[g];
-    s_treal[t] = treal ? 1 : 0;
-    s_T[t] = MIXED ? (treal ? (const void *)gt.T32r[g] : (const void *)gt.T32[g])
-                   : (treal ? (const void *)gt.T64r[g] : (const void *)gt.T64[g]);
-    const int N = gt.N[g];
-    s_D[t] = gt.dense[g] ? gt.dense[g] + el * N * N : nullptr;
-    if(MIXED) {
-      const float4 *F = reinterpret_cast<const float4 *>(gt.F32[g] + el * 8);
-      const float4 a = F[0], b = F[1];
-      s_F[t][0] = a.x; s_F[t][1] = a.y; s_F[t][2] = a.z; s_F[t][3] = a.w;
-      s_F[t][4] = b.x; s_F[t][5] = b.y; s_F[t][6] = b.z; s_F[t][7] = b.w;
-    }
-    else {
-      const double2 *F = reinterpret_cast<const double2 *>(gt.F64[g] + el * 8);
+  if(t < gt.ng) {
+    const bool treal = !CPLX || gt.treal[t];
+    s_gR[t] = treal ? 1 : 0;
+    s_gF[t] = MIXED ? (const void *)gt.F32[t] : (const void *)gt.F64[t];
+    s_gT[t] = MIXED ? (treal ? (const void *)gt.T32r[t] : (const void *)gt.T32[t])
+                    : (treal ? (const void *)gt.T64r[t] : (const void *)gt.T64[t]);
+    s_gD[t] = gt.dense[t];
+    s_gN[t] = gt.N[t];
+  }
+  int64_t c = blockIdx.x;
+  uint32_t cs = 0, len = 0, ns = 0, nel = 0, w0 = 0, w1 = 0, ev = 0;
+  if(c < nchunks) {
+    cs = chunk_c[c];
+    len = chunk_c[c + 1] - cs;
+    ns = chunk_n[c];
+    const uint32_t es = chunk_e[c];
+    nel = chunk_e[c + 1] - es;
+    w0 = (uint32_t)t < len ? words[cs + t] : 0u;
+    w1 = (uint32_t)(t + kGatherThreads) < len ? words[cs + t + kGatherThreads] : 0u;
+    ev = (uint32_t)t < nel ? elist[es + t] : 0u;
+  }
+  __syncthreads();
+  for(; c < nchunks; c += gridDim.x) {
+    // element factors of this chunk
+    if((uint32_t)t < nel) {
+      const int g = (int)(ev >> 27);
+      const int64_t el = ev & 0x7FFFFFFu;
+      s_treal[t] = s_gR[g];
+      s_T[t] = s_gT[g];
+      const int N = s_gN[g];
+      s_D[t] = s_gD[g] ? s_gD[g] + el * N * N : nullptr;
+      if(MIXED) {
+        const float4 *F = reinterpret_cast<const float4 *>(reinterpret_cast<const float *>(s_gF[g]) + el * 8);
+        const float4 a = F[0], b = F[1];
+        s_F[t][0] = a.x; s_F[t][1] = a.y; s_F[t][2] = a.z; s_F[t][3] = a.w;
+        s_F[t][4] = b.x; s_F[t][5] = b.y; s_F[t][6] = b.z; s_F[t][7] = b.w;
+      }
+      else {
+        const double2 *F = reinterpret_cast<const double2 *>(reinterpret_cast<const double *>(s_gF[g]) + el * 8);
 #pragma unroll
-      for(int q = 0; q < 4; ++q) {
-        const double2 f = F[q];
-        s_F[t][2 * q] = (R)f.x;
-        s_F[t][2 * q + 1] = (R)f.y;
+        for(int q = 0; q < 4; ++q) {
+          const double2 f = F[q];
+          s_F[t][2 * q] = (R)f.x;
+          s_F[t][2 * q + 1] = (R)f.y;
+        }
       }
     }
-  }
-  __syncthreads();
-  if(v0) gather_eval<CPLX, MIXED>(w0, logP, pmask, s_F, s_T, s_treal, s_D, s_re, s_im, s_head);
-  if(v1) gather_eval<CPLX, MIXED>(w1, logP, pmask, s_F, s_T, s_treal, s_D, s_re, s_im, s_head);
-  __syncthreads();
+    const uint32_t clen = len, cns = ns, cw0 = w0, cw1 = w1;
+    // next chunk in flight while this one is processed
+    const int64_t cn = c + gridDim.x;
+    if(cn < nchunks) {
+      cs = chunk_c[cn];
+      len = chunk_c[cn + 1] - cs;
+      ns = chunk_n[cn];
+      const uint32_t es = chunk_e[cn];
+      nel = chunk_e[cn + 1] - es;
+      w0 = (uint32_t)t < len ? words[cs + t] : 0u;
+      w1 = (uint32_t)(t + kGatherThreads) < len ? words[cs + t + kGatherThreads] : 0u;
+      ev = (uint32_t)t < nel ? elist[es + t] : 0u;
+    }
+    __syncthreads();
+    if((uint32_t)t < clen) gather_eval<CPLX, MIXED>(cw0, logP, pmask, s_F, s_T, s_treal, s_D, s_re, s_im, s_head);
+    if((uint32_t)(t + kGatherThreads) < clen)
+      gather_eval<CPLX, MIXED>(cw1, logP, pmask, s_F, s_T, s_treal, s_D, s_re, s_im, s_head);
+    __syncthreads();
 
-  // thread t owns slots 2t, 2t+1: rank of its first head = exclusive scan of
-  // the head counts
-  const int s0 = 2 * t;
-  const int h0 = (s0 < (int)len) ? s_head[s0] : 0, h1 = (s0 + 1 < (int)len) ? s_head[s0 + 1] : 0;
-  const int cnt = h0 + h1;
-  const int lane = t & 63, wv = t >> 6;
-  int incl = cnt;
+    // thread t owns slots 2t, 2t+1: rank of its first head = exclusive scan
+    // of the head counts
+    const int s0 = 2 * t;
+    const int h0 = (s0 < (int)clen) ? s_head[s0] : 0, h1 = (s0 + 1 < (int)clen) ? s_head[s0 + 1] : 0;
+    const int cnt = h0 + h1;
+    const int lane = t & 63, wv = t >> 6;
+    int incl = cnt;
 #pragma unroll
-  for(int d = 1; d < 64; d <<= 1) {
-    const int v = __shfl_up(incl, d, 64);
-    if(lane >= d) incl += v;
-  }
-  if(lane == 63) s_wsum[wv] = incl;
-  __syncthreads();
-  int rank = incl - cnt;
-  for(int k = 0; k < wv; ++k) rank += s_wsum[k];
+    for(int d = 1; d < 64; d <<= 1) {
+      const int v = __shfl_up(incl, d, 64);
+      if(lane >= d) incl += v;
+    }
+    if(lane == 63) s_wsum[wv] = incl;
+    __syncthreads();
+    int rank = incl - cnt;
+    for(int k = 0; k < wv; ++k) rank += s_wsum[k];
 #pragma unroll
-  for(int q = 0; q < 2; ++q) {
-    const int sl = s0 + q;
-    if(!(q == 0 ? h0 : h1)) continue;
-    R sre = s_re[sl], sim = CPLX ? s_im[sl] : (R)0;
-    for(int k = sl + 1; k < (int)len && !s_head[k]; ++k) {
-      sre += s_re[k];
-      if(CPLX) sim += s_im[k];
+    for(int q = 0; q < 2; ++q) {
+      const int sl = s0 + q;
+      if(!(q == 0 ? h0 : h1)) continue;
+      R sre = s_re[sl], sim = CPLX ? s_im[sl] : (R)0;
+      for(int k = sl + 1; k < (int)clen && !s_head[k]; ++k) {
+        sre += s_re[k];
+        if(CPLX) sim += s_im[k];
+      }
+      const int64_t pos = (int64_t)cns + rank++;
+      if(CPLX) {
+        if(MIXED)
+          reinterpret_cast<float2 *>(out)[pos] = make_float2((float)sre, (float)sim);
+        else
+          reinterpret_cast<double2 *>(out)[pos] = make_double2((double)sre, (double)sim);
+      }
+      else {
+        reinterpret_cast<R *>(out)[pos] = sre;
+      }
     }
-    const int64_t pos = (int64_t)ns + rank++;
-    if(CPLX) {
-      if(MIXED)
-        reinterpret_cast<float2 *>(out)[pos] = make_float2((float)sre, (float)sim);
-      else
-        reinterpret_cast<double2 *>(out)[pos] = make_double2((double)sre, (double)sim);
-    }
-    else {
-      reinterpret_cast<R *>(out)[pos] = sre;
-    }
+    __syncthreads();  // LDS is reused by the next chunk
   }
 }
 
@@ -950,6 +1034,15 @@ hipError_t launch_linear(const Group &g, const double *xyz, int nrhs, hipStream_
       a.dim = g.dim;
       a.nv = g.nv;
       a.accumulate = first ? 0 : 1;
+      const size_t lds_pp = sizeof(double) * (size_t)R.Q * g.N + sizeof(double2) * (size_t)(256 / std::max(R.Q, 1)) * R.Q;
+      if(R.Q <= 256 && lds_pp <= 64 * 1024) {
+        const int EB = 256 / R.Q;
+        const size_t lds = lds_pp;
+        hipLaunchKernelGGL(k_linear_pp, dim3((unsigned)((g.ne + EB - 1) / EB)), dim3(256), lds, s, a, g.N, R.d_w,
+                           R.d_pts, R.d_bf, g.ne, g.d_vtx, xyz, g.d_F64, be);
+        first = false;
+        continue;
+      }
       const size_t lds = sizeof(double) * (size_t)R.Q * (4 + g.N);
       const size_t dyn = lds <= 48 * 1024 ? lds : 0;  // stage the rule in LDS when it fits
       switch(g.N) {
@@ -985,8 +1078,8 @@ hipError_t launch_gather(const gfem_system &sys, int prec, hipStream_t s)
   if(sys.n_chunks == 0) return hipSuccess;
   void *out = sys.d_vals[prec];
   const GroupTable *gt = sys.d_table;
-  const dim3 grid((unsigned)sys.n_chunks), block(kGatherThreads);
-#define GFEM_GATHER_ARGS gt, sys.d_words, sys.d_chunk_c, sys.d_chunk_n, sys.d_chunk_e, sys.d_elist, out
+  const dim3 grid((unsigned)std::min<int64_t>(sys.n_chunks, sys.gather_blocks)), block(kGatherThreads);
+#define GFEM_GATHER_ARGS gt, sys.d_words, sys.d_chunk_c, sys.d_chunk_n, sys.d_chunk_e, sys.d_elist, sys.n_chunks, out
   if(sys.cplx) {
     if(prec)
       hipLaunchKernelGGL((k_gather<true, true>), grid, block, 0, s, GFEM_GATHER_ARGS);
