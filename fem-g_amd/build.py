@@ -46,7 +46,7 @@ def build(verbose=False):
     import pybind11
     py_inc = sysconfig.get_paths()["include"]
 
-    dev_objs = [os.path.join(OBJ, "kernels.o"), os.path.join(OBJ, "gfem_api.o")]
+    dev_objs = [os.path.join(OBJ, "kernels.o"), os.path.join(OBJ, "gfem_api.o"), os.path.join(OBJ, "jit.o")]
     host_srcs = ["model.cpp", "gmshfem.cpp"]
     host_objs = [os.path.join(OBJ, s.replace(".cpp", ".o")) for s in host_srcs]
     py_obj = os.path.join(OBJ, "pybind.o")
@@ -54,6 +54,7 @@ def build(verbose=False):
     jobs = [
         [HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "kernels.hip"), "-o", dev_objs[0]],
         [HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "gfem_api.cpp"), "-o", dev_objs[1]],
+        [HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "jit.cpp"), "-o", dev_objs[2]],
     ]
     jobs += [["g++", *CXX_FLAGS, "-c", os.path.join(HOST, s), "-o", o] for s, o in zip(host_srcs, host_objs)]
     jobs.append(["g++", *CXX_FLAGS, "-fvisibility=hidden", "-I", pybind11.get_include(), "-I", py_inc,
@@ -64,7 +65,7 @@ def build(verbose=False):
                 print(out)
 
     libgfem = os.path.join(OUT, "libgfem.so")
-    run([HIPCC, *HIP_FLAGS, "-shared", "-o", libgfem, *dev_objs])
+    run([HIPCC, *HIP_FLAGS, "-shared", "-o", libgfem, *dev_objs, "-L/opt/rocm/lib", "-lhiprtc"])
     libhost = os.path.join(OUT, "libgfem_host.so")
     run(["g++", "-shared", "-o", libhost, *host_objs, "-L", OUT, "-lgfem", "-Wl,-rpath,$ORIGIN"])
     run(["g++", "-shared", "-o", os.path.join(OUT, "_gfem_host" + ext), py_obj,

@@ -4,6 +4,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -459,6 +461,14 @@ gfem_status gfem_system_finalize(gfem_system *sys)
       if(t.kind == 0 && d.per_point) g.has_dense = true;
       g.dterms.push_back(d);
     }
+    // compile the per-point linear-term coefficient programs once
+    g.jit.assign(g.dterms.size(), nullptr);
+    for(size_t k = 0; k < g.dterms.size(); ++k)
+      if(g.dterms[k].kind == 1 && g.dterms[k].per_point) {
+        std::string log;
+        g.jit[k] = jit_linear_kernel(ctx->device, g.dterms[k].expr, &log);
+        if(!g.jit[k] && !log.empty()) std::fprintf(stderr, "gfem: coefficient compilation failed, interpreting:\n%s\n", log.c_str());
+      }
     const std::vector<Lat> lat = lattice_nodes(g.dim, g.order);
     std::vector<int> nodes(4 * lat.size());
     for(size_t a = 0; a < lat.size(); ++a)
@@ -535,10 +545,15 @@ gfem_status gfem_system_finalize(gfem_system *sys)
     if(n_elem >= (int64_t(1) << 32)) return fail(ctx, GFEM_ERR_TOO_LARGE, "too many elements");
     for(auto &g : sys->groups)
       if(g.ne >= (int64_t(1) << 27)) return fail(ctx, GFEM_ERR_TOO_LARGE, "more than 2^27 elements in a group");
-    {
+    // one block per chunk unless GFEM_GATHER_BLOCKS_PER_CU asks for a
+    // persistent grid of that many blocks per compute unit
+    sys->gather_blocks = int64_t(1) << 40;
+    if(const char *e = std::getenv("GFEM_GATHER_BLOCKS_PER_CU")) {
       int cus = 0;
-      if(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device) == hipSuccess && cus > 0)
-        sys->gather_blocks = (int64_t)cus * 8;
+      const int per = std::atoi(e);
+      if(per > 0 && hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device) == hipSuccess &&
+         cus > 0)
+        sys->gather_blocks = (int64_t)cus * per;
     }
     std::vector<uint32_t> c_elem((size_t)total);
     std::vector<uint16_t> c_pair((size_t)total);

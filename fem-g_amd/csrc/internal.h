@@ -105,6 +105,7 @@ struct Group {
   std::vector<std::vector<double>> term_consts;
   std::vector<Rule> rules;
   std::vector<DevTerm> dterms;
+  std::vector<hipFunction_t> jit;  // compiled linear-term kernel per dterm (or null)
   bool has_bilinear = false, has_linear = false, has_dense = false;
   int32_t *d_vtx = nullptr;
   int *d_nodes = nullptr;  // reference lattice multi-indices [N][4]
@@ -184,5 +185,9 @@ hipError_t launch_gather(const gfem_system &sys, int prec, hipStream_t s);
 hipError_t launch_rhs(const gfem_system &sys, int prec, hipStream_t s);
 hipError_t launch_ref_scatter(const gfem_system &sys, const Group &g, hipStream_t s);
 hipError_t launch_ref_rhs(const gfem_system &sys, const Group &g, hipStream_t s);
+
+// Compiled point-parallel linear-term kernel for a coefficient program
+// (jit.cpp); null when run-time compilation is off or fails.
+hipFunction_t jit_linear_kernel(int device, const DevExpr &ex, std::string *log);
 
 }  // namespace gfem

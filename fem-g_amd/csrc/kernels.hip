@@ -1035,6 +1035,35 @@ hipError_t launch_linear(const Group &g, const double *xyz, int nrhs, hipStream_
       a.nv = g.nv;
       a.accumulate = first ? 0 : 1;
       const size_t lds_pp = sizeof(double) * (size_t)R.Q * g.N + sizeof(double2) * (size_t)(256 / std::max(R.Q, 1)) * R.Q;
+      const size_t ti = (size_t)(&t - g.dterms.data());
+      hipFunction_t jf = ti < g.jit.size() ? g.jit[ti] : nullptr;
+      if(jf && R.Q <= 256 && lds_pp <= 64 * 1024) {
+        // compiled coefficient program (jit.cpp); argument layout of JitLinArgs
+        struct {
+          double consts[2 * GFEM_MAX_CONSTS];
+          double fac_re, fac_im;
+          int Q, dim, nv, N, accumulate;
+        } ja{};
+        for(int k = 0; k < 2 * GFEM_MAX_CONSTS; ++k) ja.consts[k] = t.expr.consts[k];
+        ja.fac_re = t.fac_re;
+        ja.fac_im = t.fac_im;
+        ja.Q = R.Q;
+        ja.dim = g.dim;
+        ja.nv = g.nv;
+        ja.N = g.N;
+        ja.accumulate = first ? 0 : 1;
+        const int EB = 256 / R.Q;
+        int64_t ne = g.ne;
+        const double *pw = R.d_w, *pp = R.d_pts, *pb = R.d_bf, *pF = g.d_F64;
+        const int32_t *pv = g.d_vtx;
+        double2 *pbe = be;
+        void *params[] = {&ja, &pw, &pp, &pb, &ne, &pv, &xyz, &pF, &pbe};
+        const hipError_t e = hipModuleLaunchKernel(jf, (unsigned)((g.ne + EB - 1) / EB), 1, 1, 256, 1, 1,
+                                                   (unsigned)lds_pp, s, params, nullptr);
+        if(e != hipSuccess) return e;
+        first = false;
+        continue;
+      }
       if(R.Q <= 256 && lds_pp <= 64 * 1024) {
         const int EB = 256 / R.Q;
         const size_t lds = lds_pp;

@@ -142,3 +142,20 @@ def test_symmetric_storage():
     F.assemble()
     assert_close(F.values(), P.values(), FP64_RTOL, "A")
     assert_close(F.rhs(0), P.rhs(0), FP64_RTOL, "b")
+
+
+def test_compiled_coefficients_match_interpreter(monkeypatch):
+    """Run-time compiled coefficient programs (hipRTC) and the interpreter
+    give the same load vector; both match the oracle."""
+    from gfem import workloads as W
+    print("case compiled vs interpreted coefficient programs", flush=True)
+    spec = W.helmholtz3d(n=4, order=2)
+    F1, P = _pair(spec)
+    F1.assemble()
+    monkeypatch.setenv("GFEM_JIT", "0")
+    F2 = W.build(spec)
+    F2.pre()
+    F2.assemble()
+    assert_close(F1.rhs(0), P.rhs(0), FP64_RTOL, "compiled b")
+    assert_close(F2.rhs(0), P.rhs(0), FP64_RTOL, "interpreted b")
+    assert_close(F1.rhs(0), F2.rhs(0), 1e-14, "compiled vs interpreted")
