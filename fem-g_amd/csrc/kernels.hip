@@ -280,6 +280,20 @@ __device__ __forceinline__ void phys_point(int dim, const double X[4][3], const 
   }
 }
 
+// Work-groups are dispatched round-robin over the XCDs, each with its own
+// L2.  remap() sends block b to work item (b mod X) * per + b / X, so each XCD
+// walks a contiguous range of items: neighbouring gather chunks and RHS rows
+// share elements, which then hit in that XCD's L2.
+constexpr int kXcds = 8;
+
+__device__ __forceinline__ int64_t xcd_remap(int64_t b, int64_t nb)
+{
+  const int64_t per = nb / kXcds;  // nb is a multiple of kXcds
+  return (b % kXcds) * per + b / kXcds;
+}
+
+inline int64_t xcd_grid(int64_t n) { return (n + kXcds - 1) / kXcds * kXcds; }
+
 // ------------------------------------------------------------------ k_basis
 // Lagrange basis on the gmsh reference simplex, lattice node a with
 // barycentric multi-index m: phi = prod_k prod_{l<m_k} (p lam_k - l)/(l+1).
@@ -739,7 +753,9 @@ __global__ void __launch_bounds__(kGatherThreads)
     s_gD[t] = gt.dense[t];
     s_gN[t] = gt.N[t];
   }
-  int64_t c = blockIdx.x;
+  // one chunk per block (XCD-remapped), or a persistent walk over chunks
+  const bool persistent = (int64_t)gridDim.x < nchunks;
+  int64_t c = persistent ? (int64_t)blockIdx.x : xcd_remap(blockIdx.x, gridDim.x);
   uint32_t cs = 0, len = 0, ns = 0, nel = 0, w0 = 0, w1 = 0, ev = 0;
   if(c < nchunks) {
     cs = chunk_c[c];
@@ -752,7 +768,7 @@ __global__ void __launch_bounds__(kGatherThreads)
     ev = (uint32_t)t < nel ? elist[es + t] : 0u;
   }
   __syncthreads();
-  for(; c < nchunks; c += gridDim.x) {
+  for(; c < nchunks; c = persistent ? c + gridDim.x : nchunks) {
     // element factors of this chunk
     if((uint32_t)t < nel) {
       const int g = (int)(ev >> 27);
@@ -779,7 +795,7 @@ __global__ void __launch_bounds__(kGatherThreads)
     }
     const uint32_t clen = len, cns = ns, cw0 = w0, cw1 = w1;
     // next chunk in flight while this one is processed
-    const int64_t cn = c + gridDim.x;
+    const int64_t cn = persistent ? c + gridDim.x : nchunks;
     if(cn < nchunks) {
       cs = chunk_c[cn];
       len = chunk_c[cn + 1] - cs;
@@ -847,7 +863,7 @@ __global__ void __launch_bounds__(256) k_rhs(const GroupTable *__restrict__ gtp,
                                              const uint32_t *__restrict__ lin_words, void *__restrict__ out)
 {
   const GroupTable &gt = *gtp;
-  const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t row = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if(row >= n) return;
   double2 lift = make_double2(0., 0.);
   for(int64_t k = dc_ptr[row]; k < dc_ptr[row + 1]; ++k) {
@@ -1107,7 +1123,8 @@ hipError_t launch_gather(const gfem_system &sys, int prec, hipStream_t s)
   if(sys.n_chunks == 0) return hipSuccess;
   void *out = sys.d_vals[prec];
   const GroupTable *gt = sys.d_table;
-  const dim3 grid((unsigned)std::min<int64_t>(sys.n_chunks, sys.gather_blocks)), block(kGatherThreads);
+  const int64_t nb = sys.gather_blocks < sys.n_chunks ? sys.gather_blocks : xcd_grid(sys.n_chunks);
+  const dim3 grid((unsigned)nb), block(kGatherThreads);
 #define GFEM_GATHER_ARGS gt, sys.d_words, sys.d_chunk_c, sys.d_chunk_n, sys.d_chunk_e, sys.d_elist, sys.n_chunks, out
   if(sys.cplx) {
     if(prec)
@@ -1130,7 +1147,7 @@ hipError_t launch_rhs(const gfem_system &sys, int prec, hipStream_t s)
   if(sys.n == 0) return hipSuccess;
   void *out = sys.d_rhs[prec];
   const GroupTable *gt = sys.d_table;
-  const dim3 grid(blocks_for(sys.n, 256)), block(256);
+  const dim3 grid((unsigned)xcd_grid(blocks_for(sys.n, 256))), block(256);
 #define GFEM_RHS_ARGS gt, sys.n, sys.nrhs, sys.d_dc_ptr, sys.d_dc_words, sys.d_dc_fix, sys.d_dc, sys.d_lin_ptr, sys.d_lin_words, out
   if(sys.cplx) {
     if(prec)
