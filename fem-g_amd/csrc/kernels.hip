@@ -664,7 +664,7 @@ constexpr int kGatherThreads = 256;
 
 template <bool CPLX, bool MIXED>
 __device__ __forceinline__ void gather_eval(uint32_t w, int logP, uint32_t pmask,
-                                            const typename std::conditional<MIXED, float, double>::type (*s_F)[8],
+                                            const typename std::conditional<MIXED, float, double>::type (*s_F)[kChunkElems],
                                             const void *const *s_T, const uint8_t *s_treal,
                                             const double2 *const *s_D,
                                             typename std::conditional<MIXED, float, double>::type *s_re,
@@ -675,7 +675,10 @@ __device__ __forceinline__ void gather_eval(uint32_t w, int logP, uint32_t pmask
   const int pair = (int)(w & pmask);
   const int le = (int)((w >> logP) & 0xFFu);
   const int slot = (int)((w >> (logP + 8)) & 0x3FFu);
-  const R *F = s_F[le];
+  R F[8];
+#pragma unroll
+  for(int q = 0; q < 7; ++q) F[q] = s_F[q][le];  // structure-of-arrays: no bank conflicts across elements
+  F[7] = 0;
   R vr = 0, vi = 0;
   if(!CPLX || s_treal[le]) {
     if(MIXED) {
@@ -732,7 +735,7 @@ __global__ void __launch_bounds__(kGatherThreads)
   __shared__ const double2 *s_gD[kMaxGroups];
   __shared__ int s_gN[kMaxGroups];
   __shared__ uint8_t s_gR[kMaxGroups];
-  __shared__ R s_F[kChunkElems][8];
+  __shared__ R s_F[8][kChunkElems];
   __shared__ const void *s_T[kChunkElems];
   __shared__ const double2 *s_D[kChunkElems];
   __shared__ uint8_t s_treal[kChunkElems];
@@ -780,16 +783,16 @@ __global__ void __launch_bounds__(kGatherThreads)
       if(MIXED) {
         const float4 *F = reinterpret_cast<const float4 *>(reinterpret_cast<const float *>(s_gF[g]) + el * 8);
         const float4 a = F[0], b = F[1];
-        s_F[t][0] = a.x; s_F[t][1] = a.y; s_F[t][2] = a.z; s_F[t][3] = a.w;
-        s_F[t][4] = b.x; s_F[t][5] = b.y; s_F[t][6] = b.z; s_F[t][7] = b.w;
+        s_F[0][t] = a.x; s_F[1][t] = a.y; s_F[2][t] = a.z; s_F[3][t] = a.w;
+        s_F[4][t] = b.x; s_F[5][t] = b.y; s_F[6][t] = b.z; s_F[7][t] = b.w;
       }
       else {
         const double2 *F = reinterpret_cast<const double2 *>(reinterpret_cast<const double *>(s_gF[g]) + el * 8);
 #pragma unroll
         for(int q = 0; q < 4; ++q) {
           const double2 f = F[q];
-          s_F[t][2 * q] = (R)f.x;
-          s_F[t][2 * q + 1] = (R)f.y;
+          s_F[2 * q][t] = (R)f.x;
+          s_F[2 * q + 1][t] = (R)f.y;
         }
       }
     }
