@@ -262,11 +262,7 @@ gfem_status gfem_system_destroy(gfem_system *sys)
   for(auto &g : sys->groups) free_group(g);
   dfree(sys->d_dc);
   dfree(sys->d_table);
-  dfree(sys->d_words);
-  dfree(sys->d_chunk_c);
-  dfree(sys->d_chunk_n);
-  dfree(sys->d_chunk_e);
-  dfree(sys->d_elist);
+  dfree(sys->d_rec);
   dfree(sys->d_lin_words);
   dfree(sys->d_dc_words);
   dfree(sys->d_dc_fix);
@@ -326,9 +322,38 @@ gfem_status gfem_group_add(gfem_system *sys, int dim, int type, int64_t ne, cons
       }
     }
   }
-  g.h_num.assign(dof_num, dof_num + ne * N);
-  g.h_type.assign(dof_type, dof_type + ne * N);
-  if(elem2nnz) g.h_e2n.assign(elem2nnz, elem2nnz + ne * N * N);
+  // Device storage order: elements sorted (stably) by their smallest row, so
+  // that elements sharing rows sit together in HBM and the row-ordered gather
+  // and RHS kernels read them with cache reuse.  Every sum keeps the
+  // traversal order of the reference loop: the maps are built by walking the
+  // elements in their given order through h_iperm.
+  std::vector<int64_t> key((size_t)ne), perm((size_t)ne);
+  for(int64_t e = 0; e < ne; ++e) {
+    int64_t k = INT64_MAX;
+    for(int i = 0; i < N; ++i)
+      if(is_row_type(dof_type[e * N + i])) k = std::min<int64_t>(k, dof_num[e * N + i]);
+    key[e] = k;
+    perm[e] = e;
+  }
+  std::stable_sort(perm.begin(), perm.end(), [&](int64_t a, int64_t b) { return key[a] < key[b]; });
+  const int64_t NN = (int64_t)N * N;
+  std::vector<int32_t> vtx_given(std::move(g.h_vtx));
+  g.h_vtx.resize(vtx_given.size());
+  g.h_num.resize((size_t)(ne * N));
+  g.h_type.resize((size_t)(ne * N));
+  if(elem2nnz) g.h_e2n.resize((size_t)(ne * NN));
+  g.h_iperm.resize((size_t)ne);
+  for(int64_t s = 0; s < ne; ++s) {
+    const int64_t e = perm[s];
+    g.h_iperm[e] = s;
+    for(int v = 0; v < g.nv; ++v) g.h_vtx[s * g.nv + v] = vtx_given[e * g.nv + v];
+    for(int i = 0; i < N; ++i) {
+      g.h_num[s * N + i] = dof_num[e * N + i];
+      g.h_type[s * N + i] = dof_type[e * N + i];
+    }
+    if(elem2nnz)
+      for(int64_t p = 0; p < NN; ++p) g.h_e2n[s * NN + p] = elem2nnz[e * NN + p];
+  }
   sys->groups.push_back(std::move(g));
   if(group_out) *group_out = (int)sys->groups.size() - 1;
   return GFEM_OK;
@@ -517,10 +542,10 @@ gfem_status gfem_system_finalize(gfem_system *sys)
 
   // Element -> nonzero map, transposed: the contributions of every nonzero in
   // traversal order (group, element, local pair), cut into gather chunks of
-  // whole nonzeros with <= kChunk contributions and <= kChunkElems distinct
+  // whole nonzeros with <= kChunkWords contributions and <= kChunkElems distinct
   // elements; inside a chunk the words are grouped by element.
   {
-    if(logP > 13) return fail(ctx, GFEM_ERR_UNSUPPORTED, "more than 90 dofs by element in the gather words");
+    if(logP > kMaxLogP) return fail(ctx, GFEM_ERR_UNSUPPORTED, "more than 45 dofs by element in the gather words");
     std::vector<int64_t> cnt(sys->nnz + 1, 0);
     int64_t total = 0;
     for(auto &g : sys->groups) {
@@ -545,16 +570,6 @@ gfem_status gfem_system_finalize(gfem_system *sys)
     if(n_elem >= (int64_t(1) << 32)) return fail(ctx, GFEM_ERR_TOO_LARGE, "too many elements");
     for(auto &g : sys->groups)
       if(g.ne >= (int64_t(1) << 27)) return fail(ctx, GFEM_ERR_TOO_LARGE, "more than 2^27 elements in a group");
-    // one block per chunk unless GFEM_GATHER_BLOCKS_PER_CU asks for a
-    // persistent grid of that many blocks per compute unit
-    sys->gather_blocks = int64_t(1) << 40;
-    if(const char *e = std::getenv("GFEM_GATHER_BLOCKS_PER_CU")) {
-      int cus = 0;
-      const int per = std::atoi(e);
-      if(per > 0 && hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device) == hipSuccess &&
-         cus > 0)
-        sys->gather_blocks = (int64_t)cus * per;
-    }
     std::vector<uint32_t> c_elem((size_t)total);
     std::vector<uint16_t> c_pair((size_t)total);
     {
@@ -562,7 +577,8 @@ gfem_status gfem_system_finalize(gfem_system *sys)
       for(auto &g : sys->groups) {
         if(!g.has_bilinear) continue;
         const int NN = g.N * g.N;
-        for(int64_t e = 0; e < g.ne; ++e)
+        for(int64_t eo = 0; eo < g.ne; ++eo) {
+          const int64_t e = g.h_iperm[eo];  // traversal order, storage index
           for(int p = 0; p < NN; ++p) {
             const int64_t v = g.h_e2n[e * NN + p];
             if(v < 0) continue;
@@ -570,24 +586,24 @@ gfem_status gfem_system_finalize(gfem_system *sys)
             c_elem[at] = (uint32_t)(g.e_off + e);
             c_pair[at] = (uint16_t)p;
           }
+        }
       }
     }
-    std::vector<uint32_t> words((size_t)total), cc, cn, ce, elist;
+    std::vector<uint32_t> rec, elist;
     std::vector<int32_t> mark((size_t)std::max<int64_t>(n_elem, 1), -1), local((size_t)std::max<int64_t>(n_elem, 1), 0);
     std::vector<std::pair<uint32_t, uint32_t>> tmp;
     int64_t nz = 0;
     int32_t chunk = 0;
     while(nz < sys->nnz) {
       const int64_t c0 = cnt[nz];
-      cc.push_back((uint32_t)c0);
-      cn.push_back((uint32_t)nz);
-      ce.push_back((uint32_t)elist.size());
+      const int64_t nz0 = nz;
+      elist.clear();
       int nel = 0;
-      while(nz < sys->nnz && cnt[nz + 1] - c0 <= kChunk) {
+      while(nz < sys->nnz && cnt[nz + 1] - c0 <= kChunkWords) {
         int fresh = 0;
         for(int64_t k = cnt[nz]; k < cnt[nz + 1]; ++k)
           if(mark[c_elem[k]] != chunk) ++fresh;
-        if(nel + fresh > kChunkElems) break;
+        if(nel + fresh > kChunkElems || kRecHead + (cnt[nz + 1] - c0) + nel + fresh > kRec) break;
         for(int64_t k = cnt[nz]; k < cnt[nz + 1]; ++k)
           if(mark[c_elem[k]] != chunk) {
             mark[c_elem[k]] = chunk;
@@ -608,25 +624,35 @@ gfem_status gfem_system_finalize(gfem_system *sys)
       }
       // head flags per slot
       std::vector<uint8_t> heads((size_t)(c1 - c0), 0);
-      for(int64_t z = cn.back(); z < nz; ++z) heads[cnt[z] - c0] = 1;
+      for(int64_t z = nz0; z < nz; ++z) heads[cnt[z] - c0] = 1;
       std::stable_sort(tmp.begin(), tmp.end(),
                        [](const std::pair<uint32_t, uint32_t> &a, const std::pair<uint32_t, uint32_t> &b) {
                          return a.first < b.first;
                        });
-      for(size_t k = 0; k < tmp.size(); ++k)
-        words[c0 + k] = tmp[k].first | (tmp[k].second << (logP + 8)) | (heads[tmp[k].second] ? 0x80000000u : 0u);
+      // record: header, words from the front, element list at the back
+      const size_t r0 = rec.size();
+      rec.resize(r0 + kRec, 0u);
+      uint32_t *R = &rec[r0];
+      R[0] = (uint32_t)nz0;
+      R[1] = (uint32_t)(c1 - c0);
+      R[2] = (uint32_t)nel;
+      for(size_t k = 0; k < tmp.size(); ++k) {
+        const uint32_t le = tmp[k].first >> logP, pair = tmp[k].first & ((1u << logP) - 1u);
+        const uint32_t grp = elist[le] >> 27;
+        R[kRecHead + k] = pair | (grp << logP) | (le << (logP + 4)) | (tmp[k].second << (logP + 11)) |
+                          (heads[tmp[k].second] ? 0x80000000u : 0u);
+      }
+      for(int j = 0; j < nel; ++j) R[kRec - nel + j] = elist[j];
+      sys->n_elist += nel;
       ++chunk;
     }
-    cc.push_back((uint32_t)total);
-    ce.push_back((uint32_t)elist.size());
-    sys->n_chunks = (int64_t)cn.size();
+    sys->n_chunks = (int64_t)chunk;
     sys->n_words = total;
-    sys->n_elist = (int64_t)elist.size();
-    GFEM_HIP(ctx, dupload(&sys->d_words, words.data(), words.size()));
-    GFEM_HIP(ctx, dupload(&sys->d_chunk_c, cc.data(), cc.size()));
-    GFEM_HIP(ctx, dupload(&sys->d_chunk_n, cn.data(), cn.size()));
-    GFEM_HIP(ctx, dupload(&sys->d_chunk_e, ce.data(), ce.size()));
-    GFEM_HIP(ctx, dupload(&sys->d_elist, elist.data(), elist.size()));
+    // gather variant: one block per chunk (default) or a persistent grid that
+    // prefetches records two chunks ahead (GFEM_GATHER=pipe)
+    const char *gm = std::getenv("GFEM_GATHER");
+    sys->gather_pipe = (gm && std::string(gm) == "pipe") ? 1 : 0;
+    GFEM_HIP(ctx, dupload(&sys->d_rec, rec.data(), rec.size()));
   }
 
   // RHS maps: linear contributions (row <- element key) and Dirichlet lifting
@@ -638,7 +664,8 @@ gfem_status gfem_system_finalize(gfem_system *sys)
     std::vector<int64_t> dc_fix_key;
     for(auto &g : sys->groups) {
       const int N = g.N;
-      for(int64_t e = 0; e < g.ne; ++e) {
+      for(int64_t eo = 0; eo < g.ne; ++eo) {
+        const int64_t e = g.h_iperm[eo];  // traversal order, storage index
         const int64_t eg = g.e_off + e;
         for(int i = 0; i < N; ++i) {
           const int32_t ti = g.h_type[e * N + i];
@@ -865,8 +892,8 @@ gfem_status gfem_gather_bytes(const gfem_system *sys, gfem_precision prec, doubl
 {
   if(!sys || !bytes) return GFEM_ERR_INVALID_ARG;
   const int p = prec == GFEM_PREC_MIXED ? 1 : 0;
-  // contribution words + chunk headers (3 x uint32) + chunk element lists
-  double b = 4.0 * (double)sys->n_words + 12.0 * (double)(sys->n_chunks + 1) + 4.0 * (double)sys->n_elist;
+  // chunk records (header, contribution words, element list; padding not counted)
+  double b = 4.0 * ((double)kRecHead * (double)sys->n_chunks + (double)sys->n_words + (double)sys->n_elist);
   int64_t ne = 0;
   for(auto &g : sys->groups)
     if(g.has_bilinear) ne += g.ne;

@@ -7,15 +7,21 @@
 //   reference tensor        T[N*N][8] complex, pair = i_tf*N + j_dof, the
 //                           coefficient-weighted sum over the group's
 //                           constant-coefficient bilinear terms
-//   gather chunks           runs of whole nonzeros, <= kChunk contributions
-//                           and <= kChunkElems distinct elements each, with
-//                           the chunk's element list, entries (group << 27) |
-//                           element index within the group
+//   gather chunks           runs of whole nonzeros, <= kChunkWords
+//                           contributions and <= kChunkElems distinct
+//                           elements each, one fixed-stride record of kRec
+//                           uint32 per chunk:
+//                             [0] first nonzero, [1] len (words), [2] nel,
+//                             [kRecHead, kRecHead+len)  contribution words,
+//                             [kRec-nel, kRec)         element list, entries
+//                             (group << 27) | element index within the group
+//                           so one block fetches its whole chunk at once
 //   contribution words      uint32, grouped by chunk element then pair:
 //                           bits [0,logP) local pair i_tf*N + j_dof,
-//                           [logP,logP+8) chunk element, [logP+8,logP+18)
-//                           slot = position in the chunk's nonzero order,
-//                           bit 31 = slot is the first term of its nonzero
+//                           [logP,logP+4) group, [logP+4,logP+11) chunk
+//                           element, [logP+11,logP+20) slot = position in the
+//                           chunk's nonzero order, bit 31 = slot is the first
+//                           term of its nonzero (logP <= 11, N <= 45)
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -30,8 +36,14 @@ namespace gfem {
 
 constexpr int kMaxGroups = 16;
 constexpr int kMaxN = 64;
-constexpr int kChunk = 512;       // contributions per gather block (whole nonzeros)
-constexpr int kChunkElems = 128;  // distinct elements per gather block
+constexpr int kChunk = 512;       // LDS slots per gather block
+constexpr int kChunkElems = 128;  // distinct elements per gather block (7 bits)
+constexpr int kRec = 576;         // uint32 per chunk record (2304 B)
+constexpr int kRecHead = 4;       // header words of a record
+constexpr int kRecWordEnd = 512;  // words sit in record slots [kRecHead, kRecWordEnd)
+constexpr int kChunkWords = kRecWordEnd - kRecHead;  // contributions per chunk (slot: 9 bits)
+constexpr int kMaxLogP = 11;      // pair bits in a contribution word
+static_assert(kChunkWords <= kChunk && kChunk <= 512 && kRec - kRecWordEnd <= 256, "record layout");
 
 // Coefficient program as read by the kernels.
 struct DevExpr {
@@ -100,6 +112,7 @@ struct Group {
   std::vector<int64_t> h_num;
   std::vector<int32_t> h_type;
   std::vector<int64_t> h_e2n;
+  std::vector<int64_t> h_iperm;  // given element index -> device storage index
   std::vector<gfem_term_desc> terms;
   std::vector<std::vector<int32_t>> term_ops;
   std::vector<std::vector<double>> term_consts;
@@ -152,12 +165,12 @@ struct gfem_system {
   double2 *d_dc = nullptr;
   gfem::GroupTable *d_table = nullptr;
   // fused maps
-  uint32_t *d_words = nullptr;
+  uint32_t *d_rec = nullptr;  // chunk records, n_chunks x kRec
+  int gather_pipe = 0;        // 1: persistent record-prefetching gather (GFEM_GATHER=pipe)
+  int64_t gather_grid = 0;    // persistent grid (multiple of the XCD count)
   int64_t n_words = 0;
-  uint32_t *d_chunk_c = nullptr, *d_chunk_n = nullptr, *d_chunk_e = nullptr, *d_elist = nullptr;
   int64_t n_elist = 0;
   int64_t n_chunks = 0;
-  int64_t gather_blocks = 2048;  // persistent gather grid (CUs x resident blocks)
   uint32_t *d_lin_words = nullptr, *d_dc_words = nullptr;
   int32_t *d_dc_fix = nullptr;
   int64_t *d_lin_ptr = nullptr, *d_dc_ptr = nullptr;

@@ -55,12 +55,14 @@ __device__ __forceinline__ double2 cpow_(double2 a, double2 b) {
 struct JitLinArgs {
   double consts[2 * 32];
   double fac_re, fac_im;
-  int Q, dim, nv, N, accumulate;
+  int Q, dim, nv, N, accumulate, EB;
 };
 )";
 
-// one thread per (element, Gauss point) evaluates c_q = w_q det fac f(x_q)
-// into LDS, then one thread per (element, local dof) reduces over q
+// A block takes EB consecutive elements: their vertex coordinates and
+// determinants are staged in LDS once, then one thread per (element, Gauss
+// point) evaluates c_q = w_q det fac f(x_q) into LDS, and one thread per
+// (element, local dof) reduces b_e(i) = sum_q phi_i(q) c_q in q order.
 const char *kLinearBody = R"(
 extern "C" __global__ void __launch_bounds__(256)
 gfem_jit_linear(JitLinArgs a, const double *__restrict__ w, const double *__restrict__ pts,
@@ -68,48 +70,58 @@ gfem_jit_linear(JitLinArgs a, const double *__restrict__ w, const double *__rest
                 const double *__restrict__ xyz, const double *__restrict__ F64, double2 *__restrict__ be)
 {
   extern __shared__ double s_dyn[];
-  const int Q = a.Q, N = a.N, EB = 256 / Q;
-  double *s_bf = s_dyn;
-  double2 *s_c = reinterpret_cast<double2 *>(s_dyn + Q * N);
-  for(int k = threadIdx.x; k < Q * N; k += 256) s_bf[k] = bf[k];
+  const int Q = a.Q, N = a.N, EB = a.EB, nv = a.nv;
+  double *s_bf = s_dyn;             // [Q][N]
+  double *s_u = s_bf + Q * N;       // [Q][3] reference points
+  double *s_w = s_u + 3 * Q;        // [Q]
+  double *s_X = s_w + Q;            // [EB][4][3] vertex coordinates
+  double *s_det = s_X + 12 * EB;    // [EB]
+  double2 *s_c = reinterpret_cast<double2 *>(s_det + EB + ((Q * N + 4 * Q + 13 * EB) & 1));  // [EB][Q]
   const int t = threadIdx.x;
-  const int el = t / Q, q = t - el * Q;
-  const i64 e = (i64)blockIdx.x * EB + el;
-  if(el < EB && e < ne) {
-    double X0[3], D[3][3] = {{0., 0., 0.}, {0., 0., 0.}, {0., 0., 0.}};
-    const i64 n0 = vtx[e * a.nv];
-    for(int i = 0; i < 3; ++i) X0[i] = xyz[3 * n0 + i];
-    for(int k = 1; k < a.nv; ++k) {
-      const i64 nk = vtx[e * a.nv + k];
-      for(int i = 0; i < 3; ++i) D[k - 1][i] = xyz[3 * nk + i] - X0[i];
-    }
-    const double *u = &pts[3 * q];
+  const i64 e0 = (i64)blockIdx.x * EB;
+  const int nb = (int)(ne - e0 < (i64)EB ? ne - e0 : (i64)EB);
+  for(int k = t; k < Q * N; k += 256) s_bf[k] = bf[k];
+  for(int k = t; k < 3 * Q; k += 256) s_u[k] = pts[k];
+  for(int k = t; k < Q; k += 256) s_w[k] = w[k];
+  for(int k = t; k < nb * nv * 3; k += 256) {
+    const int le = k / (3 * nv), r = k - le * 3 * nv, v = r / 3, c = r - 3 * v;
+    s_X[12 * le + r] = xyz[3 * (i64)vtx[(e0 + le) * nv + v] + c];
+  }
+  for(int k = t; k < nb; k += 256) s_det[k] = F64[(e0 + k) * 8 + 6];
+  __syncthreads();
+  for(int p = t; p < nb * Q; p += 256) {
+    const int le = p / Q, q = p - le * Q;
+    const double *X = &s_X[12 * le];
+    const double *u = &s_u[3 * q];
     double xp[3];
     if(a.dim == 1) {
       const double s = 0.5 * (u[0] + 1.);
-      for(int i = 0; i < 3; ++i) xp[i] = X0[i] + s * D[0][i];
+      for(int i = 0; i < 3; ++i) xp[i] = X[i] + s * (X[3 + i] - X[i]);
     } else {
-      for(int i = 0; i < 3; ++i) xp[i] = X0[i] + u[0] * D[0][i] + u[1] * D[1][i] + (a.dim == 3 ? u[2] * D[2][i] : 0.);
+      for(int i = 0; i < 3; ++i) {
+        double s = X[i];
+        for(int j = 0; j < a.dim; ++j) s += u[j] * (X[3 * (j + 1) + i] - X[i]);
+        xp[i] = s;
+      }
     }
     const double x = xp[0], y = xp[1], z = xp[2];
     (void)x; (void)y; (void)z;
     @EXPR@
-    const double wd = w[q] * F64[e * 8 + 6];
-    s_c[el * Q + q] = cmul(make_double2(a.fac_re * wd, a.fac_im * wd), f);
+    const double wd = s_w[q] * s_det[le];
+    s_c[p] = cmul(make_double2(a.fac_re * wd, a.fac_im * wd), f);
   }
   __syncthreads();
-  for(int k = t; k < EB * N; k += 256) {
+  for(int k = t; k < nb * N; k += 256) {
     const int le = k / N, i = k - le * N;
-    const i64 ee = (i64)blockIdx.x * EB + le;
-    if(ee >= ne) continue;
-    double2 acc = a.accumulate ? be[ee * N + i] : make_double2(0., 0.);
+    const i64 o = (e0 + le) * N + i;
+    double2 acc = a.accumulate ? be[o] : make_double2(0., 0.);
     const double2 *c = &s_c[le * Q];
-    for(int qq = 0; qq < Q; ++qq) {
-      const double b = s_bf[qq * N + i];
-      acc.x += c[qq].x * b;
-      acc.y += c[qq].y * b;
+    for(int q = 0; q < Q; ++q) {
+      const double b = s_bf[q * N + i];
+      acc.x += c[q].x * b;
+      acc.y += c[q].y * b;
     }
-    be[ee * N + i] = acc;
+    be[o] = acc;
   }
 }
 )";

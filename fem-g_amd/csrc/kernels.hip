@@ -472,50 +472,63 @@ __global__ void __launch_bounds__(128) k_linear(LinTerm a, const double *__restr
   for(int i = 0; i < N; ++i) be[e * N + i] = acc[i];
 }
 
-// Point-parallel form of the same term: one thread per (element, Gauss
-// point) evaluates c_q = w_q det conj(c_tf) f(x_q) into LDS, then one thread
-// per (element, local dof) forms b_e(i) (+)= sum_q phi_i(q) c_q.  Used when
-// the rule has at most 256 points; keeps registers low and every lane busy.
-__global__ void __launch_bounds__(256) k_linear_pp(LinTerm a, int N, const double *__restrict__ w,
+// Point-parallel form of the same term.  A block takes EB consecutive
+// elements: vertex coordinates and determinants are staged in LDS once, one
+// thread per (element, Gauss point) evaluates c_q = w_q det conj(c_tf) f(x_q)
+// into LDS, then one thread per (element, local dof) forms
+// b_e(i) (+)= sum_q phi_i(q) c_q.  Same layout as the compiled kernel.
+__global__ void __launch_bounds__(256) k_linear_pp(LinTerm a, int N, int EB, const double *__restrict__ w,
                                                    const double *__restrict__ pts, const double *__restrict__ bf,
                                                    int64_t ne, const int32_t *__restrict__ vtx,
                                                    const double *__restrict__ xyz, const double *__restrict__ F64,
                                                    double2 *__restrict__ be)
 {
   extern __shared__ double s_dyn[];
-  const int Q = a.Q;
-  const int EB = 256 / Q;  // elements per block
-  double *s_bf = s_dyn;                    // [Q][N]
-  double2 *s_c = reinterpret_cast<double2 *>(s_dyn + Q * N);  // [EB][Q]
-  for(int k = threadIdx.x; k < Q * N; k += 256) s_bf[k] = bf[k];
+  const int Q = a.Q, nv = a.nv;
+  double *s_bf = s_dyn;           // [Q][N]
+  double *s_u = s_bf + Q * N;     // [Q][3]
+  double *s_w = s_u + 3 * Q;      // [Q]
+  double *s_X = s_w + Q;          // [EB][4][3]
+  double *s_det = s_X + 12 * EB;  // [EB]
+  double2 *s_c = reinterpret_cast<double2 *>(s_det + EB + ((Q * N + 4 * Q + 13 * EB) & 1));  // [EB][Q]
   const int t = threadIdx.x;
-  const int el = t / Q, q = t - el * Q;
-  const int64_t e = (int64_t)blockIdx.x * EB + el;
-  if(el < EB && e < ne) {
-    double X[4][3];
-    load_vertices(vtx, a.nv, e, xyz, X);
+  const int64_t e0 = (int64_t)blockIdx.x * EB;
+  const int nb = (int)min((int64_t)EB, ne - e0);
+  for(int k = t; k < Q * N; k += 256) s_bf[k] = bf[k];
+  for(int k = t; k < 3 * Q; k += 256) s_u[k] = pts[k];
+  for(int k = t; k < Q; k += 256) s_w[k] = w[k];
+  for(int k = t; k < nb * nv * 3; k += 256) {
+    const int le = k / (3 * nv), r = k - le * 3 * nv, v = r / 3, c = r - 3 * v;
+    s_X[12 * le + r] = xyz[3 * (int64_t)vtx[(e0 + le) * nv + v] + c];
+  }
+  for(int k = t; k < nb; k += 256) s_det[k] = F64[(e0 + k) * 8 + 6];
+  __syncthreads();
+  for(int p = t; p < nb * Q; p += 256) {
+    const int le = p / Q, q = p - le * Q;
     double2 f = make_double2(a.c_re, a.c_im);
     if(a.per_point) {
+      double X[4][3];
+      for(int v = 0; v < 4; ++v)
+        for(int i = 0; i < 3; ++i) X[v][i] = v < nv ? s_X[12 * le + 3 * v + i] : 0.;
       double x[3];
-      phys_point(a.dim, X, &pts[3 * q], x);
+      phys_point(a.dim, X, &s_u[3 * q], x);
       f = eval_expr(&a.expr, x[0], x[1], x[2]);
     }
-    const double wd = w[q] * F64[e * 8 + 6];
-    s_c[el * Q + q] = cmul(make_double2(a.fac_re * wd, a.fac_im * wd), f);
+    const double wd = s_w[q] * s_det[le];
+    s_c[p] = cmul(make_double2(a.fac_re * wd, a.fac_im * wd), f);
   }
   __syncthreads();
-  for(int k = t; k < EB * N; k += 256) {
+  for(int k = t; k < nb * N; k += 256) {
     const int le = k / N, i = k - le * N;
-    const int64_t ee = (int64_t)blockIdx.x * EB + le;
-    if(ee >= ne) continue;
-    double2 acc = a.accumulate ? be[ee * N + i] : make_double2(0., 0.);
+    const int64_t o = (e0 + le) * N + i;
+    double2 acc = a.accumulate ? be[o] : make_double2(0., 0.);
     const double2 *c = &s_c[le * Q];
-    for(int qq = 0; qq < Q; ++qq) {
-      const double b = s_bf[qq * N + i];
-      acc.x += c[qq].x * b;
-      acc.y += c[qq].y * b;
+    for(int q = 0; q < Q; ++q) {
+      const double b = s_bf[q * N + i];
+      acc.x += c[q].x * b;
+      acc.y += c[q].y * b;
     }
-    be[ee * N + i] = acc;
+    be[o] = acc;
   }
 }
 
@@ -650,10 +663,10 @@ struct ValTraits<double2> {
 };
 
 // ------------------------------------------------------------------ k_gather
-// Atomic-free CSR assembly.  Persistent 256-thread blocks walk the chunks of
-// whole nonzeros (<= kChunk contributions, <= kChunkElems distinct elements
-// each); while a chunk is processed the next chunk's header, words and
-// element list are already in flight.  Per chunk:
+// Atomic-free CSR assembly, one 256-thread block per chunk of whole nonzeros
+// (<= kChunk contributions, <= kChunkElems distinct elements).  The chunk is
+// one fixed-stride record (header, words, element list), so the block issues
+// every load of its map at once, without a dependent header read.  Then:
 //   1. the chunk's element factors F (FP64 geometry) are staged in LDS;
 //   2. every word (grouped by element, then local pair) is evaluated as
 //      F . T[pair] and parked in LDS at its nonzero-order slot;
@@ -661,6 +674,8 @@ struct ValTraits<double2> {
 //      element traversal order of the reference's sequential loop, and writes
 //      the value once.  No atomics, bitwise reproducible.
 constexpr int kGatherThreads = 256;
+static_assert(kRec <= 3 * kGatherThreads && kRec > 2 * kGatherThreads && kRecWordEnd == 2 * kGatherThreads,
+              "three record slots per thread, words only in the first two");
 
 template <bool CPLX, bool MIXED>
 __device__ __forceinline__ void gather_eval(uint32_t w, int logP, uint32_t pmask,
@@ -673,8 +688,8 @@ __device__ __forceinline__ void gather_eval(uint32_t w, int logP, uint32_t pmask
 {
   using R = typename std::conditional<MIXED, float, double>::type;
   const int pair = (int)(w & pmask);
-  const int le = (int)((w >> logP) & 0xFFu);
-  const int slot = (int)((w >> (logP + 8)) & 0x3FFu);
+  const int le = (int)((w >> (logP + 4)) & 0x7Fu);
+  const int slot = (int)((w >> (logP + 11)) & 0x1FFu);
   R F[8];
 #pragma unroll
   for(int q = 0; q < 7; ++q) F[q] = s_F[q][le];  // structure-of-arrays: no bank conflicts across elements
@@ -724,12 +739,11 @@ __device__ __forceinline__ void gather_eval(uint32_t w, int logP, uint32_t pmask
 
 template <bool CPLX, bool MIXED>
 __global__ void __launch_bounds__(kGatherThreads)
-  k_gather(const GroupTable *__restrict__ gtp, const uint32_t *__restrict__ words, const uint32_t *__restrict__ chunk_c,
-           const uint32_t *__restrict__ chunk_n, const uint32_t *__restrict__ chunk_e,
-           const uint32_t *__restrict__ elist, int64_t nchunks, void *__restrict__ out)
+  k_gather(const GroupTable *__restrict__ gtp, const uint32_t *__restrict__ recs, int64_t nchunks,
+           void *__restrict__ out)
 {
   using R = typename std::conditional<MIXED, float, double>::type;
-  static_assert(kChunk == 2 * kGatherThreads, "two contributions per thread");
+  static_assert(kChunk == 2 * kGatherThreads, "two slots per thread in the run sums");
   __shared__ const void *s_gF[kMaxGroups];
   __shared__ const void *s_gT[kMaxGroups];
   __shared__ const double2 *s_gD[kMaxGroups];
@@ -745,6 +759,13 @@ __global__ void __launch_bounds__(kGatherThreads)
   __shared__ int s_wsum[kGatherThreads / 64];
   const GroupTable &gt = *gtp;
   const int t = threadIdx.x;
+  const int64_t c = xcd_remap(blockIdx.x, gridDim.x);
+  if(c >= nchunks) return;  // uniform per block
+  // the whole record in one round of loads: slots t, t + 256, t + 512
+  const uint32_t *rec = recs + c * kRec;
+  const uint32_t ns = rec[0], clen = rec[1], nel = rec[2];
+  const uint32_t r0 = rec[t], r1 = rec[t + kGatherThreads];
+  const uint32_t r2 = t < kRec - 2 * kGatherThreads ? rec[t + 2 * kGatherThreads] : 0u;
   const int logP = gt.logP;
   const uint32_t pmask = (1u << logP) - 1u;
   if(t < gt.ng) {
@@ -756,70 +777,237 @@ __global__ void __launch_bounds__(kGatherThreads)
     s_gD[t] = gt.dense[t];
     s_gN[t] = gt.N[t];
   }
-  // one chunk per block (XCD-remapped), or a persistent walk over chunks
-  const bool persistent = (int64_t)gridDim.x < nchunks;
-  int64_t c = persistent ? (int64_t)blockIdx.x : xcd_remap(blockIdx.x, gridDim.x);
-  uint32_t cs = 0, len = 0, ns = 0, nel = 0, w0 = 0, w1 = 0, ev = 0;
-  if(c < nchunks) {
-    cs = chunk_c[c];
-    len = chunk_c[c + 1] - cs;
-    ns = chunk_n[c];
-    const uint32_t es = chunk_e[c];
-    nel = chunk_e[c + 1] - es;
-    w0 = (uint32_t)t < len ? words[cs + t] : 0u;
-    w1 = (uint32_t)(t + kGatherThreads) < len ? words[cs + t + kGatherThreads] : 0u;
-    ev = (uint32_t)t < nel ? elist[es + t] : 0u;
+  __syncthreads();
+  // element factors: the element list sits in record slots [kRec - nel, kRec)
+  const uint32_t e_lo = (uint32_t)kRec - nel;
+  auto stage = [&](uint32_t ev, int j) {
+    const int g = (int)(ev >> 27);
+    const int64_t el = ev & 0x7FFFFFFu;
+    s_treal[j] = s_gR[g];
+    s_T[j] = s_gT[g];
+    const int N = s_gN[g];
+    s_D[j] = s_gD[g] ? s_gD[g] + el * N * N : nullptr;
+    if(MIXED) {
+      const float4 *F = reinterpret_cast<const float4 *>(reinterpret_cast<const float *>(s_gF[g]) + el * 8);
+      const float4 a = F[0], b = F[1];
+      s_F[0][j] = a.x; s_F[1][j] = a.y; s_F[2][j] = a.z; s_F[3][j] = a.w;
+      s_F[4][j] = b.x; s_F[5][j] = b.y; s_F[6][j] = b.z; s_F[7][j] = b.w;
+    }
+    else {
+      const double2 *F = reinterpret_cast<const double2 *>(reinterpret_cast<const double *>(s_gF[g]) + el * 8);
+#pragma unroll
+      for(int q = 0; q < 4; ++q) {
+        const double2 f = F[q];
+        s_F[2 * q][j] = (R)f.x;
+        s_F[2 * q + 1][j] = (R)f.y;
+      }
+    }
+  };
+  // slot t never holds the list: nel <= kChunkElems < kRec - kGatherThreads
+  if((uint32_t)(t + kGatherThreads) >= e_lo) stage(r1, t + kGatherThreads - (int)e_lo);
+  if(t < kRec - 2 * kGatherThreads && (uint32_t)(t + 2 * kGatherThreads) >= e_lo)
+    stage(r2, t + 2 * kGatherThreads - (int)e_lo);
+  __syncthreads();
+  // words sit in record slots [kRecHead, kRecHead + clen), clen <= kChunkWords
+  const uint32_t w_hi = (uint32_t)kRecHead + clen;
+  if((uint32_t)t >= (uint32_t)kRecHead && (uint32_t)t < w_hi)
+    gather_eval<CPLX, MIXED>(r0, logP, pmask, s_F, s_T, s_treal, s_D, s_re, s_im, s_head);
+  if((uint32_t)(t + kGatherThreads) < w_hi)
+    gather_eval<CPLX, MIXED>(r1, logP, pmask, s_F, s_T, s_treal, s_D, s_re, s_im, s_head);
+  __syncthreads();
+
+  // thread t owns slots 2t, 2t+1: rank of its first head = exclusive scan
+  // of the head counts
+  const int s0 = 2 * t;
+  const int h0 = (s0 < (int)clen) ? s_head[s0] : 0, h1 = (s0 + 1 < (int)clen) ? s_head[s0 + 1] : 0;
+  const int cnt = h0 + h1;
+  const int lane = t & 63, wv = t >> 6;
+  int incl = cnt;
+#pragma unroll
+  for(int d = 1; d < 64; d <<= 1) {
+    const int v = __shfl_up(incl, d, 64);
+    if(lane >= d) incl += v;
+  }
+  if(lane == 63) s_wsum[wv] = incl;
+  __syncthreads();
+  int rank = incl - cnt;
+  for(int k = 0; k < wv; ++k) rank += s_wsum[k];
+#pragma unroll
+  for(int q = 0; q < 2; ++q) {
+    const int sl = s0 + q;
+    if(!(q == 0 ? h0 : h1)) continue;
+    R sre = s_re[sl], sim = CPLX ? s_im[sl] : (R)0;
+    for(int k = sl + 1; k < (int)clen && !s_head[k]; ++k) {
+      sre += s_re[k];
+      if(CPLX) sim += s_im[k];
+    }
+    const int64_t pos = (int64_t)ns + rank++;
+    if(CPLX) {
+      if(MIXED)
+        reinterpret_cast<float2 *>(out)[pos] = make_float2((float)sre, (float)sim);
+      else
+        reinterpret_cast<double2 *>(out)[pos] = make_double2((double)sre, (double)sim);
+    }
+    else {
+      reinterpret_cast<R *>(out)[pos] = sre;
+    }
+  }
+}
+
+// Persistent variant of the same gather.  A grid of resident blocks, the
+// blocks of XCD x walking the chunks [x*per, (x+1)*per) with stride
+// P = grid / XCDs.  Chunk records are prefetched two chunks ahead; the real
+// reference-tensor rows of a block's words (group taken from the word) and
+// the chunk's element factors are requested before the record two chunks
+// ahead, so that record stays in flight through the whole evaluation of the
+// current chunk (vector-memory counters complete in issue order).
+template <bool CPLX, bool MIXED>
+__global__ void __launch_bounds__(kGatherThreads)
+  k_gather_pipe(const GroupTable *__restrict__ gtp, const uint32_t *__restrict__ recs, int64_t nchunks,
+                int64_t per_xcd, void *__restrict__ out)
+{
+  using R = typename std::conditional<MIXED, float, double>::type;
+  using TV = typename std::conditional<MIXED, float4, double2>::type;
+  constexpr int kTV = MIXED ? 2 : 4;  // vectors of one real tensor row (8 values)
+  constexpr int kFV = MIXED ? 2 : 4;  // vectors of one element's factors (8 values)
+  __shared__ const void *s_gF[kMaxGroups];
+  __shared__ const void *s_gTr[kMaxGroups];
+  __shared__ const void *s_gTc[kMaxGroups];
+  __shared__ const double2 *s_gD[kMaxGroups];
+  __shared__ int s_gN[kMaxGroups];
+  __shared__ uint8_t s_gR[kMaxGroups];
+  __shared__ R s_F[8][kChunkElems];
+  __shared__ const double2 *s_D[kChunkElems];
+  __shared__ R s_re[kChunk];
+  __shared__ R s_im[CPLX ? kChunk : 1];
+  __shared__ uint8_t s_head[kChunk];
+  __shared__ int s_wsum[kGatherThreads / 64];
+  const GroupTable &gt = *gtp;
+  const int t = threadIdx.x;
+  const int logP = gt.logP;
+  const uint32_t pmask = (1u << logP) - 1u;
+  if(t < gt.ng) {
+    s_gR[t] = (!CPLX || gt.treal[t]) ? 1 : 0;
+    s_gF[t] = MIXED ? (const void *)gt.F32[t] : (const void *)gt.F64[t];
+    s_gTr[t] = MIXED ? (const void *)gt.T32r[t] : (const void *)gt.T64r[t];
+    s_gTc[t] = MIXED ? (const void *)gt.T32[t] : (const void *)gt.T64[t];
+    s_gD[t] = gt.dense[t];
+    s_gN[t] = gt.N[t];
   }
   __syncthreads();
-  for(; c < nchunks; c = persistent ? c + gridDim.x : nchunks) {
-    // element factors of this chunk
-    if((uint32_t)t < nel) {
+  const int64_t P = gridDim.x / kXcds;
+  const int64_t cb = (int64_t)(blockIdx.x % kXcds) * per_xcd;
+  const int64_t ce = min(nchunks, cb + per_xcd);
+  int64_t c = cb + blockIdx.x / kXcds;
+  uint32_t a0 = 0, a1 = 0, a2 = 0, b0 = 0, b1 = 0, b2 = 0;
+  auto fetch = [&](int64_t cc, uint32_t &x0, uint32_t &x1, uint32_t &x2) {
+    if(cc < ce) {
+      const uint32_t *r = recs + cc * kRec;
+      x0 = r[t];
+      x1 = r[t + kGatherThreads];
+      x2 = t < kRec - 2 * kGatherThreads ? r[t + 2 * kGatherThreads] : 0u;
+    }
+  };
+  fetch(c, a0, a1, a2);
+  fetch(c + P, b0, b1, b2);
+  for(; c < ce; c += P) {
+    const uint32_t *rec = recs + c * kRec;
+    const uint32_t ns = rec[0], clen = rec[1], nel = rec[2];
+    const uint32_t e_lo = (uint32_t)kRec - nel, w_hi = (uint32_t)kRecHead + clen;
+    const bool h0 = (uint32_t)t >= (uint32_t)kRecHead && (uint32_t)t < w_hi;
+    const bool h1 = (uint32_t)(t + kGatherThreads) < w_hi;
+    // (1) real tensor rows of this thread's two words
+    TV ta[kTV], tb[kTV];
+    auto pref_T = [&](uint32_t w, bool has, TV *tv) {
+      const int g = (int)((w >> logP) & 0xFu);
+      if(has && s_gR[g]) {
+        const TV *T = reinterpret_cast<const TV *>(s_gTr[g]) + (int64_t)(w & pmask) * kTV;
+#pragma unroll
+        for(int q = 0; q < kTV; ++q) tv[q] = T[q];
+      }
+    };
+    pref_T(a0, h0, ta);
+    pref_T(a1, h1, tb);
+    // (2) element factors of the list entries held in slots a1 / a2
+    const bool e1 = (uint32_t)(t + kGatherThreads) >= e_lo;
+    const bool e2 = t < kRec - 2 * kGatherThreads && (uint32_t)(t + 2 * kGatherThreads) >= e_lo;
+    TV f1[kFV], f2[kFV];
+    auto load_F = [&](uint32_t ev, bool has, TV *f) {
+      if(has) {
+        const TV *F = reinterpret_cast<const TV *>(reinterpret_cast<const R *>(s_gF[ev >> 27]) +
+                                                   (int64_t)(ev & 0x7FFFFFFu) * 8);
+#pragma unroll
+        for(int q = 0; q < kFV; ++q) f[q] = F[q];
+      }
+    };
+    load_F(a1, e1, f1);
+    load_F(a2, e2, f2);
+    // (3) the record two chunks ahead
+    uint32_t n0 = 0, n1 = 0, n2 = 0;
+    fetch(c + 2 * P, n0, n1, n2);
+    // (4) stage the element factors
+    auto store_F = [&](uint32_t ev, const TV *f, int j) {
       const int g = (int)(ev >> 27);
       const int64_t el = ev & 0x7FFFFFFu;
-      s_treal[t] = s_gR[g];
-      s_T[t] = s_gT[g];
       const int N = s_gN[g];
-      s_D[t] = s_gD[g] ? s_gD[g] + el * N * N : nullptr;
-      if(MIXED) {
-        const float4 *F = reinterpret_cast<const float4 *>(reinterpret_cast<const float *>(s_gF[g]) + el * 8);
-        const float4 a = F[0], b = F[1];
-        s_F[0][t] = a.x; s_F[1][t] = a.y; s_F[2][t] = a.z; s_F[3][t] = a.w;
-        s_F[4][t] = b.x; s_F[5][t] = b.y; s_F[6][t] = b.z; s_F[7][t] = b.w;
+      s_D[j] = s_gD[g] ? s_gD[g] + el * N * N : nullptr;
+      const R *fv = reinterpret_cast<const R *>(f);
+#pragma unroll
+      for(int q = 0; q < 8; ++q) s_F[q][j] = fv[q];
+    };
+    if(e1) store_F(a1, f1, t + kGatherThreads - (int)e_lo);
+    if(e2) store_F(a2, f2, t + 2 * kGatherThreads - (int)e_lo);
+    __syncthreads();
+    // (5) evaluate F . T[pair] into the nonzero-order slots
+    auto eval = [&](uint32_t w, const TV *tv) {
+      const int pair = (int)(w & pmask);
+      const int g = (int)((w >> logP) & 0xFu);
+      const int le = (int)((w >> (logP + 4)) & 0x7Fu);
+      const int slot = (int)((w >> (logP + 11)) & 0x1FFu);
+      R F[8];
+#pragma unroll
+      for(int q = 0; q < 7; ++q) F[q] = s_F[q][le];
+      F[7] = 0;
+      R vr = 0, vi = 0;
+      if(!CPLX || s_gR[g]) {
+        const R *tr = reinterpret_cast<const R *>(tv);
+#pragma unroll
+        for(int q = 0; q < 7; ++q) vr += F[q] * tr[q];
       }
-      else {
-        const double2 *F = reinterpret_cast<const double2 *>(reinterpret_cast<const double *>(s_gF[g]) + el * 8);
+      else if(MIXED) {
+        const float4 *T = reinterpret_cast<const float4 *>(s_gTc[g]) + (int64_t)pair * 4;
 #pragma unroll
         for(int q = 0; q < 4; ++q) {
-          const double2 f = F[q];
-          s_F[2 * q][t] = (R)f.x;
-          s_F[2 * q + 1][t] = (R)f.y;
+          const float4 tq = T[q];
+          vr += F[2 * q] * tq.x + F[2 * q + 1] * tq.z;
+          vi += F[2 * q] * tq.y + F[2 * q + 1] * tq.w;
         }
       }
-    }
-    const uint32_t clen = len, cns = ns, cw0 = w0, cw1 = w1;
-    // next chunk in flight while this one is processed
-    const int64_t cn = persistent ? c + gridDim.x : nchunks;
-    if(cn < nchunks) {
-      cs = chunk_c[cn];
-      len = chunk_c[cn + 1] - cs;
-      ns = chunk_n[cn];
-      const uint32_t es = chunk_e[cn];
-      nel = chunk_e[cn + 1] - es;
-      w0 = (uint32_t)t < len ? words[cs + t] : 0u;
-      w1 = (uint32_t)(t + kGatherThreads) < len ? words[cs + t + kGatherThreads] : 0u;
-      ev = (uint32_t)t < nel ? elist[es + t] : 0u;
-    }
+      else {
+        const double2 *T = reinterpret_cast<const double2 *>(s_gTc[g]) + (int64_t)pair * 8;
+#pragma unroll
+        for(int q = 0; q < 7; ++q) {
+          const double2 tq = T[q];
+          vr += F[q] * (R)tq.x;
+          vi += F[q] * (R)tq.y;
+        }
+      }
+      if(s_D[le]) {
+        const double2 d = s_D[le][pair];
+        vr += (R)d.x;
+        vi += (R)d.y;
+      }
+      s_re[slot] = vr;
+      if(CPLX) s_im[slot] = vi;
+      s_head[slot] = (uint8_t)(w >> 31);
+    };
+    if(h0) eval(a0, ta);
+    if(h1) eval(a1, tb);
     __syncthreads();
-    if((uint32_t)t < clen) gather_eval<CPLX, MIXED>(cw0, logP, pmask, s_F, s_T, s_treal, s_D, s_re, s_im, s_head);
-    if((uint32_t)(t + kGatherThreads) < clen)
-      gather_eval<CPLX, MIXED>(cw1, logP, pmask, s_F, s_T, s_treal, s_D, s_re, s_im, s_head);
-    __syncthreads();
-
-    // thread t owns slots 2t, 2t+1: rank of its first head = exclusive scan
-    // of the head counts
+    // (6) runs of each nonzero summed in slot order, one store per nonzero
     const int s0 = 2 * t;
-    const int h0 = (s0 < (int)clen) ? s_head[s0] : 0, h1 = (s0 + 1 < (int)clen) ? s_head[s0 + 1] : 0;
-    const int cnt = h0 + h1;
+    const int hd0 = (s0 < (int)clen) ? s_head[s0] : 0, hd1 = (s0 + 1 < (int)clen) ? s_head[s0 + 1] : 0;
+    const int cnt = hd0 + hd1;
     const int lane = t & 63, wv = t >> 6;
     int incl = cnt;
 #pragma unroll
@@ -834,13 +1022,13 @@ __global__ void __launch_bounds__(kGatherThreads)
 #pragma unroll
     for(int q = 0; q < 2; ++q) {
       const int sl = s0 + q;
-      if(!(q == 0 ? h0 : h1)) continue;
+      if(!(q == 0 ? hd0 : hd1)) continue;
       R sre = s_re[sl], sim = CPLX ? s_im[sl] : (R)0;
       for(int k = sl + 1; k < (int)clen && !s_head[k]; ++k) {
         sre += s_re[k];
         if(CPLX) sim += s_im[k];
       }
-      const int64_t pos = (int64_t)cns + rank++;
+      const int64_t pos = (int64_t)ns + rank++;
       if(CPLX) {
         if(MIXED)
           reinterpret_cast<float2 *>(out)[pos] = make_float2((float)sre, (float)sim);
@@ -852,13 +1040,30 @@ __global__ void __launch_bounds__(kGatherThreads)
       }
     }
     __syncthreads();  // LDS is reused by the next chunk
+    a0 = b0; a1 = b1; a2 = b2;
+    b0 = n0; b1 = n1; b2 = n2;
   }
 }
 
 // ------------------------------------------------------------------ k_rhs
-// One thread per row: Dirichlet lifting A_e(i_dof, j_tf) u_D for fixed
+// kRhsLanes lanes per row: Dirichlet lifting A_e(i_dof, j_tf) u_D for fixed
 // columns (transposed local index, VectorFactory.cpp:184), then the element
-// load vectors, each in element order.  Sums in FP64.
+// load vectors.  Lane l takes the row's terms l, l + kRhsLanes, ... in
+// element order, and the lanes combine in a fixed butterfly, so sums are
+// deterministic (FP64).  Several lanes per row keep few rows in flight per
+// XCD, so the element loads they share stay in that XCD's L2.
+constexpr int kRhsLanes = 8;
+
+__device__ __forceinline__ double2 lane_sum(double2 v)
+{
+#pragma unroll
+  for(int d = 1; d < kRhsLanes; d <<= 1) {
+    v.x += __shfl_xor(v.x, d, kRhsLanes);
+    v.y += __shfl_xor(v.y, d, kRhsLanes);
+  }
+  return v;
+}
+
 template <bool CPLX, bool MIXED>
 __global__ void __launch_bounds__(256) k_rhs(const GroupTable *__restrict__ gtp, int64_t n, int nrhs, const int64_t *__restrict__ dc_ptr,
                                              const uint32_t *__restrict__ dc_words, const int32_t *__restrict__ dc_fix,
@@ -866,10 +1071,11 @@ __global__ void __launch_bounds__(256) k_rhs(const GroupTable *__restrict__ gtp,
                                              const uint32_t *__restrict__ lin_words, void *__restrict__ out)
 {
   const GroupTable &gt = *gtp;
-  const int64_t row = xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if(row >= n) return;
+  const int64_t row = (xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) / kRhsLanes;
+  const int l = (int)(threadIdx.x % kRhsLanes);
+  if(row >= n) return;  // whole lane groups leave together
   double2 lift = make_double2(0., 0.);
-  for(int64_t k = dc_ptr[row]; k < dc_ptr[row + 1]; ++k) {
+  for(int64_t k = dc_ptr[row] + l; k < dc_ptr[row + 1]; k += kRhsLanes) {
     const uint32_t w = dc_words[k];
     const int64_t eg = w >> gt.logP;
     const int pair = (int)(w & ((1u << gt.logP) - 1u));
@@ -883,9 +1089,11 @@ __global__ void __launch_bounds__(256) k_rhs(const GroupTable *__restrict__ gtp,
     }
     lift = cadd(lift, cmul(a, uD[dc_fix[k]]));
   }
+  lift = lane_sum(lift);
+  const int64_t l0 = lin_ptr[row], l1 = lin_ptr[row + 1];
   for(int r = 0; r < nrhs; ++r) {
-    double2 acc = lift;
-    for(int64_t k = lin_ptr[row]; k < lin_ptr[row + 1]; ++k) {
+    double2 acc = make_double2(0., 0.);
+    for(int64_t k = l0 + l; k < l1; k += kRhsLanes) {
       const uint32_t w = lin_words[k];
       const int64_t eg = w >> 6;
       const int il = (int)(w & 63u);
@@ -896,6 +1104,8 @@ __global__ void __launch_bounds__(256) k_rhs(const GroupTable *__restrict__ gtp,
       acc.x += b.x;
       acc.y += b.y;
     }
+    acc = cadd(lift, lane_sum(acc));
+    if(l != 0) continue;
     const int64_t pos = (int64_t)r * n + row;
     if(MIXED) {
       if(CPLX)
@@ -1053,15 +1263,19 @@ hipError_t launch_linear(const Group &g, const double *xyz, int nrhs, hipStream_
       a.dim = g.dim;
       a.nv = g.nv;
       a.accumulate = first ? 0 : 1;
-      const size_t lds_pp = sizeof(double) * (size_t)R.Q * g.N + sizeof(double2) * (size_t)(256 / std::max(R.Q, 1)) * R.Q;
+      // point-parallel kernels: EB elements per block, about 4 points per thread
+      const int EB = std::max(1, std::min(128, 1024 / std::max(R.Q, 1)));
+      const size_t lds_pp = sizeof(double) * (size_t)(R.Q * g.N + 4 * R.Q + 13 * EB + 1) +
+                            sizeof(double2) * (size_t)EB * R.Q;
+      const unsigned nblk = (unsigned)((g.ne + EB - 1) / EB);
       const size_t ti = (size_t)(&t - g.dterms.data());
       hipFunction_t jf = ti < g.jit.size() ? g.jit[ti] : nullptr;
-      if(jf && R.Q <= 256 && lds_pp <= 64 * 1024) {
+      if(jf && lds_pp <= 64 * 1024) {
         // compiled coefficient program (jit.cpp); argument layout of JitLinArgs
         struct {
           double consts[2 * GFEM_MAX_CONSTS];
           double fac_re, fac_im;
-          int Q, dim, nv, N, accumulate;
+          int Q, dim, nv, N, accumulate, EB;
         } ja{};
         for(int k = 0; k < 2 * GFEM_MAX_CONSTS; ++k) ja.consts[k] = t.expr.consts[k];
         ja.fac_re = t.fac_re;
@@ -1071,23 +1285,20 @@ hipError_t launch_linear(const Group &g, const double *xyz, int nrhs, hipStream_
         ja.nv = g.nv;
         ja.N = g.N;
         ja.accumulate = first ? 0 : 1;
-        const int EB = 256 / R.Q;
+        ja.EB = EB;
         int64_t ne = g.ne;
         const double *pw = R.d_w, *pp = R.d_pts, *pb = R.d_bf, *pF = g.d_F64;
         const int32_t *pv = g.d_vtx;
         double2 *pbe = be;
         void *params[] = {&ja, &pw, &pp, &pb, &ne, &pv, &xyz, &pF, &pbe};
-        const hipError_t e = hipModuleLaunchKernel(jf, (unsigned)((g.ne + EB - 1) / EB), 1, 1, 256, 1, 1,
-                                                   (unsigned)lds_pp, s, params, nullptr);
+        const hipError_t e = hipModuleLaunchKernel(jf, nblk, 1, 1, 256, 1, 1, (unsigned)lds_pp, s, params, nullptr);
         if(e != hipSuccess) return e;
         first = false;
         continue;
       }
-      if(R.Q <= 256 && lds_pp <= 64 * 1024) {
-        const int EB = 256 / R.Q;
-        const size_t lds = lds_pp;
-        hipLaunchKernelGGL(k_linear_pp, dim3((unsigned)((g.ne + EB - 1) / EB)), dim3(256), lds, s, a, g.N, R.d_w,
-                           R.d_pts, R.d_bf, g.ne, g.d_vtx, xyz, g.d_F64, be);
+      if(lds_pp <= 64 * 1024) {
+        hipLaunchKernelGGL(k_linear_pp, dim3(nblk), dim3(256), lds_pp, s, a, g.N, EB, R.d_w, R.d_pts, R.d_bf, g.ne,
+                           g.d_vtx, xyz, g.d_F64, be);
         first = false;
         continue;
       }
@@ -1121,14 +1332,39 @@ hipError_t launch_dense(const Group &g, const double *xyz, bool all_terms, hipSt
   return hipGetLastError();
 }
 
+template <bool CPLX, bool MIXED>
+hipError_t gather_pipe_launch(const gfem_system &sys, void *out, hipStream_t s)
+{
+  // resident grid: blocks per CU (occupancy) x CUs, a multiple of the XCD count
+  static int64_t grid[2][2] = {{0, 0}, {0, 0}};
+  int64_t &G = grid[CPLX][MIXED];
+  if(G == 0) {
+    int per_cu = 0, cus = 0, dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if(e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if(e == hipSuccess)
+      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_gather_pipe<CPLX, MIXED>, kGatherThreads, 0);
+    if(e != hipSuccess) return e;
+    G = std::max<int64_t>(kXcds, (int64_t)per_cu * cus / kXcds * kXcds);
+  }
+  const int64_t per_xcd = (sys.n_chunks + kXcds - 1) / kXcds;
+  hipLaunchKernelGGL((k_gather_pipe<CPLX, MIXED>), dim3((unsigned)G), dim3(kGatherThreads), 0, s, sys.d_table,
+                     sys.d_rec, sys.n_chunks, per_xcd, out);
+  return hipGetLastError();
+}
+
 hipError_t launch_gather(const gfem_system &sys, int prec, hipStream_t s)
 {
   if(sys.n_chunks == 0) return hipSuccess;
   void *out = sys.d_vals[prec];
+  if(sys.gather_pipe) {
+    if(sys.cplx)
+      return prec ? gather_pipe_launch<true, true>(sys, out, s) : gather_pipe_launch<true, false>(sys, out, s);
+    return prec ? gather_pipe_launch<false, true>(sys, out, s) : gather_pipe_launch<false, false>(sys, out, s);
+  }
   const GroupTable *gt = sys.d_table;
-  const int64_t nb = sys.gather_blocks < sys.n_chunks ? sys.gather_blocks : xcd_grid(sys.n_chunks);
-  const dim3 grid((unsigned)nb), block(kGatherThreads);
-#define GFEM_GATHER_ARGS gt, sys.d_words, sys.d_chunk_c, sys.d_chunk_n, sys.d_chunk_e, sys.d_elist, sys.n_chunks, out
+  const dim3 grid((unsigned)xcd_grid(sys.n_chunks)), block(kGatherThreads);
+#define GFEM_GATHER_ARGS gt, sys.d_rec, sys.n_chunks, out
   if(sys.cplx) {
     if(prec)
       hipLaunchKernelGGL((k_gather<true, true>), grid, block, 0, s, GFEM_GATHER_ARGS);
@@ -1150,7 +1386,7 @@ hipError_t launch_rhs(const gfem_system &sys, int prec, hipStream_t s)
   if(sys.n == 0) return hipSuccess;
   void *out = sys.d_rhs[prec];
   const GroupTable *gt = sys.d_table;
-  const dim3 grid((unsigned)xcd_grid(blocks_for(sys.n, 256))), block(256);
+  const dim3 grid((unsigned)xcd_grid(blocks_for(sys.n * kRhsLanes, 256))), block(256);
 #define GFEM_RHS_ARGS gt, sys.n, sys.nrhs, sys.d_dc_ptr, sys.d_dc_words, sys.d_dc_fix, sys.d_dc, sys.d_lin_ptr, sys.d_lin_words, out
   if(sys.cplx) {
     if(prec)

@@ -92,6 +92,7 @@ struct or_problem {
   or_term *terms;
   int symmetric, dofs_sort;
   double frequency;
+  int threads; /* element-loop threads of or_assemble (1: sequential, the checker) */
   /* dofs: open-addressing hash (numType, entity) -> index into dofs[] */
   int64_t ndofs, cap;
   or_dof *dofs;
@@ -609,6 +610,7 @@ or_problem *or_new(void)
 {
   or_problem *p = xcalloc(1, sizeof(or_problem));
   p->nrhs = 1;
+  p->threads = 1;
   return p;
 }
 
@@ -755,6 +757,12 @@ int or_add_linear(or_problem *p, int nent, const int *dims, const int *tags, int
   t->rhs_idx = rhs_idx;
   if(rhs_idx + 1 > p->nrhs) p->nrhs = rhs_idx + 1;
   return t->insertion;
+}
+
+int or_set_threads(or_problem *p, int threads)
+{
+  p->threads = threads > 1 ? threads : 1;
+  return 0;
 }
 
 int or_set_options(or_problem *p, int symmetric, double frequency, int dofs_sort)
@@ -1299,9 +1307,26 @@ static int64_t csr_pos(const or_problem *p, int64_t row, int64_t col)
   return lo;
 }
 
+/* dst += v; with several element-loop threads the reference adds under
+ * `omp atomic`, re and im separately (MatrixFactory.cpp:267-283,
+ * VectorFactory.cpp:121-260) */
+static void accum(cplx *dst, cplx v, int atomic)
+{
+  if(!atomic) {
+    *dst += v;
+    return;
+  }
+  double *d = (double *)dst;
+  const double re = creal(v), im = cimag(v);
+#pragma omp atomic
+  d[0] += re;
+#pragma omp atomic
+  d[1] += im;
+}
+
 /* m = #tf (rows), n = #dof (cols), values = A_e col-major (n x m): values[i*n + j] */
 static int add_values(or_problem *p, int m, int n, const int64_t *ni, const int32_t *ti, const int64_t *nj,
-                      const int32_t *tj, const cplx *values)
+                      const int32_t *tj, const cplx *values, int atomic)
 {
   int haveDC = 0;
   for(int i = 0; i < m; ++i) {
@@ -1318,26 +1343,26 @@ static int add_values(or_problem *p, int m, int n, const int64_t *ni, const int3
       if(tj[j] != AT_UNKNOWN && tj[j] != AT_UNKNOWN_BUBBLE) continue;
       if(p->symmetric && nj[j] < ni[i]) continue;
       const int64_t pos = csr_pos(p, ni[i], nj[j]);
-      p->val[pos] += values[i * n + j];
+      accum(&p->val[pos], values[i * n + j], atomic);
     }
   }
   return haveDC;
 }
 
 static void add_values_dc(or_problem *p, cplx *b, int m, int n, const int64_t *ni, const int32_t *ti,
-                          const int64_t *nj, const int32_t *tj, const cplx *values)
+                          const int64_t *nj, const int32_t *tj, const cplx *values, int atomic)
 {
   for(int i = 0; i < m; ++i) {
     if(ti[i] != AT_UNKNOWN && ti[i] != AT_UNKNOWN_BUBBLE) continue;
     for(int j = 0; j < n; ++j)
-      if(tj[j] == AT_FIXED) b[ni[i]] += values[j * n + i] * p->values_dc[nj[j]];
+      if(tj[j] == AT_FIXED) accum(&b[ni[i]], values[j * n + i] * p->values_dc[nj[j]], atomic);
   }
 }
 
-static void add_rhs(cplx *b, int n, const int64_t *ni, const int32_t *ti, const cplx *values)
+static void add_rhs(cplx *b, int n, const int64_t *ni, const int32_t *ti, const cplx *values, int atomic)
 {
   for(int i = 0; i < n; ++i)
-    if(ti[i] == AT_UNKNOWN || ti[i] == AT_UNKNOWN_BUBBLE) b[ni[i]] += values[i];
+    if(ti[i] == AT_UNKNOWN || ti[i] == AT_UNKNOWN_BUBBLE) accum(&b[ni[i]], values[i], atomic);
 }
 
 /* ------------------------------------------------------------------ */
@@ -1461,29 +1486,34 @@ int or_assemble(or_problem *p)
           ruletab *rt = xcalloc((size_t)nbil, sizeof(ruletab));
           for(int k = 0; k < nbil; ++k)
             if(rule_build(&rt[k], dim, b->order, b->order, p->terms[bil[k]].gauss)) return -3;
-          cplx *Ae = xcalloc((size_t)(N * N), sizeof(cplx));
-          cplx *tmp = xcalloc((size_t)(N * N), sizeof(cplx));
-          for(int64_t el = 0; el < b->ne; ++el) {
-            memset(Ae, 0, sizeof(cplx) * (size_t)(N * N));
-            for(int k = 0; k < nbil; ++k) {
-              const or_term *t = &p->terms[bil[k]];
-              if(!freq_module || t->dt_order == 0) {
-                bilinear_eval(p, b, el, t, &rt[k], Ae);
+          const int nth = p->threads > 1 ? p->threads : 1, atomic = nth > 1;
+#pragma omp parallel num_threads(nth) if(nth > 1)
+          {
+            cplx *Ae = xcalloc((size_t)(N * N), sizeof(cplx));
+            cplx *tmp = xcalloc((size_t)(N * N), sizeof(cplx));
+#pragma omp for schedule(static)
+            for(int64_t el = 0; el < b->ne; ++el) {
+              memset(Ae, 0, sizeof(cplx) * (size_t)(N * N));
+              for(int k = 0; k < nbil; ++k) {
+                const or_term *t = &p->terms[bil[k]];
+                if(!freq_module || t->dt_order == 0) {
+                  bilinear_eval(p, b, el, t, &rt[k], Ae);
+                }
+                else {
+                  memset(tmp, 0, sizeof(cplx) * (size_t)(N * N));
+                  bilinear_eval(p, b, el, t, &rt[k], tmp);
+                  const cplx f = (t->dt_order == 2) ? (cplx)(-w * w) : -I * w;
+                  for(int k2 = 0; k2 < N * N; ++k2) Ae[k2] += f * tmp[k2];
+                }
               }
-              else {
-                memset(tmp, 0, sizeof(cplx) * (size_t)(N * N));
-                bilinear_eval(p, b, el, t, &rt[k], tmp);
-                const cplx f = (t->dt_order == 2) ? (cplx)(-w * w) : -I * w;
-                for(int k2 = 0; k2 < N * N; ++k2) Ae[k2] += f * tmp[k2];
-              }
+              const int64_t *num = &b->num[el * N];
+              const int32_t *at = &b->atype[el * N];
+              if(add_values(p, N, N, num, at, num, at, Ae, atomic))
+                for(int r = 0; r < p->nrhs; ++r) add_values_dc(p, p->rhs[r], N, N, num, at, num, at, Ae, atomic);
             }
-            const int64_t *num = &b->num[el * N];
-            const int32_t *at = &b->atype[el * N];
-            if(add_values(p, N, N, num, at, num, at, Ae))
-              for(int r = 0; r < p->nrhs; ++r) add_values_dc(p, p->rhs[r], N, N, num, at, num, at, Ae);
+            free(Ae);
+            free(tmp);
           }
-          free(Ae);
-          free(tmp);
           for(int k = 0; k < nbil; ++k) rule_free(&rt[k]);
           free(rt);
         }
@@ -1491,16 +1521,22 @@ int or_assemble(or_problem *p)
           ruletab *rt = xcalloc((size_t)nlin, sizeof(ruletab));
           for(int k = 0; k < nlin; ++k)
             if(rule_build(&rt[k], dim, b->order, b->order, p->terms[lin[k]].gauss)) return -3;
-          cplx *be = xcalloc((size_t)(N * p->nrhs), sizeof(cplx));
-          for(int64_t el = 0; el < b->ne; ++el) {
-            memset(be, 0, sizeof(cplx) * (size_t)(N * p->nrhs));
-            for(int k = 0; k < nlin; ++k) {
-              const or_term *t = &p->terms[lin[k]];
-              linear_eval(p, b, el, t, &rt[k], &be[N * t->rhs_idx]);
+          const int nth = p->threads > 1 ? p->threads : 1, atomic = nth > 1;
+#pragma omp parallel num_threads(nth) if(nth > 1)
+          {
+            cplx *be = xcalloc((size_t)(N * p->nrhs), sizeof(cplx));
+#pragma omp for schedule(static)
+            for(int64_t el = 0; el < b->ne; ++el) {
+              memset(be, 0, sizeof(cplx) * (size_t)(N * p->nrhs));
+              for(int k = 0; k < nlin; ++k) {
+                const or_term *t = &p->terms[lin[k]];
+                linear_eval(p, b, el, t, &rt[k], &be[N * t->rhs_idx]);
+              }
+              for(int r = 0; r < p->nrhs; ++r)
+                add_rhs(p->rhs[r], N, &b->num[el * N], &b->atype[el * N], &be[N * r], atomic);
             }
-            for(int r = 0; r < p->nrhs; ++r) add_rhs(p->rhs[r], N, &b->num[el * N], &b->atype[el * N], &be[N * r]);
+            free(be);
           }
-          free(be);
           for(int k = 0; k < nlin; ++k) rule_free(&rt[k]);
           free(rt);
         }

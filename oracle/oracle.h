@@ -73,6 +73,10 @@ int or_add_linear(or_problem *p, int nent, const int *dims, const int *tags, int
                   double f_re, double f_im, int f_nops, const int *f_ops, int f_nconst, const double *f_consts,
                   double cr_re, double cr_im, int rhs_idx);
 int or_set_options(or_problem *p, int symmetric, double frequency, int dofs_sort /*0 default(Hilbert),1 none*/);
+/* Element-loop threads of or_assemble: 1 (default) is the sequential,
+ * deterministic checker; more threads scatter with atomic adds in an
+ * unspecified order, as the reference's OpenMP loop does (timing only). */
+int or_set_threads(or_problem *p, int threads);
 
 int or_pre(or_problem *p);
 int64_t or_n_unknown(const or_problem *p);

@@ -45,6 +45,7 @@ def lib():
                                     ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int, _c_ip, ctypes.c_int,
                                     _c_dp, ctypes.c_double, ctypes.c_double, ctypes.c_int]
         L.or_set_options.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_int]
+        L.or_set_threads.argtypes = [ctypes.c_void_p, ctypes.c_int]
         L.or_pre.argtypes = [ctypes.c_void_p]
         L.or_n_unknown.argtypes = [ctypes.c_void_p]
         L.or_n_unknown.restype = ctypes.c_int64
@@ -153,6 +154,11 @@ class Problem:
 
     def set_options(self, symmetric=False, frequency=0.0, dofs_sort="default"):
         self.L.or_set_options(self.h, int(symmetric), frequency, 0 if dofs_sort in ("default", "hilbert") else 1)
+
+    def set_threads(self, n):
+        """Element-loop threads of assemble(): 1 is the sequential checker; more
+        threads scatter with atomic adds (unspecified order, timing only)."""
+        self.L.or_set_threads(self.h, int(n))
 
     def pre(self):
         if self.L.or_pre(self.h) != 0:

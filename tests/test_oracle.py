@@ -324,3 +324,20 @@ def test_constant_dirichlet_gives_constant_solution():
     A = csr_matrix((np.ascontiguousarray(P.values().real), aj, ai), shape=(P.n_unknown,) * 2)
     xsol = spsolve(A.tocsc(), np.ascontiguousarray(-P.rhs(0).real))
     assert np.allclose(xsol, 2.5, atol=1e-12)
+
+
+def test_threaded_assembly_matches_sequential():
+    """The OpenMP element loop (atomic scatter, used only to time the CPU
+    baseline) assembles the same system as the sequential checker, up to the
+    order of the additions."""
+    from gfem import workloads as W
+    spec = W.helmholtz3d(n=5, order=2)
+    P = O.from_spec(spec)
+    P.pre()
+    P.assemble()
+    v1, r1 = P.values(), P.rhs(0)
+    P.set_threads(4)
+    P.assemble()
+    v4, r4 = P.values(), P.rhs(0)
+    assert np.abs(v4 - v1).max() <= 1e-13 * np.abs(v1).max()
+    assert np.abs(r4 - r1).max() <= 1e-13 * np.abs(r1).max()
