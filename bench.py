@@ -22,6 +22,19 @@ for p in (os.path.join(ROOT, "fem-g_amd"), ROOT):
         sys.path.insert(0, p)
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "gather_traffic.json")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of the gather from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (scaled by a copy of known size), when it
+    was taken on this kernel and workload; None otherwise."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    return t.get("traffic_bytes_per_launch") if t.get("kernel") == kernel else None
 
 
 def log(*a):
@@ -197,7 +210,9 @@ def main():
                    "element_dofs": edofs, "n_unknown": int(F.n_unknown), "nnz": int(F.nnz),
                    "pre_host_s": t_pre},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic("k_gather<true, false>") if args.gather == "block" and args.n == 50
+                     and args.order == 2 else None,
                      "kernel": "k_gather<complex,fp64>" if args.gather == "block" else "k_gather_pipe<complex,fp64>",
                      "kernel_ms": gather_ms,
                      "algorithmic_bytes": gbytes},

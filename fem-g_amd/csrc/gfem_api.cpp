@@ -77,7 +77,6 @@ void free_group(Group &g)
   dfree(g.d_T64r);
   dfree(g.d_T32r);
   dfree(g.d_dense);
-  dfree(g.d_be);
   dfree(g.d_e2n);
   dfree(g.d_num);
   dfree(g.d_type);
@@ -263,6 +262,7 @@ gfem_status gfem_system_destroy(gfem_system *sys)
   dfree(sys->d_dc);
   dfree(sys->d_table);
   dfree(sys->d_rec);
+  dfree(sys->d_be_all);
   dfree(sys->d_lin_words);
   dfree(sys->d_dc_words);
   dfree(sys->d_dc_fix);
@@ -512,11 +512,6 @@ gfem_status gfem_system_finalize(gfem_system *sys)
     for(const DevTerm &d : g.dterms)
       if(d.kind == 0 && !d.per_point && (d.c_re * d.fac_im + d.c_im * d.fac_re) != 0.) g.treal = false;
     if(g.has_dense) GFEM_HIP(ctx, dalloc(&g.d_dense, (size_t)(g.ne * g.N * g.N)));
-    if(g.has_linear) {
-      // right-hand sides without a linear term on this group keep zeros
-      GFEM_HIP(ctx, dalloc(&g.d_be, (size_t)(sys->nrhs * g.ne * g.N)));
-      GFEM_HIP(ctx, hipMemset(g.d_be, 0, sizeof(double2) * (size_t)(sys->nrhs * g.ne * g.N)));
-    }
     for(auto &R : g.rules) {
       GFEM_HIP(ctx, dupload(&R.d_w, R.w.data(), R.w.size()));
       GFEM_HIP(ctx, dupload(&R.d_pts, R.pts.data(), R.pts.size()));
@@ -534,6 +529,24 @@ gfem_status gfem_system_finalize(gfem_system *sys)
     for(size_t r = 0; r < g.rules.size(); ++r)
       gd.rules[r] = RuleDev{g.rules[r].Q, g.rules[r].d_w, g.rules[r].d_pts, g.rules[r].d_bf, g.rules[r].d_gbf};
     GFEM_HIP(ctx, dupload(&g.d_dev, &gd, 1));
+  }
+  // element load vectors of every group in one buffer, so a right-hand-side
+  // term is one flat index; right-hand sides without a linear term on a
+  // group keep zeros there
+  sys->be_total = 0;
+  for(auto &g : sys->groups)
+    if(g.has_linear) {
+      g.be_off = sys->be_total;
+      sys->be_total += g.ne * g.N;
+    }
+  if(sys->be_total >= (int64_t(1) << 32)) return fail(ctx, GFEM_ERR_TOO_LARGE, "too many element loads");
+  if(sys->be_total > 0) {
+    GFEM_HIP(ctx, dalloc(&sys->d_be_all, (size_t)(sys->nrhs * sys->be_total)));
+    GFEM_HIP(ctx, hipMemset(sys->d_be_all, 0, sizeof(double2) * (size_t)(sys->nrhs * sys->be_total)));
+  }
+  for(auto &g : sys->groups) {
+    g.d_be = g.has_linear ? sys->d_be_all + g.be_off : nullptr;
+    g.be_stride = sys->be_total;
   }
   {
     const GroupTable gt = make_table(sys);
@@ -684,9 +697,9 @@ gfem_status gfem_system_finalize(gfem_system *sys)
               dc_fixv.push_back((int32_t)fx);
             }
           if(g.has_linear) {
-            if(eg >= (int64_t(1) << 26)) return fail(ctx, GFEM_ERR_TOO_LARGE, "too many elements for linear map");
+            // flat index into the shared element-load buffer
             lin_key.push_back(row);
-            lin_pay.push_back((uint32_t)((eg << 6) | i));
+            lin_pay.push_back((uint32_t)(g.be_off + e * N + i));
           }
         }
       }
@@ -758,26 +771,29 @@ gfem_status gfem_assemble(gfem_system *sys, gfem_precision prec, gfem_strategy s
   clear_prof(sys);
   Prof prof{sys, s};
   if(strategy == GFEM_STRATEGY_FUSED) {
+    // geometry and per-point element matrices first; then the element loads,
+    // the RHS rows and the matrix gather
     for(auto &g : sys->groups) {
       prof.begin("geometry");
       GFEM_HIP(ctx, launch_geometry(g, ctx->d_xyz, s));
       prof.end();
-      if(g.has_linear) {
-        prof.begin("linear");
-        GFEM_HIP(ctx, launch_linear(g, ctx->d_xyz, sys->nrhs, s));
-        prof.end();
-      }
       if(g.has_dense) {
         prof.begin("dense");
         GFEM_HIP(ctx, launch_dense(g, ctx->d_xyz, false, s));
         prof.end();
       }
     }
-    prof.begin("gather");
-    GFEM_HIP(ctx, launch_gather(*sys, p, s));
-    prof.end();
+    for(auto &g : sys->groups)
+      if(g.has_linear) {
+        prof.begin("linear");
+        GFEM_HIP(ctx, launch_linear(g, ctx->d_xyz, sys->nrhs, s));
+        prof.end();
+      }
     prof.begin("rhs");
     GFEM_HIP(ctx, launch_rhs(*sys, p, s));
+    prof.end();
+    prof.begin("gather");
+    GFEM_HIP(ctx, launch_gather(*sys, p, s));
     prof.end();
     return GFEM_OK;
   }

@@ -132,7 +132,8 @@ struct Group {
   float *d_T32r = nullptr;
   bool treal = true;
   double2 *d_dense = nullptr;
-  double2 *d_be = nullptr;
+  double2 *d_be = nullptr;      // this group's element loads inside gfem_system::d_be_all
+  int64_t be_off = 0, be_stride = 0;  // offset in d_be_all, distance between right-hand sides
   // plain reference strategy
   int32_t *d_e2n = nullptr;
   int32_t *d_num = nullptr;
@@ -166,6 +167,8 @@ struct gfem_system {
   gfem::GroupTable *d_table = nullptr;
   // fused maps
   uint32_t *d_rec = nullptr;  // chunk records, n_chunks x kRec
+  double2 *d_be_all = nullptr;  // element load vectors of all groups, [nrhs][be_total]
+  int64_t be_total = 0;
   int gather_pipe = 0;        // 1: persistent record-prefetching gather (GFEM_GATHER=pipe)
   int64_t gather_grid = 0;    // persistent grid (multiple of the XCD count)
   int64_t n_words = 0;

@@ -737,6 +737,102 @@ __device__ __forceinline__ void gather_eval(uint32_t w, int logP, uint32_t pmask
   s_head[slot] = (uint8_t)(w >> 31);
 }
 
+// Sums of the nonzeros' runs of slots and their output positions, for the
+// chunk whose values sit in LDS in nonzero order (s_head marks each run's
+// first slot).  Thread t owns slots 2t, 2t+1.  A run's sum is its slots in
+// this thread plus a carry from the threads to its right: the carry is a
+// segmented suffix scan over threads (shuffles within a wave, one LDS
+// exchange across waves), so long runs cost log steps instead of a serial
+// walk.  The order of additions is fixed: results are bitwise reproducible.
+// Output position of a run = first nonzero of the chunk + rank of its head.
+template <bool CPLX, typename R>
+__device__ __forceinline__ void gather_runs(int t, uint32_t clen, uint32_t ns, const R *s_re, const R *s_im,
+                                            const uint8_t *s_head, int *s_wsum, R *s_wx, R *s_wy, int *s_wf,
+                                            void *__restrict__ out)
+{
+  constexpr int kW = kGatherThreads / 64;
+  const int s0 = 2 * t;
+  const bool in0 = s0 < (int)clen, in1 = s0 + 1 < (int)clen;
+  // slots past the chunk end count as closed heads so nothing flows across
+  const int h0 = in0 ? s_head[s0] : 1, h1 = in1 ? s_head[s0 + 1] : 1;
+  const R v0r = in0 ? s_re[s0] : (R)0, v1r = in1 ? s_re[s0 + 1] : (R)0;
+  const R v0i = (CPLX && in0) ? s_im[s0] : (R)0, v1i = (CPLX && in1) ? s_im[s0 + 1] : (R)0;
+  // part of this thread's slots that continues the run from the left
+  R xr = h0 ? (R)0 : (h1 ? v0r : v0r + v1r);
+  R xi = h0 ? (R)0 : (h1 ? v0i : v0i + v1i);
+  int f = (h0 | h1) ? 1 : 0;
+  const int lane = t & 63, wv = t >> 6;
+#pragma unroll
+  for(int d = 1; d < 64; d <<= 1) {
+    const R yr = __shfl_down(xr, d, 64);
+    const R yi = CPLX ? __shfl_down(xi, d, 64) : (R)0;
+    const int g = __shfl_down(f, d, 64);
+    if(lane + d < 64 && !f) {
+      xr += yr;
+      if(CPLX) xi += yi;
+      f = g;
+    }
+  }
+  // ranks: inclusive scan of the (in-range) head counts
+  const int cnt = (in0 ? h0 : 0) + (in1 ? h1 : 0);
+  int incl = cnt;
+#pragma unroll
+  for(int d = 1; d < 64; d <<= 1) {
+    const int v = __shfl_up(incl, d, 64);
+    if(lane >= d) incl += v;
+  }
+  if(lane == 63) s_wsum[wv] = incl;
+  if(lane == 0) {
+    s_wx[wv] = xr;
+    s_wy[wv] = xi;
+    s_wf[wv] = f;
+  }
+  __syncthreads();
+  int rank = incl - cnt;
+  for(int k = 0; k < wv; ++k) rank += s_wsum[k];
+  // carry into this wave from the waves to its right
+  R cr = 0, ci = 0;
+  for(int k = kW - 1; k > wv; --k) {
+    if(s_wf[k]) {
+      cr = s_wx[k];
+      ci = s_wy[k];
+    }
+    else {
+      cr += s_wx[k];
+      ci += s_wy[k];
+    }
+  }
+  if(!f) {
+    xr += cr;
+    xi += ci;
+  }
+  // carry into this thread = suffix value of the next thread
+  R nr = __shfl_down(xr, 1, 64), ni = CPLX ? __shfl_down(xi, 1, 64) : (R)0;
+  if(lane == 63) {
+    nr = cr;
+    ni = ci;
+  }
+  auto put = [&](int64_t pos, R re, R im) {
+    if(CPLX) {
+      if(std::is_same<R, float>::value)
+        reinterpret_cast<float2 *>(out)[pos] = make_float2((float)re, (float)im);
+      else
+        reinterpret_cast<double2 *>(out)[pos] = make_double2((double)re, (double)im);
+    }
+    else {
+      reinterpret_cast<R *>(out)[pos] = re;
+    }
+  };
+  if(in0 && h0) {
+    if(h1)
+      put((int64_t)ns + rank, v0r, v0i);
+    else
+      put((int64_t)ns + rank, v0r + (v1r + nr), v0i + (v1i + ni));
+    ++rank;
+  }
+  if(in1 && h1) put((int64_t)ns + rank, v1r + nr, v1i + ni);
+}
+
 template <bool CPLX, bool MIXED>
 __global__ void __launch_bounds__(kGatherThreads)
   k_gather(const GroupTable *__restrict__ gtp, const uint32_t *__restrict__ recs, int64_t nchunks,
@@ -757,6 +853,8 @@ __global__ void __launch_bounds__(kGatherThreads)
   __shared__ R s_im[CPLX ? kChunk : 1];
   __shared__ uint8_t s_head[kChunk];
   __shared__ int s_wsum[kGatherThreads / 64];
+  __shared__ R s_wx[kGatherThreads / 64], s_wy[kGatherThreads / 64];
+  __shared__ int s_wf[kGatherThreads / 64];
   const GroupTable &gt = *gtp;
   const int t = threadIdx.x;
   const int64_t c = xcd_remap(blockIdx.x, gridDim.x);
@@ -816,42 +914,7 @@ __global__ void __launch_bounds__(kGatherThreads)
     gather_eval<CPLX, MIXED>(r1, logP, pmask, s_F, s_T, s_treal, s_D, s_re, s_im, s_head);
   __syncthreads();
 
-  // thread t owns slots 2t, 2t+1: rank of its first head = exclusive scan
-  // of the head counts
-  const int s0 = 2 * t;
-  const int h0 = (s0 < (int)clen) ? s_head[s0] : 0, h1 = (s0 + 1 < (int)clen) ? s_head[s0 + 1] : 0;
-  const int cnt = h0 + h1;
-  const int lane = t & 63, wv = t >> 6;
-  int incl = cnt;
-#pragma unroll
-  for(int d = 1; d < 64; d <<= 1) {
-    const int v = __shfl_up(incl, d, 64);
-    if(lane >= d) incl += v;
-  }
-  if(lane == 63) s_wsum[wv] = incl;
-  __syncthreads();
-  int rank = incl - cnt;
-  for(int k = 0; k < wv; ++k) rank += s_wsum[k];
-#pragma unroll
-  for(int q = 0; q < 2; ++q) {
-    const int sl = s0 + q;
-    if(!(q == 0 ? h0 : h1)) continue;
-    R sre = s_re[sl], sim = CPLX ? s_im[sl] : (R)0;
-    for(int k = sl + 1; k < (int)clen && !s_head[k]; ++k) {
-      sre += s_re[k];
-      if(CPLX) sim += s_im[k];
-    }
-    const int64_t pos = (int64_t)ns + rank++;
-    if(CPLX) {
-      if(MIXED)
-        reinterpret_cast<float2 *>(out)[pos] = make_float2((float)sre, (float)sim);
-      else
-        reinterpret_cast<double2 *>(out)[pos] = make_double2((double)sre, (double)sim);
-    }
-    else {
-      reinterpret_cast<R *>(out)[pos] = sre;
-    }
-  }
+  gather_runs<CPLX, R>(t, clen, ns, s_re, s_im, s_head, s_wsum, s_wx, s_wy, s_wf, out);
 }
 
 // Persistent variant of the same gather.  A grid of resident blocks, the
@@ -882,6 +945,8 @@ __global__ void __launch_bounds__(kGatherThreads)
   __shared__ R s_im[CPLX ? kChunk : 1];
   __shared__ uint8_t s_head[kChunk];
   __shared__ int s_wsum[kGatherThreads / 64];
+  __shared__ R s_wx[kGatherThreads / 64], s_wy[kGatherThreads / 64];
+  __shared__ int s_wf[kGatherThreads / 64];
   const GroupTable &gt = *gtp;
   const int t = threadIdx.x;
   const int logP = gt.logP;
@@ -1004,41 +1069,8 @@ __global__ void __launch_bounds__(kGatherThreads)
     if(h0) eval(a0, ta);
     if(h1) eval(a1, tb);
     __syncthreads();
-    // (6) runs of each nonzero summed in slot order, one store per nonzero
-    const int s0 = 2 * t;
-    const int hd0 = (s0 < (int)clen) ? s_head[s0] : 0, hd1 = (s0 + 1 < (int)clen) ? s_head[s0 + 1] : 0;
-    const int cnt = hd0 + hd1;
-    const int lane = t & 63, wv = t >> 6;
-    int incl = cnt;
-#pragma unroll
-    for(int d = 1; d < 64; d <<= 1) {
-      const int v = __shfl_up(incl, d, 64);
-      if(lane >= d) incl += v;
-    }
-    if(lane == 63) s_wsum[wv] = incl;
-    __syncthreads();
-    int rank = incl - cnt;
-    for(int k = 0; k < wv; ++k) rank += s_wsum[k];
-#pragma unroll
-    for(int q = 0; q < 2; ++q) {
-      const int sl = s0 + q;
-      if(!(q == 0 ? hd0 : hd1)) continue;
-      R sre = s_re[sl], sim = CPLX ? s_im[sl] : (R)0;
-      for(int k = sl + 1; k < (int)clen && !s_head[k]; ++k) {
-        sre += s_re[k];
-        if(CPLX) sim += s_im[k];
-      }
-      const int64_t pos = (int64_t)ns + rank++;
-      if(CPLX) {
-        if(MIXED)
-          reinterpret_cast<float2 *>(out)[pos] = make_float2((float)sre, (float)sim);
-        else
-          reinterpret_cast<double2 *>(out)[pos] = make_double2((double)sre, (double)sim);
-      }
-      else {
-        reinterpret_cast<R *>(out)[pos] = sre;
-      }
-    }
+    // (6) run sums, one store per nonzero
+    gather_runs<CPLX, R>(t, clen, ns, s_re, s_im, s_head, s_wsum, s_wx, s_wy, s_wf, out);
     __syncthreads();  // LDS is reused by the next chunk
     a0 = b0; a1 = b1; a2 = b2;
     b0 = n0; b1 = n1; b2 = n2;
@@ -1068,7 +1100,8 @@ template <bool CPLX, bool MIXED>
 __global__ void __launch_bounds__(256) k_rhs(const GroupTable *__restrict__ gtp, int64_t n, int nrhs, const int64_t *__restrict__ dc_ptr,
                                              const uint32_t *__restrict__ dc_words, const int32_t *__restrict__ dc_fix,
                                              const double2 *__restrict__ uD, const int64_t *__restrict__ lin_ptr,
-                                             const uint32_t *__restrict__ lin_words, void *__restrict__ out)
+                                             const uint32_t *__restrict__ lin_words, const double2 *__restrict__ be_all,
+                                             int64_t be_total, void *__restrict__ out)
 {
   const GroupTable &gt = *gtp;
   const int64_t row = (xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) / kRhsLanes;
@@ -1094,13 +1127,7 @@ __global__ void __launch_bounds__(256) k_rhs(const GroupTable *__restrict__ gtp,
   for(int r = 0; r < nrhs; ++r) {
     double2 acc = make_double2(0., 0.);
     for(int64_t k = l0 + l; k < l1; k += kRhsLanes) {
-      const uint32_t w = lin_words[k];
-      const int64_t eg = w >> 6;
-      const int il = (int)(w & 63u);
-      const int g = find_group(gt, eg);
-      if(!gt.be[g]) continue;
-      const int64_t el = eg - gt.e_off[g];
-      const double2 b = gt.be[g][((int64_t)r * gt.ne[g] + el) * gt.N[g] + il];
+      const double2 b = be_all[(int64_t)r * be_total + lin_words[k]];
       acc.x += b.x;
       acc.y += b.y;
     }
@@ -1146,7 +1173,8 @@ template <bool CPLX>
 __global__ void __launch_bounds__(256) k_ref_rhs(int64_t ne, int N, int nrhs, int64_t n,
                                                  const int32_t *__restrict__ num, const int32_t *__restrict__ type,
                                                  const double2 *__restrict__ Ae, const double2 *__restrict__ be,
-                                                 const double2 *__restrict__ uD, void *__restrict__ out)
+                                                 int64_t be_stride, const double2 *__restrict__ uD,
+                                                 void *__restrict__ out)
 {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if(t >= ne * N) return;
@@ -1166,7 +1194,7 @@ __global__ void __launch_bounds__(256) k_ref_rhs(int64_t ne, int N, int nrhs, in
   for(int r = 0; r < nrhs; ++r) {
     double2 v = lift;
     if(be) {
-      const double2 b = be[((int64_t)r * ne + e) * N + i];
+      const double2 b = be[(int64_t)r * be_stride + e * N + i];
       v.x += b.x;
       v.y += b.y;
     }
@@ -1247,7 +1275,7 @@ hipError_t launch_linear(const Group &g, const double *xyz, int nrhs, hipStream_
 {
   if(g.ne == 0 || !g.has_linear) return hipSuccess;
   for(int r = 0; r < nrhs; ++r) {
-    double2 *be = g.d_be + (int64_t)r * g.ne * g.N;
+    double2 *be = g.d_be + (int64_t)r * g.be_stride;
     bool first = true;
     for(const DevTerm &t : g.dterms) {
       if(t.kind != 1 || t.rhs_idx != r) continue;
@@ -1387,7 +1415,7 @@ hipError_t launch_rhs(const gfem_system &sys, int prec, hipStream_t s)
   void *out = sys.d_rhs[prec];
   const GroupTable *gt = sys.d_table;
   const dim3 grid((unsigned)xcd_grid(blocks_for(sys.n * kRhsLanes, 256))), block(256);
-#define GFEM_RHS_ARGS gt, sys.n, sys.nrhs, sys.d_dc_ptr, sys.d_dc_words, sys.d_dc_fix, sys.d_dc, sys.d_lin_ptr, sys.d_lin_words, out
+#define GFEM_RHS_ARGS gt, sys.n, sys.nrhs, sys.d_dc_ptr, sys.d_dc_words, sys.d_dc_fix, sys.d_dc, sys.d_lin_ptr, sys.d_lin_words, sys.d_be_all, sys.be_total, out
   if(sys.cplx) {
     if(prec)
       hipLaunchKernelGGL((k_rhs<true, true>), grid, block, 0, s, GFEM_RHS_ARGS);
@@ -1425,10 +1453,10 @@ hipError_t launch_ref_rhs(const gfem_system &sys, const Group &g, hipStream_t s)
   const double2 *be = g.has_linear ? g.d_be : nullptr;
   if(sys.cplx)
     hipLaunchKernelGGL(k_ref_rhs<true>, dim3(blocks_for(total, 256)), dim3(256), 0, s, g.ne, g.N, sys.nrhs, sys.n,
-                       g.d_num, g.d_type, Ae, be, sys.d_dc, sys.d_rhs[0]);
+                       g.d_num, g.d_type, Ae, be, g.be_stride, sys.d_dc, sys.d_rhs[0]);
   else
     hipLaunchKernelGGL(k_ref_rhs<false>, dim3(blocks_for(total, 256)), dim3(256), 0, s, g.ne, g.N, sys.nrhs, sys.n,
-                       g.d_num, g.d_type, Ae, be, sys.d_dc, sys.d_rhs[0]);
+                       g.d_num, g.d_type, Ae, be, g.be_stride, sys.d_dc, sys.d_rhs[0]);
   return hipGetLastError();
 }
 

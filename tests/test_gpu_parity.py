@@ -178,3 +178,4 @@ def test_pipelined_gather_matches_oracle(monkeypatch, case, precision, rtol):
     assert_close(F2.values(), P.values(), rtol, "pipelined A")
     assert_close(F2.rhs(0), P.rhs(0), rtol, "pipelined b")
     assert_close(F2.values(), F1.values(), 1e-14 if precision == "fp64" else rtol, "pipelined vs block")
+
