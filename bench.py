@@ -23,6 +23,7 @@ for p in (os.path.join(ROOT, "fem-g_amd"), ROOT):
 
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "gather_traffic.json")
+GATHER_KERNEL = "k_gather<true, false, false, {}>"  # complex, FP64, 16-bit words (P2), slots per thread
 
 
 def measured_traffic(kernel):
@@ -129,8 +130,8 @@ def main():
     ap.add_argument("--cpu-n", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the mixed / reference-strategy lines")
-    ap.add_argument("--gather", choices=["block", "pipe"], default="block",
-                    help="gather variant: one block per chunk, or persistent with record prefetch")
+    ap.add_argument("--gather-slots", type=int, choices=[2, 4], default=4,
+                    help="contributions per gather thread (4: 128-thread blocks, 2: 256-thread blocks)")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="BYTES",
                     help="run a device-to-device copy of BYTES first (counter calibration)")
     args = ap.parse_args()
@@ -141,7 +142,7 @@ def main():
 
     from gfem import workloads as W
 
-    os.environ["GFEM_GATHER"] = args.gather
+    os.environ["GFEM_GATHER_SLOTS"] = str(args.gather_slots)
     if args.calib_copy:
         copy_calibration(args.calib_copy)
 
@@ -211,10 +212,8 @@ def main():
                    "pre_host_s": t_pre},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic("k_gather<true, false>") if args.gather == "block" and args.n == 50
-                     and args.order == 2 else None,
-                     "kernel": "k_gather<complex,fp64>" if args.gather == "block" else "k_gather_pipe<complex,fp64>",
-                     "kernel_ms": gather_ms,
+                     "traffic": measured_traffic(GATHER_KERNEL.format(args.gather_slots)) if args.n == 50 and args.order == 2 else None,
+                     "kernel": GATHER_KERNEL.format(args.gather_slots), "kernel_ms": gather_ms,
                      "algorithmic_bytes": gbytes},
         "cpu_baseline": cpu,
     }

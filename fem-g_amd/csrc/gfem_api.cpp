@@ -555,10 +555,12 @@ gfem_status gfem_system_finalize(gfem_system *sys)
 
   // Element -> nonzero map, transposed: the contributions of every nonzero in
   // traversal order (group, element, local pair), cut into gather chunks of
-  // whole nonzeros with <= kChunkWords contributions and <= kChunkElems distinct
-  // elements; inside a chunk the words are grouped by element.
+  // whole nonzeros with <= kChunk contributions and <= kChunkElems distinct
+  // elements.  A chunk's words are in nonzero order (slot order).
   {
-    if(logP > kMaxLogP) return fail(ctx, GFEM_ERR_UNSUPPORTED, "more than 45 dofs by element in the gather words");
+    if(logP > kMaxLogP) return fail(ctx, GFEM_ERR_UNSUPPORTED, "more than 64 dofs by element in the gather words");
+    sys->rec_wide = logP > kNarrowLogP ? 1 : 0;
+    sys->rec_stride = sys->rec_wide ? kRecWide : kRecNarrow;
     std::vector<int64_t> cnt(sys->nnz + 1, 0);
     int64_t total = 0;
     for(auto &g : sys->groups) {
@@ -576,7 +578,7 @@ gfem_status gfem_system_finalize(gfem_system *sys)
     for(int64_t i = 0; i < sys->nnz; ++i) {
       if(cnt[i + 1] == 0) return fail(ctx, GFEM_ERR_INVALID_ARG, "a stored nonzero has no element contribution");
       if(cnt[i + 1] > kChunkElems)
-        return fail(ctx, GFEM_ERR_UNSUPPORTED, "a nonzero has more than 256 element contributions");
+        return fail(ctx, GFEM_ERR_UNSUPPORTED, "a nonzero has more than 128 element contributions");
       cnt[i + 1] += cnt[i];
     }
     if(total >= (int64_t(1) << 32)) return fail(ctx, GFEM_ERR_TOO_LARGE, "too many contributions");
@@ -604,7 +606,8 @@ gfem_status gfem_system_finalize(gfem_system *sys)
     }
     std::vector<uint32_t> rec, elist;
     std::vector<int32_t> mark((size_t)std::max<int64_t>(n_elem, 1), -1), local((size_t)std::max<int64_t>(n_elem, 1), 0);
-    std::vector<std::pair<uint32_t, uint32_t>> tmp;
+    const int stride = sys->rec_stride;
+    const uint32_t head_bit = sys->rec_wide ? 0x80000000u : 0x8000u;
     int64_t nz = 0;
     int32_t chunk = 0;
     while(nz < sys->nnz) {
@@ -612,11 +615,11 @@ gfem_status gfem_system_finalize(gfem_system *sys)
       const int64_t nz0 = nz;
       elist.clear();
       int nel = 0;
-      while(nz < sys->nnz && cnt[nz + 1] - c0 <= kChunkWords) {
+      while(nz < sys->nnz && cnt[nz + 1] - c0 <= kChunk) {
         int fresh = 0;
         for(int64_t k = cnt[nz]; k < cnt[nz + 1]; ++k)
           if(mark[c_elem[k]] != chunk) ++fresh;
-        if(nel + fresh > kChunkElems || kRecHead + (cnt[nz + 1] - c0) + nel + fresh > kRec) break;
+        if(nel + fresh > kChunkElems) break;
         for(int64_t k = cnt[nz]; k < cnt[nz + 1]; ++k)
           if(mark[c_elem[k]] != chunk) {
             mark[c_elem[k]] = chunk;
@@ -629,42 +632,34 @@ gfem_status gfem_system_finalize(gfem_system *sys)
         ++nz;
       }
       const int64_t c1 = cnt[nz];
-      tmp.clear();
-      for(int64_t k = c0; k < c1; ++k) {
-        const uint32_t slot = (uint32_t)(k - c0);
-        const uint32_t key = ((uint32_t)local[c_elem[k]] << logP) | c_pair[k];
-        tmp.emplace_back(key, slot);
-      }
-      // head flags per slot
-      std::vector<uint8_t> heads((size_t)(c1 - c0), 0);
-      for(int64_t z = nz0; z < nz; ++z) heads[cnt[z] - c0] = 1;
-      std::stable_sort(tmp.begin(), tmp.end(),
-                       [](const std::pair<uint32_t, uint32_t> &a, const std::pair<uint32_t, uint32_t> &b) {
-                         return a.first < b.first;
-                       });
-      // record: header, words from the front, element list at the back
+      // record: header, words in slot order, element list at the back
       const size_t r0 = rec.size();
-      rec.resize(r0 + kRec, 0u);
+      rec.resize(r0 + (size_t)stride, 0u);
       uint32_t *R = &rec[r0];
       R[0] = (uint32_t)nz0;
       R[1] = (uint32_t)(c1 - c0);
       R[2] = (uint32_t)nel;
-      for(size_t k = 0; k < tmp.size(); ++k) {
-        const uint32_t le = tmp[k].first >> logP, pair = tmp[k].first & ((1u << logP) - 1u);
-        const uint32_t grp = elist[le] >> 27;
-        R[kRecHead + k] = pair | (grp << logP) | (le << (logP + 4)) | (tmp[k].second << (logP + 11)) |
-                          (heads[tmp[k].second] ? 0x80000000u : 0u);
+      int64_t z = nz0;
+      for(int64_t k = c0; k < c1; ++k) {
+        bool head = false;
+        if(k == cnt[z]) {  // first contribution of nonzero z
+          head = true;
+          ++z;
+        }
+        const uint32_t w = c_pair[k] | ((uint32_t)local[c_elem[k]] << logP) | (head ? head_bit : 0u);
+        const int64_t slot = k - c0;
+        if(sys->rec_wide)
+          R[kRecHead + slot] = w;
+        else
+          R[kRecHead + slot / 2] |= (slot & 1) ? (w << 16) : w;
       }
-      for(int j = 0; j < nel; ++j) R[kRec - nel + j] = elist[j];
+      for(int j = 0; j < nel; ++j) R[stride - nel + j] = elist[j];
       sys->n_elist += nel;
       ++chunk;
     }
     sys->n_chunks = (int64_t)chunk;
     sys->n_words = total;
-    // gather variant: one block per chunk (default) or a persistent grid that
-    // prefetches records two chunks ahead (GFEM_GATHER=pipe)
-    const char *gm = std::getenv("GFEM_GATHER");
-    sys->gather_pipe = (gm && std::string(gm) == "pipe") ? 1 : 0;
+    if(const char *e = std::getenv("GFEM_GATHER_SLOTS")) sys->gather_slots = std::atoi(e) == 2 ? 2 : 4;
     GFEM_HIP(ctx, dupload(&sys->d_rec, rec.data(), rec.size()));
   }
 
@@ -909,7 +904,8 @@ gfem_status gfem_gather_bytes(const gfem_system *sys, gfem_precision prec, doubl
   if(!sys || !bytes) return GFEM_ERR_INVALID_ARG;
   const int p = prec == GFEM_PREC_MIXED ? 1 : 0;
   // chunk records (header, contribution words, element list; padding not counted)
-  double b = 4.0 * ((double)kRecHead * (double)sys->n_chunks + (double)sys->n_words + (double)sys->n_elist);
+  double b = 4.0 * ((double)kRecHead * (double)sys->n_chunks + (double)sys->n_elist) +
+             (sys->rec_wide ? 4.0 : 2.0) * (double)sys->n_words;
   int64_t ne = 0;
   for(auto &g : sys->groups)
     if(g.has_bilinear) ne += g.ne;

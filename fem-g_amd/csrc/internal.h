@@ -7,21 +7,20 @@
 //   reference tensor        T[N*N][8] complex, pair = i_tf*N + j_dof, the
 //                           coefficient-weighted sum over the group's
 //                           constant-coefficient bilinear terms
-//   gather chunks           runs of whole nonzeros, <= kChunkWords
-//                           contributions and <= kChunkElems distinct
-//                           elements each, one fixed-stride record of kRec
-//                           uint32 per chunk:
+//   gather chunks           runs of whole nonzeros, <= kChunk contributions
+//                           and <= kChunkElems distinct elements each, one
+//                           fixed-stride record of uint32 per chunk:
 //                             [0] first nonzero, [1] len (words), [2] nel,
-//                             [kRecHead, kRecHead+len)  contribution words,
-//                             [kRec-nel, kRec)         element list, entries
+//                             [kRecHead, ...)   contribution words in slot
+//                                               (nonzero) order,
+//                             [stride-nel, stride) element list, entries
 //                             (group << 27) | element index within the group
 //                           so one block fetches its whole chunk at once
-//   contribution words      uint32, grouped by chunk element then pair:
-//                           bits [0,logP) local pair i_tf*N + j_dof,
-//                           [logP,logP+4) group, [logP+4,logP+11) chunk
-//                           element, [logP+11,logP+20) slot = position in the
-//                           chunk's nonzero order, bit 31 = slot is the first
-//                           term of its nonzero (logP <= 11, N <= 45)
+//   contribution words      bits [0,logP) local pair i_tf*N + j_dof,
+//                           [logP,logP+7) chunk element, top bit = first
+//                           term of its nonzero; uint16 (two per uint32,
+//                           even slot low) when logP <= kNarrowLogP, else
+//                           uint32
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -36,14 +35,13 @@ namespace gfem {
 
 constexpr int kMaxGroups = 16;
 constexpr int kMaxN = 64;
-constexpr int kChunk = 512;       // LDS slots per gather block
+constexpr int kChunk = 512;       // contributions per gather block (two per thread)
 constexpr int kChunkElems = 128;  // distinct elements per gather block (7 bits)
-constexpr int kRec = 576;         // uint32 per chunk record (2304 B)
 constexpr int kRecHead = 4;       // header words of a record
-constexpr int kRecWordEnd = 512;  // words sit in record slots [kRecHead, kRecWordEnd)
-constexpr int kChunkWords = kRecWordEnd - kRecHead;  // contributions per chunk (slot: 9 bits)
-constexpr int kMaxLogP = 11;      // pair bits in a contribution word
-static_assert(kChunkWords <= kChunk && kChunk <= 512 && kRec - kRecWordEnd <= 256, "record layout");
+constexpr int kNarrowLogP = 8;    // pair bits that fit a 16-bit word
+constexpr int kMaxLogP = 12;      // pair bits of N <= kMaxN
+constexpr int kRecNarrow = kRecHead + kChunk / 2 + kChunkElems;  // 388 uint32 (1552 B)
+constexpr int kRecWide = kRecHead + kChunk + kChunkElems;        // 644 uint32 (2576 B)
 
 // Coefficient program as read by the kernels.
 struct DevExpr {
@@ -166,10 +164,12 @@ struct gfem_system {
   double2 *d_dc = nullptr;
   gfem::GroupTable *d_table = nullptr;
   // fused maps
-  uint32_t *d_rec = nullptr;  // chunk records, n_chunks x kRec
+  uint32_t *d_rec = nullptr;  // chunk records, n_chunks x rec_stride
   double2 *d_be_all = nullptr;  // element load vectors of all groups, [nrhs][be_total]
   int64_t be_total = 0;
-  int gather_pipe = 0;        // 1: persistent record-prefetching gather (GFEM_GATHER=pipe)
+  int rec_wide = 0;           // 32-bit contribution words
+  int gather_slots = 4;       // slots per gather thread: 4 (128-thread blocks) or 2 (256)
+  int rec_stride = gfem::kRecNarrow;  // uint32 per record
   int64_t gather_grid = 0;    // persistent grid (multiple of the XCD count)
   int64_t n_words = 0;
   int64_t n_elist = 0;

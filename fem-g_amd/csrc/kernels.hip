@@ -665,36 +665,32 @@ struct ValTraits<double2> {
 // ------------------------------------------------------------------ k_gather
 // Atomic-free CSR assembly, one 256-thread block per chunk of whole nonzeros
 // (<= kChunk contributions, <= kChunkElems distinct elements).  The chunk is
-// one fixed-stride record (header, words, element list), so the block issues
-// every load of its map at once, without a dependent header read.  Then:
+// one fixed-stride record (header, words in nonzero order, element list), so
+// the block issues every load of its map at once.  Then:
 //   1. the chunk's element factors F (FP64 geometry) are staged in LDS;
-//   2. every word (grouped by element, then local pair) is evaluated as
-//      F . T[pair] and parked in LDS at its nonzero-order slot;
-//   3. the first slot of each nonzero sums its run in slot order, i.e. in the
-//      element traversal order of the reference's sequential loop, and writes
-//      the value once.  No atomics, bitwise reproducible.
-constexpr int kGatherThreads = 256;
-static_assert(kRec <= 3 * kGatherThreads && kRec > 2 * kGatherThreads && kRecWordEnd == 2 * kGatherThreads,
-              "three record slots per thread, words only in the first two");
+//   2. thread t evaluates the words of slots 2t, 2t+1 as F . T[pair] into
+//      registers (the words are already in nonzero order);
+//   3. each nonzero's run of slots is summed by a segmented scan and written
+//      once.  No atomics; the order of additions is fixed, so results are
+//      bitwise reproducible.
 
 template <bool CPLX, bool MIXED>
 __device__ __forceinline__ void gather_eval(uint32_t w, int logP, uint32_t pmask,
                                             const typename std::conditional<MIXED, float, double>::type (*s_F)[kChunkElems],
                                             const void *const *s_T, const uint8_t *s_treal,
                                             const double2 *const *s_D,
-                                            typename std::conditional<MIXED, float, double>::type *s_re,
-                                            typename std::conditional<MIXED, float, double>::type *s_im,
-                                            uint8_t *s_head)
+                                            typename std::conditional<MIXED, float, double>::type &vr,
+                                            typename std::conditional<MIXED, float, double>::type &vi)
 {
   using R = typename std::conditional<MIXED, float, double>::type;
   const int pair = (int)(w & pmask);
-  const int le = (int)((w >> (logP + 4)) & 0x7Fu);
-  const int slot = (int)((w >> (logP + 11)) & 0x1FFu);
+  const int le = (int)((w >> logP) & 0x7Fu);
   R F[8];
 #pragma unroll
   for(int q = 0; q < 7; ++q) F[q] = s_F[q][le];  // structure-of-arrays: no bank conflicts across elements
   F[7] = 0;
-  R vr = 0, vi = 0;
+  vr = 0;
+  vi = 0;
   if(!CPLX || s_treal[le]) {
     if(MIXED) {
       const float4 *T = reinterpret_cast<const float4 *>(s_T[le]) + (int64_t)pair * 2;
@@ -732,35 +728,33 @@ __device__ __forceinline__ void gather_eval(uint32_t w, int logP, uint32_t pmask
     vr += (R)d.x;
     vi += (R)d.y;
   }
-  s_re[slot] = vr;
-  if(CPLX) s_im[slot] = vi;
-  s_head[slot] = (uint8_t)(w >> 31);
 }
 
-// Sums of the nonzeros' runs of slots and their output positions, for the
-// chunk whose values sit in LDS in nonzero order (s_head marks each run's
-// first slot).  Thread t owns slots 2t, 2t+1.  A run's sum is its slots in
-// this thread plus a carry from the threads to its right: the carry is a
-// segmented suffix scan over threads (shuffles within a wave, one LDS
+// Sums of the nonzeros' runs of slots and their output positions.  Thread t
+// holds S consecutive slots (values v, first-of-nonzero flags h; slots past
+// the chunk end are flagged as heads and not written).  A run's sum is its
+// slots in this thread plus a carry from the threads to its right: the carry
+// is a segmented suffix scan over threads (shuffles within a wave, one LDS
 // exchange across waves), so long runs cost log steps instead of a serial
 // walk.  The order of additions is fixed: results are bitwise reproducible.
 // Output position of a run = first nonzero of the chunk + rank of its head.
-template <bool CPLX, typename R>
-__device__ __forceinline__ void gather_runs(int t, uint32_t clen, uint32_t ns, const R *s_re, const R *s_im,
-                                            const uint8_t *s_head, int *s_wsum, R *s_wx, R *s_wy, int *s_wf,
+template <bool CPLX, typename R, int S, int B>
+__device__ __forceinline__ void gather_runs(int t, const int *h, const bool *in, const R *vr, const R *vi,
+                                            uint32_t ns, int *s_wsum, R *s_wx, R *s_wy, int *s_wf,
                                             void *__restrict__ out)
 {
-  constexpr int kW = kGatherThreads / 64;
-  const int s0 = 2 * t;
-  const bool in0 = s0 < (int)clen, in1 = s0 + 1 < (int)clen;
-  // slots past the chunk end count as closed heads so nothing flows across
-  const int h0 = in0 ? s_head[s0] : 1, h1 = in1 ? s_head[s0 + 1] : 1;
-  const R v0r = in0 ? s_re[s0] : (R)0, v1r = in1 ? s_re[s0 + 1] : (R)0;
-  const R v0i = (CPLX && in0) ? s_im[s0] : (R)0, v1i = (CPLX && in1) ? s_im[s0 + 1] : (R)0;
+  constexpr int kW = B / 64;
   // part of this thread's slots that continues the run from the left
-  R xr = h0 ? (R)0 : (h1 ? v0r : v0r + v1r);
-  R xi = h0 ? (R)0 : (h1 ? v0i : v0i + v1i);
-  int f = (h0 | h1) ? 1 : 0;
+  R xr = 0, xi = 0;
+  int f = 0;
+#pragma unroll
+  for(int k = 0; k < S; ++k) {
+    if(h[k]) f = 1;
+    if(!f) {
+      xr += vr[k];
+      xi += vi[k];
+    }
+  }
   const int lane = t & 63, wv = t >> 6;
 #pragma unroll
   for(int d = 1; d < 64; d <<= 1) {
@@ -774,7 +768,9 @@ __device__ __forceinline__ void gather_runs(int t, uint32_t clen, uint32_t ns, c
     }
   }
   // ranks: inclusive scan of the (in-range) head counts
-  const int cnt = (in0 ? h0 : 0) + (in1 ? h1 : 0);
+  int cnt = 0;
+#pragma unroll
+  for(int k = 0; k < S; ++k) cnt += (in[k] && h[k]) ? 1 : 0;
   int incl = cnt;
 #pragma unroll
   for(int d = 1; d < 64; d <<= 1) {
@@ -788,7 +784,7 @@ __device__ __forceinline__ void gather_runs(int t, uint32_t clen, uint32_t ns, c
     s_wf[wv] = f;
   }
   __syncthreads();
-  int rank = incl - cnt;
+  int rank = incl;  // one past this thread's last head
   for(int k = 0; k < wv; ++k) rank += s_wsum[k];
   // carry into this wave from the waves to its right
   R cr = 0, ci = 0;
@@ -807,39 +803,46 @@ __device__ __forceinline__ void gather_runs(int t, uint32_t clen, uint32_t ns, c
     xi += ci;
   }
   // carry into this thread = suffix value of the next thread
-  R nr = __shfl_down(xr, 1, 64), ni = CPLX ? __shfl_down(xi, 1, 64) : (R)0;
+  R ar = __shfl_down(xr, 1, 64), ai = CPLX ? __shfl_down(xi, 1, 64) : (R)0;
   if(lane == 63) {
-    nr = cr;
-    ni = ci;
+    ar = cr;
+    ai = ci;
   }
-  auto put = [&](int64_t pos, R re, R im) {
-    if(CPLX) {
-      if(std::is_same<R, float>::value)
-        reinterpret_cast<float2 *>(out)[pos] = make_float2((float)re, (float)im);
-      else
-        reinterpret_cast<double2 *>(out)[pos] = make_double2((double)re, (double)im);
+  // runs inside the thread, right to left
+#pragma unroll
+  for(int k = S - 1; k >= 0; --k) {
+    ar = vr[k] + ar;
+    ai = vi[k] + ai;
+    if(h[k]) {
+      if(in[k]) {
+        const int64_t pos = (int64_t)ns + (--rank);
+        if(CPLX) {
+          if(std::is_same<R, float>::value)
+            reinterpret_cast<float2 *>(out)[pos] = make_float2((float)ar, (float)ai);
+          else
+            reinterpret_cast<double2 *>(out)[pos] = make_double2((double)ar, (double)ai);
+        }
+        else {
+          reinterpret_cast<R *>(out)[pos] = ar;
+        }
+      }
+      ar = 0;
+      ai = 0;
     }
-    else {
-      reinterpret_cast<R *>(out)[pos] = re;
-    }
-  };
-  if(in0 && h0) {
-    if(h1)
-      put((int64_t)ns + rank, v0r, v0i);
-    else
-      put((int64_t)ns + rank, v0r + (v1r + nr), v0i + (v1i + ni));
-    ++rank;
   }
-  if(in1 && h1) put((int64_t)ns + rank, v1r + nr, v1i + ni);
 }
 
-template <bool CPLX, bool MIXED>
-__global__ void __launch_bounds__(kGatherThreads)
+// S slots per thread, kChunk / S threads per block
+template <bool CPLX, bool MIXED, bool WIDE, int S>
+__global__ void __launch_bounds__(kChunk / S)
   k_gather(const GroupTable *__restrict__ gtp, const uint32_t *__restrict__ recs, int64_t nchunks,
            void *__restrict__ out)
 {
   using R = typename std::conditional<MIXED, float, double>::type;
-  static_assert(kChunk == 2 * kGatherThreads, "two slots per thread in the run sums");
+  constexpr int B = kChunk / S;
+  static_assert(B >= kChunkElems && (S == 2 || S == 4), "one list entry per thread at most");
+  constexpr int kStride = WIDE ? kRecWide : kRecNarrow;
+  constexpr int kList = kStride - kChunkElems;  // element-list region of a record
   __shared__ const void *s_gF[kMaxGroups];
   __shared__ const void *s_gT[kMaxGroups];
   __shared__ const double2 *s_gD[kMaxGroups];
@@ -849,21 +852,39 @@ __global__ void __launch_bounds__(kGatherThreads)
   __shared__ const void *s_T[kChunkElems];
   __shared__ const double2 *s_D[kChunkElems];
   __shared__ uint8_t s_treal[kChunkElems];
-  __shared__ R s_re[kChunk];
-  __shared__ R s_im[CPLX ? kChunk : 1];
-  __shared__ uint8_t s_head[kChunk];
-  __shared__ int s_wsum[kGatherThreads / 64];
-  __shared__ R s_wx[kGatherThreads / 64], s_wy[kGatherThreads / 64];
-  __shared__ int s_wf[kGatherThreads / 64];
+  __shared__ int s_wsum[B / 64];
+  __shared__ R s_wx[B / 64], s_wy[B / 64];
+  __shared__ int s_wf[B / 64];
   const GroupTable &gt = *gtp;
   const int t = threadIdx.x;
   const int64_t c = xcd_remap(blockIdx.x, gridDim.x);
   if(c >= nchunks) return;  // uniform per block
-  // the whole record in one round of loads: slots t, t + 256, t + 512
-  const uint32_t *rec = recs + c * kRec;
+  // the whole record in one round of loads: words of slots S t .. S t + S - 1
+  const uint32_t *rec = recs + c * kStride;
   const uint32_t ns = rec[0], clen = rec[1], nel = rec[2];
-  const uint32_t r0 = rec[t], r1 = rec[t + kGatherThreads];
-  const uint32_t r2 = t < kRec - 2 * kGatherThreads ? rec[t + 2 * kGatherThreads] : 0u;
+  uint32_t w[S];
+  if(WIDE) {
+    if(S == 4) {
+      const uint4 p = reinterpret_cast<const uint4 *>(rec + kRecHead)[t];
+      w[0] = p.x; w[1] = p.y; w[2 % S] = p.z; w[3 % S] = p.w;
+    }
+    else {
+      const uint2 p = reinterpret_cast<const uint2 *>(rec + kRecHead)[t];
+      w[0] = p.x; w[1] = p.y;
+    }
+  }
+  else {
+    if(S == 4) {
+      const uint2 p = reinterpret_cast<const uint2 *>(rec + kRecHead)[t];
+      w[0] = p.x & 0xFFFFu; w[1] = p.x >> 16; w[2 % S] = p.y & 0xFFFFu; w[3 % S] = p.y >> 16;
+    }
+    else {
+      const uint32_t p = rec[kRecHead + t];
+      w[0] = p & 0xFFFFu;
+      w[1] = p >> 16;
+    }
+  }
+  const uint32_t ev = t < kChunkElems ? rec[kList + t] : 0u;
   const int logP = gt.logP;
   const uint32_t pmask = (1u << logP) - 1u;
   if(t < gt.ng) {
@@ -876,9 +897,9 @@ __global__ void __launch_bounds__(kGatherThreads)
     s_gN[t] = gt.N[t];
   }
   __syncthreads();
-  // element factors: the element list sits in record slots [kRec - nel, kRec)
-  const uint32_t e_lo = (uint32_t)kRec - nel;
-  auto stage = [&](uint32_t ev, int j) {
+  // element factors: entry j of the list sits in record slot kStride - nel + j
+  const int j = t - (kChunkElems - (int)nel);
+  if(t < kChunkElems && j >= 0) {
     const int g = (int)(ev >> 27);
     const int64_t el = ev & 0x7FFFFFFu;
     s_treal[j] = s_gR[g];
@@ -900,181 +921,21 @@ __global__ void __launch_bounds__(kGatherThreads)
         s_F[2 * q + 1][j] = (R)f.y;
       }
     }
-  };
-  // slot t never holds the list: nel <= kChunkElems < kRec - kGatherThreads
-  if((uint32_t)(t + kGatherThreads) >= e_lo) stage(r1, t + kGatherThreads - (int)e_lo);
-  if(t < kRec - 2 * kGatherThreads && (uint32_t)(t + 2 * kGatherThreads) >= e_lo)
-    stage(r2, t + 2 * kGatherThreads - (int)e_lo);
-  __syncthreads();
-  // words sit in record slots [kRecHead, kRecHead + clen), clen <= kChunkWords
-  const uint32_t w_hi = (uint32_t)kRecHead + clen;
-  if((uint32_t)t >= (uint32_t)kRecHead && (uint32_t)t < w_hi)
-    gather_eval<CPLX, MIXED>(r0, logP, pmask, s_F, s_T, s_treal, s_D, s_re, s_im, s_head);
-  if((uint32_t)(t + kGatherThreads) < w_hi)
-    gather_eval<CPLX, MIXED>(r1, logP, pmask, s_F, s_T, s_treal, s_D, s_re, s_im, s_head);
-  __syncthreads();
-
-  gather_runs<CPLX, R>(t, clen, ns, s_re, s_im, s_head, s_wsum, s_wx, s_wy, s_wf, out);
-}
-
-// Persistent variant of the same gather.  A grid of resident blocks, the
-// blocks of XCD x walking the chunks [x*per, (x+1)*per) with stride
-// P = grid / XCDs.  Chunk records are prefetched two chunks ahead; the real
-// reference-tensor rows of a block's words (group taken from the word) and
-// the chunk's element factors are requested before the record two chunks
-// ahead, so that record stays in flight through the whole evaluation of the
-// current chunk (vector-memory counters complete in issue order).
-template <bool CPLX, bool MIXED>
-__global__ void __launch_bounds__(kGatherThreads)
-  k_gather_pipe(const GroupTable *__restrict__ gtp, const uint32_t *__restrict__ recs, int64_t nchunks,
-                int64_t per_xcd, void *__restrict__ out)
-{
-  using R = typename std::conditional<MIXED, float, double>::type;
-  using TV = typename std::conditional<MIXED, float4, double2>::type;
-  constexpr int kTV = MIXED ? 2 : 4;  // vectors of one real tensor row (8 values)
-  constexpr int kFV = MIXED ? 2 : 4;  // vectors of one element's factors (8 values)
-  __shared__ const void *s_gF[kMaxGroups];
-  __shared__ const void *s_gTr[kMaxGroups];
-  __shared__ const void *s_gTc[kMaxGroups];
-  __shared__ const double2 *s_gD[kMaxGroups];
-  __shared__ int s_gN[kMaxGroups];
-  __shared__ uint8_t s_gR[kMaxGroups];
-  __shared__ R s_F[8][kChunkElems];
-  __shared__ const double2 *s_D[kChunkElems];
-  __shared__ R s_re[kChunk];
-  __shared__ R s_im[CPLX ? kChunk : 1];
-  __shared__ uint8_t s_head[kChunk];
-  __shared__ int s_wsum[kGatherThreads / 64];
-  __shared__ R s_wx[kGatherThreads / 64], s_wy[kGatherThreads / 64];
-  __shared__ int s_wf[kGatherThreads / 64];
-  const GroupTable &gt = *gtp;
-  const int t = threadIdx.x;
-  const int logP = gt.logP;
-  const uint32_t pmask = (1u << logP) - 1u;
-  if(t < gt.ng) {
-    s_gR[t] = (!CPLX || gt.treal[t]) ? 1 : 0;
-    s_gF[t] = MIXED ? (const void *)gt.F32[t] : (const void *)gt.F64[t];
-    s_gTr[t] = MIXED ? (const void *)gt.T32r[t] : (const void *)gt.T64r[t];
-    s_gTc[t] = MIXED ? (const void *)gt.T32[t] : (const void *)gt.T64[t];
-    s_gD[t] = gt.dense[t];
-    s_gN[t] = gt.N[t];
   }
   __syncthreads();
-  const int64_t P = gridDim.x / kXcds;
-  const int64_t cb = (int64_t)(blockIdx.x % kXcds) * per_xcd;
-  const int64_t ce = min(nchunks, cb + per_xcd);
-  int64_t c = cb + blockIdx.x / kXcds;
-  uint32_t a0 = 0, a1 = 0, a2 = 0, b0 = 0, b1 = 0, b2 = 0;
-  auto fetch = [&](int64_t cc, uint32_t &x0, uint32_t &x1, uint32_t &x2) {
-    if(cc < ce) {
-      const uint32_t *r = recs + cc * kRec;
-      x0 = r[t];
-      x1 = r[t + kGatherThreads];
-      x2 = t < kRec - 2 * kGatherThreads ? r[t + 2 * kGatherThreads] : 0u;
-    }
-  };
-  fetch(c, a0, a1, a2);
-  fetch(c + P, b0, b1, b2);
-  for(; c < ce; c += P) {
-    const uint32_t *rec = recs + c * kRec;
-    const uint32_t ns = rec[0], clen = rec[1], nel = rec[2];
-    const uint32_t e_lo = (uint32_t)kRec - nel, w_hi = (uint32_t)kRecHead + clen;
-    const bool h0 = (uint32_t)t >= (uint32_t)kRecHead && (uint32_t)t < w_hi;
-    const bool h1 = (uint32_t)(t + kGatherThreads) < w_hi;
-    // (1) real tensor rows of this thread's two words
-    TV ta[kTV], tb[kTV];
-    auto pref_T = [&](uint32_t w, bool has, TV *tv) {
-      const int g = (int)((w >> logP) & 0xFu);
-      if(has && s_gR[g]) {
-        const TV *T = reinterpret_cast<const TV *>(s_gTr[g]) + (int64_t)(w & pmask) * kTV;
+  const uint32_t hb = WIDE ? 31u : 15u;
+  R vr[S], vi[S];
+  int h[S];
+  bool in[S];
 #pragma unroll
-        for(int q = 0; q < kTV; ++q) tv[q] = T[q];
-      }
-    };
-    pref_T(a0, h0, ta);
-    pref_T(a1, h1, tb);
-    // (2) element factors of the list entries held in slots a1 / a2
-    const bool e1 = (uint32_t)(t + kGatherThreads) >= e_lo;
-    const bool e2 = t < kRec - 2 * kGatherThreads && (uint32_t)(t + 2 * kGatherThreads) >= e_lo;
-    TV f1[kFV], f2[kFV];
-    auto load_F = [&](uint32_t ev, bool has, TV *f) {
-      if(has) {
-        const TV *F = reinterpret_cast<const TV *>(reinterpret_cast<const R *>(s_gF[ev >> 27]) +
-                                                   (int64_t)(ev & 0x7FFFFFFu) * 8);
-#pragma unroll
-        for(int q = 0; q < kFV; ++q) f[q] = F[q];
-      }
-    };
-    load_F(a1, e1, f1);
-    load_F(a2, e2, f2);
-    // (3) the record two chunks ahead
-    uint32_t n0 = 0, n1 = 0, n2 = 0;
-    fetch(c + 2 * P, n0, n1, n2);
-    // (4) stage the element factors
-    auto store_F = [&](uint32_t ev, const TV *f, int j) {
-      const int g = (int)(ev >> 27);
-      const int64_t el = ev & 0x7FFFFFFu;
-      const int N = s_gN[g];
-      s_D[j] = s_gD[g] ? s_gD[g] + el * N * N : nullptr;
-      const R *fv = reinterpret_cast<const R *>(f);
-#pragma unroll
-      for(int q = 0; q < 8; ++q) s_F[q][j] = fv[q];
-    };
-    if(e1) store_F(a1, f1, t + kGatherThreads - (int)e_lo);
-    if(e2) store_F(a2, f2, t + 2 * kGatherThreads - (int)e_lo);
-    __syncthreads();
-    // (5) evaluate F . T[pair] into the nonzero-order slots
-    auto eval = [&](uint32_t w, const TV *tv) {
-      const int pair = (int)(w & pmask);
-      const int g = (int)((w >> logP) & 0xFu);
-      const int le = (int)((w >> (logP + 4)) & 0x7Fu);
-      const int slot = (int)((w >> (logP + 11)) & 0x1FFu);
-      R F[8];
-#pragma unroll
-      for(int q = 0; q < 7; ++q) F[q] = s_F[q][le];
-      F[7] = 0;
-      R vr = 0, vi = 0;
-      if(!CPLX || s_gR[g]) {
-        const R *tr = reinterpret_cast<const R *>(tv);
-#pragma unroll
-        for(int q = 0; q < 7; ++q) vr += F[q] * tr[q];
-      }
-      else if(MIXED) {
-        const float4 *T = reinterpret_cast<const float4 *>(s_gTc[g]) + (int64_t)pair * 4;
-#pragma unroll
-        for(int q = 0; q < 4; ++q) {
-          const float4 tq = T[q];
-          vr += F[2 * q] * tq.x + F[2 * q + 1] * tq.z;
-          vi += F[2 * q] * tq.y + F[2 * q + 1] * tq.w;
-        }
-      }
-      else {
-        const double2 *T = reinterpret_cast<const double2 *>(s_gTc[g]) + (int64_t)pair * 8;
-#pragma unroll
-        for(int q = 0; q < 7; ++q) {
-          const double2 tq = T[q];
-          vr += F[q] * (R)tq.x;
-          vi += F[q] * (R)tq.y;
-        }
-      }
-      if(s_D[le]) {
-        const double2 d = s_D[le][pair];
-        vr += (R)d.x;
-        vi += (R)d.y;
-      }
-      s_re[slot] = vr;
-      if(CPLX) s_im[slot] = vi;
-      s_head[slot] = (uint8_t)(w >> 31);
-    };
-    if(h0) eval(a0, ta);
-    if(h1) eval(a1, tb);
-    __syncthreads();
-    // (6) run sums, one store per nonzero
-    gather_runs<CPLX, R>(t, clen, ns, s_re, s_im, s_head, s_wsum, s_wx, s_wy, s_wf, out);
-    __syncthreads();  // LDS is reused by the next chunk
-    a0 = b0; a1 = b1; a2 = b2;
-    b0 = n0; b1 = n1; b2 = n2;
+  for(int k = 0; k < S; ++k) {
+    in[k] = S * t + k < (int)clen;
+    h[k] = in[k] ? (int)((w[k] >> hb) & 1u) : 1;  // past the end: closed, not written
+    vr[k] = 0;
+    vi[k] = 0;
+    if(in[k]) gather_eval<CPLX, MIXED>(w[k], logP, pmask, s_F, s_T, s_treal, s_D, vr[k], vi[k]);
   }
+  gather_runs<CPLX, R, S, B>(t, h, in, vr, vi, ns, s_wsum, s_wx, s_wy, s_wf, out);
 }
 
 // ------------------------------------------------------------------ k_rhs
@@ -1360,52 +1221,45 @@ hipError_t launch_dense(const Group &g, const double *xyz, bool all_terms, hipSt
   return hipGetLastError();
 }
 
-template <bool CPLX, bool MIXED>
-hipError_t gather_pipe_launch(const gfem_system &sys, void *out, hipStream_t s)
-{
-  // resident grid: blocks per CU (occupancy) x CUs, a multiple of the XCD count
-  static int64_t grid[2][2] = {{0, 0}, {0, 0}};
-  int64_t &G = grid[CPLX][MIXED];
-  if(G == 0) {
-    int per_cu = 0, cus = 0, dev = 0;
-    hipError_t e = hipGetDevice(&dev);
-    if(e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    if(e == hipSuccess)
-      e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_gather_pipe<CPLX, MIXED>, kGatherThreads, 0);
-    if(e != hipSuccess) return e;
-    G = std::max<int64_t>(kXcds, (int64_t)per_cu * cus / kXcds * kXcds);
-  }
-  const int64_t per_xcd = (sys.n_chunks + kXcds - 1) / kXcds;
-  hipLaunchKernelGGL((k_gather_pipe<CPLX, MIXED>), dim3((unsigned)G), dim3(kGatherThreads), 0, s, sys.d_table,
-                     sys.d_rec, sys.n_chunks, per_xcd, out);
-  return hipGetLastError();
-}
-
 hipError_t launch_gather(const gfem_system &sys, int prec, hipStream_t s)
 {
   if(sys.n_chunks == 0) return hipSuccess;
   void *out = sys.d_vals[prec];
-  if(sys.gather_pipe) {
-    if(sys.cplx)
-      return prec ? gather_pipe_launch<true, true>(sys, out, s) : gather_pipe_launch<true, false>(sys, out, s);
-    return prec ? gather_pipe_launch<false, true>(sys, out, s) : gather_pipe_launch<false, false>(sys, out, s);
-  }
   const GroupTable *gt = sys.d_table;
-  const dim3 grid((unsigned)xcd_grid(sys.n_chunks)), block(kGatherThreads);
-#define GFEM_GATHER_ARGS gt, sys.d_rec, sys.n_chunks, out
+  const dim3 grid((unsigned)xcd_grid(sys.n_chunks));
+#define GFEM_GATHER_S(C, M, W, S)                                                                            \
+  hipLaunchKernelGGL((k_gather<C, M, W, S>), grid, dim3(kChunk / S), 0, s, gt, sys.d_rec, sys.n_chunks, out)
+#define GFEM_GATHER(C, M)                                                                                    \
+  if(sys.rec_wide) {                                                                                         \
+    if(sys.gather_slots == 4)                                                                                \
+      GFEM_GATHER_S(C, M, true, 4);                                                                          \
+    else                                                                                                     \
+      GFEM_GATHER_S(C, M, true, 2);                                                                          \
+  }                                                                                                          \
+  else {                                                                                                     \
+    if(sys.gather_slots == 4)                                                                                \
+      GFEM_GATHER_S(C, M, false, 4);                                                                         \
+    else                                                                                                     \
+      GFEM_GATHER_S(C, M, false, 2);                                                                         \
+  }
   if(sys.cplx) {
-    if(prec)
-      hipLaunchKernelGGL((k_gather<true, true>), grid, block, 0, s, GFEM_GATHER_ARGS);
-    else
-      hipLaunchKernelGGL((k_gather<true, false>), grid, block, 0, s, GFEM_GATHER_ARGS);
+    if(prec) {
+      GFEM_GATHER(true, true)
+    }
+    else {
+      GFEM_GATHER(true, false)
+    }
   }
   else {
-    if(prec)
-      hipLaunchKernelGGL((k_gather<false, true>), grid, block, 0, s, GFEM_GATHER_ARGS);
-    else
-      hipLaunchKernelGGL((k_gather<false, false>), grid, block, 0, s, GFEM_GATHER_ARGS);
+    if(prec) {
+      GFEM_GATHER(false, true)
+    }
+    else {
+      GFEM_GATHER(false, false)
+    }
   }
-#undef GFEM_GATHER_ARGS
+#undef GFEM_GATHER
+#undef GFEM_GATHER_S
   return hipGetLastError();
 }
 
