@@ -46,7 +46,8 @@ def build(verbose=False):
     import pybind11
     py_inc = sysconfig.get_paths()["include"]
 
-    dev_objs = [os.path.join(OBJ, "kernels.o"), os.path.join(OBJ, "gfem_api.o"), os.path.join(OBJ, "jit.o")]
+    dev_objs = [os.path.join(OBJ, "kernels.o"), os.path.join(OBJ, "gfem_api.o"), os.path.join(OBJ, "jit.o"),
+                os.path.join(OBJ, "solver.o")]
     host_srcs = ["model.cpp", "gmshfem.cpp"]
     host_objs = [os.path.join(OBJ, s.replace(".cpp", ".o")) for s in host_srcs]
     py_obj = os.path.join(OBJ, "pybind.o")
@@ -55,6 +56,7 @@ def build(verbose=False):
         [HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "kernels.hip"), "-o", dev_objs[0]],
         [HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "gfem_api.cpp"), "-o", dev_objs[1]],
         [HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "jit.cpp"), "-o", dev_objs[2]],
+        [HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "solver.hip"), "-o", dev_objs[3]],
     ]
     jobs += [["g++", *CXX_FLAGS, "-c", os.path.join(HOST, s), "-o", o] for s, o in zip(host_srcs, host_objs)]
     jobs.append(["g++", *CXX_FLAGS, "-fvisibility=hidden", "-I", pybind11.get_include(), "-I", py_inc,

@@ -18,8 +18,10 @@
 using namespace gfem;
 
 namespace {
-
 thread_local std::string g_err;
+}  // namespace
+
+namespace gfem {
 
 gfem_status fail(gfem_ctx *ctx, gfem_status s, const std::string &msg)
 {
@@ -34,35 +36,9 @@ gfem_status hip_fail(gfem_ctx *ctx, hipError_t e, const char *what)
   return fail(ctx, s, std::string(what) + ": " + hipGetErrorString(e));
 }
 
-#define GFEM_HIP(ctx, call)                                                                                           \
-  do {                                                                                                                \
-    hipError_t e_ = (call);                                                                                           \
-    if(e_ != hipSuccess) return hip_fail(ctx, e_, #call);                                                             \
-  } while(0)
+}  // namespace gfem
 
-template <typename T>
-hipError_t dalloc(T **p, size_t count)
-{
-  *p = nullptr;
-  if(count == 0) count = 1;
-  return hipMalloc(reinterpret_cast<void **>(p), count * sizeof(T));
-}
-
-template <typename T>
-hipError_t dupload(T **p, const T *h, size_t count)
-{
-  hipError_t e = dalloc(p, count);
-  if(e != hipSuccess) return e;
-  if(count) e = hipMemcpy(*p, h, count * sizeof(T), hipMemcpyHostToDevice);
-  return e;
-}
-
-template <typename T>
-void dfree(T *&p)
-{
-  if(p) (void)hipFree(p);
-  p = nullptr;
-}
+namespace {
 
 void free_group(Group &g)
 {
@@ -245,9 +221,31 @@ gfem_status gfem_system_create(gfem_ctx *ctx, gfem_scalar scalar, int64_t n, con
   sys->h_dc.assign(values_dc ? values_dc : nullptr, values_dc ? values_dc + 2 * n_fixed : nullptr);
   if(sys->h_dc.empty()) sys->h_dc.assign(2, 0.);
   hipError_t e = dupload(&sys->d_dc, reinterpret_cast<const double2 *>(sys->h_dc.data()), sys->h_dc.size() / 2);
+  if(e == hipSuccess) e = dupload(&sys->d_ai, ai, (size_t)n + 1);
+  if(e == hipSuccess) {
+    // the pattern on the device (solver SpMV), column indices as int32
+    std::vector<int32_t> aj32((size_t)std::max<int64_t>(nnz, 1), 0);
+    for(int64_t k = 0; k < nnz; ++k) aj32[k] = (int32_t)aj[k];
+    e = dupload(&sys->d_aj, aj32.data(), aj32.size());
+  }
   if(e != hipSuccess) {
+    dfree(sys->d_dc);
+  dfree(sys->d_ai);
+  dfree(sys->d_aj);
+  dfree(sys->d_sol_x);
+  dfree(sys->d_sol_r);
+  dfree(sys->d_sol_w);
+  dfree(sys->d_sol_V);
+  dfree(sys->d_sol_a32);
+  dfree(sys->d_sol_state);
+  dfree(sys->d_sol_y);
+  dfree(sys->d_sol_part);
+  dfree(sys->d_sol_nrm);
+  if(sys->h_sol_est) (void)hipHostFree(sys->h_sol_est);
+    dfree(sys->d_ai);
+    dfree(sys->d_aj);
     delete sys;
-    return hip_fail(ctx, e, "upload values_dc");
+    return hip_fail(ctx, e, "upload values_dc / pattern");
   }
   *out = sys;
   return GFEM_OK;
@@ -260,6 +258,18 @@ gfem_status gfem_system_destroy(gfem_system *sys)
   clear_prof(sys);
   for(auto &g : sys->groups) free_group(g);
   dfree(sys->d_dc);
+  dfree(sys->d_ai);
+  dfree(sys->d_aj);
+  dfree(sys->d_sol_x);
+  dfree(sys->d_sol_r);
+  dfree(sys->d_sol_w);
+  dfree(sys->d_sol_V);
+  dfree(sys->d_sol_a32);
+  dfree(sys->d_sol_state);
+  dfree(sys->d_sol_y);
+  dfree(sys->d_sol_part);
+  dfree(sys->d_sol_nrm);
+  if(sys->h_sol_est) (void)hipHostFree(sys->h_sol_est);
   dfree(sys->d_table);
   dfree(sys->d_rec);
   dfree(sys->d_be_all);
@@ -817,6 +827,40 @@ gfem_status gfem_assemble(gfem_system *sys, gfem_precision prec, gfem_strategy s
     GFEM_HIP(ctx, launch_ref_rhs(*sys, g, s));
     prof.end();
   }
+  return GFEM_OK;
+}
+
+gfem_status gfem_solve(gfem_system *sys, int rhs_idx, double tol, int restart, int max_iter, void *stream,
+                       gfem_solve_info *info)
+{
+  if(!sys) return fail(nullptr, GFEM_ERR_INVALID_ARG, "system is null");
+  gfem_ctx *ctx = sys->ctx;
+  if(!sys->finalized || !sys->d_vals[0]) return fail(ctx, GFEM_ERR_STATE, "solve before an FP64 assembly");
+  if(rhs_idx < 0 || rhs_idx >= sys->nrhs) return fail(ctx, GFEM_ERR_INVALID_ARG, "rhs_idx");
+  if(!(tol > 0.) || restart < 1 || restart > 1000 || max_iter < 1)
+    return fail(ctx, GFEM_ERR_INVALID_ARG, "solve: tol > 0, 1 <= restart <= 1000, max_iter >= 1");
+  if(sys->symmetric) return fail(ctx, GFEM_ERR_UNSUPPORTED, "solve: symmetric storage");
+  GFEM_HIP(ctx, hipSetDevice(ctx->device));
+  return solve_gmres_ir(sys, rhs_idx, tol, restart, max_iter, reinterpret_cast<hipStream_t>(stream), info);
+}
+
+gfem_status gfem_solution_download(gfem_system *sys, void *host, size_t bytes)
+{
+  if(!sys || !host) return fail(sys ? sys->ctx : nullptr, GFEM_ERR_INVALID_ARG, "solution_download: null argument");
+  gfem_ctx *ctx = sys->ctx;
+  if(!sys->sol_valid) return fail(ctx, GFEM_ERR_STATE, "no solution yet");
+  const size_t need = value_bytes(sys, 0) * (size_t)sys->n;
+  if(bytes < need) return fail(ctx, GFEM_ERR_INVALID_ARG, "solution_download: buffer too small");
+  GFEM_HIP(ctx, hipSetDevice(ctx->device));
+  GFEM_HIP(ctx, hipMemcpy(host, sys->d_sol_x, need, hipMemcpyDeviceToHost));
+  return GFEM_OK;
+}
+
+gfem_status gfem_solution_device(gfem_system *sys, void **dev_x)
+{
+  if(!sys || !dev_x) return fail(sys ? sys->ctx : nullptr, GFEM_ERR_INVALID_ARG, "solution_device: null argument");
+  if(!sys->sol_valid) return fail(sys->ctx, GFEM_ERR_STATE, "no solution yet");
+  *dev_x = sys->d_sol_x;
   return GFEM_OK;
 }
 

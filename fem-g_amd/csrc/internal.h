@@ -182,6 +182,14 @@ struct gfem_system {
   void *d_vals[2] = {nullptr, nullptr};
   void *d_rhs[2] = {nullptr, nullptr};
   bool built_ref = false;
+  // solver (solver.hip): CSR pattern on the device, workspace kept between solves
+  int64_t *d_ai = nullptr;
+  int32_t *d_aj = nullptr;
+  void *d_sol_x = nullptr, *d_sol_r = nullptr, *d_sol_w = nullptr, *d_sol_V = nullptr, *d_sol_a32 = nullptr;
+  double2 *d_sol_state = nullptr, *d_sol_y = nullptr, *d_sol_part = nullptr, *d_sol_nrm = nullptr;
+  void *h_sol_est = nullptr;  // pinned
+  int sol_m = 0;
+  bool sol_valid = false;
   // profiling
   bool profiling = false;
   std::vector<const char *> prof_names;
@@ -190,6 +198,45 @@ struct gfem_system {
 };
 
 namespace gfem {
+
+// Error reporting of the C ABI (gfem_api.cpp): record `msg` as the last
+// error of `ctx` (and of the calling thread) and return `s`.
+gfem_status fail(gfem_ctx *ctx, gfem_status s, const std::string &msg);
+gfem_status hip_fail(gfem_ctx *ctx, hipError_t e, const char *what);
+
+#define GFEM_HIP(ctx, call)                                                                                           \
+  do {                                                                                                                \
+    hipError_t e_ = (call);                                                                                           \
+    if(e_ != hipSuccess) return ::gfem::hip_fail(ctx, e_, #call);                                                     \
+  } while(0)
+
+template <typename T>
+hipError_t dalloc(T **p, size_t count)
+{
+  *p = nullptr;
+  if(count == 0) count = 1;
+  return hipMalloc(reinterpret_cast<void **>(p), count * sizeof(T));
+}
+
+template <typename T>
+hipError_t dupload(T **p, const T *h, size_t count)
+{
+  hipError_t e = dalloc(p, count);
+  if(e != hipSuccess) return e;
+  if(count) e = hipMemcpy(*p, h, count * sizeof(T), hipMemcpyHostToDevice);
+  return e;
+}
+
+template <typename T>
+void dfree(T *&p)
+{
+  if(p) (void)hipFree(p);
+  p = nullptr;
+}
+
+// GMRES with FP64 iterative refinement on the last FP64 assembly (solver.hip)
+gfem_status solve_gmres_ir(gfem_system *sys, int rhs_idx, double tol, int restart, int max_iter, hipStream_t s,
+                           gfem_solve_info *info);
 
 // Launchers implemented in kernels.hip.  All are asynchronous on `s`.
 hipError_t launch_basis(const Group &g, Rule &r, hipStream_t s);

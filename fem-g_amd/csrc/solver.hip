@@ -1,0 +1,504 @@
+// Device solver for A x = -b (the reference's sign convention,
+// fem-master/src/system/Solver.cpp:119-123), in place of the reference's
+// PETSc/MUMPS LU (Solver.cpp:76-160).
+//
+// GMRES with iterative refinement: each outer step computes the true residual
+// r = -b - A x in FP64 from the FP64 matrix, runs one restart cycle of
+// GMRES(m) on A d = r in FP32 (matrix values rounded once to FP32, Krylov
+// basis in FP32), and updates x += d in FP64.  Inner products are accumulated
+// in FP64 with classical Gram-Schmidt applied twice (CGS2); every reduction
+// sums block partials in a fixed order, so a solve is bitwise reproducible.
+// The Hessenberg matrix and its Givens rotations stay on the device; the
+// host reads the residual estimate every few iterations to stop a cycle.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <vector>
+
+#include "internal.h"
+
+namespace gfem {
+
+namespace {
+
+constexpr int kRowLanes = 8;       // lanes per CSR row in the SpMV kernels
+constexpr int kRedBlocks = 512;    // partial sums per reduction
+constexpr int kRedThreads = 256;
+constexpr int kDotGroup = 4;       // basis vectors per pass of the dot kernel
+
+template <bool C>
+struct Sc;
+template <>
+struct Sc<true> {
+  using f32 = float2;
+  using f64 = double2;
+};
+template <>
+struct Sc<false> {
+  using f32 = float;
+  using f64 = double;
+};
+
+__device__ __forceinline__ double2 to_d(float2 v) { return make_double2(v.x, v.y); }
+__device__ __forceinline__ double to_d(float v) { return v; }
+__device__ __forceinline__ double2 ld_d(const double2 *p, int64_t i) { return p[i]; }
+__device__ __forceinline__ double2 ld_d(const double *p, int64_t i) { return make_double2(p[i], 0.); }
+__device__ __forceinline__ double2 ld_d(const float2 *p, int64_t i) { return make_double2(p[i].x, p[i].y); }
+__device__ __forceinline__ double2 ld_d(const float *p, int64_t i) { return make_double2(p[i], 0.); }
+__device__ __forceinline__ void st(double2 *p, int64_t i, double2 v) { p[i] = v; }
+__device__ __forceinline__ void st(double *p, int64_t i, double2 v) { p[i] = v.x; }
+__device__ __forceinline__ void st(float2 *p, int64_t i, double2 v) { p[i] = make_float2((float)v.x, (float)v.y); }
+__device__ __forceinline__ void st(float *p, int64_t i, double2 v) { p[i] = (float)v.x; }
+
+__device__ __forceinline__ double2 zmul(double2 a, double2 b)
+{
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 zconjmul(double2 a, double2 b)  // conj(a) b
+{
+  return make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
+}
+
+__device__ __forceinline__ double2 lanes_sum(double2 v)
+{
+#pragma unroll
+  for(int d = 1; d < kRowLanes; d <<= 1) {
+    v.x += __shfl_xor(v.x, d, kRowLanes);
+    v.y += __shfl_xor(v.y, d, kRowLanes);
+  }
+  return v;
+}
+
+// y = A x (FP32 values and vectors, FP32 products summed per lane in FP32,
+// lanes combined in a fixed butterfly)
+template <bool C>
+__global__ void __launch_bounds__(256) k_spmv32(int64_t n, const int64_t *__restrict__ ai, const int32_t *__restrict__ aj,
+                                                const typename Sc<C>::f32 *__restrict__ a,
+                                                const typename Sc<C>::f32 *__restrict__ x,
+                                                typename Sc<C>::f32 *__restrict__ y)
+{
+  const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kRowLanes;
+  const int l = (int)(threadIdx.x % kRowLanes);
+  if(row >= n) return;  // whole lane groups leave together
+  float sr = 0.f, si = 0.f;
+  for(int64_t k = ai[row] + l; k < ai[row + 1]; k += kRowLanes) {
+    const double2 av = ld_d(a, k), xv = ld_d(x, aj[k]);
+    sr += (float)av.x * (float)xv.x - (float)av.y * (float)xv.y;
+    si += (float)av.x * (float)xv.y + (float)av.y * (float)xv.x;
+  }
+  const double2 s = lanes_sum(make_double2(sr, si));
+  if(l == 0) st(y, row, s);
+}
+
+// r = -b - A x in FP64
+template <bool C>
+__global__ void __launch_bounds__(256) k_residual64(int64_t n, const int64_t *__restrict__ ai,
+                                                    const int32_t *__restrict__ aj,
+                                                    const typename Sc<C>::f64 *__restrict__ a,
+                                                    const typename Sc<C>::f64 *__restrict__ x,
+                                                    const typename Sc<C>::f64 *__restrict__ b,
+                                                    typename Sc<C>::f64 *__restrict__ r)
+{
+  const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kRowLanes;
+  const int l = (int)(threadIdx.x % kRowLanes);
+  if(row >= n) return;
+  double2 s = make_double2(0., 0.);
+  for(int64_t k = ai[row] + l; k < ai[row + 1]; k += kRowLanes) {
+    const double2 p = zmul(ld_d(a, k), ld_d(x, aj[k]));
+    s.x += p.x;
+    s.y += p.y;
+  }
+  s = lanes_sum(s);
+  if(l == 0) {
+    const double2 bv = ld_d(b, row);
+    st(r, row, make_double2(-bv.x - s.x, -bv.y - s.y));
+  }
+}
+
+// block partials of conj(V_k) . w for k in [k0, k0 + kg), FP64 accumulation;
+// V rows are `ld` apart.  part[(k) * kRedBlocks + block]
+template <typename T>
+__global__ void __launch_bounds__(kRedThreads) k_dots(int64_t n, int k0, int kg, const T *__restrict__ V, int64_t ld,
+                                                      const T *__restrict__ w, double2 *__restrict__ part)
+{
+  __shared__ double2 s_red[kRedThreads / 64][kDotGroup];
+  double2 acc[kDotGroup];
+#pragma unroll
+  for(int q = 0; q < kDotGroup; ++q) acc[q] = make_double2(0., 0.);
+  for(int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const double2 wv = ld_d(w, i);
+#pragma unroll
+    for(int q = 0; q < kDotGroup; ++q)
+      if(q < kg) {
+        const double2 p = zconjmul(ld_d(V, (int64_t)(k0 + q) * ld + i), wv);
+        acc[q].x += p.x;
+        acc[q].y += p.y;
+      }
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for(int q = 0; q < kDotGroup; ++q) {
+#pragma unroll
+    for(int d = 32; d >= 1; d >>= 1) {
+      acc[q].x += __shfl_xor(acc[q].x, d, 64);
+      acc[q].y += __shfl_xor(acc[q].y, d, 64);
+    }
+    if(lane == 0) s_red[wv][q] = acc[q];
+  }
+  __syncthreads();
+  if(threadIdx.x < kDotGroup && (int)threadIdx.x < kg) {
+    double2 s = make_double2(0., 0.);
+    for(int k = 0; k < kRedThreads / 64; ++k) {
+      s.x += s_red[k][threadIdx.x].x;
+      s.y += s_red[k][threadIdx.x].y;
+    }
+    part[(int64_t)(k0 + threadIdx.x) * kRedBlocks + blockIdx.x] = s;
+  }
+}
+
+// out[k] (+)= sum over blocks of part[k][.], fixed order; one block per k
+__global__ void __launch_bounds__(kRedThreads) k_dots_final(const double2 *__restrict__ part, double2 *__restrict__ out,
+                                                            int accumulate)
+{
+  __shared__ double2 s_red[kRedThreads / 64];
+  const int k = blockIdx.x;
+  double2 s = make_double2(0., 0.);
+  for(int b = threadIdx.x; b < kRedBlocks; b += kRedThreads) {
+    s.x += part[(int64_t)k * kRedBlocks + b].x;
+    s.y += part[(int64_t)k * kRedBlocks + b].y;
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for(int d = 32; d >= 1; d >>= 1) {
+    s.x += __shfl_xor(s.x, d, 64);
+    s.y += __shfl_xor(s.y, d, 64);
+  }
+  if(lane == 0) s_red[wv] = s;
+  __syncthreads();
+  if(threadIdx.x == 0) {
+    double2 t = make_double2(0., 0.);
+    for(int q = 0; q < kRedThreads / 64; ++q) {
+      t.x += s_red[q].x;
+      t.y += s_red[q].y;
+    }
+    if(accumulate) {
+      out[k].x += t.x;
+      out[k].y += t.y;
+    }
+    else {
+      out[k] = t;
+    }
+  }
+}
+
+// w -= sum_{k<=j} V_k h_k  (h FP64, arithmetic in FP64, w FP32)
+template <typename T>
+__global__ void __launch_bounds__(256) k_orth_update(int64_t n, int j1, const T *__restrict__ V, int64_t ld,
+                                                     const double2 *__restrict__ h, T *__restrict__ w)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(i >= n) return;
+  double2 s = ld_d(w, i);
+  for(int k = 0; k < j1; ++k) {
+    const double2 p = zmul(ld_d(V, (int64_t)k * ld + i), h[k]);
+    s.x -= p.x;
+    s.y -= p.y;
+  }
+  st(w, i, s);
+}
+
+// dst = src * scale (scale = 1/norm from a device scalar); FP64 or FP32 source
+template <typename S, typename T>
+__global__ void __launch_bounds__(256) k_scale(int64_t n, const S *__restrict__ src, const double2 *__restrict__ nrm2,
+                                               T *__restrict__ dst)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(i >= n) return;
+  const double nv = sqrt(nrm2->x);
+  const double inv = nv > 0. ? 1. / nv : 0.;
+  const double2 v = ld_d(src, i);
+  st(dst, i, make_double2(v.x * inv, v.y * inv));
+}
+
+// FP64 -> FP32 copy of the matrix values
+template <typename S, typename T>
+__global__ void __launch_bounds__(256) k_round(int64_t n, const S *__restrict__ src, T *__restrict__ dst)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(i < n) st(dst, i, ld_d(src, i));
+}
+
+// x (FP64) += sum_k V_k y_k
+template <typename T, typename X>
+__global__ void __launch_bounds__(256) k_update_x(int64_t n, int j1, const T *__restrict__ V, int64_t ld,
+                                                  const double2 *__restrict__ y, X *__restrict__ x)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(i >= n) return;
+  double2 s = ld_d(x, i);
+  for(int k = 0; k < j1; ++k) {
+    const double2 p = zmul(ld_d(V, (int64_t)k * ld + i), y[k]);
+    s.x += p.x;
+    s.y += p.y;
+  }
+  st(x, i, s);
+}
+
+// col[i] += y[i], i < k
+__global__ void k_add_into(int k, const double2 *__restrict__ y, double2 *__restrict__ col)
+{
+  const int i = threadIdx.x + blockIdx.x * blockDim.x;
+  if(i < k) {
+    col[i].x += y[i].x;
+    col[i].y += y[i].y;
+  }
+}
+
+// Hessenberg bookkeeping, one thread.  st layout (double2):
+//   H[(m+1) x m] column-major at 0, cs[m], sn[m], g[m+1], est at the end.
+// Column j holds h_{0..j,j} (from the dots) and h_{j+1,j}^2 in slot j+1
+// (squared norm from the norm pass).  Applies the previous rotations, makes
+// rotation j, updates g and the residual estimate |g_{j+1}|.
+__global__ void k_givens(int m, int j, double2 *__restrict__ stt)
+{
+  if(threadIdx.x != 0 || blockIdx.x != 0) return;
+  double2 *H = stt;
+  double2 *cs = stt + (int64_t)(m + 1) * m;
+  double2 *sn = cs + m;
+  double2 *g = sn + m;
+  double2 *est = g + m + 1;
+  double2 *col = H + (int64_t)j * (m + 1);
+  col[j + 1] = make_double2(sqrt(col[j + 1].x), 0.);
+  for(int i = 0; i < j; ++i) {
+    // [c s; -conj(s) c] with real c
+    const double c = cs[i].x;
+    const double2 s = sn[i];
+    const double2 a = col[i], b = col[i + 1];
+    const double2 sb = zmul(s, b);
+    const double2 csa = zconjmul(s, a);  // conj(s) a
+    col[i] = make_double2(c * a.x + sb.x, c * a.y + sb.y);
+    col[i + 1] = make_double2(-csa.x + c * b.x, -csa.y + c * b.y);
+  }
+  const double2 a = col[j], b = col[j + 1];
+  const double na = hypot(a.x, a.y), nb = hypot(b.x, b.y);
+  double c;
+  double2 s;
+  if(nb == 0.) {
+    c = 1.;
+    s = make_double2(0., 0.);
+  }
+  else if(na == 0.) {
+    c = 0.;
+    s = make_double2(b.x / nb, -b.y / nb);  // conj(b)/|b|
+    // s . b = |b|
+  }
+  else {
+    const double t = hypot(na, nb);
+    c = na / t;
+    // s = (a/|a|) conj(b) / t
+    const double2 ua = make_double2(a.x / na, a.y / na);
+    s = make_double2((ua.x * b.x + ua.y * b.y) / t, (ua.y * b.x - ua.x * b.y) / t);
+  }
+  cs[j] = make_double2(c, 0.);
+  sn[j] = s;
+  const double2 sb = zmul(s, b);
+  col[j] = make_double2(c * a.x + sb.x, c * a.y + sb.y);
+  col[j + 1] = make_double2(0., 0.);
+  const double2 gj = g[j];
+  const double2 csg = zconjmul(s, gj);
+  g[j] = make_double2(c * gj.x, c * gj.y);
+  g[j + 1] = make_double2(-csg.x, -csg.y);
+  est->x = hypot(g[j + 1].x, g[j + 1].y);
+}
+
+// back substitution R y = g over the first k columns (one thread); y -> out
+__global__ void k_backsolve(int m, int k, const double2 *__restrict__ stt, double2 *__restrict__ y)
+{
+  if(threadIdx.x != 0 || blockIdx.x != 0) return;
+  const double2 *H = stt;
+  const double2 *g = stt + (int64_t)(m + 1) * m + 2 * m;
+  for(int i = k - 1; i >= 0; --i) {
+    double2 s = g[i];
+    for(int l = i + 1; l < k; ++l) {
+      const double2 p = zmul(H[(int64_t)l * (m + 1) + i], y[l]);
+      s.x -= p.x;
+      s.y -= p.y;
+    }
+    const double2 d = H[(int64_t)i * (m + 1) + i];
+    const double dd = d.x * d.x + d.y * d.y;
+    y[i] = dd > 0. ? make_double2((s.x * d.x + s.y * d.y) / dd, (s.y * d.x - s.x * d.y) / dd) : make_double2(0., 0.);
+  }
+}
+
+// start a cycle: g = [beta, 0, ...], beta = sqrt(nrm2)
+__global__ void k_cycle_start(int m, const double2 *__restrict__ nrm2, double2 *__restrict__ stt)
+{
+  const int i = threadIdx.x + blockIdx.x * blockDim.x;
+  double2 *g = stt + (int64_t)(m + 1) * m + 2 * m;
+  if(i <= m) g[i] = make_double2(i == 0 ? sqrt(nrm2->x) : 0., 0.);
+}
+
+unsigned blocks(int64_t n, int t) { return (unsigned)((n + t - 1) / t); }
+
+template <bool C>
+struct Gmres {
+  using T32 = typename Sc<C>::f32;
+  using T64 = typename Sc<C>::f64;
+  gfem_system *sys;
+  hipStream_t s;
+  int64_t n;
+  int m;
+
+  // h[k0..k0+j1) (+)= V_k^H w  (FP64 sums in a fixed order)
+  template <typename T>
+  hipError_t dots(const T *V, int64_t ld, int j1, const T *w, double2 *out, int accumulate)
+  {
+    for(int k0 = 0; k0 < j1; k0 += kDotGroup) {
+      const int kg = std::min(kDotGroup, j1 - k0);
+      hipLaunchKernelGGL(k_dots<T>, dim3(kRedBlocks), dim3(kRedThreads), 0, s, n, k0, kg, V, ld, w,
+                         sys->d_sol_part);
+    }
+    hipLaunchKernelGGL(k_dots_final, dim3((unsigned)j1), dim3(kRedThreads), 0, s, sys->d_sol_part, out, accumulate);
+    return hipGetLastError();
+  }
+};
+
+}  // namespace
+
+template <bool C>
+static gfem_status gmres_ir(gfem_system *sys, int rhs_idx, double tol, int m, int max_iter, hipStream_t s,
+                            gfem_solve_info *info)
+{
+  using T32 = typename Sc<C>::f32;
+  using T64 = typename Sc<C>::f64;
+  gfem_ctx *ctx = sys->ctx;
+  const int64_t n = sys->n, nnz = sys->nnz;
+  auto chk = [&](hipError_t e) { return e == hipSuccess; };
+  // workspace (kept between solves while the sizes fit)
+  const size_t need_v = sizeof(T32) * (size_t)n * (size_t)(m + 1);
+  if(sys->sol_m != m || !sys->d_sol_V) {
+    // everything sized by the restart length m
+    sys->sol_m = 0;
+    dfree(sys->d_sol_V);
+    dfree(sys->d_sol_state);
+    dfree(sys->d_sol_y);
+    dfree(sys->d_sol_part);
+    if(!chk(hipMalloc(&sys->d_sol_V, need_v))) return fail(ctx, GFEM_ERR_OUT_OF_MEMORY, "solver basis allocation");
+    if(!chk(hipMalloc(&sys->d_sol_part, sizeof(double2) * (size_t)kRedBlocks * (size_t)(m + 2))))
+      return fail(ctx, GFEM_ERR_OUT_OF_MEMORY, "solver reduction allocation");
+    if(!chk(hipMalloc(&sys->d_sol_state, sizeof(double2) * ((size_t)(m + 1) * m + 3 * (size_t)m + 2))))
+      return fail(ctx, GFEM_ERR_OUT_OF_MEMORY, "solver state allocation");
+    if(!chk(hipMalloc(&sys->d_sol_y, sizeof(double2) * (size_t)(m + 1))))
+      return fail(ctx, GFEM_ERR_OUT_OF_MEMORY, "solver state allocation");
+    sys->sol_m = m;
+  }
+  if(!sys->d_sol_x) {
+    if(!chk(hipMalloc(&sys->d_sol_x, sizeof(T64) * (size_t)n)) || !chk(hipMalloc(&sys->d_sol_r, sizeof(T64) * (size_t)n)) ||
+       !chk(hipMalloc(&sys->d_sol_w, sizeof(T32) * (size_t)n)) ||
+       !chk(hipMalloc(&sys->d_sol_a32, sizeof(T32) * (size_t)std::max<int64_t>(nnz, 1))) ||
+       !chk(hipMalloc(&sys->d_sol_nrm, sizeof(double2) * 2)))
+      return fail(ctx, GFEM_ERR_OUT_OF_MEMORY, "solver workspace allocation");
+    if(!chk(hipHostMalloc(&sys->h_sol_est, sizeof(double2) * 2)))
+      return fail(ctx, GFEM_ERR_OUT_OF_MEMORY, "solver host scalar allocation");
+  }
+  T64 *x = reinterpret_cast<T64 *>(sys->d_sol_x);
+  T64 *r = reinterpret_cast<T64 *>(sys->d_sol_r);
+  T32 *w = reinterpret_cast<T32 *>(sys->d_sol_w);
+  T32 *V = reinterpret_cast<T32 *>(sys->d_sol_V);
+  T32 *a32 = reinterpret_cast<T32 *>(sys->d_sol_a32);
+  const T64 *a64 = reinterpret_cast<const T64 *>(sys->d_vals[0]);
+  const T64 *b = reinterpret_cast<const T64 *>(sys->d_rhs[0]) + (int64_t)rhs_idx * n;
+  double2 *stt = sys->d_sol_state;
+  double2 *H = stt;
+  double2 *est = stt + (int64_t)(m + 1) * m + 3 * m + 1;
+  double2 *nrm = sys->d_sol_nrm;
+  double2 *hest = reinterpret_cast<double2 *>(sys->h_sol_est);
+  Gmres<C> G{sys, s, n, m};
+  const unsigned rb = blocks(n * kRowLanes, 256), vb = blocks(n, 256);
+
+  GFEM_HIP(ctx, hipMemsetAsync(x, 0, sizeof(T64) * (size_t)n, s));
+  hipLaunchKernelGGL((k_round<T64, T32>), dim3(blocks(nnz, 256)), dim3(256), 0, s, nnz, a64, a32);
+  // ||b||
+  GFEM_HIP(ctx, G.dots(b, n, 1, b, nrm + 1, 0));
+  GFEM_HIP(ctx, hipMemcpyAsync(hest, nrm, sizeof(double2) * 2, hipMemcpyDeviceToHost, s));
+  GFEM_HIP(ctx, hipStreamSynchronize(s));
+  const double bnorm = std::sqrt(hest[1].x);
+  int iters = 0, cycles = 0;
+  double rel = bnorm > 0. ? 1. : 0.;
+  bool converged = bnorm == 0.;
+  while(!converged && iters < max_iter) {
+    // true residual in FP64
+    hipLaunchKernelGGL(k_residual64<C>, dim3(rb), dim3(256), 0, s, n, sys->d_ai, sys->d_aj, a64, x, b, r);
+    GFEM_HIP(ctx, G.dots(r, n, 1, r, nrm, 0));
+    GFEM_HIP(ctx, hipMemcpyAsync(hest, nrm, sizeof(double2), hipMemcpyDeviceToHost, s));
+    GFEM_HIP(ctx, hipStreamSynchronize(s));
+    const double beta = std::sqrt(hest[0].x);
+    rel = beta / bnorm;
+    if(rel <= tol) {
+      converged = true;
+      break;
+    }
+    ++cycles;
+    // v0 = r / beta (FP32), g = [beta, 0 ...]
+    hipLaunchKernelGGL((k_scale<T64, T32>), dim3(vb), dim3(256), 0, s, n, r, nrm, V);
+    hipLaunchKernelGGL(k_cycle_start, dim3(1), dim3(64 * ((m + 64) / 64)), 0, s, m, nrm, stt);
+    // the cycle stops once the estimate reaches the outer target or the
+    // FP32 floor relative to this cycle's residual
+    const double stop = std::max(0.5 * tol * bnorm, 2e-6 * beta);
+    int k = 0;
+    for(int j = 0; j < m && iters < max_iter; ++j) {
+      T32 *vj = V + (int64_t)j * n;
+      hipLaunchKernelGGL(k_spmv32<C>, dim3(rb), dim3(256), 0, s, n, sys->d_ai, sys->d_aj, a32, vj, w);
+      double2 *col = H + (int64_t)j * (m + 1);
+      // CGS2: h = V^H w, w -= V h, twice
+      GFEM_HIP(ctx, G.dots(V, n, j + 1, w, col, 0));
+      hipLaunchKernelGGL(k_orth_update<T32>, dim3(vb), dim3(256), 0, s, n, j + 1, V, n, col, w);
+      GFEM_HIP(ctx, G.dots(V, n, j + 1, w, sys->d_sol_y, 0));
+      hipLaunchKernelGGL(k_orth_update<T32>, dim3(vb), dim3(256), 0, s, n, j + 1, V, n, sys->d_sol_y, w);
+      hipLaunchKernelGGL(k_add_into, dim3(1), dim3(64 * ((j + 64) / 64)), 0, s, j + 1, sys->d_sol_y, col);
+      // h_{j+1,j}^2 into the column, then v_{j+1} = w / h_{j+1,j}
+      GFEM_HIP(ctx, G.dots(w, n, 1, w, col + j + 1, 0));
+      hipLaunchKernelGGL((k_scale<T32, T32>), dim3(vb), dim3(256), 0, s, n, w, col + j + 1, V + (int64_t)(j + 1) * n);
+      hipLaunchKernelGGL(k_givens, dim3(1), dim3(64), 0, s, m, j, stt);
+      ++iters;
+      k = j + 1;
+      if((j % 5) == 4 || j == m - 1) {
+        GFEM_HIP(ctx, hipMemcpyAsync(hest, est, sizeof(double2), hipMemcpyDeviceToHost, s));
+        GFEM_HIP(ctx, hipStreamSynchronize(s));
+        if(!(hest[0].x > stop)) break;  // also stops on a NaN estimate
+      }
+    }
+    // x += V y, y = R^-1 g
+    hipLaunchKernelGGL(k_backsolve, dim3(1), dim3(64), 0, s, m, k, stt, sys->d_sol_y);
+    hipLaunchKernelGGL((k_update_x<T32, T64>), dim3(vb), dim3(256), 0, s, n, k, V, n, sys->d_sol_y, x);
+    GFEM_HIP(ctx, hipGetLastError());
+  }
+  if(!converged) {
+    // final true residual
+    hipLaunchKernelGGL(k_residual64<C>, dim3(rb), dim3(256), 0, s, n, sys->d_ai, sys->d_aj, a64, x, b, r);
+    GFEM_HIP(ctx, G.dots(r, n, 1, r, nrm, 0));
+    GFEM_HIP(ctx, hipMemcpyAsync(hest, nrm, sizeof(double2), hipMemcpyDeviceToHost, s));
+    GFEM_HIP(ctx, hipStreamSynchronize(s));
+    rel = bnorm > 0. ? std::sqrt(hest[0].x) / bnorm : 0.;
+    converged = rel <= tol;
+  }
+  sys->sol_valid = true;
+  if(info) {
+    info->iterations = iters;
+    info->cycles = cycles;
+    info->rel_residual = rel;
+    info->converged = converged ? 1 : 0;
+  }
+  return GFEM_OK;
+}
+
+gfem_status solve_gmres_ir(gfem_system *sys, int rhs_idx, double tol, int restart, int max_iter, hipStream_t s,
+                           gfem_solve_info *info)
+{
+  return sys->cplx ? gmres_ir<true>(sys, rhs_idx, tol, restart, max_iter, s, info)
+                   : gmres_ir<false>(sys, rhs_idx, tol, restart, max_iter, s, info);
+}
+
+}  // namespace gfem

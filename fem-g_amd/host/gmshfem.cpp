@@ -402,6 +402,15 @@ struct ScalarKind<std::complex<float>> {
   static constexpr bool complex = true;
 };
 
+template <class T>
+T scalarFrom(const std::complex<double> &v)
+{
+  if constexpr(ScalarKind<T>::complex)
+    return T((typename T::value_type)v.real(), (typename T::value_type)v.imag());
+  else
+    return T(v.real());
+}
+
 static int parseGauss(const std::string &s)
 {
   if(s.rfind("Gauss", 0) != 0) throw common::Exception("unknown integration type '" + s + "'");
@@ -822,6 +831,55 @@ common::Timer Formulation<T>::assemble(const AssembleOptions &opt)
   }
   time.tock();
   return time;
+}
+
+template <class T>
+common::Timer Formulation<T>::solve(const SolveOptions &opt)
+{
+  common::Timer time;
+  time.tick();
+  _lastError.clear();
+  _solveInfo = SolveInfo();
+  try {
+    if(!_sys) throw common::Exception("solve before assemble");
+    gfem_ctx *ctx = common::deviceContext();
+    gfem_solve_info info{};
+    gfem_status st = gfem_solve(_sys, (int)opt.rhsIdx, opt.tol, opt.restart, opt.maxIter, opt.stream, &info);
+    if(st != GFEM_OK) throw common::Exception(std::string("gfem_solve: ") + gfem_ctx_last_error(ctx));
+    _solveInfo.iterations = info.iterations;
+    _solveInfo.cycles = info.cycles;
+    _solveInfo.relResidual = info.rel_residual;
+    _solveInfo.converged = info.converged != 0;
+    if(opt.download) {
+      int64_t nnz = 0, n = 0;
+      int vb = 0;
+      gfem_system_info(_sys, GFEM_PREC_FP64, &nnz, &n, &vb);
+      std::vector<char> raw((size_t)(vb * n));
+      st = gfem_solution_download(_sys, raw.data(), raw.size());
+      if(st != GFEM_OK) throw common::Exception(std::string("gfem_solution_download: ") + gfem_ctx_last_error(ctx));
+      convert(raw, false, ScalarKind<T>::complex, _solution, n);
+    }
+    if(!_solveInfo.converged)
+      std::fprintf(stderr, "Warning : solver stopped at relative residual %g after %d iterations\n",
+                   _solveInfo.relResidual, _solveInfo.iterations);
+  }
+  catch(const std::exception &e) {
+    _lastError = e.what();
+    std::fprintf(stderr, "Error   : Unable to solve the system\nError   : %s\n", e.what());
+    std::fill(_solution.begin(), _solution.end(), T(0));
+  }
+  time.tock();
+  return time;
+}
+
+template <class T>
+T Formulation<T>::nodeValue(int64_t tag) const
+{
+  const auto d = nodeDof(tag);
+  if(d.second == 1 && d.first >= 1 && (size_t)d.first <= _solution.size()) return _solution[(size_t)d.first - 1];
+  if(d.second == 2 && d.first >= 1 && (size_t)d.first <= _fixedValues.size())
+    return scalarFrom<T>(_fixedValues[(size_t)d.first - 1]);
+  return T(0);
 }
 
 template class Formulation<double>;

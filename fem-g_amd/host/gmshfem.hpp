@@ -164,6 +164,22 @@ struct AssembleOptions {
   bool download = true;     // copy A and b back to the host
 };
 
+// Solver options (device GMRES with FP64 iterative refinement, A x = -b)
+struct SolveOptions {
+  double tol = 1e-8;      // ||-b - A x|| / ||b||
+  int restart = 50;       // GMRES(restart)
+  int maxIter = 20000;    // inner iterations
+  unsigned int rhsIdx = 0;
+  void *stream = nullptr; // hipStream_t
+  bool download = true;   // copy x back to the host
+};
+
+struct SolveInfo {
+  int iterations = 0, cycles = 0;
+  double relResidual = 0.;
+  bool converged = false;
+};
+
 template <class T>
 class Formulation {
  public:
@@ -186,6 +202,8 @@ class Formulation {
 
   common::Timer pre(DofsSort sort = DofsSort::Default);
   common::Timer assemble(const AssembleOptions &opt = AssembleOptions());
+  // reference: Timer solve(bool reusePreconditioner); needs an FP64 assembly
+  common::Timer solve(const SolveOptions &opt = SolveOptions());
 
   // results of pre()
   uint64_t getNumberOfUnknownDof() const { return _nUnknown; }
@@ -204,6 +222,11 @@ class Formulation {
   const std::vector<T> &getLHS() const { return _values; }
   const std::vector<T> &getRHS(unsigned int i = 0) const { return _rhs.at(i); }
   const std::string &lastError() const { return _lastError; }
+  // results of solve(): unknown dofs in numDof order, and the value at a node
+  // (the unknown's value, or the Dirichlet value of a fixed dof)
+  const std::vector<T> &getSolution() const { return _solution; }
+  const SolveInfo &solveInfo() const { return _solveInfo; }
+  T nodeValue(int64_t tag) const;
   gfem_system *system() const { return _sys; }
   double preHostSeconds() const { return _preSeconds; }
 
@@ -232,6 +255,8 @@ class Formulation {
   gfem_system *_sys = nullptr;
   std::vector<T> _values;
   std::vector<std::vector<T>> _rhs;
+  std::vector<T> _solution;
+  SolveInfo _solveInfo;
   std::string _lastError;
   double _preSeconds = 0.;
 

@@ -96,6 +96,34 @@ void bind_formulation(py::module_ &m, const char *name)
       },
       py::arg("precision") = "fp64", py::arg("strategy") = "fused", py::arg("download") = true)
     .def(
+      "solve",
+      [](F &f, double tol, int restart, int max_iter, unsigned rhs_idx, bool download) {
+        problem::SolveOptions o;
+        o.tol = tol;
+        o.restart = restart;
+        o.maxIter = max_iter;
+        o.rhsIdx = rhs_idx;
+        o.download = download;
+        double t;
+        {
+          py::gil_scoped_release rel;
+          t = f.solve(o).time();
+        }
+        if(!f.lastError().empty()) throw std::runtime_error(f.lastError());
+        const problem::SolveInfo &si = f.solveInfo();
+        py::dict d;
+        d["seconds"] = t;
+        d["iterations"] = si.iterations;
+        d["cycles"] = si.cycles;
+        d["rel_residual"] = si.relResidual;
+        d["converged"] = si.converged;
+        return d;
+      },
+      py::arg("tol") = 1e-8, py::arg("restart") = 50, py::arg("max_iter") = 20000, py::arg("rhs_idx") = 0,
+      py::arg("download") = true)
+    .def("solution", [](const F &f) { return to_numpy(f.getSolution()); })
+    .def("node_value", &F::nodeValue)
+    .def(
       "assemble_async",
       [](F &f, const std::string &prec, const std::string &strat) {
         check(gfem_assemble(f.system(), parse_precision(prec), parse_strategy(strat), nullptr), "gfem_assemble");

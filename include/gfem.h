@@ -211,6 +211,28 @@ gfem_status gfem_rhs_download(gfem_system *sys, gfem_precision prec, int rhs_idx
 /* Device pointers of the last assembled values / RHS (owned by the system). */
 gfem_status gfem_values_device(gfem_system *sys, gfem_precision prec, void **dev_values, void **dev_rhs0);
 
+/* Result of gfem_solve. */
+typedef struct gfem_solve_info {
+  int iterations;      /* inner GMRES iterations (FP32 Krylov steps) */
+  int cycles;          /* restart cycles, each preceded by an FP64 residual */
+  double rel_residual; /* ||-b - A x|| / ||b|| in FP64 at exit */
+  int converged;       /* rel_residual <= tol */
+} gfem_solve_info;
+
+/* Solve A x = -b for right-hand side rhs_idx of the last FP64 assembly,
+ * keeping the reference's sign convention (fem-master/src/system/Solver.cpp:
+ * 119-123); replaces Solver<T>::solve (Solver.cpp:76-160, PETSc/MUMPS LU).
+ * Restarted GMRES(restart) in FP32 inside FP64 iterative refinement: each
+ * cycle starts from the FP64 residual; stops at rel_residual <= tol or after
+ * max_iter inner iterations.  x stays on the device (gfem_solution_*).
+ * General (non-symmetric) storage only. */
+gfem_status gfem_solve(gfem_system *sys, int rhs_idx, double tol, int restart, int max_iter, void *stream,
+                       gfem_solve_info *info);
+/* Copy the last solution (n scalars of the system's FP64 type) to the host. */
+gfem_status gfem_solution_download(gfem_system *sys, void *host, size_t bytes);
+/* Device pointer of the last solution (owned by the system). */
+gfem_status gfem_solution_device(gfem_system *sys, void **dev_x);
+
 /* Per-kernel timing of the last gfem_assemble on its stream, from HIP events
  * recorded around each launch when profiling is on.  names[i] points to
  * static strings.  Returns the number of kernels timed in *count. */

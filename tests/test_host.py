@@ -168,3 +168,13 @@ def test_formulation_error_paths():
         F.integral(gfem.dof(u), gfem.tf(u), gfem.Domain(2, 1), "Legendre3")
     with pytest.raises(ValueError):
         gfem.Formulation("x", "f16")
+
+
+def test_solve_before_assemble_is_an_error():
+    """solve() needs an assembled system: the error surfaces as the
+    reference's exception (logged, solution left empty), without a device."""
+    from gfem import workloads as W
+    F = W.build(W.poisson2d(n=4, order=1))
+    F.pre()
+    with pytest.raises(RuntimeError, match="solve before assemble"):
+        F.solve()
