@@ -1,0 +1,847 @@
+// Batched solves: many assembled systems (requests) iterate together.
+//
+// Each request owns its matrix values (assembled with its own parameters into
+// its own buffers by gfem_assemble_into), right-hand side, FP64 iterate and
+// residual, FP32 Krylov basis and Hessenberg state, all carved from one
+// device arena whose size is the batch's HBM budget.  A step runs one
+// GMRES(m) restart cycle for every live request: each kernel launch covers
+// all requests (row-parallel kernels over a block -> (slot, first row) map,
+// reductions over a (block, slot) grid), requests advance in lock-step
+// through the cycle and drop out when their residual estimate is reached.
+// After the cycle each request takes its FP64 residual; the converged ones
+// complete and free their slot.  Requests join between steps (continuous
+// batching).  A request's arithmetic does not depend on the others in the
+// batch: its result is bitwise the same alone or in a mix.
+//
+// The algorithm per request is the one of solver.hip (GMRES in FP32 with FP64
+// iterative refinement for A x = -b, fem-master/src/system/Solver.cpp:119-123).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <map>
+#include <vector>
+
+#include "internal.h"
+
+namespace gfem {
+
+namespace {
+
+constexpr int kLanes = 8;                       // lanes per CSR row
+constexpr int kRowsPerBlock = 256 / kLanes;      // 32 rows per 256-thread block
+constexpr int kElemsPerBlock = 256;              // vector kernels: one element per thread
+constexpr int kRed = 64;                         // reduction blocks per request
+constexpr int kCheck = 5;                        // iterations between convergence checks
+
+// device view of one request slot
+struct BItem {
+  int64_t n, nnz;
+  const int64_t *ai;
+  const int32_t *aj;
+  const double2 *a64;
+  const float2 *a32;
+  const double2 *b;
+  double2 *x, *r;
+  float2 *w, *V;     // V: (m+1) rows of n
+  double2 *st;       // H[(m+1) m] | cs[m] | sn[m] | g[m+1] | est | y[m+1] | nrm[2]
+  double2 *part;     // (m+2) rows of kRed partial sums
+};
+
+__device__ __forceinline__ double2 zmul(double2 a, double2 b)
+{
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 zconjmul(double2 a, double2 b)  // conj(a) b
+{
+  return make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ double2 f2d(float2 v) { return make_double2(v.x, v.y); }
+
+struct St {  // offsets into BItem::st
+  int m;
+  __device__ int64_t H(int j, int i) const { return (int64_t)j * (m + 1) + i; }
+  __device__ int64_t cs() const { return (int64_t)(m + 1) * m; }
+  __device__ int64_t sn() const { return cs() + m; }
+  __device__ int64_t g() const { return sn() + m; }
+  __device__ int64_t est() const { return g() + m + 1; }
+  __device__ int64_t y() const { return est() + 1; }
+  __device__ int64_t nrm() const { return y() + m + 1; }
+};
+
+__device__ __forceinline__ double2 lanes_sum(double2 v)
+{
+#pragma unroll
+  for(int d = 1; d < kLanes; d <<= 1) {
+    v.x += __shfl_xor(v.x, d, kLanes);
+    v.y += __shfl_xor(v.y, d, kLanes);
+  }
+  return v;
+}
+
+// w = A32 V_j (FP32 products, lanes combined in a fixed butterfly)
+__global__ void __launch_bounds__(256) bk_spmv32(const BItem *__restrict__ items, const int *__restrict__ act,
+                                                 const int2 *__restrict__ rblk, int j)
+{
+  const int2 bb = rblk[blockIdx.x];
+  if(!act[bb.x]) return;
+  const BItem &I = items[bb.x];
+  const int64_t row = (int64_t)bb.y + threadIdx.x / kLanes;
+  const int l = threadIdx.x % kLanes;
+  if(row >= I.n) return;
+  const float2 *x = I.V + (int64_t)j * I.n;
+  float sr = 0.f, si = 0.f;
+  for(int64_t k = I.ai[row] + l; k < I.ai[row + 1]; k += kLanes) {
+    const float2 a = I.a32[k], v = x[I.aj[k]];
+    sr += a.x * v.x - a.y * v.y;
+    si += a.x * v.y + a.y * v.x;
+  }
+  const double2 s = lanes_sum(make_double2(sr, si));
+  if(l == 0) I.w[row] = make_float2((float)s.x, (float)s.y);
+}
+
+// r = -b - A64 x (FP64)
+__global__ void __launch_bounds__(256) bk_residual(const BItem *__restrict__ items, const int *__restrict__ act,
+                                                   const int2 *__restrict__ rblk)
+{
+  const int2 bb = rblk[blockIdx.x];
+  if(!act[bb.x]) return;
+  const BItem &I = items[bb.x];
+  const int64_t row = (int64_t)bb.y + threadIdx.x / kLanes;
+  const int l = threadIdx.x % kLanes;
+  if(row >= I.n) return;
+  double2 s = make_double2(0., 0.);
+  for(int64_t k = I.ai[row] + l; k < I.ai[row + 1]; k += kLanes) {
+    const double2 p = zmul(I.a64[k], I.x[I.aj[k]]);
+    s.x += p.x;
+    s.y += p.y;
+  }
+  s = lanes_sum(s);
+  if(l == 0) I.r[row] = make_double2(-I.b[row].x - s.x, -I.b[row].y - s.y);
+}
+
+// Partial inner products, grid (kRed, slots).  mode:
+//   0: conj(V_k) . w, k <= j     1: w . w     2: r . r     3: b . b
+__global__ void __launch_bounds__(256) bk_dots(const BItem *__restrict__ items, const int *__restrict__ act, int mode,
+                                               int j)
+{
+  const int it = blockIdx.y;
+  if(!act[it]) return;
+  const BItem &I = items[it];
+  const int nk = mode == 0 ? j + 1 : 1;
+  __shared__ double2 s_red[4];
+  for(int k = 0; k < nk; ++k) {
+    double2 acc = make_double2(0., 0.);
+    for(int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < I.n; i += (int64_t)gridDim.x * blockDim.x) {
+      double2 p;
+      if(mode == 0) {
+        p = zconjmul(f2d(I.V[(int64_t)k * I.n + i]), f2d(I.w[i]));
+      }
+      else if(mode == 1) {
+        const double2 v = f2d(I.w[i]);
+        p = make_double2(v.x * v.x + v.y * v.y, 0.);
+      }
+      else {
+        const double2 v = mode == 2 ? I.r[i] : I.b[i];
+        p = make_double2(v.x * v.x + v.y * v.y, 0.);
+      }
+      acc.x += p.x;
+      acc.y += p.y;
+    }
+#pragma unroll
+    for(int d = 32; d >= 1; d >>= 1) {
+      acc.x += __shfl_xor(acc.x, d, 64);
+      acc.y += __shfl_xor(acc.y, d, 64);
+    }
+    if((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if(threadIdx.x == 0) {
+      double2 t = make_double2(0., 0.);
+      for(int q = 0; q < 4; ++q) {
+        t.x += s_red[q].x;
+        t.y += s_red[q].y;
+      }
+      I.part[(int64_t)k * kRed + blockIdx.x] = t;
+    }
+    __syncthreads();
+  }
+}
+
+// Sum the partials (fixed order), grid (m+2, slots).  Destination by mode:
+//   0: H column j (first pass, pass = 0) or y (second pass, pass = 1)
+//   1: H[j+1, j] = |w|^2     2: nrm[0] = |r|^2     3: nrm[1] = |b|^2
+__global__ void __launch_bounds__(64) bk_dots_final(const BItem *__restrict__ items, const int *__restrict__ act,
+                                                    int m, int mode, int j, int pass)
+{
+  const int it = blockIdx.y, k = blockIdx.x;
+  if(!act[it]) return;
+  const int nk = mode == 0 ? j + 1 : 1;
+  if(k >= nk) return;
+  const BItem &I = items[it];
+  double2 s = I.part[(int64_t)k * kRed + threadIdx.x];
+#pragma unroll
+  for(int d = 32; d >= 1; d >>= 1) {
+    s.x += __shfl_xor(s.x, d, 64);
+    s.y += __shfl_xor(s.y, d, 64);
+  }
+  if(threadIdx.x != 0) return;
+  const St o{m};
+  double2 *st = I.st;
+  if(mode == 0)
+    st[pass == 0 ? o.H(j, k) : o.y() + k] = s;
+  else if(mode == 1)
+    st[o.H(j, j + 1)] = s;
+  else
+    st[o.nrm() + (mode == 2 ? 0 : 1)] = s;
+}
+
+// w -= sum_{k<=j} V_k h_k with h = H column j (pass 0) or y (pass 1)
+__global__ void __launch_bounds__(256) bk_orth(const BItem *__restrict__ items, const int *__restrict__ act,
+                                               const int2 *__restrict__ vblk, int m, int j, int pass)
+{
+  const int2 bb = vblk[blockIdx.x];
+  if(!act[bb.x]) return;
+  const BItem &I = items[bb.x];
+  const int64_t i = (int64_t)bb.y + threadIdx.x;
+  if(i >= I.n) return;
+  const St o{m};
+  const double2 *h = I.st + (pass == 0 ? o.H(j, 0) : o.y());
+  double2 s = f2d(I.w[i]);
+  for(int k = 0; k <= j; ++k) {
+    const double2 p = zmul(f2d(I.V[(int64_t)k * I.n + i]), h[k]);
+    s.x -= p.x;
+    s.y -= p.y;
+  }
+  I.w[i] = make_float2((float)s.x, (float)s.y);
+}
+
+// H column j += y (second Gram-Schmidt pass), grid (slots)
+__global__ void bk_add_y(const BItem *__restrict__ items, const int *__restrict__ act, int m, int j)
+{
+  const int it = blockIdx.x;
+  if(!act[it]) return;
+  const St o{m};
+  double2 *st = items[it].st;
+  for(int k = threadIdx.x; k <= j; k += blockDim.x) {
+    st[o.H(j, k)].x += st[o.y() + k].x;
+    st[o.H(j, k)].y += st[o.y() + k].y;
+  }
+}
+
+// V_{j+1} = w / |w| (pass j >= 0) or V_0 = r / |r| (j < 0)
+__global__ void __launch_bounds__(256) bk_scale(const BItem *__restrict__ items, const int *__restrict__ act,
+                                                const int2 *__restrict__ vblk, int m, int j)
+{
+  const int2 bb = vblk[blockIdx.x];
+  if(!act[bb.x]) return;
+  const BItem &I = items[bb.x];
+  const int64_t i = (int64_t)bb.y + threadIdx.x;
+  if(i >= I.n) return;
+  const St o{m};
+  const double n2 = j < 0 ? I.st[o.nrm()].x : I.st[o.H(j, j + 1)].x;
+  const double nv = sqrt(n2);
+  const double inv = nv > 0. ? 1. / nv : 0.;
+  const double2 v = j < 0 ? I.r[i] : f2d(I.w[i]);
+  I.V[(int64_t)(j + 1) * I.n + i] = make_float2((float)(v.x * inv), (float)(v.y * inv));
+}
+
+// g = [|r|, 0, ...], cycle iteration count 0; grid (slots)
+__global__ void bk_cycle_start(const BItem *__restrict__ items, const int *__restrict__ act, int *__restrict__ kcnt,
+                               int m)
+{
+  const int it = blockIdx.x;
+  if(!act[it]) return;
+  const St o{m};
+  double2 *st = items[it].st;
+  for(int i = threadIdx.x; i <= m; i += blockDim.x)
+    st[o.g() + i] = make_double2(i == 0 ? sqrt(st[o.nrm()].x) : 0., 0.);
+  if(threadIdx.x == 0) kcnt[it] = 0;
+}
+
+// Givens rotations of column j, residual estimate |g_{j+1}|; grid (slots)
+__global__ void bk_givens(const BItem *__restrict__ items, const int *__restrict__ act, int *__restrict__ kcnt, int m,
+                          int j)
+{
+  const int it = blockIdx.x;
+  if(!act[it] || threadIdx.x != 0) return;
+  const St o{m};
+  double2 *st = items[it].st;
+  double2 *col = st + o.H(j, 0);
+  double2 *cs = st + o.cs(), *sn = st + o.sn(), *g = st + o.g();
+  col[j + 1] = make_double2(sqrt(col[j + 1].x), 0.);
+  for(int i = 0; i < j; ++i) {
+    const double c = cs[i].x;
+    const double2 s = sn[i];
+    const double2 a = col[i], b = col[i + 1];
+    const double2 sb = zmul(s, b), csa = zconjmul(s, a);
+    col[i] = make_double2(c * a.x + sb.x, c * a.y + sb.y);
+    col[i + 1] = make_double2(-csa.x + c * b.x, -csa.y + c * b.y);
+  }
+  const double2 a = col[j], b = col[j + 1];
+  const double na = hypot(a.x, a.y), nb = hypot(b.x, b.y);
+  double c;
+  double2 s;
+  if(nb == 0.) {
+    c = 1.;
+    s = make_double2(0., 0.);
+  }
+  else if(na == 0.) {
+    c = 0.;
+    s = make_double2(b.x / nb, -b.y / nb);
+  }
+  else {
+    const double t = hypot(na, nb);
+    c = na / t;
+    const double2 ua = make_double2(a.x / na, a.y / na);
+    s = make_double2((ua.x * b.x + ua.y * b.y) / t, (ua.y * b.x - ua.x * b.y) / t);
+  }
+  cs[j] = make_double2(c, 0.);
+  sn[j] = s;
+  const double2 sb = zmul(s, b);
+  col[j] = make_double2(c * a.x + sb.x, c * a.y + sb.y);
+  col[j + 1] = make_double2(0., 0.);
+  const double2 gj = g[j], csg = zconjmul(s, gj);
+  g[j] = make_double2(c * gj.x, c * gj.y);
+  g[j + 1] = make_double2(-csg.x, -csg.y);
+  st[o.est()] = make_double2(hypot(g[j + 1].x, g[j + 1].y), 0.);
+  kcnt[it] = j + 1;
+}
+
+// y = R^-1 g over the request's iterations of this cycle; grid (slots)
+__global__ void bk_backsolve(const BItem *__restrict__ items, const int *__restrict__ live,
+                             const int *__restrict__ kcnt, int m)
+{
+  const int it = blockIdx.x;
+  if(!live[it] || threadIdx.x != 0) return;
+  const St o{m};
+  double2 *st = items[it].st;
+  const int k = kcnt[it];
+  for(int i = k - 1; i >= 0; --i) {
+    double2 s = st[o.g() + i];
+    for(int l = i + 1; l < k; ++l) {
+      const double2 p = zmul(st[o.H(l, i)], st[o.y() + l]);
+      s.x -= p.x;
+      s.y -= p.y;
+    }
+    const double2 d = st[o.H(i, i)];
+    const double dd = d.x * d.x + d.y * d.y;
+    st[o.y() + i] = dd > 0. ? make_double2((s.x * d.x + s.y * d.y) / dd, (s.y * d.x - s.x * d.y) / dd)
+                            : make_double2(0., 0.);
+  }
+}
+
+// x += sum_k V_k y_k (FP64)
+__global__ void __launch_bounds__(256) bk_update_x(const BItem *__restrict__ items, const int *__restrict__ live,
+                                                   const int *__restrict__ kcnt, const int2 *__restrict__ vblk, int m)
+{
+  const int2 bb = vblk[blockIdx.x];
+  if(!live[bb.x]) return;
+  const BItem &I = items[bb.x];
+  const int64_t i = (int64_t)bb.y + threadIdx.x;
+  if(i >= I.n) return;
+  const St o{m};
+  const int k = kcnt[bb.x];
+  double2 s = I.x[i];
+  for(int l = 0; l < k; ++l) {
+    const double2 p = zmul(f2d(I.V[(int64_t)l * I.n + i]), I.st[o.y() + l]);
+    s.x += p.x;
+    s.y += p.y;
+  }
+  I.x[i] = s;
+}
+
+// per-slot scalars to one array: est, |r|^2, |b|^2
+__global__ void bk_collect(const BItem *__restrict__ items, int nslots, int m, double *__restrict__ out)
+{
+  const int it = blockIdx.x * blockDim.x + threadIdx.x;
+  if(it >= nslots || !items[it].st) return;
+  const St o{m};
+  out[3 * it] = items[it].st[o.est()].x;
+  out[3 * it + 1] = items[it].st[o.nrm()].x;
+  out[3 * it + 2] = items[it].st[o.nrm() + 1].x;
+}
+
+__global__ void bk_round(int64_t nnz, const double2 *__restrict__ a, float2 *__restrict__ b)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(i < nnz) b[i] = make_float2((float)a[i].x, (float)a[i].y);
+}
+
+__global__ void bk_zero(int64_t n, double2 *__restrict__ x)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(i < n) x[i] = make_double2(0., 0.);
+}
+
+// first-fit allocator over one device allocation
+class Arena {
+ public:
+  bool init(size_t bytes)
+  {
+    if(hipMalloc(&base_, bytes) != hipSuccess) return false;
+    free_[0] = bytes;
+    size_ = bytes;
+    return true;
+  }
+  ~Arena()
+  {
+    if(base_) (void)hipFree(base_);
+  }
+  // offset of a block of `bytes`, or -1
+  int64_t alloc(size_t bytes)
+  {
+    bytes = (bytes + 255) / 256 * 256;
+    for(auto it = free_.begin(); it != free_.end(); ++it)
+      if(it->second >= bytes) {
+        const size_t off = it->first, sz = it->second;
+        free_.erase(it);
+        if(sz > bytes) free_[off + bytes] = sz - bytes;
+        used_[off] = bytes;
+        return (int64_t)off;
+      }
+    return -1;
+  }
+  void release(int64_t off)
+  {
+    auto u = used_.find((size_t)off);
+    if(u == used_.end()) return;
+    size_t o = u->first, sz = u->second;
+    used_.erase(u);
+    auto nx = free_.lower_bound(o);
+    if(nx != free_.end() && o + sz == nx->first) {
+      sz += nx->second;
+      nx = free_.erase(nx);
+    }
+    if(nx != free_.begin()) {
+      auto pv = std::prev(nx);
+      if(pv->first + pv->second == o) {
+        pv->second += sz;
+        return;
+      }
+    }
+    free_[o] = sz;
+  }
+  char *at(int64_t off) const { return static_cast<char *>(base_) + off; }
+  size_t size() const { return size_; }
+
+ private:
+  void *base_ = nullptr;
+  size_t size_ = 0;
+  std::map<size_t, size_t> free_, used_;
+};
+
+size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+
+// bytes of one request on `sys` (layout of Slot::bind)
+size_t request_bytes(const gfem_system *sys, int m)
+{
+  const size_t n = (size_t)sys->n, nnz = (size_t)std::max<int64_t>(sys->nnz, 1);
+  const size_t st = (size_t)(m + 1) * m + 3 * (size_t)m + 2 + (size_t)(m + 1) + 2;
+  return align256(16 * nnz) + align256(8 * nnz) + align256(16 * n * (size_t)sys->nrhs) + 2 * align256(16 * n) +
+         align256(8 * n) + align256(8 * n * (size_t)(m + 1)) + align256(16 * st) + align256(16 * (size_t)(m + 2) * kRed);
+}
+
+}  // namespace
+}  // namespace gfem
+
+using namespace gfem;
+
+struct gfem_batch {
+  struct Slot {
+    bool used = false, live = false;
+    int64_t id = -1;
+    gfem_system *sys = nullptr;
+    int64_t off = -1;
+    BItem item{};
+    double tol = 0., bnorm = 0., beta = 0., stop = 0., rel = 0.;
+    int max_iter = 0, iters = 0, cycles = 0;
+    bool converged = false;
+  };
+  gfem_ctx *ctx = nullptr;
+  int m = 50;
+  int max_items = 0;
+  Arena arena;
+  std::vector<Slot> slots;
+  BItem *d_items = nullptr;
+  int *d_act = nullptr, *d_live = nullptr, *d_kcnt = nullptr;
+  int2 *d_rblk = nullptr, *d_vblk = nullptr;
+  int64_t cap_rblk = 0, cap_vblk = 0;
+  double *d_scal = nullptr;
+  int *h_flags = nullptr;   // pinned: act | live
+  double *h_scal = nullptr; // pinned: 3 per slot
+  int64_t next_id = 1;
+  std::vector<int64_t> done;          // completed, not yet fetched
+  std::map<int64_t, int> where;       // id -> slot
+  bool maps_dirty = true;
+  int64_t n_rblk = 0, n_vblk = 0;
+};
+
+namespace {
+
+gfem_status upload_items(gfem_batch *B, hipStream_t s)
+{
+  std::vector<BItem> v(B->slots.size());
+  for(size_t i = 0; i < v.size(); ++i) v[i] = B->slots[i].used ? B->slots[i].item : BItem{};
+  GFEM_HIP(B->ctx, hipMemcpyAsync(B->d_items, v.data(), sizeof(BItem) * v.size(), hipMemcpyHostToDevice, s));
+  // the source is a local vector: wait for the copy
+  GFEM_HIP(B->ctx, hipStreamSynchronize(s));
+  return GFEM_OK;
+}
+
+// block maps over the live slots
+gfem_status build_maps(gfem_batch *B, hipStream_t s)
+{
+  std::vector<int2> rb, vb;
+  for(size_t i = 0; i < B->slots.size(); ++i) {
+    const auto &S = B->slots[i];
+    if(!S.live) continue;
+    for(int64_t r = 0; r < S.item.n; r += kRowsPerBlock) rb.push_back(make_int2((int)i, (int)r));
+    for(int64_t r = 0; r < S.item.n; r += kElemsPerBlock) vb.push_back(make_int2((int)i, (int)r));
+  }
+  if(rb.empty()) rb.push_back(make_int2(0, 0));
+  if(vb.empty()) vb.push_back(make_int2(0, 0));
+  gfem_ctx *ctx = B->ctx;
+  if((int64_t)rb.size() > B->cap_rblk) {
+    dfree(B->d_rblk);
+    B->cap_rblk = (int64_t)rb.size() * 2;
+    GFEM_HIP(ctx, dalloc(&B->d_rblk, (size_t)B->cap_rblk));
+  }
+  if((int64_t)vb.size() > B->cap_vblk) {
+    dfree(B->d_vblk);
+    B->cap_vblk = (int64_t)vb.size() * 2;
+    GFEM_HIP(ctx, dalloc(&B->d_vblk, (size_t)B->cap_vblk));
+  }
+  GFEM_HIP(ctx, hipMemcpyAsync(B->d_rblk, rb.data(), sizeof(int2) * rb.size(), hipMemcpyHostToDevice, s));
+  GFEM_HIP(ctx, hipMemcpyAsync(B->d_vblk, vb.data(), sizeof(int2) * vb.size(), hipMemcpyHostToDevice, s));
+  GFEM_HIP(ctx, hipStreamSynchronize(s));  // the sources are local vectors
+  B->n_rblk = (int64_t)rb.size();
+  B->n_vblk = (int64_t)vb.size();
+  B->maps_dirty = false;
+  return GFEM_OK;
+}
+
+// push act (per slot) and live flags through the pinned buffer; the stream
+// is drained first so no earlier copy from that buffer is still pending
+gfem_status push_flags(gfem_batch *B, const std::vector<int> &act, hipStream_t s)
+{
+  GFEM_HIP(B->ctx, hipStreamSynchronize(s));
+  const size_t ns = B->slots.size();
+  for(size_t i = 0; i < ns; ++i) {
+    B->h_flags[i] = act[i];
+    B->h_flags[ns + i] = B->slots[i].live ? 1 : 0;
+  }
+  GFEM_HIP(B->ctx, hipMemcpyAsync(B->d_act, B->h_flags, sizeof(int) * ns, hipMemcpyHostToDevice, s));
+  GFEM_HIP(B->ctx, hipMemcpyAsync(B->d_live, B->h_flags + ns, sizeof(int) * ns, hipMemcpyHostToDevice, s));
+  return GFEM_OK;
+}
+
+gfem_status collect(gfem_batch *B, hipStream_t s)
+{
+  const int ns = (int)B->slots.size();
+  hipLaunchKernelGGL(bk_collect, dim3((unsigned)((ns + 63) / 64)), dim3(64), 0, s, B->d_items, ns, B->m, B->d_scal);
+  GFEM_HIP(B->ctx, hipMemcpyAsync(B->h_scal, B->d_scal, sizeof(double) * 3 * ns, hipMemcpyDeviceToHost, s));
+  GFEM_HIP(B->ctx, hipStreamSynchronize(s));
+  return GFEM_OK;
+}
+
+// residuals and norms of the slots flagged in act; then for each: done when
+// converged / out of iterations, else prepared for the next cycle
+gfem_status residual_and_prepare(gfem_batch *B, std::vector<int> &act, hipStream_t s, bool first)
+{
+  gfem_ctx *ctx = B->ctx;
+  const int ns = (int)B->slots.size(), m = B->m;
+  gfem_status st = push_flags(B, act, s);
+  if(st != GFEM_OK) return st;
+  hipLaunchKernelGGL(bk_residual, dim3((unsigned)B->n_rblk), dim3(256), 0, s, B->d_items, B->d_act, B->d_rblk);
+  hipLaunchKernelGGL(bk_dots, dim3(kRed, (unsigned)ns), dim3(256), 0, s, B->d_items, B->d_act, 2, 0);
+  hipLaunchKernelGGL(bk_dots_final, dim3(1, (unsigned)ns), dim3(64), 0, s, B->d_items, B->d_act, m, 2, 0, 0);
+  if(first) {
+    hipLaunchKernelGGL(bk_dots, dim3(kRed, (unsigned)ns), dim3(256), 0, s, B->d_items, B->d_act, 3, 0);
+    hipLaunchKernelGGL(bk_dots_final, dim3(1, (unsigned)ns), dim3(64), 0, s, B->d_items, B->d_act, m, 3, 0, 0);
+  }
+  GFEM_HIP(ctx, hipGetLastError());
+  st = collect(B, s);
+  if(st != GFEM_OK) return st;
+  static const bool debug = std::getenv("GFEM_BATCH_DEBUG") != nullptr;
+  for(int i = 0; i < ns; ++i) {
+    if(!act[i]) continue;
+    auto &S = B->slots[i];
+    if(first) S.bnorm = std::sqrt(B->h_scal[3 * i + 2]);
+    S.beta = std::sqrt(B->h_scal[3 * i + 1]);
+    if(debug)
+      std::fprintf(stderr, "gfem batch: slot %d id %lld n %lld |b| %.6e |r| %.6e iters %d\n", i, (long long)S.id,
+                   (long long)S.item.n, S.bnorm, S.beta, S.iters);
+    S.rel = S.bnorm > 0. ? S.beta / S.bnorm : 0.;
+    if(S.rel <= S.tol || S.iters >= S.max_iter || !(S.rel == S.rel)) {
+      S.converged = S.rel <= S.tol;
+      S.live = false;
+      act[i] = 0;
+      B->done.push_back(S.id);
+      B->maps_dirty = true;
+      continue;
+    }
+    S.stop = std::max(0.5 * S.tol * S.bnorm, 2e-6 * S.beta);
+  }
+  // V_0 = r / |r|, g = [|r|, 0 ...] for the slots that go on
+  st = push_flags(B, act, s);
+  if(st != GFEM_OK) return st;
+  hipLaunchKernelGGL(bk_scale, dim3((unsigned)B->n_vblk), dim3(256), 0, s, B->d_items, B->d_act, B->d_vblk, m, -1);
+  hipLaunchKernelGGL(bk_cycle_start, dim3((unsigned)ns), dim3(64), 0, s, B->d_items, B->d_act, B->d_kcnt, m);
+  GFEM_HIP(ctx, hipGetLastError());
+  return GFEM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+gfem_status gfem_batch_request_bytes(const gfem_system *sys, int restart, size_t *bytes)
+{
+  if(!sys || !bytes || restart < 1) return fail(sys ? sys->ctx : nullptr, GFEM_ERR_INVALID_ARG, "request_bytes");
+  *bytes = request_bytes(sys, restart);
+  return GFEM_OK;
+}
+
+gfem_status gfem_batch_create(gfem_ctx *ctx, int restart, size_t hbm_bytes, int max_items, gfem_batch **out)
+{
+  if(!ctx || !out || restart < 1 || restart > 1000 || max_items < 1 || max_items > 65535 || hbm_bytes == 0)
+    return fail(ctx, GFEM_ERR_INVALID_ARG, "batch_create: bad arguments");
+  *out = nullptr;
+  GFEM_HIP(ctx, hipSetDevice(ctx->device));
+  gfem_batch *B = new(std::nothrow) gfem_batch();
+  if(!B) return fail(ctx, GFEM_ERR_OUT_OF_MEMORY, "host allocation failed");
+  B->ctx = ctx;
+  B->m = restart;
+  B->max_items = max_items;
+  B->slots.resize((size_t)max_items);
+  bool ok = B->arena.init(hbm_bytes) && dalloc(&B->d_items, (size_t)max_items) == hipSuccess &&
+            dalloc(&B->d_act, (size_t)max_items) == hipSuccess && dalloc(&B->d_live, (size_t)max_items) == hipSuccess &&
+            dalloc(&B->d_kcnt, (size_t)max_items) == hipSuccess &&
+            dalloc(&B->d_scal, (size_t)(3 * max_items)) == hipSuccess &&
+            hipHostMalloc(&B->h_flags, sizeof(int) * 2 * (size_t)max_items) == hipSuccess &&
+            hipHostMalloc(&B->h_scal, sizeof(double) * 3 * (size_t)max_items) == hipSuccess;
+  if(ok) ok = hipMemset(B->d_items, 0, sizeof(BItem) * (size_t)max_items) == hipSuccess;
+  if(!ok) {
+    gfem_batch_destroy(B);
+    return fail(ctx, GFEM_ERR_OUT_OF_MEMORY, "batch_create: device memory");
+  }
+  *out = B;
+  return GFEM_OK;
+}
+
+gfem_status gfem_batch_destroy(gfem_batch *B)
+{
+  if(!B) return GFEM_OK;
+  (void)hipSetDevice(B->ctx->device);
+  (void)hipDeviceSynchronize();
+  dfree(B->d_items);
+  dfree(B->d_act);
+  dfree(B->d_live);
+  dfree(B->d_kcnt);
+  dfree(B->d_scal);
+  dfree(B->d_rblk);
+  dfree(B->d_vblk);
+  if(B->h_flags) (void)hipHostFree(B->h_flags);
+  if(B->h_scal) (void)hipHostFree(B->h_scal);
+  delete B;
+  return GFEM_OK;
+}
+
+gfem_status gfem_batch_submit(gfem_batch *B, gfem_system *sys, double tol, int max_iter, void *stream, int64_t *id)
+{
+  if(!B || !sys || !id) return fail(B ? B->ctx : nullptr, GFEM_ERR_INVALID_ARG, "batch_submit: null argument");
+  gfem_ctx *ctx = B->ctx;
+  if(sys->ctx != ctx) return fail(ctx, GFEM_ERR_INVALID_ARG, "batch_submit: system of another context");
+  if(!sys->finalized) return fail(ctx, GFEM_ERR_STATE, "batch_submit: system not finalized");
+  if(!sys->cplx) return fail(ctx, GFEM_ERR_UNSUPPORTED, "batch_submit: complex systems only");
+  if(sys->symmetric) return fail(ctx, GFEM_ERR_UNSUPPORTED, "batch_submit: symmetric storage");
+  if(sys->n >= (int64_t(1) << 31)) return fail(ctx, GFEM_ERR_TOO_LARGE, "batch_submit: system too large");
+  if(!(tol > 0.) || max_iter < 1) return fail(ctx, GFEM_ERR_INVALID_ARG, "batch_submit: tol > 0, max_iter >= 1");
+  int slot = -1;
+  for(int i = 0; i < (int)B->slots.size(); ++i)
+    if(!B->slots[i].used) {
+      slot = i;
+      break;
+    }
+  if(slot < 0) return fail(ctx, GFEM_ERR_BUSY, "batch_submit: every request slot is taken");
+  const int m = B->m;
+  const int64_t off = B->arena.alloc(request_bytes(sys, m));
+  if(off < 0) return fail(ctx, GFEM_ERR_BUSY, "batch_submit: the HBM budget is taken");
+  GFEM_HIP(ctx, hipSetDevice(ctx->device));
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  auto &S = B->slots[slot];
+  S = gfem_batch::Slot();
+  S.used = true;
+  S.live = true;
+  S.id = B->next_id++;
+  S.sys = sys;
+  S.off = off;
+  S.tol = tol;
+  S.max_iter = max_iter;
+  const size_t n = (size_t)sys->n, nnz = (size_t)std::max<int64_t>(sys->nnz, 1);
+  char *p = B->arena.at(off);
+  auto take = [&](size_t bytes) {
+    char *q = p;
+    p += align256(bytes);
+    return q;
+  };
+  BItem &I = S.item;
+  I.n = sys->n;
+  I.nnz = sys->nnz;
+  I.ai = sys->d_ai;
+  I.aj = sys->d_aj;
+  double2 *a64 = reinterpret_cast<double2 *>(take(16 * nnz));
+  float2 *a32 = reinterpret_cast<float2 *>(take(8 * nnz));
+  double2 *rhs = reinterpret_cast<double2 *>(take(16 * n * (size_t)sys->nrhs));
+  I.a64 = a64;
+  I.a32 = a32;
+  I.b = rhs;
+  I.x = reinterpret_cast<double2 *>(take(16 * n));
+  I.r = reinterpret_cast<double2 *>(take(16 * n));
+  I.w = reinterpret_cast<float2 *>(take(8 * n));
+  I.V = reinterpret_cast<float2 *>(take(8 * n * (size_t)(m + 1)));
+  I.st = reinterpret_cast<double2 *>(take(16 * ((size_t)(m + 1) * m + 3 * (size_t)m + 2 + (size_t)(m + 1) + 2)));
+  I.part = reinterpret_cast<double2 *>(take(16 * (size_t)(m + 2) * kRed));
+  B->where[S.id] = slot;
+  *id = S.id;
+  // assemble this request's system with the system's current parameters
+  gfem_status st = gfem_assemble_into(sys, GFEM_PREC_FP64, a64, rhs, stream);
+  if(st != GFEM_OK) {
+    B->where.erase(S.id);
+    B->arena.release(off);
+    S = gfem_batch::Slot();
+    return st;
+  }
+  hipLaunchKernelGGL(bk_round, dim3((unsigned)((nnz + 255) / 256)), dim3(256), 0, s, (int64_t)sys->nnz, a64, a32);
+  hipLaunchKernelGGL(bk_zero, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, (int64_t)n, I.x);
+  GFEM_HIP(ctx, hipGetLastError());
+  st = upload_items(B, s);
+  if(st != GFEM_OK) return st;
+  // the initial residual (= -b) and |b| of this request only
+  st = build_maps(B, s);
+  if(st != GFEM_OK) return st;
+  std::vector<int> act(B->slots.size(), 0);
+  act[slot] = 1;
+  return residual_and_prepare(B, act, s, true);
+}
+
+gfem_status gfem_batch_step(gfem_batch *B, void *stream, int *n_live)
+{
+  if(!B) return fail(nullptr, GFEM_ERR_INVALID_ARG, "batch is null");
+  gfem_ctx *ctx = B->ctx;
+  GFEM_HIP(ctx, hipSetDevice(ctx->device));
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  const int ns = (int)B->slots.size(), m = B->m;
+  std::vector<int> act(ns, 0);
+  int live = 0;
+  for(int i = 0; i < ns; ++i)
+    if(B->slots[i].live) {
+      act[i] = 1;
+      ++live;
+    }
+  if(live == 0) {
+    if(n_live) *n_live = 0;
+    return GFEM_OK;
+  }
+  gfem_status st = GFEM_OK;
+  if(B->maps_dirty) {
+    st = build_maps(B, s);
+    if(st != GFEM_OK) return st;
+  }
+  st = push_flags(B, act, s);
+  if(st != GFEM_OK) return st;
+  const dim3 red(kRed, (unsigned)ns), fin((unsigned)(m + 2), (unsigned)ns);
+  for(int j = 0; j < m; ++j) {
+    int running = 0;
+    for(int i = 0; i < ns; ++i) running += act[i];
+    if(running == 0) break;
+    hipLaunchKernelGGL(bk_spmv32, dim3((unsigned)B->n_rblk), dim3(256), 0, s, B->d_items, B->d_act, B->d_rblk, j);
+    for(int pass = 0; pass < 2; ++pass) {
+      hipLaunchKernelGGL(bk_dots, red, dim3(256), 0, s, B->d_items, B->d_act, 0, j);
+      hipLaunchKernelGGL(bk_dots_final, fin, dim3(64), 0, s, B->d_items, B->d_act, m, 0, j, pass);
+      hipLaunchKernelGGL(bk_orth, dim3((unsigned)B->n_vblk), dim3(256), 0, s, B->d_items, B->d_act, B->d_vblk, m,
+                         j, pass);
+    }
+    hipLaunchKernelGGL(bk_add_y, dim3((unsigned)ns), dim3(64), 0, s, B->d_items, B->d_act, m, j);
+    hipLaunchKernelGGL(bk_dots, red, dim3(256), 0, s, B->d_items, B->d_act, 1, j);
+    hipLaunchKernelGGL(bk_dots_final, fin, dim3(64), 0, s, B->d_items, B->d_act, m, 1, j, 0);
+    hipLaunchKernelGGL(bk_scale, dim3((unsigned)B->n_vblk), dim3(256), 0, s, B->d_items, B->d_act, B->d_vblk, m, j);
+    hipLaunchKernelGGL(bk_givens, dim3((unsigned)ns), dim3(64), 0, s, B->d_items, B->d_act, B->d_kcnt, m, j);
+    GFEM_HIP(ctx, hipGetLastError());
+    for(int i = 0; i < ns; ++i)
+      if(act[i]) B->slots[i].iters++;
+    if((j % kCheck) == kCheck - 1 && j < m - 1) {
+      st = collect(B, s);
+      if(st != GFEM_OK) return st;
+      bool changed = false;
+      for(int i = 0; i < ns; ++i) {
+        auto &S = B->slots[i];
+        if(act[i] && (!(B->h_scal[3 * i] > S.stop) || S.iters >= S.max_iter)) {
+          act[i] = 0;  // stays live: its x is updated after the cycle
+          changed = true;
+        }
+      }
+      if(changed) {
+        st = push_flags(B, act, s);
+        if(st != GFEM_OK) return st;
+      }
+    }
+  }
+  for(int i = 0; i < ns; ++i)
+    if(B->slots[i].live) B->slots[i].cycles++;
+  hipLaunchKernelGGL(bk_backsolve, dim3((unsigned)ns), dim3(64), 0, s, B->d_items, B->d_live, B->d_kcnt, m);
+  hipLaunchKernelGGL(bk_update_x, dim3((unsigned)B->n_vblk), dim3(256), 0, s, B->d_items, B->d_live, B->d_kcnt,
+                     B->d_vblk, m);
+  GFEM_HIP(ctx, hipGetLastError());
+  for(int i = 0; i < ns; ++i) act[i] = B->slots[i].live ? 1 : 0;
+  st = residual_and_prepare(B, act, s, false);
+  if(st != GFEM_OK) return st;
+  if(n_live) {
+    int l = 0;
+    for(const auto &S : B->slots) l += S.live ? 1 : 0;
+    *n_live = l;
+  }
+  return GFEM_OK;
+}
+
+gfem_status gfem_batch_done(gfem_batch *B, int max, int64_t *ids, int *count)
+{
+  if(!B || !count || (max > 0 && !ids)) return fail(B ? B->ctx : nullptr, GFEM_ERR_INVALID_ARG, "batch_done");
+  const int c = std::min<int>(max, (int)B->done.size());
+  for(int i = 0; i < c; ++i) ids[i] = B->done[i];
+  B->done.erase(B->done.begin(), B->done.begin() + c);
+  *count = c;
+  return GFEM_OK;
+}
+
+gfem_status gfem_batch_result(gfem_batch *B, int64_t id, void *x_host, size_t bytes, gfem_solve_info *info)
+{
+  if(!B) return fail(nullptr, GFEM_ERR_INVALID_ARG, "batch is null");
+  gfem_ctx *ctx = B->ctx;
+  auto w = B->where.find(id);
+  if(w == B->where.end()) return fail(ctx, GFEM_ERR_INVALID_ARG, "batch_result: unknown request");
+  auto &S = B->slots[w->second];
+  if(S.live) return fail(ctx, GFEM_ERR_STATE, "batch_result: request still running");
+  if(x_host) {
+    const size_t need = 16 * (size_t)S.item.n;
+    if(bytes < need) return fail(ctx, GFEM_ERR_INVALID_ARG, "batch_result: buffer too small");
+    GFEM_HIP(ctx, hipSetDevice(ctx->device));
+    GFEM_HIP(ctx, hipMemcpy(x_host, S.item.x, need, hipMemcpyDeviceToHost));
+  }
+  if(info) {
+    info->iterations = S.iters;
+    info->cycles = S.cycles;
+    info->rel_residual = S.rel;
+    info->converged = S.converged ? 1 : 0;
+  }
+  B->arena.release(S.off);
+  B->where.erase(w);
+  S = gfem_batch::Slot();
+  return GFEM_OK;
+}
+
+}  // extern "C"

@@ -152,6 +152,7 @@ const char *gfem_status_string(gfem_status s)
   case GFEM_ERR_NO_UNKNOWNS: return "no unknown dofs";
   case GFEM_ERR_TOO_LARGE: return "index too large for 32-bit device maps";
   case GFEM_ERR_NONAFFINE: return "non-affine element";
+  case GFEM_ERR_BUSY: return "batch full";
   }
   return "unknown status";
 }
@@ -228,22 +229,16 @@ gfem_status gfem_system_create(gfem_ctx *ctx, gfem_scalar scalar, int64_t n, con
     for(int64_t k = 0; k < nnz; ++k) aj32[k] = (int32_t)aj[k];
     e = dupload(&sys->d_aj, aj32.data(), aj32.size());
   }
+  if(e == hipSuccess && ctx->d_xyz) {
+    e = dalloc(&sys->d_xyz, (size_t)(3 * ctx->n_nodes));
+    if(e == hipSuccess)
+      e = hipMemcpy(sys->d_xyz, ctx->d_xyz, sizeof(double) * 3 * (size_t)ctx->n_nodes, hipMemcpyDeviceToDevice);
+  }
   if(e != hipSuccess) {
     dfree(sys->d_dc);
-  dfree(sys->d_ai);
-  dfree(sys->d_aj);
-  dfree(sys->d_sol_x);
-  dfree(sys->d_sol_r);
-  dfree(sys->d_sol_w);
-  dfree(sys->d_sol_V);
-  dfree(sys->d_sol_a32);
-  dfree(sys->d_sol_state);
-  dfree(sys->d_sol_y);
-  dfree(sys->d_sol_part);
-  dfree(sys->d_sol_nrm);
-  if(sys->h_sol_est) (void)hipHostFree(sys->h_sol_est);
     dfree(sys->d_ai);
     dfree(sys->d_aj);
+    dfree(sys->d_xyz);
     delete sys;
     return hip_fail(ctx, e, "upload values_dc / pattern");
   }
@@ -260,6 +255,7 @@ gfem_status gfem_system_destroy(gfem_system *sys)
   dfree(sys->d_dc);
   dfree(sys->d_ai);
   dfree(sys->d_aj);
+  dfree(sys->d_xyz);
   dfree(sys->d_sol_x);
   dfree(sys->d_sol_r);
   dfree(sys->d_sol_w);
@@ -423,7 +419,7 @@ gfem_status gfem_system_finalize(gfem_system *sys)
   if(!sys) return fail(nullptr, GFEM_ERR_INVALID_ARG, "system is null");
   gfem_ctx *ctx = sys->ctx;
   if(sys->finalized) return fail(ctx, GFEM_ERR_STATE, "finalize called twice");
-  if(!ctx->d_xyz) return fail(ctx, GFEM_ERR_STATE, "nodes not uploaded");
+  if(!sys->d_xyz) return fail(ctx, GFEM_ERR_STATE, "nodes not uploaded before gfem_system_create");
   GFEM_HIP(ctx, hipSetDevice(ctx->device));
   // module: AFrequency when a bilinear term carries a time derivative
   // (problem/Formulation.cpp:1095-1149)
@@ -758,7 +754,27 @@ static gfem_status ensure_reference_maps(gfem_system *sys)
   return GFEM_OK;
 }
 
-gfem_status gfem_assemble(gfem_system *sys, gfem_precision prec, gfem_strategy strategy, void *stream)
+// Parameters changed by gfem_term_update since the last assembly: rebuild the
+// reference tensors of the groups whose constant coefficients moved and the
+// device group table when a tensor turned complex or real.
+static gfem_status apply_updates(gfem_system *sys, hipStream_t s)
+{
+  gfem_ctx *ctx = sys->ctx;
+  for(auto &g : sys->groups)
+    if(g.tensor_dirty) {
+      GFEM_HIP(ctx, launch_tensor(g, s));
+      g.tensor_dirty = false;
+    }
+  if(sys->table_dirty) {
+    const GroupTable gt = make_table(sys);
+    GFEM_HIP(ctx, hipMemcpyAsync(sys->d_table, &gt, sizeof(gt), hipMemcpyHostToDevice, s));
+    GFEM_HIP(ctx, hipStreamSynchronize(s));  // the source lives on this stack frame
+    sys->table_dirty = false;
+  }
+  return GFEM_OK;
+}
+
+static gfem_status check_assemble(gfem_system *sys, gfem_precision prec, gfem_strategy strategy)
 {
   if(!sys) return fail(nullptr, GFEM_ERR_INVALID_ARG, "system is null");
   gfem_ctx *ctx = sys->ctx;
@@ -768,10 +784,38 @@ gfem_status gfem_assemble(gfem_system *sys, gfem_precision prec, gfem_strategy s
     return fail(ctx, GFEM_ERR_INVALID_ARG, "strategy");
   if(strategy == GFEM_STRATEGY_REFERENCE && prec != GFEM_PREC_FP64)
     return fail(ctx, GFEM_ERR_UNSUPPORTED, "the reference strategy is FP64 only");
-  GFEM_HIP(ctx, hipSetDevice(ctx->device));
-  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  return GFEM_OK;
+}
+
+static gfem_status assemble_impl(gfem_system *sys, gfem_precision prec, gfem_strategy strategy, void *vals_out,
+                                 void *rhs_out, hipStream_t s);
+
+gfem_status gfem_assemble(gfem_system *sys, gfem_precision prec, gfem_strategy strategy, void *stream)
+{
+  gfem_status st = check_assemble(sys, prec, strategy);
+  if(st != GFEM_OK) return st;
+  GFEM_HIP(sys->ctx, hipSetDevice(sys->ctx->device));
   const int p = (int)prec;
-  gfem_status st = ensure_outputs(sys, p);
+  st = ensure_outputs(sys, p);
+  if(st != GFEM_OK) return st;
+  return assemble_impl(sys, prec, strategy, sys->d_vals[p], sys->d_rhs[p], reinterpret_cast<hipStream_t>(stream));
+}
+
+gfem_status gfem_assemble_into(gfem_system *sys, gfem_precision prec, void *vals, void *rhs, void *stream)
+{
+  gfem_status st = check_assemble(sys, prec, GFEM_STRATEGY_FUSED);
+  if(st != GFEM_OK) return st;
+  if(!vals || !rhs) return fail(sys->ctx, GFEM_ERR_INVALID_ARG, "assemble_into: null output");
+  GFEM_HIP(sys->ctx, hipSetDevice(sys->ctx->device));
+  return assemble_impl(sys, prec, GFEM_STRATEGY_FUSED, vals, rhs, reinterpret_cast<hipStream_t>(stream));
+}
+
+static gfem_status assemble_impl(gfem_system *sys, gfem_precision prec, gfem_strategy strategy, void *vals_out,
+                                 void *rhs_out, hipStream_t s)
+{
+  gfem_ctx *ctx = sys->ctx;
+  const int p = (int)prec;
+  gfem_status st = apply_updates(sys, s);
   if(st != GFEM_OK) return st;
   clear_prof(sys);
   Prof prof{sys, s};
@@ -780,25 +824,25 @@ gfem_status gfem_assemble(gfem_system *sys, gfem_precision prec, gfem_strategy s
     // the RHS rows and the matrix gather
     for(auto &g : sys->groups) {
       prof.begin("geometry");
-      GFEM_HIP(ctx, launch_geometry(g, ctx->d_xyz, s));
+      GFEM_HIP(ctx, launch_geometry(g, sys->d_xyz, s));
       prof.end();
       if(g.has_dense) {
         prof.begin("dense");
-        GFEM_HIP(ctx, launch_dense(g, ctx->d_xyz, false, s));
+        GFEM_HIP(ctx, launch_dense(g, sys->d_xyz, false, s));
         prof.end();
       }
     }
     for(auto &g : sys->groups)
       if(g.has_linear) {
         prof.begin("linear");
-        GFEM_HIP(ctx, launch_linear(g, ctx->d_xyz, sys->nrhs, s));
+        GFEM_HIP(ctx, launch_linear(g, sys->d_xyz, sys->nrhs, s));
         prof.end();
       }
     prof.begin("rhs");
-    GFEM_HIP(ctx, launch_rhs(*sys, p, s));
+    GFEM_HIP(ctx, launch_rhs(*sys, p, rhs_out, s));
     prof.end();
     prof.begin("gather");
-    GFEM_HIP(ctx, launch_gather(*sys, p, s));
+    GFEM_HIP(ctx, launch_gather(*sys, p, vals_out, s));
     prof.end();
     return GFEM_OK;
   }
@@ -808,16 +852,16 @@ gfem_status gfem_assemble(gfem_system *sys, gfem_precision prec, gfem_strategy s
   GFEM_HIP(ctx, hipMemsetAsync(sys->d_rhs[0], 0, value_bytes(sys, 0) * (size_t)(sys->n * sys->nrhs), s));
   for(auto &g : sys->groups) {
     prof.begin("geometry");
-    GFEM_HIP(ctx, launch_geometry(g, ctx->d_xyz, s));
+    GFEM_HIP(ctx, launch_geometry(g, sys->d_xyz, s));
     prof.end();
     if(g.has_linear) {
       prof.begin("linear");
-      GFEM_HIP(ctx, launch_linear(g, ctx->d_xyz, sys->nrhs, s));
+      GFEM_HIP(ctx, launch_linear(g, sys->d_xyz, sys->nrhs, s));
       prof.end();
     }
     if(g.has_bilinear) {
       prof.begin("ref_dense");
-      GFEM_HIP(ctx, launch_dense(g, ctx->d_xyz, true, s));
+      GFEM_HIP(ctx, launch_dense(g, sys->d_xyz, true, s));
       prof.end();
       prof.begin("ref_scatter");
       GFEM_HIP(ctx, launch_ref_scatter(*sys, g, s));
@@ -827,6 +871,47 @@ gfem_status gfem_assemble(gfem_system *sys, gfem_precision prec, gfem_strategy s
     GFEM_HIP(ctx, launch_ref_rhs(*sys, g, s));
     prof.end();
   }
+  return GFEM_OK;
+}
+
+gfem_status gfem_term_update(gfem_system *sys, int group, int term, double coef_re, double coef_im, int n_consts,
+                             const double *consts, void *stream)
+{
+  if(!sys) return fail(nullptr, GFEM_ERR_INVALID_ARG, "system is null");
+  gfem_ctx *ctx = sys->ctx;
+  if(!sys->finalized) return fail(ctx, GFEM_ERR_STATE, "term_update before finalize");
+  if(group < 0 || group >= (int)sys->groups.size()) return fail(ctx, GFEM_ERR_INVALID_ARG, "bad group index");
+  Group &g = sys->groups[group];
+  if(term < 0 || term >= (int)g.dterms.size()) return fail(ctx, GFEM_ERR_INVALID_ARG, "bad term index");
+  DevTerm &d = g.dterms[term];
+  if(d.per_point) {
+    // same program, new constants; a real program must stay real (its
+    // compiled kernel evaluates in real arithmetic)
+    if(n_consts != d.expr.n_consts || (n_consts > 0 && !consts))
+      return fail(ctx, GFEM_ERR_INVALID_ARG, "term_update: the coefficient program has a different shape");
+    for(int q = 0; q < n_consts; ++q)
+      if(d.expr.real_only && consts[2 * q + 1] != 0.)
+        return fail(ctx, GFEM_ERR_INVALID_ARG, "term_update: a real coefficient program cannot take complex constants");
+    for(int q = 0; q < 2 * n_consts; ++q) d.expr.consts[q] = consts[q];
+  }
+  else {
+    if(n_consts != 0) return fail(ctx, GFEM_ERR_INVALID_ARG, "term_update: the term has a constant coefficient");
+    d.c_re = coef_re;
+    d.c_im = coef_im;
+    if(d.kind == 0) g.tensor_dirty = true;
+  }
+  bool treal = true;
+  for(const DevTerm &e : g.dterms)
+    if(e.kind == 0 && !e.per_point && (e.c_re * e.fac_im + e.c_im * e.fac_re) != 0.) treal = false;
+  if(treal != g.treal) {
+    g.treal = treal;
+    sys->table_dirty = true;
+  }
+  GFEM_HIP(ctx, hipSetDevice(ctx->device));
+  // the host descriptor changes again with the next update: wait for the copy
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  GFEM_HIP(ctx, hipMemcpyAsync(g.d_terms + term, &d, sizeof(DevTerm), hipMemcpyHostToDevice, s));
+  GFEM_HIP(ctx, hipStreamSynchronize(s));
   return GFEM_OK;
 }
 

@@ -129,6 +129,7 @@ struct Group {
   double *d_T64r = nullptr;
   float *d_T32r = nullptr;
   bool treal = true;
+  bool tensor_dirty = false;  // a constant bilinear coefficient changed
   double2 *d_dense = nullptr;
   double2 *d_be = nullptr;      // this group's element loads inside gfem_system::d_be_all
   int64_t be_off = 0, be_stride = 0;  // offset in d_be_all, distance between right-hand sides
@@ -153,6 +154,10 @@ struct gfem_system {
   gfem_scalar scalar = GFEM_SCALAR_F64;
   bool cplx = false;
   int64_t n = 0, nnz = 0, n_fixed = 0;
+  // node coordinates of the system's mesh: a copy of the context's nodes at
+  // gfem_system_create, so systems on different meshes can be assembled in
+  // any order
+  double *d_xyz = nullptr;
   int symmetric = 0;
   int nrhs = 1;
   double frequency = 0.;
@@ -190,6 +195,8 @@ struct gfem_system {
   void *h_sol_est = nullptr;  // pinned
   int sol_m = 0;
   bool sol_valid = false;
+  // parameters changed by gfem_term_update, applied at the next assembly
+  bool table_dirty = false;
   // profiling
   bool profiling = false;
   std::vector<const char *> prof_names;
@@ -244,8 +251,8 @@ hipError_t launch_tensor(const Group &g, hipStream_t s);
 hipError_t launch_geometry(const Group &g, const double *xyz, hipStream_t s);
 hipError_t launch_linear(const Group &g, const double *xyz, int nrhs, hipStream_t s);
 hipError_t launch_dense(const Group &g, const double *xyz, bool all_terms, hipStream_t s);
-hipError_t launch_gather(const gfem_system &sys, int prec, hipStream_t s);
-hipError_t launch_rhs(const gfem_system &sys, int prec, hipStream_t s);
+hipError_t launch_gather(const gfem_system &sys, int prec, void *vals, hipStream_t s);
+hipError_t launch_rhs(const gfem_system &sys, int prec, void *rhs, hipStream_t s);
 hipError_t launch_ref_scatter(const gfem_system &sys, const Group &g, hipStream_t s);
 hipError_t launch_ref_rhs(const gfem_system &sys, const Group &g, hipStream_t s);
 

@@ -1221,10 +1221,9 @@ hipError_t launch_dense(const Group &g, const double *xyz, bool all_terms, hipSt
   return hipGetLastError();
 }
 
-hipError_t launch_gather(const gfem_system &sys, int prec, hipStream_t s)
+hipError_t launch_gather(const gfem_system &sys, int prec, void *out, hipStream_t s)
 {
   if(sys.n_chunks == 0) return hipSuccess;
-  void *out = sys.d_vals[prec];
   const GroupTable *gt = sys.d_table;
   const dim3 grid((unsigned)xcd_grid(sys.n_chunks));
 #define GFEM_GATHER_S(C, M, W, S)                                                                            \
@@ -1263,10 +1262,9 @@ hipError_t launch_gather(const gfem_system &sys, int prec, hipStream_t s)
   return hipGetLastError();
 }
 
-hipError_t launch_rhs(const gfem_system &sys, int prec, hipStream_t s)
+hipError_t launch_rhs(const gfem_system &sys, int prec, void *out, hipStream_t s)
 {
   if(sys.n == 0) return hipSuccess;
-  void *out = sys.d_rhs[prec];
   const GroupTable *gt = sys.d_table;
   const dim3 grid((unsigned)xcd_grid(blocks_for(sys.n * kRhsLanes, 256))), block(256);
 #define GFEM_RHS_ARGS gt, sys.n, sys.nrhs, sys.d_dc_ptr, sys.d_dc_words, sys.d_dc_fix, sys.d_dc, sys.d_lin_ptr, sys.d_lin_words, sys.d_be_all, sys.be_total, out

@@ -16,7 +16,7 @@ except ImportError as e:  # pragma: no cover - exercised only on a broken build
         "gfem native extension not built: run `python fem-g_amd/build.py` "
         f"(or __graft_entry__.build()) first ({e})") from e
 
-from ._gfem_host import (Domain, Field, Model, ScalarFunction, Unknown, abs, cos,  # noqa: F401,E402
+from ._gfem_host import (Domain, Field, Model, ScalarFunction, Server, Unknown, abs, cos,  # noqa: F401,E402
                          dof, dt2_dof, dt_dof, exp, from_arrays, gauss_rule, grad,
                          hilbert_index, hilbert_key, lattice_nodes, pow, set_current_model, set_device,
                          sin, sqrt, tf, unit_cube, unit_square, x, y, z)

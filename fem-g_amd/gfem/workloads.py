@@ -84,6 +84,18 @@ def helmholtz3d(n=50, order=2, k=2 * math.pi * 5, jitter=0.15, seed=4321, center
         scalar=scalar)
 
 
+# term tags of helmholtz3d (order of its integral() calls)
+HELMHOLTZ_MASS, HELMHOLTZ_ROBIN, HELMHOLTZ_SOURCE = 1, 2, 3
+
+
+def helmholtz_params(k, center=(0.5, 0.5, 0.5), sigma=0.05):
+    """Term coefficients of one Helmholtz request on a helmholtz3d
+    formulation: wavenumber k and Gaussian source position."""
+    from . import ScalarFunction
+    return [(HELMHOLTZ_MASS, ScalarFunction(-k * k)), (HELMHOLTZ_ROBIN, ScalarFunction(-1j * k)),
+            (HELMHOLTZ_SOURCE, -gaussian_source(center, sigma))]
+
+
 def highorder3d(n=32, order=4, jitter=0.15, seed=99, gauss=8, scalar="f64"):
     m = unit_cube(n, order, jitter, seed)
     omega = [(3, 1)]

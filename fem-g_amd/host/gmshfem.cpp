@@ -485,6 +485,27 @@ unsigned int Formulation<T>::integral(const function::ScalarFunction &f, const e
 }
 
 template <class T>
+void Formulation<T>::setTermCoefficient(unsigned int tag, const function::ScalarFunction &f)
+{
+  if(tag >= _terms.size()) throw common::Exception("setTermCoefficient: unknown term " + std::to_string(tag));
+  Term &t = _terms[tag];
+  function::ScalarFunction &c = t.bilinear ? t.lhs.coef : t.source;
+  if(c.isConstant() != f.isConstant() || (!f.isConstant() && f.ops() != c.ops()))
+    throw common::Exception("setTermCoefficient: the coefficient must keep its form (constant, or the same program)");
+  c = f;
+  if(!_sys) return;
+  gfem_ctx *ctx = common::deviceContext();
+  for(const auto &gk : _termSlots[tag]) {
+    gfem_status st;
+    if(f.isConstant())
+      st = gfem_term_update(_sys, gk.first, gk.second, f.value().real(), f.value().imag(), 0, nullptr, nullptr);
+    else
+      st = gfem_term_update(_sys, gk.first, gk.second, 0., 0., (int)f.consts().size() / 2, f.consts().data(), nullptr);
+    if(st != GFEM_OK) throw common::Exception(std::string("gfem_term_update: ") + gfem_ctx_last_error(ctx));
+  }
+}
+
+template <class T>
 std::pair<int64_t, int> Formulation<T>::nodeDof(int64_t tag) const
 {
   if(tag < 1 || tag >= (int64_t)_dofNum.size()) return {0, 0};
@@ -718,9 +739,11 @@ void Formulation<T>::upload()
     for(int type : M.elementTypes(dim))
       for(const auto &ent : M.entities(dim))
         if(const Pass *p = findPass(dim, ent.second, type)) order.push_back(p);
+  _termSlots.assign(_terms.size(), {});
   for(const Pass *p : order) {
     const model::Block *b = M.block(p->dim, p->tag, p->type);
     int g = -1;
+    int k = 0;
     st = gfem_group_add(_sys, p->dim, p->type, b->size(), b->conn.data(), p->num.data(), p->atype.data(),
                         p->e2n.empty() ? nullptr : p->e2n.data(), &g);
     if(st != GFEM_OK) throw common::Exception(std::string("gfem_group_add: ") + gfem_ctx_last_error(ctx));
@@ -750,6 +773,7 @@ void Formulation<T>::upload()
       }
       st = gfem_group_add_term(_sys, g, &d);
       if(st != GFEM_OK) throw common::Exception(std::string("gfem_group_add_term: ") + gfem_ctx_last_error(ctx));
+      _termSlots[ti].emplace_back(g, k++);
     }
   }
   st = gfem_system_finalize(_sys);

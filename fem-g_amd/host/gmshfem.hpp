@@ -196,6 +196,13 @@ class Formulation {
                         const std::string &integrationType, ProductType product = ProductType::Hermitian,
                         unsigned int rhsIdx = 0);
 
+  // New coefficient of term `tag`: the dof-side coefficient of a bilinear
+  // term or the source of a linear term.  A constant stays a constant, an
+  // expression keeps its program and changes its constants; applied on the
+  // device from the next assembly.  (The reference binds coefficients when a
+  // term is created; this serves repeated solves on one mesh.)
+  void setTermCoefficient(unsigned int tag, const function::ScalarFunction &f);
+
   void setSymmetric(bool s) { _symmetric = s; }
   void setAngularFrequency(double w) { _frequency = w; }
   unsigned int numRHS() const { return _numRHS; }
@@ -252,6 +259,7 @@ class Formulation {
   std::vector<std::complex<double>> _fixedValues;
   std::vector<int64_t> _ai, _aj;
   std::vector<Pass> _passes;
+  std::vector<std::vector<std::pair<int, int>>> _termSlots;  // term -> (group, index in group) on the device
   gfem_system *_sys = nullptr;
   std::vector<T> _values;
   std::vector<std::vector<T>> _rhs;

@@ -11,6 +11,7 @@
 
 #include "../csrc/refelem.h"
 #include "gmshfem.hpp"
+#include "serve.hpp"
 
 namespace py = pybind11;
 using namespace gmshfem;
@@ -74,6 +75,7 @@ void bind_formulation(py::module_ &m, const char *name)
       },
       py::arg("source"), py::arg("rhs"), py::arg("domain"), py::arg("integration"), py::arg("hermitian") = true,
       py::arg("rhs_idx") = 0)
+    .def("set_term_coefficient", &F::setTermCoefficient, py::arg("tag"), py::arg("coefficient"))
     .def("set_symmetric", &F::setSymmetric)
     .def("set_angular_frequency", &F::setAngularFrequency)
     .def("num_rhs", &F::numRHS)
@@ -274,6 +276,61 @@ PYBIND11_MODULE(_gfem_host, m)
   bind_formulation<float>(m, "Formulation_f32");
   bind_formulation<std::complex<double>>(m, "Formulation_c128");
   bind_formulation<std::complex<float>>(m, "Formulation_c64");
+
+  py::class_<serve::Server>(m, "Server")
+    .def(py::init<int, size_t, int>(), py::arg("restart") = 50, py::arg("hbm_bytes") = size_t(16) << 30,
+         py::arg("max_requests") = 256)
+    .def("add_mesh", &serve::Server::addMesh, py::arg("formulation"), py::keep_alive<1, 2>())
+    .def("request_bytes", &serve::Server::requestBytes)
+    .def(
+      "run",
+      [](serve::Server &sv, const py::list &schedule, bool keep) {
+        std::vector<serve::Request> reqs;
+        for(const auto &o : schedule) {
+          const py::dict d = py::reinterpret_borrow<py::dict>(o);
+          serve::Request r;
+          r.mesh = d["mesh"].cast<int>();
+          r.arrival = d["arrival"].cast<double>();
+          if(d.contains("tol")) r.tol = d["tol"].cast<double>();
+          if(d.contains("max_iter")) r.maxIter = d["max_iter"].cast<int>();
+          if(d.contains("params"))
+            for(const auto &p : d["params"]) {
+              const py::tuple t = py::reinterpret_borrow<py::tuple>(p);
+              r.params.emplace_back(t[0].cast<unsigned>(), t[1].cast<ScalarFunction>());
+            }
+          reqs.push_back(std::move(r));
+        }
+        std::vector<serve::Completed> done;
+        {
+          py::gil_scoped_release rel;
+          done = sv.run(reqs, keep);
+        }
+        py::list out;
+        for(const auto &c : done) {
+          py::dict d;
+          d["request"] = c.request;
+          d["arrival"] = c.arrival;
+          d["admitted"] = c.admitted;
+          d["done"] = c.done;
+          d["latency"] = c.latency();
+          d["iterations"] = c.iterations;
+          d["cycles"] = c.cycles;
+          d["rel_residual"] = c.relResidual;
+          d["converged"] = c.converged;
+          if(keep) d["x"] = to_numpy(c.x);
+          out.append(d);
+        }
+        return out;
+      },
+      py::arg("schedule"), py::arg("keep_solutions") = false)
+    .def_property_readonly("stats", [](const serve::Server &sv) {
+      py::dict d;
+      d["seconds"] = sv.stats().seconds;
+      d["steps"] = sv.stats().steps;
+      d["max_live"] = sv.stats().maxLive;
+      d["peak_bytes"] = sv.stats().peakBytes;
+      return d;
+    });
 
   m.def("hilbert_key", [](int dim, std::vector<float> f, std::vector<float> mn, std::vector<float> mx) {
     return reorder::hilbertKey(dim, f.data(), mn.data(), mx.data());

@@ -69,7 +69,8 @@ typedef enum {
   GFEM_ERR_STATE = 6,          /* call order violated (e.g. assemble before finalize) */
   GFEM_ERR_NO_UNKNOWNS = 7,    /* system has no unknown DOF (Formulation.cpp:1179-1184) */
   GFEM_ERR_TOO_LARGE = 8,      /* index does not fit the 32-bit device maps   */
-  GFEM_ERR_NONAFFINE = 9       /* element is not affine (curved high-order node) */
+  GFEM_ERR_NONAFFINE = 9,      /* element is not affine (curved high-order node) */
+  GFEM_ERR_BUSY = 10           /* batch: no free request slot or memory now; retry after a step */
 } gfem_status;
 
 /* dofs::AssembleType of an element key (fem-master/src/dofs/Dof.h:27-35) */
@@ -167,7 +168,9 @@ gfem_status gfem_ctx_destroy(gfem_ctx *ctx);
 /* last error message of this context (never NULL) */
 const char *gfem_ctx_last_error(const gfem_ctx *ctx);
 
-/* Mesh nodes, node tag t <-> xyz[3*(t-1) .. 3*(t-1)+2]. */
+/* Mesh nodes, node tag t <-> xyz[3*(t-1) .. 3*(t-1)+2].  A system keeps a
+ * copy of the nodes current at its gfem_system_create, so systems of
+ * several meshes coexist on one context. */
 gfem_status gfem_nodes_upload(gfem_ctx *ctx, int64_t n_nodes, const double *xyz);
 
 /* System = CSR pattern (unknown rows x unknown cols) + RHS vectors.
@@ -197,6 +200,22 @@ gfem_status gfem_system_finalize(gfem_system *sys);
 /* Assemble A and every b on `stream` (a hipStream_t, NULL = null stream).
  * Asynchronous: values are ready when the stream reaches this point. */
 gfem_status gfem_assemble(gfem_system *sys, gfem_precision prec, gfem_strategy strategy, void *stream);
+/* Same (fused strategy), writing the nnz values and the nrhs x n right-hand
+ * sides into caller-owned device buffers, so several parameter sets of one
+ * system can be held at once.  No counterpart in the reference, whose values
+ * live in one MatrixFactory (fem-master/src/system/MatrixFactory.h:67). */
+gfem_status gfem_assemble_into(gfem_system *sys, gfem_precision prec, void *vals, void *rhs, void *stream);
+
+/* Change the constant coefficient (coef_re, coef_im) of term `term` (order of
+ * gfem_group_add_term) of `group`, or, for an expression coefficient, its
+ * constants (n_consts pairs; the program itself must not change), on a
+ * finalized system.  Ordered on `stream`; applied from the next assembly.
+ * No counterpart in the reference, where a coefficient is bound when the
+ * term is created (fem-master/src/problem/Formulation.h:103-180): it serves
+ * requests that reuse one mesh, pattern and set of maps with new wavenumbers
+ * or source positions. */
+gfem_status gfem_term_update(gfem_system *sys, int group, int term, double coef_re, double coef_im, int n_consts,
+                             const double *consts, void *stream);
 
 /* Wait for the work queued on `stream`. */
 gfem_status gfem_stream_sync(gfem_ctx *ctx, void *stream);
@@ -232,6 +251,30 @@ gfem_status gfem_solve(gfem_system *sys, int rhs_idx, double tol, int restart, i
 gfem_status gfem_solution_download(gfem_system *sys, void *host, size_t bytes);
 /* Device pointer of the last solution (owned by the system). */
 gfem_status gfem_solution_device(gfem_system *sys, void **dev_x);
+
+/* Batched solves (continuous batching): requests on complex systems with
+ * general storage, each assembled with the system's parameters at
+ * submission into its own buffers (carved from the batch's HBM budget), all
+ * iterating together.  No counterpart in the reference, which solves one
+ * system at a time (fem-master/src/system/Solver.cpp:76-160); callers are
+ * serving loops that keep meshes and patterns resident and overlap many
+ * solves on one GPU.  Algorithm and result of each request as gfem_solve
+ * (rhs 0), bitwise the same whatever else is in the batch. */
+typedef struct gfem_batch gfem_batch;
+gfem_status gfem_batch_create(gfem_ctx *ctx, int restart, size_t hbm_bytes, int max_requests, gfem_batch **out);
+gfem_status gfem_batch_destroy(gfem_batch *batch);
+/* HBM one request on `sys` takes from the budget. */
+gfem_status gfem_batch_request_bytes(const gfem_system *sys, int restart, size_t *bytes);
+/* Admit a request: assemble now (current term parameters of `sys`), start
+ * solving A x = -b to tol.  GFEM_ERR_BUSY when no slot or memory is free. */
+gfem_status gfem_batch_submit(gfem_batch *batch, gfem_system *sys, double tol, int max_iter, void *stream,
+                              int64_t *id);
+/* One restart cycle of every running request; *n_live = still running. */
+gfem_status gfem_batch_step(gfem_batch *batch, void *stream, int *n_live);
+/* Ids of the requests that ended since the last call (up to max). */
+gfem_status gfem_batch_done(gfem_batch *batch, int max, int64_t *ids, int *count);
+/* Solution (n complex128, may be NULL) and info of an ended request; frees it. */
+gfem_status gfem_batch_result(gfem_batch *batch, int64_t id, void *x_host, size_t bytes, gfem_solve_info *info);
 
 /* Per-kernel timing of the last gfem_assemble on its stream, from HIP events
  * recorded around each launch when profiling is on.  names[i] points to

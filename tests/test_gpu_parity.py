@@ -180,3 +180,20 @@ def test_gather_two_slots_per_thread(monkeypatch, case, precision, rtol):
     assert_close(F2.values(), P.values(), rtol, "2-slot A")
     assert_close(F2.rhs(0), P.rhs(0), rtol, "2-slot b")
     assert_close(F2.values(), F1.values(), 1e-14 if precision == "fp64" else rtol, "2 vs 4 slots")
+
+
+def test_systems_on_two_meshes_assemble_in_any_order():
+    """Two formulations on different meshes, uploaded one after the other,
+    then assembled alternately: each keeps its own nodes."""
+    from gfem import workloads as W
+    print("case two meshes interleaved", flush=True)
+    s1, s2 = W.helmholtz3d(n=5, order=1), W.poisson2d(n=9, order=2)
+    F1, P1 = _pair(s1)
+    F2, P2 = _pair(s2)
+    F1.assemble()
+    F2.assemble()
+    F1.assemble()
+    assert_close(F1.values(), P1.values(), FP64_RTOL, "mesh 1 A")
+    assert_close(F1.rhs(0), P1.rhs(0), FP64_RTOL, "mesh 1 b")
+    F2.assemble()
+    assert_close(F2.values(), P2.values(), FP64_RTOL, "mesh 2 A")

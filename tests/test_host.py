@@ -178,3 +178,21 @@ def test_solve_before_assemble_is_an_error():
     F.pre()
     with pytest.raises(RuntimeError, match="solve before assemble"):
         F.solve()
+
+
+def test_term_coefficient_keeps_its_form():
+    """set_term_coefficient: a constant stays a constant and an expression
+    keeps its program (its constants may change); checked before any device
+    work."""
+    import gfem
+    from gfem import workloads as W
+    F = W.build(W.helmholtz3d(n=3, order=1))
+    F.pre()
+    with pytest.raises(Exception, match="keep its form"):
+        F.set_term_coefficient(W.HELMHOLTZ_MASS, gfem.exp(gfem.x()))
+    with pytest.raises(Exception, match="keep its form"):
+        F.set_term_coefficient(W.HELMHOLTZ_SOURCE, gfem.ScalarFunction(1.0))
+    for tag, f in W.helmholtz_params(3.0, (0.2, 0.3, 0.4)):
+        F.set_term_coefficient(tag, f)
+    with pytest.raises(Exception, match="unknown term"):
+        F.set_term_coefficient(17, gfem.ScalarFunction(1.0))
