@@ -120,6 +120,68 @@ def cpu_baseline(args):
                       f"({edofs} element-DOFs): {res[nt]:.2f} s on {nt} threads, {res[1]:.2f} s on 1 thread"}
 
 
+# resident meshes of the serving stream: unit cubes, (cells per side, order),
+# 10k - 500k DOFs, orders 1 - 3
+SERVE_MESHES = [(21, 1), (30, 1), (40, 1), (15, 2), (25, 2), (39, 2), (10, 3), (20, 3)]
+
+
+def serving(args):
+    """Helmholtz request stream: Poisson arrivals at `rate`/s, each a solve on
+    one of the resident meshes with k ~ U[2pi, 16pi] and a random source
+    position, batched on the GPU as requests arrive (continuous batching,
+    admission by HBM budget).  Latency = arrival to solution on the host."""
+    import numpy as np
+    from gfem import Server, workloads as W
+    t0 = time.perf_counter()
+    Fs = []
+    for n, order in SERVE_MESHES:
+        F = W.build(W.helmholtz3d(n=n, order=order, k=10.0))
+        F.pre()
+        Fs.append(F)
+    sv = Server(restart=50, hbm_bytes=args.serve_hbm_gb << 30, max_requests=256)
+    for F in Fs:
+        sv.add_mesh(F)
+    log(f"serving: {len(Fs)} meshes resident ({time.perf_counter() - t0:.1f} s)")
+    rng = np.random.default_rng(2024)
+    arrivals = np.cumsum(rng.exponential(1.0 / args.serve_rate, args.serve_requests))
+    sched = []
+    for i in range(args.serve_requests):
+        m = int(rng.integers(0, len(Fs)))
+        k = float(rng.uniform(2 * np.pi, 16 * np.pi))
+        c = tuple(float(v) for v in rng.uniform(0.25, 0.75, 3))
+        sched.append(dict(mesh=m, arrival=float(arrivals[i]), params=W.helmholtz_params(k, c), tol=args.serve_tol,
+                          max_iter=30000))
+    out = sv.run(sched)
+    lat = np.array([r["latency"] for r in out])
+    its = np.array([r["iterations"] for r in out])
+    st = sv.stats
+    res = {
+        "requests": len(out), "arrival_rate_per_s": args.serve_rate, "tol": args.serve_tol,
+        "mesh_dofs": [int(F.n_unknown) for F in Fs],
+        "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
+        "solves_per_s": len(out) / st["seconds"], "seconds": st["seconds"],
+        "mean_iterations": float(its.mean()), "converged": int(sum(r["converged"] for r in out)),
+        "max_rel_residual": float(max(r["rel_residual"] for r in out)),
+        "max_live": st["max_live"], "peak_hbm_GB": st["peak_bytes"] / 2**30,
+    }
+    log(f"serving: {res['requests']} requests p50 {res['p50_latency_s']:.2f} s p99 {res['p99_latency_s']:.2f} s "
+        f"{res['solves_per_s']:.2f} solves/s, mean {res['mean_iterations']:.0f} iterations")
+    # the same requests one at a time (same solver, no batching), on a sample
+    nb = min(args.serve_baseline, len(sched))
+    t1 = time.perf_counter()
+    for s in sched[:nb]:
+        F = Fs[s["mesh"]]
+        for tag, f in s["params"]:
+            F.set_term_coefficient(tag, f)
+        F.assemble(download=False)
+        F.solve(tol=args.serve_tol, restart=50, max_iter=30000, download=False)
+    dt = time.perf_counter() - t1
+    res["one_at_a_time_solves_per_s"] = nb / dt
+    res["one_at_a_time_sample"] = nb
+    log(f"one at a time: {nb} requests in {dt:.2f} s")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -127,9 +189,15 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=50)
     ap.add_argument("--order", type=int, default=2)
-    ap.add_argument("--cpu-n", type=int, default=40)
+    ap.add_argument("--cpu-n", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the mixed / reference-strategy lines")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving stream")
+    ap.add_argument("--serve-requests", type=int, default=256)
+    ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--serve-tol", type=float, default=1e-10)
+    ap.add_argument("--serve-hbm-gb", type=int, default=64)
+    ap.add_argument("--serve-baseline", type=int, default=16, help="requests timed one at a time")
     ap.add_argument("--gather-slots", type=int, choices=[2, 4], default=4,
                     help="contributions per gather thread (4: 128-thread blocks, 2: 256-thread blocks)")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="BYTES",
@@ -184,6 +252,10 @@ def main():
                                        "what": "per-Gauss-point dense element matrices + atomic CSR scatter"}
         log(f"reference-order fp64: {sec_r * 1e3:.3f} ms/step  {edofs / sec_r / 1e9:.2f} GDOF/s")
 
+    serve = None
+    if not args.no_serve:
+        serve = serving(args)
+
     cpu = None
     if not args.no_cpu:
         try:
@@ -218,6 +290,8 @@ def main():
         "cpu_baseline": cpu,
     }
     line.update(extra)
+    if serve is not None:
+        line["serving"] = serve
     print(json.dumps(line), flush=True)
 
 

@@ -126,6 +126,10 @@ __global__ void __launch_bounds__(256) bk_residual(const BItem *__restrict__ ite
 
 // Partial inner products, grid (kRed, slots).  mode:
 //   0: conj(V_k) . w, k <= j     1: w . w     2: r . r     3: b . b
+// Mode 0 takes the basis vectors kGroup at a time, so w is read once per
+// group.
+constexpr int kGroup = 8;
+
 __global__ void __launch_bounds__(256) bk_dots(const BItem *__restrict__ items, const int *__restrict__ act, int mode,
                                                int j)
 {
@@ -133,39 +137,46 @@ __global__ void __launch_bounds__(256) bk_dots(const BItem *__restrict__ items, 
   if(!act[it]) return;
   const BItem &I = items[it];
   const int nk = mode == 0 ? j + 1 : 1;
-  __shared__ double2 s_red[4];
-  for(int k = 0; k < nk; ++k) {
-    double2 acc = make_double2(0., 0.);
+  __shared__ double2 s_red[4][kGroup];
+  for(int k0 = 0; k0 < nk; k0 += kGroup) {
+    const int kg = min(kGroup, nk - k0);
+    double2 acc[kGroup];
+#pragma unroll
+    for(int q = 0; q < kGroup; ++q) acc[q] = make_double2(0., 0.);
     for(int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < I.n; i += (int64_t)gridDim.x * blockDim.x) {
-      double2 p;
       if(mode == 0) {
-        p = zconjmul(f2d(I.V[(int64_t)k * I.n + i]), f2d(I.w[i]));
-      }
-      else if(mode == 1) {
-        const double2 v = f2d(I.w[i]);
-        p = make_double2(v.x * v.x + v.y * v.y, 0.);
+        const double2 wv = f2d(I.w[i]);
+#pragma unroll
+        for(int q = 0; q < kGroup; ++q)
+          if(q < kg) {
+            const double2 p = zconjmul(f2d(I.V[(int64_t)(k0 + q) * I.n + i]), wv);
+            acc[q].x += p.x;
+            acc[q].y += p.y;
+          }
       }
       else {
-        const double2 v = mode == 2 ? I.r[i] : I.b[i];
-        p = make_double2(v.x * v.x + v.y * v.y, 0.);
+        const double2 v = mode == 1 ? f2d(I.w[i]) : (mode == 2 ? I.r[i] : I.b[i]);
+        acc[0].x += v.x * v.x + v.y * v.y;
       }
-      acc.x += p.x;
-      acc.y += p.y;
     }
 #pragma unroll
-    for(int d = 32; d >= 1; d >>= 1) {
-      acc.x += __shfl_xor(acc.x, d, 64);
-      acc.y += __shfl_xor(acc.y, d, 64);
-    }
-    if((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = acc;
-    __syncthreads();
-    if(threadIdx.x == 0) {
-      double2 t = make_double2(0., 0.);
-      for(int q = 0; q < 4; ++q) {
-        t.x += s_red[q].x;
-        t.y += s_red[q].y;
+    for(int q = 0; q < kGroup; ++q) {
+      if(q >= kg) break;
+#pragma unroll
+      for(int d = 32; d >= 1; d >>= 1) {
+        acc[q].x += __shfl_xor(acc[q].x, d, 64);
+        acc[q].y += __shfl_xor(acc[q].y, d, 64);
       }
-      I.part[(int64_t)k * kRed + blockIdx.x] = t;
+      if((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6][q] = acc[q];
+    }
+    __syncthreads();
+    if((int)threadIdx.x < kg) {
+      double2 t = make_double2(0., 0.);
+      for(int w = 0; w < 4; ++w) {
+        t.x += s_red[w][threadIdx.x].x;
+        t.y += s_red[w][threadIdx.x].y;
+      }
+      I.part[(int64_t)(k0 + threadIdx.x) * kRed + blockIdx.x] = t;
     }
     __syncthreads();
   }
