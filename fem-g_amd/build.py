@@ -68,7 +68,8 @@ def build(verbose=False):
                 print(out)
 
     libgfem = os.path.join(OUT, "libgfem.so")
-    run([HIPCC, *HIP_FLAGS, "-shared", "-o", libgfem, *dev_objs, "-L/opt/rocm/lib", "-lhiprtc"])
+    run([HIPCC, *HIP_FLAGS, "-shared", "-o", libgfem, *dev_objs, "-L/opt/rocm/lib", "-lhiprtc", "-lrocblas",
+         "-Wl,-rpath,/opt/rocm/lib"])
     libhost = os.path.join(OUT, "libgfem_host.so")
     run(["g++", "-shared", "-o", libhost, *host_objs, "-L", OUT, "-lgfem", "-Wl,-rpath,$ORIGIN"])
     run(["g++", "-shared", "-o", os.path.join(OUT, "_gfem_host" + ext), py_obj,

@@ -11,6 +11,8 @@
 #include <string>
 #include <vector>
 
+#include <rocblas/rocblas.h>
+
 #include "gfem.h"
 #include "internal.h"
 #include "refelem.h"
@@ -52,7 +54,12 @@ void free_group(Group &g)
   dfree(g.d_T32);
   dfree(g.d_T64r);
   dfree(g.d_T32r);
-  dfree(g.d_dense);
+  dfree(g.d_dense64);
+  dfree(g.d_dense32);
+  dfree(g.d_R64);
+  dfree(g.d_R32);
+  dfree(g.d_A64);
+  dfree(g.d_A32);
   dfree(g.d_e2n);
   dfree(g.d_num);
   dfree(g.d_type);
@@ -111,7 +118,7 @@ void clear_prof(gfem_system *sys)
 
 size_t value_bytes(const gfem_system *sys, int prec) { return (prec ? 4 : 8) * (sys->cplx ? 2 : 1); }
 
-GroupTable make_table(const gfem_system *sys)
+GroupTable make_table(const gfem_system *sys, bool dense_strategy)
 {
   GroupTable gt{};
   gt.ng = (int)sys->groups.size();
@@ -127,7 +134,9 @@ GroupTable make_table(const gfem_system *sys)
     gt.T64r[g] = G.d_T64r;
     gt.T32r[g] = G.d_T32r;
     gt.treal[g] = G.treal ? 1 : 0;
-    gt.dense[g] = G.has_dense ? G.d_dense : nullptr;
+    const bool dense = dense_strategy ? G.has_bilinear : G.has_dense;
+    gt.dense64[g] = dense ? G.d_dense64 : nullptr;
+    gt.dense32[g] = dense ? G.d_dense32 : nullptr;
     gt.be[g] = G.has_linear ? G.d_be : nullptr;
     gt.ne[g] = G.ne;
   }
@@ -177,6 +186,7 @@ gfem_status gfem_ctx_destroy(gfem_ctx *ctx)
   if(!ctx) return GFEM_OK;
   (void)hipSetDevice(ctx->device);
   dfree(ctx->d_xyz);
+  if(ctx->blas) (void)rocblas_destroy_handle(static_cast<rocblas_handle>(ctx->blas));
   delete ctx;
   return GFEM_OK;
 }
@@ -267,6 +277,7 @@ gfem_status gfem_system_destroy(gfem_system *sys)
   dfree(sys->d_sol_nrm);
   if(sys->h_sol_est) (void)hipHostFree(sys->h_sol_est);
   dfree(sys->d_table);
+  dfree(sys->d_table_dense);
   dfree(sys->d_rec);
   dfree(sys->d_be_all);
   dfree(sys->d_lin_words);
@@ -517,7 +528,21 @@ gfem_status gfem_system_finalize(gfem_system *sys)
     g.treal = true;
     for(const DevTerm &d : g.dterms)
       if(d.kind == 0 && !d.per_point && (d.c_re * d.fac_im + d.c_im * d.fac_re) != 0.) g.treal = false;
-    if(g.has_dense) GFEM_HIP(ctx, dalloc(&g.d_dense, (size_t)(g.ne * g.N * g.N)));
+    // one Gauss rule shared by every bilinear term: the GEMM form applies
+    g.gemm_rule = -1;
+    for(const DevTerm &d : g.dterms)
+      if(d.kind == 0) {
+        if(g.gemm_rule == -1)
+          g.gemm_rule = d.rule;
+        else if(g.gemm_rule != d.rule)
+          g.gemm_rule = -2;
+      }
+    if(g.gemm_rule < 0) g.gemm_rule = -1;
+    if(g.has_dense) {
+      const size_t planes = sys->cplx ? 2 : 1;
+      GFEM_HIP(ctx, dalloc(&g.d_dense64, (size_t)g.ne * planes * g.N * g.N));
+      GFEM_HIP(ctx, dalloc(&g.d_dense32, (size_t)g.ne * planes * g.N * g.N));
+    }
     for(auto &R : g.rules) {
       GFEM_HIP(ctx, dupload(&R.d_w, R.w.data(), R.w.size()));
       GFEM_HIP(ctx, dupload(&R.d_pts, R.pts.data(), R.pts.size()));
@@ -555,7 +580,7 @@ gfem_status gfem_system_finalize(gfem_system *sys)
     g.be_stride = sys->be_total;
   }
   {
-    const GroupTable gt = make_table(sys);
+    const GroupTable gt = make_table(sys, false);
     GFEM_HIP(ctx, dupload(&sys->d_table, &gt, 1));
   }
 
@@ -747,7 +772,7 @@ static gfem_status ensure_reference_maps(gfem_system *sys)
       std::vector<int32_t> e2n(g.h_e2n.size());
       for(size_t i = 0; i < e2n.size(); ++i) e2n[i] = (int32_t)g.h_e2n[i];
       GFEM_HIP(ctx, dupload(&g.d_e2n, e2n.data(), e2n.size()));
-      if(!g.d_dense) GFEM_HIP(ctx, dalloc(&g.d_dense, (size_t)(g.ne * NN)));
+      if(!g.d_dense64) GFEM_HIP(ctx, dalloc(&g.d_dense64, (size_t)(g.ne * NN) * (sys->cplx ? 2 : 1)));
     }
   }
   sys->built_ref = true;
@@ -766,11 +791,98 @@ static gfem_status apply_updates(gfem_system *sys, hipStream_t s)
       g.tensor_dirty = false;
     }
   if(sys->table_dirty) {
-    const GroupTable gt = make_table(sys);
+    const GroupTable gt = make_table(sys, false);
     GFEM_HIP(ctx, hipMemcpyAsync(sys->d_table, &gt, sizeof(gt), hipMemcpyHostToDevice, s));
     GFEM_HIP(ctx, hipStreamSynchronize(s));  // the source lives on this stack frame
     sys->table_dirty = false;
   }
+  return GFEM_OK;
+}
+
+// GEMM form of the per-point element matrices (GFEM_DENSE_GEMM=0 keeps the
+// direct per-point kernel)
+static bool gemm_enabled()
+{
+  static const bool on = [] {
+    const char *e = std::getenv("GFEM_DENSE_GEMM");
+    return !(e && e[0] == '0');
+  }();
+  return on;
+}
+
+// reference products R of a group (once) and its A rows / element matrices
+// for one precision
+static gfem_status ensure_gemm(gfem_system *sys, Group &g, bool fp32)
+{
+  gfem_ctx *ctx = sys->ctx;
+  const size_t planes = sys->cplx ? 2 : 1, NN = (size_t)g.N * g.N;
+  const size_t K = (size_t)7 * g.rules[g.gemm_rule].Q;
+  if(!g.d_R64) {
+    GFEM_HIP(ctx, dalloc(&g.d_R64, K * NN));
+    GFEM_HIP(ctx, dalloc(&g.d_R32, K * NN));
+    GFEM_HIP(ctx, launch_ref_products(g, nullptr));
+    GFEM_HIP(ctx, hipDeviceSynchronize());
+  }
+  if(fp32) {
+    if(!g.d_A32) GFEM_HIP(ctx, dalloc(&g.d_A32, (size_t)g.ne * planes * K));
+    if(!g.d_dense32) GFEM_HIP(ctx, dalloc(&g.d_dense32, (size_t)g.ne * planes * NN));
+  }
+  else {
+    if(!g.d_A64) GFEM_HIP(ctx, dalloc(&g.d_A64, (size_t)g.ne * planes * K));
+    if(!g.d_dense64) GFEM_HIP(ctx, dalloc(&g.d_dense64, (size_t)g.ne * planes * NN));
+  }
+  return GFEM_OK;
+}
+
+// element matrices of group g: dense[ne*planes][N*N] = A[ne*planes][7Q] R[7Q][N*N]
+// (row-major; as column-major, dense^T = R^T A^T with no transposes)
+static gfem_status element_gemm(gfem_system *sys, Group &g, bool fp32, int planes, hipStream_t s)
+{
+  gfem_ctx *ctx = sys->ctx;
+  if(!ctx->blas) {
+    rocblas_handle h = nullptr;
+    if(rocblas_create_handle(&h) != rocblas_status_success)
+      return fail(ctx, GFEM_ERR_HIP, "rocblas_create_handle failed");
+    ctx->blas = h;
+  }
+  rocblas_handle h = static_cast<rocblas_handle>(ctx->blas);
+  if(rocblas_set_stream(h, s) != rocblas_status_success) return fail(ctx, GFEM_ERR_HIP, "rocblas_set_stream failed");
+  const int64_t m = (int64_t)g.N * g.N, n = g.ne * planes, k = (int64_t)7 * g.rules[g.gemm_rule].Q;
+  if(n == 0) return GFEM_OK;
+  if(n >= (int64_t(1) << 31)) return fail(ctx, GFEM_ERR_TOO_LARGE, "element GEMM larger than 2^31 rows");
+  rocblas_status r;
+  if(fp32) {
+    const float one = 1.f, zero = 0.f;
+    r = rocblas_sgemm(h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)m, (rocblas_int)n,
+                      (rocblas_int)k, &one, g.d_R32, (rocblas_int)m, g.d_A32, (rocblas_int)k, &zero, g.d_dense32,
+                      (rocblas_int)m);
+  }
+  else {
+    const double one = 1., zero = 0.;
+    r = rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)m, (rocblas_int)n,
+                      (rocblas_int)k, &one, g.d_R64, (rocblas_int)m, g.d_A64, (rocblas_int)k, &zero, g.d_dense64,
+                      (rocblas_int)m);
+  }
+  if(r != rocblas_status_success) return fail(ctx, GFEM_ERR_HIP, "rocblas gemm failed");
+  return GFEM_OK;
+}
+
+// the dense strategy: element matrices for every bilinear group, one shared
+// Gauss rule each, and the group table that points at them
+static gfem_status ensure_dense_strategy(gfem_system *sys)
+{
+  gfem_ctx *ctx = sys->ctx;
+  if(sys->d_table_dense) return GFEM_OK;
+  const size_t planes = sys->cplx ? 2 : 1;
+  for(auto &g : sys->groups) {
+    if(!g.has_bilinear) continue;
+    if(g.gemm_rule < 0)
+      return fail(ctx, GFEM_ERR_UNSUPPORTED, "dense strategy: the bilinear terms of a group use different Gauss rules");
+    if(!g.d_dense64) GFEM_HIP(ctx, dalloc(&g.d_dense64, (size_t)g.ne * planes * g.N * g.N));
+    if(!g.d_dense32) GFEM_HIP(ctx, dalloc(&g.d_dense32, (size_t)g.ne * planes * g.N * g.N));
+  }
+  const GroupTable gt = make_table(sys, true);
+  GFEM_HIP(ctx, dupload(&sys->d_table_dense, &gt, 1));
   return GFEM_OK;
 }
 
@@ -780,7 +892,7 @@ static gfem_status check_assemble(gfem_system *sys, gfem_precision prec, gfem_st
   gfem_ctx *ctx = sys->ctx;
   if(!sys->finalized) return fail(ctx, GFEM_ERR_STATE, "assemble before finalize");
   if(prec != GFEM_PREC_FP64 && prec != GFEM_PREC_MIXED) return fail(ctx, GFEM_ERR_INVALID_ARG, "precision");
-  if(strategy != GFEM_STRATEGY_FUSED && strategy != GFEM_STRATEGY_REFERENCE)
+  if(strategy != GFEM_STRATEGY_FUSED && strategy != GFEM_STRATEGY_REFERENCE && strategy != GFEM_STRATEGY_DENSE)
     return fail(ctx, GFEM_ERR_INVALID_ARG, "strategy");
   if(strategy == GFEM_STRATEGY_REFERENCE && prec != GFEM_PREC_FP64)
     return fail(ctx, GFEM_ERR_UNSUPPORTED, "the reference strategy is FP64 only");
@@ -819,16 +931,37 @@ static gfem_status assemble_impl(gfem_system *sys, gfem_precision prec, gfem_str
   if(st != GFEM_OK) return st;
   clear_prof(sys);
   Prof prof{sys, s};
-  if(strategy == GFEM_STRATEGY_FUSED) {
+  if(strategy == GFEM_STRATEGY_FUSED || strategy == GFEM_STRATEGY_DENSE) {
     // geometry and per-point element matrices first; then the element loads,
-    // the RHS rows and the matrix gather
+    // the RHS rows and the matrix gather.  The dense strategy puts every
+    // bilinear term into the element matrices and leaves the tensors out.
+    const bool all = strategy == GFEM_STRATEGY_DENSE;
+    const bool fp32 = prec == GFEM_PREC_MIXED;
+    const int planes = sys->cplx ? 2 : 1;
+    if(all) {
+      st = ensure_dense_strategy(sys);
+      if(st != GFEM_OK) return st;
+    }
     for(auto &g : sys->groups) {
       prof.begin("geometry");
       GFEM_HIP(ctx, launch_geometry(g, sys->d_xyz, s));
       prof.end();
-      if(g.has_dense) {
+      if(!(all ? g.has_bilinear : g.has_dense)) continue;
+      if(g.gemm_rule >= 0 && gemm_enabled()) {
+        st = ensure_gemm(sys, g, fp32);
+        if(st != GFEM_OK) return st;
+        prof.begin("coef_rows");
+        GFEM_HIP(ctx, launch_coef_rows(g, sys->d_xyz, all, planes, fp32, s));
+        prof.end();
+        prof.begin("element_gemm");
+        st = element_gemm(sys, g, fp32, planes, s);
+        prof.end();
+        if(st != GFEM_OK) return st;
+      }
+      else {
         prof.begin("dense");
-        GFEM_HIP(ctx, launch_dense(g, sys->d_xyz, false, s));
+        GFEM_HIP(ctx, launch_dense(g, sys->d_xyz, all, planes, s));
+        if(fp32) GFEM_HIP(ctx, launch_dense_to32(g, planes, s));
         prof.end();
       }
     }
@@ -839,10 +972,10 @@ static gfem_status assemble_impl(gfem_system *sys, gfem_precision prec, gfem_str
         prof.end();
       }
     prof.begin("rhs");
-    GFEM_HIP(ctx, launch_rhs(*sys, p, rhs_out, s));
+    GFEM_HIP(ctx, launch_rhs(*sys, p, all ? 0 : 1, rhs_out, s));
     prof.end();
     prof.begin("gather");
-    GFEM_HIP(ctx, launch_gather(*sys, p, vals_out, s));
+    GFEM_HIP(ctx, launch_gather(*sys, p, all ? 0 : 1, vals_out, s));
     prof.end();
     return GFEM_OK;
   }
@@ -861,7 +994,7 @@ static gfem_status assemble_impl(gfem_system *sys, gfem_precision prec, gfem_str
     }
     if(g.has_bilinear) {
       prof.begin("ref_dense");
-      GFEM_HIP(ctx, launch_dense(g, sys->d_xyz, true, s));
+      GFEM_HIP(ctx, launch_dense(g, sys->d_xyz, true, sys->cplx ? 2 : 1, s));
       prof.end();
       prof.begin("ref_scatter");
       GFEM_HIP(ctx, launch_ref_scatter(*sys, g, s));

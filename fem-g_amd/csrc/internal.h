@@ -78,7 +78,10 @@ struct GroupTable {
   const double *T64r[kMaxGroups];  // [N*N][8] real parts
   const float *T32r[kMaxGroups];   // [N*N][8] real parts
   int treal[kMaxGroups];           // the group's tensor has no imaginary part
-  const double2 *dense[kMaxGroups];  // [ne][N*N] per-point bilinear terms, or null
+  // per-point element matrices [ne][planes][N*N] (planes: real, and imaginary
+  // for complex systems) in FP64 / FP32, or null
+  const double *dense64[kMaxGroups];
+  const float *dense32[kMaxGroups];
   const double2 *be[kMaxGroups];     // [nrhs][ne][N] linear terms, or null
   int64_t ne[kMaxGroups];
 };
@@ -130,7 +133,17 @@ struct Group {
   float *d_T32r = nullptr;
   bool treal = true;
   bool tensor_dirty = false;  // a constant bilinear coefficient changed
-  double2 *d_dense = nullptr;
+  // per-point element matrices, planes layout [ne][planes][N*N]
+  double *d_dense64 = nullptr;
+  float *d_dense32 = nullptr;
+  // GEMM form of the per-point terms: A rows [ne*planes][7Q] (per element and
+  // point: det G (6 components) times the stiffness coefficient, det times the
+  // mass coefficient) times R [7Q][N*N] (weights times reference products)
+  int gemm_rule = -1;  // the rule all GEMM-path terms share, or -1
+  double *d_R64 = nullptr;
+  float *d_R32 = nullptr;
+  double *d_A64 = nullptr;
+  float *d_A32 = nullptr;
   double2 *d_be = nullptr;      // this group's element loads inside gfem_system::d_be_all
   int64_t be_off = 0, be_stride = 0;  // offset in d_be_all, distance between right-hand sides
   // plain reference strategy
@@ -143,6 +156,7 @@ struct Group {
 
 struct gfem_ctx {
   int device = 0;
+  void *blas = nullptr;  // rocblas_handle, created at the first GEMM
   double *d_xyz = nullptr;
   int64_t n_nodes = 0;
   std::vector<double> h_xyz;
@@ -167,7 +181,8 @@ struct gfem_system {
   std::vector<gfem::Group> groups;
   std::vector<double> h_dc;
   double2 *d_dc = nullptr;
-  gfem::GroupTable *d_table = nullptr;
+  gfem::GroupTable *d_table = nullptr;        // fused strategy
+  gfem::GroupTable *d_table_dense = nullptr;  // every bilinear term in the element matrices
   // fused maps
   uint32_t *d_rec = nullptr;  // chunk records, n_chunks x rec_stride
   double2 *d_be_all = nullptr;  // element load vectors of all groups, [nrhs][be_total]
@@ -250,9 +265,14 @@ hipError_t launch_basis(const Group &g, Rule &r, hipStream_t s);
 hipError_t launch_tensor(const Group &g, hipStream_t s);
 hipError_t launch_geometry(const Group &g, const double *xyz, hipStream_t s);
 hipError_t launch_linear(const Group &g, const double *xyz, int nrhs, hipStream_t s);
-hipError_t launch_dense(const Group &g, const double *xyz, bool all_terms, hipStream_t s);
-hipError_t launch_gather(const gfem_system &sys, int prec, void *vals, hipStream_t s);
-hipError_t launch_rhs(const gfem_system &sys, int prec, void *rhs, hipStream_t s);
+hipError_t launch_dense(const Group &g, const double *xyz, bool all_terms, int planes, hipStream_t s);
+hipError_t launch_dense_to32(const Group &g, int planes, hipStream_t s);
+hipError_t launch_ref_products(const Group &g, hipStream_t s);
+hipError_t launch_coef_rows(const Group &g, const double *xyz, bool all_terms, int planes, bool fp32, hipStream_t s);
+// tensor = 0: the constant-coefficient tensors are left out (every bilinear
+// term is in the per-point element matrices)
+hipError_t launch_gather(const gfem_system &sys, int prec, int tensor, void *vals, hipStream_t s);
+hipError_t launch_rhs(const gfem_system &sys, int prec, int tensor, void *rhs, hipStream_t s);
 hipError_t launch_ref_scatter(const gfem_system &sys, const Group &g, hipStream_t s);
 hipError_t launch_ref_rhs(const gfem_system &sys, const Group &g, hipStream_t s);
 

@@ -537,7 +537,7 @@ __global__ void __launch_bounds__(256) k_linear_pp(LinTerm a, int N, int EB, con
 // A_e(j,i) += w det (c E_j) . conj(c_tf E_i)
 __global__ void __launch_bounds__(256) k_dense(const GroupDev *__restrict__ gd, int all_terms, int64_t ne,
                                                const int32_t *__restrict__ vtx, const double *__restrict__ xyz,
-                                               double2 *__restrict__ Ae)
+                                               int planes, double *__restrict__ Ae)
 {
   const int N = gd->N;
   const int NN = N * N;
@@ -589,7 +589,92 @@ __global__ void __launch_bounds__(256) k_dense(const GroupDev *__restrict__ gd, 
     }
     acc = cadd(acc, cmul(make_double2(dt->fac_re, dt->fac_im), sum));
   }
-  Ae[e * NN + pair] = acc;
+  // planes layout: [e][real, imaginary][N*N]
+  Ae[e * planes * NN + pair] = acc.x;
+  if(planes == 2) Ae[e * planes * NN + NN + pair] = acc.y;
+}
+
+__global__ void __launch_bounds__(256) k_dense_to32(int64_t total, const double *__restrict__ a, float *__restrict__ b)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(i < total) b[i] = (float)a[i];
+}
+
+// ------------------------------------------------------------------ GEMM form of the per-point terms
+// For affine elements, A_e(j,i) = sum_q sum_s A_e(q,s) R(q,s; i*N+j) with
+//   R(q,s; pair) = w_q * (d_a phi_j d_b phi_i [+ d_b phi_j d_a phi_i]) for the
+//                  6 components s = (00,11,22,01,02,12) of G = J^-1 J^-T,
+//                  w_q * phi_j phi_i for s = 6 (mass),
+//   A_e(q,s)     = det G_s * sum over stiffness terms of fac c(x_q), and
+//                  det * sum over mass terms of fac c(x_q) for s = 6,
+// the same quantity as the reference's per-point B^T W B
+// (fem-master/src/term/BilinearTerm.cpp:303-343): the element matrices are
+// one GEMM  dense[e*planes + p][pair] = sum_k A[e*planes + p][k] R[k][pair].
+template <typename T>
+__global__ void __launch_bounds__(256) k_ref_products(int N, int Q, const double *__restrict__ w,
+                                                      const double *__restrict__ bf, const double *__restrict__ gbf,
+                                                      T *__restrict__ R)
+{
+  const int NN = N * N;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(t >= (int64_t)7 * Q * NN) return;
+  const int k = (int)(t / NN), pair = (int)(t % NN);
+  const int q = k / 7, s = k % 7;
+  const int i = pair / N, j = pair % N;
+  const double *gj = &gbf[(q * N + j) * 3], *gi = &gbf[(q * N + i) * 3];
+  double v;
+  switch(s) {
+  case 0: v = gj[0] * gi[0]; break;
+  case 1: v = gj[1] * gi[1]; break;
+  case 2: v = gj[2] * gi[2]; break;
+  case 3: v = gj[0] * gi[1] + gj[1] * gi[0]; break;
+  case 4: v = gj[0] * gi[2] + gj[2] * gi[0]; break;
+  case 5: v = gj[1] * gi[2] + gj[2] * gi[1]; break;
+  default: v = bf[q * N + j] * bf[q * N + i]; break;
+  }
+  R[t] = (T)(w[q] * v);
+}
+
+// one thread per (element, Gauss point): the 7 entries of A for that point,
+// per plane (real part, then imaginary part for complex systems)
+template <typename T>
+__global__ void __launch_bounds__(256) k_coef_rows(const GroupDev *__restrict__ gd, int all_terms, int rule, int64_t ne,
+                                                   const int32_t *__restrict__ vtx, const double *__restrict__ xyz,
+                                                   const double *__restrict__ F64, int planes, T *__restrict__ A)
+{
+  const RuleDev &R = gd->rules[rule];
+  const int Q = R.Q;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(t >= ne * Q) return;
+  const int64_t e = t / Q;
+  const int q = (int)(t % Q);
+  double X[4][3];
+  load_vertices(vtx, gd->nv, e, xyz, X);
+  double x[3];
+  phys_point(gd->dim, X, &R.pts[3 * q], x);
+  double2 stiff = make_double2(0., 0.), mass = make_double2(0., 0.);
+  const int nterms = __builtin_amdgcn_readfirstlane(gd->nterms);
+  for(int k = 0; k < nterms; ++k) {
+    const DevTerm *dt = &gd->terms[k];
+    if(__builtin_amdgcn_readfirstlane(dt->kind) != 0) continue;
+    const bool pp = __builtin_amdgcn_readfirstlane(dt->per_point) != 0;
+    if(!all_terms && !pp) continue;
+    const double2 c = pp ? eval_expr(&dt->expr, x[0], x[1], x[2]) : make_double2(dt->c_re, dt->c_im);
+    const double2 v = cmul(make_double2(dt->fac_re, dt->fac_im), c);
+    if(__builtin_amdgcn_readfirstlane(dt->form_dof) == 1)
+      stiff = cadd(stiff, v);
+    else
+      mass = cadd(mass, v);
+  }
+  const double *F = &F64[e * 8];  // det G00, G11, G22, G01, G02, G12, det
+  const int64_t K = (int64_t)7 * Q;
+  for(int p = 0; p < planes; ++p) {
+    const double cs = p == 0 ? stiff.x : stiff.y, cm = p == 0 ? mass.x : mass.y;
+    T *row = A + (e * planes + p) * K + (int64_t)q * 7;
+#pragma unroll
+    for(int s = 0; s < 6; ++s) row[s] = (T)(F[s] * cs);
+    row[6] = (T)(F[6] * cm);
+  }
 }
 
 // ------------------------------------------------------------------ element value
@@ -602,49 +687,53 @@ __device__ __forceinline__ int find_group(const GroupTable &gt, int64_t eg)
 
 // value of local entry `pair` of global element slot `eg`
 template <bool CPLX>
-__device__ __forceinline__ double2 elem_value64(const GroupTable &gt, int64_t eg, int pair)
+__device__ __forceinline__ double2 elem_value64(const GroupTable &gt, int tensor, int64_t eg, int pair)
 {
   const int g = find_group(gt, eg);
   const int64_t el = eg - gt.e_off[g];
-  const double2 *F = reinterpret_cast<const double2 *>(gt.F64[g] + el * 8);
-  const double2 *T = gt.T64[g] + (int64_t)pair * 8;
   double2 v = make_double2(0., 0.);
+  if(tensor) {
+    const double2 *F = reinterpret_cast<const double2 *>(gt.F64[g] + el * 8);
+    const double2 *T = gt.T64[g] + (int64_t)pair * 8;
 #pragma unroll
-  for(int k = 0; k < 4; ++k) {
-    const double2 f = F[k];
-    const double2 t0 = T[2 * k], t1 = T[2 * k + 1];
-    v.x += f.x * t0.x + f.y * t1.x;
-    if(CPLX) v.y += f.x * t0.y + f.y * t1.y;
+    for(int k = 0; k < 4; ++k) {
+      const double2 f = F[k];
+      const double2 t0 = T[2 * k], t1 = T[2 * k + 1];
+      v.x += f.x * t0.x + f.y * t1.x;
+      if(CPLX) v.y += f.x * t0.y + f.y * t1.y;
+    }
   }
-  if(gt.dense[g]) {
-    const int N = gt.N[g];
-    const double2 d = gt.dense[g][el * N * N + pair];
-    v.x += d.x;
-    v.y += d.y;
+  if(gt.dense64[g]) {
+    const int NN = gt.N[g] * gt.N[g];
+    const double *d = gt.dense64[g] + el * (CPLX ? 2 : 1) * NN + pair;
+    v.x += d[0];
+    if(CPLX) v.y += d[NN];
   }
   return v;
 }
 
 template <bool CPLX>
-__device__ __forceinline__ float2 elem_value32(const GroupTable &gt, int64_t eg, int pair)
+__device__ __forceinline__ float2 elem_value32(const GroupTable &gt, int tensor, int64_t eg, int pair)
 {
   const int g = find_group(gt, eg);
   const int64_t el = eg - gt.e_off[g];
-  const float4 *F = reinterpret_cast<const float4 *>(gt.F32[g] + el * 8);
-  const float4 *T = reinterpret_cast<const float4 *>(gt.T32[g] + (int64_t)pair * 8);
   float2 v = make_float2(0.f, 0.f);
+  if(tensor) {
+    const float4 *F = reinterpret_cast<const float4 *>(gt.F32[g] + el * 8);
+    const float4 *T = reinterpret_cast<const float4 *>(gt.T32[g] + (int64_t)pair * 8);
 #pragma unroll
-  for(int k = 0; k < 2; ++k) {
-    const float4 f = F[k];
-    const float4 ta = T[2 * k], tb = T[2 * k + 1];
-    v.x += f.x * ta.x + f.y * ta.z + f.z * tb.x + f.w * tb.z;
-    if(CPLX) v.y += f.x * ta.y + f.y * ta.w + f.z * tb.y + f.w * tb.w;
+    for(int k = 0; k < 2; ++k) {
+      const float4 f = F[k];
+      const float4 ta = T[2 * k], tb = T[2 * k + 1];
+      v.x += f.x * ta.x + f.y * ta.z + f.z * tb.x + f.w * tb.z;
+      if(CPLX) v.y += f.x * ta.y + f.y * ta.w + f.z * tb.y + f.w * tb.w;
+    }
   }
-  if(gt.dense[g]) {
-    const int N = gt.N[g];
-    const double2 d = gt.dense[g][el * N * N + pair];
-    v.x += (float)d.x;
-    v.y += (float)d.y;
+  if(gt.dense32[g]) {
+    const int NN = gt.N[g] * gt.N[g];
+    const float *d = gt.dense32[g] + el * (CPLX ? 2 : 1) * NN + pair;
+    v.x += d[0];
+    if(CPLX) v.y += d[NN];
   }
   return v;
 }
@@ -675,32 +764,39 @@ struct ValTraits<double2> {
 //      bitwise reproducible.
 
 template <bool CPLX, bool MIXED>
-__device__ __forceinline__ void gather_eval(uint32_t w, int logP, uint32_t pmask,
+__device__ __forceinline__ void gather_eval(uint32_t w, int logP, uint32_t pmask, int tensor,
                                             const typename std::conditional<MIXED, float, double>::type (*s_F)[kChunkElems],
                                             const void *const *s_T, const uint8_t *s_treal,
-                                            const double2 *const *s_D,
+                                            const typename std::conditional<MIXED, float, double>::type *const *s_D,
+                                            const int *s_DNN,
                                             typename std::conditional<MIXED, float, double>::type &vr,
                                             typename std::conditional<MIXED, float, double>::type &vi)
 {
   using R = typename std::conditional<MIXED, float, double>::type;
   const int pair = (int)(w & pmask);
   const int le = (int)((w >> logP) & 0x7Fu);
+  vr = 0;
+  vi = 0;
+  if(s_D[le]) {  // per-point element matrix, planes [real, imaginary][N*N]
+    const R *d = s_D[le] + pair;
+    vr = d[0];
+    if(CPLX) vi = d[s_DNN[le]];
+  }
+  if(!tensor) return;
   R F[8];
 #pragma unroll
   for(int q = 0; q < 7; ++q) F[q] = s_F[q][le];  // structure-of-arrays: no bank conflicts across elements
   F[7] = 0;
-  vr = 0;
-  vi = 0;
   if(!CPLX || s_treal[le]) {
     if(MIXED) {
       const float4 *T = reinterpret_cast<const float4 *>(s_T[le]) + (int64_t)pair * 2;
       const float4 a = T[0], b = T[1];
-      vr = F[0] * a.x + F[1] * a.y + F[2] * a.z + F[3] * a.w + F[4] * b.x + F[5] * b.y + F[6] * b.z;
+      vr += F[0] * a.x + F[1] * a.y + F[2] * a.z + F[3] * a.w + F[4] * b.x + F[5] * b.y + F[6] * b.z;
     }
     else {
       const double2 *T = reinterpret_cast<const double2 *>(s_T[le]) + (int64_t)pair * 4;
       const double2 a = T[0], b = T[1], c = T[2], d = T[3];
-      vr = F[0] * a.x + F[1] * a.y + F[2] * b.x + F[3] * b.y + F[4] * c.x + F[5] * c.y + F[6] * d.x;
+      vr += F[0] * a.x + F[1] * a.y + F[2] * b.x + F[3] * b.y + F[4] * c.x + F[5] * c.y + F[6] * d.x;
     }
   }
   else {
@@ -722,11 +818,6 @@ __device__ __forceinline__ void gather_eval(uint32_t w, int logP, uint32_t pmask
         vi += F[q] * tq.y;
       }
     }
-  }
-  if(s_D[le]) {
-    const double2 d = s_D[le][pair];
-    vr += (R)d.x;
-    vi += (R)d.y;
   }
 }
 
@@ -835,7 +926,7 @@ __device__ __forceinline__ void gather_runs(int t, const int *h, const bool *in,
 // S slots per thread, kChunk / S threads per block
 template <bool CPLX, bool MIXED, bool WIDE, int S>
 __global__ void __launch_bounds__(kChunk / S)
-  k_gather(const GroupTable *__restrict__ gtp, const uint32_t *__restrict__ recs, int64_t nchunks,
+  k_gather(const GroupTable *__restrict__ gtp, const uint32_t *__restrict__ recs, int64_t nchunks, int tensor,
            void *__restrict__ out)
 {
   using R = typename std::conditional<MIXED, float, double>::type;
@@ -845,12 +936,13 @@ __global__ void __launch_bounds__(kChunk / S)
   constexpr int kList = kStride - kChunkElems;  // element-list region of a record
   __shared__ const void *s_gF[kMaxGroups];
   __shared__ const void *s_gT[kMaxGroups];
-  __shared__ const double2 *s_gD[kMaxGroups];
+  __shared__ const R *s_gD[kMaxGroups];
   __shared__ int s_gN[kMaxGroups];
   __shared__ uint8_t s_gR[kMaxGroups];
   __shared__ R s_F[8][kChunkElems];
   __shared__ const void *s_T[kChunkElems];
-  __shared__ const double2 *s_D[kChunkElems];
+  __shared__ const R *s_D[kChunkElems];
+  __shared__ int s_DNN[kChunkElems];
   __shared__ uint8_t s_treal[kChunkElems];
   __shared__ int s_wsum[B / 64];
   __shared__ R s_wx[B / 64], s_wy[B / 64];
@@ -893,7 +985,7 @@ __global__ void __launch_bounds__(kChunk / S)
     s_gF[t] = MIXED ? (const void *)gt.F32[t] : (const void *)gt.F64[t];
     s_gT[t] = MIXED ? (treal ? (const void *)gt.T32r[t] : (const void *)gt.T32[t])
                     : (treal ? (const void *)gt.T64r[t] : (const void *)gt.T64[t]);
-    s_gD[t] = gt.dense[t];
+    s_gD[t] = MIXED ? (const R *)gt.dense32[t] : (const R *)gt.dense64[t];
     s_gN[t] = gt.N[t];
   }
   __syncthreads();
@@ -905,7 +997,8 @@ __global__ void __launch_bounds__(kChunk / S)
     s_treal[j] = s_gR[g];
     s_T[j] = s_gT[g];
     const int N = s_gN[g];
-    s_D[j] = s_gD[g] ? s_gD[g] + el * N * N : nullptr;
+    s_D[j] = s_gD[g] ? s_gD[g] + el * (CPLX ? 2 : 1) * N * N : nullptr;
+    s_DNN[j] = N * N;
     if(MIXED) {
       const float4 *F = reinterpret_cast<const float4 *>(reinterpret_cast<const float *>(s_gF[g]) + el * 8);
       const float4 a = F[0], b = F[1];
@@ -933,7 +1026,7 @@ __global__ void __launch_bounds__(kChunk / S)
     h[k] = in[k] ? (int)((w[k] >> hb) & 1u) : 1;  // past the end: closed, not written
     vr[k] = 0;
     vi[k] = 0;
-    if(in[k]) gather_eval<CPLX, MIXED>(w[k], logP, pmask, s_F, s_T, s_treal, s_D, vr[k], vi[k]);
+    if(in[k]) gather_eval<CPLX, MIXED>(w[k], logP, pmask, tensor, s_F, s_T, s_treal, s_D, s_DNN, vr[k], vi[k]);
   }
   gather_runs<CPLX, R, S, B>(t, h, in, vr, vi, ns, s_wsum, s_wx, s_wy, s_wf, out);
 }
@@ -958,7 +1051,7 @@ __device__ __forceinline__ double2 lane_sum(double2 v)
 }
 
 template <bool CPLX, bool MIXED>
-__global__ void __launch_bounds__(256) k_rhs(const GroupTable *__restrict__ gtp, int64_t n, int nrhs, const int64_t *__restrict__ dc_ptr,
+__global__ void __launch_bounds__(256) k_rhs(const GroupTable *__restrict__ gtp, int tensor, int64_t n, int nrhs, const int64_t *__restrict__ dc_ptr,
                                              const uint32_t *__restrict__ dc_words, const int32_t *__restrict__ dc_fix,
                                              const double2 *__restrict__ uD, const int64_t *__restrict__ lin_ptr,
                                              const uint32_t *__restrict__ lin_words, const double2 *__restrict__ be_all,
@@ -975,11 +1068,11 @@ __global__ void __launch_bounds__(256) k_rhs(const GroupTable *__restrict__ gtp,
     const int pair = (int)(w & ((1u << gt.logP) - 1u));
     double2 a;
     if(MIXED) {
-      const float2 v = elem_value32<CPLX>(gt, eg, pair);
+      const float2 v = elem_value32<CPLX>(gt, tensor, eg, pair);
       a = make_double2(v.x, v.y);
     }
     else {
-      a = elem_value64<CPLX>(gt, eg, pair);
+      a = elem_value64<CPLX>(gt, tensor, eg, pair);
     }
     lift = cadd(lift, cmul(a, uD[dc_fix[k]]));
   }
@@ -1012,14 +1105,16 @@ __global__ void __launch_bounds__(256) k_rhs(const GroupTable *__restrict__ gtp,
 
 // ------------------------------------------------------------------ reference strategy
 template <bool CPLX>
-__global__ void __launch_bounds__(256) k_ref_scatter(int64_t total, const double2 *__restrict__ Ae,
+__global__ void __launch_bounds__(256) k_ref_scatter(int64_t total, int NN, const double *__restrict__ Ae,
                                                      const int32_t *__restrict__ e2n, void *__restrict__ out)
 {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if(t >= total) return;
   const int32_t nz = e2n[t];
   if(nz < 0) return;
-  const double2 v = Ae[t];
+  const int64_t e = t / NN, pair = t % NN;
+  const double *a = Ae + e * (CPLX ? 2 : 1) * NN + pair;  // planes [real, imaginary]
+  const double2 v = make_double2(a[0], CPLX ? a[NN] : 0.);
   if(CPLX) {
     double *o = reinterpret_cast<double *>(out) + 2 * (int64_t)nz;
     unsafeAtomicAdd(o, v.x);
@@ -1033,7 +1128,7 @@ __global__ void __launch_bounds__(256) k_ref_scatter(int64_t total, const double
 template <bool CPLX>
 __global__ void __launch_bounds__(256) k_ref_rhs(int64_t ne, int N, int nrhs, int64_t n,
                                                  const int32_t *__restrict__ num, const int32_t *__restrict__ type,
-                                                 const double2 *__restrict__ Ae, const double2 *__restrict__ be,
+                                                 const double *__restrict__ Ae, const double2 *__restrict__ be,
                                                  int64_t be_stride, const double2 *__restrict__ uD,
                                                  void *__restrict__ out)
 {
@@ -1048,7 +1143,9 @@ __global__ void __launch_bounds__(256) k_ref_rhs(int64_t ne, int N, int nrhs, in
   if(Ae) {
     for(int j = 0; j < N; ++j) {
       if(type[e * N + j] != GFEM_AT_FIXED) continue;
-      const double2 a = Ae[e * N * N + (int64_t)j * N + i];
+      const int64_t NN = (int64_t)N * N;
+      const double *ap = Ae + e * (CPLX ? 2 : 1) * NN + (int64_t)j * N + i;
+      const double2 a = make_double2(ap[0], CPLX ? ap[NN] : 0.);
       lift = cadd(lift, cmul(a, uD[num[e * N + j]]));
     }
   }
@@ -1212,22 +1309,54 @@ hipError_t launch_linear(const Group &g, const double *xyz, int nrhs, hipStream_
   return hipGetLastError();
 }
 
-hipError_t launch_dense(const Group &g, const double *xyz, bool all_terms, hipStream_t s)
+hipError_t launch_dense(const Group &g, const double *xyz, bool all_terms, int planes, hipStream_t s)
 {
   if(g.ne == 0) return hipSuccess;
   const int64_t total = g.ne * (int64_t)g.N * g.N;
   hipLaunchKernelGGL(k_dense, dim3(blocks_for(total, 256)), dim3(256), 0, s, g.d_dev, all_terms ? 1 : 0, g.ne,
-                     g.d_vtx, xyz, g.d_dense);
+                     g.d_vtx, xyz, planes, g.d_dense64);
   return hipGetLastError();
 }
 
-hipError_t launch_gather(const gfem_system &sys, int prec, void *out, hipStream_t s)
+hipError_t launch_dense_to32(const Group &g, int planes, hipStream_t s)
+{
+  const int64_t total = g.ne * planes * (int64_t)g.N * g.N;
+  if(total == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_dense_to32, dim3(blocks_for(total, 256)), dim3(256), 0, s, total, g.d_dense64, g.d_dense32);
+  return hipGetLastError();
+}
+
+hipError_t launch_ref_products(const Group &g, hipStream_t s)
+{
+  const Rule &R = g.rules[g.gemm_rule];
+  const int64_t total = (int64_t)7 * R.Q * g.N * g.N;
+  hipLaunchKernelGGL(k_ref_products<double>, dim3(blocks_for(total, 256)), dim3(256), 0, s, g.N, R.Q, R.d_w, R.d_bf,
+                     R.d_gbf, g.d_R64);
+  hipLaunchKernelGGL(k_ref_products<float>, dim3(blocks_for(total, 256)), dim3(256), 0, s, g.N, R.Q, R.d_w, R.d_bf,
+                     R.d_gbf, g.d_R32);
+  return hipGetLastError();
+}
+
+hipError_t launch_coef_rows(const Group &g, const double *xyz, bool all_terms, int planes, bool fp32, hipStream_t s)
+{
+  const int64_t total = g.ne * (int64_t)g.rules[g.gemm_rule].Q;
+  if(total == 0) return hipSuccess;
+  if(fp32)
+    hipLaunchKernelGGL(k_coef_rows<float>, dim3(blocks_for(total, 256)), dim3(256), 0, s, g.d_dev, all_terms ? 1 : 0,
+                       g.gemm_rule, g.ne, g.d_vtx, xyz, g.d_F64, planes, g.d_A32);
+  else
+    hipLaunchKernelGGL(k_coef_rows<double>, dim3(blocks_for(total, 256)), dim3(256), 0, s, g.d_dev, all_terms ? 1 : 0,
+                       g.gemm_rule, g.ne, g.d_vtx, xyz, g.d_F64, planes, g.d_A64);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather(const gfem_system &sys, int prec, int tensor, void *out, hipStream_t s)
 {
   if(sys.n_chunks == 0) return hipSuccess;
-  const GroupTable *gt = sys.d_table;
+  const GroupTable *gt = tensor ? sys.d_table : sys.d_table_dense;
   const dim3 grid((unsigned)xcd_grid(sys.n_chunks));
 #define GFEM_GATHER_S(C, M, W, S)                                                                            \
-  hipLaunchKernelGGL((k_gather<C, M, W, S>), grid, dim3(kChunk / S), 0, s, gt, sys.d_rec, sys.n_chunks, out)
+  hipLaunchKernelGGL((k_gather<C, M, W, S>), grid, dim3(kChunk / S), 0, s, gt, sys.d_rec, sys.n_chunks, tensor, out)
 #define GFEM_GATHER(C, M)                                                                                    \
   if(sys.rec_wide) {                                                                                         \
     if(sys.gather_slots == 4)                                                                                \
@@ -1262,12 +1391,12 @@ hipError_t launch_gather(const gfem_system &sys, int prec, void *out, hipStream_
   return hipGetLastError();
 }
 
-hipError_t launch_rhs(const gfem_system &sys, int prec, void *out, hipStream_t s)
+hipError_t launch_rhs(const gfem_system &sys, int prec, int tensor, void *out, hipStream_t s)
 {
   if(sys.n == 0) return hipSuccess;
-  const GroupTable *gt = sys.d_table;
+  const GroupTable *gt = tensor ? sys.d_table : sys.d_table_dense;
   const dim3 grid((unsigned)xcd_grid(blocks_for(sys.n * kRhsLanes, 256))), block(256);
-#define GFEM_RHS_ARGS gt, sys.n, sys.nrhs, sys.d_dc_ptr, sys.d_dc_words, sys.d_dc_fix, sys.d_dc, sys.d_lin_ptr, sys.d_lin_words, sys.d_be_all, sys.be_total, out
+#define GFEM_RHS_ARGS gt, tensor, sys.n, sys.nrhs, sys.d_dc_ptr, sys.d_dc_words, sys.d_dc_fix, sys.d_dc, sys.d_lin_ptr, sys.d_lin_words, sys.d_be_all, sys.be_total, out
   if(sys.cplx) {
     if(prec)
       hipLaunchKernelGGL((k_rhs<true, true>), grid, block, 0, s, GFEM_RHS_ARGS);
@@ -1288,12 +1417,13 @@ hipError_t launch_ref_scatter(const gfem_system &sys, const Group &g, hipStream_
 {
   if(g.ne == 0 || !g.has_bilinear) return hipSuccess;
   const int64_t total = g.ne * (int64_t)g.N * g.N;
+  const int NN = g.N * g.N;
   if(sys.cplx)
-    hipLaunchKernelGGL(k_ref_scatter<true>, dim3(blocks_for(total, 256)), dim3(256), 0, s, total, g.d_dense, g.d_e2n,
-                       sys.d_vals[0]);
+    hipLaunchKernelGGL(k_ref_scatter<true>, dim3(blocks_for(total, 256)), dim3(256), 0, s, total, NN, g.d_dense64,
+                       g.d_e2n, sys.d_vals[0]);
   else
-    hipLaunchKernelGGL(k_ref_scatter<false>, dim3(blocks_for(total, 256)), dim3(256), 0, s, total, g.d_dense, g.d_e2n,
-                       sys.d_vals[0]);
+    hipLaunchKernelGGL(k_ref_scatter<false>, dim3(blocks_for(total, 256)), dim3(256), 0, s, total, NN, g.d_dense64,
+                       g.d_e2n, sys.d_vals[0]);
   return hipGetLastError();
 }
 
@@ -1301,7 +1431,7 @@ hipError_t launch_ref_rhs(const gfem_system &sys, const Group &g, hipStream_t s)
 {
   if(g.ne == 0) return hipSuccess;
   const int64_t total = g.ne * (int64_t)g.N;
-  const double2 *Ae = g.has_bilinear ? g.d_dense : nullptr;
+  const double *Ae = g.has_bilinear ? g.d_dense64 : nullptr;
   const double2 *be = g.has_linear ? g.d_be : nullptr;
   if(sys.cplx)
     hipLaunchKernelGGL(k_ref_rhs<true>, dim3(blocks_for(total, 256)), dim3(256), 0, s, g.ne, g.N, sys.nrhs, sys.n,

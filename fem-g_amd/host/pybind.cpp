@@ -38,7 +38,8 @@ gfem_strategy parse_strategy(const std::string &s)
 {
   if(s == "fused") return GFEM_STRATEGY_FUSED;
   if(s == "reference") return GFEM_STRATEGY_REFERENCE;
-  throw std::invalid_argument("strategy must be 'fused' or 'reference'");
+  if(s == "dense") return GFEM_STRATEGY_DENSE;
+  throw std::invalid_argument("strategy must be 'fused', 'reference' or 'dense'");
 }
 
 problem::DofsSort parse_sort(const std::string &s)

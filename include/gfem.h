@@ -109,7 +109,12 @@ typedef enum {
  *               element loop) + atomic scatter; the plain FP64 GPU assembler. */
 typedef enum {
   GFEM_STRATEGY_FUSED = 0,
-  GFEM_STRATEGY_REFERENCE = 1
+  GFEM_STRATEGY_REFERENCE = 1,
+  /* every bilinear term as per-point element matrices B^T W B (the
+   * reference's algorithm, BilinearTerm.cpp:303-343) computed as one GEMM per
+   * group on the matrix cores (FP64, or FP32 for GFEM_PREC_MIXED), then the
+   * atomic-free gather; needs one Gauss rule per group */
+  GFEM_STRATEGY_DENSE = 2
 } gfem_strategy;
 
 /* Coefficient bytecode: pairs (op, arg), evaluated on a complex stack.

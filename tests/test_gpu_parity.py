@@ -113,7 +113,7 @@ def test_multiple_rhs_scalar_product_expression_coefficient():
         dict(kind="linear", domain=omega, gauss=3, source=-2.0, tf_coef=1j, rhs_idx=1),
     ]
     F, P = _pair(spec)
-    for strat in ("fused", "reference"):
+    for strat in ("fused", "reference", "dense"):
         F.assemble(strategy=strat)
         assert_close(F.values(), P.values(), FP64_RTOL, strat + " A")
         for r in range(2):
@@ -197,3 +197,42 @@ def test_systems_on_two_meshes_assemble_in_any_order():
     assert_close(F1.rhs(0), P1.rhs(0), FP64_RTOL, "mesh 1 b")
     F2.assemble()
     assert_close(F2.values(), P2.values(), FP64_RTOL, "mesh 2 A")
+
+
+DENSE_FP32_RTOL = 1e-4  # FP32 products summed over 7Q terms in the GEMM
+
+
+@pytest.mark.parametrize("case", ["poisson2d_p2", "helmholtz3d_p2", "highorder3d_p4", "poisson2d_p3"])
+def test_dense_strategy_matches_oracle(case):
+    """Every bilinear term as per-point element matrices (one GEMM per group
+    on the matrix cores): FP64 to the FP64 bar, FP32 GEMM to its own bar."""
+    print(f"case dense strategy {case}", flush=True)
+    F, P = _pair(_cases()[case]())
+    F.assemble(precision="fp64", strategy="dense")
+    assert_close(F.values(), P.values(), FP64_RTOL, case + " A (dense fp64)")
+    assert_close(F.rhs(0), P.rhs(0), FP64_RTOL, case + " b (dense fp64)")
+    F.assemble(precision="mixed", strategy="dense")
+    assert_close(F.values(), P.values(), DENSE_FP32_RTOL, case + " A (dense fp32)")
+    # the fused strategy afterwards is unaffected by the dense buffers
+    F.assemble(precision="fp64", strategy="fused")
+    assert_close(F.values(), P.values(), FP64_RTOL, case + " A (fused after dense)")
+
+
+def test_per_point_gemm_matches_direct_kernel(monkeypatch):
+    """Per-point bilinear coefficients on P3 triangles: the GEMM form and the
+    direct per-point kernel (GFEM_DENSE_GEMM=0) agree with the oracle."""
+    from gfem import workloads as W, x, y
+    print("case per-point gemm vs direct", flush=True)
+    spec = W.poisson2d(n=6, order=3)
+    spec.terms[0] = dict(kind="bilinear", domain=spec.field_domain, gauss=6, form=1, coef=2.0 + x() * x() - y())
+    spec.terms.append(dict(kind="bilinear", domain=spec.field_domain, gauss=6, form=0, coef=1.0 + y()))
+    F1, P = _pair(spec)
+    F1.assemble()
+    monkeypatch.setenv("GFEM_DENSE_GEMM", "0")
+    F2 = W.build(spec)
+    F2.pre()
+    F2.assemble()
+    assert_close(F1.values(), P.values(), FP64_RTOL, "gemm A")
+    assert_close(F2.values(), P.values(), FP64_RTOL, "direct A")
+    F1.assemble(precision="mixed")
+    assert_close(F1.values(), P.values(), DENSE_FP32_RTOL, "gemm A (fp32)")
