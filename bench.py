@@ -182,6 +182,52 @@ def serving(args):
     return res
 
 
+def highorder(args):
+    """Order-4 tetrahedra (32^3 x 6 = 196,608 tets, N = 35), mass + stiffness,
+    Gauss8: element matrices as per-point B^T W B on the matrix cores (FP64 and
+    FP32 GEMM, Jacobian inverses in FP64) against the tensor-form FP64 path;
+    per-entry error of the FP32 matrix-core path against FP64."""
+    import numpy as np
+    from gfem import gauss_rule, workloads as W
+    t0 = time.perf_counter()
+    spec = W.highorder3d(n=args.ho_n, order=4)
+    F = W.build(spec)
+    F.pre()
+    F.assemble(download=False)
+    log(f"highorder: {spec.name} n={F.n_unknown} nnz={F.nnz} set-up {time.perf_counter() - t0:.1f} s")
+    edofs = spec.element_dofs()
+    out = {"workload": f"unit cube {args.ho_n}^3x6 P4 tets, mass + stiffness, Gauss8", "elements": edofs // 35,
+           "element_dofs": edofs, "n_unknown": int(F.n_unknown), "nnz": int(F.nnz)}
+    ref = None
+    for name, prec, strat in (("tensor_fp64", "fp64", "fused"), ("gemm_fp64", "fp64", "dense"),
+                              ("gemm_fp32", "mixed", "dense")):
+        sec = timed_steps(F, max(2, args.steps // 4), 1, prec, strat)
+        prof, _ = kernel_profile(F, prec, strat, reps=3)
+        rec = {"ms_per_step": sec * 1e3, "GDOF_s": edofs / sec / 1e9,
+               "kernel_ms": {k: round(v, 4) for k, v in prof.items()}}
+        if strat == "dense":
+            K = 7 * len(gauss_rule(3, 8)[1])  # points of the Gauss8 tetrahedron rule
+            flops = 2.0 * 35 * 35 * K * (edofs // 35)
+            gemm_ms = prof.get("element_gemm", float("nan"))
+            peak = 78.6 if prec == "fp64" else 157.3
+            rec["gemm_tflops"] = flops / (gemm_ms * 1e-3) / 1e12
+            rec["gemm_peak_tflops"] = peak
+            rec["gemm_frac"] = rec["gemm_tflops"] / peak
+            rec["gemm_shape"] = f"M=N*N=1225, N=elements={edofs // 35}, K=7Q={K}"
+        F.assemble(precision=prec, strategy=strat)
+        vals = F.values()
+        if ref is None:
+            ref = vals
+        else:
+            err = np.abs(vals - ref)
+            floor = 1e-2 * np.abs(ref).max()
+            rec["max_rel_err_vs_tensor_fp64"] = float((err / (np.abs(ref) + floor)).max())
+        out[name] = rec
+        log(f"highorder {name}: {sec * 1e3:.2f} ms/step {edofs / sec / 1e9:.2f} GDOF/s "
+            f"{'' if strat != 'dense' else '%.1f TFLOP/s GEMM' % rec['gemm_tflops']}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -193,6 +239,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the mixed / reference-strategy lines")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving stream")
+    ap.add_argument("--no-highorder", action="store_true", help="skip the order-4 matrix-core comparison")
+    ap.add_argument("--ho-n", type=int, default=32)
     ap.add_argument("--serve-requests", type=int, default=256)
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-tol", type=float, default=1e-10)
@@ -252,6 +300,10 @@ def main():
                                        "what": "per-Gauss-point dense element matrices + atomic CSR scatter"}
         log(f"reference-order fp64: {sec_r * 1e3:.3f} ms/step  {edofs / sec_r / 1e9:.2f} GDOF/s")
 
+    ho = None
+    if not args.no_highorder:
+        ho = highorder(args)
+
     serve = None
     if not args.no_serve:
         serve = serving(args)
@@ -292,6 +344,8 @@ def main():
     line.update(extra)
     if serve is not None:
         line["serving"] = serve
+    if ho is not None:
+        line["highorder"] = ho
     print(json.dumps(line), flush=True)
 
 
