@@ -278,6 +278,7 @@ def main():
     value = edofs / sec / 1e9
     log(f"fused fp64: {sec * 1e3:.3f} ms/step  {value:.2f} GDOF/s")
 
+    tuning = {"fp64": F.tuning("fp64")}
     prof, raw = kernel_profile(F, "fp64", "fused")
     gather_ms = prof.get("gather", float("nan"))
     gbytes = F.gather_bytes("fp64")
@@ -340,6 +341,7 @@ def main():
                      "kernel": GATHER_KERNEL.format(args.gather_slots), "kernel_ms": gather_ms,
                      "algorithmic_bytes": gbytes},
         "cpu_baseline": cpu,
+        "autotuned": tuning,
     }
     line.update(extra)
     if serve is not None:

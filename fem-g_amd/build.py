@@ -48,7 +48,7 @@ def build(verbose=False):
 
     dev_objs = [os.path.join(OBJ, "kernels.o"), os.path.join(OBJ, "gfem_api.o"), os.path.join(OBJ, "jit.o"),
                 os.path.join(OBJ, "solver.o"), os.path.join(OBJ, "batch.o")]
-    host_srcs = ["model.cpp", "gmshfem.cpp", "serve.cpp"]
+    host_srcs = ["model.cpp", "msh.cpp", "gmshfem.cpp", "serve.cpp"]
     host_objs = [os.path.join(OBJ, s.replace(".cpp", ".o")) for s in host_srcs]
     py_obj = os.path.join(OBJ, "pybind.o")
 

@@ -690,7 +690,11 @@ gfem_status gfem_system_finalize(gfem_system *sys)
     }
     sys->n_chunks = (int64_t)chunk;
     sys->n_words = total;
-    if(const char *e = std::getenv("GFEM_GATHER_SLOTS")) sys->gather_slots = std::atoi(e) == 2 ? 2 : 4;
+    if(const char *e = std::getenv("GFEM_GATHER_SLOTS")) {
+      // a fixed choice, not tuned
+      sys->gather_slots[0] = sys->gather_slots[1] = std::atoi(e) == 2 ? 2 : 4;
+      sys->tuned[0] = sys->tuned[1] = true;
+    }
     GFEM_HIP(ctx, dupload(&sys->d_rec, rec.data(), rec.size()));
   }
 
@@ -886,6 +890,83 @@ static gfem_status ensure_dense_strategy(gfem_system *sys)
   return GFEM_OK;
 }
 
+// Autotuner: at the first assembly in a precision, time the tile choices of
+// the linear-term and gather kernels on this system's own data (one warm-up
+// and three timed launches each, HIP events on the assembly stream) and keep
+// the fastest.  GFEM_AUTOTUNE=0 keeps the defaults.
+static bool autotune_enabled()
+{
+  static const bool on = [] {
+    const char *e = std::getenv("GFEM_AUTOTUNE");
+    return !(e && e[0] == '0');
+  }();
+  return on;
+}
+
+static gfem_status autotune(gfem_system *sys, int p, int tensor, void *vals, void *rhs, hipStream_t s)
+{
+  gfem_ctx *ctx = sys->ctx;
+  hipEvent_t a = nullptr, b = nullptr;
+  GFEM_HIP(ctx, hipEventCreate(&a));
+  GFEM_HIP(ctx, hipEventCreate(&b));
+  gfem_status st = GFEM_OK;
+  auto timed = [&](auto &&launch, float &ms) -> hipError_t {
+    hipError_t e = launch();
+    if(e == hipSuccess) e = hipEventRecord(a, s);
+    for(int r = 0; r < 3 && e == hipSuccess; ++r) e = launch();
+    if(e == hipSuccess) e = hipEventRecord(b, s);
+    if(e == hipSuccess) e = hipEventSynchronize(b);
+    if(e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+    return e;
+  };
+  bool any_linear = false;
+  for(const auto &g : sys->groups) any_linear = any_linear || g.has_linear;
+  if(any_linear) {
+    float best = 1e30f;
+    int pick = sys->linear_eb_shift[p];
+    for(int shift = 1; shift <= 3; ++shift) {
+      float ms = 0.f;
+      const hipError_t e = timed(
+        [&]() {
+          for(const auto &g : sys->groups)
+            if(g.has_linear) {
+              const hipError_t r = launch_linear(g, sys->d_xyz, sys->nrhs, shift, s);
+              if(r != hipSuccess) return r;
+            }
+          return hipSuccess;
+        },
+        ms);
+      if(e != hipSuccess) {
+        st = hip_fail(ctx, e, "autotune linear");
+        break;
+      }
+      if(ms < best) {
+        best = ms;
+        pick = shift;
+      }
+    }
+    sys->linear_eb_shift[p] = pick;
+  }
+  if(st == GFEM_OK && sys->n_chunks > 0) {
+    // the slot count changes the order of the run sums: the default (4)
+    // stays unless the other is clearly faster, so the choice, and with it
+    // the rounding, is stable from run to run
+    float t[2] = {0.f, 0.f};
+    const int cand[2] = {4, 2};
+    for(int c = 0; c < 2 && st == GFEM_OK; ++c) {
+      sys->gather_slots[p] = cand[c];
+      const hipError_t e = timed([&]() { return launch_gather(*sys, p, tensor, vals, s); }, t[c]);
+      if(e != hipSuccess) st = hip_fail(ctx, e, "autotune gather");
+    }
+    sys->gather_slots[p] = (st == GFEM_OK && t[1] < 0.95f * t[0]) ? 2 : 4;
+  }
+  (void)rhs;
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  if(st == GFEM_OK) sys->tuned[p] = true;
+  return st;
+}
+
 static gfem_status check_assemble(gfem_system *sys, gfem_precision prec, gfem_strategy strategy)
 {
   if(!sys) return fail(nullptr, GFEM_ERR_INVALID_ARG, "system is null");
@@ -965,10 +1046,14 @@ static gfem_status assemble_impl(gfem_system *sys, gfem_precision prec, gfem_str
         prof.end();
       }
     }
+    if(!sys->tuned[p] && autotune_enabled()) {
+      st = autotune(sys, p, all ? 0 : 1, vals_out, rhs_out, s);
+      if(st != GFEM_OK) return st;
+    }
     for(auto &g : sys->groups)
       if(g.has_linear) {
         prof.begin("linear");
-        GFEM_HIP(ctx, launch_linear(g, sys->d_xyz, sys->nrhs, s));
+        GFEM_HIP(ctx, launch_linear(g, sys->d_xyz, sys->nrhs, sys->linear_eb_shift[p], s));
         prof.end();
       }
     prof.begin("rhs");
@@ -989,7 +1074,7 @@ static gfem_status assemble_impl(gfem_system *sys, gfem_precision prec, gfem_str
     prof.end();
     if(g.has_linear) {
       prof.begin("linear");
-      GFEM_HIP(ctx, launch_linear(g, sys->d_xyz, sys->nrhs, s));
+      GFEM_HIP(ctx, launch_linear(g, sys->d_xyz, sys->nrhs, sys->linear_eb_shift[p], s));
       prof.end();
     }
     if(g.has_bilinear) {
@@ -1045,6 +1130,17 @@ gfem_status gfem_term_update(gfem_system *sys, int group, int term, double coef_
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   GFEM_HIP(ctx, hipMemcpyAsync(g.d_terms + term, &d, sizeof(DevTerm), hipMemcpyHostToDevice, s));
   GFEM_HIP(ctx, hipStreamSynchronize(s));
+  return GFEM_OK;
+}
+
+gfem_status gfem_tuning(const gfem_system *sys, gfem_precision prec, int *gather_slots, int *linear_eb_shift,
+                        int *spmv_lanes)
+{
+  if(!sys) return fail(nullptr, GFEM_ERR_INVALID_ARG, "system is null");
+  const int p = prec == GFEM_PREC_MIXED ? 1 : 0;
+  if(gather_slots) *gather_slots = sys->gather_slots[p];
+  if(linear_eb_shift) *linear_eb_shift = sys->linear_eb_shift[p];
+  if(spmv_lanes) *spmv_lanes = sys->spmv_lanes;
   return GFEM_OK;
 }
 

@@ -188,7 +188,14 @@ struct gfem_system {
   double2 *d_be_all = nullptr;  // element load vectors of all groups, [nrhs][be_total]
   int64_t be_total = 0;
   int rec_wide = 0;           // 32-bit contribution words
-  int gather_slots = 4;       // slots per gather thread: 4 (128-thread blocks) or 2 (256)
+  // tile choices (autotuned per precision at the first assembly, see
+  // gfem_api.cpp autotune): gather slots per thread 4 (128-thread blocks) or
+  // 2 (256); linear-term elements per block ~ (256 << linear_eb_shift) / Q
+  int gather_slots[2] = {4, 4};
+  int linear_eb_shift[2] = {2, 2};
+  bool tuned[2] = {false, false};
+  int spmv_lanes = 8;         // solver SpMV lanes per row (autotuned at the first solve)
+  bool spmv_tuned = false;
   int rec_stride = gfem::kRecNarrow;  // uint32 per record
   int64_t gather_grid = 0;    // persistent grid (multiple of the XCD count)
   int64_t n_words = 0;
@@ -264,7 +271,7 @@ gfem_status solve_gmres_ir(gfem_system *sys, int rhs_idx, double tol, int restar
 hipError_t launch_basis(const Group &g, Rule &r, hipStream_t s);
 hipError_t launch_tensor(const Group &g, hipStream_t s);
 hipError_t launch_geometry(const Group &g, const double *xyz, hipStream_t s);
-hipError_t launch_linear(const Group &g, const double *xyz, int nrhs, hipStream_t s);
+hipError_t launch_linear(const Group &g, const double *xyz, int nrhs, int eb_shift, hipStream_t s);
 hipError_t launch_dense(const Group &g, const double *xyz, bool all_terms, int planes, hipStream_t s);
 hipError_t launch_dense_to32(const Group &g, int planes, hipStream_t s);
 hipError_t launch_ref_products(const Group &g, hipStream_t s);

@@ -1229,7 +1229,7 @@ static void linear_n(const LinTerm &a, const Rule &R, size_t lds, const Group &g
                        g.ne, g.d_vtx, xyz, g.d_F64, be);
 }
 
-hipError_t launch_linear(const Group &g, const double *xyz, int nrhs, hipStream_t s)
+hipError_t launch_linear(const Group &g, const double *xyz, int nrhs, int eb_shift, hipStream_t s)
 {
   if(g.ne == 0 || !g.has_linear) return hipSuccess;
   for(int r = 0; r < nrhs; ++r) {
@@ -1250,7 +1250,7 @@ hipError_t launch_linear(const Group &g, const double *xyz, int nrhs, hipStream_
       a.nv = g.nv;
       a.accumulate = first ? 0 : 1;
       // point-parallel kernels: EB elements per block, about 4 points per thread
-      const int EB = std::max(1, std::min(128, 1024 / std::max(R.Q, 1)));
+      const int EB = std::max(1, std::min(128, (256 << eb_shift) / std::max(R.Q, 1)));
       const size_t lds_pp = sizeof(double) * (size_t)(R.Q * g.N + 4 * R.Q + 13 * EB + 1) +
                             sizeof(double2) * (size_t)EB * R.Q;
       const unsigned nblk = (unsigned)((g.ne + EB - 1) / EB);
@@ -1359,13 +1359,13 @@ hipError_t launch_gather(const gfem_system &sys, int prec, int tensor, void *out
   hipLaunchKernelGGL((k_gather<C, M, W, S>), grid, dim3(kChunk / S), 0, s, gt, sys.d_rec, sys.n_chunks, tensor, out)
 #define GFEM_GATHER(C, M)                                                                                    \
   if(sys.rec_wide) {                                                                                         \
-    if(sys.gather_slots == 4)                                                                                \
+    if(sys.gather_slots[prec] == 4)                                                                          \
       GFEM_GATHER_S(C, M, true, 4);                                                                          \
     else                                                                                                     \
       GFEM_GATHER_S(C, M, true, 2);                                                                          \
   }                                                                                                          \
   else {                                                                                                     \
-    if(sys.gather_slots == 4)                                                                                \
+    if(sys.gather_slots[prec] == 4)                                                                          \
       GFEM_GATHER_S(C, M, false, 4);                                                                         \
     else                                                                                                     \
       GFEM_GATHER_S(C, M, false, 2);                                                                         \

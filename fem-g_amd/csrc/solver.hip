@@ -15,6 +15,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <vector>
 
 #include "internal.h"
@@ -61,35 +62,50 @@ __device__ __forceinline__ double2 zconjmul(double2 a, double2 b)  // conj(a) b
   return make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
 }
 
+template <int L = kRowLanes>
 __device__ __forceinline__ double2 lanes_sum(double2 v)
 {
 #pragma unroll
-  for(int d = 1; d < kRowLanes; d <<= 1) {
-    v.x += __shfl_xor(v.x, d, kRowLanes);
-    v.y += __shfl_xor(v.y, d, kRowLanes);
+  for(int d = 1; d < L; d <<= 1) {
+    v.x += __shfl_xor(v.x, d, L);
+    v.y += __shfl_xor(v.y, d, L);
   }
   return v;
 }
 
 // y = A x (FP32 values and vectors, FP32 products summed per lane in FP32,
-// lanes combined in a fixed butterfly)
-template <bool C>
+// L lanes per row combined in a fixed butterfly; L is autotuned)
+template <bool C, int L>
 __global__ void __launch_bounds__(256) k_spmv32(int64_t n, const int64_t *__restrict__ ai, const int32_t *__restrict__ aj,
                                                 const typename Sc<C>::f32 *__restrict__ a,
                                                 const typename Sc<C>::f32 *__restrict__ x,
                                                 typename Sc<C>::f32 *__restrict__ y)
 {
-  const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / kRowLanes;
-  const int l = (int)(threadIdx.x % kRowLanes);
+  const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / L;
+  const int l = (int)(threadIdx.x % L);
   if(row >= n) return;  // whole lane groups leave together
   float sr = 0.f, si = 0.f;
-  for(int64_t k = ai[row] + l; k < ai[row + 1]; k += kRowLanes) {
+  for(int64_t k = ai[row] + l; k < ai[row + 1]; k += L) {
     const double2 av = ld_d(a, k), xv = ld_d(x, aj[k]);
     sr += (float)av.x * (float)xv.x - (float)av.y * (float)xv.y;
     si += (float)av.x * (float)xv.y + (float)av.y * (float)xv.x;
   }
-  const double2 s = lanes_sum(make_double2(sr, si));
+  const double2 s = lanes_sum<L>(make_double2(sr, si));
   if(l == 0) st(y, row, s);
+}
+
+template <bool C>
+hipError_t spmv32(int lanes, int64_t n, const int64_t *ai, const int32_t *aj, const typename Sc<C>::f32 *a,
+                  const typename Sc<C>::f32 *x, typename Sc<C>::f32 *y, hipStream_t s)
+{
+  const unsigned nb = (unsigned)((n * lanes + 255) / 256);
+  if(lanes == 4)
+    hipLaunchKernelGGL((k_spmv32<C, 4>), dim3(nb), dim3(256), 0, s, n, ai, aj, a, x, y);
+  else if(lanes == 16)
+    hipLaunchKernelGGL((k_spmv32<C, 16>), dim3(nb), dim3(256), 0, s, n, ai, aj, a, x, y);
+  else
+    hipLaunchKernelGGL((k_spmv32<C, 8>), dim3(nb), dim3(256), 0, s, n, ai, aj, a, x, y);
+  return hipGetLastError();
 }
 
 // r = -b - A x in FP64
@@ -420,6 +436,39 @@ static gfem_status gmres_ir(gfem_system *sys, int rhs_idx, double tol, int m, in
 
   GFEM_HIP(ctx, hipMemsetAsync(x, 0, sizeof(T64) * (size_t)n, s));
   hipLaunchKernelGGL((k_round<T64, T32>), dim3(blocks(nnz, 256)), dim3(256), 0, s, nnz, a64, a32);
+  if(!sys->spmv_tuned) {
+    // autotune the SpMV lanes per row on this matrix (one warm-up and three
+    // timed products each; w is scratch here)
+    const char *at = std::getenv("GFEM_AUTOTUNE");
+    if(!(at && at[0] == '0')) {
+      hipEvent_t e0, e1;
+      GFEM_HIP(ctx, hipEventCreate(&e0));
+      GFEM_HIP(ctx, hipEventCreate(&e1));
+      // the lane count changes the rounding of the FP32 products: the default
+      // (8) stays unless another is clearly faster (stable from run to run)
+      float best = 1e30f, def = 1e30f;
+      int pick = 8;
+      hipLaunchKernelGGL((k_round<T64, T32>), dim3(blocks(n, 256)), dim3(256), 0, s, n, b, V);
+      for(int lanes : {8, 4, 16}) {
+        GFEM_HIP(ctx, spmv32<C>(lanes, n, sys->d_ai, sys->d_aj, a32, V, w, s));
+        GFEM_HIP(ctx, hipEventRecord(e0, s));
+        for(int r = 0; r < 3; ++r) GFEM_HIP(ctx, spmv32<C>(lanes, n, sys->d_ai, sys->d_aj, a32, V, w, s));
+        GFEM_HIP(ctx, hipEventRecord(e1, s));
+        GFEM_HIP(ctx, hipEventSynchronize(e1));
+        float ms = 0.f;
+        GFEM_HIP(ctx, hipEventElapsedTime(&ms, e0, e1));
+        if(lanes == 8) def = ms;
+        if(ms < best && (lanes == 8 || ms < 0.95f * def)) {
+          best = ms;
+          pick = lanes;
+        }
+      }
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      sys->spmv_lanes = pick;
+    }
+    sys->spmv_tuned = true;
+  }
   // ||b||
   GFEM_HIP(ctx, G.dots(b, n, 1, b, nrm + 1, 0));
   GFEM_HIP(ctx, hipMemcpyAsync(hest, nrm, sizeof(double2) * 2, hipMemcpyDeviceToHost, s));
@@ -450,7 +499,7 @@ static gfem_status gmres_ir(gfem_system *sys, int rhs_idx, double tol, int m, in
     int k = 0;
     for(int j = 0; j < m && iters < max_iter; ++j) {
       T32 *vj = V + (int64_t)j * n;
-      hipLaunchKernelGGL(k_spmv32<C>, dim3(rb), dim3(256), 0, s, n, sys->d_ai, sys->d_aj, a32, vj, w);
+      GFEM_HIP(ctx, spmv32<C>(sys->spmv_lanes, n, sys->d_ai, sys->d_aj, a32, vj, w, s));
       double2 *col = H + (int64_t)j * (m + 1);
       // CGS2: h = V^H w, w -= V h, twice
       GFEM_HIP(ctx, G.dots(V, n, j + 1, w, col, 0));

@@ -18,8 +18,8 @@ except ImportError as e:  # pragma: no cover - exercised only on a broken build
 
 from ._gfem_host import (Domain, Field, Model, ScalarFunction, Server, Unknown, abs, cos,  # noqa: F401,E402
                          dof, dt2_dof, dt_dof, exp, from_arrays, gauss_rule, grad,
-                         hilbert_index, hilbert_key, lattice_nodes, pow, set_current_model, set_device,
-                         sin, sqrt, tf, unit_cube, unit_square, x, y, z)
+                         hilbert_index, hilbert_key, lattice_nodes, pow, read_msh, set_current_model, set_device,
+                         sin, sqrt, tf, unit_cube, unit_square, write_msh, x, y, z)
 
 _FORMULATIONS = {
     "f64": _h.Formulation_f64,

@@ -24,11 +24,20 @@ struct Block {
   int64_t size() const { return npe ? (int64_t)conn.size() / npe : 0; }
 };
 
+// A physical group: a named set of entities of one dimension.
+struct Physical {
+  int dim = 0, tag = 0;
+  std::string name;
+  std::vector<int> entities;
+};
+
 struct Model {
   std::string name;
   int dim = 0;
   std::vector<double> xyz;  // node tag t -> xyz[3(t-1) .. 3(t-1)+2]
   std::vector<Block> blocks;
+  std::vector<Physical> physicals;
+  std::vector<int64_t> fileNodeTags;  // tags of nodes 1..n in the file they came from (empty: the same)
 
   int64_t numNodes() const { return (int64_t)xyz.size() / 3; }
   // entities in gmsh order: dimension ascending, tag ascending
@@ -38,6 +47,9 @@ struct Model {
   std::vector<int> elementTypes(int dim) const;
   const Block *block(int dim, int tag, int type) const;
   void boundingBox(double mn[3], double mx[3]) const;
+  // entities of physical group (dim, tag); empty when there is none
+  std::vector<int> physicalEntities(int dim, int tag) const;
+  const Physical *physical(const std::string &name) const;
   uint64_t uid = 0;  // unique id, changes with every new model
 };
 
@@ -57,6 +69,15 @@ std::shared_ptr<Model> unitCube(int n, int order, double jitter, uint64_t seed);
 // Build a model from raw arrays (node coordinates + element blocks).
 std::shared_ptr<Model> fromArrays(const std::string &name, int dim, const std::vector<double> &xyz,
                                   const std::vector<Block> &blocks);
+
+// Gmsh MSH 4.1 files, ASCII (the format gmsh::open / gmsh::write exchange with
+// GmshFEM; element node order is gmsh's, as everywhere in this model).  The
+// reader takes $PhysicalNames, $Entities (physical tags), $Nodes and
+// $Elements and skips other sections; node tags are compacted to 1..n in
+// increasing order (the file's tags are kept in fileNodeTags).  Errors throw
+// std::runtime_error with the line number.
+std::shared_ptr<Model> readMsh(const std::string &path);
+void writeMsh(const Model &m, const std::string &path);
 
 // the "current" model, as gmsh::model keeps one
 void setCurrent(std::shared_ptr<Model> m);

@@ -144,6 +144,17 @@ void bind_formulation(py::module_ &m, const char *name)
            for(int k = 0; k < n && k < 256; ++k) out.append(py::make_tuple(std::string(names[k]), ms[k]));
            return out;
          })
+    .def("tuning",
+         [](F &f, const std::string &prec) {
+           int gs = 0, eb = 0, lanes = 0;
+           check(gfem_tuning(f.system(), parse_precision(prec), &gs, &eb, &lanes), "gfem_tuning");
+           py::dict d;
+           d["gather_slots"] = gs;
+           d["linear_eb_shift"] = eb;
+           d["spmv_lanes"] = lanes;
+           return d;
+         },
+         py::arg("precision") = "fp64")
     .def("gather_bytes",
          [](F &f, const std::string &prec) {
            double b = 0.;
@@ -187,6 +198,15 @@ PYBIND11_MODULE(_gfem_host, m)
            return a;
          })
     .def("entities", &model::Model::entities, py::arg("dim") = -1)
+    .def_property_readonly("physicals",
+                           [](const model::Model &mo) {
+                             py::list out;
+                             for(const auto &p : mo.physicals)
+                               out.append(py::dict(py::arg("dim") = p.dim, py::arg("tag") = p.tag,
+                                                   py::arg("name") = p.name, py::arg("entities") = p.entities));
+                             return out;
+                           })
+    .def_property_readonly("file_node_tags", [](const model::Model &mo) { return mo.fileNodeTags; })
     .def("blocks", [](const model::Model &mo) {
       py::list out;
       for(const auto &b : mo.blocks) {
@@ -202,6 +222,8 @@ PYBIND11_MODULE(_gfem_host, m)
   m.def("unit_cube", &model::unitCube, py::arg("n"), py::arg("order") = 1, py::arg("jitter") = 0.,
         py::arg("seed") = 0);
   m.def("set_current_model", &model::setCurrent);
+  m.def("read_msh", &model::readMsh, py::arg("path"));
+  m.def("write_msh", &model::writeMsh, py::arg("model"), py::arg("path"));
   m.def("from_arrays", [](const std::string &name, int dim, py::array_t<double> xyz, py::list blocks) {
     std::vector<double> x(xyz.data(), xyz.data() + xyz.size());
     std::vector<model::Block> bl;

@@ -287,6 +287,13 @@ gfem_status gfem_batch_result(gfem_batch *batch, int64_t id, void *x_host, size_
 gfem_status gfem_set_profiling(gfem_system *sys, int on);
 gfem_status gfem_last_kernel_times(const gfem_system *sys, int max, const char **names, double *ms, int *count);
 
+/* Tile choices in use for `prec` (picked by timing the candidates on the
+ * system at its first assembly / solve unless GFEM_AUTOTUNE=0): gather slots
+ * per thread (2 or 4), linear-term elements per block ~ (256 << shift) / Q,
+ * solver SpMV lanes per row (4, 8 or 16).  No counterpart in the reference. */
+gfem_status gfem_tuning(const gfem_system *sys, gfem_precision prec, int *gather_slots, int *linear_eb_shift,
+                        int *spmv_lanes);
+
 /* Algorithmic HBM bytes of the fused matrix gather kernel for `prec`
  * (contribution map + element factors + CSR values written). */
 gfem_status gfem_gather_bytes(const gfem_system *sys, gfem_precision prec, double *bytes);

@@ -236,3 +236,25 @@ def test_per_point_gemm_matches_direct_kernel(monkeypatch):
     assert_close(F2.values(), P.values(), FP64_RTOL, "direct A")
     F1.assemble(precision="mixed")
     assert_close(F1.values(), P.values(), DENSE_FP32_RTOL, "gemm A (fp32)")
+
+
+def test_autotuned_assembly_matches_untuned(monkeypatch):
+    """The autotuner's choices are in their candidate sets and the tuned
+    assembly equals the untuned one (GFEM_AUTOTUNE=0) to round-off."""
+    from gfem import workloads as W
+    print("case autotune", flush=True)
+    spec = W.helmholtz3d(n=6, order=2)
+    F1, P = _pair(spec)
+    F1.assemble()
+    F1.assemble(precision="mixed")
+    for prec in ("fp64", "mixed"):
+        t = F1.tuning(prec)
+        assert t["gather_slots"] in (2, 4) and t["linear_eb_shift"] in (1, 2, 3), t
+    monkeypatch.setenv("GFEM_AUTOTUNE", "0")
+    F2 = W.build(spec)
+    F2.pre()
+    F2.assemble()
+    assert F2.tuning("fp64") == dict(gather_slots=4, linear_eb_shift=2, spmv_lanes=8)
+    assert_close(F1.values(), F2.values(), 1e-14, "tuned vs untuned A")
+    assert_close(F1.rhs(0), F2.rhs(0), 1e-14, "tuned vs untuned b")
+    assert_close(F1.values(), P.values(), FP64_RTOL, "tuned A")

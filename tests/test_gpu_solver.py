@@ -110,3 +110,11 @@ def test_node_values_carry_dirichlet_data():
     assert info["converged"]
     for tag in range(1, spec.model.num_nodes + 1):
         assert abs(F.node_value(tag) - 1.0) < 1e-9
+
+
+def test_spmv_lanes_are_tuned_at_the_first_solve():
+    from gfem import workloads as W
+    print("case spmv autotune", flush=True)
+    F, info = _solve(W.helmholtz3d(n=6, order=2), tol=1e-10)
+    assert info["converged"]
+    assert F.tuning("fp64")["spmv_lanes"] in (4, 8, 16)
