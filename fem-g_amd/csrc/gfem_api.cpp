@@ -690,6 +690,13 @@ gfem_status gfem_system_finalize(gfem_system *sys)
     }
     sys->n_chunks = (int64_t)chunk;
     sys->n_words = total;
+    // switches of this system (environment at finalize): GFEM_AUTOTUNE=0 keeps
+    // the default tile choices, GFEM_DENSE_GEMM=0 computes per-point element
+    // matrices with the direct kernel instead of the GEMM form
+    const char *at = std::getenv("GFEM_AUTOTUNE");
+    sys->autotune = !(at && at[0] == '0');
+    const char *dg = std::getenv("GFEM_DENSE_GEMM");
+    sys->dense_gemm = !(dg && dg[0] == '0');
     if(const char *e = std::getenv("GFEM_GATHER_SLOTS")) {
       // a fixed choice, not tuned
       sys->gather_slots[0] = sys->gather_slots[1] = std::atoi(e) == 2 ? 2 : 4;
@@ -803,16 +810,6 @@ static gfem_status apply_updates(gfem_system *sys, hipStream_t s)
   return GFEM_OK;
 }
 
-// GEMM form of the per-point element matrices (GFEM_DENSE_GEMM=0 keeps the
-// direct per-point kernel)
-static bool gemm_enabled()
-{
-  static const bool on = [] {
-    const char *e = std::getenv("GFEM_DENSE_GEMM");
-    return !(e && e[0] == '0');
-  }();
-  return on;
-}
 
 // reference products R of a group (once) and its A rows / element matrices
 // for one precision
@@ -893,15 +890,7 @@ static gfem_status ensure_dense_strategy(gfem_system *sys)
 // Autotuner: at the first assembly in a precision, time the tile choices of
 // the linear-term and gather kernels on this system's own data (one warm-up
 // and three timed launches each, HIP events on the assembly stream) and keep
-// the fastest.  GFEM_AUTOTUNE=0 keeps the defaults.
-static bool autotune_enabled()
-{
-  static const bool on = [] {
-    const char *e = std::getenv("GFEM_AUTOTUNE");
-    return !(e && e[0] == '0');
-  }();
-  return on;
-}
+// the fastest.  GFEM_AUTOTUNE=0 (read at finalize) keeps the defaults.
 
 static gfem_status autotune(gfem_system *sys, int p, int tensor, void *vals, void *rhs, hipStream_t s)
 {
@@ -1028,7 +1017,7 @@ static gfem_status assemble_impl(gfem_system *sys, gfem_precision prec, gfem_str
       GFEM_HIP(ctx, launch_geometry(g, sys->d_xyz, s));
       prof.end();
       if(!(all ? g.has_bilinear : g.has_dense)) continue;
-      if(g.gemm_rule >= 0 && gemm_enabled()) {
+      if(g.gemm_rule >= 0 && (all || sys->dense_gemm)) {
         st = ensure_gemm(sys, g, fp32);
         if(st != GFEM_OK) return st;
         prof.begin("coef_rows");
@@ -1046,7 +1035,7 @@ static gfem_status assemble_impl(gfem_system *sys, gfem_precision prec, gfem_str
         prof.end();
       }
     }
-    if(!sys->tuned[p] && autotune_enabled()) {
+    if(!sys->tuned[p] && sys->autotune) {
       st = autotune(sys, p, all ? 0 : 1, vals_out, rhs_out, s);
       if(st != GFEM_OK) return st;
     }

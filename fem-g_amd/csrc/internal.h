@@ -194,6 +194,8 @@ struct gfem_system {
   int gather_slots[2] = {4, 4};
   int linear_eb_shift[2] = {2, 2};
   bool tuned[2] = {false, false};
+  bool autotune = true;        // GFEM_AUTOTUNE at finalize
+  bool dense_gemm = true;      // GFEM_DENSE_GEMM at finalize: GEMM form of per-point terms
   int spmv_lanes = 8;         // solver SpMV lanes per row (autotuned at the first solve)
   bool spmv_tuned = false;
   int rec_stride = gfem::kRecNarrow;  // uint32 per record

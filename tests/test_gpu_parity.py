@@ -245,8 +245,8 @@ def test_autotuned_assembly_matches_untuned(monkeypatch):
     print("case autotune", flush=True)
     spec = W.helmholtz3d(n=6, order=2)
     F1, P = _pair(spec)
-    F1.assemble()
     F1.assemble(precision="mixed")
+    F1.assemble()  # FP64 last: values() below are the FP64 assembly
     for prec in ("fp64", "mixed"):
         t = F1.tuning(prec)
         assert t["gather_slots"] in (2, 4) and t["linear_eb_shift"] in (1, 2, 3), t
