@@ -63,6 +63,7 @@ void free_group(Group &g)
   dfree(g.d_e2n);
   dfree(g.d_num);
   dfree(g.d_type);
+  dfree(g.d_iperm);
   for(auto &r : g.rules) {
     dfree(r.d_w);
     dfree(r.d_pts);
@@ -225,6 +226,7 @@ gfem_status gfem_system_create(gfem_ctx *ctx, gfem_scalar scalar, int64_t n, con
   sys->cplx = (scalar == GFEM_SCALAR_C128 || scalar == GFEM_SCALAR_C64);
   sys->n = n;
   sys->nnz = nnz;
+  sys->n_nodes = ctx->n_nodes;
   sys->symmetric = symmetric;
   sys->n_fixed = n_fixed;
   sys->nrhs = n_rhs;
@@ -371,6 +373,7 @@ gfem_status gfem_group_add(gfem_system *sys, int dim, int type, int64_t ne, cons
     if(elem2nnz)
       for(int64_t p = 0; p < NN; ++p) g.h_e2n[s * NN + p] = elem2nnz[e * NN + p];
   }
+  g.host_maps = true;
   sys->groups.push_back(std::move(g));
   if(group_out) *group_out = (int)sys->groups.size() - 1;
   return GFEM_OK;
@@ -425,165 +428,13 @@ gfem_status gfem_group_add_term(gfem_system *sys, int group, const gfem_term_des
   return GFEM_OK;
 }
 
-gfem_status gfem_system_finalize(gfem_system *sys)
+// Gather records and RHS maps built on the host from the host copies of the
+// maps (GFEM_HOST_FINALIZE=1): the reference for the device builder.
+static gfem_status build_maps_host(gfem_system *sys)
 {
-  if(!sys) return fail(nullptr, GFEM_ERR_INVALID_ARG, "system is null");
   gfem_ctx *ctx = sys->ctx;
-  if(sys->finalized) return fail(ctx, GFEM_ERR_STATE, "finalize called twice");
-  if(!sys->d_xyz) return fail(ctx, GFEM_ERR_STATE, "nodes not uploaded before gfem_system_create");
-  GFEM_HIP(ctx, hipSetDevice(ctx->device));
-  // module: AFrequency when a bilinear term carries a time derivative
-  // (problem/Formulation.cpp:1095-1149)
-  sys->freq_module = false;
-  for(auto &g : sys->groups)
-    for(auto &t : g.terms)
-      if(t.kind == 0 && t.dt_order != 0) sys->freq_module = true;
-  if(sys->freq_module && !sys->cplx)
-    return fail(ctx, GFEM_ERR_UNSUPPORTED,
-                "only a complex formulation can be assembled into 'Ax = b' with mass or damping terms");
-  int maxNN = 1;
-  for(auto &g : sys->groups) maxNN = std::max(maxNN, g.N * g.N);
-  int logP = 0;
-  while((1 << logP) < maxNN) ++logP;
-  sys->logP = logP;
+  const int logP = sys->logP;
   const int64_t n_elem = sys->groups.empty() ? 0 : sys->groups.back().e_off + sys->groups.back().ne;
-  const double w = sys->frequency;
-  for(auto &g : sys->groups) {
-    if(g.terms.size() > 16) return fail(ctx, GFEM_ERR_UNSUPPORTED, "more than 16 terms on one element group");
-    // rules: one per distinct Gauss degree
-    g.dterms.clear();
-    for(size_t k = 0; k < g.terms.size(); ++k) {
-      const gfem_term_desc &t = g.terms[k];
-      int r = -1;
-      for(size_t q = 0; q < g.rules.size(); ++q)
-        if(g.rules[q].gauss == t.gauss) r = (int)q;
-      if(r < 0) {
-        if(g.rules.size() >= 8) return fail(ctx, GFEM_ERR_UNSUPPORTED, "more than 8 Gauss rules in a group");
-        Rule R;
-        R.gauss = t.gauss;
-        R.Q = gauss_rule(g.dim, t.gauss, R.pts, R.w);
-        g.rules.push_back(std::move(R));
-        r = (int)g.rules.size() - 1;
-      }
-      DevTerm d{};
-      d.kind = t.kind;
-      d.rule = r;
-      d.form_dof = t.form_dof;
-      d.form_tf = t.form_tf;
-      d.rhs_idx = t.rhs_idx;
-      d.per_point = t.n_ops > 0 ? 1 : 0;
-      d.c_re = t.coef_re;
-      d.c_im = t.coef_im;
-      // tf-side factor: conj(c_tf) for Hermitian products (BilinearTerm.cpp:324,
-      // LinearTerm.cpp:189), times -w^2 / -i w for dt^2 / dt terms
-      // in the AFrequency module (Assembler.cpp:285-296)
-      double fr = t.tf_coef_re, fi = t.hermitian ? -t.tf_coef_im : t.tf_coef_im;
-      if(t.kind == 0 && sys->freq_module && t.dt_order == 2) {
-        fr *= -w * w;
-        fi *= -w * w;
-      }
-      else if(t.kind == 0 && sys->freq_module && t.dt_order == 1) {
-        const double nr = fi * w, ni = -fr * w;  // (fr + i fi) * (-i w)
-        fr = nr;
-        fi = ni;
-      }
-      d.fac_re = fr;
-      d.fac_im = fi;
-      d.expr.n_ops = t.n_ops;
-      d.expr.n_consts = t.n_consts;
-      for(int q = 0; q < 2 * t.n_ops; ++q) d.expr.ops[q] = g.term_ops[k][q];
-      for(int q = 0; q < 2 * t.n_consts; ++q) d.expr.consts[q] = g.term_consts[k][q];
-      // real-only programs (no imaginary constant, no sqrt / pow that may
-      // leave the real line) are evaluated in real arithmetic on the device
-      d.expr.real_only = 1;
-      for(int q = 0; q < t.n_consts; ++q)
-        if(g.term_consts[k][2 * q + 1] != 0.) d.expr.real_only = 0;
-      for(int q = 0; q < t.n_ops; ++q)
-        if(g.term_ops[k][2 * q] == GFEM_OP_SQRT || g.term_ops[k][2 * q] == GFEM_OP_POW) d.expr.real_only = 0;
-      if(t.kind == 0 && d.per_point) g.has_dense = true;
-      g.dterms.push_back(d);
-    }
-    // compile the per-point linear-term coefficient programs once
-    g.jit.assign(g.dterms.size(), nullptr);
-    for(size_t k = 0; k < g.dterms.size(); ++k)
-      if(g.dterms[k].kind == 1 && g.dterms[k].per_point) {
-        std::string log;
-        g.jit[k] = jit_linear_kernel(ctx->device, g.dterms[k].expr, &log);
-        if(!g.jit[k] && !log.empty()) std::fprintf(stderr, "gfem: coefficient compilation failed, interpreting:\n%s\n", log.c_str());
-      }
-    const std::vector<Lat> lat = lattice_nodes(g.dim, g.order);
-    std::vector<int> nodes(4 * lat.size());
-    for(size_t a = 0; a < lat.size(); ++a)
-      for(int k = 0; k < 4; ++k) nodes[4 * a + k] = lat[a][k];
-    GFEM_HIP(ctx, dupload(&g.d_nodes, nodes.data(), nodes.size()));
-    GFEM_HIP(ctx, dupload(&g.d_vtx, g.h_vtx.data(), g.h_vtx.size()));
-    GFEM_HIP(ctx, dupload(&g.d_terms, g.dterms.data(), g.dterms.size()));
-    GFEM_HIP(ctx, dalloc(&g.d_F64, (size_t)(g.ne * 8)));
-    GFEM_HIP(ctx, dalloc(&g.d_F32, (size_t)(g.ne * 8)));
-    GFEM_HIP(ctx, dalloc(&g.d_T64, (size_t)(g.N * g.N * 8)));
-    GFEM_HIP(ctx, dalloc(&g.d_T32, (size_t)(g.N * g.N * 8)));
-    GFEM_HIP(ctx, dalloc(&g.d_T64r, (size_t)(g.N * g.N * 8)));
-    GFEM_HIP(ctx, dalloc(&g.d_T32r, (size_t)(g.N * g.N * 8)));
-    // the tensor is real when every constant-coefficient bilinear factor is
-    g.treal = true;
-    for(const DevTerm &d : g.dterms)
-      if(d.kind == 0 && !d.per_point && (d.c_re * d.fac_im + d.c_im * d.fac_re) != 0.) g.treal = false;
-    // one Gauss rule shared by every bilinear term: the GEMM form applies
-    g.gemm_rule = -1;
-    for(const DevTerm &d : g.dterms)
-      if(d.kind == 0) {
-        if(g.gemm_rule == -1)
-          g.gemm_rule = d.rule;
-        else if(g.gemm_rule != d.rule)
-          g.gemm_rule = -2;
-      }
-    if(g.gemm_rule < 0) g.gemm_rule = -1;
-    if(g.has_dense) {
-      const size_t planes = sys->cplx ? 2 : 1;
-      GFEM_HIP(ctx, dalloc(&g.d_dense64, (size_t)g.ne * planes * g.N * g.N));
-      GFEM_HIP(ctx, dalloc(&g.d_dense32, (size_t)g.ne * planes * g.N * g.N));
-    }
-    for(auto &R : g.rules) {
-      GFEM_HIP(ctx, dupload(&R.d_w, R.w.data(), R.w.size()));
-      GFEM_HIP(ctx, dupload(&R.d_pts, R.pts.data(), R.pts.size()));
-      GFEM_HIP(ctx, dalloc(&R.d_bf, (size_t)(R.Q * g.N)));
-      GFEM_HIP(ctx, dalloc(&R.d_gbf, (size_t)(R.Q * g.N * 3)));
-      GFEM_HIP(ctx, launch_basis(g, R, nullptr));
-    }
-    GFEM_HIP(ctx, launch_tensor(g, nullptr));
-    GroupDev gd{};
-    gd.dim = g.dim;
-    gd.nv = g.nv;
-    gd.N = g.N;
-    gd.nterms = (int)g.dterms.size();
-    gd.terms = g.d_terms;
-    for(size_t r = 0; r < g.rules.size(); ++r)
-      gd.rules[r] = RuleDev{g.rules[r].Q, g.rules[r].d_w, g.rules[r].d_pts, g.rules[r].d_bf, g.rules[r].d_gbf};
-    GFEM_HIP(ctx, dupload(&g.d_dev, &gd, 1));
-  }
-  // element load vectors of every group in one buffer, so a right-hand-side
-  // term is one flat index; right-hand sides without a linear term on a
-  // group keep zeros there
-  sys->be_total = 0;
-  for(auto &g : sys->groups)
-    if(g.has_linear) {
-      g.be_off = sys->be_total;
-      sys->be_total += g.ne * g.N;
-    }
-  if(sys->be_total >= (int64_t(1) << 32)) return fail(ctx, GFEM_ERR_TOO_LARGE, "too many element loads");
-  if(sys->be_total > 0) {
-    GFEM_HIP(ctx, dalloc(&sys->d_be_all, (size_t)(sys->nrhs * sys->be_total)));
-    GFEM_HIP(ctx, hipMemset(sys->d_be_all, 0, sizeof(double2) * (size_t)(sys->nrhs * sys->be_total)));
-  }
-  for(auto &g : sys->groups) {
-    g.d_be = g.has_linear ? sys->d_be_all + g.be_off : nullptr;
-    g.be_stride = sys->be_total;
-  }
-  {
-    const GroupTable gt = make_table(sys, false);
-    GFEM_HIP(ctx, dupload(&sys->d_table, &gt, 1));
-  }
-
   // Element -> nonzero map, transposed: the contributions of every nonzero in
   // traversal order (group, element, local pair), cut into gather chunks of
   // whole nonzeros with <= kChunk contributions and <= kChunkElems distinct
@@ -690,18 +541,6 @@ gfem_status gfem_system_finalize(gfem_system *sys)
     }
     sys->n_chunks = (int64_t)chunk;
     sys->n_words = total;
-    // switches of this system (environment at finalize): GFEM_AUTOTUNE=0 keeps
-    // the default tile choices, GFEM_DENSE_GEMM=0 computes per-point element
-    // matrices with the direct kernel instead of the GEMM form
-    const char *at = std::getenv("GFEM_AUTOTUNE");
-    sys->autotune = !(at && at[0] == '0');
-    const char *dg = std::getenv("GFEM_DENSE_GEMM");
-    sys->dense_gemm = !(dg && dg[0] == '0');
-    if(const char *e = std::getenv("GFEM_GATHER_SLOTS")) {
-      // a fixed choice, not tuned
-      sys->gather_slots[0] = sys->gather_slots[1] = std::atoi(e) == 2 ? 2 : 4;
-      sys->tuned[0] = sys->tuned[1] = true;
-    }
     GFEM_HIP(ctx, dupload(&sys->d_rec, rec.data(), rec.size()));
   }
 
@@ -755,6 +594,218 @@ gfem_status gfem_system_finalize(gfem_system *sys)
     GFEM_HIP(ctx, dupload(&sys->d_dc_words, dw.data(), dw.size()));
     GFEM_HIP(ctx, dupload(&sys->d_dc_fix, df.data(), df.size()));
   }
+  return GFEM_OK;
+}
+
+gfem_status gfem_system_finalize(gfem_system *sys)
+{
+  if(!sys) return fail(nullptr, GFEM_ERR_INVALID_ARG, "system is null");
+  gfem_ctx *ctx = sys->ctx;
+  if(sys->finalized) return fail(ctx, GFEM_ERR_STATE, "finalize called twice");
+  if(!sys->d_xyz) return fail(ctx, GFEM_ERR_STATE, "nodes not uploaded before gfem_system_create");
+  GFEM_HIP(ctx, hipSetDevice(ctx->device));
+  // module: AFrequency when a bilinear term carries a time derivative
+  // (problem/Formulation.cpp:1095-1149)
+  sys->freq_module = false;
+  for(auto &g : sys->groups)
+    for(auto &t : g.terms)
+      if(t.kind == 0 && t.dt_order != 0) sys->freq_module = true;
+  if(sys->freq_module && !sys->cplx)
+    return fail(ctx, GFEM_ERR_UNSUPPORTED,
+                "only a complex formulation can be assembled into 'Ax = b' with mass or damping terms");
+  int maxNN = 1;
+  for(auto &g : sys->groups) maxNN = std::max(maxNN, g.N * g.N);
+  int logP = 0;
+  while((1 << logP) < maxNN) ++logP;
+  sys->logP = logP;
+  const int64_t n_elem = sys->groups.empty() ? 0 : sys->groups.back().e_off + sys->groups.back().ne;
+  const double w = sys->frequency;
+  for(auto &g : sys->groups) {
+    if(g.terms.size() > 16) return fail(ctx, GFEM_ERR_UNSUPPORTED, "more than 16 terms on one element group");
+    // rules: one per distinct Gauss degree
+    g.dterms.clear();
+    for(size_t k = 0; k < g.terms.size(); ++k) {
+      const gfem_term_desc &t = g.terms[k];
+      int r = -1;
+      for(size_t q = 0; q < g.rules.size(); ++q)
+        if(g.rules[q].gauss == t.gauss) r = (int)q;
+      if(r < 0) {
+        if(g.rules.size() >= 8) return fail(ctx, GFEM_ERR_UNSUPPORTED, "more than 8 Gauss rules in a group");
+        Rule R;
+        R.gauss = t.gauss;
+        R.Q = gauss_rule(g.dim, t.gauss, R.pts, R.w);
+        g.rules.push_back(std::move(R));
+        r = (int)g.rules.size() - 1;
+      }
+      DevTerm d{};
+      d.kind = t.kind;
+      d.rule = r;
+      d.form_dof = t.form_dof;
+      d.form_tf = t.form_tf;
+      d.rhs_idx = t.rhs_idx;
+      d.per_point = t.n_ops > 0 ? 1 : 0;
+      d.c_re = t.coef_re;
+      d.c_im = t.coef_im;
+      // tf-side factor: conj(c_tf) for Hermitian products (BilinearTerm.cpp:324,
+      // LinearTerm.cpp:189), times -w^2 / -i w for dt^2 / dt terms
+      // in the AFrequency module (Assembler.cpp:285-296)
+      double fr = t.tf_coef_re, fi = t.hermitian ? -t.tf_coef_im : t.tf_coef_im;
+      if(t.kind == 0 && sys->freq_module && t.dt_order == 2) {
+        fr *= -w * w;
+        fi *= -w * w;
+      }
+      else if(t.kind == 0 && sys->freq_module && t.dt_order == 1) {
+        const double nr = fi * w, ni = -fr * w;  // (fr + i fi) * (-i w)
+        fr = nr;
+        fi = ni;
+      }
+      d.fac_re = fr;
+      d.fac_im = fi;
+      d.expr.n_ops = t.n_ops;
+      d.expr.n_consts = t.n_consts;
+      for(int q = 0; q < 2 * t.n_ops; ++q) d.expr.ops[q] = g.term_ops[k][q];
+      for(int q = 0; q < 2 * t.n_consts; ++q) d.expr.consts[q] = g.term_consts[k][q];
+      // real-only programs (no imaginary constant, no sqrt / pow that may
+      // leave the real line) are evaluated in real arithmetic on the device
+      d.expr.real_only = 1;
+      for(int q = 0; q < t.n_consts; ++q)
+        if(g.term_consts[k][2 * q + 1] != 0.) d.expr.real_only = 0;
+      for(int q = 0; q < t.n_ops; ++q)
+        if(g.term_ops[k][2 * q] == GFEM_OP_SQRT || g.term_ops[k][2 * q] == GFEM_OP_POW) d.expr.real_only = 0;
+      if(t.kind == 0 && d.per_point) g.has_dense = true;
+      g.dterms.push_back(d);
+    }
+    // compile the per-point linear-term coefficient programs once
+    g.jit.assign(g.dterms.size(), nullptr);
+    for(size_t k = 0; k < g.dterms.size(); ++k)
+      if(g.dterms[k].kind == 1 && g.dterms[k].per_point) {
+        std::string log;
+        g.jit[k] = jit_linear_kernel(ctx->device, g.dterms[k].expr, &log);
+        if(!g.jit[k] && !log.empty()) std::fprintf(stderr, "gfem: coefficient compilation failed, interpreting:\n%s\n", log.c_str());
+      }
+    const std::vector<Lat> lat = lattice_nodes(g.dim, g.order);
+    std::vector<int> nodes(4 * lat.size());
+    for(size_t a = 0; a < lat.size(); ++a)
+      for(int k = 0; k < 4; ++k) nodes[4 * a + k] = lat[a][k];
+    GFEM_HIP(ctx, dupload(&g.d_nodes, nodes.data(), nodes.size()));
+    if(!g.d_vtx) GFEM_HIP(ctx, dupload(&g.d_vtx, g.h_vtx.data(), g.h_vtx.size()));
+    GFEM_HIP(ctx, dupload(&g.d_terms, g.dterms.data(), g.dterms.size()));
+    GFEM_HIP(ctx, dalloc(&g.d_F64, (size_t)(g.ne * 8)));
+    GFEM_HIP(ctx, dalloc(&g.d_F32, (size_t)(g.ne * 8)));
+    GFEM_HIP(ctx, dalloc(&g.d_T64, (size_t)(g.N * g.N * 8)));
+    GFEM_HIP(ctx, dalloc(&g.d_T32, (size_t)(g.N * g.N * 8)));
+    GFEM_HIP(ctx, dalloc(&g.d_T64r, (size_t)(g.N * g.N * 8)));
+    GFEM_HIP(ctx, dalloc(&g.d_T32r, (size_t)(g.N * g.N * 8)));
+    // the tensor is real when every constant-coefficient bilinear factor is
+    g.treal = true;
+    for(const DevTerm &d : g.dterms)
+      if(d.kind == 0 && !d.per_point && (d.c_re * d.fac_im + d.c_im * d.fac_re) != 0.) g.treal = false;
+    // one Gauss rule shared by every bilinear term: the GEMM form applies
+    g.gemm_rule = -1;
+    for(const DevTerm &d : g.dterms)
+      if(d.kind == 0) {
+        if(g.gemm_rule == -1)
+          g.gemm_rule = d.rule;
+        else if(g.gemm_rule != d.rule)
+          g.gemm_rule = -2;
+      }
+    if(g.gemm_rule < 0) g.gemm_rule = -1;
+    if(g.has_dense) {
+      const size_t planes = sys->cplx ? 2 : 1;
+      GFEM_HIP(ctx, dalloc(&g.d_dense64, (size_t)g.ne * planes * g.N * g.N));
+      GFEM_HIP(ctx, dalloc(&g.d_dense32, (size_t)g.ne * planes * g.N * g.N));
+    }
+    for(auto &R : g.rules) {
+      GFEM_HIP(ctx, dupload(&R.d_w, R.w.data(), R.w.size()));
+      GFEM_HIP(ctx, dupload(&R.d_pts, R.pts.data(), R.pts.size()));
+      GFEM_HIP(ctx, dalloc(&R.d_bf, (size_t)(R.Q * g.N)));
+      GFEM_HIP(ctx, dalloc(&R.d_gbf, (size_t)(R.Q * g.N * 3)));
+      GFEM_HIP(ctx, launch_basis(g, R, nullptr));
+    }
+    GFEM_HIP(ctx, launch_tensor(g, nullptr));
+    GroupDev gd{};
+    gd.dim = g.dim;
+    gd.nv = g.nv;
+    gd.N = g.N;
+    gd.nterms = (int)g.dterms.size();
+    gd.terms = g.d_terms;
+    for(size_t r = 0; r < g.rules.size(); ++r)
+      gd.rules[r] = RuleDev{g.rules[r].Q, g.rules[r].d_w, g.rules[r].d_pts, g.rules[r].d_bf, g.rules[r].d_gbf};
+    GFEM_HIP(ctx, dupload(&g.d_dev, &gd, 1));
+  }
+  // element load vectors of every group in one buffer, so a right-hand-side
+  // term is one flat index; right-hand sides without a linear term on a
+  // group keep zeros there
+  sys->be_total = 0;
+  for(auto &g : sys->groups)
+    if(g.has_linear) {
+      g.be_off = sys->be_total;
+      sys->be_total += g.ne * g.N;
+    }
+  if(sys->be_total >= (int64_t(1) << 32)) return fail(ctx, GFEM_ERR_TOO_LARGE, "too many element loads");
+  if(sys->be_total > 0) {
+    GFEM_HIP(ctx, dalloc(&sys->d_be_all, (size_t)(sys->nrhs * sys->be_total)));
+    GFEM_HIP(ctx, hipMemset(sys->d_be_all, 0, sizeof(double2) * (size_t)(sys->nrhs * sys->be_total)));
+  }
+  for(auto &g : sys->groups) {
+    g.d_be = g.has_linear ? sys->d_be_all + g.be_off : nullptr;
+    g.be_stride = sys->be_total;
+  }
+  {
+    const GroupTable gt = make_table(sys, false);
+    GFEM_HIP(ctx, dupload(&sys->d_table, &gt, 1));
+  }
+
+  // switches of this system (environment at finalize)
+  // the default tile choices, GFEM_DENSE_GEMM=0 computes per-point element
+  // matrices with the direct kernel instead of the GEMM form
+  const char *at = std::getenv("GFEM_AUTOTUNE");
+  sys->autotune = !(at && at[0] == '0');
+  const char *dg = std::getenv("GFEM_DENSE_GEMM");
+  sys->dense_gemm = !(dg && dg[0] == '0');
+  if(const char *e = std::getenv("GFEM_GATHER_SLOTS")) {
+    // a fixed choice, not tuned
+    sys->gather_slots[0] = sys->gather_slots[1] = std::atoi(e) == 2 ? 2 : 4;
+    sys->tuned[0] = sys->tuned[1] = true;
+  }
+  // device copies of the storage-order maps of the groups given by the host
+  for(auto &g : sys->groups) {
+    if(!g.host_maps) continue;
+    std::vector<int32_t> v(g.h_num.size());
+    for(size_t i = 0; i < v.size(); ++i) v[i] = (int32_t)g.h_num[i];
+    GFEM_HIP(ctx, dupload(&g.d_num, v.data(), v.size()));
+    GFEM_HIP(ctx, dupload(&g.d_type, g.h_type.data(), g.h_type.size()));
+    if(g.has_bilinear && g.h_e2n.size() == (size_t)(g.ne * g.N * g.N)) {
+      std::vector<int32_t> m(g.h_e2n.size());
+      for(size_t i = 0; i < m.size(); ++i) {
+        const int64_t x = g.h_e2n[i];
+        if(x < -1 || x >= sys->nnz) return fail(ctx, GFEM_ERR_INVALID_ARG, "element->nonzero map out of range");
+        m[i] = (int32_t)x;
+      }
+      GFEM_HIP(ctx, dupload(&g.d_e2n, m.data(), m.size()));
+    }
+    std::vector<int32_t> ip(g.h_iperm.size());
+    for(size_t i = 0; i < ip.size(); ++i) ip[i] = (int32_t)g.h_iperm[i];
+    GFEM_HIP(ctx, dupload(&g.d_iperm, ip.data(), ip.size()));
+  }
+  // gather records and RHS maps: built on the device; GFEM_HOST_FINALIZE=1
+  // builds them on the host from the host copies of the maps (the same
+  // records: a cross-check of the device builder)
+  {
+    const char *hf = std::getenv("GFEM_HOST_FINALIZE");
+    bool host = hf && hf[0] == '1';
+    for(auto &g : sys->groups)
+      if(!g.host_maps) host = false;
+    const gfem_status st = host ? build_maps_host(sys) : build_maps_device(sys);
+    if(st != GFEM_OK) return st;
+  }
+  for(auto &g : sys->groups) {
+    std::vector<int32_t>().swap(g.h_vtx);
+    std::vector<int64_t>().swap(g.h_num);
+    std::vector<int32_t>().swap(g.h_type);
+    std::vector<int64_t>().swap(g.h_e2n);
+    std::vector<int64_t>().swap(g.h_iperm);
+  }
   GFEM_HIP(ctx, hipDeviceSynchronize());
   sys->finalized = true;
   return GFEM_OK;
@@ -773,19 +824,11 @@ static gfem_status ensure_reference_maps(gfem_system *sys)
 {
   if(sys->built_ref) return GFEM_OK;
   gfem_ctx *ctx = sys->ctx;
-  for(auto &g : sys->groups) {
-    const int64_t NN = (int64_t)g.N * g.N;
-    std::vector<int32_t> num(g.h_num.size());
-    for(size_t i = 0; i < num.size(); ++i) num[i] = (int32_t)g.h_num[i];
-    GFEM_HIP(ctx, dupload(&g.d_num, num.data(), num.size()));
-    GFEM_HIP(ctx, dupload(&g.d_type, g.h_type.data(), g.h_type.size()));
-    if(g.has_bilinear) {
-      std::vector<int32_t> e2n(g.h_e2n.size());
-      for(size_t i = 0; i < e2n.size(); ++i) e2n[i] = (int32_t)g.h_e2n[i];
-      GFEM_HIP(ctx, dupload(&g.d_e2n, e2n.data(), e2n.size()));
-      if(!g.d_dense64) GFEM_HIP(ctx, dalloc(&g.d_dense64, (size_t)(g.ne * NN) * (sys->cplx ? 2 : 1)));
-    }
-  }
+  // the storage-order maps (d_num, d_type, d_e2n) are on the device since
+  // finalize; the plain strategy only needs its element-matrix buffer
+  for(auto &g : sys->groups)
+    if(g.has_bilinear && !g.d_dense64)
+      GFEM_HIP(ctx, dalloc(&g.d_dense64, (size_t)(g.ne * g.N * g.N) * (sys->cplx ? 2 : 1)));
   sys->built_ref = true;
   return GFEM_OK;
 }
@@ -1259,6 +1302,65 @@ gfem_status gfem_gather_bytes(const gfem_system *sys, gfem_precision prec, doubl
   b += (double)ne * 8.0 * (p ? 4.0 : 8.0);
   b += (double)value_bytes(sys, p) * (double)sys->nnz;
   *bytes = b;
+  return GFEM_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
+
+gfem_status gfem_system_maps(const gfem_system *sys, int which, void *host, size_t bytes, int64_t *count)
+{
+  if(!sys || !count) return fail(sys ? sys->ctx : nullptr, GFEM_ERR_INVALID_ARG, "system_maps: null argument");
+  gfem_ctx *ctx = sys->ctx;
+  if(!sys->finalized) return fail(ctx, GFEM_ERR_STATE, "system_maps before finalize");
+  const void *src = nullptr;
+  int64_t n = 0;
+  size_t es = 4;
+  switch(which) {
+  case 0: src = sys->d_rec; n = sys->n_chunks * sys->rec_stride; break;
+  case 1: src = sys->d_lin_ptr; n = sys->n + 1; es = 8; break;
+  case 2: src = sys->d_lin_words; n = sys->n_lin; break;
+  case 3: src = sys->d_dc_ptr; n = sys->n + 1; es = 8; break;
+  case 4: src = sys->d_dc_words; n = sys->n_dcw; break;
+  case 5: src = sys->d_dc_fix; n = sys->n_dcw; break;
+  default: return fail(ctx, GFEM_ERR_INVALID_ARG, "system_maps: unknown map");
+  }
+  *count = n;
+  if(!host || n == 0) return GFEM_OK;
+  if(bytes < es * (size_t)n) return fail(ctx, GFEM_ERR_INVALID_ARG, "system_maps: buffer too small");
+  GFEM_HIP(ctx, hipSetDevice(ctx->device));
+  GFEM_HIP(ctx, hipMemcpy(host, src, es * (size_t)n, hipMemcpyDeviceToHost));
+  return GFEM_OK;
+}
+
+gfem_status gfem_group_maps(const gfem_system *sys, int group, int which, int32_t *host, size_t capacity,
+                            int64_t *count)
+{
+  if(!sys || !count) return fail(sys ? sys->ctx : nullptr, GFEM_ERR_INVALID_ARG, "group_maps: null argument");
+  gfem_ctx *ctx = sys->ctx;
+  if(!sys->finalized) return fail(ctx, GFEM_ERR_STATE, "group_maps before finalize");
+  if(group == -1) {
+    *count = (int64_t)sys->groups.size();
+    return GFEM_OK;
+  }
+  if(group < 0 || group >= (int)sys->groups.size()) return fail(ctx, GFEM_ERR_INVALID_ARG, "bad group index");
+  const Group &g = sys->groups[group];
+  const int32_t *src = nullptr;
+  int64_t n = 0;
+  switch(which) {
+  case 0: src = g.d_vtx; n = g.ne * g.nv; break;
+  case 1: src = g.d_num; n = g.ne * g.N; break;
+  case 2: src = g.d_type; n = g.ne * g.N; break;
+  case 3: src = g.d_e2n; n = g.d_e2n ? g.ne * g.N * g.N : 0; break;
+  case 4: src = g.d_iperm; n = g.ne; break;
+  default: return fail(ctx, GFEM_ERR_INVALID_ARG, "group_maps: unknown map");
+  }
+  *count = n;
+  if(!host || n == 0) return GFEM_OK;
+  if(capacity < (size_t)n) return fail(ctx, GFEM_ERR_INVALID_ARG, "group_maps: buffer too small");
+  GFEM_HIP(ctx, hipSetDevice(ctx->device));
+  GFEM_HIP(ctx, hipMemcpy(host, src, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost));
   return GFEM_OK;
 }
 

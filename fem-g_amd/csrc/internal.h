@@ -146,10 +146,14 @@ struct Group {
   float *d_A32 = nullptr;
   double2 *d_be = nullptr;      // this group's element loads inside gfem_system::d_be_all
   int64_t be_off = 0, be_stride = 0;  // offset in d_be_all, distance between right-hand sides
-  // plain reference strategy
+  // maps in storage order on the device (from finalize): numDof-1 and
+  // AssembleType per key, element -> nonzero map (bilinear groups), given
+  // element index -> storage index
   int32_t *d_e2n = nullptr;
   int32_t *d_num = nullptr;
   int32_t *d_type = nullptr;
+  int32_t *d_iperm = nullptr;
+  bool host_maps = false;  // h_vtx/h_num/h_type/h_e2n/h_iperm given by the host (until finalize)
 };
 
 }  // namespace gfem
@@ -168,6 +172,7 @@ struct gfem_system {
   gfem_scalar scalar = GFEM_SCALAR_F64;
   bool cplx = false;
   int64_t n = 0, nnz = 0, n_fixed = 0;
+  int64_t n_nodes = 0;  // nodes of the system's mesh
   // node coordinates of the system's mesh: a copy of the context's nodes at
   // gfem_system_create, so systems on different meshes can be assembled in
   // any order
@@ -264,6 +269,9 @@ void dfree(T *&p)
   if(p) (void)hipFree(p);
   p = nullptr;
 }
+
+// gather records and RHS maps of a system built on the device (setup.hip)
+gfem_status build_maps_device(gfem_system *sys);
 
 // GMRES with FP64 iterative refinement on the last FP64 assembly (solver.hip)
 gfem_status solve_gmres_ir(gfem_system *sys, int rhs_idx, double tol, int restart, int max_iter, hipStream_t s,

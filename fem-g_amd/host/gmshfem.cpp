@@ -7,7 +7,9 @@
 #include <cstring>
 #include <map>
 #include <mutex>
+#include <tuple>
 
+#include "../csrc/hilbert.h"
 #include "../csrc/refelem.h"
 
 namespace gmshfem {
@@ -266,81 +268,9 @@ Unknown operator*(const function::ScalarFunction &c, const Unknown &u)
 
 namespace reorder {
 
-namespace {
-// Compact Hilbert index tables (state = direction << dim | entry point,
-// digit = coordinate bits of one level), built from the gray-code entry /
-// direction sequences of fem-master/src/reorder/Hilbert.cpp:17-128.
-struct Curve {
-  int dim = 0;
-  uint8_t nextState[32][8] = {};
-  uint8_t digitOut[32][8] = {};
-
-  explicit Curve(int d) : dim(d)
-  {
-    const int N = 1 << d;
-    const unsigned mask = (1u << d) - 1u;
-    auto gray = [](unsigned i) { return i ^ (i >> 1); };
-    auto grayInv = [d](unsigned g) {
-      unsigned v = g;
-      for(int j = 1; j < d; ++j) v ^= g >> j;
-      return v;
-    };
-    auto rotr = [d, mask](unsigned k, int r) { return ((k >> r) | (k << (d - r))) & mask; };
-    auto rotl = [d, mask](unsigned k, int r) { return ((k << r) | (k >> (d - r))) & mask; };
-    unsigned entry[8], dir[8];
-    entry[0] = 0;
-    for(int i = 1; i < N; ++i) entry[i] = gray(2u * ((i - 1) / 2));
-    dir[0] = 0;
-    for(int i = 1; i < N / 2; ++i) dir[i] = ((i % 2 == 0) ? gray(i - 1) : gray(i)) % d;
-    for(int i = N / 2; i < N; ++i) dir[i] = dir[N - i - 1];
-    for(int dd = 0; dd < d; ++dd)
-      for(int e = 0; e < N; ++e) {
-        const int s = (dd << d) | e;
-        for(int l = 0; l < N; ++l) {
-          const unsigned w = grayInv(rotr((unsigned)l ^ (unsigned)e, dd + 1));
-          digitOut[s][l] = (uint8_t)w;
-          const unsigned ne = (unsigned)e ^ rotl(entry[w], dd + 1);
-          const unsigned nd = (dd + dir[w] + 1) % d;
-          nextState[s][l] = (uint8_t)((nd << d) | ne);
-        }
-      }
-  }
-
-  uint64_t index(const uint32_t *u, int top) const
-  {
-    uint64_t h = 0;
-    unsigned s = 0;
-    for(int m = top; m >= 0; --m) {
-      unsigned l = 0;
-      for(int k = 0; k < dim; ++k) l |= ((u[k] >> m) & 1u) << k;
-      h = (h << dim) ^ digitOut[s][l];
-      s = nextState[s][l];
-    }
-    return h;
-  }
-};
-
-// float quantisation of Hilbert<dim>::apply (Hilbert.cpp:228-271): scale by
-// 2^(bits-1)/(max-min), no fused multiply-add
-struct Quant {
-  int bits;
-  float scale[3], shift[3];
-  Quant(int dim, const float *mn, const float *mx)
-  {
-    bits = 64 / dim;
-    const unsigned two = 1u << (bits - 1);
-    for(int i = 0; i < dim; ++i) {
-      scale[i] = (float)two / (mx[i] - mn[i]);
-      shift[i] = -mn[i] * scale[i];
-    }
-  }
-  uint32_t operator()(int i, float f) const
-  {
-    volatile float t = f * scale[i];
-    return (uint32_t)(t + shift[i]);
-  }
-};
-}  // namespace
+// curve tables and quantisation: csrc/hilbert.h (Hilbert.cpp:17-271)
+using Curve = gfem::HilbertCurve;
+using Quant = gfem::HilbertQuant;
 
 uint64_t hilbertIndex(int dim, const uint32_t *u, int top) { return Curve(dim).index(u, top); }
 
@@ -431,6 +361,7 @@ template <class T>
 Formulation<T>::~Formulation()
 {
   releaseSystem();
+  releasePre();
 }
 
 template <class T>
@@ -520,24 +451,44 @@ const typename Formulation<T>::Pass *Formulation<T>::findPass(int dim, int tag, 
   return nullptr;
 }
 
+// the pass with its element maps on the host (copied from the device after
+// a device pre())
+template <class T>
+const typename Formulation<T>::Pass *Formulation<T>::hostPass(int dim, int tag, int type) const
+{
+  const Pass *cp = findPass(dim, tag, type);
+  if(!cp || cp->onHost || !_pre) return cp;
+  Pass &p = _passes[(size_t)(cp - _passes.data())];
+  const model::Block *b = _model->block(dim, tag, type);
+  p.num.resize(b->conn.size());
+  p.atype.resize(b->conn.size());
+  p.e2n.assign(p.bilinear ? b->conn.size() * (size_t)b->npe : 0, 0);
+  const gfem_status st = gfem_pre_pass_maps(_pre, (int)(cp - _passes.data()), p.num.data(), p.atype.data(),
+                                            p.bilinear ? p.e2n.data() : nullptr);
+  if(st != GFEM_OK)
+    throw common::Exception(std::string("gfem_pre_pass_maps: ") + gfem_ctx_last_error(common::deviceContext()));
+  p.onHost = true;
+  return &p;
+}
+
 template <class T>
 const std::vector<int64_t> *Formulation<T>::blockDofs(int dim, int tag, int type) const
 {
-  const Pass *p = findPass(dim, tag, type);
+  const Pass *p = hostPass(dim, tag, type);
   return p ? &p->num : nullptr;
 }
 
 template <class T>
 const std::vector<int32_t> *Formulation<T>::blockTypes(int dim, int tag, int type) const
 {
-  const Pass *p = findPass(dim, tag, type);
+  const Pass *p = hostPass(dim, tag, type);
   return p ? &p->atype : nullptr;
 }
 
 template <class T>
 const std::vector<int64_t> *Formulation<T>::blockNonzeros(int dim, int tag, int type) const
 {
-  const Pass *p = findPass(dim, tag, type);
+  const Pass *p = hostPass(dim, tag, type);
   return p ? &p->e2n : nullptr;
 }
 
@@ -548,11 +499,13 @@ common::Timer Formulation<T>::pre(DofsSort sort)
   common::Timer time;
   time.tick();
   releaseSystem();
+  releasePre();
   _ai.clear();
   _aj.clear();
   _passes.clear();
   _values.clear();
   _rhs.clear();
+  _lastError.clear();
   _model = model::current();
   if(!_model) throw common::Exception("no current model");
   if(!_field) throw common::Exception("formulation '" + _name + "' has no term");
@@ -562,7 +515,46 @@ common::Timer Formulation<T>::pre(DofsSort sort)
   _dofType.assign(nn + 1, 0);
   _fixedValues.clear();
   _nFixed = _nUnknown = 0;
+  for(const auto &ent : _field->domain().entities())
+    for(int type : M.elementTypes(ent.first, ent.second)) {
+      const model::Block *b = M.block(ent.first, ent.second, type);
+      if(b->order != _field->order() && b->dim > 0)
+        throw common::Exception("Lagrange order differs from the mesh order");
+    }
+  // assembly passes: every (entity, element type) carrying a term
+  for(const auto &ent : M.entities()) {
+    std::vector<unsigned> terms;
+    bool bilinear = false;
+    for(unsigned t = 0; t < _terms.size(); ++t)
+      if(_terms[t].activated && _terms[t].dom.have(ent)) {
+        terms.push_back(t);
+        bilinear = bilinear || _terms[t].bilinear;
+      }
+    if(terms.empty()) continue;
+    for(int type : M.elementTypes(ent.first, ent.second)) {
+      if(M.block(ent.first, ent.second, type)->dim == 0) continue;
+      Pass p;
+      p.dim = ent.first;
+      p.tag = ent.second;
+      p.type = type;
+      p.terms = terms;
+      p.bilinear = bilinear;
+      _passes.push_back(std::move(p));
+    }
+  }
+  if(_preDevice)
+    preDevice(sort);
+  else
+    preHost(sort);
+  time.tock();
+  _preSeconds = time.time();
+  return time;
+}
 
+template <class T>
+void Formulation<T>::preHost(DofsSort sort)
+{
+  const model::Model &M = *_model;
   // fixed dofs: constraint order, entity order, element type order, key order
   // (DofsFactory.cpp:39-74); Lagrange keys are the element node tags
   for(const auto &c : _field->constraints()) {
@@ -596,8 +588,7 @@ common::Timer Formulation<T>::pre(DofsSort sort)
   }
   if(_nUnknown == 0) {
     _lastError = "no unknown dofs";
-    time.tock();
-    return time;
+    return;
   }
   // Hilbert renumbering (DofsManager.cpp:118-239)
   if(sort != DofsSort::None && M.dim >= 2) {
@@ -615,30 +606,16 @@ common::Timer Formulation<T>::pre(DofsSort sort)
     for(size_t i = 0; i < order.size(); ++i) _dofNum[unknownTags[order[i]]] = (int64_t)i + 1;
   }
 
-  // element maps of every (entity, type) carrying a term (FieldInterface::fillIndices)
-  _passes.clear();
-  for(const auto &ent : M.entities()) {
-    std::vector<unsigned> terms;
-    for(unsigned t = 0; t < _terms.size(); ++t)
-      if(_terms[t].activated && _terms[t].dom.have(ent)) terms.push_back(t);
-    if(terms.empty()) continue;
-    for(int type : M.elementTypes(ent.first, ent.second)) {
-      const model::Block *b = M.block(ent.first, ent.second, type);
-      if(b->dim == 0) continue;
-      Pass p;
-      p.dim = ent.first;
-      p.tag = ent.second;
-      p.type = type;
-      p.terms = terms;
-      p.num.resize(b->conn.size());
-      p.atype.resize(b->conn.size());
-      for(size_t k = 0; k < b->conn.size(); ++k) {
-        const int64_t tag = b->conn[k];
-        const int t = _dofType[tag];
-        p.num[k] = t ? _dofNum[tag] - 1 : 0;
-        p.atype[k] = t == 1 ? GFEM_AT_UNKNOWN : (t == 2 ? GFEM_AT_FIXED : GFEM_AT_UNDEFINED);
-      }
-      _passes.push_back(std::move(p));
+  // element maps of every pass (FieldInterface::fillIndices)
+  for(auto &p : _passes) {
+    const model::Block *b = M.block(p.dim, p.tag, p.type);
+    p.num.resize(b->conn.size());
+    p.atype.resize(b->conn.size());
+    for(size_t k = 0; k < b->conn.size(); ++k) {
+      const int64_t tag = b->conn[k];
+      const int t = _dofType[tag];
+      p.num[k] = t ? _dofNum[tag] - 1 : 0;
+      p.atype[k] = t == 1 ? GFEM_AT_UNKNOWN : (t == 2 ? GFEM_AT_FIXED : GFEM_AT_UNDEFINED);
     }
   }
 
@@ -646,13 +623,8 @@ common::Timer Formulation<T>::pre(DofsSort sort)
   // row = tf dof, col = dof; unknown x unknown; upper triangle if symmetric
   const int64_t n = (int64_t)_nUnknown;
   std::vector<int64_t> cnt(n + 1, 0);
-  auto hasBilinear = [&](const Pass &p) {
-    for(unsigned t : p.terms)
-      if(_terms[t].bilinear) return true;
-    return false;
-  };
   for(const auto &p : _passes) {
-    if(!hasBilinear(p)) continue;
+    if(!p.bilinear) continue;
     const int N = M.block(p.dim, p.tag, p.type)->npe;
     const int64_t ne = (int64_t)p.num.size() / N;
     for(int64_t e = 0; e < ne; ++e) {
@@ -667,7 +639,7 @@ common::Timer Formulation<T>::pre(DofsSort sort)
   {
     std::vector<int64_t> fill(cnt.begin(), cnt.end() - 1);
     for(const auto &p : _passes) {
-      if(!hasBilinear(p)) continue;
+      if(!p.bilinear) continue;
       const int N = M.block(p.dim, p.tag, p.type)->npe;
       const int64_t ne = (int64_t)p.num.size() / N;
       for(int64_t e = 0; e < ne; ++e)
@@ -693,7 +665,7 @@ common::Timer Formulation<T>::pre(DofsSort sort)
   std::vector<int64_t>().swap(cols);
   // element -> nonzero map
   for(auto &p : _passes) {
-    if(!hasBilinear(p)) continue;
+    if(!p.bilinear) continue;
     const int N = M.block(p.dim, p.tag, p.type)->npe;
     const int64_t ne = (int64_t)p.num.size() / N;
     p.e2n.assign((size_t)(ne * N * N), -1);
@@ -710,9 +682,115 @@ common::Timer Formulation<T>::pre(DofsSort sort)
         }
       }
   }
-  time.tock();
-  _preSeconds = time.time();
-  return time;
+}
+
+// pre() on the device (include/gfem.h, gfem_pre_*): the same block
+// sequences as preHost, numbered, renumbered and patterned on the GPU.  The
+// host keeps the DOF numbering per node, the Dirichlet values and ai; aj and
+// the element maps stay on the device until asked for.
+template <class T>
+void Formulation<T>::preDevice(DofsSort sort)
+{
+  const model::Model &M = *_model;
+  common::uploadNodes(M);
+  gfem_ctx *ctx = common::deviceContext();
+  auto check = [ctx](gfem_status st, const char *what) {
+    if(st != GFEM_OK) throw common::Exception(std::string(what) + ": " + gfem_ctx_last_error(ctx));
+  };
+  check(gfem_pre_create(ctx, &_pre), "gfem_pre_create");
+  std::map<std::tuple<int, int, int>, int> ids;
+  auto blockOf = [&](int dim, int tag, int type) {
+    const auto key = std::make_tuple(dim, tag, type);
+    const auto it = ids.find(key);
+    if(it != ids.end()) return it->second;
+    const model::Block *b = M.block(dim, tag, type);
+    int id = -1;
+    check(gfem_pre_add_block(_pre, b->size(), b->npe, b->conn.data(), &id), "gfem_pre_add_block");
+    ids[key] = id;
+    return id;
+  };
+  // fixed dofs: constraint order, entity order, element type order
+  // (DofsFactory.cpp:39-74); unknown dofs over the field's domain
+  // (DofsFactory.cpp:406-460)
+  std::vector<int> fseq, fcon, useq;
+  const auto &cons = _field->constraints();
+  for(size_t c = 0; c < cons.size(); ++c)
+    for(const auto &ent : cons[c].first.entities())
+      for(int type : M.elementTypes(ent.first, ent.second)) {
+        fseq.push_back(blockOf(ent.first, ent.second, type));
+        fcon.push_back((int)c);
+      }
+  for(const auto &ent : _field->domain().entities())
+    for(int type : M.elementTypes(ent.first, ent.second)) useq.push_back(blockOf(ent.first, ent.second, type));
+  // Hilbert renumbering box (DofsManager.cpp:118-239)
+  int hdim = 0;
+  float fmn[3] = {0.f, 0.f, 0.f}, fmx[3] = {0.f, 0.f, 0.f};
+  if(sort != DofsSort::None && M.dim >= 2) {
+    hdim = M.dim;
+    double mn[3], mx[3];
+    M.boundingBox(mn, mx);
+    for(int j = 0; j < 3; ++j) {
+      fmn[j] = (float)(mn[j] - 0.000001 * std::fabs(mn[j]));
+      fmx[j] = (float)(mx[j] + 0.000001 * std::fabs(mx[j]));
+    }
+  }
+  int64_t nf = 0, nu = 0;
+  check(gfem_pre_number(_pre, (int)fseq.size(), fseq.data(), (int)useq.size(), useq.data(), hdim, fmn, fmx, &nf, &nu),
+        "gfem_pre_number");
+  _nFixed = (uint64_t)nf;
+  _nUnknown = (uint64_t)nu;
+  {
+    const int64_t nn = M.numNodes();
+    std::vector<int64_t> num((size_t)nn);
+    std::vector<int8_t> type((size_t)nn);
+    std::vector<int64_t> ftag((size_t)nf);
+    std::vector<int32_t> fseqOf((size_t)nf);
+    check(gfem_pre_dofs(_pre, num.data(), type.data(), ftag.data(), fseqOf.data()), "gfem_pre_dofs");
+    for(int64_t t = 1; t <= nn; ++t) {
+      _dofNum[t] = num[t - 1];
+      _dofType[t] = type[t - 1];
+    }
+    _fixedValues.resize((size_t)nf);
+    for(int64_t f = 0; f < nf; ++f) {
+      const double *X = &M.xyz[3 * (ftag[f] - 1)];
+      _fixedValues[f] = cons[fcon[fseqOf[f]]].second.evaluate(X[0], X[1], X[2]);
+    }
+  }
+  if(_nUnknown == 0) {
+    _lastError = "no unknown dofs";
+    releasePre();
+    return;
+  }
+  std::vector<int> pblocks, pbil;
+  for(auto &p : _passes) {
+    pblocks.push_back(blockOf(p.dim, p.tag, p.type));
+    pbil.push_back(p.bilinear ? 1 : 0);
+    p.onHost = false;
+  }
+  int64_t nnz = 0;
+  check(gfem_pre_pattern(_pre, (int)pblocks.size(), pblocks.data(), pbil.data(), _symmetric ? 1 : 0, &nnz),
+        "gfem_pre_pattern");
+  _ai.assign((size_t)_nUnknown + 1, 0);
+  check(gfem_pre_csr(_pre, _ai.data(), nullptr), "gfem_pre_csr");
+}
+
+template <class T>
+void Formulation<T>::releasePre()
+{
+  if(_pre) gfem_pre_destroy(_pre);
+  _pre = nullptr;
+}
+
+template <class T>
+const std::vector<int64_t> &Formulation<T>::aj() const
+{
+  if(_pre && _aj.empty() && !_ai.empty() && _ai.back() > 0) {
+    std::vector<int64_t> aj((size_t)_ai.back());
+    if(gfem_pre_csr(_pre, nullptr, aj.data()) != GFEM_OK)
+      throw common::Exception(std::string("gfem_pre_csr: ") + gfem_ctx_last_error(common::deviceContext()));
+    _aj.swap(aj);
+  }
+  return _aj;
 }
 
 // Hand the pattern, the Dirichlet values and one group per assembly pass to
@@ -724,15 +802,22 @@ void Formulation<T>::upload()
 {
   const model::Model &M = *_model;
   const int64_t n = (int64_t)_nUnknown;
-  common::uploadNodes(M);
   gfem_ctx *ctx = common::deviceContext();
   std::vector<double> dc(2 * std::max<uint64_t>(_nFixed, 1), 0.);
   for(uint64_t i = 0; i < _nFixed; ++i) {
     dc[2 * i] = _fixedValues[i].real();
     dc[2 * i + 1] = _fixedValues[i].imag();
   }
-  gfem_status st = gfem_system_create(ctx, ScalarKind<T>::value, n, _ai.data(), _aj.data(), _symmetric ? 1 : 0,
-                                      (int64_t)_nFixed, dc.data(), (int)_numRHS, _frequency, &_sys);
+  gfem_status st;
+  if(_pre) {
+    // pattern and maps already on the device (pre() on the device)
+    st = gfem_system_create_pre(ctx, ScalarKind<T>::value, _pre, dc.data(), (int)_numRHS, _frequency, &_sys);
+  }
+  else {
+    common::uploadNodes(M);
+    st = gfem_system_create(ctx, ScalarKind<T>::value, n, _ai.data(), _aj.data(), _symmetric ? 1 : 0,
+                            (int64_t)_nFixed, dc.data(), (int)_numRHS, _frequency, &_sys);
+  }
   if(st != GFEM_OK) throw common::Exception(std::string("gfem_system_create: ") + gfem_ctx_last_error(ctx));
   std::vector<const Pass *> order;
   for(int dim = 3; dim >= 0; --dim)
@@ -744,8 +829,11 @@ void Formulation<T>::upload()
     const model::Block *b = M.block(p->dim, p->tag, p->type);
     int g = -1;
     int k = 0;
-    st = gfem_group_add(_sys, p->dim, p->type, b->size(), b->conn.data(), p->num.data(), p->atype.data(),
-                        p->e2n.empty() ? nullptr : p->e2n.data(), &g);
+    if(_pre)
+      st = gfem_group_add_pre(_sys, _pre, (int)(p - _passes.data()), p->dim, p->type, &g);
+    else
+      st = gfem_group_add(_sys, p->dim, p->type, b->size(), b->conn.data(), p->num.data(), p->atype.data(),
+                          p->e2n.empty() ? nullptr : p->e2n.data(), &g);
     if(st != GFEM_OK) throw common::Exception(std::string("gfem_group_add: ") + gfem_ctx_last_error(ctx));
     for(unsigned ti : p->terms) {
       const Term &t = _terms[ti];

@@ -207,7 +207,12 @@ class Formulation {
   void setAngularFrequency(double w) { _frequency = w; }
   unsigned int numRHS() const { return _numRHS; }
 
+  // pre(): DOF numbering, pattern and element maps.  On the device by
+  // default (gfem_pre_*: bit-identical results, kept on the GPU for the
+  // system); setPreOnDevice(false) runs the host implementation.
   common::Timer pre(DofsSort sort = DofsSort::Default);
+  void setPreOnDevice(bool d) { _preDevice = d; }
+  bool preOnDevice() const { return _preDevice; }
   common::Timer assemble(const AssembleOptions &opt = AssembleOptions());
   // reference: Timer solve(bool reusePreconditioner); needs an FP64 assembly
   common::Timer solve(const SolveOptions &opt = SolveOptions());
@@ -217,7 +222,7 @@ class Formulation {
   uint64_t getNumberOfFixedDof() const { return _nFixed; }
   uint64_t getNumberOfNonZeros() const { return _ai.empty() ? 0 : (uint64_t)_ai.back(); }
   const std::vector<int64_t> &ai() const { return _ai; }
-  const std::vector<int64_t> &aj() const { return _aj; }
+  const std::vector<int64_t> &aj() const;
   // numDof (1-based) and type (1 unknown, 2 fixed, 0 none) of the dof of a node tag
   std::pair<int64_t, int> nodeDof(int64_t tag) const;
   // element maps of the block (dim, tag, type): numDof-1 / AssembleType per key
@@ -244,6 +249,8 @@ class Formulation {
     std::vector<int32_t> atype;
     std::vector<int64_t> e2n;
     std::vector<unsigned> terms;
+    bool bilinear = false;
+    bool onHost = true;  // maps above filled (device pre(): copied on request)
   };
 
   std::string _name;
@@ -257,8 +264,11 @@ class Formulation {
   std::vector<int64_t> _dofNum;     // node tag -> numDof (1-based)
   std::vector<int8_t> _dofType;     // node tag -> 1 unknown, 2 fixed
   std::vector<std::complex<double>> _fixedValues;
-  std::vector<int64_t> _ai, _aj;
-  std::vector<Pass> _passes;
+  std::vector<int64_t> _ai;
+  mutable std::vector<int64_t> _aj;  // device pre(): copied on request
+  mutable std::vector<Pass> _passes;
+  bool _preDevice = true;
+  gfem_pre *_pre = nullptr;  // device pre() results
   std::vector<std::vector<std::pair<int, int>>> _termSlots;  // term -> (group, index in group) on the device
   gfem_system *_sys = nullptr;
   std::vector<T> _values;
@@ -269,7 +279,11 @@ class Formulation {
   double _preSeconds = 0.;
 
   const Pass *findPass(int dim, int tag, int type) const;
+  const Pass *hostPass(int dim, int tag, int type) const;  // with its maps on the host
   void releaseSystem();
+  void releasePre();
+  void preHost(DofsSort sort);
+  void preDevice(DofsSort sort);
   void upload();
 };
 

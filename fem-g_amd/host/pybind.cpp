@@ -81,7 +81,17 @@ void bind_formulation(py::module_ &m, const char *name)
     .def("set_angular_frequency", &F::setAngularFrequency)
     .def("num_rhs", &F::numRHS)
     .def(
-      "pre", [](F &f, const std::string &s) { return f.pre(parse_sort(s)).time(); }, py::arg("sort") = "default")
+      "pre",
+      [](F &f, const std::string &s, bool device) {
+        f.setPreOnDevice(device);
+        double t;
+        {
+          py::gil_scoped_release rel;
+          t = f.pre(parse_sort(s)).time();
+        }
+        return t;
+      },
+      py::arg("sort") = "default", py::arg("device") = true)
     .def(
       "assemble",
       [](F &f, const std::string &prec, const std::string &strat, bool download) {
@@ -172,6 +182,42 @@ void bind_formulation(py::module_ &m, const char *name)
     .def(
       "rhs", [](const F &f, unsigned i) { return to_numpy(f.getRHS(i)); }, py::arg("idx") = 0)
     .def("node_dof", &F::nodeDof)
+    .def("system_maps",
+         [](const F &f, int which) {
+           int64_t n = 0;
+           check(gfem_system_maps(f.system(), which, nullptr, 0, &n), "gfem_system_maps");
+           const bool wide = which == 1 || which == 3;
+           if(wide) {
+             py::array_t<int64_t> a((py::ssize_t)n);
+             check(gfem_system_maps(f.system(), which, a.mutable_data(), sizeof(int64_t) * (size_t)n, &n),
+                   "gfem_system_maps");
+             return py::object(a);
+           }
+           if(which == 5) {
+             py::array_t<int32_t> a((py::ssize_t)n);
+             check(gfem_system_maps(f.system(), which, a.mutable_data(), sizeof(int32_t) * (size_t)n, &n),
+                   "gfem_system_maps");
+             return py::object(a);
+           }
+           py::array_t<uint32_t> a((py::ssize_t)n);
+           check(gfem_system_maps(f.system(), which, a.mutable_data(), sizeof(uint32_t) * (size_t)n, &n),
+                 "gfem_system_maps");
+           return py::object(a);
+         })
+    .def("num_groups",
+         [](const F &f) {
+           int64_t n = 0;
+           check(gfem_group_maps(f.system(), -1, 0, nullptr, 0, &n), "gfem_group_maps");
+           return n;
+         })
+    .def("group_maps",
+         [](const F &f, int g, int which) {
+           int64_t n = 0;
+           check(gfem_group_maps(f.system(), g, which, nullptr, 0, &n), "gfem_group_maps");
+           py::array_t<int32_t> a((py::ssize_t)n);
+           check(gfem_group_maps(f.system(), g, which, a.mutable_data(), (size_t)n, &n), "gfem_group_maps");
+           return a;
+         })
     .def("block_maps", [](const F &f, int dim, int tag, int type) -> py::object {
       const auto *num = f.blockDofs(dim, tag, type);
       if(!num) return py::none();

@@ -199,8 +199,81 @@ gfem_status gfem_group_add(gfem_system *sys, int dim, int gmsh_type, int64_t ne,
 gfem_status gfem_group_add_term(gfem_system *sys, int group, const gfem_term_desc *term);
 
 /* Build the device maps (nonzero-sorted element map, RHS maps, reference
- * tensors).  Must follow the last gfem_group_add[_term]. */
+ * tensors).  Must follow the last gfem_group_add[_term].  The maps are built
+ * on the device; the environment variable GFEM_HOST_FINALIZE=1 builds the
+ * same maps on the host when every group came from gfem_group_add. */
 gfem_status gfem_system_finalize(gfem_system *sys);
+
+/* ---- pre() on the device ------------------------------------------------
+ * DOF creation and numbering, Hilbert renumbering, element -> DOF maps, CSR
+ * pattern and element -> nonzero maps computed on the GPU and kept there for
+ * gfem_system_create_pre / gfem_group_add_pre.  Results are identical to the
+ * host pre(): first-touch numbering, stable Hilbert order of the same float
+ * keys, sorted unique columns per row.
+ *   gfem_pre_number    DofsFactory::generateFixedDofs / _generateUnknownDofs,
+ *                      dofs/DofsFactory.cpp:39-74, 406-460;
+ *                      DofsManager::reorderWithHilbert, dofs/DofsManager.cpp:118-239,
+ *                      reorder/Hilbert.cpp:225-341
+ *   gfem_pre_pattern   FieldInterface::fillIndices, field/FieldInterface.cpp:959-1076;
+ *                      Pattern::build + MatrixFactory::addPattern / finalizePattern,
+ *                      term/Pattern.cpp:54-72, system/MatrixFactory.cpp:187-213, 458-539
+ *   gfem_system_create_pre / gfem_group_add_pre
+ *                      gfem_system_create / gfem_group_add on those results
+ *                      (no host copy of the pattern or the maps) */
+typedef struct gfem_pre gfem_pre;
+
+/* A workspace on the context's current nodes (gfem_nodes_upload; copied). */
+gfem_status gfem_pre_create(gfem_ctx *ctx, gfem_pre **out);
+gfem_status gfem_pre_destroy(gfem_pre *pre);
+/* An element block: ne elements of npe keys, node tags (1-based) in gmsh
+ * local order; returns its index. */
+gfem_status gfem_pre_add_block(gfem_pre *pre, int64_t ne, int npe, const int64_t *conn, int *block_out);
+/* Fixed DOFs: first touch over the blocks fixed_blocks[] in that order (one
+ * entry per (constraint, entity, element type); a block may repeat).
+ * Unknown DOFs: first touch over unknown_blocks[], skipping fixed nodes.
+ * hilbert_dim 2 or 3: unknown DOFs renumbered along the Hilbert curve of the
+ * box [bbox_min, bbox_max] (floats, widened as DofsManager.cpp:153-156);
+ * 0: first-touch order kept. */
+gfem_status gfem_pre_number(gfem_pre *pre, int n_fixed_blocks, const int *fixed_blocks, int n_unknown_blocks,
+                            const int *unknown_blocks, int hilbert_dim, const float *bbox_min,
+                            const float *bbox_max, int64_t *n_fixed, int64_t *n_unknown);
+/* Passes: one block each, with or without bilinear terms.  Element maps of
+ * every pass; pattern (unknown x unknown, row = tf DOF; upper triangle when
+ * symmetric) and element -> nonzero maps of the bilinear passes. */
+gfem_status gfem_pre_pattern(gfem_pre *pre, int n_passes, const int *pass_blocks, const int *pass_bilinear,
+                             int symmetric, int64_t *nnz);
+/* Copies to the host (any pointer may be NULL).
+ *   node_num[n_nodes]: numDof (1-based) of node tag t at [t-1], 0 when none;
+ *   node_type[n_nodes]: 1 unknown, 2 fixed, 0 none;
+ *   fixed_tags[n_fixed]: node tag of each fixed DOF in numDof order;
+ *   fixed_seq[n_fixed]: index into fixed_blocks[] of its first occurrence. */
+gfem_status gfem_pre_dofs(const gfem_pre *pre, int64_t *node_num, int8_t *node_type, int64_t *fixed_tags,
+                          int32_t *fixed_seq);
+gfem_status gfem_pre_csr(const gfem_pre *pre, int64_t *ai, int64_t *aj);
+/* Element maps of pass `pass`, given element order, as gfem_group_add takes
+ * them: dof_num[ne*npe], dof_type[ne*npe], elem2nnz[ne*npe*npe]. */
+gfem_status gfem_pre_pass_maps(const gfem_pre *pre, int pass, int64_t *dof_num, int32_t *dof_type,
+                               int64_t *elem2nnz);
+gfem_status gfem_system_create_pre(gfem_ctx *ctx, gfem_scalar scalar, const gfem_pre *pre, const double *values_dc,
+                                   int n_rhs, double frequency, gfem_system **out);
+gfem_status gfem_group_add_pre(gfem_system *sys, const gfem_pre *pre, int pass, int dim, int gmsh_type,
+                               int *group_out);
+
+/* Inspection of the maps a finalized system holds (tests and tools; no
+ * counterpart in the reference).  `which`:
+ *   0 gather records (uint32, n_chunks x record stride)
+ *   1 linear RHS row pointers (int64, n+1)     2 linear RHS words (uint32)
+ *   3 lifting row pointers (int64, n+1)        4 lifting words (uint32)
+ *   5 lifting fixed DOFs (int32)
+ * With host == NULL only *count (entries) is set. */
+gfem_status gfem_system_maps(const gfem_system *sys, int which, void *host, size_t bytes, int64_t *count);
+/* Storage-order maps of group `group` (int32 each): which 0 vertices
+ * (node index) [ne][dim+1], 1 numDof-1 [ne][npe], 2 AssembleType [ne][npe],
+ * 3 element -> nonzero [ne][npe*npe] (bilinear groups), 4 given index ->
+ * storage index [ne].  *count = entries; copied when host != NULL and
+ * capacity >= *count.  group = -1 sets *count to the number of groups. */
+gfem_status gfem_group_maps(const gfem_system *sys, int group, int which, int32_t *host, size_t capacity,
+                            int64_t *count);
 
 /* Assemble A and every b on `stream` (a hipStream_t, NULL = null stream).
  * Asynchronous: values are ready when the stream reaches this point. */

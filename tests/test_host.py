@@ -37,7 +37,7 @@ def _specs():
 def test_pre_matches_oracle_bit_exact(name):
     spec = _specs()[name]
     F = W.build(spec)
-    F.pre(spec.dofs_sort)
+    F.pre(spec.dofs_sort, device=False)
     P = O.from_spec(spec)
     P.pre()
     assert (F.n_unknown, F.n_fixed, F.nnz) == (P.n_unknown, P.n_fixed, P.nnz)
@@ -77,7 +77,7 @@ def test_structured_nnz_closed_form():
     spec = W.poisson2d(n=n, order=1, jitter=0.0)
     spec.dirichlet = []
     F = W.build(spec)
-    F.pre()
+    F.pre(device=False)
     # graph of the diagonal split: each vertex connects to itself, 4 axis
     # neighbours and 2 diagonal neighbours (i+1,j+1), (i-1,j-1)
     cnt = 0
@@ -158,7 +158,7 @@ def test_formulation_error_paths():
     F = gfem.Formulation("empty")
     gfem.set_current_model(spec.model)
     with pytest.raises(Exception):
-        F.pre()  # no term
+        F.pre(device=False)  # no term
     u = gfem.Field("u", gfem.Domain(2, 1), 1)
     with pytest.raises(Exception):
         F.integral(gfem.tf(u), gfem.dof(u), gfem.Domain(2, 1), "Gauss2")  # tf on the dof side
@@ -175,7 +175,7 @@ def test_solve_before_assemble_is_an_error():
     reference's exception (logged, solution left empty), without a device."""
     from gfem import workloads as W
     F = W.build(W.poisson2d(n=4, order=1))
-    F.pre()
+    F.pre(device=False)
     with pytest.raises(RuntimeError, match="solve before assemble"):
         F.solve()
 
@@ -187,7 +187,7 @@ def test_term_coefficient_keeps_its_form():
     import gfem
     from gfem import workloads as W
     F = W.build(W.helmholtz3d(n=3, order=1))
-    F.pre()
+    F.pre(device=False)
     with pytest.raises(Exception, match="keep its form"):
         F.set_term_coefficient(W.HELMHOLTZ_MASS, gfem.exp(gfem.x()))
     with pytest.raises(Exception, match="keep its form"):

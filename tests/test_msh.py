@@ -37,10 +37,10 @@ def test_read_back_mesh_preprocesses_identically(tmp_path):
     p = str(tmp_path / "cube.msh")
     gfem.write_msh(spec.model, p)
     F1 = W.build(spec)
-    F1.pre()
+    F1.pre(device=False)
     spec.model = gfem.read_msh(p)
     F2 = W.build(spec)
-    F2.pre()
+    F2.pre(device=False)
     assert np.array_equal(F1.ai(), F2.ai()) and np.array_equal(F1.aj(), F2.aj())
 
 
