@@ -239,6 +239,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true", help="skip the mixed / reference-strategy lines")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving stream")
+    ap.add_argument("--no-setup-host", action="store_true", help="skip the host set-up (pre + maps) timing")
     ap.add_argument("--no-highorder", action="store_true", help="skip the order-4 matrix-core comparison")
     ap.add_argument("--ho-n", type=int, default=32)
     ap.add_argument("--serve-requests", type=int, default=256)
@@ -267,12 +268,28 @@ def main():
     t_mesh = time.perf_counter() - t0
     log(f"mesh {spec.name}: {t_mesh:.2f} s")
     F = W.build(spec)
-    t_pre = F.pre()
-    log(f"pre: {t_pre:.2f} s  n={F.n_unknown} nnz={F.nnz}")
+    t_pre = F.pre()  # on the device
+    log(f"pre (device): {t_pre:.2f} s  n={F.n_unknown} nnz={F.nnz}")
     edofs = spec.element_dofs()
     t0 = time.perf_counter()
-    F.assemble(download=False)  # first call uploads the system to the device
-    log(f"first assemble (upload + maps): {time.perf_counter() - t0:.2f} s")
+    F.assemble(download=False)  # first call builds the system's maps on the device
+    t_first = time.perf_counter() - t0
+    log(f"first assemble (system + maps, device): {t_first:.2f} s")
+    setup = {"device": {"pre_s": t_pre, "first_assemble_s": t_first, "total_s": t_pre + t_first}}
+    if not args.no_setup_host:
+        # the same set-up on the host: pre() on the CPU, maps built on the CPU
+        Fh = W.build(spec)
+        th_pre = Fh.pre(device=False)
+        os.environ["GFEM_HOST_FINALIZE"] = "1"
+        t0 = time.perf_counter()
+        Fh.assemble(download=False)
+        th_first = time.perf_counter() - t0
+        del os.environ["GFEM_HOST_FINALIZE"]
+        del Fh
+        setup["host"] = {"pre_s": th_pre, "first_assemble_s": th_first, "total_s": th_pre + th_first}
+        setup["speedup"] = setup["host"]["total_s"] / setup["device"]["total_s"]
+        log(f"set-up host: pre {th_pre:.2f} s + first assemble {th_first:.2f} s; device "
+            f"{t_pre + t_first:.2f} s ({setup['speedup']:.1f}x)")
 
     sec = timed_steps(F, args.steps, args.warmup, "fp64", "fused")
     value = edofs / sec / 1e9
@@ -333,8 +350,7 @@ def main():
         "config": {"workload": f"helmholtz3d unit cube {args.n}^3x6 jittered P{args.order} tets + Robin "
                                f"P{args.order} boundary triangles, k=10pi, Gaussian source, complex128",
                    "elements": sum(b["conn"].shape[0] for b in spec.model.blocks() if b["dim"] >= 2),
-                   "element_dofs": edofs, "n_unknown": int(F.n_unknown), "nnz": int(F.nnz),
-                   "pre_host_s": t_pre},
+                   "element_dofs": edofs, "n_unknown": int(F.n_unknown), "nnz": int(F.nnz)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": measured_traffic(GATHER_KERNEL.format(args.gather_slots)) if args.n == 50 and args.order == 2 else None,
@@ -343,6 +359,7 @@ def main():
         "cpu_baseline": cpu,
         "autotuned": tuning,
     }
+    line["setup"] = setup
     line.update(extra)
     if serve is not None:
         line["serving"] = serve

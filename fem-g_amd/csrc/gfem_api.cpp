@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -24,6 +25,27 @@ thread_local std::string g_err;
 }  // namespace
 
 namespace gfem {
+
+static double wall_seconds()
+{
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+SetupTimer::SetupTimer()
+{
+  const char *e = std::getenv("GFEM_SETUP_TIMING");
+  on = e && e[0] == '1';
+  t = wall_seconds();
+}
+
+void SetupTimer::mark(const char *what)
+{
+  if(!on) return;
+  (void)hipDeviceSynchronize();
+  const double n = wall_seconds();
+  std::fprintf(stderr, "gfem set-up: %-28s %8.3f s\n", what, n - t);
+  t = n;
+}
 
 gfem_status fail(gfem_ctx *ctx, gfem_status s, const std::string &msg)
 {
@@ -438,7 +460,8 @@ static gfem_status build_maps_host(gfem_system *sys)
   // Element -> nonzero map, transposed: the contributions of every nonzero in
   // traversal order (group, element, local pair), cut into gather chunks of
   // whole nonzeros with <= kChunk contributions and <= kChunkElems distinct
-  // elements.  A chunk's words are in nonzero order (slot order).
+  // elements, never across a multiple of kChunkSegment nonzeros.  A chunk's
+  // words are in nonzero order (slot order).
   {
     if(logP > kMaxLogP) return fail(ctx, GFEM_ERR_UNSUPPORTED, "more than 64 dofs by element in the gather words");
     sys->rec_wide = logP > kNarrowLogP ? 1 : 0;
@@ -497,7 +520,8 @@ static gfem_status build_maps_host(gfem_system *sys)
       const int64_t nz0 = nz;
       elist.clear();
       int nel = 0;
-      while(nz < sys->nnz && cnt[nz + 1] - c0 <= kChunk) {
+      const int64_t seg_end = std::min<int64_t>(sys->nnz, (nz0 / kChunkSegment + 1) * kChunkSegment);
+      while(nz < seg_end && cnt[nz + 1] - c0 <= kChunk) {
         int fresh = 0;
         for(int64_t k = cnt[nz]; k < cnt[nz + 1]; ++k)
           if(mark[c_elem[k]] != chunk) ++fresh;
@@ -604,6 +628,7 @@ gfem_status gfem_system_finalize(gfem_system *sys)
   if(sys->finalized) return fail(ctx, GFEM_ERR_STATE, "finalize called twice");
   if(!sys->d_xyz) return fail(ctx, GFEM_ERR_STATE, "nodes not uploaded before gfem_system_create");
   GFEM_HIP(ctx, hipSetDevice(ctx->device));
+  SetupTimer timer;
   // module: AFrequency when a bilinear term carries a time derivative
   // (problem/Formulation.cpp:1095-1149)
   sys->freq_module = false;
@@ -733,6 +758,7 @@ gfem_status gfem_system_finalize(gfem_system *sys)
       gd.rules[r] = RuleDev{g.rules[r].Q, g.rules[r].d_w, g.rules[r].d_pts, g.rules[r].d_bf, g.rules[r].d_gbf};
     GFEM_HIP(ctx, dupload(&g.d_dev, &gd, 1));
   }
+  timer.mark("groups (rules, tensors, jit)");
   // element load vectors of every group in one buffer, so a right-hand-side
   // term is one flat index; right-hand sides without a linear term on a
   // group keep zeros there
@@ -788,6 +814,7 @@ gfem_status gfem_system_finalize(gfem_system *sys)
     for(size_t i = 0; i < ip.size(); ++i) ip[i] = (int32_t)g.h_iperm[i];
     GFEM_HIP(ctx, dupload(&g.d_iperm, ip.data(), ip.size()));
   }
+  timer.mark("load buffers, host maps up");
   // gather records and RHS maps: built on the device; GFEM_HOST_FINALIZE=1
   // builds them on the host from the host copies of the maps (the same
   // records: a cross-check of the device builder)
@@ -798,6 +825,7 @@ gfem_status gfem_system_finalize(gfem_system *sys)
       if(!g.host_maps) host = false;
     const gfem_status st = host ? build_maps_host(sys) : build_maps_device(sys);
     if(st != GFEM_OK) return st;
+    timer.mark(host ? "maps (host)" : "maps (device)");
   }
   for(auto &g : sys->groups) {
     std::vector<int32_t>().swap(g.h_vtx);

@@ -42,6 +42,9 @@ constexpr int kNarrowLogP = 8;    // pair bits that fit a 16-bit word
 constexpr int kMaxLogP = 12;      // pair bits of N <= kMaxN
 constexpr int kRecNarrow = kRecHead + kChunk / 2 + kChunkElems;  // 388 uint32 (1552 B)
 constexpr int kRecWide = kRecHead + kChunk + kChunkElems;        // 644 uint32 (2576 B)
+// gather chunks never cross a multiple of kChunkSegment nonzeros, so the
+// segments are cut into chunks independently (in parallel on the device)
+constexpr int64_t kChunkSegment = 16384;
 
 // Coefficient program as read by the kernels.
 struct DevExpr {
@@ -272,6 +275,15 @@ void dfree(T *&p)
 
 // gather records and RHS maps of a system built on the device (setup.hip)
 gfem_status build_maps_device(gfem_system *sys);
+
+// Wall-clock stages of the set-up on stderr when GFEM_SETUP_TIMING=1 (the
+// device is synchronised at each mark).
+struct SetupTimer {
+  bool on = false;
+  double t = 0.;
+  SetupTimer();
+  void mark(const char *what);
+};
 
 // GMRES with FP64 iterative refinement on the last FP64 assembly (solver.hip)
 gfem_status solve_gmres_ir(gfem_system *sys, int rhs_idx, double tol, int restart, int max_iter, hipStream_t s,

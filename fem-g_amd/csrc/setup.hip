@@ -22,9 +22,8 @@
 //   - gather records: contributions stably sorted by nonzero (traversal
 //     order inside a nonzero), cut into chunks by the same greedy rule
 //     (whole nonzeros, <= kChunk contributions, <= kChunkElems distinct
-//     elements), computed by speculative parallel walks that are re-run
-//     from the corrected entry point until every segment agrees with its
-//     predecessor.
+//     elements, never across a multiple of kChunkSegment nonzeros), one
+//     wavefront per segment.
 // All integer work: deterministic by construction (atomics only count).
 #include <hip/hip_runtime.h>
 
@@ -541,36 +540,29 @@ struct Window {
 
 constexpr int kWalkWaves = 4;
 
-// One wavefront walks one segment [b0, b1) of the nonzeros from its entry
-// E[s] (the first chunk start at or after b0), cutting chunks with the host
-// builder's greedy rule: whole nonzeros while the chunk holds <= kChunk
-// contributions and <= kChunkElems distinct elements (counted against the
-// elements of the chunk before the nonzero).  The chunk's element list is
-// held two per lane; membership is one ballot.  flags[z] = 1 marks a chunk
-// start; a walk writes flags only inside its own segment.  On a re-walk
-// (early = 1) reaching a start of the previous walk ends it: the greedy rule
-// is memoryless, so the rest of the chain and the exit X[s] are unchanged.
+// One wavefront cuts one segment [b0, b1) of kChunkSegment nonzeros into
+// gather chunks with the host builder's greedy rule: whole nonzeros while the
+// chunk holds <= kChunk contributions and <= kChunkElems distinct elements
+// (counted against the elements of the chunk before the nonzero), a new
+// chunk at every segment start.  The chunk's element list is held two per
+// lane; membership is one ballot.  flags[z] = 1 marks a chunk start.
 __global__ __launch_bounds__(64 * kWalkWaves) void k_chunk_walk(
-  const uint32_t *__restrict__ cptr, const uint32_t *__restrict__ cel, uint32_t nnz, uint32_t total, uint32_t L,
-  uint32_t S, const uint32_t *__restrict__ E, uint32_t *__restrict__ X, const uint32_t *__restrict__ dirty,
-  uint32_t *__restrict__ flags, int early)
+  const uint32_t *__restrict__ cptr, const uint32_t *__restrict__ cel, uint32_t nnz, uint32_t total, uint32_t S,
+  uint32_t *__restrict__ flags)
 {
   const int lane = threadIdx.x & 63;
   const uint32_t s = blockIdx.x * kWalkWaves + (threadIdx.x >> 6);
-  if(s >= S || !dirty[s]) return;
-  const uint32_t b0 = s * L;
-  const uint32_t b1 = (s + 1 == S) ? nnz : b0 + L;
-  uint32_t z = E[s];
-  for(uint32_t q = b0 + lane; q < min(z, b1); q += 64) flags[q] = 0u;
+  if(s >= S) return;
+  const uint32_t b0 = s * (uint32_t)kChunkSegment;
+  const uint32_t b1 = min(nnz, b0 + (uint32_t)kChunkSegment);
   Window wp, wc;
+  uint32_t z = b0;
   while(z < b1) {
-    if(early && flags[z]) return;
-    if(lane == 0) flags[z] = 1u;
     const uint32_t c0 = wp.get(cptr, z, nnz + 1, lane);
     uint32_t m0 = 0xFFFFFFFFu, m1 = 0xFFFFFFFFu;
     int nel = 0;
     uint32_t zz = z;
-    while(zz < nnz) {
+    while(zz < b1) {
       const uint32_t clo = wp.get(cptr, zz, nnz + 1, lane);
       const uint32_t chi = wp.get(cptr, zz + 1, nnz + 1, lane);
       if(chi - c0 > (uint32_t)kChunk) break;
@@ -594,29 +586,9 @@ __global__ __launch_bounds__(64 * kWalkWaves) void k_chunk_walk(
       }
       ++zz;
     }
-    for(uint32_t q = z + 1 + lane; q < min(zz, b1); q += 64) flags[q] = 0u;
+    for(uint32_t q = z + lane; q < zz; q += 64) flags[q] = q == z ? 1u : 0u;
     z = zz;
   }
-  if(lane == 0) X[s] = z;
-}
-
-__global__ void k_chunk_fixup(uint32_t S, uint32_t *__restrict__ E, const uint32_t *__restrict__ X,
-                              uint32_t *__restrict__ dirty, uint32_t *__restrict__ changed)
-{
-  const int64_t s = gid();
-  if(s >= S) return;
-  if(s == 0) {
-    dirty[0] = 0u;
-    return;
-  }
-  const uint32_t x = X[s - 1];
-  if(x != E[s]) {
-    E[s] = x;
-    dirty[s] = 1u;
-    atomicOr(changed, 1u);
-  }
-  else
-    dirty[s] = 0u;
 }
 
 __global__ void k_chunk_starts(const uint32_t *__restrict__ flags, const uint32_t *__restrict__ idx, uint32_t nnz,
@@ -823,6 +795,7 @@ gfem_status build_maps_device(gfem_system *sys)
   gfem_ctx *ctx = sys->ctx;
   const int logP = sys->logP;
   const int ng = (int)sys->groups.size();
+  SetupTimer timer;
   Temp tmp;
   Buf<uint32_t> err;
   SETUP_HIP(err.alloc(1));
@@ -897,8 +870,10 @@ gfem_status build_maps_device(gfem_system *sys)
     if(herr & kErrNoContribution) return fail(ctx, GFEM_ERR_INVALID_ARG, "a stored nonzero has no element contribution");
     if(herr & kErrTooManyContrib)
       return fail(ctx, GFEM_ERR_UNSUPPORTED, "a nonzero has more than 128 element contributions");
+    timer.mark("  contribution keys");
     SETUP_HIP(sort_pairs(tmp, key.p, skey.p, val.p, sval.p, (uint32_t)total, bitlen((uint64_t)nnz)));
     SETUP_HIP(exclusive_scan(tmp, cnt.p, cptr.p, (size_t)nnz + 1));
+    timer.mark("  contribution sort");
     uint32_t nvalid = 0;
     SETUP_HIP(to_host(&nvalid, cptr.p + nnz));
     key.reset();
@@ -913,37 +888,16 @@ gfem_status build_maps_device(gfem_system *sys)
       SETUP_LAUNCH();
     }
     sval.reset();
-    // chunk starts: speculative segment walks until every entry agrees
+    // chunk starts: one walk per segment of kChunkSegment nonzeros
     uint32_t n_chunks = 0;
     Buf<uint32_t> starts;
     if(nnz > 0) {
-      const uint32_t L = std::max<uint32_t>(1024u, (uint32_t)((nnz + 16383u) / 16384u));
-      const uint32_t S = (nnz + L - 1) / L;
-      Buf<uint32_t> flags, E, X, dirty, changed;
+      const uint32_t S = (uint32_t)((nnz + kChunkSegment - 1) / kChunkSegment);
+      Buf<uint32_t> flags;
       SETUP_HIP(flags.alloc(nnz));
-      SETUP_HIP(E.alloc(S));
-      SETUP_HIP(X.alloc(S));
-      SETUP_HIP(dirty.alloc(S));
-      SETUP_HIP(changed.alloc(1));
-      {
-        std::vector<uint32_t> e0(S);
-        for(uint32_t s = 0; s < S; ++s) e0[s] = s * L;
-        SETUP_HIP(hipMemcpy(E.p, e0.data(), sizeof(uint32_t) * S, hipMemcpyHostToDevice));
-      }
-      SETUP_HIP(fill<uint32_t>(dirty.p, 1u, S));
-      const unsigned wblocks = (S + kWalkWaves - 1) / kWalkWaves;
-      for(uint32_t it = 0;; ++it) {
-        if(it > S + 1) return fail(ctx, GFEM_ERR_STATE, "chunk walk did not settle");
-        k_chunk_walk<<<wblocks, 64 * kWalkWaves>>>(cptr.p, cel.p, nnz, nvalid, L, S, E.p, X.p, dirty.p, flags.p,
-                                                   it > 0 ? 1 : 0);
-        SETUP_LAUNCH();
-        SETUP_HIP(hipMemset(changed.p, 0, sizeof(uint32_t)));
-        k_chunk_fixup<<<nblk(S), kTB>>>(S, E.p, X.p, dirty.p, changed.p);
-        SETUP_LAUNCH();
-        uint32_t ch = 0;
-        SETUP_HIP(to_host(&ch, changed.p));
-        if(!ch) break;
-      }
+      k_chunk_walk<<<(S + kWalkWaves - 1) / kWalkWaves, 64 * kWalkWaves>>>(cptr.p, cel.p, nnz, nvalid, S, flags.p);
+      SETUP_LAUNCH();
+      timer.mark("  chunk walk");
       Buf<uint32_t> idx;
       SETUP_HIP(idx.alloc(nnz));
       SETUP_HIP(exclusive_scan(tmp, flags.p, idx.p, nnz));
@@ -969,6 +923,7 @@ gfem_status build_maps_device(gfem_system *sys)
     }
     unsigned long long hnel = 0;
     SETUP_HIP(to_host(&hnel, nel.p));
+    timer.mark("  records");
     sys->n_chunks = n_chunks;
     sys->n_words = nvalid;
     sys->n_elist = (int64_t)hnel;
@@ -1046,6 +1001,7 @@ gfem_status build_maps_device(gfem_system *sys)
         return fail(ctx, GFEM_ERR_TOO_LARGE, "too many elements for the Dirichlet lifting map");
     }
     sys->n_dcw = n_dc;
+    timer.mark("  rhs maps");
   }
   SETUP_HIP(hipDeviceSynchronize());
   return GFEM_OK;
@@ -1478,6 +1434,7 @@ gfem_status gfem_group_add_pre(gfem_system *sys, const gfem_pre *pre, int pass, 
   const int N = g.N;
   const int64_t ne = g.ne;
   Temp tmp;
+  SetupTimer timer;
   // affinity of the high-order nodes
   if(ne > 0 && g.order > 1) {
     const std::vector<Lat> lat = lattice_nodes(dim, et.order);
@@ -1536,6 +1493,7 @@ gfem_status gfem_group_add_pre(gfem_system *sys, const gfem_pre *pre, int pass, 
   }
   sys->groups.push_back(std::move(g));
   if(group_out) *group_out = (int)sys->groups.size() - 1;
+  timer.mark("group_add_pre");
   return GFEM_OK;
 }
 
