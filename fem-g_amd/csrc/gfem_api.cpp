@@ -705,7 +705,8 @@ gfem_status gfem_system_finalize(gfem_system *sys)
     for(size_t k = 0; k < g.dterms.size(); ++k)
       if(g.dterms[k].kind == 1 && g.dterms[k].per_point) {
         std::string log;
-        g.jit[k] = jit_linear_kernel(ctx->device, g.dterms[k].expr, &log);
+        g.jit[k] = jit_linear_kernel(ctx->device, g.dterms[k].expr, g.rules[g.dterms[k].rule].Q, g.N, g.nv, g.dim,
+                                     &log);
         if(!g.jit[k] && !log.empty()) std::fprintf(stderr, "gfem: coefficient compilation failed, interpreting:\n%s\n", log.c_str());
       }
     const std::vector<Lat> lat = lattice_nodes(g.dim, g.order);

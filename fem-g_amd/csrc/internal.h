@@ -307,6 +307,8 @@ hipError_t launch_ref_rhs(const gfem_system &sys, const Group &g, hipStream_t s)
 
 // Compiled point-parallel linear-term kernel for a coefficient program
 // (jit.cpp); null when run-time compilation is off or fails.
-hipFunction_t jit_linear_kernel(int device, const DevExpr &ex, std::string *log);
+// The kernel is specialised for the group's shape: Q Gauss points, N dofs,
+// nv vertices, dimension dim.
+hipFunction_t jit_linear_kernel(int device, const DevExpr &ex, int Q, int N, int nv, int dim, std::string *log);
 
 }  // namespace gfem

@@ -13,6 +13,7 @@
 
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <map>
 #include <mutex>
 #include <sstream>
@@ -59,70 +60,63 @@ struct JitLinArgs {
 };
 )";
 
-// A block takes EB consecutive elements: their vertex coordinates and
-// determinants are staged in LDS once, then one thread per (element, Gauss
-// point) evaluates c_q = w_q det fac f(x_q) into LDS, and one thread per
-// (element, local dof) reduces b_e(i) = sum_q phi_i(q) c_q in q order.
+// One thread per element, the group's shape fixed at compile time (Q Gauss
+// points, N dofs, NV vertices, dimension DIM): the element's vertices and
+// determinant in registers, x_q, f(x_q) and c_q = w_q det fac f(x_q) per
+// point, b_e(i) = sum_q phi_i(q) c_q accumulated in registers in q order
+// (onto the element load already there for a second term on the same
+// right-hand side).  Rule and basis are read with uniform (scalar) loads;
+// no LDS, no barrier.
 const char *kLinearBody = R"(
-extern "C" __global__ void __launch_bounds__(256)
+extern "C" __global__ void __launch_bounds__(512)
 gfem_jit_linear(JitLinArgs a, const double *__restrict__ w, const double *__restrict__ pts,
                 const double *__restrict__ bf, i64 ne, const int *__restrict__ vtx,
                 const double *__restrict__ xyz, const double *__restrict__ F64, double2 *__restrict__ be)
 {
-  extern __shared__ double s_dyn[];
-  const int Q = a.Q, N = a.N, EB = a.EB, nv = a.nv;
-  double *s_bf = s_dyn;             // [Q][N]
-  double *s_u = s_bf + Q * N;       // [Q][3] reference points
-  double *s_w = s_u + 3 * Q;        // [Q]
-  double *s_X = s_w + Q;            // [EB][4][3] vertex coordinates
-  double *s_det = s_X + 12 * EB;    // [EB]
-  double2 *s_c = reinterpret_cast<double2 *>(s_det + EB + ((Q * N + 4 * Q + 13 * EB) & 1));  // [EB][Q]
-  const int t = threadIdx.x;
-  const i64 e0 = (i64)blockIdx.x * EB;
-  const int nb = (int)(ne - e0 < (i64)EB ? ne - e0 : (i64)EB);
-  for(int k = t; k < Q * N; k += 256) s_bf[k] = bf[k];
-  for(int k = t; k < 3 * Q; k += 256) s_u[k] = pts[k];
-  for(int k = t; k < Q; k += 256) s_w[k] = w[k];
-  for(int k = t; k < nb * nv * 3; k += 256) {
-    const int le = k / (3 * nv), r = k - le * 3 * nv, v = r / 3, c = r - 3 * v;
-    s_X[12 * le + r] = xyz[3 * (i64)vtx[(e0 + le) * nv + v] + c];
+  constexpr int Q = @Q@, N = @N@, NV = @NV@, DIM = @DIM@;
+  const i64 e = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if(e >= ne) return;
+  double X[NV][3];
+#pragma unroll
+  for(int v = 0; v < NV; ++v) {
+    const i64 n = vtx[e * NV + v];
+#pragma unroll
+    for(int i = 0; i < 3; ++i) X[v][i] = xyz[3 * n + i];
   }
-  for(int k = t; k < nb; k += 256) s_det[k] = F64[(e0 + k) * 8 + 6];
-  __syncthreads();
-  for(int p = t; p < nb * Q; p += 256) {
-    const int le = p / Q, q = p - le * Q;
-    const double *X = &s_X[12 * le];
-    const double *u = &s_u[3 * q];
+  const double det = F64[e * 8 + 6];
+  double2 *o = be + e * N;
+  double2 acc[N];
+#pragma unroll
+  for(int i = 0; i < N; ++i) acc[i] = a.accumulate ? o[i] : make_double2(0., 0.);
+  for(int q = 0; q < Q; ++q) {
     double xp[3];
-    if(a.dim == 1) {
-      const double s = 0.5 * (u[0] + 1.);
-      for(int i = 0; i < 3; ++i) xp[i] = X[i] + s * (X[3 + i] - X[i]);
+    if(DIM == 1) {
+      const double s = 0.5 * (pts[3 * q] + 1.);
+#pragma unroll
+      for(int i = 0; i < 3; ++i) xp[i] = X[0][i] + s * (X[1][i] - X[0][i]);
     } else {
+#pragma unroll
       for(int i = 0; i < 3; ++i) {
-        double s = X[i];
-        for(int j = 0; j < a.dim; ++j) s += u[j] * (X[3 * (j + 1) + i] - X[i]);
+        double s = X[0][i];
+#pragma unroll
+        for(int j = 0; j < DIM; ++j) s += pts[3 * q + j] * (X[j + 1][i] - X[0][i]);
         xp[i] = s;
       }
     }
     const double x = xp[0], y = xp[1], z = xp[2];
     (void)x; (void)y; (void)z;
     @EXPR@
-    const double wd = s_w[q] * s_det[le];
-    s_c[p] = cmul(make_double2(a.fac_re * wd, a.fac_im * wd), f);
-  }
-  __syncthreads();
-  for(int k = t; k < nb * N; k += 256) {
-    const int le = k / N, i = k - le * N;
-    const i64 o = (e0 + le) * N + i;
-    double2 acc = a.accumulate ? be[o] : make_double2(0., 0.);
-    const double2 *c = &s_c[le * Q];
-    for(int q = 0; q < Q; ++q) {
-      const double b = s_bf[q * N + i];
-      acc.x += c[q].x * b;
-      acc.y += c[q].y * b;
+    const double wd = w[q] * det;
+    const double2 c = cmul(make_double2(a.fac_re * wd, a.fac_im * wd), f);
+#pragma unroll
+    for(int i = 0; i < N; ++i) {
+      const double b = bf[q * N + i];
+      acc[i].x += c.x * b;
+      acc[i].y += c.y * b;
     }
-    be[o] = acc;
   }
+#pragma unroll
+  for(int i = 0; i < N; ++i) o[i] = acc[i];
 }
 )";
 
@@ -229,12 +223,14 @@ bool jit_enabled()
 
 }  // namespace
 
-hipFunction_t jit_linear_kernel(int device, const DevExpr &ex, std::string *log)
+hipFunction_t jit_linear_kernel(int device, const DevExpr &ex, int Q, int N, int nv, int dim, std::string *log)
 {
   if(!jit_enabled()) return nullptr;
   std::string src = std::string(kPrelude) + kLinearBody;
   const std::string code = expr_code(ex);
   src.replace(src.find("@EXPR@"), 6, code);
+  const std::pair<const char *, int> shape[] = {{"@Q@", Q}, {"@N@", N}, {"@NV@", nv}, {"@DIM@", dim}};
+  for(const auto &kv : shape) src.replace(src.find(kv.first), std::strlen(kv.first), std::to_string(kv.second));
   Cache &C = cache();
   std::lock_guard<std::mutex> lk(C.m);
   auto key = std::make_pair(device, src);

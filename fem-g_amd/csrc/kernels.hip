@@ -1256,8 +1256,9 @@ hipError_t launch_linear(const Group &g, const double *xyz, int nrhs, int eb_shi
       const unsigned nblk = (unsigned)((g.ne + EB - 1) / EB);
       const size_t ti = (size_t)(&t - g.dterms.data());
       hipFunction_t jf = ti < g.jit.size() ? g.jit[ti] : nullptr;
-      if(jf && lds_pp <= 64 * 1024) {
-        // compiled coefficient program (jit.cpp); argument layout of JitLinArgs
+      if(jf) {
+        // compiled coefficient program (jit.cpp), one thread per element in
+        // blocks of 64 << eb_shift threads; argument layout of JitLinArgs
         struct {
           double consts[2 * GFEM_MAX_CONSTS];
           double fac_re, fac_im;
@@ -1277,7 +1278,9 @@ hipError_t launch_linear(const Group &g, const double *xyz, int nrhs, int eb_shi
         const int32_t *pv = g.d_vtx;
         double2 *pbe = be;
         void *params[] = {&ja, &pw, &pp, &pb, &ne, &pv, &xyz, &pF, &pbe};
-        const hipError_t e = hipModuleLaunchKernel(jf, nblk, 1, 1, 256, 1, 1, (unsigned)lds_pp, s, params, nullptr);
+        const unsigned bs = 64u << std::min(std::max(eb_shift, 1), 3);
+        const unsigned jblk = (unsigned)((g.ne + bs - 1) / bs);
+        const hipError_t e = hipModuleLaunchKernel(jf, jblk, 1, 1, bs, 1, 1, 0, s, params, nullptr);
         if(e != hipSuccess) return e;
         first = false;
         continue;
