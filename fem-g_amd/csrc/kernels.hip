@@ -763,45 +763,52 @@ struct ValTraits<double2> {
 //      once.  No atomics; the order of additions is fixed, so results are
 //      bitwise reproducible.
 
+// Value of one contribution word: pair (i, j), chunk element le.  The
+// element's factors F (7 per element, structure of arrays) sit in LDS with
+// its group and index; the group's reference tensor row [pair][8] is read
+// from memory (it stays in the caches).  Per-point element matrices (planes
+// [real, imaginary][N*N]) are added first.
 template <bool CPLX, bool MIXED>
 __device__ __forceinline__ void gather_eval(uint32_t w, int logP, uint32_t pmask, int tensor,
                                             const typename std::conditional<MIXED, float, double>::type (*s_F)[kChunkElems],
-                                            const void *const *s_T, const uint8_t *s_treal,
-                                            const typename std::conditional<MIXED, float, double>::type *const *s_D,
-                                            const int *s_DNN,
+                                            const uint8_t *s_eg, const uint32_t *s_el, const uint8_t *s_gR,
+                                            const int *s_gNN, const void *const *s_gT,
+                                            const typename std::conditional<MIXED, float, double>::type *const *s_gD,
                                             typename std::conditional<MIXED, float, double>::type &vr,
                                             typename std::conditional<MIXED, float, double>::type &vi)
 {
   using R = typename std::conditional<MIXED, float, double>::type;
   const int pair = (int)(w & pmask);
   const int le = (int)((w >> logP) & 0x7Fu);
+  const int g = s_eg[le];
   vr = 0;
   vi = 0;
-  if(s_D[le]) {  // per-point element matrix, planes [real, imaginary][N*N]
-    const R *d = s_D[le] + pair;
+  if(s_gD[g]) {  // per-point element matrix, planes [real, imaginary][N*N]
+    const int NN = s_gNN[g];
+    const R *d = s_gD[g] + (int64_t)s_el[le] * (CPLX ? 2 : 1) * NN + pair;
     vr = d[0];
-    if(CPLX) vi = d[s_DNN[le]];
+    if(CPLX) vi = d[NN];
   }
   if(!tensor) return;
   R F[8];
 #pragma unroll
   for(int q = 0; q < 7; ++q) F[q] = s_F[q][le];  // structure-of-arrays: no bank conflicts across elements
   F[7] = 0;
-  if(!CPLX || s_treal[le]) {
+  if(!CPLX || s_gR[g]) {
     if(MIXED) {
-      const float4 *T = reinterpret_cast<const float4 *>(s_T[le]) + (int64_t)pair * 2;
+      const float4 *T = reinterpret_cast<const float4 *>(s_gT[g]) + (int64_t)pair * 2;
       const float4 a = T[0], b = T[1];
       vr += F[0] * a.x + F[1] * a.y + F[2] * a.z + F[3] * a.w + F[4] * b.x + F[5] * b.y + F[6] * b.z;
     }
     else {
-      const double2 *T = reinterpret_cast<const double2 *>(s_T[le]) + (int64_t)pair * 4;
+      const double2 *T = reinterpret_cast<const double2 *>(s_gT[g]) + (int64_t)pair * 4;
       const double2 a = T[0], b = T[1], c = T[2], d = T[3];
       vr += F[0] * a.x + F[1] * a.y + F[2] * b.x + F[3] * b.y + F[4] * c.x + F[5] * c.y + F[6] * d.x;
     }
   }
   else {
     if(MIXED) {
-      const float4 *T = reinterpret_cast<const float4 *>(s_T[le]) + (int64_t)pair * 4;
+      const float4 *T = reinterpret_cast<const float4 *>(s_gT[g]) + (int64_t)pair * 4;
 #pragma unroll
       for(int q = 0; q < 4; ++q) {
         const float4 tq = T[q];  // components 2q (x,y) and 2q+1 (z,w)
@@ -810,7 +817,7 @@ __device__ __forceinline__ void gather_eval(uint32_t w, int logP, uint32_t pmask
       }
     }
     else {
-      const double2 *T = reinterpret_cast<const double2 *>(s_T[le]) + (int64_t)pair * 8;
+      const double2 *T = reinterpret_cast<const double2 *>(s_gT[g]) + (int64_t)pair * 8;
 #pragma unroll
       for(int q = 0; q < 7; ++q) {
         const double2 tq = T[q];
@@ -925,7 +932,7 @@ __device__ __forceinline__ void gather_runs(int t, const int *h, const bool *in,
 
 // S slots per thread, kChunk / S threads per block
 template <bool CPLX, bool MIXED, bool WIDE, int S>
-__global__ void __launch_bounds__(kChunk / S)
+__global__ void __launch_bounds__(kChunk / S, 8)
   k_gather(const GroupTable *__restrict__ gtp, const uint32_t *__restrict__ recs, int64_t nchunks, int tensor,
            void *__restrict__ out)
 {
@@ -934,16 +941,15 @@ __global__ void __launch_bounds__(kChunk / S)
   static_assert(B >= kChunkElems && (S == 2 || S == 4), "one list entry per thread at most");
   constexpr int kStride = WIDE ? kRecWide : kRecNarrow;
   constexpr int kList = kStride - kChunkElems;  // element-list region of a record
+  using R2 = typename std::conditional<MIXED, float2, double2>::type;
   __shared__ const void *s_gF[kMaxGroups];
   __shared__ const void *s_gT[kMaxGroups];
   __shared__ const R *s_gD[kMaxGroups];
-  __shared__ int s_gN[kMaxGroups];
+  __shared__ int s_gNN[kMaxGroups];
   __shared__ uint8_t s_gR[kMaxGroups];
-  __shared__ R s_F[8][kChunkElems];
-  __shared__ const void *s_T[kChunkElems];
-  __shared__ const R *s_D[kChunkElems];
-  __shared__ int s_DNN[kChunkElems];
-  __shared__ uint8_t s_treal[kChunkElems];
+  __shared__ R s_F[7][kChunkElems];
+  __shared__ uint32_t s_el[kChunkElems];
+  __shared__ uint8_t s_eg[kChunkElems];
   __shared__ int s_wsum[B / 64];
   __shared__ R s_wx[B / 64], s_wy[B / 64];
   __shared__ int s_wf[B / 64];
@@ -979,41 +985,31 @@ __global__ void __launch_bounds__(kChunk / S)
   const uint32_t ev = t < kChunkElems ? rec[kList + t] : 0u;
   const int logP = gt.logP;
   const uint32_t pmask = (1u << logP) - 1u;
-  if(t < gt.ng) {
+  const int ng = gt.ng;
+  if(t < ng) {
     const bool treal = !CPLX || gt.treal[t];
     s_gR[t] = treal ? 1 : 0;
     s_gF[t] = MIXED ? (const void *)gt.F32[t] : (const void *)gt.F64[t];
     s_gT[t] = MIXED ? (treal ? (const void *)gt.T32r[t] : (const void *)gt.T32[t])
                     : (treal ? (const void *)gt.T64r[t] : (const void *)gt.T64[t]);
     s_gD[t] = MIXED ? (const R *)gt.dense32[t] : (const R *)gt.dense64[t];
-    s_gN[t] = gt.N[t];
+    s_gNN[t] = gt.N[t] * gt.N[t];
   }
   __syncthreads();
-  // element factors: entry j of the list sits in record slot kStride - nel + j
+  // chunk elements: entry j of the list sits in record slot kStride - nel + j
   const int j = t - (kChunkElems - (int)nel);
   if(t < kChunkElems && j >= 0) {
     const int g = (int)(ev >> 27);
-    const int64_t el = ev & 0x7FFFFFFu;
-    s_treal[j] = s_gR[g];
-    s_T[j] = s_gT[g];
-    const int N = s_gN[g];
-    s_D[j] = s_gD[g] ? s_gD[g] + el * (CPLX ? 2 : 1) * N * N : nullptr;
-    s_DNN[j] = N * N;
-    if(MIXED) {
-      const float4 *F = reinterpret_cast<const float4 *>(reinterpret_cast<const float *>(s_gF[g]) + el * 8);
-      const float4 a = F[0], b = F[1];
-      s_F[0][j] = a.x; s_F[1][j] = a.y; s_F[2][j] = a.z; s_F[3][j] = a.w;
-      s_F[4][j] = b.x; s_F[5][j] = b.y; s_F[6][j] = b.z; s_F[7][j] = b.w;
-    }
-    else {
-      const double2 *F = reinterpret_cast<const double2 *>(reinterpret_cast<const double *>(s_gF[g]) + el * 8);
-#pragma unroll
-      for(int q = 0; q < 4; ++q) {
-        const double2 f = F[q];
-        s_F[2 * q][j] = (R)f.x;
-        s_F[2 * q + 1][j] = (R)f.y;
-      }
-    }
+    s_eg[j] = (uint8_t)g;
+    s_el[j] = ev & 0x7FFFFFFu;
+  }
+  __syncthreads();
+  // element factors: 4 lanes per element, 2 factors each (one 64-byte row)
+  for(int idx = t; idx < (int)nel * 4; idx += B) {
+    const int jj = idx >> 2, q = idx & 3;
+    const R2 f = reinterpret_cast<const R2 *>(reinterpret_cast<const R *>(s_gF[s_eg[jj]]) + (int64_t)s_el[jj] * 8)[q];
+    s_F[2 * q][jj] = f.x;
+    if(q < 3) s_F[2 * q + 1][jj] = f.y;
   }
   __syncthreads();
   const uint32_t hb = WIDE ? 31u : 15u;
@@ -1026,7 +1022,8 @@ __global__ void __launch_bounds__(kChunk / S)
     h[k] = in[k] ? (int)((w[k] >> hb) & 1u) : 1;  // past the end: closed, not written
     vr[k] = 0;
     vi[k] = 0;
-    if(in[k]) gather_eval<CPLX, MIXED>(w[k], logP, pmask, tensor, s_F, s_T, s_treal, s_D, s_DNN, vr[k], vi[k]);
+    if(in[k])
+      gather_eval<CPLX, MIXED>(w[k], logP, pmask, tensor, s_F, s_eg, s_el, s_gR, s_gNN, s_gT, s_gD, vr[k], vi[k]);
   }
   gather_runs<CPLX, R, S, B>(t, h, in, vr, vi, ns, s_wsum, s_wx, s_wy, s_wf, out);
 }
