@@ -247,7 +247,7 @@ def main():
     ap.add_argument("--serve-tol", type=float, default=1e-10)
     ap.add_argument("--serve-hbm-gb", type=int, default=64)
     ap.add_argument("--serve-baseline", type=int, default=16, help="requests timed one at a time")
-    ap.add_argument("--gather-slots", type=int, choices=[2, 4], default=4,
+    ap.add_argument("--gather-slots", type=int, choices=[2, 4, 8], default=4,
                     help="contributions per gather thread (4: 128-thread blocks, 2: 256-thread blocks)")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="BYTES",
                     help="run a device-to-device copy of BYTES first (counter calibration)")

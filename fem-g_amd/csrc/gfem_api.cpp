@@ -792,7 +792,8 @@ gfem_status gfem_system_finalize(gfem_system *sys)
   sys->dense_gemm = !(dg && dg[0] == '0');
   if(const char *e = std::getenv("GFEM_GATHER_SLOTS")) {
     // a fixed choice, not tuned
-    sys->gather_slots[0] = sys->gather_slots[1] = std::atoi(e) == 2 ? 2 : 4;
+    const int v = std::atoi(e);
+    sys->gather_slots[0] = sys->gather_slots[1] = (v == 2 || v == 8) ? v : 4;
     sys->tuned[0] = sys->tuned[1] = true;
   }
   // device copies of the storage-order maps of the groups given by the host
@@ -1010,16 +1011,19 @@ static gfem_status autotune(gfem_system *sys, int p, int tensor, void *vals, voi
   }
   if(st == GFEM_OK && sys->n_chunks > 0) {
     // the slot count changes the order of the run sums: the default (4)
-    // stays unless the other is clearly faster, so the choice, and with it
+    // stays unless another is clearly faster, so the choice, and with it
     // the rounding, is stable from run to run
-    float t[2] = {0.f, 0.f};
-    const int cand[2] = {4, 2};
-    for(int c = 0; c < 2 && st == GFEM_OK; ++c) {
+    float t[3] = {0.f, 0.f, 0.f};
+    const int cand[3] = {4, 2, 8};
+    for(int c = 0; c < 3 && st == GFEM_OK; ++c) {
       sys->gather_slots[p] = cand[c];
       const hipError_t e = timed([&]() { return launch_gather(*sys, p, tensor, vals, s); }, t[c]);
       if(e != hipSuccess) st = hip_fail(ctx, e, "autotune gather");
     }
-    sys->gather_slots[p] = (st == GFEM_OK && t[1] < 0.95f * t[0]) ? 2 : 4;
+    int pick = 0;
+    for(int c = 1; c < 3; ++c)
+      if(st == GFEM_OK && t[c] < 0.95f * t[0] && t[c] < t[pick]) pick = c;
+    sys->gather_slots[p] = cand[pick];
   }
   (void)rhs;
   (void)hipEventDestroy(a);

@@ -932,13 +932,13 @@ __device__ __forceinline__ void gather_runs(int t, const int *h, const bool *in,
 
 // S slots per thread, kChunk / S threads per block
 template <bool CPLX, bool MIXED, bool WIDE, int S>
-__global__ void __launch_bounds__(kChunk / S, 8)
+__global__ void __launch_bounds__(kChunk / S, S == 8 ? 5 : 8)
   k_gather(const GroupTable *__restrict__ gtp, const uint32_t *__restrict__ recs, int64_t nchunks, int tensor,
            void *__restrict__ out)
 {
   using R = typename std::conditional<MIXED, float, double>::type;
   constexpr int B = kChunk / S;
-  static_assert(B >= kChunkElems && (S == 2 || S == 4), "one list entry per thread at most");
+  static_assert(S == 2 || S == 4 || S == 8, "2, 4 or 8 slots per thread");
   constexpr int kStride = WIDE ? kRecWide : kRecNarrow;
   constexpr int kList = kStride - kChunkElems;  // element-list region of a record
   using R2 = typename std::conditional<MIXED, float2, double2>::type;
@@ -961,10 +961,16 @@ __global__ void __launch_bounds__(kChunk / S, 8)
   const uint32_t *rec = recs + c * kStride;
   const uint32_t ns = rec[0], clen = rec[1], nel = rec[2];
   uint32_t w[S];
-  if(WIDE) {
-    if(S == 4) {
+  if constexpr(WIDE) {
+    if constexpr(S == 8) {
+      const uint4 p = reinterpret_cast<const uint4 *>(rec + kRecHead)[2 * t];
+      const uint4 q = reinterpret_cast<const uint4 *>(rec + kRecHead)[2 * t + 1];
+      w[0] = p.x; w[1] = p.y; w[2] = p.z; w[3] = p.w;
+      w[4] = q.x; w[5] = q.y; w[6] = q.z; w[7] = q.w;
+    }
+    else if constexpr(S == 4) {
       const uint4 p = reinterpret_cast<const uint4 *>(rec + kRecHead)[t];
-      w[0] = p.x; w[1] = p.y; w[2 % S] = p.z; w[3 % S] = p.w;
+      w[0] = p.x; w[1] = p.y; w[2] = p.z; w[3] = p.w;
     }
     else {
       const uint2 p = reinterpret_cast<const uint2 *>(rec + kRecHead)[t];
@@ -972,9 +978,14 @@ __global__ void __launch_bounds__(kChunk / S, 8)
     }
   }
   else {
-    if(S == 4) {
+    if constexpr(S == 8) {
+      const uint4 p = reinterpret_cast<const uint4 *>(rec + kRecHead)[t];
+      w[0] = p.x & 0xFFFFu; w[1] = p.x >> 16; w[2] = p.y & 0xFFFFu; w[3] = p.y >> 16;
+      w[4] = p.z & 0xFFFFu; w[5] = p.z >> 16; w[6] = p.w & 0xFFFFu; w[7] = p.w >> 16;
+    }
+    else if constexpr(S == 4) {
       const uint2 p = reinterpret_cast<const uint2 *>(rec + kRecHead)[t];
-      w[0] = p.x & 0xFFFFu; w[1] = p.x >> 16; w[2 % S] = p.y & 0xFFFFu; w[3 % S] = p.y >> 16;
+      w[0] = p.x & 0xFFFFu; w[1] = p.x >> 16; w[2] = p.y & 0xFFFFu; w[3] = p.y >> 16;
     }
     else {
       const uint32_t p = rec[kRecHead + t];
@@ -982,7 +993,6 @@ __global__ void __launch_bounds__(kChunk / S, 8)
       w[1] = p >> 16;
     }
   }
-  const uint32_t ev = t < kChunkElems ? rec[kList + t] : 0u;
   const int logP = gt.logP;
   const uint32_t pmask = (1u << logP) - 1u;
   const int ng = gt.ng;
@@ -997,10 +1007,9 @@ __global__ void __launch_bounds__(kChunk / S, 8)
   }
   __syncthreads();
   // chunk elements: entry j of the list sits in record slot kStride - nel + j
-  const int j = t - (kChunkElems - (int)nel);
-  if(t < kChunkElems && j >= 0) {
-    const int g = (int)(ev >> 27);
-    s_eg[j] = (uint8_t)g;
+  for(int j = t; j < (int)nel; j += B) {
+    const uint32_t ev = rec[kStride - (int)nel + j];
+    s_eg[j] = (uint8_t)(ev >> 27);
     s_el[j] = ev & 0x7FFFFFFu;
   }
   __syncthreads();
@@ -1359,13 +1368,17 @@ hipError_t launch_gather(const gfem_system &sys, int prec, int tensor, void *out
   hipLaunchKernelGGL((k_gather<C, M, W, S>), grid, dim3(kChunk / S), 0, s, gt, sys.d_rec, sys.n_chunks, tensor, out)
 #define GFEM_GATHER(C, M)                                                                                    \
   if(sys.rec_wide) {                                                                                         \
-    if(sys.gather_slots[prec] == 4)                                                                          \
+    if(sys.gather_slots[prec] == 8)                                                                          \
+      GFEM_GATHER_S(C, M, true, 8);                                                                          \
+    else if(sys.gather_slots[prec] == 4)                                                                     \
       GFEM_GATHER_S(C, M, true, 4);                                                                          \
     else                                                                                                     \
       GFEM_GATHER_S(C, M, true, 2);                                                                          \
   }                                                                                                          \
   else {                                                                                                     \
-    if(sys.gather_slots[prec] == 4)                                                                          \
+    if(sys.gather_slots[prec] == 8)                                                                          \
+      GFEM_GATHER_S(C, M, false, 8);                                                                         \
+    else if(sys.gather_slots[prec] == 4)                                                                     \
       GFEM_GATHER_S(C, M, false, 4);                                                                         \
     else                                                                                                     \
       GFEM_GATHER_S(C, M, false, 2);                                                                         \

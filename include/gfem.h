@@ -362,8 +362,10 @@ gfem_status gfem_last_kernel_times(const gfem_system *sys, int max, const char *
 
 /* Tile choices in use for `prec` (picked by timing the candidates on the
  * system at its first assembly / solve unless GFEM_AUTOTUNE=0): gather slots
- * per thread (2 or 4), linear-term elements per block ~ (256 << shift) / Q,
- * solver SpMV lanes per row (4, 8 or 16).  No counterpart in the reference. */
+ * per thread (4, 2 or 8: 128-, 256- or 64-thread blocks per chunk),
+ * linear-term block shift (compiled kernels: 64 << shift threads, one per
+ * element; interpreted: ~(256 << shift) / Q elements per block), solver
+ * SpMV lanes per row (4, 8 or 16).  No counterpart in the reference. */
 gfem_status gfem_tuning(const gfem_system *sys, gfem_precision prec, int *gather_slots, int *linear_eb_shift,
                         int *spmv_lanes);
 

@@ -162,24 +162,28 @@ def test_compiled_coefficients_match_interpreter(monkeypatch):
 
 
 
+@pytest.mark.parametrize("slots", [2, 8])
 @pytest.mark.parametrize("case", ["helmholtz3d_p2", "poisson2d_p3", "highorder3d_p4"])
 @pytest.mark.parametrize("precision,rtol", [("fp64", FP64_RTOL), ("mixed", MIXED_RTOL)])
-def test_gather_two_slots_per_thread(monkeypatch, case, precision, rtol):
-    """The gather with 2 slots per thread (256-thread blocks) assembles the
-    same matrix as the default 4 (128-thread blocks), to the same bars
-    (16-bit words for P2/P3 triangles, 32-bit words for P4 tetrahedra)."""
+def test_gather_slots_per_thread(monkeypatch, case, precision, rtol, slots):
+    """The gather with 2 or 8 slots per thread (256- or 64-thread blocks)
+    assembles the same matrix as the default 4 (128-thread blocks), to the
+    same bars (16-bit words for P2/P3 triangles, 32-bit words for P4
+    tetrahedra)."""
     from gfem import workloads as W
-    print(f"case gather 2 slots {case} {precision}", flush=True)
+    print(f"case gather {slots} slots {case} {precision}", flush=True)
     spec = _cases()[case]()
+    monkeypatch.setenv("GFEM_GATHER_SLOTS", "4")
     F1, P = _pair(spec)
     F1.assemble(precision=precision)
-    monkeypatch.setenv("GFEM_GATHER_SLOTS", "2")
+    monkeypatch.setenv("GFEM_GATHER_SLOTS", str(slots))
     F2 = W.build(spec)
     F2.pre()
     F2.assemble(precision=precision)
-    assert_close(F2.values(), P.values(), rtol, "2-slot A")
-    assert_close(F2.rhs(0), P.rhs(0), rtol, "2-slot b")
-    assert_close(F2.values(), F1.values(), 1e-14 if precision == "fp64" else rtol, "2 vs 4 slots")
+    assert F2.tuning(precision)["gather_slots"] == slots
+    assert_close(F2.values(), P.values(), rtol, f"{slots}-slot A")
+    assert_close(F2.rhs(0), P.rhs(0), rtol, f"{slots}-slot b")
+    assert_close(F2.values(), F1.values(), 1e-14 if precision == "fp64" else rtol, f"{slots} vs 4 slots")
 
 
 def test_systems_on_two_meshes_assemble_in_any_order():
@@ -249,7 +253,7 @@ def test_autotuned_assembly_matches_untuned(monkeypatch):
     F1.assemble()  # FP64 last: values() below are the FP64 assembly
     for prec in ("fp64", "mixed"):
         t = F1.tuning(prec)
-        assert t["gather_slots"] in (2, 4) and t["linear_eb_shift"] in (1, 2, 3), t
+        assert t["gather_slots"] in (2, 4, 8) and t["linear_eb_shift"] in (1, 2, 3), t
     monkeypatch.setenv("GFEM_AUTOTUNE", "0")
     F2 = W.build(spec)
     F2.pre()
