@@ -35,7 +35,8 @@ namespace {
 constexpr int kLanes = 8;                       // lanes per CSR row
 constexpr int kRowsPerBlock = 256 / kLanes;      // 32 rows per 256-thread block
 constexpr int kElemsPerBlock = 256;              // vector kernels: one element per thread
-constexpr int kRed = 64;                         // reduction blocks per request
+constexpr int kDotSpan = 4096;                   // rows per reduction block (at least)
+constexpr int kPartMax = 256;                    // reduction blocks per request (at most)
 constexpr int kCheck = 5;                        // iterations between convergence checks
 
 // device view of one request slot
@@ -49,8 +50,13 @@ struct BItem {
   double2 *x, *r;
   float2 *w, *V;     // V: (m+1) rows of n
   double2 *st;       // H[(m+1) m] | cs[m] | sn[m] | g[m+1] | est | y[m+1] | nrm[2]
-  double2 *part;     // (m+2) rows of kRed partial sums
+  double2 *part;     // (m+2) rows of kPartMax partial sums
+  int nparts;        // reduction blocks of this request (a function of n only)
 };
+
+// rows per reduction block of a request of n rows: kDotSpan, or more so that
+// a request has at most kPartMax blocks
+inline int64_t dot_span(int64_t n) { return std::max<int64_t>(kDotSpan, (n + kPartMax - 1) / kPartMax); }
 
 __device__ __forceinline__ double2 zmul(double2 a, double2 b)
 {
@@ -124,18 +130,21 @@ __global__ void __launch_bounds__(256) bk_residual(const BItem *__restrict__ ite
   if(l == 0) I.r[row] = make_double2(-I.b[row].x - s.x, -I.b[row].y - s.y);
 }
 
-// Partial inner products, grid (kRed, slots).  mode:
+// Partial inner products, one block per (request, row range) of the dot map
+// (slot, part, first row, end row): each request has nparts blocks in
+// proportion to its size, so large and small requests finish together.
+// mode:
 //   0: conj(V_k) . w, k <= j     1: w . w     2: r . r     3: b . b
 // Mode 0 takes the basis vectors kGroup at a time, so w is read once per
 // group.
 constexpr int kGroup = 8;
 
-__global__ void __launch_bounds__(256) bk_dots(const BItem *__restrict__ items, const int *__restrict__ act, int mode,
-                                               int j)
+__global__ void __launch_bounds__(256) bk_dots(const BItem *__restrict__ items, const int *__restrict__ act,
+                                               const int4 *__restrict__ dblk, int mode, int j)
 {
-  const int it = blockIdx.y;
-  if(!act[it]) return;
-  const BItem &I = items[it];
+  const int4 db = dblk[blockIdx.x];
+  if(!act[db.x]) return;
+  const BItem &I = items[db.x];
   const int nk = mode == 0 ? j + 1 : 1;
   __shared__ double2 s_red[4][kGroup];
   for(int k0 = 0; k0 < nk; k0 += kGroup) {
@@ -143,7 +152,7 @@ __global__ void __launch_bounds__(256) bk_dots(const BItem *__restrict__ items, 
     double2 acc[kGroup];
 #pragma unroll
     for(int q = 0; q < kGroup; ++q) acc[q] = make_double2(0., 0.);
-    for(int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < I.n; i += (int64_t)gridDim.x * blockDim.x) {
+    for(int64_t i = (int64_t)db.z + threadIdx.x; i < (int64_t)db.w; i += blockDim.x) {
       if(mode == 0) {
         const double2 wv = f2d(I.w[i]);
 #pragma unroll
@@ -176,7 +185,7 @@ __global__ void __launch_bounds__(256) bk_dots(const BItem *__restrict__ items, 
         t.x += s_red[w][threadIdx.x].x;
         t.y += s_red[w][threadIdx.x].y;
       }
-      I.part[(int64_t)(k0 + threadIdx.x) * kRed + blockIdx.x] = t;
+      I.part[(int64_t)(k0 + threadIdx.x) * kPartMax + db.y] = t;
     }
     __syncthreads();
   }
@@ -193,7 +202,12 @@ __global__ void __launch_bounds__(64) bk_dots_final(const BItem *__restrict__ it
   const int nk = mode == 0 ? j + 1 : 1;
   if(k >= nk) return;
   const BItem &I = items[it];
-  double2 s = I.part[(int64_t)k * kRed + threadIdx.x];
+  double2 s = make_double2(0., 0.);
+  for(int p = threadIdx.x; p < I.nparts; p += 64) {
+    const double2 v = I.part[(int64_t)k * kPartMax + p];
+    s.x += v.x;
+    s.y += v.y;
+  }
 #pragma unroll
   for(int d = 32; d >= 1; d >>= 1) {
     s.x += __shfl_xor(s.x, d, 64);
@@ -453,7 +467,7 @@ size_t request_bytes(const gfem_system *sys, int m)
   const size_t n = (size_t)sys->n, nnz = (size_t)std::max<int64_t>(sys->nnz, 1);
   const size_t st = (size_t)(m + 1) * m + 3 * (size_t)m + 2 + (size_t)(m + 1) + 2;
   return align256(16 * nnz) + align256(8 * nnz) + align256(16 * n * (size_t)sys->nrhs) + 2 * align256(16 * n) +
-         align256(8 * n) + align256(8 * n * (size_t)(m + 1)) + align256(16 * st) + align256(16 * (size_t)(m + 2) * kRed);
+         align256(8 * n) + align256(8 * n * (size_t)(m + 1)) + align256(16 * st) + align256(16 * (size_t)(m + 2) * kPartMax);
 }
 
 }  // namespace
@@ -480,7 +494,8 @@ struct gfem_batch {
   BItem *d_items = nullptr;
   int *d_act = nullptr, *d_live = nullptr, *d_kcnt = nullptr;
   int2 *d_rblk = nullptr, *d_vblk = nullptr;
-  int64_t cap_rblk = 0, cap_vblk = 0;
+  int4 *d_dblk = nullptr;
+  int64_t cap_rblk = 0, cap_vblk = 0, cap_dblk = 0;
   double *d_scal = nullptr;
   int *h_flags = nullptr;   // pinned: act | live
   double *h_scal = nullptr; // pinned: 3 per slot
@@ -488,7 +503,7 @@ struct gfem_batch {
   std::vector<int64_t> done;          // completed, not yet fetched
   std::map<int64_t, int> where;       // id -> slot
   bool maps_dirty = true;
-  int64_t n_rblk = 0, n_vblk = 0;
+  int64_t n_rblk = 0, n_vblk = 0, n_dblk = 0;
 };
 
 namespace {
@@ -507,15 +522,26 @@ gfem_status upload_items(gfem_batch *B, hipStream_t s)
 gfem_status build_maps(gfem_batch *B, hipStream_t s)
 {
   std::vector<int2> rb, vb;
+  std::vector<int4> db;
   for(size_t i = 0; i < B->slots.size(); ++i) {
     const auto &S = B->slots[i];
     if(!S.live) continue;
     for(int64_t r = 0; r < S.item.n; r += kRowsPerBlock) rb.push_back(make_int2((int)i, (int)r));
     for(int64_t r = 0; r < S.item.n; r += kElemsPerBlock) vb.push_back(make_int2((int)i, (int)r));
+    const int64_t span = dot_span(S.item.n);
+    for(int64_t r = 0, p = 0; r < S.item.n; r += span, ++p)
+      db.push_back(make_int4((int)i, (int)p, (int)r, (int)std::min<int64_t>(S.item.n, r + span)));
   }
   if(rb.empty()) rb.push_back(make_int2(0, 0));
   if(vb.empty()) vb.push_back(make_int2(0, 0));
+  if(db.empty()) db.push_back(make_int4(0, 0, 0, 0));
   gfem_ctx *ctx = B->ctx;
+  if((int64_t)db.size() > B->cap_dblk) {
+    dfree(B->d_dblk);
+    B->cap_dblk = (int64_t)db.size() * 2;
+    GFEM_HIP(ctx, dalloc(&B->d_dblk, (size_t)B->cap_dblk));
+  }
+  GFEM_HIP(ctx, hipMemcpyAsync(B->d_dblk, db.data(), sizeof(int4) * db.size(), hipMemcpyHostToDevice, s));
   if((int64_t)rb.size() > B->cap_rblk) {
     dfree(B->d_rblk);
     B->cap_rblk = (int64_t)rb.size() * 2;
@@ -531,6 +557,7 @@ gfem_status build_maps(gfem_batch *B, hipStream_t s)
   GFEM_HIP(ctx, hipStreamSynchronize(s));  // the sources are local vectors
   B->n_rblk = (int64_t)rb.size();
   B->n_vblk = (int64_t)vb.size();
+  B->n_dblk = (int64_t)db.size();
   B->maps_dirty = false;
   return GFEM_OK;
 }
@@ -568,10 +595,10 @@ gfem_status residual_and_prepare(gfem_batch *B, std::vector<int> &act, hipStream
   gfem_status st = push_flags(B, act, s);
   if(st != GFEM_OK) return st;
   hipLaunchKernelGGL(bk_residual, dim3((unsigned)B->n_rblk), dim3(256), 0, s, B->d_items, B->d_act, B->d_rblk);
-  hipLaunchKernelGGL(bk_dots, dim3(kRed, (unsigned)ns), dim3(256), 0, s, B->d_items, B->d_act, 2, 0);
+  hipLaunchKernelGGL(bk_dots, dim3((unsigned)B->n_dblk), dim3(256), 0, s, B->d_items, B->d_act, B->d_dblk, 2, 0);
   hipLaunchKernelGGL(bk_dots_final, dim3(1, (unsigned)ns), dim3(64), 0, s, B->d_items, B->d_act, m, 2, 0, 0);
   if(first) {
-    hipLaunchKernelGGL(bk_dots, dim3(kRed, (unsigned)ns), dim3(256), 0, s, B->d_items, B->d_act, 3, 0);
+    hipLaunchKernelGGL(bk_dots, dim3((unsigned)B->n_dblk), dim3(256), 0, s, B->d_items, B->d_act, B->d_dblk, 3, 0);
     hipLaunchKernelGGL(bk_dots_final, dim3(1, (unsigned)ns), dim3(64), 0, s, B->d_items, B->d_act, m, 3, 0, 0);
   }
   GFEM_HIP(ctx, hipGetLastError());
@@ -656,6 +683,7 @@ gfem_status gfem_batch_destroy(gfem_batch *B)
   dfree(B->d_scal);
   dfree(B->d_rblk);
   dfree(B->d_vblk);
+  dfree(B->d_dblk);
   if(B->h_flags) (void)hipHostFree(B->h_flags);
   if(B->h_scal) (void)hipHostFree(B->h_scal);
   delete B;
@@ -716,7 +744,8 @@ gfem_status gfem_batch_submit(gfem_batch *B, gfem_system *sys, double tol, int m
   I.w = reinterpret_cast<float2 *>(take(8 * n));
   I.V = reinterpret_cast<float2 *>(take(8 * n * (size_t)(m + 1)));
   I.st = reinterpret_cast<double2 *>(take(16 * ((size_t)(m + 1) * m + 3 * (size_t)m + 2 + (size_t)(m + 1) + 2)));
-  I.part = reinterpret_cast<double2 *>(take(16 * (size_t)(m + 2) * kRed));
+  I.part = reinterpret_cast<double2 *>(take(16 * (size_t)(m + 2) * kPartMax));
+  I.nparts = (int)((sys->n + dot_span(sys->n) - 1) / dot_span(sys->n));
   B->where[S.id] = slot;
   *id = S.id;
   // assemble this request's system with the system's current parameters
@@ -765,20 +794,20 @@ gfem_status gfem_batch_step(gfem_batch *B, void *stream, int *n_live)
   }
   st = push_flags(B, act, s);
   if(st != GFEM_OK) return st;
-  const dim3 red(kRed, (unsigned)ns), fin((unsigned)(m + 2), (unsigned)ns);
+  const dim3 red((unsigned)B->n_dblk), fin((unsigned)(m + 2), (unsigned)ns);
   for(int j = 0; j < m; ++j) {
     int running = 0;
     for(int i = 0; i < ns; ++i) running += act[i];
     if(running == 0) break;
     hipLaunchKernelGGL(bk_spmv32, dim3((unsigned)B->n_rblk), dim3(256), 0, s, B->d_items, B->d_act, B->d_rblk, j);
     for(int pass = 0; pass < 2; ++pass) {
-      hipLaunchKernelGGL(bk_dots, red, dim3(256), 0, s, B->d_items, B->d_act, 0, j);
+      hipLaunchKernelGGL(bk_dots, red, dim3(256), 0, s, B->d_items, B->d_act, B->d_dblk, 0, j);
       hipLaunchKernelGGL(bk_dots_final, fin, dim3(64), 0, s, B->d_items, B->d_act, m, 0, j, pass);
       hipLaunchKernelGGL(bk_orth, dim3((unsigned)B->n_vblk), dim3(256), 0, s, B->d_items, B->d_act, B->d_vblk, m,
                          j, pass);
     }
     hipLaunchKernelGGL(bk_add_y, dim3((unsigned)ns), dim3(64), 0, s, B->d_items, B->d_act, m, j);
-    hipLaunchKernelGGL(bk_dots, red, dim3(256), 0, s, B->d_items, B->d_act, 1, j);
+    hipLaunchKernelGGL(bk_dots, red, dim3(256), 0, s, B->d_items, B->d_act, B->d_dblk, 1, j);
     hipLaunchKernelGGL(bk_dots_final, fin, dim3(64), 0, s, B->d_items, B->d_act, m, 1, j, 0);
     hipLaunchKernelGGL(bk_scale, dim3((unsigned)B->n_vblk), dim3(256), 0, s, B->d_items, B->d_act, B->d_vblk, m, j);
     hipLaunchKernelGGL(bk_givens, dim3((unsigned)ns), dim3(64), 0, s, B->d_items, B->d_act, B->d_kcnt, m, j);
