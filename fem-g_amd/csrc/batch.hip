@@ -91,8 +91,9 @@ __device__ __forceinline__ double2 lanes_sum(double2 v)
 
 // w = A32 V_j (FP32 products, lanes combined in a fixed butterfly)
 __global__ void __launch_bounds__(256) bk_spmv32(const BItem *__restrict__ items, const int *__restrict__ act,
-                                                 const int2 *__restrict__ rblk, int j)
+                                                 const int2 *__restrict__ rblk, int j, const int *__restrict__ dj)
 {
+  if(dj) j = *dj;
   const int2 bb = rblk[blockIdx.x];
   if(!act[bb.x]) return;
   const BItem &I = items[bb.x];
@@ -140,8 +141,10 @@ __global__ void __launch_bounds__(256) bk_residual(const BItem *__restrict__ ite
 constexpr int kGroup = 8;
 
 __global__ void __launch_bounds__(256) bk_dots(const BItem *__restrict__ items, const int *__restrict__ act,
-                                               const int4 *__restrict__ dblk, int mode, int j)
+                                               const int4 *__restrict__ dblk, int mode, int j,
+                                               const int *__restrict__ dj)
 {
+  if(dj) j = *dj;
   const int4 db = dblk[blockIdx.x];
   if(!act[db.x]) return;
   const BItem &I = items[db.x];
@@ -195,8 +198,9 @@ __global__ void __launch_bounds__(256) bk_dots(const BItem *__restrict__ items, 
 //   0: H column j (first pass, pass = 0) or y (second pass, pass = 1)
 //   1: H[j+1, j] = |w|^2     2: nrm[0] = |r|^2     3: nrm[1] = |b|^2
 __global__ void __launch_bounds__(64) bk_dots_final(const BItem *__restrict__ items, const int *__restrict__ act,
-                                                    int m, int mode, int j, int pass)
+                                                    int m, int mode, int j, int pass, const int *__restrict__ dj)
 {
+  if(dj) j = *dj;
   const int it = blockIdx.y, k = blockIdx.x;
   if(!act[it]) return;
   const int nk = mode == 0 ? j + 1 : 1;
@@ -226,8 +230,10 @@ __global__ void __launch_bounds__(64) bk_dots_final(const BItem *__restrict__ it
 
 // w -= sum_{k<=j} V_k h_k with h = H column j (pass 0) or y (pass 1)
 __global__ void __launch_bounds__(256) bk_orth(const BItem *__restrict__ items, const int *__restrict__ act,
-                                               const int2 *__restrict__ vblk, int m, int j, int pass)
+                                               const int2 *__restrict__ vblk, int m, int j, int pass,
+                                               const int *__restrict__ dj)
 {
+  if(dj) j = *dj;
   const int2 bb = vblk[blockIdx.x];
   if(!act[bb.x]) return;
   const BItem &I = items[bb.x];
@@ -245,8 +251,10 @@ __global__ void __launch_bounds__(256) bk_orth(const BItem *__restrict__ items, 
 }
 
 // H column j += y (second Gram-Schmidt pass), grid (slots)
-__global__ void bk_add_y(const BItem *__restrict__ items, const int *__restrict__ act, int m, int j)
+__global__ void bk_add_y(const BItem *__restrict__ items, const int *__restrict__ act, int m, int j,
+                         const int *__restrict__ dj)
 {
+  if(dj) j = *dj;
   const int it = blockIdx.x;
   if(!act[it]) return;
   const St o{m};
@@ -259,8 +267,9 @@ __global__ void bk_add_y(const BItem *__restrict__ items, const int *__restrict_
 
 // V_{j+1} = w / |w| (pass j >= 0) or V_0 = r / |r| (j < 0)
 __global__ void __launch_bounds__(256) bk_scale(const BItem *__restrict__ items, const int *__restrict__ act,
-                                                const int2 *__restrict__ vblk, int m, int j)
+                                                const int2 *__restrict__ vblk, int m, int j, const int *__restrict__ dj)
 {
+  if(dj) j = *dj;
   const int2 bb = vblk[blockIdx.x];
   if(!act[bb.x]) return;
   const BItem &I = items[bb.x];
@@ -289,8 +298,9 @@ __global__ void bk_cycle_start(const BItem *__restrict__ items, const int *__res
 
 // Givens rotations of column j, residual estimate |g_{j+1}|; grid (slots)
 __global__ void bk_givens(const BItem *__restrict__ items, const int *__restrict__ act, int *__restrict__ kcnt, int m,
-                          int j)
+                          int j, const int *__restrict__ dj)
 {
+  if(dj) j = *dj;
   const int it = blockIdx.x;
   if(!act[it] || threadIdx.x != 0) return;
   const St o{m};
@@ -394,6 +404,17 @@ __global__ void bk_round(int64_t nnz, const double2 *__restrict__ a, float2 *__r
 {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if(i < nnz) b[i] = make_float2((float)a[i].x, (float)a[i].y);
+}
+
+// Krylov index of a captured iteration, kept on the device
+__global__ void bk_set_j(int *__restrict__ dj, int v)
+{
+  if(threadIdx.x == 0) *dj = v;
+}
+
+__global__ void bk_next_j(int *__restrict__ dj)
+{
+  if(threadIdx.x == 0) *dj += 1;
 }
 
 __global__ void bk_zero(int64_t n, double2 *__restrict__ x)
@@ -504,6 +525,13 @@ struct gfem_batch {
   std::map<int64_t, int> where;       // id -> slot
   bool maps_dirty = true;
   int64_t n_rblk = 0, n_vblk = 0, n_dblk = 0;
+  // Krylov iterations as HIP graphs (1 and kCheck iterations), captured on
+  // cap_stream with the Krylov index read from d_j; valid until the block
+  // maps change.  GFEM_BATCH_GRAPH=0 launches the kernels one by one.
+  bool use_graph = true;
+  int *d_j = nullptr;
+  hipStream_t cap_stream = nullptr;
+  hipGraphExec_t g1 = nullptr, g5 = nullptr;
 };
 
 namespace {
@@ -516,6 +544,13 @@ gfem_status upload_items(gfem_batch *B, hipStream_t s)
   // the source is a local vector: wait for the copy
   GFEM_HIP(B->ctx, hipStreamSynchronize(s));
   return GFEM_OK;
+}
+
+void drop_graphs(gfem_batch *B)
+{
+  if(B->g1) (void)hipGraphExecDestroy(B->g1);
+  if(B->g5) (void)hipGraphExecDestroy(B->g5);
+  B->g1 = B->g5 = nullptr;
 }
 
 // block maps over the live slots
@@ -559,6 +594,7 @@ gfem_status build_maps(gfem_batch *B, hipStream_t s)
   B->n_vblk = (int64_t)vb.size();
   B->n_dblk = (int64_t)db.size();
   B->maps_dirty = false;
+  drop_graphs(B);
   return GFEM_OK;
 }
 
@@ -575,6 +611,46 @@ gfem_status push_flags(gfem_batch *B, const std::vector<int> &act, hipStream_t s
   GFEM_HIP(B->ctx, hipMemcpyAsync(B->d_act, B->h_flags, sizeof(int) * ns, hipMemcpyHostToDevice, s));
   GFEM_HIP(B->ctx, hipMemcpyAsync(B->d_live, B->h_flags + ns, sizeof(int) * ns, hipMemcpyHostToDevice, s));
   return GFEM_OK;
+}
+
+// One Krylov iteration j of every active request: w = A V_j, two
+// Gram-Schmidt passes, |w|, V_{j+1}, Givens rotation.  With dj the index is
+// read from device memory (captured graphs), else j is used.
+void launch_iteration(gfem_batch *B, hipStream_t s, int j, const int *dj)
+{
+  const int ns = (int)B->slots.size(), m = B->m;
+  const dim3 red((unsigned)B->n_dblk), fin((unsigned)(m + 2), (unsigned)ns);
+  hipLaunchKernelGGL(bk_spmv32, dim3((unsigned)B->n_rblk), dim3(256), 0, s, B->d_items, B->d_act, B->d_rblk, j, dj);
+  for(int pass = 0; pass < 2; ++pass) {
+    hipLaunchKernelGGL(bk_dots, red, dim3(256), 0, s, B->d_items, B->d_act, B->d_dblk, 0, j, dj);
+    hipLaunchKernelGGL(bk_dots_final, fin, dim3(64), 0, s, B->d_items, B->d_act, m, 0, j, pass, dj);
+    hipLaunchKernelGGL(bk_orth, dim3((unsigned)B->n_vblk), dim3(256), 0, s, B->d_items, B->d_act, B->d_vblk, m, j,
+                       pass, dj);
+  }
+  hipLaunchKernelGGL(bk_add_y, dim3((unsigned)ns), dim3(64), 0, s, B->d_items, B->d_act, m, j, dj);
+  hipLaunchKernelGGL(bk_dots, red, dim3(256), 0, s, B->d_items, B->d_act, B->d_dblk, 1, j, dj);
+  hipLaunchKernelGGL(bk_dots_final, fin, dim3(64), 0, s, B->d_items, B->d_act, m, 1, j, 0, dj);
+  hipLaunchKernelGGL(bk_scale, dim3((unsigned)B->n_vblk), dim3(256), 0, s, B->d_items, B->d_act, B->d_vblk, m, j, dj);
+  hipLaunchKernelGGL(bk_givens, dim3((unsigned)ns), dim3(64), 0, s, B->d_items, B->d_act, B->d_kcnt, m, j, dj);
+}
+
+// a graph of `iters` iterations, each followed by d_j += 1 (kernels only)
+hipError_t capture(gfem_batch *B, int iters, hipGraphExec_t *out)
+{
+  hipStream_t cs = B->cap_stream;
+  hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+  if(e != hipSuccess) return e;
+  for(int it = 0; it < iters; ++it) {
+    launch_iteration(B, cs, 0, B->d_j);
+    hipLaunchKernelGGL(bk_next_j, dim3(1), dim3(64), 0, cs, B->d_j);
+  }
+  hipGraph_t g = nullptr;
+  const hipError_t le = hipGetLastError();
+  e = hipStreamEndCapture(cs, &g);
+  if(le != hipSuccess) e = le;
+  if(e == hipSuccess) e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+  if(g) (void)hipGraphDestroy(g);
+  return e;
 }
 
 gfem_status collect(gfem_batch *B, hipStream_t s)
@@ -595,11 +671,14 @@ gfem_status residual_and_prepare(gfem_batch *B, std::vector<int> &act, hipStream
   gfem_status st = push_flags(B, act, s);
   if(st != GFEM_OK) return st;
   hipLaunchKernelGGL(bk_residual, dim3((unsigned)B->n_rblk), dim3(256), 0, s, B->d_items, B->d_act, B->d_rblk);
-  hipLaunchKernelGGL(bk_dots, dim3((unsigned)B->n_dblk), dim3(256), 0, s, B->d_items, B->d_act, B->d_dblk, 2, 0);
-  hipLaunchKernelGGL(bk_dots_final, dim3(1, (unsigned)ns), dim3(64), 0, s, B->d_items, B->d_act, m, 2, 0, 0);
+  hipLaunchKernelGGL(bk_dots, dim3((unsigned)B->n_dblk), dim3(256), 0, s, B->d_items, B->d_act, B->d_dblk, 2, 0,
+                     nullptr);
+  hipLaunchKernelGGL(bk_dots_final, dim3(1, (unsigned)ns), dim3(64), 0, s, B->d_items, B->d_act, m, 2, 0, 0, nullptr);
   if(first) {
-    hipLaunchKernelGGL(bk_dots, dim3((unsigned)B->n_dblk), dim3(256), 0, s, B->d_items, B->d_act, B->d_dblk, 3, 0);
-    hipLaunchKernelGGL(bk_dots_final, dim3(1, (unsigned)ns), dim3(64), 0, s, B->d_items, B->d_act, m, 3, 0, 0);
+    hipLaunchKernelGGL(bk_dots, dim3((unsigned)B->n_dblk), dim3(256), 0, s, B->d_items, B->d_act, B->d_dblk, 3, 0,
+                       nullptr);
+    hipLaunchKernelGGL(bk_dots_final, dim3(1, (unsigned)ns), dim3(64), 0, s, B->d_items, B->d_act, m, 3, 0, 0,
+                       nullptr);
   }
   GFEM_HIP(ctx, hipGetLastError());
   st = collect(B, s);
@@ -627,7 +706,8 @@ gfem_status residual_and_prepare(gfem_batch *B, std::vector<int> &act, hipStream
   // V_0 = r / |r|, g = [|r|, 0 ...] for the slots that go on
   st = push_flags(B, act, s);
   if(st != GFEM_OK) return st;
-  hipLaunchKernelGGL(bk_scale, dim3((unsigned)B->n_vblk), dim3(256), 0, s, B->d_items, B->d_act, B->d_vblk, m, -1);
+  hipLaunchKernelGGL(bk_scale, dim3((unsigned)B->n_vblk), dim3(256), 0, s, B->d_items, B->d_act, B->d_vblk, m, -1,
+                     nullptr);
   hipLaunchKernelGGL(bk_cycle_start, dim3((unsigned)ns), dim3(64), 0, s, B->d_items, B->d_act, B->d_kcnt, m);
   GFEM_HIP(ctx, hipGetLastError());
   return GFEM_OK;
@@ -658,11 +738,14 @@ gfem_status gfem_batch_create(gfem_ctx *ctx, int restart, size_t hbm_bytes, int 
   B->slots.resize((size_t)max_items);
   bool ok = B->arena.init(hbm_bytes) && dalloc(&B->d_items, (size_t)max_items) == hipSuccess &&
             dalloc(&B->d_act, (size_t)max_items) == hipSuccess && dalloc(&B->d_live, (size_t)max_items) == hipSuccess &&
-            dalloc(&B->d_kcnt, (size_t)max_items) == hipSuccess &&
+            dalloc(&B->d_kcnt, (size_t)max_items) == hipSuccess && dalloc(&B->d_j, 1) == hipSuccess &&
+            hipStreamCreate(&B->cap_stream) == hipSuccess &&
             dalloc(&B->d_scal, (size_t)(3 * max_items)) == hipSuccess &&
             hipHostMalloc(&B->h_flags, sizeof(int) * 2 * (size_t)max_items) == hipSuccess &&
             hipHostMalloc(&B->h_scal, sizeof(double) * 3 * (size_t)max_items) == hipSuccess;
   if(ok) ok = hipMemset(B->d_items, 0, sizeof(BItem) * (size_t)max_items) == hipSuccess;
+  const char *ge = std::getenv("GFEM_BATCH_GRAPH");
+  B->use_graph = !(ge && ge[0] == '0');
   if(!ok) {
     gfem_batch_destroy(B);
     return fail(ctx, GFEM_ERR_OUT_OF_MEMORY, "batch_create: device memory");
@@ -684,6 +767,9 @@ gfem_status gfem_batch_destroy(gfem_batch *B)
   dfree(B->d_rblk);
   dfree(B->d_vblk);
   dfree(B->d_dblk);
+  drop_graphs(B);
+  dfree(B->d_j);
+  if(B->cap_stream) (void)hipStreamDestroy(B->cap_stream);
   if(B->h_flags) (void)hipHostFree(B->h_flags);
   if(B->h_scal) (void)hipHostFree(B->h_scal);
   delete B;
@@ -794,27 +880,30 @@ gfem_status gfem_batch_step(gfem_batch *B, void *stream, int *n_live)
   }
   st = push_flags(B, act, s);
   if(st != GFEM_OK) return st;
-  const dim3 red((unsigned)B->n_dblk), fin((unsigned)(m + 2), (unsigned)ns);
-  for(int j = 0; j < m; ++j) {
+  if(B->use_graph) hipLaunchKernelGGL(bk_set_j, dim3(1), dim3(64), 0, s, B->d_j, 0);
+  for(int j = 0; j < m;) {
     int running = 0;
     for(int i = 0; i < ns; ++i) running += act[i];
     if(running == 0) break;
-    hipLaunchKernelGGL(bk_spmv32, dim3((unsigned)B->n_rblk), dim3(256), 0, s, B->d_items, B->d_act, B->d_rblk, j);
-    for(int pass = 0; pass < 2; ++pass) {
-      hipLaunchKernelGGL(bk_dots, red, dim3(256), 0, s, B->d_items, B->d_act, B->d_dblk, 0, j);
-      hipLaunchKernelGGL(bk_dots_final, fin, dim3(64), 0, s, B->d_items, B->d_act, m, 0, j, pass);
-      hipLaunchKernelGGL(bk_orth, dim3((unsigned)B->n_vblk), dim3(256), 0, s, B->d_items, B->d_act, B->d_vblk, m,
-                         j, pass);
+    // iterations up to the next convergence check (every kCheck iterations)
+    const int seg = std::min(kCheck - j % kCheck, m - j);
+    if(B->use_graph) {
+      hipError_t e = hipSuccess;
+      if(seg == kCheck && !B->g5) e = capture(B, kCheck, &B->g5);
+      if(e == hipSuccess && seg != kCheck && !B->g1) e = capture(B, 1, &B->g1);
+      if(e != hipSuccess) return hip_fail(ctx, e, "batch: graph capture");
+      for(int r = 0; r < (seg == kCheck ? 1 : seg) && e == hipSuccess; ++r)
+        e = hipGraphLaunch(seg == kCheck ? B->g5 : B->g1, s);
+      if(e != hipSuccess) return hip_fail(ctx, e, "batch: graph launch");
     }
-    hipLaunchKernelGGL(bk_add_y, dim3((unsigned)ns), dim3(64), 0, s, B->d_items, B->d_act, m, j);
-    hipLaunchKernelGGL(bk_dots, red, dim3(256), 0, s, B->d_items, B->d_act, B->d_dblk, 1, j);
-    hipLaunchKernelGGL(bk_dots_final, fin, dim3(64), 0, s, B->d_items, B->d_act, m, 1, j, 0);
-    hipLaunchKernelGGL(bk_scale, dim3((unsigned)B->n_vblk), dim3(256), 0, s, B->d_items, B->d_act, B->d_vblk, m, j);
-    hipLaunchKernelGGL(bk_givens, dim3((unsigned)ns), dim3(64), 0, s, B->d_items, B->d_act, B->d_kcnt, m, j);
-    GFEM_HIP(ctx, hipGetLastError());
+    else {
+      for(int r = 0; r < seg; ++r) launch_iteration(B, s, j + r, nullptr);
+      GFEM_HIP(ctx, hipGetLastError());
+    }
     for(int i = 0; i < ns; ++i)
-      if(act[i]) B->slots[i].iters++;
-    if((j % kCheck) == kCheck - 1 && j < m - 1) {
+      if(act[i]) B->slots[i].iters += seg;
+    j += seg;
+    if(j % kCheck == 0 && j < m) {
       st = collect(B, s);
       if(st != GFEM_OK) return st;
       bool changed = false;

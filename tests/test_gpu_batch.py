@@ -129,3 +129,27 @@ def test_admission_respects_the_hbm_budget():
     # later arrivals waited for memory: admitted after earlier ones ended
     first_done = min(r["done"] for r in out)
     assert max(r["admitted"] for r in out) >= first_done
+
+
+def test_graph_and_eager_batches_agree(monkeypatch):
+    """The batched Krylov iterations replayed as HIP graphs (default) and
+    launched one kernel at a time (GFEM_BATCH_GRAPH=0) give bitwise equal
+    solutions and iteration counts.  A restart length of 23 runs both graphs
+    (5 iterations between convergence checks, then single iterations); each is
+    replayed many times with the host's checks and flag uploads between."""
+    from gfem import Server
+    print("case graph vs eager batch", flush=True)
+    meshes = [(5, 1), (4, 2)]
+    Fs = [_resident(m) for m in meshes]
+    reqs = [(0, 9.0, (0.5, 0.5, 0.5)), (1, 14.0, (0.4, 0.6, 0.5)), (0, 25.0, (0.3, 0.5, 0.6))]
+    out = {}
+    for mode in ("1", "0"):
+        monkeypatch.setenv("GFEM_BATCH_GRAPH", mode)
+        sv = Server(restart=23, hbm_bytes=1 << 30, max_requests=8)
+        for F in Fs:
+            sv.add_mesh(F)
+        out[mode] = {r["request"]: r for r in sv.run(_schedule(reqs, gap=0.001), keep_solutions=True)}
+    for i in range(len(reqs)):
+        assert out["1"][i]["converged"]
+        assert out["1"][i]["iterations"] == out["0"][i]["iterations"], i
+        assert np.array_equal(out["1"][i]["x"], out["0"][i]["x"]), i
