@@ -138,7 +138,7 @@ def serving(args):
         F = W.build(W.helmholtz3d(n=n, order=order, k=10.0))
         F.pre()
         Fs.append(F)
-    sv = Server(restart=50, hbm_bytes=args.serve_hbm_gb << 30, max_requests=256)
+    sv = Server(restart=30, hbm_bytes=args.serve_hbm_gb << 30, max_requests=256)
     for F in Fs:
         sv.add_mesh(F)
     log(f"serving: {len(Fs)} meshes resident ({time.perf_counter() - t0:.1f} s)")
@@ -174,7 +174,7 @@ def serving(args):
         for tag, f in s["params"]:
             F.set_term_coefficient(tag, f)
         F.assemble(download=False)
-        F.solve(tol=args.serve_tol, restart=50, max_iter=30000, download=False)
+        F.solve(tol=args.serve_tol, restart=30, max_iter=30000, download=False)
     dt = time.perf_counter() - t1
     res["one_at_a_time_solves_per_s"] = nb / dt
     res["one_at_a_time_sample"] = nb

@@ -51,6 +51,7 @@ struct BItem {
   float2 *w, *V;     // V: (m+1) rows of n
   double2 *st;       // H[(m+1) m] | cs[m] | sn[m] | g[m+1] | est | y[m+1] | nrm[2]
   double2 *part;     // (m+2) rows of kPartMax partial sums
+  const double2 *dinv;  // right preconditioner: inverse diagonal of A (ones without it)
   int nparts;        // reduction blocks of this request (a function of n only)
 };
 
@@ -380,13 +381,14 @@ __global__ void __launch_bounds__(256) bk_update_x(const BItem *__restrict__ ite
   if(i >= I.n) return;
   const St o{m};
   const int k = kcnt[bb.x];
-  double2 s = I.x[i];
+  double2 s = make_double2(0., 0.);
   for(int l = 0; l < k; ++l) {
     const double2 p = zmul(f2d(I.V[(int64_t)l * I.n + i]), I.st[o.y() + l]);
     s.x += p.x;
     s.y += p.y;
   }
-  I.x[i] = s;
+  const double2 d = zmul(I.dinv[i], s);  // x += D^-1 V y (right preconditioning)
+  I.x[i] = make_double2(I.x[i].x + d.x, I.x[i].y + d.y);
 }
 
 // per-slot scalars to one array: est, |r|^2, |b|^2
@@ -400,10 +402,36 @@ __global__ void bk_collect(const BItem *__restrict__ items, int nslots, int m, d
   out[3 * it + 2] = items[it].st[o.nrm() + 1].x;
 }
 
-__global__ void bk_round(int64_t nnz, const double2 *__restrict__ a, float2 *__restrict__ b)
+// Right scaling: the Krylov iteration runs on A D^-1 with D the 2-norms of
+// the rows of A (for the symmetric patterns assembled here, those of its
+// columns), so the FP32 copy of the matrix is A with its columns scaled by
+// D^-1 and the correction of a cycle is D^-1 V y.  Unlike the diagonal, the
+// norms stay away from zero on indefinite (Helmholtz) matrices.  The FP64
+// residual -b - A x of each cycle is unchanged: the solution and its
+// tolerance are the unscaled ones.
+__global__ void bk_diag_inv(int64_t n, const int64_t *__restrict__ ai, const int32_t *__restrict__ aj,
+                            const double2 *__restrict__ a, int scale, double2 *__restrict__ dinv)
+{
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(r >= n) return;
+  double d = 1.;
+  if(scale) {
+    double q = 0.;
+    for(int64_t k = ai[r]; k < ai[r + 1]; ++k) q += a[k].x * a[k].x + a[k].y * a[k].y;
+    if(q > 0.) d = 1. / sqrt(q);
+  }
+  (void)aj;
+  dinv[r] = make_double2(d, 0.);
+}
+
+// FP32 copy of A D^-1
+__global__ void bk_round_cols(int64_t nnz, const double2 *__restrict__ a, const int32_t *__restrict__ aj,
+                              const double2 *__restrict__ dinv, float2 *__restrict__ b)
 {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if(i < nnz) b[i] = make_float2((float)a[i].x, (float)a[i].y);
+  if(i >= nnz) return;
+  const double2 v = zmul(a[i], dinv[aj[i]]);
+  b[i] = make_float2((float)v.x, (float)v.y);
 }
 
 // Krylov index of a captured iteration, kept on the device
@@ -487,7 +515,7 @@ size_t request_bytes(const gfem_system *sys, int m)
 {
   const size_t n = (size_t)sys->n, nnz = (size_t)std::max<int64_t>(sys->nnz, 1);
   const size_t st = (size_t)(m + 1) * m + 3 * (size_t)m + 2 + (size_t)(m + 1) + 2;
-  return align256(16 * nnz) + align256(8 * nnz) + align256(16 * n * (size_t)sys->nrhs) + 2 * align256(16 * n) +
+  return align256(16 * nnz) + align256(8 * nnz) + align256(16 * n * (size_t)sys->nrhs) + 3 * align256(16 * n) +
          align256(8 * n) + align256(8 * n * (size_t)(m + 1)) + align256(16 * st) + align256(16 * (size_t)(m + 2) * kPartMax);
 }
 
@@ -529,6 +557,7 @@ struct gfem_batch {
   // cap_stream with the Krylov index read from d_j; valid until the block
   // maps change.  GFEM_BATCH_GRAPH=0 launches the kernels one by one.
   bool use_graph = true;
+  bool jacobi = true;  // right scaling by row norms (GFEM_KRYLOV_SCALE=0: none)
   int *d_j = nullptr;
   hipStream_t cap_stream = nullptr;
   hipGraphExec_t g1 = nullptr, g5 = nullptr;
@@ -746,6 +775,8 @@ gfem_status gfem_batch_create(gfem_ctx *ctx, int restart, size_t hbm_bytes, int 
   if(ok) ok = hipMemset(B->d_items, 0, sizeof(BItem) * (size_t)max_items) == hipSuccess;
   const char *ge = std::getenv("GFEM_BATCH_GRAPH");
   B->use_graph = !(ge && ge[0] == '0');
+  const char *je = std::getenv("GFEM_KRYLOV_SCALE");
+  B->jacobi = !(je && je[0] == '0');
   if(!ok) {
     gfem_batch_destroy(B);
     return fail(ctx, GFEM_ERR_OUT_OF_MEMORY, "batch_create: device memory");
@@ -827,6 +858,8 @@ gfem_status gfem_batch_submit(gfem_batch *B, gfem_system *sys, double tol, int m
   I.b = rhs;
   I.x = reinterpret_cast<double2 *>(take(16 * n));
   I.r = reinterpret_cast<double2 *>(take(16 * n));
+  double2 *dinv = reinterpret_cast<double2 *>(take(16 * n));
+  I.dinv = dinv;
   I.w = reinterpret_cast<float2 *>(take(8 * n));
   I.V = reinterpret_cast<float2 *>(take(8 * n * (size_t)(m + 1)));
   I.st = reinterpret_cast<double2 *>(take(16 * ((size_t)(m + 1) * m + 3 * (size_t)m + 2 + (size_t)(m + 1) + 2)));
@@ -842,7 +875,10 @@ gfem_status gfem_batch_submit(gfem_batch *B, gfem_system *sys, double tol, int m
     S = gfem_batch::Slot();
     return st;
   }
-  hipLaunchKernelGGL(bk_round, dim3((unsigned)((nnz + 255) / 256)), dim3(256), 0, s, (int64_t)sys->nnz, a64, a32);
+  hipLaunchKernelGGL(bk_diag_inv, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, (int64_t)n, sys->d_ai, sys->d_aj,
+                     a64, B->jacobi ? 1 : 0, dinv);
+  hipLaunchKernelGGL(bk_round_cols, dim3((unsigned)((nnz + 255) / 256)), dim3(256), 0, s, (int64_t)sys->nnz, a64,
+                     sys->d_aj, dinv, a32);
   hipLaunchKernelGGL(bk_zero, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, (int64_t)n, I.x);
   GFEM_HIP(ctx, hipGetLastError());
   st = upload_items(B, s);

@@ -299,6 +299,7 @@ gfem_status gfem_system_destroy(gfem_system *sys)
   dfree(sys->d_sol_y);
   dfree(sys->d_sol_part);
   dfree(sys->d_sol_nrm);
+  dfree(sys->d_sol_dinv);
   if(sys->h_sol_est) (void)hipHostFree(sys->h_sol_est);
   dfree(sys->d_table);
   dfree(sys->d_table_dense);

@@ -224,6 +224,7 @@ struct gfem_system {
   int32_t *d_aj = nullptr;
   void *d_sol_x = nullptr, *d_sol_r = nullptr, *d_sol_w = nullptr, *d_sol_V = nullptr, *d_sol_a32 = nullptr;
   double2 *d_sol_state = nullptr, *d_sol_y = nullptr, *d_sol_part = nullptr, *d_sol_nrm = nullptr;
+  double *d_sol_dinv = nullptr;  // right scaling of the Krylov iteration (1 / row norms)
   void *h_sol_est = nullptr;  // pinned
   int sol_m = 0;
   bool sol_valid = false;

@@ -246,20 +246,54 @@ __global__ void __launch_bounds__(256) k_round(int64_t n, const S *__restrict__ 
   if(i < n) st(dst, i, ld_d(src, i));
 }
 
-// x (FP64) += sum_k V_k y_k
+// Right scaling of the Krylov iteration: it runs on A D^-1 with D the
+// 2-norms of the rows of A (for the symmetric patterns assembled here, those
+// of its columns), so the FP32 copy of the matrix has its columns scaled by
+// D^-1 and a cycle's correction is D^-1 V y; the FP64 residual -b - A x is
+// the unscaled one.  dinv = 1 / D (1 for an empty row, or without scaling).
+template <typename S>
+__global__ void __launch_bounds__(256) k_row_scale(int64_t n, const int64_t *__restrict__ ai, const S *__restrict__ a,
+                                                   int scale, double *__restrict__ dinv)
+{
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(r >= n) return;
+  double q = 0.;
+  if(scale)
+    for(int64_t k = ai[r]; k < ai[r + 1]; ++k) {
+      const double2 v = ld_d(a, k);
+      q += v.x * v.x + v.y * v.y;
+    }
+  dinv[r] = q > 0. ? 1. / sqrt(q) : 1.;
+}
+
+// FP32 copy of A D^-1
+template <typename S, typename T>
+__global__ void __launch_bounds__(256) k_round_cols(int64_t nnz, const S *__restrict__ a, const int32_t *__restrict__ aj,
+                                                    const double *__restrict__ dinv, T *__restrict__ dst)
+{
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(k >= nnz) return;
+  const double2 v = ld_d(a, k);
+  const double d = dinv[aj[k]];
+  st(dst, k, make_double2(v.x * d, v.y * d));
+}
+
+// x (FP64) += D^-1 sum_k V_k y_k
 template <typename T, typename X>
 __global__ void __launch_bounds__(256) k_update_x(int64_t n, int j1, const T *__restrict__ V, int64_t ld,
-                                                  const double2 *__restrict__ y, X *__restrict__ x)
+                                                  const double2 *__restrict__ y, const double *__restrict__ dinv,
+                                                  X *__restrict__ x)
 {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if(i >= n) return;
-  double2 s = ld_d(x, i);
+  double2 s = make_double2(0., 0.);
   for(int k = 0; k < j1; ++k) {
     const double2 p = zmul(ld_d(V, (int64_t)k * ld + i), y[k]);
     s.x += p.x;
     s.y += p.y;
   }
-  st(x, i, s);
+  const double2 xi = ld_d(x, i);
+  st(x, i, make_double2(xi.x + dinv[i] * s.x, xi.y + dinv[i] * s.y));
 }
 
 // col[i] += y[i], i < k
@@ -414,7 +448,8 @@ static gfem_status gmres_ir(gfem_system *sys, int rhs_idx, double tol, int m, in
     if(!chk(hipMalloc(&sys->d_sol_x, sizeof(T64) * (size_t)n)) || !chk(hipMalloc(&sys->d_sol_r, sizeof(T64) * (size_t)n)) ||
        !chk(hipMalloc(&sys->d_sol_w, sizeof(T32) * (size_t)n)) ||
        !chk(hipMalloc(&sys->d_sol_a32, sizeof(T32) * (size_t)std::max<int64_t>(nnz, 1))) ||
-       !chk(hipMalloc(&sys->d_sol_nrm, sizeof(double2) * 2)))
+       !chk(hipMalloc(&sys->d_sol_nrm, sizeof(double2) * 2)) ||
+       !chk(hipMalloc(&sys->d_sol_dinv, sizeof(double) * (size_t)std::max<int64_t>(n, 1))))
       return fail(ctx, GFEM_ERR_OUT_OF_MEMORY, "solver workspace allocation");
     if(!chk(hipHostMalloc(&sys->h_sol_est, sizeof(double2) * 2)))
       return fail(ctx, GFEM_ERR_OUT_OF_MEMORY, "solver host scalar allocation");
@@ -435,7 +470,14 @@ static gfem_status gmres_ir(gfem_system *sys, int rhs_idx, double tol, int m, in
   const unsigned rb = blocks(n * kRowLanes, 256), vb = blocks(n, 256);
 
   GFEM_HIP(ctx, hipMemsetAsync(x, 0, sizeof(T64) * (size_t)n, s));
-  hipLaunchKernelGGL((k_round<T64, T32>), dim3(blocks(nnz, 256)), dim3(256), 0, s, nnz, a64, a32);
+  {
+    const char *ks = std::getenv("GFEM_KRYLOV_SCALE");
+    const int scale = (ks && ks[0] == '0') ? 0 : 1;
+    hipLaunchKernelGGL((k_row_scale<T64>), dim3(blocks(n, 256)), dim3(256), 0, s, n, sys->d_ai, a64, scale,
+                       sys->d_sol_dinv);
+    hipLaunchKernelGGL((k_round_cols<T64, T32>), dim3(blocks(nnz, 256)), dim3(256), 0, s, nnz, a64, sys->d_aj,
+                       sys->d_sol_dinv, a32);
+  }
   if(!sys->spmv_tuned) {
     // autotune the SpMV lanes per row on this matrix (one warm-up and three
     // timed products each; w is scratch here)
@@ -521,7 +563,8 @@ static gfem_status gmres_ir(gfem_system *sys, int rhs_idx, double tol, int m, in
     }
     // x += V y, y = R^-1 g
     hipLaunchKernelGGL(k_backsolve, dim3(1), dim3(64), 0, s, m, k, stt, sys->d_sol_y);
-    hipLaunchKernelGGL((k_update_x<T32, T64>), dim3(vb), dim3(256), 0, s, n, k, V, n, sys->d_sol_y, x);
+    hipLaunchKernelGGL((k_update_x<T32, T64>), dim3(vb), dim3(256), 0, s, n, k, V, n, sys->d_sol_y, sys->d_sol_dinv,
+                       x);
     GFEM_HIP(ctx, hipGetLastError());
   }
   if(!converged) {
