@@ -316,6 +316,8 @@ def main():
         sec_r = timed_steps(F, ref_steps, 1, "fp64", "reference")
         extra["reference_gpu_fp64"] = {"value": edofs / sec_r / 1e9, "ms_per_step": sec_r * 1e3,
                                        "what": "per-Gauss-point dense element matrices + atomic CSR scatter"}
+        # the mixed-precision engine against the plain FP64 GPU assembler
+        extra["mixed_vs_plain_fp64_gpu"] = sec_r / sec_m
         log(f"reference-order fp64: {sec_r * 1e3:.3f} ms/step  {edofs / sec_r / 1e9:.2f} GDOF/s")
 
     ho = None
