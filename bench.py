@@ -184,23 +184,26 @@ def serving(args):
 
 def highorder(args):
     """Order-4 tetrahedra (32^3 x 6 = 196,608 tets, N = 35), mass + stiffness,
-    Gauss8: element matrices as per-point B^T W B on the matrix cores (FP64 and
-    FP32 GEMM, Jacobian inverses in FP64) against the tensor-form FP64 path;
-    per-entry error of the FP32 matrix-core path against FP64."""
+    Gauss8: element matrices as per-point B^T W B on the matrix cores (FP64,
+    FP32 and split-bf16 GEMM, Jacobian inverses in FP64) against the
+    tensor-form FP64 path; per-entry error of the FP32 and bf16x3 matrix-core
+    paths against FP64."""
     import numpy as np
     from gfem import gauss_rule, workloads as W
     t0 = time.perf_counter()
     spec = W.highorder3d(n=args.ho_n, order=4)
     F = W.build(spec)
     F.pre()
-    F.assemble(download=False)
+    F.assemble(download=False)  # creates the system
     log(f"highorder: {spec.name} n={F.n_unknown} nnz={F.nnz} set-up {time.perf_counter() - t0:.1f} s")
     edofs = spec.element_dofs()
     out = {"workload": f"unit cube {args.ho_n}^3x6 P4 tets, mass + stiffness, Gauss8", "elements": edofs // 35,
            "element_dofs": edofs, "n_unknown": int(F.n_unknown), "nnz": int(F.nnz)}
     ref = None
     for name, prec, strat in (("tensor_fp64", "fp64", "fused"), ("gemm_fp64", "fp64", "dense"),
-                              ("gemm_fp32", "mixed", "dense")):
+                              ("gemm_fp32", "mixed", "dense"), ("gemm_bf16x3", "mixed", "dense")):
+        split = name == "gemm_bf16x3"
+        F.set_gemm_mode("bf16x3" if split else "fp32")
         sec = timed_steps(F, max(2, args.steps // 4), 1, prec, strat)
         prof, _ = kernel_profile(F, prec, strat, reps=3)
         rec = {"ms_per_step": sec * 1e3, "GDOF_s": edofs / sec / 1e9,
@@ -209,11 +212,13 @@ def highorder(args):
             K = 7 * len(gauss_rule(3, 8)[1])  # points of the Gauss8 tetrahedron rule
             flops = 2.0 * 35 * 35 * K * (edofs // 35)
             gemm_ms = prof.get("element_gemm", float("nan"))
-            peak = 78.6 if prec == "fp64" else 157.3
+            peak = 78.6 if prec == "fp64" else (2500.0 if split else 157.3)
             rec["gemm_tflops"] = flops / (gemm_ms * 1e-3) / 1e12
+            # the split form runs 3 bf16 products per FP32 one
+            rec["gemm_mfma_tflops"] = rec["gemm_tflops"] * (3 if split else 1)
             rec["gemm_peak_tflops"] = peak
-            rec["gemm_frac"] = rec["gemm_tflops"] / peak
-            rec["gemm_shape"] = f"M=N*N=1225, N=elements={edofs // 35}, K=7Q={K}"
+            rec["gemm_frac"] = rec["gemm_mfma_tflops"] / peak
+            rec["gemm_shape"] = f"M=N*N=1225, N=elements={edofs // 35}, K=7Q={K}" + (" (x3 bf16)" if split else "")
         F.assemble(precision=prec, strategy=strat)
         vals = F.values()
         if ref is None:

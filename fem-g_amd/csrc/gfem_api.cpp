@@ -82,6 +82,8 @@ void free_group(Group &g)
   dfree(g.d_R32);
   dfree(g.d_A64);
   dfree(g.d_A32);
+  dfree(g.d_Rb);
+  dfree(g.d_Ab);
   dfree(g.d_e2n);
   dfree(g.d_num);
   dfree(g.d_type);
@@ -791,6 +793,8 @@ gfem_status gfem_system_finalize(gfem_system *sys)
   sys->autotune = !(at && at[0] == '0');
   const char *dg = std::getenv("GFEM_DENSE_GEMM");
   sys->dense_gemm = !(dg && dg[0] == '0');
+  const char *gb = std::getenv("GFEM_GEMM_BF16X3");
+  if(gb && gb[0] == '1') sys->gemm_mode = GFEM_GEMM_BF16X3;
   if(const char *e = std::getenv("GFEM_GATHER_SLOTS")) {
     // a fixed choice, not tuned
     const int v = std::atoi(e);
@@ -886,8 +890,8 @@ static gfem_status apply_updates(gfem_system *sys, hipStream_t s)
 
 
 // reference products R of a group (once) and its A rows / element matrices
-// for one precision
-static gfem_status ensure_gemm(gfem_system *sys, Group &g, bool fp32)
+// for one precision (split: the bf16 hi/lo form of R and A)
+static gfem_status ensure_gemm(gfem_system *sys, Group &g, bool fp32, bool split)
 {
   gfem_ctx *ctx = sys->ctx;
   const size_t planes = sys->cplx ? 2 : 1, NN = (size_t)g.N * g.N;
@@ -898,7 +902,16 @@ static gfem_status ensure_gemm(gfem_system *sys, Group &g, bool fp32)
     GFEM_HIP(ctx, launch_ref_products(g, nullptr));
     GFEM_HIP(ctx, hipDeviceSynchronize());
   }
-  if(fp32) {
+  if(split) {
+    if(!g.d_Rb) {
+      GFEM_HIP(ctx, dalloc(&g.d_Rb, 3 * K * NN));
+      GFEM_HIP(ctx, launch_ref_split(g, nullptr));
+      GFEM_HIP(ctx, hipDeviceSynchronize());
+    }
+    if(!g.d_Ab) GFEM_HIP(ctx, dalloc(&g.d_Ab, (size_t)g.ne * planes * 3 * K));
+    if(!g.d_dense32) GFEM_HIP(ctx, dalloc(&g.d_dense32, (size_t)g.ne * planes * NN));
+  }
+  else if(fp32) {
     if(!g.d_A32) GFEM_HIP(ctx, dalloc(&g.d_A32, (size_t)g.ne * planes * K));
     if(!g.d_dense32) GFEM_HIP(ctx, dalloc(&g.d_dense32, (size_t)g.ne * planes * NN));
   }
@@ -910,8 +923,9 @@ static gfem_status ensure_gemm(gfem_system *sys, Group &g, bool fp32)
 }
 
 // element matrices of group g: dense[ne*planes][N*N] = A[ne*planes][7Q] R[7Q][N*N]
-// (row-major; as column-major, dense^T = R^T A^T with no transposes)
-static gfem_status element_gemm(gfem_system *sys, Group &g, bool fp32, int planes, hipStream_t s)
+// (row-major; as column-major, dense^T = R^T A^T with no transposes).  The
+// split form is the same product over 3*7Q bf16 columns with FP32 sums.
+static gfem_status element_gemm(gfem_system *sys, Group &g, bool fp32, bool split, int planes, hipStream_t s)
 {
   gfem_ctx *ctx = sys->ctx;
   if(!ctx->blas) {
@@ -926,7 +940,17 @@ static gfem_status element_gemm(gfem_system *sys, Group &g, bool fp32, int plane
   if(n == 0) return GFEM_OK;
   if(n >= (int64_t(1) << 31)) return fail(ctx, GFEM_ERR_TOO_LARGE, "element GEMM larger than 2^31 rows");
   rocblas_status r;
-  if(fp32) {
+  if(split) {
+    if(3 * k >= (int64_t(1) << 31)) return fail(ctx, GFEM_ERR_TOO_LARGE, "split element GEMM: 3*7Q too large");
+    const float one = 1.f, zero = 0.f;
+    // Rb is stored [N*N][3*7Q] (K-contiguous): the transposed operand
+    r = rocblas_gemm_ex(h, rocblas_operation_transpose, rocblas_operation_none, (rocblas_int)m, (rocblas_int)n,
+                        (rocblas_int)(3 * k), &one, g.d_Rb, rocblas_datatype_bf16_r, (rocblas_int)(3 * k), g.d_Ab,
+                        rocblas_datatype_bf16_r, (rocblas_int)(3 * k), &zero, g.d_dense32, rocblas_datatype_f32_r,
+                        (rocblas_int)m, g.d_dense32, rocblas_datatype_f32_r, (rocblas_int)m, rocblas_datatype_f32_r,
+                        rocblas_gemm_algo_standard, 0, 0);
+  }
+  else if(fp32) {
     const float one = 1.f, zero = 0.f;
     r = rocblas_sgemm(h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)m, (rocblas_int)n,
                       (rocblas_int)k, &one, g.d_R32, (rocblas_int)m, g.d_A32, (rocblas_int)k, &zero, g.d_dense32,
@@ -1095,13 +1119,14 @@ static gfem_status assemble_impl(gfem_system *sys, gfem_precision prec, gfem_str
       prof.end();
       if(!(all ? g.has_bilinear : g.has_dense)) continue;
       if(g.gemm_rule >= 0 && (all || sys->dense_gemm)) {
-        st = ensure_gemm(sys, g, fp32);
+        const bool split = fp32 && sys->gemm_mode == GFEM_GEMM_BF16X3;
+        st = ensure_gemm(sys, g, fp32, split);
         if(st != GFEM_OK) return st;
         prof.begin("coef_rows");
-        GFEM_HIP(ctx, launch_coef_rows(g, sys->d_xyz, all, planes, fp32, s));
+        GFEM_HIP(ctx, launch_coef_rows(g, sys->d_xyz, all, planes, fp32, split, s));
         prof.end();
         prof.begin("element_gemm");
-        st = element_gemm(sys, g, fp32, planes, s);
+        st = element_gemm(sys, g, fp32, split, planes, s);
         prof.end();
         if(st != GFEM_OK) return st;
       }
@@ -1291,6 +1316,13 @@ gfem_status gfem_values_device(gfem_system *sys, gfem_precision prec, void **dv,
   const int p = prec == GFEM_PREC_MIXED ? 1 : 0;
   if(dv) *dv = sys->d_vals[p];
   if(dr) *dr = sys->d_rhs[p];
+  return GFEM_OK;
+}
+
+gfem_status gfem_set_gemm_mode(gfem_system *sys, gfem_gemm_mode mode)
+{
+  if(!sys || (mode != GFEM_GEMM_FP32 && mode != GFEM_GEMM_BF16X3)) return GFEM_ERR_INVALID_ARG;
+  sys->gemm_mode = mode;
   return GFEM_OK;
 }
 

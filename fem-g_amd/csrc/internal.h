@@ -147,6 +147,11 @@ struct Group {
   float *d_R32 = nullptr;
   double *d_A64 = nullptr;
   float *d_A32 = nullptr;
+  // the split form (bf16 bits): Rb = [hi(R); lo(R); hi(R)] as 3*7Q rows of
+  // N*N, and per row c of A: [hi(A_c), hi(A_c), lo(A_c)], so one bf16 GEMM
+  // over 3*7Q sums hi*hi + lo*hi + hi*lo with FP32 accumulation
+  uint16_t *d_Rb = nullptr;
+  uint16_t *d_Ab = nullptr;
   double2 *d_be = nullptr;      // this group's element loads inside gfem_system::d_be_all
   int64_t be_off = 0, be_stride = 0;  // offset in d_be_all, distance between right-hand sides
   // maps in storage order on the device (from finalize): numDof-1 and
@@ -204,6 +209,7 @@ struct gfem_system {
   bool tuned[2] = {false, false};
   bool autotune = true;        // GFEM_AUTOTUNE at finalize
   bool dense_gemm = true;      // GFEM_DENSE_GEMM at finalize: GEMM form of per-point terms
+  int gemm_mode = 0;           // GEMM arithmetic under the mixed precision (gfem_gemm_mode)
   int spmv_lanes = 8;         // solver SpMV lanes per row (autotuned at the first solve)
   bool spmv_tuned = false;
   int rec_stride = gfem::kRecNarrow;  // uint32 per record
@@ -298,7 +304,9 @@ hipError_t launch_linear(const Group &g, const double *xyz, int nrhs, int eb_shi
 hipError_t launch_dense(const Group &g, const double *xyz, bool all_terms, int planes, hipStream_t s);
 hipError_t launch_dense_to32(const Group &g, int planes, hipStream_t s);
 hipError_t launch_ref_products(const Group &g, hipStream_t s);
-hipError_t launch_coef_rows(const Group &g, const double *xyz, bool all_terms, int planes, bool fp32, hipStream_t s);
+hipError_t launch_coef_rows(const Group &g, const double *xyz, bool all_terms, int planes, bool fp32, bool split,
+                            hipStream_t s);
+hipError_t launch_ref_split(const Group &g, hipStream_t s);
 // tensor = 0: the constant-coefficient tensors are left out (every bilinear
 // term is in the per-point element matrices)
 hipError_t launch_gather(const gfem_system &sys, int prec, int tensor, void *vals, hipStream_t s);

@@ -635,46 +635,129 @@ __global__ void __launch_bounds__(256) k_ref_products(int N, int Q, const double
   R[t] = (T)(w[q] * v);
 }
 
-// one thread per (element, Gauss point): the 7 entries of A for that point,
-// per plane (real part, then imaginary part for complex systems)
-template <typename T>
-__global__ void __launch_bounds__(256) k_coef_rows(const GroupDev *__restrict__ gd, int all_terms, int rule, int64_t ne,
-                                                   const int32_t *__restrict__ vtx, const double *__restrict__ xyz,
-                                                   const double *__restrict__ F64, int planes, T *__restrict__ A)
+// bfloat16 bits of x, rounded to nearest even (finite x)
+__device__ __forceinline__ uint32_t bf16_bits(float x)
 {
+  const uint32_t u = __float_as_uint(x);
+  return (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
+}
+
+// x = hi + lo + O(2^-17 |x|) with hi, lo bfloat16: hi = bf16(x), lo = bf16(x - hi)
+__device__ __forceinline__ void bf16_split(double x, uint16_t &hi, uint16_t &lo)
+{
+  const uint32_t h = bf16_bits((float)x);
+  hi = (uint16_t)h;
+  lo = (uint16_t)bf16_bits((float)(x - (double)__uint_as_float(h << 16)));
+}
+
+// R [K][NN] (K = 7Q) -> Rb [NN][3K], row `pair` = [hi(R_pair), lo(R_pair),
+// hi(R_pair)]: K-contiguous like the rows of A, so both GEMM operands stream
+// along the inner dimension
+__global__ void __launch_bounds__(256) k_ref_split(int64_t total, int64_t NN, const double *__restrict__ R,
+                                                   uint16_t *__restrict__ Rb)
+{
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(t >= total) return;
+  const int64_t K = total / NN, kk = t / NN, pair = t - kk * NN;
+  uint16_t hi, lo;
+  bf16_split(R[t], hi, lo);
+  uint16_t *row = Rb + pair * 3 * K;
+  row[kk] = hi;
+  row[K + kk] = lo;
+  row[2 * K + kk] = hi;
+}
+
+// one thread per (element, Gauss point): the 7 entries of A for that point,
+// per plane (real part, then imaginary part for complex systems).  A block's
+// rows are staged in LDS and written back with consecutive lanes on
+// consecutive addresses (a block's points are consecutive entries of each
+// row of A).  SPLIT writes row c of A as [hi(A_c), hi(A_c), lo(A_c)]
+// (bf16 bits) into Ab.  PP: some coefficient is an expression of x (without
+// it the kernel leaves out the point coordinates and the interpreter).
+template <typename T, bool SPLIT> struct CoefStage {
+  using type = T;
+};
+template <typename T> struct CoefStage<T, true> {
+  using type = uint32_t;  // hi | lo << 16
+};
+
+constexpr int kCoefBlock = 256;
+
+template <typename T, bool SPLIT, bool PP>
+__global__ void __launch_bounds__(kCoefBlock) k_coef_rows(const GroupDev *__restrict__ gd, int all_terms, int rule,
+                                                          int64_t ne, const int32_t *__restrict__ vtx,
+                                                          const double *__restrict__ xyz,
+                                                          const double *__restrict__ F64, int planes,
+                                                          T *__restrict__ A, uint16_t *__restrict__ Ab = nullptr)
+{
+  using S = typename CoefStage<T, SPLIT>::type;
+  __shared__ S st[2][kCoefBlock * 7];
   const RuleDev &R = gd->rules[rule];
   const int Q = R.Q;
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if(t >= ne * Q) return;
-  const int64_t e = t / Q;
-  const int q = (int)(t % Q);
-  double X[4][3];
-  load_vertices(vtx, gd->nv, e, xyz, X);
-  double x[3];
-  phys_point(gd->dim, X, &R.pts[3 * q], x);
-  double2 stiff = make_double2(0., 0.), mass = make_double2(0., 0.);
-  const int nterms = __builtin_amdgcn_readfirstlane(gd->nterms);
-  for(int k = 0; k < nterms; ++k) {
-    const DevTerm *dt = &gd->terms[k];
-    if(__builtin_amdgcn_readfirstlane(dt->kind) != 0) continue;
-    const bool pp = __builtin_amdgcn_readfirstlane(dt->per_point) != 0;
-    if(!all_terms && !pp) continue;
-    const double2 c = pp ? eval_expr(&dt->expr, x[0], x[1], x[2]) : make_double2(dt->c_re, dt->c_im);
-    const double2 v = cmul(make_double2(dt->fac_re, dt->fac_im), c);
-    if(__builtin_amdgcn_readfirstlane(dt->form_dof) == 1)
-      stiff = cadd(stiff, v);
-    else
-      mass = cadd(mass, v);
-  }
-  const double *F = &F64[e * 8];  // det G00, G11, G22, G01, G02, G12, det
-  const int64_t K = (int64_t)7 * Q;
-  for(int p = 0; p < planes; ++p) {
-    const double cs = p == 0 ? stiff.x : stiff.y, cm = p == 0 ? mass.x : mass.y;
-    T *row = A + (e * planes + p) * K + (int64_t)q * 7;
+  const int64_t total = ne * Q, t0 = (int64_t)blockIdx.x * kCoefBlock;
+  // point t0 + i is (e0 + (q0 + i) / Q, (q0 + i) % Q): one wide division per block
+  const int64_t e0 = t0 / Q;
+  const uint32_t q0 = (uint32_t)(t0 - e0 * Q);
+  if(t0 + threadIdx.x < total) {
+    const uint32_t r = q0 + threadIdx.x, de = r / (uint32_t)Q;
+    const int64_t e = e0 + de;
+    const int q = (int)(r - de * (uint32_t)Q);
+    double x[3] = {0., 0., 0.};
+    if constexpr(PP) {
+      double X[4][3];
+      load_vertices(vtx, gd->nv, e, xyz, X);
+      phys_point(gd->dim, X, &R.pts[3 * q], x);
+    }
+    double2 stiff = make_double2(0., 0.), mass = make_double2(0., 0.);
+    const int nterms = __builtin_amdgcn_readfirstlane(gd->nterms);
+    for(int k = 0; k < nterms; ++k) {
+      const DevTerm *dt = &gd->terms[k];
+      if(__builtin_amdgcn_readfirstlane(dt->kind) != 0) continue;
+      const bool pp = PP && __builtin_amdgcn_readfirstlane(dt->per_point) != 0;
+      if(!all_terms && !pp) continue;
+      double2 c = make_double2(dt->c_re, dt->c_im);
+      if constexpr(PP)
+        if(pp) c = eval_expr(&dt->expr, x[0], x[1], x[2]);
+      const double2 v = cmul(make_double2(dt->fac_re, dt->fac_im), c);
+      if(__builtin_amdgcn_readfirstlane(dt->form_dof) == 1)
+        stiff = cadd(stiff, v);
+      else
+        mass = cadd(mass, v);
+    }
+    const double *F = &F64[e * 8];  // det G00, G11, G22, G01, G02, G12, det
+    for(int p = 0; p < planes; ++p) {
+      const double cs = p == 0 ? stiff.x : stiff.y, cm = p == 0 ? mass.x : mass.y;
 #pragma unroll
-    for(int s = 0; s < 6; ++s) row[s] = (T)(F[s] * cs);
-    row[6] = (T)(F[6] * cm);
+      for(int s = 0; s < 7; ++s) {
+        const double v = F[s] * (s < 6 ? cs : cm);
+        if constexpr(SPLIT) {
+          uint16_t hi, lo;
+          bf16_split(v, hi, lo);
+          st[p][threadIdx.x * 7 + s] = (uint32_t)hi | (uint32_t)lo << 16;
+        }
+        else
+          st[p][threadIdx.x * 7 + s] = (T)v;
+      }
+    }
   }
+  __syncthreads();
+  const int npts = (int)min((int64_t)kCoefBlock, total - t0);
+  const int64_t K = (int64_t)7 * Q;
+  for(int p = 0; p < planes; ++p)
+    for(int j = threadIdx.x; j < 7 * npts; j += kCoefBlock) {
+      const uint32_t tl = (uint32_t)j / 7u, r = q0 + tl, de = r / (uint32_t)Q;
+      const int64_t e = e0 + de;
+      const int64_t col = (int64_t)(r - de * (uint32_t)Q) * 7 + (j - 7 * (int)tl);
+      const S v = st[p][j];
+      if constexpr(SPLIT) {
+        uint16_t *row = Ab + (e * planes + p) * 3 * K + col;
+        row[0] = (uint16_t)(v & 0xffffu);
+        row[K] = (uint16_t)(v & 0xffffu);
+        row[2 * K] = (uint16_t)(v >> 16);
+      }
+      else
+        A[(e * planes + p) * K + col] = v;
+    }
 }
 
 // ------------------------------------------------------------------ element value
@@ -1346,16 +1429,38 @@ hipError_t launch_ref_products(const Group &g, hipStream_t s)
   return hipGetLastError();
 }
 
-hipError_t launch_coef_rows(const Group &g, const double *xyz, bool all_terms, int planes, bool fp32, hipStream_t s)
+hipError_t launch_ref_split(const Group &g, hipStream_t s)
+{
+  const int64_t total = (int64_t)7 * g.rules[g.gemm_rule].Q * g.N * g.N;
+  hipLaunchKernelGGL(k_ref_split, dim3(blocks_for(total, 256)), dim3(256), 0, s, total, (int64_t)g.N * g.N, g.d_R64,
+                     g.d_Rb);
+  return hipGetLastError();
+}
+
+template <typename T, bool SPLIT>
+static void coef_rows_pp(const Group &g, const double *xyz, bool all_terms, int planes, T *A, hipStream_t s)
 {
   const int64_t total = g.ne * (int64_t)g.rules[g.gemm_rule].Q;
-  if(total == 0) return hipSuccess;
-  if(fp32)
-    hipLaunchKernelGGL(k_coef_rows<float>, dim3(blocks_for(total, 256)), dim3(256), 0, s, g.d_dev, all_terms ? 1 : 0,
-                       g.gemm_rule, g.ne, g.d_vtx, xyz, g.d_F64, planes, g.d_A32);
+  const dim3 grid(blocks_for(total, kCoefBlock)), block(kCoefBlock);
+  uint16_t *Ab = SPLIT ? g.d_Ab : nullptr;
+  if(g.has_dense)  // a bilinear term with a coefficient that depends on x
+    hipLaunchKernelGGL((k_coef_rows<T, SPLIT, true>), grid, block, 0, s, g.d_dev, all_terms ? 1 : 0, g.gemm_rule, g.ne,
+                       g.d_vtx, xyz, g.d_F64, planes, A, Ab);
   else
-    hipLaunchKernelGGL(k_coef_rows<double>, dim3(blocks_for(total, 256)), dim3(256), 0, s, g.d_dev, all_terms ? 1 : 0,
-                       g.gemm_rule, g.ne, g.d_vtx, xyz, g.d_F64, planes, g.d_A64);
+    hipLaunchKernelGGL((k_coef_rows<T, SPLIT, false>), grid, block, 0, s, g.d_dev, all_terms ? 1 : 0, g.gemm_rule,
+                       g.ne, g.d_vtx, xyz, g.d_F64, planes, A, Ab);
+}
+
+hipError_t launch_coef_rows(const Group &g, const double *xyz, bool all_terms, int planes, bool fp32, bool split,
+                            hipStream_t s)
+{
+  if(g.ne == 0) return hipSuccess;
+  if(split)
+    coef_rows_pp<float, true>(g, xyz, all_terms, planes, nullptr, s);
+  else if(fp32)
+    coef_rows_pp<float, false>(g, xyz, all_terms, planes, g.d_A32, s);
+  else
+    coef_rows_pp<double, false>(g, xyz, all_terms, planes, g.d_A64, s);
   return hipGetLastError();
 }
 

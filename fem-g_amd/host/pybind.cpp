@@ -143,6 +143,12 @@ void bind_formulation(py::module_ &m, const char *name)
       },
       py::arg("precision") = "fp64", py::arg("strategy") = "fused")
     .def("sync", [](F &) { check(gfem_stream_sync(common::deviceContext(), nullptr), "gfem_stream_sync"); })
+    .def("set_gemm_mode",
+         [](F &f, const std::string &m) {
+           if(m != "fp32" && m != "bf16x3") throw std::invalid_argument("gemm mode: fp32 or bf16x3");
+           if(!f.system()) throw std::runtime_error("set_gemm_mode: the system exists after the first assemble()");
+           check(gfem_set_gemm_mode(f.system(), m == "fp32" ? GFEM_GEMM_FP32 : GFEM_GEMM_BF16X3), "set_gemm_mode");
+         })
     .def("set_profiling", [](F &f, bool on) { check(gfem_set_profiling(f.system(), on ? 1 : 0), "set_profiling"); })
     .def("kernel_times",
          [](F &f) {

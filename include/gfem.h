@@ -354,6 +354,22 @@ gfem_status gfem_batch_done(gfem_batch *batch, int max, int64_t *ids, int *count
 /* Solution (n complex128, may be NULL) and info of an ended request; frees it. */
 gfem_status gfem_batch_result(gfem_batch *batch, int64_t id, void *x_host, size_t bytes, gfem_solve_info *info);
 
+/* Arithmetic of the element-matrix GEMM (GFEM_STRATEGY_DENSE, and per-point
+ * terms in the fused strategy) under GFEM_PREC_MIXED:
+ *   GFEM_GEMM_FP32   : FP32 operands on the FP32 matrix cores (default);
+ *   GFEM_GEMM_BF16X3 : each operand x split as hi = bf16(x), lo = bf16(x - hi);
+ *                      hi*hi + lo*hi + hi*lo summed in FP32 on the bf16 matrix
+ *                      cores (one GEMM over three times the inner dimension;
+ *                      the dropped lo*lo term is ~2^-16 of each product).
+ * Read from GFEM_GEMM_BF16X3=1 at finalize, or set here at any time before an
+ * assembly.  No counterpart in the reference, whose element matrices are
+ * FP64 (term/BilinearTerm.cpp:303-343). */
+typedef enum {
+  GFEM_GEMM_FP32 = 0,
+  GFEM_GEMM_BF16X3 = 1
+} gfem_gemm_mode;
+gfem_status gfem_set_gemm_mode(gfem_system *sys, gfem_gemm_mode mode);
+
 /* Per-kernel timing of the last gfem_assemble on its stream, from HIP events
  * recorded around each launch when profiling is on.  names[i] points to
  * static strings.  Returns the number of kernels timed in *count. */

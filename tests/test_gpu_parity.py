@@ -222,6 +222,35 @@ def test_dense_strategy_matches_oracle(case):
     assert_close(F.values(), P.values(), FP64_RTOL, case + " A (fused after dense)")
 
 
+# three bf16 products per FP32 one: each operand is hi + lo to ~2^-18, lo*lo is
+# dropped (~2^-18), so each product is good to ~3 * 2^-18 ~ 1e-5; Helmholtz
+# entries cancel the stiffness against k^2 times the mass, which amplifies that
+# ~30x on the worst entries (measured 4.4e-4 on helmholtz3d_p2)
+BF16X3_RTOL = 1e-3
+
+
+@pytest.mark.parametrize("case", ["helmholtz3d_p2", "highorder3d_p4", "poisson2d_p3"])
+def test_bf16x3_gemm_matches_oracle(case):
+    """The element GEMM on the bf16 matrix cores with split operands: within
+    its bar of the oracle (bf16 alone, lo terms dropped, is ~2^-8 per product
+    and fails it), and within the same bar of the FP32 GEMM; FP32 again after
+    switching back gives the FP32 values bit for bit."""
+    print(f"case bf16x3 gemm {case}", flush=True)
+    F, P = _pair(_cases()[case]())
+    F.assemble(precision="mixed", strategy="dense")
+    v32 = F.values()
+    F.set_gemm_mode("bf16x3")
+    F.assemble(precision="mixed", strategy="dense")
+    vb = F.values()
+    assert_close(vb, P.values(), BF16X3_RTOL, case + " A (dense bf16x3)")
+    assert_close(F.rhs(0), P.rhs(0), DENSE_FP32_RTOL, case + " b (dense bf16x3)")
+    assert_close(vb, v32, BF16X3_RTOL, case + " bf16x3 vs fp32 GEMM")
+    assert not np.array_equal(vb, v32)  # the split path ran
+    F.set_gemm_mode("fp32")
+    F.assemble(precision="mixed", strategy="dense")
+    assert np.array_equal(F.values(), v32)
+
+
 def test_per_point_gemm_matches_direct_kernel(monkeypatch):
     """Per-point bilinear coefficients on P3 triangles: the GEMM form and the
     direct per-point kernel (GFEM_DENSE_GEMM=0) agree with the oracle."""
