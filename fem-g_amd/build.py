@@ -69,7 +69,7 @@ def build(verbose=False):
                 print(out)
 
     libgfem = os.path.join(OUT, "libgfem.so")
-    run([HIPCC, *HIP_FLAGS, "-shared", "-o", libgfem, *dev_objs, "-L/opt/rocm/lib", "-lhiprtc", "-lrocblas",
+    run([HIPCC, *HIP_FLAGS, "-shared", "-o", libgfem, *dev_objs, "-L/opt/rocm/lib", "-lhiprtc", "-lrocblas", "-lhipblaslt",
          "-Wl,-rpath,/opt/rocm/lib"])
     libhost = os.path.join(OUT, "libgfem_host.so")
     run(["g++", "-shared", "-o", libhost, *host_objs, "-L", OUT, "-lgfem", "-Wl,-rpath,$ORIGIN"])

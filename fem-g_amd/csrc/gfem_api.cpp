@@ -12,6 +12,7 @@
 #include <string>
 #include <vector>
 
+#include <hipblaslt/hipblaslt.h>
 #include <rocblas/rocblas.h>
 
 #include "gfem.h"
@@ -64,8 +65,30 @@ gfem_status hip_fail(gfem_ctx *ctx, hipError_t e, const char *what)
 
 namespace {
 
+// hipBLASLt plan of a group's split GEMM: descriptors, the kernel kept (the
+// fastest of the library's candidates, timed on the group's own operands)
+// and its workspace
+struct LtPlan {
+  hipblasLtMatmulDesc_t desc = nullptr;
+  hipblasLtMatrixLayout_t la = nullptr, lb = nullptr, lc = nullptr;
+  hipblasLtMatmulAlgo_t algo{};
+  size_t ws = 0;
+  void *d_ws = nullptr;
+  int candidates = 0;
+  ~LtPlan()
+  {
+    if(d_ws) (void)hipFree(d_ws);
+    if(la) (void)hipblasLtMatrixLayoutDestroy(la);
+    if(lb) (void)hipblasLtMatrixLayoutDestroy(lb);
+    if(lc) (void)hipblasLtMatrixLayoutDestroy(lc);
+    if(desc) (void)hipblasLtMatmulDescDestroy(desc);
+  }
+};
+
 void free_group(Group &g)
 {
+  delete static_cast<LtPlan *>(g.lt_plan);
+  g.lt_plan = nullptr;
   dfree(g.d_vtx);
   dfree(g.d_nodes);
   dfree(g.d_terms);
@@ -212,6 +235,7 @@ gfem_status gfem_ctx_destroy(gfem_ctx *ctx)
   (void)hipSetDevice(ctx->device);
   dfree(ctx->d_xyz);
   if(ctx->blas) (void)rocblas_destroy_handle(static_cast<rocblas_handle>(ctx->blas));
+  if(ctx->blaslt) (void)hipblasLtDestroy(static_cast<hipblasLtHandle_t>(ctx->blaslt));
   delete ctx;
   return GFEM_OK;
 }
@@ -922,6 +946,86 @@ static gfem_status ensure_gemm(gfem_system *sys, Group &g, bool fp32, bool split
   return GFEM_OK;
 }
 
+// the split GEMM through hipBLASLt: Cᵀ[N*N][ne*planes] = Rbᵀ Ab with both
+// operands K-contiguous (op(A) = T), FP32 accumulation and output.  The
+// library's first kLtCandidates candidates are timed (one warm-up and two
+// timed runs each) and the fastest kept; with GFEM_AUTOTUNE=0 its first
+// candidate.
+constexpr int kLtCandidates = 16;
+static gfem_status lt_plan(gfem_system *sys, Group &g, int64_t m, int64_t n, int64_t k3, hipStream_t s)
+{
+  gfem_ctx *ctx = sys->ctx;
+  if(g.lt_plan) return GFEM_OK;
+  if(!ctx->blaslt) {
+    hipblasLtHandle_t h = nullptr;
+    if(hipblasLtCreate(&h) != HIPBLAS_STATUS_SUCCESS) return fail(ctx, GFEM_ERR_HIP, "hipblasLtCreate failed");
+    ctx->blaslt = h;
+  }
+  hipblasLtHandle_t h = static_cast<hipblasLtHandle_t>(ctx->blaslt);
+  auto *P = new LtPlan;
+  auto bad = [&](const char *what) {
+    delete P;
+    return fail(ctx, GFEM_ERR_HIP, std::string("split GEMM: ") + what);
+  };
+  const hipblasOperation_t opA = HIPBLAS_OP_T, opB = HIPBLAS_OP_N;
+  if(hipblasLtMatmulDescCreate(&P->desc, HIPBLAS_COMPUTE_32F, HIP_R_32F) != HIPBLAS_STATUS_SUCCESS ||
+     hipblasLtMatmulDescSetAttribute(P->desc, HIPBLASLT_MATMUL_DESC_TRANSA, &opA, sizeof(opA)) !=
+         HIPBLAS_STATUS_SUCCESS ||
+     hipblasLtMatmulDescSetAttribute(P->desc, HIPBLASLT_MATMUL_DESC_TRANSB, &opB, sizeof(opB)) !=
+         HIPBLAS_STATUS_SUCCESS)
+    return bad("matmul descriptor");
+  if(hipblasLtMatrixLayoutCreate(&P->la, HIP_R_16BF, (uint64_t)k3, (uint64_t)m, k3) != HIPBLAS_STATUS_SUCCESS ||
+     hipblasLtMatrixLayoutCreate(&P->lb, HIP_R_16BF, (uint64_t)k3, (uint64_t)n, k3) != HIPBLAS_STATUS_SUCCESS ||
+     hipblasLtMatrixLayoutCreate(&P->lc, HIP_R_32F, (uint64_t)m, (uint64_t)n, m) != HIPBLAS_STATUS_SUCCESS)
+    return bad("matrix layouts");
+  hipblasLtMatmulPreference_t pref = nullptr;
+  uint64_t max_ws = uint64_t(64) << 20;
+  hipblasLtMatmulHeuristicResult_t cand[kLtCandidates];
+  int nc = 0;
+  if(hipblasLtMatmulPreferenceCreate(&pref) != HIPBLAS_STATUS_SUCCESS) return bad("preference");
+  const bool ok = hipblasLtMatmulPreferenceSetAttribute(pref, HIPBLASLT_MATMUL_PREF_MAX_WORKSPACE_BYTES, &max_ws,
+                                                        sizeof(max_ws)) == HIPBLAS_STATUS_SUCCESS &&
+                  hipblasLtMatmulAlgoGetHeuristic(h, P->desc, P->la, P->lb, P->lc, P->lc, pref, kLtCandidates, cand, &nc) ==
+                      HIPBLAS_STATUS_SUCCESS;
+  (void)hipblasLtMatmulPreferenceDestroy(pref);
+  if(!ok || nc <= 0) return bad("no kernel for this shape");
+  size_t ws = 0;
+  for(int c = 0; c < nc; ++c) ws = std::max(ws, cand[c].workspaceSize);
+  if(ws && hipMalloc(&P->d_ws, ws) != hipSuccess) return bad("workspace allocation");
+  P->ws = ws;
+  P->candidates = nc;
+  int best = 0;
+  if(sys->autotune && nc > 1) {
+    hipEvent_t a = nullptr, b = nullptr;
+    if(hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return bad("events");
+    const float one = 1.f, zero = 0.f;
+    float best_ms = 0.f;
+    for(int c = 0; c < nc; ++c) {
+      float ms = 0.f;
+      bool run_ok = true;
+      for(int it = 0; it < 3 && run_ok; ++it) {
+        if(it == 1) run_ok = hipEventRecord(a, s) == hipSuccess;
+        run_ok = run_ok && hipblasLtMatmul(h, P->desc, &one, g.d_Rb, P->la, g.d_Ab, P->lb, &zero, g.d_dense32, P->lc,
+                                           g.d_dense32, P->lc, &cand[c].algo, P->d_ws, P->ws, s) ==
+                               HIPBLAS_STATUS_SUCCESS;
+      }
+      run_ok = run_ok && hipEventRecord(b, s) == hipSuccess && hipEventSynchronize(b) == hipSuccess &&
+               hipEventElapsedTime(&ms, a, b) == hipSuccess;
+      if(std::getenv("GFEM_SETUP_TIMING"))
+        std::fprintf(stderr, "gfem: split GEMM candidate %d/%d: %.3f ms%s\n", c, nc, ms / 2, run_ok ? "" : " (failed)");
+      if(run_ok && (c == 0 || ms < best_ms)) {
+        best = c;
+        best_ms = ms;
+      }
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+  }
+  P->algo = cand[best].algo;
+  g.lt_plan = P;
+  return GFEM_OK;
+}
+
 // element matrices of group g: dense[ne*planes][N*N] = A[ne*planes][7Q] R[7Q][N*N]
 // (row-major; as column-major, dense^T = R^T A^T with no transposes).  The
 // split form is the same product over 3*7Q bf16 columns with FP32 sums.
@@ -942,13 +1046,16 @@ static gfem_status element_gemm(gfem_system *sys, Group &g, bool fp32, bool spli
   rocblas_status r;
   if(split) {
     if(3 * k >= (int64_t(1) << 31)) return fail(ctx, GFEM_ERR_TOO_LARGE, "split element GEMM: 3*7Q too large");
+    const gfem_status st = lt_plan(sys, g, m, n, 3 * k, s);
+    if(st != GFEM_OK) return st;
+    const LtPlan *P = static_cast<const LtPlan *>(g.lt_plan);
     const float one = 1.f, zero = 0.f;
     // Rb is stored [N*N][3*7Q] (K-contiguous): the transposed operand
-    r = rocblas_gemm_ex(h, rocblas_operation_transpose, rocblas_operation_none, (rocblas_int)m, (rocblas_int)n,
-                        (rocblas_int)(3 * k), &one, g.d_Rb, rocblas_datatype_bf16_r, (rocblas_int)(3 * k), g.d_Ab,
-                        rocblas_datatype_bf16_r, (rocblas_int)(3 * k), &zero, g.d_dense32, rocblas_datatype_f32_r,
-                        (rocblas_int)m, g.d_dense32, rocblas_datatype_f32_r, (rocblas_int)m, rocblas_datatype_f32_r,
-                        rocblas_gemm_algo_standard, 0, 0);
+    if(hipblasLtMatmul(static_cast<hipblasLtHandle_t>(ctx->blaslt), P->desc, &one, g.d_Rb, P->la, g.d_Ab, P->lb,
+                       &zero, g.d_dense32, P->lc, g.d_dense32, P->lc, &P->algo, P->d_ws, P->ws, s) !=
+       HIPBLAS_STATUS_SUCCESS)
+      return fail(ctx, GFEM_ERR_HIP, "hipblasLtMatmul failed");
+    return GFEM_OK;
   }
   else if(fp32) {
     const float one = 1.f, zero = 0.f;

@@ -147,11 +147,13 @@ struct Group {
   float *d_R32 = nullptr;
   double *d_A64 = nullptr;
   float *d_A32 = nullptr;
-  // the split form (bf16 bits): Rb = [hi(R); lo(R); hi(R)] as 3*7Q rows of
-  // N*N, and per row c of A: [hi(A_c), hi(A_c), lo(A_c)], so one bf16 GEMM
-  // over 3*7Q sums hi*hi + lo*hi + hi*lo with FP32 accumulation
+  // the split form (bf16 bits): Rb [N*N][3*7Q], row `pair` = [hi(R_pair),
+  // lo(R_pair), hi(R_pair)], and Ab [ne*planes][3*7Q], row c = [hi(A_c),
+  // hi(A_c), lo(A_c)], so one bf16 GEMM over 3*7Q sums hi*hi + lo*hi + hi*lo
+  // with FP32 accumulation
   uint16_t *d_Rb = nullptr;
   uint16_t *d_Ab = nullptr;
+  void *lt_plan = nullptr;  // hipBLASLt plan of the split GEMM (gfem_api.cpp)
   double2 *d_be = nullptr;      // this group's element loads inside gfem_system::d_be_all
   int64_t be_off = 0, be_stride = 0;  // offset in d_be_all, distance between right-hand sides
   // maps in storage order on the device (from finalize): numDof-1 and
@@ -169,6 +171,7 @@ struct Group {
 struct gfem_ctx {
   int device = 0;
   void *blas = nullptr;  // rocblas_handle, created at the first GEMM
+  void *blaslt = nullptr;  // hipblasLtHandle_t, created at the first split GEMM
   double *d_xyz = nullptr;
   int64_t n_nodes = 0;
   std::vector<double> h_xyz;
