@@ -534,6 +534,7 @@ struct gfem_batch {
     double tol = 0., bnorm = 0., beta = 0., stop = 0., rel = 0.;
     int max_iter = 0, iters = 0, cycles = 0;
     bool converged = false;
+    bool pending = false;  // submitted; initial residual at the next step
   };
   gfem_ctx *ctx = nullptr;
   int m = 50;
@@ -564,16 +565,6 @@ struct gfem_batch {
 };
 
 namespace {
-
-gfem_status upload_items(gfem_batch *B, hipStream_t s)
-{
-  std::vector<BItem> v(B->slots.size());
-  for(size_t i = 0; i < v.size(); ++i) v[i] = B->slots[i].used ? B->slots[i].item : BItem{};
-  GFEM_HIP(B->ctx, hipMemcpyAsync(B->d_items, v.data(), sizeof(BItem) * v.size(), hipMemcpyHostToDevice, s));
-  // the source is a local vector: wait for the copy
-  GFEM_HIP(B->ctx, hipStreamSynchronize(s));
-  return GFEM_OK;
-}
 
 void drop_graphs(gfem_batch *B)
 {
@@ -880,15 +871,14 @@ gfem_status gfem_batch_submit(gfem_batch *B, gfem_system *sys, double tol, int m
   hipLaunchKernelGGL(bk_round_cols, dim3((unsigned)((nnz + 255) / 256)), dim3(256), 0, s, (int64_t)sys->nnz, a64,
                      sys->d_aj, dinv, a32);
   hipLaunchKernelGGL(bk_zero, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, (int64_t)n, I.x);
+  // this slot's descriptor only, in stream order (no wait)
+  hipLaunchKernelGGL(k_put<BItem>, dim3(1), dim3(64), 0, s, B->d_items + slot, S.item);
   GFEM_HIP(ctx, hipGetLastError());
-  st = upload_items(B, s);
-  if(st != GFEM_OK) return st;
-  // the initial residual (= -b) and |b| of this request only
-  st = build_maps(B, s);
-  if(st != GFEM_OK) return st;
-  std::vector<int> act(B->slots.size(), 0);
-  act[slot] = 1;
-  return residual_and_prepare(B, act, s, true);
+  // block maps, initial residual (= -b) and |b| at the next step, for every
+  // request submitted until then at once
+  S.pending = true;
+  B->maps_dirty = true;
+  return GFEM_OK;
 }
 
 gfem_status gfem_batch_step(gfem_batch *B, void *stream, int *n_live)
@@ -898,18 +888,34 @@ gfem_status gfem_batch_step(gfem_batch *B, void *stream, int *n_live)
   GFEM_HIP(ctx, hipSetDevice(ctx->device));
   hipStream_t s = reinterpret_cast<hipStream_t>(stream);
   const int ns = (int)B->slots.size(), m = B->m;
+  gfem_status st = GFEM_OK;
+  // requests submitted since the last step: initial residual and |b|, all at
+  // once (those with r = 0 end here)
   std::vector<int> act(ns, 0);
-  int live = 0;
+  bool fresh = false;
   for(int i = 0; i < ns; ++i)
-    if(B->slots[i].live) {
+    if(B->slots[i].live && B->slots[i].pending) {
+      B->slots[i].pending = false;
       act[i] = 1;
-      ++live;
+      fresh = true;
     }
+  if(fresh) {
+    if(B->maps_dirty) {
+      st = build_maps(B, s);
+      if(st != GFEM_OK) return st;
+    }
+    st = residual_and_prepare(B, act, s, true);
+    if(st != GFEM_OK) return st;
+  }
+  int live = 0;
+  for(int i = 0; i < ns; ++i) {
+    act[i] = B->slots[i].live ? 1 : 0;
+    live += act[i];
+  }
   if(live == 0) {
     if(n_live) *n_live = 0;
     return GFEM_OK;
   }
-  gfem_status st = GFEM_OK;
   if(B->maps_dirty) {
     st = build_maps(B, s);
     if(st != GFEM_OK) return st;
@@ -1003,6 +1009,10 @@ gfem_status gfem_batch_result(gfem_batch *B, int64_t id, void *x_host, size_t by
     info->rel_residual = S.rel;
     info->converged = S.converged ? 1 : 0;
   }
+  // an unused slot holds an empty descriptor (kernels over every slot skip it)
+  GFEM_HIP(ctx, hipSetDevice(ctx->device));
+  hipLaunchKernelGGL(k_put<BItem>, dim3(1), dim3(64), 0, nullptr, B->d_items + w->second, BItem{});
+  GFEM_HIP(ctx, hipGetLastError());
   B->arena.release(S.off);
   B->where.erase(w);
   S = gfem_batch::Slot();

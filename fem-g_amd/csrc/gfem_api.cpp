@@ -905,8 +905,7 @@ static gfem_status apply_updates(gfem_system *sys, hipStream_t s)
     }
   if(sys->table_dirty) {
     const GroupTable gt = make_table(sys, false);
-    GFEM_HIP(ctx, hipMemcpyAsync(sys->d_table, &gt, sizeof(gt), hipMemcpyHostToDevice, s));
-    GFEM_HIP(ctx, hipStreamSynchronize(s));  // the source lives on this stack frame
+    GFEM_HIP(ctx, launch_put_table(sys->d_table, gt, s));
     sys->table_dirty = false;
   }
   return GFEM_OK;
@@ -1324,10 +1323,8 @@ gfem_status gfem_term_update(gfem_system *sys, int group, int term, double coef_
     sys->table_dirty = true;
   }
   GFEM_HIP(ctx, hipSetDevice(ctx->device));
-  // the host descriptor changes again with the next update: wait for the copy
-  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-  GFEM_HIP(ctx, hipMemcpyAsync(g.d_terms + term, &d, sizeof(DevTerm), hipMemcpyHostToDevice, s));
-  GFEM_HIP(ctx, hipStreamSynchronize(s));
+  // in stream order, without waiting: the descriptor travels as a kernel argument
+  GFEM_HIP(ctx, launch_put_term(g.d_terms + term, d, reinterpret_cast<hipStream_t>(stream)));
   return GFEM_OK;
 }
 

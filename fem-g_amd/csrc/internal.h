@@ -310,6 +310,19 @@ hipError_t launch_ref_products(const Group &g, hipStream_t s);
 hipError_t launch_coef_rows(const Group &g, const double *xyz, bool all_terms, int planes, bool fp32, bool split,
                             hipStream_t s);
 hipError_t launch_ref_split(const Group &g, hipStream_t s);
+// A host descriptor copied into device memory in stream order: the value
+// travels as a kernel argument (at most 4 KiB), so the host copy may change
+// as soon as the launch returns and nothing waits for the stream.
+template <typename T>
+__global__ void __launch_bounds__(64) k_put(T *__restrict__ dst, const T v)
+{
+  static_assert(sizeof(T) % 8 == 0 && alignof(T) >= 8 && sizeof(T) <= 4000, "k_put copies 8-byte words");
+  uint64_t *d = reinterpret_cast<uint64_t *>(dst);
+  const uint64_t *src = reinterpret_cast<const uint64_t *>(&v);
+  for(int i = threadIdx.x; i < (int)(sizeof(T) / 8); i += blockDim.x) d[i] = src[i];
+}
+hipError_t launch_put_term(DevTerm *dst, const DevTerm &v, hipStream_t s);
+hipError_t launch_put_table(GroupTable *dst, const GroupTable &v, hipStream_t s);
 // tensor = 0: the constant-coefficient tensors are left out (every bilinear
 // term is in the per-point element matrices)
 hipError_t launch_gather(const gfem_system &sys, int prec, int tensor, void *vals, hipStream_t s);

@@ -1429,6 +1429,19 @@ hipError_t launch_ref_products(const Group &g, hipStream_t s)
   return hipGetLastError();
 }
 
+// ------------------------------------------------------------------ descriptor updates
+hipError_t launch_put_term(DevTerm *dst, const DevTerm &v, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_put<DevTerm>, dim3(1), dim3(64), 0, s, dst, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_put_table(GroupTable *dst, const GroupTable &v, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_put<GroupTable>, dim3(1), dim3(64), 0, s, dst, v);
+  return hipGetLastError();
+}
+
 hipError_t launch_ref_split(const Group &g, hipStream_t s)
 {
   const int64_t total = (int64_t)7 * g.rules[g.gemm_rule].Q * g.N * g.N;

@@ -343,8 +343,10 @@ gfem_status gfem_batch_create(gfem_ctx *ctx, int restart, size_t hbm_bytes, int 
 gfem_status gfem_batch_destroy(gfem_batch *batch);
 /* HBM one request on `sys` takes from the budget. */
 gfem_status gfem_batch_request_bytes(const gfem_system *sys, int restart, size_t *bytes);
-/* Admit a request: assemble now (current term parameters of `sys`), start
- * solving A x = -b to tol.  GFEM_ERR_BUSY when no slot or memory is free. */
+/* Admit a request: assemble now (current term parameters of `sys`; enqueued
+ * on `stream`, nothing waits), start solving A x = -b to tol at the next
+ * gfem_batch_step, which prepares every request submitted since the last step
+ * at once.  GFEM_ERR_BUSY when no slot or memory is free. */
 gfem_status gfem_batch_submit(gfem_batch *batch, gfem_system *sys, double tol, int max_iter, void *stream,
                               int64_t *id);
 /* One restart cycle of every running request; *n_live = still running. */

@@ -153,3 +153,28 @@ def test_graph_and_eager_batches_agree(monkeypatch):
         assert out["1"][i]["converged"]
         assert out["1"][i]["iterations"] == out["0"][i]["iterations"], i
         assert np.array_equal(out["1"][i]["x"], out["0"][i]["x"]), i
+
+
+def test_uneven_instances_orders_1_2_3():
+    """Uneven instances in one batch: P1, P2 and P3 meshes of different sizes,
+    k from 2 pi to 16 pi, GMRES(50) with the FP32 Krylov iteration and FP64
+    refinement, all admitted at once: every solution within 1e-8 of a direct
+    LU of the oracle's system for its parameters."""
+    from gfem import Server
+    print("case uneven instances P1/P2/P3", flush=True)
+    meshes = [(6, 1), (3, 2), (2, 3), (3, 3)]
+    Fs = [_resident(m) for m in meshes]
+    sv = Server(restart=50, hbm_bytes=1 << 30, max_requests=16)
+    for F in Fs:
+        sv.add_mesh(F)
+    reqs = [(0, 2 * np.pi, (0.5, 0.5, 0.5)), (1, 11.0, (0.35, 0.6, 0.45)), (2, 16 * np.pi, (0.6, 0.4, 0.5)),
+            (3, 7.5, (0.4, 0.4, 0.65)), (2, 3 * np.pi, (0.5, 0.3, 0.5)), (3, 13 * np.pi, (0.55, 0.5, 0.35))]
+    out = sv.run(_schedule(reqs), keep_solutions=True)
+    assert len(out) == len(reqs)
+    assert sv.stats["max_live"] == len(reqs)
+    for r in out:
+        m, k, c = reqs[r["request"]]
+        assert r["converged"], r
+        x_ref, _ = _direct(_spec(meshes[m], k, c))
+        err = np.linalg.norm(r["x"] - x_ref) / np.linalg.norm(x_ref)
+        assert err <= SOL_RTOL, (err, r["request"], meshes[m], k, r["iterations"])
