@@ -1076,6 +1076,12 @@ __global__ void __launch_bounds__(kChunk / S, S == 8 ? 5 : 8)
       w[1] = p >> 16;
     }
   }
+  // the element-list region (all kChunkElems slots, fixed offset) in the same
+  // round of loads as the words: the list's length (header) is not waited for
+  constexpr int kLP = (kChunkElems + B - 1) / B;  // list slots per thread
+  uint32_t lv[kLP];
+#pragma unroll
+  for(int k = 0; k < kLP; ++k) lv[k] = (k * B + t < kChunkElems) ? rec[kList + k * B + t] : 0u;
   const int logP = gt.logP;
   const uint32_t pmask = (1u << logP) - 1u;
   const int ng = gt.ng;
@@ -1088,12 +1094,15 @@ __global__ void __launch_bounds__(kChunk / S, S == 8 ? 5 : 8)
     s_gD[t] = MIXED ? (const R *)gt.dense32[t] : (const R *)gt.dense64[t];
     s_gNN[t] = gt.N[t] * gt.N[t];
   }
-  __syncthreads();
-  // chunk elements: entry j of the list sits in record slot kStride - nel + j
-  for(int j = t; j < (int)nel; j += B) {
-    const uint32_t ev = rec[kStride - (int)nel + j];
-    s_eg[j] = (uint8_t)(ev >> 27);
-    s_el[j] = ev & 0x7FFFFFFu;
+  // chunk elements: entry j of the list sits in record slot kStride - nel + j,
+  // list slot i = kChunkElems - nel + j
+#pragma unroll
+  for(int k = 0; k < kLP; ++k) {
+    const int j = k * B + t - (kChunkElems - (int)nel);
+    if(j >= 0 && k * B + t < kChunkElems) {
+      s_eg[j] = (uint8_t)(lv[k] >> 27);
+      s_el[j] = lv[k] & 0x7FFFFFFu;
+    }
   }
   __syncthreads();
   // element factors: 4 lanes per element, 2 factors each (one 64-byte row)
