@@ -648,6 +648,19 @@ static gfem_status build_maps_host(gfem_system *sys)
   return GFEM_OK;
 }
 
+// every linear term contributes real element loads: a real factor and a real
+// source (constant, or a program that keeps real values real)
+static bool linear_terms_real(const gfem_system *sys)
+{
+  for(const auto &g : sys->groups)
+    for(const DevTerm &t : g.dterms) {
+      if(t.kind != 1) continue;
+      if(t.fac_im != 0.) return false;
+      if(t.per_point ? !t.expr.real_only : t.c_im != 0.) return false;
+    }
+  return true;
+}
+
 gfem_status gfem_system_finalize(gfem_system *sys)
 {
   if(!sys) return fail(nullptr, GFEM_ERR_INVALID_ARG, "system is null");
@@ -805,6 +818,7 @@ gfem_status gfem_system_finalize(gfem_system *sys)
     g.d_be = g.has_linear ? sys->d_be_all + g.be_off : nullptr;
     g.be_stride = sys->be_total;
   }
+  sys->be_real = linear_terms_real(sys);
   {
     const GroupTable gt = make_table(sys, false);
     GFEM_HIP(ctx, dupload(&sys->d_table, &gt, 1));
@@ -1123,7 +1137,7 @@ static gfem_status autotune(gfem_system *sys, int p, int tensor, void *vals, voi
         [&]() {
           for(const auto &g : sys->groups)
             if(g.has_linear) {
-              const hipError_t r = launch_linear(g, sys->d_xyz, sys->nrhs, shift, s);
+              const hipError_t r = launch_linear(g, sys->d_xyz, sys->nrhs, shift, sys->be_real, s);
               if(r != hipSuccess) return r;
             }
           return hipSuccess;
@@ -1250,7 +1264,7 @@ static gfem_status assemble_impl(gfem_system *sys, gfem_precision prec, gfem_str
     for(auto &g : sys->groups)
       if(g.has_linear) {
         prof.begin("linear");
-        GFEM_HIP(ctx, launch_linear(g, sys->d_xyz, sys->nrhs, sys->linear_eb_shift[p], s));
+        GFEM_HIP(ctx, launch_linear(g, sys->d_xyz, sys->nrhs, sys->linear_eb_shift[p], sys->be_real, s));
         prof.end();
       }
     prof.begin("rhs");
@@ -1271,7 +1285,7 @@ static gfem_status assemble_impl(gfem_system *sys, gfem_precision prec, gfem_str
     prof.end();
     if(g.has_linear) {
       prof.begin("linear");
-      GFEM_HIP(ctx, launch_linear(g, sys->d_xyz, sys->nrhs, sys->linear_eb_shift[p], s));
+      GFEM_HIP(ctx, launch_linear(g, sys->d_xyz, sys->nrhs, sys->linear_eb_shift[p], sys->be_real, s));
       prof.end();
     }
     if(g.has_bilinear) {
@@ -1315,6 +1329,16 @@ gfem_status gfem_term_update(gfem_system *sys, int group, int term, double coef_
     d.c_im = coef_im;
     if(d.kind == 0) g.tensor_dirty = true;
   }
+  GFEM_HIP(ctx, hipSetDevice(ctx->device));
+  const bool be_real = linear_terms_real(sys);
+  if(be_real != sys->be_real) {
+    // the element loads change layout: clear them, so right-hand sides a
+    // group has no term for read zeros in the new layout too
+    sys->be_real = be_real;
+    if(sys->d_be_all)
+      GFEM_HIP(ctx, launch_zero(sys->d_be_all, sizeof(double2) * (size_t)(sys->nrhs * sys->be_total),
+                                reinterpret_cast<hipStream_t>(stream)));
+  }
   bool treal = true;
   for(const DevTerm &e : g.dterms)
     if(e.kind == 0 && !e.per_point && (e.c_re * e.fac_im + e.c_im * e.fac_re) != 0.) treal = false;
@@ -1322,7 +1346,6 @@ gfem_status gfem_term_update(gfem_system *sys, int group, int term, double coef_
     g.treal = treal;
     sys->table_dirty = true;
   }
-  GFEM_HIP(ctx, hipSetDevice(ctx->device));
   // in stream order, without waiting: the descriptor travels as a kernel argument
   GFEM_HIP(ctx, launch_put_term(g.d_terms + term, d, reinterpret_cast<hipStream_t>(stream)));
   return GFEM_OK;

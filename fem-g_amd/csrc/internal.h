@@ -202,6 +202,9 @@ struct gfem_system {
   // fused maps
   uint32_t *d_rec = nullptr;  // chunk records, n_chunks x rec_stride
   double2 *d_be_all = nullptr;  // element load vectors of all groups, [nrhs][be_total]
+  // every linear term real (real source or program, real factor): element
+  // loads stored as double at the same flat index (the first half of d_be_all)
+  bool be_real = false;
   int64_t be_total = 0;
   int rec_wide = 0;           // 32-bit contribution words
   // tile choices (autotuned per precision at the first assembly, see
@@ -303,7 +306,7 @@ gfem_status solve_gmres_ir(gfem_system *sys, int rhs_idx, double tol, int restar
 hipError_t launch_basis(const Group &g, Rule &r, hipStream_t s);
 hipError_t launch_tensor(const Group &g, hipStream_t s);
 hipError_t launch_geometry(const Group &g, const double *xyz, hipStream_t s);
-hipError_t launch_linear(const Group &g, const double *xyz, int nrhs, int eb_shift, hipStream_t s);
+hipError_t launch_linear(const Group &g, const double *xyz, int nrhs, int eb_shift, bool be_real, hipStream_t s);
 hipError_t launch_dense(const Group &g, const double *xyz, bool all_terms, int planes, hipStream_t s);
 hipError_t launch_dense_to32(const Group &g, int planes, hipStream_t s);
 hipError_t launch_ref_products(const Group &g, hipStream_t s);
@@ -323,6 +326,8 @@ __global__ void __launch_bounds__(64) k_put(T *__restrict__ dst, const T v)
 }
 hipError_t launch_put_term(DevTerm *dst, const DevTerm &v, hipStream_t s);
 hipError_t launch_put_table(GroupTable *dst, const GroupTable &v, hipStream_t s);
+// zero `bytes` (a multiple of 8) at p, in stream order, with a kernel
+hipError_t launch_zero(void *p, size_t bytes, hipStream_t s);
 // tensor = 0: the constant-coefficient tensors are left out (every bilinear
 // term is in the per-point element matrices)
 hipError_t launch_gather(const gfem_system &sys, int prec, int tensor, void *vals, hipStream_t s);

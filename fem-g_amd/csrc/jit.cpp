@@ -56,7 +56,7 @@ __device__ __forceinline__ double2 cpow_(double2 a, double2 b) {
 struct JitLinArgs {
   double consts[2 * 32];
   double fac_re, fac_im;
-  int Q, dim, nv, N, accumulate, EB;
+  int Q, dim, nv, N, accumulate, EB, real_out;
 };
 )";
 
@@ -86,8 +86,11 @@ gfem_jit_linear(JitLinArgs a, const double *__restrict__ w, const double *__rest
   const double det = F64[e * 8 + 6];
   double2 *o = be + e * N;
   double2 acc[N];
+  const bool ro = a.real_out != 0;  // element loads stored as double (imaginary part 0)
+  const double *orr = reinterpret_cast<const double *>(be) + e * N;
 #pragma unroll
-  for(int i = 0; i < N; ++i) acc[i] = a.accumulate ? o[i] : make_double2(0., 0.);
+  for(int i = 0; i < N; ++i)
+    acc[i] = a.accumulate ? (ro ? make_double2(orr[i], 0.) : o[i]) : make_double2(0., 0.);
   for(int q = 0; q < Q; ++q) {
     double xp[3];
     if(DIM == 1) {
@@ -108,15 +111,28 @@ gfem_jit_linear(JitLinArgs a, const double *__restrict__ w, const double *__rest
     @EXPR@
     const double wd = w[q] * det;
     const double2 c = cmul(make_double2(a.fac_re * wd, a.fac_im * wd), f);
+    if(ro) {
 #pragma unroll
-    for(int i = 0; i < N; ++i) {
-      const double b = bf[q * N + i];
-      acc[i].x += c.x * b;
-      acc[i].y += c.y * b;
+      for(int i = 0; i < N; ++i) acc[i].x += c.x * bf[q * N + i];
+    }
+    else {
+#pragma unroll
+      for(int i = 0; i < N; ++i) {
+        const double b = bf[q * N + i];
+        acc[i].x += c.x * b;
+        acc[i].y += c.y * b;
+      }
     }
   }
+  if(ro) {
+    double *ow = reinterpret_cast<double *>(be) + e * N;
 #pragma unroll
-  for(int i = 0; i < N; ++i) o[i] = acc[i];
+    for(int i = 0; i < N; ++i) ow[i] = acc[i].x;
+  }
+  else {
+#pragma unroll
+    for(int i = 0; i < N; ++i) o[i] = acc[i];
+  }
 }
 )";
 

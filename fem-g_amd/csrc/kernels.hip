@@ -422,7 +422,22 @@ struct LinTerm {
   DevExpr expr;
   double fac_re, fac_im, c_re, c_im;
   int per_point, Q, dim, nv, accumulate;
+  int real_out;  // element loads stored as double (every linear term of the system real)
 };
+
+// element load i of element e: complex, or real (imaginary part 0) in the
+// double layout
+__device__ __forceinline__ double2 be_load(const double2 *be, int real_out, int64_t k)
+{
+  return real_out ? make_double2(reinterpret_cast<const double *>(be)[k], 0.) : be[k];
+}
+__device__ __forceinline__ void be_store(double2 *be, int real_out, int64_t k, double2 v)
+{
+  if(real_out)
+    reinterpret_cast<double *>(be)[k] = v.x;
+  else
+    be[k] = v;
+}
 
 template <int N, bool STAGED>
 __global__ void __launch_bounds__(128) k_linear(LinTerm a, const double *__restrict__ w, const double *__restrict__ pts,
@@ -450,7 +465,7 @@ __global__ void __launch_bounds__(128) k_linear(LinTerm a, const double *__restr
   const double det = F64[e * 8 + 6];
   double2 acc[N];
 #pragma unroll
-  for(int i = 0; i < N; ++i) acc[i] = a.accumulate ? be[e * N + i] : make_double2(0., 0.);
+  for(int i = 0; i < N; ++i) acc[i] = a.accumulate ? be_load(be, a.real_out, e * N + i) : make_double2(0., 0.);
   const double2 fac = make_double2(a.fac_re, a.fac_im), cst = make_double2(a.c_re, a.c_im);
   for(int q = 0; q < Q; ++q) {
     double2 f = cst;
@@ -469,7 +484,7 @@ __global__ void __launch_bounds__(128) k_linear(LinTerm a, const double *__restr
     }
   }
 #pragma unroll
-  for(int i = 0; i < N; ++i) be[e * N + i] = acc[i];
+  for(int i = 0; i < N; ++i) be_store(be, a.real_out, e * N + i, acc[i]);
 }
 
 // Point-parallel form of the same term.  A block takes EB consecutive
@@ -521,14 +536,14 @@ __global__ void __launch_bounds__(256) k_linear_pp(LinTerm a, int N, int EB, con
   for(int k = t; k < nb * N; k += 256) {
     const int le = k / N, i = k - le * N;
     const int64_t o = (e0 + le) * N + i;
-    double2 acc = a.accumulate ? be[o] : make_double2(0., 0.);
+    double2 acc = a.accumulate ? be_load(be, a.real_out, o) : make_double2(0., 0.);
     const double2 *c = &s_c[le * Q];
     for(int q = 0; q < Q; ++q) {
       const double b = s_bf[q * N + i];
       acc.x += c[q].x * b;
       acc.y += c[q].y * b;
     }
-    be[o] = acc;
+    be_store(be, a.real_out, o, acc);
   }
 }
 
@@ -1153,7 +1168,7 @@ __global__ void __launch_bounds__(256) k_rhs(const GroupTable *__restrict__ gtp,
                                              const uint32_t *__restrict__ dc_words, const int32_t *__restrict__ dc_fix,
                                              const double2 *__restrict__ uD, const int64_t *__restrict__ lin_ptr,
                                              const uint32_t *__restrict__ lin_words, const double2 *__restrict__ be_all,
-                                             int64_t be_total, void *__restrict__ out)
+                                             int64_t be_total, int be_real, void *__restrict__ out)
 {
   const GroupTable &gt = *gtp;
   const int64_t row = (xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) / kRhsLanes;
@@ -1178,10 +1193,16 @@ __global__ void __launch_bounds__(256) k_rhs(const GroupTable *__restrict__ gtp,
   const int64_t l0 = lin_ptr[row], l1 = lin_ptr[row + 1];
   for(int r = 0; r < nrhs; ++r) {
     double2 acc = make_double2(0., 0.);
-    for(int64_t k = l0 + l; k < l1; k += kRhsLanes) {
-      const double2 b = be_all[(int64_t)r * be_total + lin_words[k]];
-      acc.x += b.x;
-      acc.y += b.y;
+    if(be_real) {
+      const double *ber = reinterpret_cast<const double *>(be_all) + (int64_t)r * be_total;
+      for(int64_t k = l0 + l; k < l1; k += kRhsLanes) acc.x += ber[lin_words[k]];
+    }
+    else {
+      for(int64_t k = l0 + l; k < l1; k += kRhsLanes) {
+        const double2 b = be_all[(int64_t)r * be_total + lin_words[k]];
+        acc.x += b.x;
+        acc.y += b.y;
+      }
     }
     acc = cadd(lift, lane_sum(acc));
     if(l != 0) continue;
@@ -1227,7 +1248,7 @@ template <bool CPLX>
 __global__ void __launch_bounds__(256) k_ref_rhs(int64_t ne, int N, int nrhs, int64_t n,
                                                  const int32_t *__restrict__ num, const int32_t *__restrict__ type,
                                                  const double *__restrict__ Ae, const double2 *__restrict__ be,
-                                                 int64_t be_stride, const double2 *__restrict__ uD,
+                                                 int64_t be_stride, int be_real, const double2 *__restrict__ uD,
                                                  void *__restrict__ out)
 {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1250,7 +1271,7 @@ __global__ void __launch_bounds__(256) k_ref_rhs(int64_t ne, int N, int nrhs, in
   for(int r = 0; r < nrhs; ++r) {
     double2 v = lift;
     if(be) {
-      const double2 b = be[(int64_t)r * be_stride + e * N + i];
+      const double2 b = be_load(be, be_real, (int64_t)r * be_stride + e * N + i);
       v.x += b.x;
       v.y += b.y;
     }
@@ -1327,7 +1348,7 @@ static void linear_n(const LinTerm &a, const Rule &R, size_t lds, const Group &g
                        g.ne, g.d_vtx, xyz, g.d_F64, be);
 }
 
-hipError_t launch_linear(const Group &g, const double *xyz, int nrhs, int eb_shift, hipStream_t s)
+hipError_t launch_linear(const Group &g, const double *xyz, int nrhs, int eb_shift, bool be_real, hipStream_t s)
 {
   if(g.ne == 0 || !g.has_linear) return hipSuccess;
   for(int r = 0; r < nrhs; ++r) {
@@ -1347,6 +1368,7 @@ hipError_t launch_linear(const Group &g, const double *xyz, int nrhs, int eb_shi
       a.dim = g.dim;
       a.nv = g.nv;
       a.accumulate = first ? 0 : 1;
+      a.real_out = be_real ? 1 : 0;
       // point-parallel kernels: EB elements per block, about 4 points per thread
       const int EB = std::max(1, std::min(128, (256 << eb_shift) / std::max(R.Q, 1)));
       const size_t lds_pp = sizeof(double) * (size_t)(R.Q * g.N + 4 * R.Q + 13 * EB + 1) +
@@ -1360,7 +1382,7 @@ hipError_t launch_linear(const Group &g, const double *xyz, int nrhs, int eb_shi
         struct {
           double consts[2 * GFEM_MAX_CONSTS];
           double fac_re, fac_im;
-          int Q, dim, nv, N, accumulate, EB;
+          int Q, dim, nv, N, accumulate, EB, real_out;
         } ja{};
         for(int k = 0; k < 2 * GFEM_MAX_CONSTS; ++k) ja.consts[k] = t.expr.consts[k];
         ja.fac_re = t.fac_re;
@@ -1371,6 +1393,7 @@ hipError_t launch_linear(const Group &g, const double *xyz, int nrhs, int eb_shi
         ja.N = g.N;
         ja.accumulate = first ? 0 : 1;
         ja.EB = EB;
+        ja.real_out = be_real ? 1 : 0;
         int64_t ne = g.ne;
         const double *pw = R.d_w, *pp = R.d_pts, *pb = R.d_bf, *pF = g.d_F64;
         const int32_t *pv = g.d_vtx;
@@ -1442,6 +1465,21 @@ hipError_t launch_ref_products(const Group &g, hipStream_t s)
 hipError_t launch_put_term(DevTerm *dst, const DevTerm &v, hipStream_t s)
 {
   hipLaunchKernelGGL(k_put<DevTerm>, dim3(1), dim3(64), 0, s, dst, v);
+  return hipGetLastError();
+}
+
+__global__ void __launch_bounds__(256) k_zero(uint64_t *__restrict__ p, int64_t words)
+{
+  for(int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += (int64_t)gridDim.x * blockDim.x)
+    p[i] = 0;
+}
+
+hipError_t launch_zero(void *p, size_t bytes, hipStream_t s)
+{
+  const int64_t words = (int64_t)(bytes / 8);
+  if(words == 0) return hipSuccess;
+  const unsigned grid = (unsigned)std::min<int64_t>((words + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_zero, dim3(grid), dim3(256), 0, s, reinterpret_cast<uint64_t *>(p), words);
   return hipGetLastError();
 }
 
@@ -1536,7 +1574,7 @@ hipError_t launch_rhs(const gfem_system &sys, int prec, int tensor, void *out, h
   if(sys.n == 0) return hipSuccess;
   const GroupTable *gt = tensor ? sys.d_table : sys.d_table_dense;
   const dim3 grid((unsigned)xcd_grid(blocks_for(sys.n * kRhsLanes, 256))), block(256);
-#define GFEM_RHS_ARGS gt, tensor, sys.n, sys.nrhs, sys.d_dc_ptr, sys.d_dc_words, sys.d_dc_fix, sys.d_dc, sys.d_lin_ptr, sys.d_lin_words, sys.d_be_all, sys.be_total, out
+#define GFEM_RHS_ARGS gt, tensor, sys.n, sys.nrhs, sys.d_dc_ptr, sys.d_dc_words, sys.d_dc_fix, sys.d_dc, sys.d_lin_ptr, sys.d_lin_words, sys.d_be_all, sys.be_total, sys.be_real ? 1 : 0, out
   if(sys.cplx) {
     if(prec)
       hipLaunchKernelGGL((k_rhs<true, true>), grid, block, 0, s, GFEM_RHS_ARGS);
@@ -1575,10 +1613,10 @@ hipError_t launch_ref_rhs(const gfem_system &sys, const Group &g, hipStream_t s)
   const double2 *be = g.has_linear ? g.d_be : nullptr;
   if(sys.cplx)
     hipLaunchKernelGGL(k_ref_rhs<true>, dim3(blocks_for(total, 256)), dim3(256), 0, s, g.ne, g.N, sys.nrhs, sys.n,
-                       g.d_num, g.d_type, Ae, be, g.be_stride, sys.d_dc, sys.d_rhs[0]);
+                       g.d_num, g.d_type, Ae, be, g.be_stride, sys.be_real ? 1 : 0, sys.d_dc, sys.d_rhs[0]);
   else
     hipLaunchKernelGGL(k_ref_rhs<false>, dim3(blocks_for(total, 256)), dim3(256), 0, s, g.ne, g.N, sys.nrhs, sys.n,
-                       g.d_num, g.d_type, Ae, be, g.be_stride, sys.d_dc, sys.d_rhs[0]);
+                       g.d_num, g.d_type, Ae, be, g.be_stride, sys.be_real ? 1 : 0, sys.d_dc, sys.d_rhs[0]);
   return hipGetLastError();
 }
 
