@@ -71,7 +71,7 @@ const char *kLinearBody = R"(
 extern "C" __global__ void __launch_bounds__(512)
 gfem_jit_linear(JitLinArgs a, const double *__restrict__ w, const double *__restrict__ pts,
                 const double *__restrict__ bf, i64 ne, const int *__restrict__ vtx,
-                const double *__restrict__ xyz, const double *__restrict__ F64, double2 *__restrict__ be)
+                const double *__restrict__ xyz, const double *__restrict__ F64, void *__restrict__ be)
 {
   constexpr int Q = @Q@, N = @N@, NV = @NV@, DIM = @DIM@;
   const i64 e = (i64)blockIdx.x * blockDim.x + threadIdx.x;
@@ -84,7 +84,7 @@ gfem_jit_linear(JitLinArgs a, const double *__restrict__ w, const double *__rest
     for(int i = 0; i < 3; ++i) X[v][i] = xyz[3 * n + i];
   }
   const double det = F64[e * 8 + 6];
-  double2 *o = be + e * N;
+  double2 *o = reinterpret_cast<double2 *>(be) + e * N;
   double2 acc[N];
   const bool ro = a.real_out != 0;  // element loads stored as double (imaginary part 0)
   const double *orr = reinterpret_cast<const double *>(be) + e * N;

@@ -425,25 +425,26 @@ struct LinTerm {
   int real_out;  // element loads stored as double (every linear term of the system real)
 };
 
-// element load i of element e: complex, or real (imaginary part 0) in the
-// double layout
-__device__ __forceinline__ double2 be_load(const double2 *be, int real_out, int64_t k)
+// element load k after a group's base: complex (double2 units), or real
+// (double units, imaginary part 0)
+__device__ __forceinline__ double2 be_load(const void *be, int real_out, int64_t k)
 {
-  return real_out ? make_double2(reinterpret_cast<const double *>(be)[k], 0.) : be[k];
+  return real_out ? make_double2(reinterpret_cast<const double *>(be)[k], 0.)
+                  : reinterpret_cast<const double2 *>(be)[k];
 }
-__device__ __forceinline__ void be_store(double2 *be, int real_out, int64_t k, double2 v)
+__device__ __forceinline__ void be_store(void *be, int real_out, int64_t k, double2 v)
 {
   if(real_out)
     reinterpret_cast<double *>(be)[k] = v.x;
   else
-    be[k] = v;
+    reinterpret_cast<double2 *>(be)[k] = v;
 }
 
 template <int N, bool STAGED>
 __global__ void __launch_bounds__(128) k_linear(LinTerm a, const double *__restrict__ w, const double *__restrict__ pts,
                                                 const double *__restrict__ bf, int64_t ne,
                                                 const int32_t *__restrict__ vtx, const double *__restrict__ xyz,
-                                                const double *__restrict__ F64, double2 *__restrict__ be)
+                                                const double *__restrict__ F64, void *__restrict__ be)
 {
   extern __shared__ double s_tab[];
   const int Q = a.Q;
@@ -496,7 +497,7 @@ __global__ void __launch_bounds__(256) k_linear_pp(LinTerm a, int N, int EB, con
                                                    const double *__restrict__ pts, const double *__restrict__ bf,
                                                    int64_t ne, const int32_t *__restrict__ vtx,
                                                    const double *__restrict__ xyz, const double *__restrict__ F64,
-                                                   double2 *__restrict__ be)
+                                                   void *__restrict__ be)
 {
   extern __shared__ double s_dyn[];
   const int Q = a.Q, nv = a.nv;
@@ -1247,7 +1248,7 @@ __global__ void __launch_bounds__(256) k_ref_scatter(int64_t total, int NN, cons
 template <bool CPLX>
 __global__ void __launch_bounds__(256) k_ref_rhs(int64_t ne, int N, int nrhs, int64_t n,
                                                  const int32_t *__restrict__ num, const int32_t *__restrict__ type,
-                                                 const double *__restrict__ Ae, const double2 *__restrict__ be,
+                                                 const double *__restrict__ Ae, const void *__restrict__ be,
                                                  int64_t be_stride, int be_real, const double2 *__restrict__ uD,
                                                  void *__restrict__ out)
 {
@@ -1287,6 +1288,16 @@ __global__ void __launch_bounds__(256) k_ref_rhs(int64_t ne, int N, int nrhs, in
 }
 
 inline unsigned blocks_for(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+// a group's element loads of right-hand side r inside gfem_system::d_be_all
+// ([nrhs][be_total], flat index be_off + e N + i): in double2 units, or in
+// double units when every linear term is real (the index k_rhs reads with)
+inline void *group_be(const Group &g, int r, bool be_real)
+{
+  double2 *all = g.d_be - g.be_off;
+  if(be_real) return reinterpret_cast<double *>(all) + (int64_t)r * g.be_stride + g.be_off;
+  return g.d_be + (int64_t)r * g.be_stride;
+}
 
 }  // namespace
 
@@ -1337,7 +1348,7 @@ hipError_t launch_geometry(const Group &g, const double *xyz, hipStream_t s)
 }
 
 template <int N>
-static void linear_n(const LinTerm &a, const Rule &R, size_t lds, const Group &g, const double *xyz, double2 *be,
+static void linear_n(const LinTerm &a, const Rule &R, size_t lds, const Group &g, const double *xyz, void *be,
                      hipStream_t s)
 {
   if(lds)
@@ -1352,7 +1363,7 @@ hipError_t launch_linear(const Group &g, const double *xyz, int nrhs, int eb_shi
 {
   if(g.ne == 0 || !g.has_linear) return hipSuccess;
   for(int r = 0; r < nrhs; ++r) {
-    double2 *be = g.d_be + (int64_t)r * g.be_stride;
+    void *be = group_be(g, r, be_real);
     bool first = true;
     for(const DevTerm &t : g.dterms) {
       if(t.kind != 1 || t.rhs_idx != r) continue;
@@ -1397,7 +1408,7 @@ hipError_t launch_linear(const Group &g, const double *xyz, int nrhs, int eb_shi
         int64_t ne = g.ne;
         const double *pw = R.d_w, *pp = R.d_pts, *pb = R.d_bf, *pF = g.d_F64;
         const int32_t *pv = g.d_vtx;
-        double2 *pbe = be;
+        void *pbe = be;
         void *params[] = {&ja, &pw, &pp, &pb, &ne, &pv, &xyz, &pF, &pbe};
         const unsigned bs = 64u << std::min(std::max(eb_shift, 1), 3);
         const unsigned jblk = (unsigned)((g.ne + bs - 1) / bs);
@@ -1610,7 +1621,7 @@ hipError_t launch_ref_rhs(const gfem_system &sys, const Group &g, hipStream_t s)
   if(g.ne == 0) return hipSuccess;
   const int64_t total = g.ne * (int64_t)g.N;
   const double *Ae = g.has_bilinear ? g.d_dense64 : nullptr;
-  const double2 *be = g.has_linear ? g.d_be : nullptr;
+  const void *be = g.has_linear ? group_be(g, 0, sys.be_real) : nullptr;
   if(sys.cplx)
     hipLaunchKernelGGL(k_ref_rhs<true>, dim3(blocks_for(total, 256)), dim3(256), 0, s, g.ne, g.N, sys.nrhs, sys.n,
                        g.d_num, g.d_type, Ae, be, g.be_stride, sys.be_real ? 1 : 0, sys.d_dc, sys.d_rhs[0]);

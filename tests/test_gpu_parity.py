@@ -291,3 +291,35 @@ def test_autotuned_assembly_matches_untuned(monkeypatch):
     assert_close(F1.values(), F2.values(), 1e-14, "tuned vs untuned A")
     assert_close(F1.rhs(0), F2.rhs(0), 1e-14, "tuned vs untuned b")
     assert_close(F1.values(), P.values(), FP64_RTOL, "tuned A")
+
+
+def test_element_load_layout_follows_term_updates():
+    """Element loads are stored real while every linear term is real, complex
+    otherwise.  Two right-hand sides, each with a source on one group only
+    (triangles for b0, boundary curves for b1): a coefficient update that
+    makes the b0 source complex, then real again, gives the oracle's b0 and
+    b1 each time (a group's missing right-hand side reads zeros in either
+    layout)."""
+    from gfem import ScalarFunction, workloads as W
+    from oracle import oracle as O
+    print("case element-load layout flips", flush=True)
+    spec = W.poisson2d(n=6, order=2)
+    spec.scalar = "c128"
+    spec.dirichlet = []
+    omega, gamma = spec.field_domain, [(1, 1), (1, 2), (1, 3), (1, 4)]
+    spec.terms = [dict(kind="bilinear", domain=omega, gauss=4, form=1),
+                  dict(kind="linear", domain=omega, gauss=4, source=1.0, rhs_idx=0),
+                  dict(kind="linear", domain=gamma, gauss=4, source=2.0, rhs_idx=1)]
+    F = W.build(spec)
+    F.pre()
+    for c in (1.0, 1.0 + 2.0j, 3.0, -0.5j):
+        spec.terms[1]["source"] = c
+        F.set_term_coefficient(1, ScalarFunction(c))
+        F.assemble()
+        P = O.from_spec(spec)
+        P.pre()
+        P.assemble()
+        assert_close(F.values(), P.values(), FP64_RTOL, f"A c={c}")
+        for r in range(2):
+            assert_close(F.rhs(r), P.rhs(r), FP64_RTOL, f"b{r} c={c}")
+        assert np.abs(P.rhs(1)).max() > 0 and np.abs(P.rhs(0)).max() > 0
