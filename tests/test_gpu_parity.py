@@ -315,11 +315,12 @@ def test_element_load_layout_follows_term_updates():
     for c in (1.0, 1.0 + 2.0j, 3.0, -0.5j):
         spec.terms[1]["source"] = c
         F.set_term_coefficient(1, ScalarFunction(c))
-        F.assemble()
         P = O.from_spec(spec)
         P.pre()
         P.assemble()
-        assert_close(F.values(), P.values(), FP64_RTOL, f"A c={c}")
-        for r in range(2):
-            assert_close(F.rhs(r), P.rhs(r), FP64_RTOL, f"b{r} c={c}")
         assert np.abs(P.rhs(1)).max() > 0 and np.abs(P.rhs(0)).max() > 0
+        for strat in ("fused", "reference"):
+            F.assemble(strategy=strat)
+            assert_close(F.values(), P.values(), FP64_RTOL, f"A c={c} {strat}")
+            for r in range(2):
+                assert_close(F.rhs(r), P.rhs(r), FP64_RTOL, f"b{r} c={c} {strat}")
