@@ -1170,7 +1170,21 @@ static gfem_status autotune(gfem_system *sys, int p, int tensor, void *vals, voi
       if(st == GFEM_OK && t[c] < 0.95f * t[0] && t[c] < t[pick]) pick = c;
     sys->gather_slots[p] = cand[pick];
   }
-  (void)rhs;
+  if(st == GFEM_OK && sys->n > 0) {
+    // lanes per RHS row: also an order of sums, so the default (2) stays
+    // unless another is clearly faster
+    float t[3] = {0.f, 0.f, 0.f};
+    const int cand[3] = {2, 4, 8};
+    for(int c = 0; c < 3 && st == GFEM_OK; ++c) {
+      sys->rhs_lanes[p] = cand[c];
+      const hipError_t e = timed([&]() { return launch_rhs(*sys, p, tensor, rhs, s); }, t[c]);
+      if(e != hipSuccess) st = hip_fail(ctx, e, "autotune rhs");
+    }
+    int pick = 0;
+    for(int c = 1; c < 3; ++c)
+      if(st == GFEM_OK && t[c] < 0.95f * t[0] && t[c] < t[pick]) pick = c;
+    sys->rhs_lanes[p] = cand[pick];
+  }
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
   if(st == GFEM_OK) sys->tuned[p] = true;
@@ -1352,13 +1366,14 @@ gfem_status gfem_term_update(gfem_system *sys, int group, int term, double coef_
 }
 
 gfem_status gfem_tuning(const gfem_system *sys, gfem_precision prec, int *gather_slots, int *linear_eb_shift,
-                        int *spmv_lanes)
+                        int *spmv_lanes, int *rhs_lanes)
 {
   if(!sys) return fail(nullptr, GFEM_ERR_INVALID_ARG, "system is null");
   const int p = prec == GFEM_PREC_MIXED ? 1 : 0;
   if(gather_slots) *gather_slots = sys->gather_slots[p];
   if(linear_eb_shift) *linear_eb_shift = sys->linear_eb_shift[p];
   if(spmv_lanes) *spmv_lanes = sys->spmv_lanes;
+  if(rhs_lanes) *rhs_lanes = sys->rhs_lanes[p];
   return GFEM_OK;
 }
 

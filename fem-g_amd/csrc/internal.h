@@ -212,6 +212,7 @@ struct gfem_system {
   // 2 (256); linear-term elements per block ~ (256 << linear_eb_shift) / Q
   int gather_slots[2] = {4, 4};
   int linear_eb_shift[2] = {2, 2};
+  int rhs_lanes[2] = {2, 2};    // k_rhs lanes per row: 2, 4 or 8
   bool tuned[2] = {false, false};
   bool autotune = true;        // GFEM_AUTOTUNE at finalize
   bool dense_gemm = true;      // GFEM_DENSE_GEMM at finalize: GEMM form of per-point terms

@@ -1146,25 +1146,25 @@ __global__ void __launch_bounds__(kChunk / S, S == 8 ? 5 : 8)
 }
 
 // ------------------------------------------------------------------ k_rhs
-// kRhsLanes lanes per row: Dirichlet lifting A_e(i_dof, j_tf) u_D for fixed
-// columns (transposed local index, VectorFactory.cpp:184), then the element
-// load vectors.  Lane l takes the row's terms l, l + kRhsLanes, ... in
-// element order, and the lanes combine in a fixed butterfly, so sums are
-// deterministic (FP64).  Several lanes per row keep few rows in flight per
-// XCD, so the element loads they share stay in that XCD's L2.
-constexpr int kRhsLanes = 8;
-
+// L lanes per row (2, 4 or 8, autotuned): Dirichlet lifting
+// A_e(i_dof, j_tf) u_D for fixed columns (transposed local index,
+// VectorFactory.cpp:184), then the element load vectors.  Lane l takes the
+// row's terms l, l + L, ... in element order, and the lanes combine in a
+// fixed butterfly, so sums are deterministic (FP64) for a given L.  Few lanes
+// give each lane several independent loads in flight (rows have ~7 terms on
+// P2 tetrahedra); more lanes suit long rows (many fixed columns).
+template <int L>
 __device__ __forceinline__ double2 lane_sum(double2 v)
 {
 #pragma unroll
-  for(int d = 1; d < kRhsLanes; d <<= 1) {
-    v.x += __shfl_xor(v.x, d, kRhsLanes);
-    v.y += __shfl_xor(v.y, d, kRhsLanes);
+  for(int d = 1; d < L; d <<= 1) {
+    v.x += __shfl_xor(v.x, d, L);
+    v.y += __shfl_xor(v.y, d, L);
   }
   return v;
 }
 
-template <bool CPLX, bool MIXED>
+template <bool CPLX, bool MIXED, int L>
 __global__ void __launch_bounds__(256) k_rhs(const GroupTable *__restrict__ gtp, int tensor, int64_t n, int nrhs, const int64_t *__restrict__ dc_ptr,
                                              const uint32_t *__restrict__ dc_words, const int32_t *__restrict__ dc_fix,
                                              const double2 *__restrict__ uD, const int64_t *__restrict__ lin_ptr,
@@ -1172,11 +1172,11 @@ __global__ void __launch_bounds__(256) k_rhs(const GroupTable *__restrict__ gtp,
                                              int64_t be_total, int be_real, void *__restrict__ out)
 {
   const GroupTable &gt = *gtp;
-  const int64_t row = (xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) / kRhsLanes;
-  const int l = (int)(threadIdx.x % kRhsLanes);
+  const int64_t row = (xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) / L;
+  const int l = (int)(threadIdx.x % L);
   if(row >= n) return;  // whole lane groups leave together
   double2 lift = make_double2(0., 0.);
-  for(int64_t k = dc_ptr[row] + l; k < dc_ptr[row + 1]; k += kRhsLanes) {
+  for(int64_t k = dc_ptr[row] + l; k < dc_ptr[row + 1]; k += L) {
     const uint32_t w = dc_words[k];
     const int64_t eg = w >> gt.logP;
     const int pair = (int)(w & ((1u << gt.logP) - 1u));
@@ -1190,22 +1190,22 @@ __global__ void __launch_bounds__(256) k_rhs(const GroupTable *__restrict__ gtp,
     }
     lift = cadd(lift, cmul(a, uD[dc_fix[k]]));
   }
-  lift = lane_sum(lift);
+  lift = lane_sum<L>(lift);
   const int64_t l0 = lin_ptr[row], l1 = lin_ptr[row + 1];
   for(int r = 0; r < nrhs; ++r) {
     double2 acc = make_double2(0., 0.);
     if(be_real) {
       const double *ber = reinterpret_cast<const double *>(be_all) + (int64_t)r * be_total;
-      for(int64_t k = l0 + l; k < l1; k += kRhsLanes) acc.x += ber[lin_words[k]];
+      for(int64_t k = l0 + l; k < l1; k += L) acc.x += ber[lin_words[k]];
     }
     else {
-      for(int64_t k = l0 + l; k < l1; k += kRhsLanes) {
+      for(int64_t k = l0 + l; k < l1; k += L) {
         const double2 b = be_all[(int64_t)r * be_total + lin_words[k]];
         acc.x += b.x;
         acc.y += b.y;
       }
     }
-    acc = cadd(lift, lane_sum(acc));
+    acc = cadd(lift, lane_sum<L>(acc));
     if(l != 0) continue;
     const int64_t pos = (int64_t)r * n + row;
     if(MIXED) {
@@ -1580,25 +1580,36 @@ hipError_t launch_gather(const gfem_system &sys, int prec, int tensor, void *out
   return hipGetLastError();
 }
 
-hipError_t launch_rhs(const gfem_system &sys, int prec, int tensor, void *out, hipStream_t s)
+template <int L>
+static void rhs_lanes(const gfem_system &sys, int prec, int tensor, void *out, hipStream_t s)
 {
-  if(sys.n == 0) return hipSuccess;
   const GroupTable *gt = tensor ? sys.d_table : sys.d_table_dense;
-  const dim3 grid((unsigned)xcd_grid(blocks_for(sys.n * kRhsLanes, 256))), block(256);
+  const dim3 grid((unsigned)xcd_grid(blocks_for(sys.n * L, 256))), block(256);
 #define GFEM_RHS_ARGS gt, tensor, sys.n, sys.nrhs, sys.d_dc_ptr, sys.d_dc_words, sys.d_dc_fix, sys.d_dc, sys.d_lin_ptr, sys.d_lin_words, sys.d_be_all, sys.be_total, sys.be_real ? 1 : 0, out
   if(sys.cplx) {
     if(prec)
-      hipLaunchKernelGGL((k_rhs<true, true>), grid, block, 0, s, GFEM_RHS_ARGS);
+      hipLaunchKernelGGL((k_rhs<true, true, L>), grid, block, 0, s, GFEM_RHS_ARGS);
     else
-      hipLaunchKernelGGL((k_rhs<true, false>), grid, block, 0, s, GFEM_RHS_ARGS);
+      hipLaunchKernelGGL((k_rhs<true, false, L>), grid, block, 0, s, GFEM_RHS_ARGS);
   }
   else {
     if(prec)
-      hipLaunchKernelGGL((k_rhs<false, true>), grid, block, 0, s, GFEM_RHS_ARGS);
+      hipLaunchKernelGGL((k_rhs<false, true, L>), grid, block, 0, s, GFEM_RHS_ARGS);
     else
-      hipLaunchKernelGGL((k_rhs<false, false>), grid, block, 0, s, GFEM_RHS_ARGS);
+      hipLaunchKernelGGL((k_rhs<false, false, L>), grid, block, 0, s, GFEM_RHS_ARGS);
   }
 #undef GFEM_RHS_ARGS
+}
+
+hipError_t launch_rhs(const gfem_system &sys, int prec, int tensor, void *out, hipStream_t s)
+{
+  if(sys.n == 0) return hipSuccess;
+  switch(sys.rhs_lanes[prec ? 1 : 0]) {
+  case 2: rhs_lanes<2>(sys, prec, tensor, out, s); break;
+  case 4: rhs_lanes<4>(sys, prec, tensor, out, s); break;
+  case 8: rhs_lanes<8>(sys, prec, tensor, out, s); break;
+  default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 

@@ -162,12 +162,13 @@ void bind_formulation(py::module_ &m, const char *name)
          })
     .def("tuning",
          [](F &f, const std::string &prec) {
-           int gs = 0, eb = 0, lanes = 0;
-           check(gfem_tuning(f.system(), parse_precision(prec), &gs, &eb, &lanes), "gfem_tuning");
+           int gs = 0, eb = 0, lanes = 0, rl = 0;
+           check(gfem_tuning(f.system(), parse_precision(prec), &gs, &eb, &lanes, &rl), "gfem_tuning");
            py::dict d;
            d["gather_slots"] = gs;
            d["linear_eb_shift"] = eb;
            d["spmv_lanes"] = lanes;
+           d["rhs_lanes"] = rl;
            return d;
          },
          py::arg("precision") = "fp64")

@@ -383,9 +383,10 @@ gfem_status gfem_last_kernel_times(const gfem_system *sys, int max, const char *
  * per thread (4, 2 or 8: 128-, 256- or 64-thread blocks per chunk),
  * linear-term block shift (compiled kernels: 64 << shift threads, one per
  * element; interpreted: ~(256 << shift) / Q elements per block), solver
- * SpMV lanes per row (4, 8 or 16).  No counterpart in the reference. */
+ * SpMV lanes per row (4, 8 or 16), RHS lanes per row (2, 4 or 8).  Any
+ * pointer may be NULL.  No counterpart in the reference. */
 gfem_status gfem_tuning(const gfem_system *sys, gfem_precision prec, int *gather_slots, int *linear_eb_shift,
-                        int *spmv_lanes);
+                        int *spmv_lanes, int *rhs_lanes);
 
 /* Algorithmic HBM bytes of the fused matrix gather kernel for `prec`
  * (contribution map + element factors + CSR values written). */

@@ -283,11 +283,12 @@ def test_autotuned_assembly_matches_untuned(monkeypatch):
     for prec in ("fp64", "mixed"):
         t = F1.tuning(prec)
         assert t["gather_slots"] in (2, 4, 8) and t["linear_eb_shift"] in (1, 2, 3), t
+        assert t["rhs_lanes"] in (2, 4, 8), t
     monkeypatch.setenv("GFEM_AUTOTUNE", "0")
     F2 = W.build(spec)
     F2.pre()
     F2.assemble()
-    assert F2.tuning("fp64") == dict(gather_slots=4, linear_eb_shift=2, spmv_lanes=8)
+    assert F2.tuning("fp64") == dict(gather_slots=4, linear_eb_shift=2, spmv_lanes=8, rhs_lanes=2)
     assert_close(F1.values(), F2.values(), 1e-14, "tuned vs untuned A")
     assert_close(F1.rhs(0), F2.rhs(0), 1e-14, "tuned vs untuned b")
     assert_close(F1.values(), P.values(), FP64_RTOL, "tuned A")
