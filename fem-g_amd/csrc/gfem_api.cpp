@@ -1247,10 +1247,10 @@ static gfem_status assemble_impl(gfem_system *sys, gfem_precision prec, gfem_str
       st = ensure_dense_strategy(sys);
       if(st != GFEM_OK) return st;
     }
+    prof.begin("geometry");
+    GFEM_HIP(ctx, launch_geometry_all(*sys, sys->d_xyz, fp32, s));
+    prof.end();
     for(auto &g : sys->groups) {
-      prof.begin("geometry");
-      GFEM_HIP(ctx, launch_geometry(g, sys->d_xyz, s));
-      prof.end();
       if(!(all ? g.has_bilinear : g.has_dense)) continue;
       if(g.gemm_rule >= 0 && (all || sys->dense_gemm)) {
         const bool split = fp32 && sys->gemm_mode == GFEM_GEMM_BF16X3;

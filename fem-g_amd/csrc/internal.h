@@ -307,6 +307,10 @@ gfem_status solve_gmres_ir(gfem_system *sys, int rhs_idx, double tol, int restar
 hipError_t launch_basis(const Group &g, Rule &r, hipStream_t s);
 hipError_t launch_tensor(const Group &g, hipStream_t s);
 hipError_t launch_geometry(const Group &g, const double *xyz, hipStream_t s);
+// every group of the system in one launch (fp32: also the FP32 copy of the
+// factors, read by mixed-precision assemblies only); launch_geometry writes
+// the FP64 factors of one group
+hipError_t launch_geometry_all(const gfem_system &sys, const double *xyz, bool fp32, hipStream_t s);
 hipError_t launch_linear(const Group &g, const double *xyz, int nrhs, int eb_shift, bool be_real, hipStream_t s);
 hipError_t launch_dense(const Group &g, const double *xyz, bool all_terms, int planes, hipStream_t s);
 hipError_t launch_dense_to32(const Group &g, int planes, hipStream_t s);

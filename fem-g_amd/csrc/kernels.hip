@@ -390,12 +390,10 @@ __global__ void k_tensor(TensorArgs a, double2 *__restrict__ T64, float2 *__rest
 }
 
 // ------------------------------------------------------------------ k_geometry
-__global__ void __launch_bounds__(256) k_geometry(int dim, int nv, int64_t ne, const int32_t *__restrict__ vtx,
-                                                  const double *__restrict__ xyz, double *__restrict__ F64,
-                                                  float *__restrict__ F32)
+__device__ __forceinline__ void geometry_one(int dim, int nv, int64_t e, const int32_t *__restrict__ vtx,
+                                             const double *__restrict__ xyz, double *__restrict__ F64,
+                                             float *__restrict__ F32)
 {
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if(e >= ne) return;
   double X[4][3];
   load_vertices(vtx, nv, e, xyz, X);
   Geo g;
@@ -406,10 +404,41 @@ __global__ void __launch_bounds__(256) k_geometry(int dim, int nv, int64_t ne, c
   const double d = g.det;
   const double f[8] = {d * G[0][0], d * G[1][1], d * G[2][2], d * G[0][1], d * G[0][2], d * G[1][2], d, 0.};
   double2 *o2 = reinterpret_cast<double2 *>(&F64[e * 8]);
-  float4 *o4 = reinterpret_cast<float4 *>(&F32[e * 8]);
   for(int k = 0; k < 4; ++k) o2[k] = make_double2(f[2 * k], f[2 * k + 1]);
-  o4[0] = make_float4((float)f[0], (float)f[1], (float)f[2], (float)f[3]);
-  o4[1] = make_float4((float)f[4], (float)f[5], (float)f[6], 0.f);
+  if(F32) {  // the FP32 copy only for mixed-precision assemblies
+    float4 *o4 = reinterpret_cast<float4 *>(&F32[e * 8]);
+    o4[0] = make_float4((float)f[0], (float)f[1], (float)f[2], (float)f[3]);
+    o4[1] = make_float4((float)f[4], (float)f[5], (float)f[6], 0.f);
+  }
+}
+
+__global__ void __launch_bounds__(256) k_geometry(int dim, int nv, int64_t ne, const int32_t *__restrict__ vtx,
+                                                  const double *__restrict__ xyz, double *__restrict__ F64,
+                                                  float *__restrict__ F32)
+{
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(e < ne) geometry_one(dim, nv, e, vtx, xyz, F64, F32);
+}
+
+// every group of a system in one launch: thread t takes element t - off[g]
+// of the group g with off[g] <= t < off[g + 1] (groups are few and
+// contiguous, so the scan is short and uniform within most waves)
+struct GeomGroups {
+  int ng;
+  int dim[kMaxGroups], nv[kMaxGroups];
+  int64_t off[kMaxGroups + 1];
+  const int32_t *vtx[kMaxGroups];
+  double *F64[kMaxGroups];
+  float *F32[kMaxGroups];
+};
+
+__global__ void __launch_bounds__(256) k_geometry_all(const GeomGroups gg, const double *__restrict__ xyz)
+{
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(t >= gg.off[gg.ng]) return;
+  int g = 0;
+  while(g + 1 < gg.ng && t >= gg.off[g + 1]) ++g;
+  geometry_one(gg.dim[g], gg.nv[g], t - gg.off[g], gg.vtx[g], xyz, gg.F64[g], gg.F32[g]);
 }
 
 // ------------------------------------------------------------------ k_linear
@@ -1339,11 +1368,32 @@ hipError_t launch_tensor(const Group &g, hipStream_t s)
   return hipGetLastError();
 }
 
+hipError_t launch_geometry_all(const gfem_system &sys, const double *xyz, bool fp32, hipStream_t s)
+{
+  GeomGroups gg{};
+  gg.ng = (int)sys.groups.size();
+  if(gg.ng == 0) return hipSuccess;
+  if(gg.ng > kMaxGroups) return hipErrorInvalidValue;
+  for(int g = 0; g < gg.ng; ++g) {
+    const Group &G = sys.groups[g];
+    gg.dim[g] = G.dim;
+    gg.nv[g] = G.nv;
+    gg.off[g + 1] = gg.off[g] + G.ne;
+    gg.vtx[g] = G.d_vtx;
+    gg.F64[g] = G.d_F64;
+    gg.F32[g] = fp32 ? G.d_F32 : nullptr;
+  }
+  const int64_t total = gg.off[gg.ng];
+  if(total == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_geometry_all, dim3(blocks_for(total, 256)), dim3(256), 0, s, gg, xyz);
+  return hipGetLastError();
+}
+
 hipError_t launch_geometry(const Group &g, const double *xyz, hipStream_t s)
 {
   if(g.ne == 0) return hipSuccess;
   hipLaunchKernelGGL(k_geometry, dim3(blocks_for(g.ne, 256)), dim3(256), 0, s, g.dim, g.nv, g.ne, g.d_vtx, xyz,
-                     g.d_F64, g.d_F32);
+                     g.d_F64, nullptr);  // FP64 only
   return hipGetLastError();
 }
 
