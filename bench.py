@@ -179,6 +179,28 @@ def serving(args):
     res["one_at_a_time_solves_per_s"] = nb / dt
     res["one_at_a_time_sample"] = nb
     log(f"one at a time: {nb} requests in {dt:.2f} s")
+    # solution error against a direct sparse LU of the FP64 system (the
+    # reference solves with MUMPS LU): requests on the smallest mesh, served
+    # through the same batch loop, to the stream's tolerance
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as sla
+    small = min(range(len(Fs)), key=lambda i: Fs[i].n_unknown)
+    F = Fs[small]
+    chk = Server(restart=30, hbm_bytes=1 << 30, max_requests=4)
+    chk.add_mesh(F)
+    errs = []
+    for s in [s for s in sched if s["mesh"] == small][:3]:
+        r = chk.run([dict(s, mesh=0, arrival=0.0)], keep_solutions=True)[0]
+        for tag, f in s["params"]:
+            F.set_term_coefficient(tag, f)
+        F.assemble()
+        A = sp.csr_matrix((F.values(), F.aj(), F.ai()), shape=(F.n_unknown, F.n_unknown)).tocsc()
+        xd = sla.spsolve(A, -F.rhs(0))
+        errs.append(float(np.linalg.norm(r["x"] - xd) / np.linalg.norm(xd)))
+    res["solution_check"] = {"mesh_dofs": int(F.n_unknown), "requests": len(errs),
+                             "max_rel_err_vs_direct_lu": max(errs) if errs else None}
+    log(f"solution check: {len(errs)} requests on {F.n_unknown} DOFs, max rel err vs direct LU "
+        f"{max(errs) if errs else float('nan'):.2e}")
     return res
 
 
