@@ -90,6 +90,7 @@ void free_group(Group &g)
   delete static_cast<LtPlan *>(g.lt_plan);
   g.lt_plan = nullptr;
   dfree(g.d_vtx);
+  dfree(g.d_conn);
   dfree(g.d_nodes);
   dfree(g.d_terms);
   dfree(g.d_dev);
@@ -177,10 +178,12 @@ GroupTable make_table(const gfem_system *sys, bool dense_strategy)
     gt.N[g] = G.N;
     gt.F64[g] = G.d_F64;
     gt.F32[g] = G.d_F32;
-    gt.T64[g] = G.d_T64;
-    gt.T32[g] = G.d_T32;
-    gt.T64r[g] = G.d_T64r;
-    gt.T32r[g] = G.d_T32r;
+    // curved groups: no reference tensor (every bilinear term is in the
+    // per-point element matrices)
+    gt.T64[g] = G.curved ? nullptr : G.d_T64;
+    gt.T32[g] = G.curved ? nullptr : G.d_T32;
+    gt.T64r[g] = G.curved ? nullptr : G.d_T64r;
+    gt.T32r[g] = G.curved ? nullptr : G.d_T32r;
     gt.treal[g] = G.treal ? 1 : 0;
     const bool dense = dense_strategy ? G.has_bilinear : G.has_dense;
     gt.dense64[g] = dense ? G.d_dense64 : nullptr;
@@ -367,7 +370,8 @@ gfem_status gfem_group_add(gfem_system *sys, int dim, int type, int64_t ne, cons
   g.ne = ne;
   g.e_off = sys->groups.empty() ? 0 : sys->groups.back().e_off + sys->groups.back().ne;
   const int N = g.N;
-  // vertices + affinity check of the high-order nodes
+  // vertices + affinity of the high-order nodes (a curved element makes the
+  // group curved: point-by-point geometry)
   const std::vector<Lat> lat = lattice_nodes(dim, et.order);
   g.h_vtx.resize((size_t)(ne * g.nv));
   const std::vector<double> &X = ctx->h_xyz;
@@ -385,8 +389,7 @@ gfem_status gfem_group_add(gfem_system *sys, int dim, int type, int64_t ne, cons
         double x = 0.;
         for(int v = 0; v < g.nv; ++v) x += lat[k][v] * X[3 * (conn[e * N + v] - 1) + i];
         x /= et.order;
-        if(std::fabs(x - X[3 * (conn[e * N + k] - 1) + i]) > 1e-9 * (h + 1e-300))
-          return fail(ctx, GFEM_ERR_NONAFFINE, "element " + std::to_string(e) + " has a curved high-order node");
+        if(std::fabs(x - X[3 * (conn[e * N + k] - 1) + i]) > 1e-9 * (h + 1e-300)) g.curved = true;
       }
     }
   }
@@ -411,10 +414,13 @@ gfem_status gfem_group_add(gfem_system *sys, int dim, int type, int64_t ne, cons
   g.h_type.resize((size_t)(ne * N));
   if(elem2nnz) g.h_e2n.resize((size_t)(ne * NN));
   g.h_iperm.resize((size_t)ne);
+  if(g.curved) g.h_conn.resize((size_t)(ne * N));
   for(int64_t s = 0; s < ne; ++s) {
     const int64_t e = perm[s];
     g.h_iperm[e] = s;
     for(int v = 0; v < g.nv; ++v) g.h_vtx[s * g.nv + v] = vtx_given[e * g.nv + v];
+    if(g.curved)
+      for(int k = 0; k < N; ++k) g.h_conn[s * N + k] = (int32_t)(conn[e * N + k] - 1);
     for(int i = 0; i < N; ++i) {
       g.h_num[s * N + i] = dof_num[e * N + i];
       g.h_type[s * N + i] = dof_type[e * N + i];
@@ -737,13 +743,13 @@ gfem_status gfem_system_finalize(gfem_system *sys)
         if(g.term_consts[k][2 * q + 1] != 0.) d.expr.real_only = 0;
       for(int q = 0; q < t.n_ops; ++q)
         if(g.term_ops[k][2 * q] == GFEM_OP_SQRT || g.term_ops[k][2 * q] == GFEM_OP_POW) d.expr.real_only = 0;
-      if(t.kind == 0 && d.per_point) g.has_dense = true;
+      if(t.kind == 0 && (d.per_point || g.curved)) g.has_dense = true;
       g.dterms.push_back(d);
     }
     // compile the per-point linear-term coefficient programs once
     g.jit.assign(g.dterms.size(), nullptr);
     for(size_t k = 0; k < g.dterms.size(); ++k)
-      if(g.dterms[k].kind == 1 && g.dterms[k].per_point) {
+      if(g.dterms[k].kind == 1 && g.dterms[k].per_point && !g.curved) {
         std::string log;
         g.jit[k] = jit_linear_kernel(ctx->device, g.dterms[k].expr, g.rules[g.dterms[k].rule].Q, g.N, g.nv, g.dim,
                                      &log);
@@ -797,6 +803,8 @@ gfem_status gfem_system_finalize(gfem_system *sys)
     gd.terms = g.d_terms;
     for(size_t r = 0; r < g.rules.size(); ++r)
       gd.rules[r] = RuleDev{g.rules[r].Q, g.rules[r].d_w, g.rules[r].d_pts, g.rules[r].d_bf, g.rules[r].d_gbf};
+    if(g.curved && !g.d_conn) GFEM_HIP(ctx, dupload(&g.d_conn, g.h_conn.data(), g.h_conn.size()));
+    gd.conn = g.curved ? g.d_conn : nullptr;
     GFEM_HIP(ctx, dupload(&g.d_dev, &gd, 1));
   }
   timer.mark("groups (rules, tensors, jit)");
@@ -1257,7 +1265,7 @@ static gfem_status assemble_impl(gfem_system *sys, gfem_precision prec, gfem_str
         st = ensure_gemm(sys, g, fp32, split);
         if(st != GFEM_OK) return st;
         prof.begin("coef_rows");
-        GFEM_HIP(ctx, launch_coef_rows(g, sys->d_xyz, all, planes, fp32, split, s));
+        GFEM_HIP(ctx, launch_coef_rows(g, sys->d_xyz, all || g.curved, planes, fp32, split, s));
         prof.end();
         prof.begin("element_gemm");
         st = element_gemm(sys, g, fp32, split, planes, s);
@@ -1266,7 +1274,7 @@ static gfem_status assemble_impl(gfem_system *sys, gfem_precision prec, gfem_str
       }
       else {
         prof.begin("dense");
-        GFEM_HIP(ctx, launch_dense(g, sys->d_xyz, all, planes, s));
+        GFEM_HIP(ctx, launch_dense(g, sys->d_xyz, all || g.curved, planes, s));
         if(fp32) GFEM_HIP(ctx, launch_dense_to32(g, planes, s));
         prof.end();
       }

@@ -98,6 +98,9 @@ struct GroupDev {
   int dim, nv, N, nterms;
   const DevTerm *terms;
   RuleDev rules[8];
+  // curved groups: node indices of every element [ne][N] (storage order), for
+  // the isoparametric map at each Gauss point; null for affine groups
+  const int32_t *conn;
 };
 
 struct Rule {
@@ -164,6 +167,13 @@ struct Group {
   int32_t *d_type = nullptr;
   int32_t *d_iperm = nullptr;
   bool host_maps = false;  // h_vtx/h_num/h_type/h_e2n/h_iperm given by the host (until finalize)
+  // curved: some element has a high-order node off the affine image of its
+  // vertices.  Its terms are integrated point by point with the
+  // isoparametric Jacobian (no reference tensor): every bilinear term as
+  // per-point element matrices, linear terms by the interpreted kernel.
+  bool curved = false;
+  std::vector<int32_t> h_conn;  // [ne][N] node indices, storage order (host groups, until finalize)
+  int32_t *d_conn = nullptr;    // [ne][N] node indices, storage order (curved groups)
 };
 
 }  // namespace gfem

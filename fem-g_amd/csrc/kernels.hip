@@ -191,20 +191,16 @@ __device__ __forceinline__ void cross(const double *a, const double *b, double *
   r[2] = a[0] * b[1] - a[1] * b[0];
 }
 
-__device__ void element_geometry(int dim, const double X[4][3], Geo &g)
+// det and Kinv from the first `dim` columns of J (c), completing the missing
+// columns with unit normals
+__device__ void geometry_complete(int dim, Geo &g)
 {
-  for(int j = 0; j < 3; ++j)
-    for(int i = 0; i < 3; ++i) g.c[j][i] = 0.;
   if(dim == 3) {
-    for(int j = 0; j < 3; ++j)
-      for(int i = 0; i < 3; ++i) g.c[j][i] = X[j + 1][i] - X[0][i];
     double n[3];
     cross(g.c[1], g.c[2], n);
     g.det = g.c[0][0] * n[0] + g.c[0][1] * n[1] + g.c[0][2] * n[2];
   }
   else if(dim == 2) {
-    for(int j = 0; j < 2; ++j)
-      for(int i = 0; i < 3; ++i) g.c[j][i] = X[j + 1][i] - X[0][i];
     double n[3];
     cross(g.c[0], g.c[1], n);
     const double a = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
@@ -212,7 +208,6 @@ __device__ void element_geometry(int dim, const double X[4][3], Geo &g)
     g.det = a;
   }
   else {
-    for(int i = 0; i < 3; ++i) g.c[0][i] = 0.5 * (X[1][i] - X[0][i]);
     const double a = sqrt(g.c[0][0] * g.c[0][0] + g.c[0][1] * g.c[0][1] + g.c[0][2] * g.c[0][2]);
     double t[3] = {g.c[0][0] / a, g.c[0][1] / a, g.c[0][2] / a};
     double z[3] = {0., 0., 1.};
@@ -247,6 +242,55 @@ __device__ void element_geometry(int dim, const double X[4][3], Geo &g)
   g.Ki[0][2] = (A[0][1] * A[1][2] - A[0][2] * A[1][1]) * id;
   g.Ki[1][2] = (A[0][2] * A[1][0] - A[0][0] * A[1][2]) * id;
   g.Ki[2][2] = (A[0][0] * A[1][1] - A[0][1] * A[1][0]) * id;
+}
+
+// affine element: J's columns from the vertices (the gmsh line is [-1, 1])
+__device__ void element_geometry(int dim, const double X[4][3], Geo &g)
+{
+  for(int j = 0; j < 3; ++j)
+    for(int i = 0; i < 3; ++i) g.c[j][i] = 0.;
+  for(int j = 0; j < dim; ++j)
+    for(int i = 0; i < 3; ++i) g.c[j][i] = dim == 1 ? 0.5 * (X[1][i] - X[0][i]) : X[j + 1][i] - X[0][i];
+  geometry_complete(dim, g);
+}
+
+// curved element at one Gauss point (isoparametric, the element's own
+// Lagrange basis): c_j = sum_n X_n dpsi_n/du_j, x = sum_n psi_n X_n
+// (gmsh getJacobians at the point), from all N nodes of the element
+__device__ void curved_geometry(int dim, int N, const int32_t *__restrict__ conn, int64_t e,
+                                const double *__restrict__ xyz, const double *__restrict__ bf_q,
+                                const double *__restrict__ gbf_q, Geo &g, double *x)
+{
+  for(int j = 0; j < 3; ++j)
+    for(int i = 0; i < 3; ++i) g.c[j][i] = 0.;
+  x[0] = x[1] = x[2] = 0.;
+  for(int n = 0; n < N; ++n) {
+    const int64_t v = conn[e * N + n];
+    const double X[3] = {xyz[3 * v], xyz[3 * v + 1], xyz[3 * v + 2]};
+    const double b = bf_q[n];
+    for(int i = 0; i < 3; ++i) x[i] += b * X[i];
+    for(int j = 0; j < dim; ++j) {
+      const double d = gbf_q[3 * n + j];
+      for(int i = 0; i < 3; ++i) g.c[j][i] += d * X[i];
+    }
+  }
+  geometry_complete(dim, g);
+}
+
+// the 7 factors det G_00, G_11, G_22, G_01, G_02, G_12, det (G = J^-1 J^-T)
+__device__ __forceinline__ void geometry_factors(const Geo &g, double f[7])
+{
+  double G[3][3];
+  for(int a = 0; a < 3; ++a)
+    for(int b = 0; b < 3; ++b) G[a][b] = g.Ki[0][a] * g.Ki[0][b] + g.Ki[1][a] * g.Ki[1][b] + g.Ki[2][a] * g.Ki[2][b];
+  const double d = g.det;
+  f[0] = d * G[0][0];
+  f[1] = d * G[1][1];
+  f[2] = d * G[2][2];
+  f[3] = d * G[0][1];
+  f[4] = d * G[0][2];
+  f[5] = d * G[1][2];
+  f[6] = d;
 }
 
 __device__ __forceinline__ void load_vertices(const int32_t *__restrict__ vtx, int nv, int64_t e,
@@ -398,11 +442,9 @@ __device__ __forceinline__ void geometry_one(int dim, int nv, int64_t e, const i
   load_vertices(vtx, nv, e, xyz, X);
   Geo g;
   element_geometry(dim, X, g);
-  double G[3][3];
-  for(int a = 0; a < 3; ++a)
-    for(int b = 0; b < 3; ++b) G[a][b] = g.Ki[0][a] * g.Ki[0][b] + g.Ki[1][a] * g.Ki[1][b] + g.Ki[2][a] * g.Ki[2][b];
-  const double d = g.det;
-  const double f[8] = {d * G[0][0], d * G[1][1], d * G[2][2], d * G[0][1], d * G[0][2], d * G[1][2], d, 0.};
+  double f[8];
+  geometry_factors(g, f);
+  f[7] = 0.;
   double2 *o2 = reinterpret_cast<double2 *>(&F64[e * 8]);
   for(int k = 0; k < 4; ++k) o2[k] = make_double2(f[2 * k], f[2 * k + 1]);
   if(F32) {  // the FP32 copy only for mixed-precision assemblies
@@ -452,6 +494,8 @@ struct LinTerm {
   double fac_re, fac_im, c_re, c_im;
   int per_point, Q, dim, nv, accumulate;
   int real_out;  // element loads stored as double (every linear term of the system real)
+  const int32_t *conn;  // curved group: node indices [ne][N] (isoparametric points), else null
+  const double *gbf;    // basis gradients [Q][N][3] for the curved map
 };
 
 // element load k after a group's base: complex (double2 units), or real
@@ -499,12 +543,20 @@ __global__ void __launch_bounds__(128) k_linear(LinTerm a, const double *__restr
   const double2 fac = make_double2(a.fac_re, a.fac_im), cst = make_double2(a.c_re, a.c_im);
   for(int q = 0; q < Q; ++q) {
     double2 f = cst;
-    if(a.per_point) {
+    double dq = det;
+    if(a.conn) {  // curved group: this point's position and det
+      Geo gq;
+      double x[3];
+      curved_geometry(a.dim, N, a.conn, e, xyz, &tb[q * N], &a.gbf[3 * q * N], gq, x);
+      dq = gq.det;
+      if(a.per_point) f = eval_expr(&a.expr, x[0], x[1], x[2]);
+    }
+    else if(a.per_point) {
       double x[3];
       phys_point(a.dim, X, &tp[3 * q], x);
       f = eval_expr(&a.expr, x[0], x[1], x[2]);
     }
-    const double wd = tw[q] * det;
+    const double wd = tw[q] * dq;
     const double2 cf = cmul(make_double2(fac.x * wd, fac.y * wd), f);
 #pragma unroll
     for(int i = 0; i < N; ++i) {
@@ -592,10 +644,11 @@ __global__ void __launch_bounds__(256) k_dense(const GroupDev *__restrict__ gd, 
   const int pair = (int)(t % NN);
   const int i = pair / N, j = pair % N;
   const int dim = gd->dim;
+  const int32_t *conn = gd->conn;  // curved group: geometry at every point
   double X[4][3];
   load_vertices(vtx, gd->nv, e, xyz, X);
   Geo g;
-  element_geometry(dim, X, g);
+  if(!conn) element_geometry(dim, X, g);
   double2 acc = make_double2(0., 0.);
   const int nterms = __builtin_amdgcn_readfirstlane(gd->nterms);
   for(int k = 0; k < nterms; ++k) {
@@ -608,14 +661,18 @@ __global__ void __launch_bounds__(256) k_dense(const GroupDev *__restrict__ gd, 
     const int fd = __builtin_amdgcn_readfirstlane(dt->form_dof);
     double2 sum = make_double2(0., 0.);
     for(int q = 0; q < Q; ++q) {
+      Geo gq;
+      double xq[3];
+      if(conn) curved_geometry(dim, N, conn, e, xyz, &R.bf[q * N], &R.gbf[3 * q * N], gq, xq);
+      const Geo &G = conn ? gq : g;
       double s;
       if(fd == 1) {
         const double *gj = &R.gbf[(q * N + j) * 3];
         const double *gi = &R.gbf[(q * N + i) * 3];
         s = 0.;
         for(int c = 0; c < 3; ++c) {
-          const double ej = g.Ki[c][0] * gj[0] + g.Ki[c][1] * gj[1] + g.Ki[c][2] * gj[2];
-          const double ei = g.Ki[c][0] * gi[0] + g.Ki[c][1] * gi[1] + g.Ki[c][2] * gi[2];
+          const double ej = G.Ki[c][0] * gj[0] + G.Ki[c][1] * gj[1] + G.Ki[c][2] * gj[2];
+          const double ei = G.Ki[c][0] * gi[0] + G.Ki[c][1] * gi[1] + G.Ki[c][2] * gi[2];
           s += ej * ei;
         }
       }
@@ -625,10 +682,16 @@ __global__ void __launch_bounds__(256) k_dense(const GroupDev *__restrict__ gd, 
       double2 c = make_double2(dt->c_re, dt->c_im);
       if(pp) {
         double x[3];
-        phys_point(dim, X, &R.pts[3 * q], x);
+        if(conn) {
+          x[0] = xq[0];
+          x[1] = xq[1];
+          x[2] = xq[2];
+        }
+        else
+          phys_point(dim, X, &R.pts[3 * q], x);
         c = eval_expr(&dt->expr, x[0], x[1], x[2]);
       }
-      const double wd = R.w[q] * g.det * s;
+      const double wd = R.w[q] * G.det * s;
       sum.x += wd * c.x;
       sum.y += wd * c.y;
     }
@@ -748,7 +811,16 @@ __global__ void __launch_bounds__(kCoefBlock) k_coef_rows(const GroupDev *__rest
     const int64_t e = e0 + de;
     const int q = (int)(r - de * (uint32_t)Q);
     double x[3] = {0., 0., 0.};
-    if constexpr(PP) {
+    const double *F = &F64[e * 8];  // det G00, G11, G22, G01, G02, G12, det
+    double Fq[7];
+    if(gd->conn) {  // curved group: this point's own Jacobian and position
+      const int N = gd->N;
+      Geo gq;
+      curved_geometry(gd->dim, N, gd->conn, e, xyz, &R.bf[q * N], &R.gbf[3 * q * N], gq, x);
+      geometry_factors(gq, Fq);
+      F = Fq;
+    }
+    else if constexpr(PP) {
       double X[4][3];
       load_vertices(vtx, gd->nv, e, xyz, X);
       phys_point(gd->dim, X, &R.pts[3 * q], x);
@@ -769,7 +841,6 @@ __global__ void __launch_bounds__(kCoefBlock) k_coef_rows(const GroupDev *__rest
       else
         mass = cadd(mass, v);
     }
-    const double *F = &F64[e * 8];  // det G00, G11, G22, G01, G02, G12, det
     for(int p = 0; p < planes; ++p) {
       const double cs = p == 0 ? stiff.x : stiff.y, cm = p == 0 ? mass.x : mass.y;
 #pragma unroll
@@ -820,7 +891,7 @@ __device__ __forceinline__ double2 elem_value64(const GroupTable &gt, int tensor
   const int g = find_group(gt, eg);
   const int64_t el = eg - gt.e_off[g];
   double2 v = make_double2(0., 0.);
-  if(tensor) {
+  if(tensor && gt.T64[g]) {  // curved groups have no reference tensor
     const double2 *F = reinterpret_cast<const double2 *>(gt.F64[g] + el * 8);
     const double2 *T = gt.T64[g] + (int64_t)pair * 8;
 #pragma unroll
@@ -846,7 +917,7 @@ __device__ __forceinline__ float2 elem_value32(const GroupTable &gt, int tensor,
   const int g = find_group(gt, eg);
   const int64_t el = eg - gt.e_off[g];
   float2 v = make_float2(0.f, 0.f);
-  if(tensor) {
+  if(tensor && gt.T32[g]) {  // curved groups have no reference tensor
     const float4 *F = reinterpret_cast<const float4 *>(gt.F32[g] + el * 8);
     const float4 *T = reinterpret_cast<const float4 *>(gt.T32[g] + (int64_t)pair * 8);
 #pragma unroll
@@ -917,7 +988,7 @@ __device__ __forceinline__ void gather_eval(uint32_t w, int logP, uint32_t pmask
     vr = d[0];
     if(CPLX) vi = d[NN];
   }
-  if(!tensor) return;
+  if(!tensor || !s_gT[g]) return;  // curved groups have no reference tensor
   R F[8];
 #pragma unroll
   for(int q = 0; q < 7; ++q) F[q] = s_F[q][le];  // structure-of-arrays: no bank conflicts across elements
@@ -1430,13 +1501,15 @@ hipError_t launch_linear(const Group &g, const double *xyz, int nrhs, int eb_shi
       a.nv = g.nv;
       a.accumulate = first ? 0 : 1;
       a.real_out = be_real ? 1 : 0;
+      a.conn = g.curved ? g.d_conn : nullptr;  // curved: per-point geometry, interpreted kernel
+      a.gbf = R.d_gbf;
       // point-parallel kernels: EB elements per block, about 4 points per thread
       const int EB = std::max(1, std::min(128, (256 << eb_shift) / std::max(R.Q, 1)));
       const size_t lds_pp = sizeof(double) * (size_t)(R.Q * g.N + 4 * R.Q + 13 * EB + 1) +
                             sizeof(double2) * (size_t)EB * R.Q;
       const unsigned nblk = (unsigned)((g.ne + EB - 1) / EB);
       const size_t ti = (size_t)(&t - g.dterms.data());
-      hipFunction_t jf = ti < g.jit.size() ? g.jit[ti] : nullptr;
+      hipFunction_t jf = (ti < g.jit.size() && !g.curved) ? g.jit[ti] : nullptr;
       if(jf) {
         // compiled coefficient program (jit.cpp), one thread per element in
         // blocks of 64 << eb_shift threads; argument layout of JitLinArgs
@@ -1467,7 +1540,7 @@ hipError_t launch_linear(const Group &g, const double *xyz, int nrhs, int eb_shi
         first = false;
         continue;
       }
-      if(lds_pp <= 64 * 1024) {
+      if(lds_pp <= 64 * 1024 && !g.curved) {
         hipLaunchKernelGGL(k_linear_pp, dim3(nblk), dim3(256), lds_pp, s, a, g.N, EB, R.d_w, R.d_pts, R.d_bf, g.ne,
                            g.d_vtx, xyz, g.d_F64, be);
         first = false;

@@ -1435,7 +1435,8 @@ gfem_status gfem_group_add_pre(gfem_system *sys, const gfem_pre *pre, int pass, 
   const int64_t ne = g.ne;
   Temp tmp;
   SetupTimer timer;
-  // affinity of the high-order nodes
+  // affinity of the high-order nodes: a curved element makes the group
+  // curved (point-by-point geometry from all N nodes)
   if(ne > 0 && g.order > 1) {
     const std::vector<Lat> lat = lattice_nodes(dim, et.order);
     std::vector<int> l(4 * lat.size());
@@ -1451,7 +1452,7 @@ gfem_status gfem_group_add_pre(gfem_system *sys, const gfem_pre *pre, int pass, 
     SETUP_LAUNCH();
     unsigned long long hb = 0;
     SETUP_HIP(to_host(&hb, bad.p));
-    if(hb != ~0ull) return fail(ctx, GFEM_ERR_NONAFFINE, "element " + std::to_string(hb) + " has a curved high-order node");
+    g.curved = hb != ~0ull;
   }
   // storage order: stable by smallest row
   Buf<uint32_t> key, idx, skey, perm;
@@ -1462,6 +1463,7 @@ gfem_status gfem_group_add_pre(gfem_system *sys, const gfem_pre *pre, int pass, 
   gfem_status st = GFEM_OK;
   auto cleanup = [&]() {
     dfree(g.d_vtx);
+    dfree(g.d_conn);
     dfree(g.d_num);
     dfree(g.d_type);
     dfree(g.d_e2n);
@@ -1472,12 +1474,14 @@ gfem_status gfem_group_add_pre(gfem_system *sys, const gfem_pre *pre, int pass, 
   if(e == hipSuccess) e = dalloc(&g.d_type, (size_t)(ne * N));
   if(e == hipSuccess && P.bilinear) e = dalloc(&g.d_e2n, (size_t)(ne * N * N));
   if(e == hipSuccess) e = dalloc(&g.d_iperm, (size_t)ne);
+  if(e == hipSuccess && g.curved) e = dalloc(&g.d_conn, (size_t)(ne * N));
   if(e == hipSuccess && ne > 0) {
     k_group_keys<<<nblk(ne), kTB>>>(P.d_num, P.d_type, ne, N, key.p, idx.p);
     e = hipGetLastError();
     if(e == hipSuccess) e = sort_pairs(tmp, key.p, skey.p, idx.p, perm.p, (uint32_t)ne, 32);
     if(e == hipSuccess) {
       k_gather_rows<<<nblk(ne * g.nv), kTB>>>(perm.p, ne, g.nv, N, B.d_conn, g.d_vtx);
+      if(g.curved) k_gather_rows<<<nblk(ne * N), kTB>>>(perm.p, ne, N, N, B.d_conn, g.d_conn);
       k_gather_rows<<<nblk(ne * N), kTB>>>(perm.p, ne, N, N, P.d_num, g.d_num);
       k_gather_rows<<<nblk(ne * N), kTB>>>(perm.p, ne, N, N, P.d_type, g.d_type);
       if(P.bilinear) k_gather_rows<<<nblk(ne * N * N), kTB>>>(perm.p, ne, N * N, N * N, P.d_e2n, g.d_e2n);

@@ -69,7 +69,7 @@ typedef enum {
   GFEM_ERR_STATE = 6,          /* call order violated (e.g. assemble before finalize) */
   GFEM_ERR_NO_UNKNOWNS = 7,    /* system has no unknown DOF (Formulation.cpp:1179-1184) */
   GFEM_ERR_TOO_LARGE = 8,      /* index does not fit the 32-bit device maps   */
-  GFEM_ERR_NONAFFINE = 9,      /* element is not affine (curved high-order node) */
+  GFEM_ERR_NONAFFINE = 9,      /* reserved: curved elements are accepted (integrated point by point) */
   GFEM_ERR_BUSY = 10           /* batch: no free request slot or memory now; retry after a step */
 } gfem_status;
 

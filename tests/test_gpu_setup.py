@@ -157,23 +157,80 @@ def test_host_groups_on_the_device_builder():
     assert_close(F.rhs(0), P.rhs(0), 1e-12, "b")
 
 
-@pytest.mark.parametrize("device", [False, True])
-def test_curved_element_is_rejected(device):
-    """a P2 edge node moved off its edge: assemble() fails with the
-    non-affine error (the engine's straight-element geometry) on both set-up
-    paths"""
-    from gfem import from_arrays, workloads as W
-    print(f"case curved element device={device}", flush=True)
-    spec = W.poisson2d(n=4, order=2)
+def _curved(spec, amp):
+    """the spec's mesh with every high-order (non-vertex) node moved by the
+    smooth field amp * (sin 2 pi y, sin 2 pi x, sin 2 pi (x + y)): conforming
+    curved elements, boundary edges included"""
+    from gfem import from_arrays
     m = spec.model
     xyz = m.xyz().copy()
     blocks = m.blocks()
-    tri = [b for b in blocks if b["dim"] == 2][0]
-    mid = int(tri["conn"][3, 4]) - 1  # an edge node of element 3
-    xyz[mid, 0] += 0.01
-    spec.model = from_arrays("curved", m.dim, xyz,
+    ho = set()
+    for b in blocks:
+        nv = b["dim"] + 1
+        if b["dim"] > 0 and b["conn"].shape[1] > nv:
+            ho.update(int(t) for t in b["conn"][:, nv:].ravel())
+    i = np.array(sorted(ho), dtype=np.int64) - 1
+    x, y = xyz[i, 0].copy(), xyz[i, 1].copy()
+    xyz[i, 0] += amp * np.sin(2 * np.pi * y)
+    xyz[i, 1] += amp * np.sin(2 * np.pi * x)
+    if m.dim == 3:
+        xyz[i, 2] += amp * np.sin(2 * np.pi * (x + y))
+    spec.model = from_arrays(m.name + "_curved", m.dim, xyz,
                              [dict(tag=b["tag"], type=b["type"], conn=b["conn"]) for b in blocks])
+    return spec
+
+
+def _curved_specs():
+    from gfem import workloads as W, x, y
+    p2 = W.poisson2d(n=6, order=2)
+    p2.terms[0] = dict(kind="bilinear", domain=p2.field_domain, gauss=4, form=1, coef=1.0 + x() * y())
+    p2.terms.append(dict(kind="bilinear", domain=p2.field_domain, gauss=4, form=0, coef=0.5))
+    return {
+        "poisson2d_p2": (p2, 0.008),
+        "poisson2d_p3": (W.poisson2d(n=5, order=3), 0.004),
+        "helmholtz3d_p2": (W.helmholtz3d(n=3, order=2), 0.01),
+    }
+
+
+@pytest.mark.parametrize("device", [False, True])
+@pytest.mark.parametrize("case", ["poisson2d_p2", "poisson2d_p3", "helmholtz3d_p2"])
+def test_curved_elements_match_oracle(case, device):
+    """Curved high-order elements (a high-order node off the affine image of
+    the vertices): J, det and x from all nodes at every Gauss point, as the
+    reference's gmsh getJacobians (Assembler.cpp:130-145).  The engine
+    integrates such groups point by point (per-point element matrices, no
+    reference tensor).  FP64 within 1e-12 of the oracle for the fused, dense
+    and reference strategies; mixed within the FP32 element-matrix bar.  The
+    curving changes the oracle's matrix well beyond that bar."""
+    from gfem import workloads as W
+    from oracle import oracle as O
+    from _helpers import assert_close
+    print(f"case curved {case} device pre={device}", flush=True)
+    spec, amp = _curved_specs()[case]
+    # the curving matters: entrywise against the straight mesh, with the
+    # coordinate-independent numbering (no Hilbert renumbering) on both
+    spec.dofs_sort = "none"
+    P0 = O.from_spec(spec)
+    P0.pre()
+    P0.assemble()
+    spec = _curved(spec, amp)
+    Pn = O.from_spec(spec)
+    Pn.pre()
+    Pn.assemble()
+    d = np.abs(Pn.values() - P0.values()).max() / np.abs(P0.values()).max()
+    assert 1e-3 < d < 0.5, d
+    spec.dofs_sort = "default"
+    P = O.from_spec(spec)
+    P.pre()
+    P.assemble()
+    ref = P.values()
     F = W.build(spec)
     F.pre(device=device)
-    with pytest.raises(RuntimeError, match="curved"):
-        F.assemble()
+    for strat in ("fused", "dense", "reference"):
+        F.assemble(precision="fp64", strategy=strat)
+        assert_close(F.values(), ref, 1e-12, f"{case} A {strat}")
+        assert_close(F.rhs(0), P.rhs(0), 1e-12, f"{case} b {strat}")
+    F.assemble(precision="mixed")
+    assert_close(F.values(), ref, 1e-4, f"{case} A mixed")
+    assert_close(F.rhs(0), P.rhs(0), 1e-4, f"{case} b mixed")
