@@ -193,7 +193,7 @@ __device__ __forceinline__ void cross(const double *a, const double *b, double *
 
 // det and Kinv from the first `dim` columns of J (c), completing the missing
 // columns with unit normals
-__device__ void geometry_complete(int dim, Geo &g)
+__device__ __forceinline__ void geometry_complete(int dim, Geo &g)
 {
   if(dim == 3) {
     double n[3];
@@ -245,32 +245,39 @@ __device__ void geometry_complete(int dim, Geo &g)
 }
 
 // affine element: J's columns from the vertices (the gmsh line is [-1, 1])
-__device__ void element_geometry(int dim, const double X[4][3], Geo &g)
+__device__ __forceinline__ void element_geometry(int dim, const double X[4][3], Geo &g)
 {
+  // constant indices after unrolling (a runtime bound would put c in scratch)
+#pragma unroll
   for(int j = 0; j < 3; ++j)
-    for(int i = 0; i < 3; ++i) g.c[j][i] = 0.;
-  for(int j = 0; j < dim; ++j)
-    for(int i = 0; i < 3; ++i) g.c[j][i] = dim == 1 ? 0.5 * (X[1][i] - X[0][i]) : X[j + 1][i] - X[0][i];
+#pragma unroll
+    for(int i = 0; i < 3; ++i)
+      g.c[j][i] = j >= dim ? 0. : (dim == 1 ? 0.5 * (X[1][i] - X[0][i]) : X[j + 1][i] - X[0][i]);
   geometry_complete(dim, g);
 }
 
 // curved element at one Gauss point (isoparametric, the element's own
 // Lagrange basis): c_j = sum_n X_n dpsi_n/du_j, x = sum_n psi_n X_n
 // (gmsh getJacobians at the point), from all N nodes of the element
-__device__ void curved_geometry(int dim, int N, const int32_t *__restrict__ conn, int64_t e,
-                                const double *__restrict__ xyz, const double *__restrict__ bf_q,
-                                const double *__restrict__ gbf_q, Geo &g, double *x)
+__device__ __forceinline__ void curved_geometry(int dim, int N, const int32_t *__restrict__ conn, int64_t e,
+                                                const double *__restrict__ xyz, const double *__restrict__ bf_q,
+                                                const double *__restrict__ gbf_q, Geo &g, double *x)
 {
+#pragma unroll
   for(int j = 0; j < 3; ++j)
+#pragma unroll
     for(int i = 0; i < 3; ++i) g.c[j][i] = 0.;
   x[0] = x[1] = x[2] = 0.;
   for(int n = 0; n < N; ++n) {
     const int64_t v = conn[e * N + n];
     const double X[3] = {xyz[3 * v], xyz[3 * v + 1], xyz[3 * v + 2]};
     const double b = bf_q[n];
+#pragma unroll
     for(int i = 0; i < 3; ++i) x[i] += b * X[i];
-    for(int j = 0; j < dim; ++j) {
-      const double d = gbf_q[3 * n + j];
+#pragma unroll
+    for(int j = 0; j < 3; ++j) {
+      const double d = j < dim ? gbf_q[3 * n + j] : 0.;  // gradients hold 3 components
+#pragma unroll
       for(int i = 0; i < 3; ++i) g.c[j][i] += d * X[i];
     }
   }
@@ -661,10 +668,12 @@ __global__ void __launch_bounds__(256) k_dense(const GroupDev *__restrict__ gd, 
     const int fd = __builtin_amdgcn_readfirstlane(dt->form_dof);
     double2 sum = make_double2(0., 0.);
     for(int q = 0; q < Q; ++q) {
-      Geo gq;
+      Geo G;
       double xq[3];
-      if(conn) curved_geometry(dim, N, conn, e, xyz, &R.bf[q * N], &R.gbf[3 * q * N], gq, xq);
-      const Geo &G = conn ? gq : g;
+      if(conn)
+        curved_geometry(dim, N, conn, e, xyz, &R.bf[q * N], &R.gbf[3 * q * N], G, xq);
+      else
+        G = g;
       double s;
       if(fd == 1) {
         const double *gj = &R.gbf[(q * N + j) * 3];
@@ -811,19 +820,23 @@ __global__ void __launch_bounds__(kCoefBlock) k_coef_rows(const GroupDev *__rest
     const int64_t e = e0 + de;
     const int q = (int)(r - de * (uint32_t)Q);
     double x[3] = {0., 0., 0.};
-    const double *F = &F64[e * 8];  // det G00, G11, G22, G01, G02, G12, det
-    double Fq[7];
+    double F[7];  // det G00, G11, G22, G01, G02, G12, det
     if(gd->conn) {  // curved group: this point's own Jacobian and position
       const int N = gd->N;
       Geo gq;
       curved_geometry(gd->dim, N, gd->conn, e, xyz, &R.bf[q * N], &R.gbf[3 * q * N], gq, x);
-      geometry_factors(gq, Fq);
-      F = Fq;
+      geometry_factors(gq, F);
     }
-    else if constexpr(PP) {
-      double X[4][3];
-      load_vertices(vtx, gd->nv, e, xyz, X);
-      phys_point(gd->dim, X, &R.pts[3 * q], x);
+    else {
+#pragma unroll
+      for(int s = 0; s < 7; ++s) F[s] = F64[e * 8 + s];
+    }
+    if constexpr(PP) {
+      if(!gd->conn) {
+        double X[4][3];
+        load_vertices(vtx, gd->nv, e, xyz, X);
+        phys_point(gd->dim, X, &R.pts[3 * q], x);
+      }
     }
     double2 stiff = make_double2(0., 0.), mass = make_double2(0., 0.);
     const int nterms = __builtin_amdgcn_readfirstlane(gd->nterms);

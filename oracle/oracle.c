@@ -384,9 +384,166 @@ static void gauss_jacobi01(int n, double alpha, double *x, double *w)
   }
 }
 
+/* Symmetric simplex rules (positive weights, interior points): rows       */
+/* (u, v, w, weight) on the reference triangle (area 1/2) / tetrahedron    */
+/* (volume 1/6), the same table as the engine's refelem.h; pinned by the   */
+/* monomial-exactness test.  Beyond the table: collapsed Gauss-Jacobi.      */
+// 1 point, exact to degree 1
+static const double kTri1[1][4] = {
+  {0.33333333333333331, 0.33333333333333331, 0, 0.5},
+};
+// 3 points, exact to degree 2
+static const double kTri2[3][4] = {
+  {0.16666666666666666, 0.66666666666666674, 0, 0.16666666666666666},
+  {0.66666666666666674, 0.16666666666666666, 0, 0.16666666666666666},
+  {0.16666666666666666, 0.16666666666666666, 0, 0.16666666666666666},
+};
+// 6 points, exact to degree 4
+static const double kTri4[6][4] = {
+  {0.44594849091596472, 0.44594849091596472, 0, 0.11169079483900558},
+  {0.10810301816807055, 0.44594849091596472, 0, 0.11169079483900558},
+  {0.44594849091596472, 0.10810301816807055, 0, 0.11169079483900558},
+  {0.091576213509770979, 0.81684757298045807, 0, 0.054975871827661095},
+  {0.81684757298045807, 0.091576213509770979, 0, 0.054975871827661095},
+  {0.091576213509770979, 0.091576213509770979, 0, 0.054975871827661095},
+};
+// 7 points, exact to degree 5
+static const double kTri5[7][4] = {
+  {0.33333333333333331, 0.33333333333333331, 0, 0.11250000000000335},
+  {0.47014206410511605, 0.47014206410511605, 0, 0.066197076394251611},
+  {0.059715871789767894, 0.47014206410511605, 0, 0.066197076394251611},
+  {0.47014206410511605, 0.059715871789767894, 0, 0.066197076394251611},
+  {0.10128650732345675, 0.79742698535308654, 0, 0.062969590272413958},
+  {0.79742698535308654, 0.10128650732345675, 0, 0.062969590272413958},
+  {0.10128650732345675, 0.10128650732345675, 0, 0.062969590272413958},
+};
+// 12 points, exact to degree 6
+static const double kTri6[12][4] = {
+  {0.24928674517092644, 0.50142650965814717, 0, 0.05839313786317591},
+  {0.50142650965814717, 0.24928674517092644, 0, 0.05839313786317591},
+  {0.24928674517092644, 0.24928674517092644, 0, 0.05839313786317591},
+  {0.31035245103377096, 0.63650249912139989, 0, 0.041425537809195001},
+  {0.63650249912139989, 0.31035245103377096, 0, 0.041425537809195001},
+  {0.053145049844829151, 0.63650249912139989, 0, 0.041425537809195001},
+  {0.63650249912139989, 0.053145049844829151, 0, 0.041425537809195001},
+  {0.053145049844829151, 0.31035245103377096, 0, 0.041425537809195001},
+  {0.31035245103377096, 0.053145049844829151, 0, 0.041425537809195001},
+  {0.063089014491498452, 0.8738219710170031, 0, 0.025422453185100776},
+  {0.8738219710170031, 0.063089014491498452, 0, 0.025422453185100776},
+  {0.063089014491498452, 0.063089014491498452, 0, 0.025422453185100776},
+};
+// 16 points, exact to degree 8
+static const double kTri8[16][4] = {
+  {0.33333333333333331, 0.33333333333333331, 0, 0.07215780383892359},
+  {0.17056930775180987, 0.65886138449638021, 0, 0.051608685267353016},
+  {0.65886138449638021, 0.17056930775180987, 0, 0.051608685267353016},
+  {0.17056930775180987, 0.17056930775180987, 0, 0.051608685267353016},
+  {0.45929258829276531, 0.45929258829276531, 0, 0.047545817133622131},
+  {0.081414823414469373, 0.45929258829276531, 0, 0.047545817133622131},
+  {0.45929258829276531, 0.081414823414469373, 0, 0.047545817133622131},
+  {0.050547228317028847, 0.89890554336594231, 0, 0.016229248811595872},
+  {0.89890554336594231, 0.050547228317028847, 0, 0.016229248811595872},
+  {0.050547228317028847, 0.050547228317028847, 0, 0.016229248811595872},
+  {0.26311282963448862, 0.72849239295548485, 0, 0.013615157087227228},
+  {0.72849239295548485, 0.26311282963448862, 0, 0.013615157087227228},
+  {0.0083947774100264407, 0.72849239295548485, 0, 0.013615157087227228},
+  {0.72849239295548485, 0.0083947774100264407, 0, 0.013615157087227228},
+  {0.0083947774100264407, 0.26311282963448862, 0, 0.013615157087227228},
+  {0.26311282963448862, 0.0083947774100264407, 0, 0.013615157087227228},
+};
+// 1 point, exact to degree 1
+static const double kTet1[1][4] = {
+  {0.25, 0.25, 0.25, 0.16666666666666666},
+};
+// 4 points, exact to degree 2
+static const double kTet2[4][4] = {
+  {0.1381966011250105, 0.1381966011250105, 0.58541019662496852, 0.041666666666666664},
+  {0.1381966011250105, 0.58541019662496852, 0.1381966011250105, 0.041666666666666664},
+  {0.58541019662496852, 0.1381966011250105, 0.1381966011250105, 0.041666666666666664},
+  {0.1381966011250105, 0.1381966011250105, 0.1381966011250105, 0.041666666666666664},
+};
+// 14 points, exact to degree 5
+static const double kTet5[14][4] = {
+  {0.31088591926330045, 0.31088591926330045, 0.31088591926330045, 0.018781320953002337},
+  {0.067342242210098657, 0.31088591926330045, 0.31088591926330045, 0.018781320953002337},
+  {0.31088591926330045, 0.067342242210098657, 0.31088591926330045, 0.018781320953002337},
+  {0.31088591926330045, 0.31088591926330045, 0.067342242210098657, 0.018781320953002337},
+  {0.092735250310890707, 0.092735250310890707, 0.72179424906732792, 0.012248840519393511},
+  {0.092735250310890707, 0.72179424906732792, 0.092735250310890707, 0.012248840519393511},
+  {0.72179424906732792, 0.092735250310890707, 0.092735250310890707, 0.012248840519393511},
+  {0.092735250310890707, 0.092735250310890707, 0.092735250310890707, 0.012248840519393511},
+  {0.045503704125651745, 0.45449629587434826, 0.45449629587434826, 0.007091003462847207},
+  {0.45449629587434826, 0.045503704125651745, 0.45449629587434826, 0.007091003462847207},
+  {0.45449629587434826, 0.45449629587434826, 0.045503704125651745, 0.007091003462847207},
+  {0.045503704125651745, 0.045503704125651745, 0.45449629587434826, 0.007091003462847207},
+  {0.045503704125651745, 0.45449629587434826, 0.045503704125651745, 0.007091003462847207},
+  {0.45449629587434826, 0.045503704125651745, 0.045503704125651745, 0.007091003462847207},
+};
+// 24 points, exact to degree 6
+static const double kTet6[24][4] = {
+  {0.32233789014227748, 0.32233789014227748, 0.32233789014227748, 0.0092261969239417659},
+  {0.032986329573167605, 0.32233789014227748, 0.32233789014227748, 0.0092261969239417659},
+  {0.32233789014227748, 0.032986329573167605, 0.32233789014227748, 0.0092261969239417659},
+  {0.32233789014227748, 0.32233789014227748, 0.032986329573167605, 0.0092261969239417659},
+  {0.063661001875016499, 0.26967233145831676, 0.60300566479165019, 0.0080357142857140525},
+  {0.063661001875016499, 0.60300566479165019, 0.26967233145831676, 0.0080357142857140525},
+  {0.26967233145831676, 0.063661001875016499, 0.60300566479165019, 0.0080357142857140525},
+  {0.26967233145831676, 0.60300566479165019, 0.063661001875016499, 0.0080357142857140525},
+  {0.60300566479165019, 0.063661001875016499, 0.26967233145831676, 0.0080357142857140525},
+  {0.60300566479165019, 0.26967233145831676, 0.063661001875016499, 0.0080357142857140525},
+  {0.063661001875016499, 0.063661001875016499, 0.60300566479165019, 0.0080357142857140525},
+  {0.063661001875016499, 0.60300566479165019, 0.063661001875016499, 0.0080357142857140525},
+  {0.60300566479165019, 0.063661001875016499, 0.063661001875016499, 0.0080357142857140525},
+  {0.063661001875016499, 0.063661001875016499, 0.26967233145831676, 0.0080357142857140525},
+  {0.063661001875016499, 0.26967233145831676, 0.063661001875016499, 0.0080357142857140525},
+  {0.26967233145831676, 0.063661001875016499, 0.063661001875016499, 0.0080357142857140525},
+  {0.21460287125914035, 0.21460287125914035, 0.35619138622257895, 0.0066537917096959063},
+  {0.21460287125914035, 0.35619138622257895, 0.21460287125914035, 0.0066537917096959063},
+  {0.35619138622257895, 0.21460287125914035, 0.21460287125914035, 0.0066537917096959063},
+  {0.21460287125914035, 0.21460287125914035, 0.21460287125914035, 0.0066537917096959063},
+  {0.040673958534612614, 0.040673958534612614, 0.87797812439616219, 0.0016795351758868431},
+  {0.040673958534612614, 0.87797812439616219, 0.040673958534612614, 0.0016795351758868431},
+  {0.87797812439616219, 0.040673958534612614, 0.040673958534612614, 0.0016795351758868431},
+  {0.040673958534612614, 0.040673958534612614, 0.040673958534612614, 0.0016795351758868431},
+};
+
+static const double (*simplex_table(int dim, int deg, int *q))[4]
+{
+  *q = 0;
+  if(dim == 2) {
+    if(deg <= 1) { *q = 1; return kTri1; }
+    if(deg == 2) { *q = 3; return kTri2; }
+    if(deg <= 4) { *q = 6; return kTri4; }
+    if(deg == 5) { *q = 7; return kTri5; }
+    if(deg == 6) { *q = 12; return kTri6; }
+    if(deg <= 8) { *q = 16; return kTri8; }
+  }
+  if(dim == 3) {
+    if(deg <= 1) { *q = 1; return kTet1; }
+    if(deg == 2) { *q = 4; return kTet2; }
+    if(deg <= 5) { *q = 14; return kTet5; }
+    if(deg == 6) { *q = 24; return kTet6; }
+  }
+  return NULL;
+}
+
 int or_gauss_rule(int dim, int deg, double *pts, double *w, int maxq)
 {
   if(deg < 0) deg = 0;
+  {
+    int nq = 0;
+    const double(*t)[4] = simplex_table(dim, deg, &nq);
+    if(t) {
+      if(nq > maxq) return -1;
+      for(int k = 0; k < nq; ++k) {
+        pts[3 * k] = t[k][0];
+        pts[3 * k + 1] = t[k][1];
+        pts[3 * k + 2] = t[k][2];
+        w[k] = t[k][3];
+      }
+      return nq;
+    }
+  }
   const int n = (deg + 2) / 2; /* ceil((deg+1)/2) */
   if(n > 16) return -1;
   double x[3][16], wx[3][16];

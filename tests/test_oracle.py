@@ -27,16 +27,43 @@ def _simplex_monomial(dim, a):
 
 
 @pytest.mark.parametrize("dim", [1, 2, 3])
-@pytest.mark.parametrize("deg", [0, 1, 2, 3, 4, 5, 8, 9])
+@pytest.mark.parametrize("deg", [0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 12])
 def test_gauss_rule_exactness(dim, deg):
     pts, w = O.gauss_rule(dim, deg)
     assert np.all(w > 0)
+    # symmetric table rules: every point strictly inside the element
+    if dim > 1:
+        assert np.all(pts[:, :dim] > 0) and np.all(pts[:, :dim].sum(axis=1) < 1)
     for a in product(range(deg + 1), repeat=dim):
         if sum(a) > deg:
             continue
         a3 = list(a) + [0] * (3 - dim)
         val = np.sum(w * pts[:, 0] ** a3[0] * pts[:, 1] ** a3[1] * pts[:, 2] ** a3[2])
         assert abs(val - _simplex_monomial(dim, a3)) < 1e-14, (dim, deg, a)
+
+
+@pytest.mark.parametrize("dim", [1, 2, 3])
+@pytest.mark.parametrize("deg", [0, 1, 2, 3, 4, 5, 6, 7, 8, 9, 12])
+def test_engine_rules_are_the_oracle_rules(dim, deg):
+    """the engine's GaussN (csrc/refelem.h) and the oracle's are the same
+    points and weights in the same order (symmetric table rules: triangles to
+    degree 8 with 1/3/6/7/12/16 points, tetrahedra to degree 6 with
+    1/4/14/24 points; collapsed Gauss-Jacobi beyond)"""
+    from gfem import gauss_rule
+    pe, we = gauss_rule(dim, deg)
+    po, wo = O.gauss_rule(dim, deg)
+    pe, we = np.asarray(pe, dtype=np.float64).reshape(-1, 3), np.asarray(we, dtype=np.float64)
+    assert pe.shape == po.shape
+    expect = {2: [1, 1, 3, 6, 6, 7, 12, 16, 16], 3: [1, 1, 4, 14, 14, 14, 24]}
+    if dim in expect and deg < len(expect[dim]):
+        # the shared table: bit for bit
+        assert len(wo) == expect[dim][deg]
+        np.testing.assert_array_equal(pe, po)
+        np.testing.assert_array_equal(we, wo)
+    else:
+        # computed rules (Gauss-Jacobi nodes by each side's own eigen-solver)
+        np.testing.assert_allclose(pe, po, rtol=0, atol=1e-14)
+        np.testing.assert_allclose(we, wo, rtol=1e-13, atol=0)
 
 
 # ---------------------------------------------------------------- Lagrange basis
