@@ -904,8 +904,16 @@ common::Timer Formulation<T>::assemble(const AssembleOptions &opt)
   common::Timer time;
   time.tick();
   _lastError.clear();
+  if(_nUnknown == 0) {
+    // Formulation.cpp:1179-1184: a warning, nothing assembled (A and every b
+    // stay empty), no error
+    std::fprintf(stderr, "Warning : There are no unknown dofs to assemble. Did you forget to pre-process the formulation?\n");
+    _values.clear();
+    _rhs.assign(_numRHS, std::vector<T>());
+    time.tock();
+    return time;
+  }
   try {
-    if(_ai.empty()) throw common::Exception("there are no unknown dofs to assemble; did you forget to pre-process?");
     if(!_sys) {
       try {
         upload();

@@ -325,3 +325,42 @@ def test_element_load_layout_follows_term_updates():
             assert_close(F.values(), P.values(), FP64_RTOL, f"A c={c} {strat}")
             for r in range(2):
                 assert_close(F.rhs(r), P.rhs(r), FP64_RTOL, f"b{r} c={c} {strat}")
+
+
+def _degenerate_specs():
+    from gfem import workloads as W
+    allfixed = W.poisson2d(n=1, order=1)            # 2 triangles, every node on the Dirichlet boundary
+    linonly = W.poisson2d(n=4, order=2)             # a source and no bilinear term: no nonzeros
+    linonly.dirichlet = []
+    linonly.terms = [linonly.terms[1]]
+    bilonly = W.poisson2d(n=4, order=2)             # no source: b is the Dirichlet lift alone
+    bilonly.dirichlet = [(bilonly.dirichlet[0][0], 2.0)]
+    bilonly.terms = [bilonly.terms[0]]
+    return {"all_fixed": allfixed, "linear_only": linonly, "bilinear_only": bilonly}
+
+
+@pytest.mark.parametrize("device", [False, True])
+@pytest.mark.parametrize("case", ["all_fixed", "linear_only", "bilinear_only"])
+def test_degenerate_systems_match_oracle(case, device):
+    """Edge cases of the assembly: no unknown DOF at all (empty A and b), a
+    formulation without bilinear terms (empty pattern, b only), and one
+    without linear terms (b = the Dirichlet lift).  Every strategy and the
+    mixed path give the oracle's (empty) arrays."""
+    from oracle import oracle as O
+    from gfem import workloads as W
+    print(f"case degenerate {case} device pre={device}", flush=True)
+    spec = _degenerate_specs()[case]
+    P = O.from_spec(spec)
+    P.pre()
+    P.assemble()
+    F = W.build(spec)
+    F.pre(device=device)
+    assert (F.n_unknown, F.nnz) == (P.n_unknown, len(P.values()))
+    for strat in ("fused", "reference", "dense"):
+        F.assemble(strategy=strat)
+        assert F.values().shape == P.values().shape and F.rhs(0).shape == P.rhs(0).shape
+        assert_close(F.values(), P.values(), FP64_RTOL, f"{case} A {strat}")
+        assert_close(F.rhs(0), P.rhs(0), FP64_RTOL, f"{case} b {strat}")
+    F.assemble(precision="mixed")
+    assert_close(F.values(), P.values(), MIXED_RTOL, f"{case} A mixed")
+    assert_close(F.rhs(0), P.rhs(0), MIXED_RTOL, f"{case} b mixed")
