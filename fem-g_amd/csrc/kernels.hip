@@ -481,13 +481,51 @@ struct GeomGroups {
   float *F32[kMaxGroups];
 };
 
+// A block whose elements all belong to one group stages its factors in LDS
+// and writes them as one contiguous run (each store instruction then covers
+// whole cache lines, where a thread-per-element store of a 64-byte record
+// touches 64 lines); a block that straddles two groups stores directly.
 __global__ void __launch_bounds__(256) k_geometry_all(const GeomGroups gg, const double *__restrict__ xyz)
 {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if(t >= gg.off[gg.ng]) return;
-  int g = 0;
-  while(g + 1 < gg.ng && t >= gg.off[g + 1]) ++g;
-  geometry_one(gg.dim[g], gg.nv[g], t - gg.off[g], gg.vtx[g], xyz, gg.F64[g], gg.F32[g]);
+  constexpr int kB = 256, kPad = 5;  // double2 per element in LDS (4 + 1 padding)
+  __shared__ double2 s_f[kB * kPad];
+  const int64_t total = gg.off[gg.ng];
+  const int64_t t0 = (int64_t)blockIdx.x * kB;
+  const int64_t t = t0 + threadIdx.x;
+  int g0 = 0;
+  while(g0 + 1 < gg.ng && t0 >= gg.off[g0 + 1]) ++g0;
+  const int64_t tend = min(t0 + kB, total);
+  if(tend > gg.off[g0 + 1]) {  // uniform across the block
+    if(t >= total) return;
+    int g = g0;
+    while(g + 1 < gg.ng && t >= gg.off[g + 1]) ++g;
+    geometry_one(gg.dim[g], gg.nv[g], t - gg.off[g], gg.vtx[g], xyz, gg.F64[g], gg.F32[g]);
+    return;
+  }
+  if(t < tend) {
+    const int64_t e = t - gg.off[g0];
+    double X[4][3];
+    load_vertices(gg.vtx[g0], gg.nv[g0], e, xyz, X);
+    Geo g;
+    element_geometry(gg.dim[g0], X, g);
+    double f[8];
+    geometry_factors(g, f);
+    f[7] = 0.;
+#pragma unroll
+    for(int k = 0; k < 4; ++k) s_f[threadIdx.x * kPad + k] = make_double2(f[2 * k], f[2 * k + 1]);
+  }
+  __syncthreads();
+  const int cnt = (int)(tend - t0);
+  const int64_t e0 = t0 - gg.off[g0];
+  double2 *o2 = reinterpret_cast<double2 *>(gg.F64[g0] + e0 * 8);
+  for(int p = threadIdx.x; p < 4 * cnt; p += kB) o2[p] = s_f[(p >> 2) * kPad + (p & 3)];
+  if(gg.F32[g0]) {  // the FP32 copy only for mixed-precision assemblies
+    float4 *o4 = reinterpret_cast<float4 *>(gg.F32[g0] + e0 * 8);
+    for(int p = threadIdx.x; p < 2 * cnt; p += kB) {
+      const double2 a = s_f[(p >> 1) * kPad + 2 * (p & 1)], b = s_f[(p >> 1) * kPad + 2 * (p & 1) + 1];
+      o4[p] = make_float4((float)a.x, (float)a.y, (float)b.x, (float)b.y);
+    }
+  }
 }
 
 // ------------------------------------------------------------------ k_linear
