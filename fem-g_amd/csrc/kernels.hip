@@ -1343,17 +1343,35 @@ __global__ void __launch_bounds__(256) k_rhs(const GroupTable *__restrict__ gtp,
   }
   lift = lane_sum<L>(lift);
   const int64_t l0 = lin_ptr[row], l1 = lin_ptr[row + 1];
+  // kU terms of the lane at a time: their words, then their element loads,
+  // are all in flight together; the sum still runs in term order
+  constexpr int kU = 4;
   for(int r = 0; r < nrhs; ++r) {
     double2 acc = make_double2(0., 0.);
-    if(be_real) {
-      const double *ber = reinterpret_cast<const double *>(be_all) + (int64_t)r * be_total;
-      for(int64_t k = l0 + l; k < l1; k += L) acc.x += ber[lin_words[k]];
-    }
-    else {
-      for(int64_t k = l0 + l; k < l1; k += L) {
-        const double2 b = be_all[(int64_t)r * be_total + lin_words[k]];
-        acc.x += b.x;
-        acc.y += b.y;
+    for(int64_t k0 = l0 + l; k0 < l1; k0 += kU * L) {
+      uint32_t wv[kU];
+#pragma unroll
+      for(int u = 0; u < kU; ++u) wv[u] = k0 + u * L < l1 ? lin_words[k0 + u * L] : 0u;
+      if(be_real) {
+        const double *ber = reinterpret_cast<const double *>(be_all) + (int64_t)r * be_total;
+        double v[kU];
+#pragma unroll
+        for(int u = 0; u < kU; ++u) v[u] = k0 + u * L < l1 ? ber[wv[u]] : 0.;
+#pragma unroll
+        for(int u = 0; u < kU; ++u)
+          if(k0 + u * L < l1) acc.x += v[u];
+      }
+      else {
+        double2 v[kU];
+#pragma unroll
+        for(int u = 0; u < kU; ++u)
+          v[u] = k0 + u * L < l1 ? be_all[(int64_t)r * be_total + wv[u]] : make_double2(0., 0.);
+#pragma unroll
+        for(int u = 0; u < kU; ++u)
+          if(k0 + u * L < l1) {
+            acc.x += v[u].x;
+            acc.y += v[u].y;
+          }
       }
     }
     acc = cadd(lift, lane_sum<L>(acc));
