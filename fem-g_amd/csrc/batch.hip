@@ -103,10 +103,27 @@ __global__ void __launch_bounds__(256) bk_spmv32(const BItem *__restrict__ items
   if(row >= I.n) return;
   const float2 *x = I.V + (int64_t)j * I.n;
   float sr = 0.f, si = 0.f;
-  for(int64_t k = I.ai[row] + l; k < I.ai[row + 1]; k += kLanes) {
-    const float2 a = I.a32[k], v = x[I.aj[k]];
-    sr += a.x * v.x - a.y * v.y;
-    si += a.x * v.y + a.y * v.x;
+  // kU terms of the lane at a time (columns and values, then the x entries,
+  // in flight together), summed in term order
+  constexpr int kU = 4;
+  const int64_t k1 = I.ai[row + 1];
+  for(int64_t k0 = I.ai[row] + l; k0 < k1; k0 += kU * kLanes) {
+    int32_t c[kU];
+    float2 a[kU], v[kU];
+#pragma unroll
+    for(int u = 0; u < kU; ++u) {
+      const bool in = k0 + u * kLanes < k1;
+      c[u] = in ? I.aj[k0 + u * kLanes] : 0;
+      a[u] = in ? I.a32[k0 + u * kLanes] : make_float2(0.f, 0.f);
+    }
+#pragma unroll
+    for(int u = 0; u < kU; ++u) v[u] = k0 + u * kLanes < k1 ? x[c[u]] : make_float2(0.f, 0.f);
+#pragma unroll
+    for(int u = 0; u < kU; ++u)
+      if(k0 + u * kLanes < k1) {
+        sr += a[u].x * v[u].x - a[u].y * v[u].y;
+        si += a[u].x * v[u].y + a[u].y * v[u].x;
+      }
   }
   const double2 s = lanes_sum(make_double2(sr, si));
   if(l == 0) I.w[row] = make_float2((float)s.x, (float)s.y);

@@ -85,10 +85,27 @@ __global__ void __launch_bounds__(256) k_spmv32(int64_t n, const int64_t *__rest
   const int l = (int)(threadIdx.x % L);
   if(row >= n) return;  // whole lane groups leave together
   float sr = 0.f, si = 0.f;
-  for(int64_t k = ai[row] + l; k < ai[row + 1]; k += L) {
-    const double2 av = ld_d(a, k), xv = ld_d(x, aj[k]);
-    sr += (float)av.x * (float)xv.x - (float)av.y * (float)xv.y;
-    si += (float)av.x * (float)xv.y + (float)av.y * (float)xv.x;
+  // kU terms of the lane at a time: columns and values, then the x entries,
+  // are in flight together; the products are still summed in term order
+  constexpr int kU = 4;
+  const int64_t k1 = ai[row + 1];
+  for(int64_t k0 = ai[row] + l; k0 < k1; k0 += kU * L) {
+    int32_t c[kU];
+    double2 av[kU], xv[kU];
+#pragma unroll
+    for(int u = 0; u < kU; ++u) {
+      const bool in = k0 + u * L < k1;
+      c[u] = in ? aj[k0 + u * L] : 0;
+      av[u] = in ? ld_d(a, k0 + u * L) : make_double2(0., 0.);
+    }
+#pragma unroll
+    for(int u = 0; u < kU; ++u) xv[u] = k0 + u * L < k1 ? ld_d(x, c[u]) : make_double2(0., 0.);
+#pragma unroll
+    for(int u = 0; u < kU; ++u)
+      if(k0 + u * L < k1) {
+        sr += (float)av[u].x * (float)xv[u].x - (float)av[u].y * (float)xv[u].y;
+        si += (float)av[u].x * (float)xv[u].y + (float)av[u].y * (float)xv[u].x;
+      }
   }
   const double2 s = lanes_sum<L>(make_double2(sr, si));
   if(l == 0) st(y, row, s);
