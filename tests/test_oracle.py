@@ -355,7 +355,7 @@ def test_constant_dirichlet_gives_constant_solution():
 
 def test_threaded_assembly_matches_sequential():
     """The OpenMP element loop (atomic scatter, used only to time the CPU
-    baseline) assembles the same system as the sequential checker, up to the
+    baseline) assembles the same system as the sequential oracle loop, up to the
     order of the additions."""
     from gfem import workloads as W
     spec = W.helmholtz3d(n=5, order=2)
