@@ -330,6 +330,8 @@ gfem_status gfem_system_destroy(gfem_system *sys)
   dfree(sys->d_sol_nrm);
   dfree(sys->d_sol_dinv);
   if(sys->h_sol_est) (void)hipHostFree(sys->h_sol_est);
+  for(hipGraphExec_t g : sys->sol_graphs) (void)hipGraphExecDestroy(g);
+  if(sys->sol_cap) (void)hipStreamDestroy(sys->sol_cap);
   dfree(sys->d_table);
   dfree(sys->d_table_dense);
   dfree(sys->d_rec);

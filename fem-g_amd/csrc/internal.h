@@ -34,6 +34,7 @@
 namespace gfem {
 
 constexpr int kMaxGroups = 16;
+constexpr int kSolGraphBlock = 5;  // Krylov iterations between convergence checks (one graph each)
 constexpr int kMaxN = 64;
 constexpr int kChunk = 512;       // contributions per gather block (two per thread)
 constexpr int kChunkElems = 128;  // distinct elements per gather block (7 bits)
@@ -251,6 +252,12 @@ struct gfem_system {
   void *h_sol_est = nullptr;  // pinned
   int sol_m = 0;
   bool sol_valid = false;
+  // Krylov iterations captured in HIP graphs, one graph per block of
+  // kSolGraphBlock iterations of a restart cycle; valid for the workspace
+  // above, the restart length sol_graph_m and the SpMV lanes sol_graph_lanes
+  std::vector<hipGraphExec_t> sol_graphs;
+  int sol_graph_m = 0, sol_graph_lanes = 0;
+  hipStream_t sol_cap = nullptr;  // capture stream (default priority)
   // parameters changed by gfem_term_update, applied at the next assembly
   bool table_dirty = false;
   // profiling

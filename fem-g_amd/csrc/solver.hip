@@ -452,6 +452,8 @@ static gfem_status gmres_ir(gfem_system *sys, int rhs_idx, double tol, int m, in
     dfree(sys->d_sol_state);
     dfree(sys->d_sol_y);
     dfree(sys->d_sol_part);
+    for(hipGraphExec_t g : sys->sol_graphs) (void)hipGraphExecDestroy(g);  // they hold the old workspace
+    sys->sol_graphs.clear();
     if(!chk(hipMalloc(&sys->d_sol_V, need_v))) return fail(ctx, GFEM_ERR_OUT_OF_MEMORY, "solver basis allocation");
     if(!chk(hipMalloc(&sys->d_sol_part, sizeof(double2) * (size_t)kRedBlocks * (size_t)(m + 2))))
       return fail(ctx, GFEM_ERR_OUT_OF_MEMORY, "solver reduction allocation");
@@ -528,6 +530,61 @@ static gfem_status gmres_ir(gfem_system *sys, int rhs_idx, double tol, int m, in
     }
     sys->spmv_tuned = true;
   }
+  // One Krylov iteration j on stream st: w = A v_j, two Gram-Schmidt passes,
+  // |w|, v_{j+1}, Givens rotation.  Every argument is fixed for the system's
+  // workspace, restart length and SpMV lanes, so a block of iterations can be
+  // captured once and replayed.
+  auto iteration = [&](int j, hipStream_t st) -> hipError_t {
+    Gmres<C> Gs{sys, st, n, m};
+    T32 *vj = V + (int64_t)j * n;
+    hipError_t e = spmv32<C>(sys->spmv_lanes, n, sys->d_ai, sys->d_aj, a32, vj, w, st);
+    if(e != hipSuccess) return e;
+    double2 *col = H + (int64_t)j * (m + 1);
+    // CGS2: h = V^H w, w -= V h, twice
+    if((e = Gs.dots(V, n, j + 1, w, col, 0)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_orth_update<T32>, dim3(vb), dim3(256), 0, st, n, j + 1, V, n, col, w);
+    if((e = Gs.dots(V, n, j + 1, w, sys->d_sol_y, 0)) != hipSuccess) return e;
+    hipLaunchKernelGGL(k_orth_update<T32>, dim3(vb), dim3(256), 0, st, n, j + 1, V, n, sys->d_sol_y, w);
+    hipLaunchKernelGGL(k_add_into, dim3(1), dim3(64 * ((j + 64) / 64)), 0, st, j + 1, sys->d_sol_y, col);
+    // h_{j+1,j}^2 into the column, then v_{j+1} = w / h_{j+1,j}
+    if((e = Gs.dots(w, n, 1, w, col + j + 1, 0)) != hipSuccess) return e;
+    hipLaunchKernelGGL((k_scale<T32, T32>), dim3(vb), dim3(256), 0, st, n, w, col + j + 1, V + (int64_t)(j + 1) * n);
+    hipLaunchKernelGGL(k_givens, dim3(1), dim3(64), 0, st, m, j, stt);
+    return hipGetLastError();
+  };
+  // HIP graphs of kSolGraphBlock iterations (launch overhead dominates small
+  // systems); GFEM_SOLVER_GRAPHS=0 launches every kernel instead.  A failed
+  // capture falls back to the direct launches.
+  const char *gs_env = std::getenv("GFEM_SOLVER_GRAPHS");
+  bool use_graphs = !(gs_env && gs_env[0] == '0');
+  if(use_graphs && (sys->sol_graphs.empty() || sys->sol_graph_m != m || sys->sol_graph_lanes != sys->spmv_lanes)) {
+    for(hipGraphExec_t g : sys->sol_graphs) (void)hipGraphExecDestroy(g);
+    sys->sol_graphs.clear();
+    hipError_t e = sys->sol_cap ? hipSuccess : hipStreamCreateWithFlags(&sys->sol_cap, hipStreamNonBlocking);
+    for(int j0 = 0; j0 < m && e == hipSuccess; j0 += kSolGraphBlock) {
+      e = hipStreamBeginCapture(sys->sol_cap, hipStreamCaptureModeThreadLocal);
+      if(e != hipSuccess) break;
+      hipError_t le = hipSuccess;
+      for(int j = j0; j < std::min(m, j0 + kSolGraphBlock) && le == hipSuccess; ++j) le = iteration(j, sys->sol_cap);
+      hipGraph_t g = nullptr;
+      e = hipStreamEndCapture(sys->sol_cap, &g);
+      if(le != hipSuccess) e = le;
+      hipGraphExec_t ge = nullptr;
+      if(e == hipSuccess) e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+      if(g) (void)hipGraphDestroy(g);
+      if(e == hipSuccess) sys->sol_graphs.push_back(ge);
+    }
+    if(e != hipSuccess) {
+      (void)hipGetLastError();
+      for(hipGraphExec_t g : sys->sol_graphs) (void)hipGraphExecDestroy(g);
+      sys->sol_graphs.clear();
+      use_graphs = false;
+    }
+    else {
+      sys->sol_graph_m = m;
+      sys->sol_graph_lanes = sys->spmv_lanes;
+    }
+  }
   // ||b||
   GFEM_HIP(ctx, G.dots(b, n, 1, b, nrm + 1, 0));
   GFEM_HIP(ctx, hipMemcpyAsync(hest, nrm, sizeof(double2) * 2, hipMemcpyDeviceToHost, s));
@@ -555,24 +612,24 @@ static gfem_status gmres_ir(gfem_system *sys, int rhs_idx, double tol, int m, in
     // the cycle stops once the estimate reaches the outer target or the
     // FP32 floor relative to this cycle's residual
     const double stop = std::max(0.5 * tol * bnorm, 2e-6 * beta);
+    // iterations in blocks of kSolGraphBlock, the estimate checked after each
+    // block (and at the end of the cycle); a block that would pass max_iter
+    // runs iteration by iteration
     int k = 0;
-    for(int j = 0; j < m && iters < max_iter; ++j) {
-      T32 *vj = V + (int64_t)j * n;
-      GFEM_HIP(ctx, spmv32<C>(sys->spmv_lanes, n, sys->d_ai, sys->d_aj, a32, vj, w, s));
-      double2 *col = H + (int64_t)j * (m + 1);
-      // CGS2: h = V^H w, w -= V h, twice
-      GFEM_HIP(ctx, G.dots(V, n, j + 1, w, col, 0));
-      hipLaunchKernelGGL(k_orth_update<T32>, dim3(vb), dim3(256), 0, s, n, j + 1, V, n, col, w);
-      GFEM_HIP(ctx, G.dots(V, n, j + 1, w, sys->d_sol_y, 0));
-      hipLaunchKernelGGL(k_orth_update<T32>, dim3(vb), dim3(256), 0, s, n, j + 1, V, n, sys->d_sol_y, w);
-      hipLaunchKernelGGL(k_add_into, dim3(1), dim3(64 * ((j + 64) / 64)), 0, s, j + 1, sys->d_sol_y, col);
-      // h_{j+1,j}^2 into the column, then v_{j+1} = w / h_{j+1,j}
-      GFEM_HIP(ctx, G.dots(w, n, 1, w, col + j + 1, 0));
-      hipLaunchKernelGGL((k_scale<T32, T32>), dim3(vb), dim3(256), 0, s, n, w, col + j + 1, V + (int64_t)(j + 1) * n);
-      hipLaunchKernelGGL(k_givens, dim3(1), dim3(64), 0, s, m, j, stt);
-      ++iters;
-      k = j + 1;
-      if((j % 5) == 4 || j == m - 1) {
+    for(int j = 0; j < m && iters < max_iter;) {
+      const int jb = std::min(m, j + kSolGraphBlock);
+      if(use_graphs && j % kSolGraphBlock == 0 && iters + (jb - j) <= max_iter) {
+        GFEM_HIP(ctx, hipGraphLaunch(sys->sol_graphs[j / kSolGraphBlock], s));
+        iters += jb - j;
+        j = jb;
+      }
+      else {
+        GFEM_HIP(ctx, iteration(j, s));
+        ++iters;
+        ++j;
+      }
+      k = j;
+      if(j % kSolGraphBlock == 0 || j == m) {
         GFEM_HIP(ctx, hipMemcpyAsync(hest, est, sizeof(double2), hipMemcpyDeviceToHost, s));
         GFEM_HIP(ctx, hipStreamSynchronize(s));
         if(!(hest[0].x > stop)) break;  // also stops on a NaN estimate

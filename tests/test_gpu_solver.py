@@ -118,3 +118,23 @@ def test_spmv_lanes_are_tuned_at_the_first_solve():
     F, info = _solve(W.helmholtz3d(n=6, order=2), tol=1e-10)
     assert info["converged"]
     assert F.tuning("fp64")["spmv_lanes"] in (4, 8, 16)
+
+
+@pytest.mark.parametrize("restart,max_iter", [(20, 100000), (23, 37)])
+def test_graph_replayed_iterations_match_direct_launches(monkeypatch, restart, max_iter):
+    """Krylov iterations replayed from HIP graphs (blocks of 5) run the same
+    kernels in the same order as direct launches: the same iteration count and
+    a bitwise equal solution.  Also with a restart length that is not a
+    multiple of the block and an iteration limit that ends inside a block."""
+    from gfem import workloads as W
+    print(f"case solver graphs restart={restart} max_iter={max_iter}", flush=True)
+    spec = W.helmholtz3d(n=5, order=2)
+    monkeypatch.setenv("GFEM_SOLVER_GRAPHS", "0")
+    F, i0 = _solve(spec, tol=1e-10, restart=restart, max_iter=max_iter)
+    x0 = F.solution().copy()
+    monkeypatch.setenv("GFEM_SOLVER_GRAPHS", "1")
+    i1 = F.solve(tol=1e-10, restart=restart, max_iter=max_iter)
+    assert np.array_equal(F.solution(), x0)
+    assert i0["iterations"] == i1["iterations"] and i0["converged"] == i1["converged"]
+    i2 = F.solve(tol=1e-10, restart=restart, max_iter=max_iter)  # the graphs are reused
+    assert np.array_equal(F.solution(), x0) and i2["iterations"] == i0["iterations"]
