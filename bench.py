@@ -204,6 +204,26 @@ def serving(args):
     return res
 
 
+def poisson2d_leg(args):
+    """The 2-D Poisson workload: -Laplace(u) = 1 on the unit square, 64x64
+    cells x 2 jittered P1 triangles (8192 elements), Gauss2, Dirichlet u = 0,
+    FP64; a correctness-size problem whose assembly is launch-bound."""
+    from gfem import workloads as W
+    spec = W.poisson2d(n=64, order=1)
+    F = W.build(spec)
+    F.pre()
+    F.assemble(download=False)
+    edofs = spec.element_dofs()
+    steps = max(args.steps, 50)
+    sec = timed_steps(F, steps, args.warmup, "fp64", "fused")
+    prof, _ = kernel_profile(F, "fp64", "fused")
+    log(f"poisson2d 64x64 P1 fp64: {sec * 1e3:.4f} ms/step  {edofs / sec / 1e9:.3f} GDOF/s")
+    return {"workload": "poisson2d unit square 64x64x2 jittered P1 triangles, Gauss2, Dirichlet 0, real64",
+            "elements": edofs // 3, "element_dofs": edofs, "n_unknown": int(F.n_unknown), "nnz": int(F.nnz),
+            "value": edofs / sec / 1e9, "unit": "GDOF/s", "ms_per_step": sec * 1e3, "steps": steps,
+            "kernel_ms": {k: round(v, 5) for k, v in prof.items()}}
+
+
 def batched(args):
     """Uneven Helmholtz instances solved in one batch: tet meshes log-uniform
     in 2k-200k elements (n^3 x 6 cubes), order from {1, 2, 3}, k ~ U[2pi,
@@ -439,6 +459,35 @@ def main():
         # the mixed-precision engine against the plain FP64 GPU assembler
         extra["mixed_vs_plain_fp64_gpu"] = sec_r / sec_m
         log(f"reference-order fp64: {sec_r * 1e3:.3f} ms/step  {edofs / sec_r / 1e9:.2f} GDOF/s")
+
+    if not args.no_extra:
+        # a step that follows a change of the term parameters (new k and source
+        # position, as every serving request makes): the reference tensors are
+        # rebuilt on the stream inside the timed region
+        import math
+        ks = [2 * math.pi * (3 + 0.5 * (i % 7)) for i in range(args.steps + args.warmup)]
+
+        def updated_step(i):
+            for tag, f in W.helmholtz_params(ks[i], (0.5, 0.45 + 0.01 * (i % 5), 0.5)):
+                F.set_term_coefficient(tag, f)
+            F.assemble_async("fp64", "fused")
+
+        for i in range(args.warmup):
+            updated_step(i)
+        F.sync()
+        t1 = time.perf_counter()
+        for i in range(args.steps):
+            updated_step(args.warmup + i)
+        F.sync()
+        sec_u = (time.perf_counter() - t1) / args.steps
+        extra["after_term_update_fp64"] = {"value": edofs / sec_u / 1e9, "ms_per_step": sec_u * 1e3,
+                                           "what": "gfem_term_update of k, ik and the source centre, then the "
+                                                   "FP64 assembly (reference tensors rebuilt each step)"}
+        log(f"fp64 after term update: {sec_u * 1e3:.3f} ms/step  {edofs / sec_u / 1e9:.2f} GDOF/s")
+        # restore the workload's own parameters
+        for tag, f in W.helmholtz_params(2 * math.pi * 5):
+            F.set_term_coefficient(tag, f)
+        extra["poisson2d"] = poisson2d_leg(args)
 
     ho = None
     if not args.no_highorder:

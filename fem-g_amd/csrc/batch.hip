@@ -1026,10 +1026,14 @@ gfem_status gfem_batch_result(gfem_batch *B, int64_t id, void *x_host, size_t by
     info->rel_residual = S.rel;
     info->converged = S.converged ? 1 : 0;
   }
-  // an unused slot holds an empty descriptor (kernels over every slot skip it)
+  // an unused slot holds an empty descriptor (kernels over every slot skip it).
+  // The clear completes before this call returns: the next submit writes the
+  // slot on the caller's stream, which need not be ordered after the null
+  // stream (hipStreamNonBlocking), so an unfinished clear could overwrite it.
   GFEM_HIP(ctx, hipSetDevice(ctx->device));
   hipLaunchKernelGGL(k_put<BItem>, dim3(1), dim3(64), 0, nullptr, B->d_items + w->second, BItem{});
   GFEM_HIP(ctx, hipGetLastError());
+  GFEM_HIP(ctx, hipStreamSynchronize(nullptr));
   B->arena.release(S.off);
   B->where.erase(w);
   S = gfem_batch::Slot();

@@ -363,6 +363,16 @@ gfem_status gfem_group_add(gfem_system *sys, int dim, int type, int64_t ne, cons
   if((int)sys->groups.size() >= kMaxGroups)
     return fail(ctx, GFEM_ERR_UNSUPPORTED, "more than " + std::to_string(kMaxGroups) + " element groups");
   if(et.npe > kMaxN) return fail(ctx, GFEM_ERR_UNSUPPORTED, "too many dofs by element");
+  // Global and linked keys (Dof.h:27-35) need the reference's _indicesLC /
+  // global-row handling (VectorFactory.cpp:181-208), which this engine does
+  // not implement: refuse them rather than drop their contributions.
+  for(int64_t i = 0; i < ne * (int64_t)et.npe; ++i) {
+    const int32_t t = dof_type[i];
+    if(t == GFEM_AT_UNKNOWN_GLOBAL || t == GFEM_AT_LINKED || t == GFEM_AT_LINKED_BUBBLE)
+      return fail(ctx, GFEM_ERR_UNSUPPORTED, "assemble type " + std::to_string(t) + " (global / linked) not supported");
+    if(t != GFEM_AT_UNDEFINED && t != GFEM_AT_FIXED && t != GFEM_AT_UNKNOWN && t != GFEM_AT_UNKNOWN_BUBBLE)
+      return fail(ctx, GFEM_ERR_INVALID_ARG, "unknown assemble type " + std::to_string(t));
+  }
   Group g;
   g.dim = dim;
   g.type = type;

@@ -6,18 +6,24 @@
  * complex128, element order, dense A_e, binary-search scatter: a plain
  * restatement of the reference algorithm (file:line cited per function).
  *
- * Pinning (tests/test_oracle.py):
- *   hilbert key / sort      golden vectors computed by an independent numpy
- *                           restatement + ordering invariants (tests/golden)
+ * Pinning (tests/test_oracle.py, tests/test_golden.py):
+ *   hilbert index / key     visiting orders of 2x2, 4x4, 2x2x2 and 4x4x4 grids
+ *                           derived by hand from the reference tables
+ *                           (tests/golden/hilbert_orders.json), raw index and
+ *                           quantised key; each fixture rejects the transposed
+ *                           and the reflected curve; face-adjacent path property
  *   gauss rules             exactness on monomials up to the stated degree
  *   lagrange basis          Kronecker property at nodes, partition of unity,
  *                           polynomial reproduction
  *   geometry                det = 2*area / 6*vol; J^-T grad reproduces linears
  *   element terms           P1 cotangent stiffness, P1 mass closed forms,
  *                           load = area/3, K*1 = 0, brute-force dense assembly
+ *   AFrequency folding      lone dt^2 / dt mass terms on a P1 tet equal
+ *                           -w^2 / -i w times the closed-form mass matrix
  *   scatter / pattern       structured-grid nnz closed form, sorted rows,
  *                           dense brute force on tiny meshes, A x = -b patch test
- *   DOF numbering           hand-checked 2x2 grid fixture (tests/golden)
+ *   DOF numbering           hand-derived numbering of 4x4 and 4x4x4 Dirichlet
+ *                           P1 grids (tests/golden), grid-neighbour walk
  * parity unpinned (gmsh-side, no gmsh offline): exact gmsh Gauss tables,
  * gmsh's getBoundingBox tolerance, robin_hood iteration order (only breaks
  * Hilbert-key ties), gmsh node ordering of order>=3 elements.

@@ -120,15 +120,17 @@ def test_spmv_lanes_are_tuned_at_the_first_solve():
     assert F.tuning("fp64")["spmv_lanes"] in (4, 8, 16)
 
 
-@pytest.mark.parametrize("restart,max_iter", [(20, 100000), (23, 37)])
-def test_graph_replayed_iterations_match_direct_launches(monkeypatch, restart, max_iter):
+@pytest.mark.parametrize("problem,restart,max_iter", [("helmholtz", 20, 100000), ("helmholtz", 23, 37),
+                                                      ("poisson", 20, 100000), ("poisson", 23, 37)])
+def test_graph_replayed_iterations_match_direct_launches(monkeypatch, problem, restart, max_iter):
     """Krylov iterations replayed from HIP graphs (blocks of 5) run the same
     kernels in the same order as direct launches: the same iteration count and
     a bitwise equal solution.  Also with a restart length that is not a
-    multiple of the block and an iteration limit that ends inside a block."""
+    multiple of the block and an iteration limit that ends inside a block.
+    Both the complex (Helmholtz) and the real (Poisson) solver."""
     from gfem import workloads as W
-    print(f"case solver graphs restart={restart} max_iter={max_iter}", flush=True)
-    spec = W.helmholtz3d(n=5, order=2)
+    print(f"case solver graphs {problem} restart={restart} max_iter={max_iter}", flush=True)
+    spec = W.helmholtz3d(n=5, order=2) if problem == "helmholtz" else W.poisson2d(n=24, order=2)
     monkeypatch.setenv("GFEM_SOLVER_GRAPHS", "0")
     F, i0 = _solve(spec, tol=1e-10, restart=restart, max_iter=max_iter)
     x0 = F.solution().copy()
