@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -23,6 +24,20 @@ using namespace gfem;
 
 namespace {
 thread_local std::string g_err;
+
+// Scratch of gfem_assemble_into_noalloc: the device group table, the element
+// factors and the element loads of one assembly, carved from one array that
+// the code object brings with it (no allocation call).  Each GPU holds its own
+// copy until the process ends; every call on a GPU shares it, so calls that
+// overlap on one GPU race.  Host code never uses the array's name as an
+// address: it takes the device address from hipGetSymbolAddress, once per
+// device, in gfem_ctx_create.
+constexpr size_t kNoAllocScratchBytes = size_t(256) << 20;
+__device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxScratchDevices];
+bool g_scratch_tried[kMaxScratchDevices];
 }  // namespace
 
 namespace gfem {
@@ -228,6 +243,18 @@ gfem_status gfem_ctx_create(int device, gfem_ctx **out)
   gfem_ctx *ctx = new(std::nothrow) gfem_ctx();
   if(!ctx) return fail(nullptr, GFEM_ERR_OUT_OF_MEMORY, "host allocation failed");
   ctx->device = device;
+  // device address of this GPU's scratch array (looked up once per device,
+  // with the device current; a failure only disables the no-allocation entry)
+  if(device < kMaxScratchDevices) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if(!g_scratch_tried[device]) {
+      g_scratch_tried[device] = true;
+      void *ptr = nullptr;
+      if(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) == hipSuccess) g_scratch_ptr[device] = ptr;
+      else (void)hipGetLastError();
+    }
+    ctx->scratch = g_scratch_ptr[device];
+  }
   *out = ctx;
   return GFEM_OK;
 }
@@ -1245,6 +1272,101 @@ gfem_status gfem_assemble_into(gfem_system *sys, gfem_precision prec, void *vals
   if(!vals || !rhs) return fail(sys->ctx, GFEM_ERR_INVALID_ARG, "assemble_into: null output");
   GFEM_HIP(sys->ctx, hipSetDevice(sys->ctx->device));
   return assemble_impl(sys, prec, GFEM_STRATEGY_FUSED, vals, rhs, reinterpret_cast<hipStream_t>(stream));
+}
+
+// The scratch buffers of one fused assembly (element factors, element loads,
+// group table) pointed at the no-allocation scratch array for the length of a
+// call, and restored on every return path.
+namespace {
+struct ScratchSwap {
+  gfem_system *sys;
+  std::vector<double *> f64;
+  std::vector<float *> f32;
+  std::vector<double2 *> be;
+  double2 *be_all;
+  GroupTable *table;
+  explicit ScratchSwap(gfem_system *s) : sys(s), be_all(s->d_be_all), table(s->d_table)
+  {
+    for(const Group &g : s->groups) {
+      f64.push_back(g.d_F64);
+      f32.push_back(g.d_F32);
+      be.push_back(g.d_be);
+    }
+  }
+  ~ScratchSwap()
+  {
+    for(size_t k = 0; k < sys->groups.size(); ++k) {
+      sys->groups[k].d_F64 = f64[k];
+      sys->groups[k].d_F32 = f32[k];
+      sys->groups[k].d_be = be[k];
+    }
+    sys->d_be_all = be_all;
+    sys->d_table = table;
+  }
+};
+}  // namespace
+
+gfem_status gfem_assemble_into_noalloc(gfem_system *sys, gfem_precision prec, void *vals, void *rhs, void *stream)
+{
+  gfem_status st = check_assemble(sys, prec, GFEM_STRATEGY_FUSED);
+  if(st != GFEM_OK) return st;
+  gfem_ctx *ctx = sys->ctx;
+  if(!vals || !rhs) return fail(ctx, GFEM_ERR_INVALID_ARG, "assemble_into_noalloc: null output");
+  if(!ctx->scratch)
+    return fail(ctx, GFEM_ERR_HIP, "assemble_into_noalloc: no device address for the scratch array on this GPU");
+  for(const Group &g : sys->groups)
+    if(g.has_dense)
+      return fail(ctx, GFEM_ERR_UNSUPPORTED,
+                  "assemble_into_noalloc: per-point element matrices (expression coefficients of bilinear terms, "
+                  "curved elements) live in the system's own buffers");
+  const int p = (int)prec;
+  const bool fp32 = prec == GFEM_PREC_MIXED;
+  // layout: group table, factors of every group (FP64, + FP32 for mixed),
+  // element loads [nrhs][be_total]; 256-byte aligned pieces
+  auto up = [](size_t b) { return (b + 255) & ~size_t(255); };
+  const size_t ng = sys->groups.size();
+  std::vector<size_t> o64(ng), o32(ng);
+  size_t off = up(sizeof(GroupTable));
+  for(size_t k = 0; k < ng; ++k) {
+    const size_t ne = (size_t)sys->groups[k].ne;
+    o64[k] = off;
+    off += up(ne * 8 * sizeof(double));
+    o32[k] = off;
+    if(fp32) off += up(ne * 8 * sizeof(float));
+  }
+  const size_t be_off = off, be_bytes = sizeof(double2) * (size_t)sys->nrhs * (size_t)sys->be_total;
+  off += up(be_bytes);
+  if(off > kNoAllocScratchBytes)
+    return fail(ctx, GFEM_ERR_TOO_LARGE,
+                "assemble_into_noalloc: the assembly needs " + std::to_string(off) + " bytes of scratch, more than the " +
+                  std::to_string(kNoAllocScratchBytes) + " the library holds");
+  GFEM_HIP(ctx, hipSetDevice(ctx->device));
+  const hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  // coefficient changes land in the system's own tensors and table first
+  st = apply_updates(sys, s);
+  if(st != GFEM_OK) return st;
+  ScratchSwap keep(sys);
+  unsigned char *base = static_cast<unsigned char *>(ctx->scratch);
+  sys->d_table = reinterpret_cast<GroupTable *>(base);
+  sys->d_be_all = be_bytes ? reinterpret_cast<double2 *>(base + be_off) : nullptr;
+  for(size_t k = 0; k < ng; ++k) {
+    Group &g = sys->groups[k];
+    g.d_F64 = reinterpret_cast<double *>(base + o64[k]);
+    g.d_F32 = fp32 ? reinterpret_cast<float *>(base + o32[k]) : nullptr;
+    g.d_be = g.has_linear ? sys->d_be_all + g.be_off : nullptr;
+  }
+  // the array keeps the last call's contents: element loads start from zero
+  // (right-hand sides a group has no term for read zeros), the table is this
+  // system's
+  if(be_bytes) GFEM_HIP(ctx, launch_zero(sys->d_be_all, be_bytes, s));
+  GFEM_HIP(ctx, launch_put_table(sys->d_table, make_table(sys, false), s));
+  // the fused launches of assemble_impl, with the tile choices in place
+  GFEM_HIP(ctx, launch_geometry_all(*sys, sys->d_xyz, fp32, s));
+  for(auto &g : sys->groups)
+    if(g.has_linear) GFEM_HIP(ctx, launch_linear(g, sys->d_xyz, sys->nrhs, sys->linear_eb_shift[p], sys->be_real, s));
+  GFEM_HIP(ctx, launch_rhs(*sys, p, 1, rhs, s));
+  GFEM_HIP(ctx, launch_gather(*sys, p, 1, vals, s));
+  return GFEM_OK;
 }
 
 static gfem_status assemble_impl(gfem_system *sys, gfem_precision prec, gfem_strategy strategy, void *vals_out,

@@ -187,6 +187,7 @@ struct gfem_ctx {
   int64_t n_nodes = 0;
   std::vector<double> h_xyz;
   std::string err;
+  void *scratch = nullptr;  // device address of this GPU's no-allocation scratch array
 };
 
 struct gfem_system {

@@ -149,6 +149,12 @@ void bind_formulation(py::module_ &m, const char *name)
            if(!f.system()) throw std::runtime_error("set_gemm_mode: the system exists after the first assemble()");
            check(gfem_set_gemm_mode(f.system(), m == "fp32" ? GFEM_GEMM_FP32 : GFEM_GEMM_BF16X3), "set_gemm_mode");
          })
+    .def("system_handle",
+         [](F &f) {
+           // the gfem_system* behind this formulation, for calls through the C ABI
+           if(!f.system()) throw std::runtime_error("system_handle: the system exists after the first assemble()");
+           return reinterpret_cast<uintptr_t>(f.system());
+         })
     .def("set_profiling", [](F &f, bool on) { check(gfem_set_profiling(f.system(), on ? 1 : 0), "set_profiling"); })
     .def("kernel_times",
          [](F &f) {

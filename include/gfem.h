@@ -283,6 +283,21 @@ gfem_status gfem_assemble(gfem_system *sys, gfem_precision prec, gfem_strategy s
  * system can be held at once.  No counterpart in the reference, whose values
  * live in one MatrixFactory (fem-master/src/system/MatrixFactory.h:67). */
 gfem_status gfem_assemble_into(gfem_system *sys, gfem_precision prec, void *vals, void *rhs, void *stream);
+/* gfem_assemble_into that allocates nothing: no counterpart in the reference.
+ * For callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * scratch of the assembly (group table, element factors, element loads) comes
+ * from a fixed array of 256 MiB that the library holds on each GPU it runs on
+ * until the process ends; GFEM_ERR_TOO_LARGE when the system needs more
+ * (about 64 B per element, +32 B under MIXED, + 16 B per element DOF and
+ * right-hand side).  Fused strategy only: GFEM_ERR_UNSUPPORTED for systems
+ * with per-point element matrices (expression coefficients of bilinear terms,
+ * curved elements).  Uses the tile choices in place (the defaults until an
+ * allocating assembly has tuned them).  Results equal gfem_assemble_into's
+ * with the same tile choices, bit for bit.  Every call on a GPU shares its
+ * array, which keeps the last call's contents: calls that overlap on one GPU
+ * (streams, threads or processes) race. */
+gfem_status gfem_assemble_into_noalloc(gfem_system *sys, gfem_precision prec, void *vals, void *rhs, void *stream);
 
 /* Change the constant coefficient (coef_re, coef_im) of term `term` (order of
  * gfem_group_add_term) of `group`, or, for an expression coefficient, its
