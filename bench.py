@@ -301,10 +301,10 @@ def batched(args):
 
 def highorder(args):
     """Order-4 tetrahedra (32^3 x 6 = 196,608 tets, N = 35), mass + stiffness,
-    Gauss8: element matrices as per-point B^T W B on the matrix cores (FP64,
-    FP32 and split-bf16 GEMM, Jacobian inverses in FP64) against the
-    tensor-form FP64 path; per-entry error of the FP32 and bf16x3 matrix-core
-    paths against FP64."""
+    Gauss8: element matrices as per-point B^T W B on the matrix cores (the
+    hand-written MFMA kernel in FP64, FP32 and split-bf16, Jacobian inverses
+    in FP64) against the tensor-form FP64 path; per-entry error of the FP32
+    and bf16x3 matrix-core paths against FP64."""
     import numpy as np
     from gfem import gauss_rule, workloads as W
     t0 = time.perf_counter()
@@ -335,7 +335,10 @@ def highorder(args):
             rec["gemm_mfma_tflops"] = rec["gemm_tflops"] * (3 if split else 1)
             rec["gemm_peak_tflops"] = peak
             rec["gemm_frac"] = rec["gemm_mfma_tflops"] / peak
-            rec["gemm_shape"] = f"M=N*N=1225, N=elements={edofs // 35}, K=7Q={K}" + (" (x3 bf16)" if split else "")
+            rec["gemm_shape"] = (f"M=elements={edofs // 35}, N=N*N=1225, K=7Q={K} (component-major, each component's "
+                                 f"points padded to a multiple of 32 or 64; padding not counted)" +
+                                 (" x3 bf16 products" if split else ""))
+            rec["gemm_kernel"] = "k_elem_mm (hand-written MFMA, csrc/elemmm.hip)"
         F.assemble(precision=prec, strategy=strat)
         vals = F.values()
         if ref is None:

@@ -47,7 +47,8 @@ def build(verbose=False):
     py_inc = sysconfig.get_paths()["include"]
 
     dev_objs = [os.path.join(OBJ, "kernels.o"), os.path.join(OBJ, "gfem_api.o"), os.path.join(OBJ, "jit.o"),
-                os.path.join(OBJ, "solver.o"), os.path.join(OBJ, "batch.o"), os.path.join(OBJ, "setup.o")]
+                os.path.join(OBJ, "solver.o"), os.path.join(OBJ, "batch.o"), os.path.join(OBJ, "setup.o"),
+                os.path.join(OBJ, "elemmm.o")]
     host_srcs = ["model.cpp", "msh.cpp", "gmshfem.cpp", "serve.cpp"]
     host_objs = [os.path.join(OBJ, s.replace(".cpp", ".o")) for s in host_srcs]
     py_obj = os.path.join(OBJ, "pybind.o")
@@ -59,6 +60,7 @@ def build(verbose=False):
         [HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "solver.hip"), "-o", dev_objs[3]],
         [HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "batch.hip"), "-o", dev_objs[4]],
         [HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "setup.hip"), "-o", dev_objs[5]],
+        [HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "elemmm.hip"), "-o", dev_objs[6]],
     ]
     jobs += [["g++", *CXX_FLAGS, "-c", os.path.join(HOST, s), "-o", o] for s, o in zip(host_srcs, host_objs)]
     jobs.append(["g++", *CXX_FLAGS, "-fvisibility=hidden", "-I", pybind11.get_include(), "-I", py_inc,
@@ -69,7 +71,7 @@ def build(verbose=False):
                 print(out)
 
     libgfem = os.path.join(OUT, "libgfem.so")
-    run([HIPCC, *HIP_FLAGS, "-shared", "-o", libgfem, *dev_objs, "-L/opt/rocm/lib", "-lhiprtc", "-lrocblas", "-lhipblaslt",
+    run([HIPCC, *HIP_FLAGS, "-shared", "-o", libgfem, *dev_objs, "-L/opt/rocm/lib", "-lhiprtc",
          "-Wl,-rpath,/opt/rocm/lib"])
     libhost = os.path.join(OUT, "libgfem_host.so")
     run(["g++", "-shared", "-o", libhost, *host_objs, "-L", OUT, "-lgfem", "-Wl,-rpath,$ORIGIN"])

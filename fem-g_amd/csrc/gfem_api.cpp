@@ -13,8 +13,6 @@
 #include <string>
 #include <vector>
 
-#include <hipblaslt/hipblaslt.h>
-#include <rocblas/rocblas.h>
 
 #include "gfem.h"
 #include "internal.h"
@@ -80,30 +78,8 @@ gfem_status hip_fail(gfem_ctx *ctx, hipError_t e, const char *what)
 
 namespace {
 
-// hipBLASLt plan of a group's split GEMM: descriptors, the kernel kept (the
-// fastest of the library's candidates, timed on the group's own operands)
-// and its workspace
-struct LtPlan {
-  hipblasLtMatmulDesc_t desc = nullptr;
-  hipblasLtMatrixLayout_t la = nullptr, lb = nullptr, lc = nullptr;
-  hipblasLtMatmulAlgo_t algo{};
-  size_t ws = 0;
-  void *d_ws = nullptr;
-  int candidates = 0;
-  ~LtPlan()
-  {
-    if(d_ws) (void)hipFree(d_ws);
-    if(la) (void)hipblasLtMatrixLayoutDestroy(la);
-    if(lb) (void)hipblasLtMatrixLayoutDestroy(lb);
-    if(lc) (void)hipblasLtMatrixLayoutDestroy(lc);
-    if(desc) (void)hipblasLtMatmulDescDestroy(desc);
-  }
-};
-
 void free_group(Group &g)
 {
-  delete static_cast<LtPlan *>(g.lt_plan);
-  g.lt_plan = nullptr;
   dfree(g.d_vtx);
   dfree(g.d_conn);
   dfree(g.d_nodes);
@@ -117,12 +93,12 @@ void free_group(Group &g)
   dfree(g.d_T32r);
   dfree(g.d_dense64);
   dfree(g.d_dense32);
-  dfree(g.d_R64);
-  dfree(g.d_R32);
+  dfree(g.d_Rt64);
+  dfree(g.d_Rt32);
+  dfree(g.d_Rth);
+  dfree(g.d_Rtl);
   dfree(g.d_A64);
   dfree(g.d_A32);
-  dfree(g.d_Rb);
-  dfree(g.d_Ab);
   dfree(g.d_e2n);
   dfree(g.d_num);
   dfree(g.d_type);
@@ -264,8 +240,6 @@ gfem_status gfem_ctx_destroy(gfem_ctx *ctx)
   if(!ctx) return GFEM_OK;
   (void)hipSetDevice(ctx->device);
   dfree(ctx->d_xyz);
-  if(ctx->blas) (void)rocblas_destroy_handle(static_cast<rocblas_handle>(ctx->blas));
-  if(ctx->blaslt) (void)hipblasLtDestroy(static_cast<hipblasLtHandle_t>(ctx->blaslt));
   delete ctx;
   return GFEM_OK;
 }
@@ -973,163 +947,68 @@ static gfem_status apply_updates(gfem_system *sys, hipStream_t s)
 }
 
 
-// reference products R of a group (once) and its A rows / element matrices
-// for one precision (split: the bf16 hi/lo form of R and A)
-static gfem_status ensure_gemm(gfem_system *sys, Group &g, bool fp32, bool split)
+// Element matrices of group g on the matrix cores (csrc/elemmm.hip):
+// dense[ne*planes][N*N] = A[ne*planes][7Q] R[7Q][N*N].  A is generated
+// inside the kernel when every bilinear term of an affine group is in the
+// element matrices with a constant coefficient (all_terms, no expression),
+// else k_coef_rows writes it first.
+static int gemm_arith(bool fp32, bool split) { return split ? kMmBf16x3 : (fp32 ? kMmF32 : kMmF64); }
+
+static bool gemm_generated(const Group &g, bool all_terms)
+{
+  if(!all_terms || g.curved) return false;
+  for(const DevTerm &t : g.dterms)
+    if(t.kind == 0 && t.per_point) return false;
+  return true;
+}
+
+// R_T of a group for one arithmetic (once), the A rows when they are read,
+// and the element matrices of the precision
+static gfem_status ensure_gemm(gfem_system *sys, Group &g, bool fp32, bool split, bool generated)
 {
   gfem_ctx *ctx = sys->ctx;
   const size_t planes = sys->cplx ? 2 : 1, NN = (size_t)g.N * g.N;
   const size_t K = (size_t)7 * g.rules[g.gemm_rule].Q;
-  if(!g.d_R64) {
-    GFEM_HIP(ctx, dalloc(&g.d_R64, K * NN));
-    GFEM_HIP(ctx, dalloc(&g.d_R32, K * NN));
-    GFEM_HIP(ctx, launch_ref_products(g, nullptr));
+  int64_t NNp, Kp;
+  elem_mm_dims(g, NNp, Kp);
+  const size_t nr = (size_t)(NNp * Kp);
+  const int mode = gemm_arith(fp32, split);
+  if(mode == kMmF64 && !g.d_Rt64) {
+    GFEM_HIP(ctx, dalloc(&g.d_Rt64, nr));
+    GFEM_HIP(ctx, launch_ref_products_t(g, mode, g.d_Rt64, nullptr, nullptr));
     GFEM_HIP(ctx, hipDeviceSynchronize());
   }
-  if(split) {
-    if(!g.d_Rb) {
-      GFEM_HIP(ctx, dalloc(&g.d_Rb, 3 * K * NN));
-      GFEM_HIP(ctx, launch_ref_split(g, nullptr));
-      GFEM_HIP(ctx, hipDeviceSynchronize());
-    }
-    if(!g.d_Ab) GFEM_HIP(ctx, dalloc(&g.d_Ab, (size_t)g.ne * planes * 3 * K));
-    if(!g.d_dense32) GFEM_HIP(ctx, dalloc(&g.d_dense32, (size_t)g.ne * planes * NN));
+  if(mode == kMmF32 && !g.d_Rt32) {
+    GFEM_HIP(ctx, dalloc(&g.d_Rt32, nr));
+    GFEM_HIP(ctx, launch_ref_products_t(g, mode, g.d_Rt32, nullptr, nullptr));
+    GFEM_HIP(ctx, hipDeviceSynchronize());
   }
-  else if(fp32) {
-    if(!g.d_A32) GFEM_HIP(ctx, dalloc(&g.d_A32, (size_t)g.ne * planes * K));
-    if(!g.d_dense32) GFEM_HIP(ctx, dalloc(&g.d_dense32, (size_t)g.ne * planes * NN));
+  if(mode == kMmBf16x3 && !g.d_Rth) {
+    GFEM_HIP(ctx, dalloc(&g.d_Rth, nr));
+    GFEM_HIP(ctx, dalloc(&g.d_Rtl, nr));
+    GFEM_HIP(ctx, launch_ref_products_t(g, mode, g.d_Rth, g.d_Rtl, nullptr));
+    GFEM_HIP(ctx, hipDeviceSynchronize());
   }
-  else {
-    if(!g.d_A64) GFEM_HIP(ctx, dalloc(&g.d_A64, (size_t)g.ne * planes * K));
-    if(!g.d_dense64) GFEM_HIP(ctx, dalloc(&g.d_dense64, (size_t)g.ne * planes * NN));
+  if(!generated) {
+    if(fp32 && !g.d_A32) GFEM_HIP(ctx, dalloc(&g.d_A32, (size_t)g.ne * planes * K));
+    if(!fp32 && !g.d_A64) GFEM_HIP(ctx, dalloc(&g.d_A64, (size_t)g.ne * planes * K));
   }
+  if(fp32 && !g.d_dense32) GFEM_HIP(ctx, dalloc(&g.d_dense32, (size_t)g.ne * planes * NN));
+  if(!fp32 && !g.d_dense64) GFEM_HIP(ctx, dalloc(&g.d_dense64, (size_t)g.ne * planes * NN));
   return GFEM_OK;
 }
 
-// the split GEMM through hipBLASLt: Cᵀ[N*N][ne*planes] = Rbᵀ Ab with both
-// operands K-contiguous (op(A) = T), FP32 accumulation and output.  The
-// library's first kLtCandidates candidates are timed (one warm-up and two
-// timed runs each) and the fastest kept; with GFEM_AUTOTUNE=0 its first
-// candidate.
-constexpr int kLtCandidates = 16;
-static gfem_status lt_plan(gfem_system *sys, Group &g, int64_t m, int64_t n, int64_t k3, hipStream_t s)
+static gfem_status element_gemm(gfem_system *sys, Group &g, bool fp32, bool split, bool generated, int planes,
+                                hipStream_t s)
 {
   gfem_ctx *ctx = sys->ctx;
-  if(g.lt_plan) return GFEM_OK;
-  if(!ctx->blaslt) {
-    hipblasLtHandle_t h = nullptr;
-    if(hipblasLtCreate(&h) != HIPBLAS_STATUS_SUCCESS) return fail(ctx, GFEM_ERR_HIP, "hipblasLtCreate failed");
-    ctx->blaslt = h;
-  }
-  hipblasLtHandle_t h = static_cast<hipblasLtHandle_t>(ctx->blaslt);
-  auto *P = new LtPlan;
-  auto bad = [&](const char *what) {
-    delete P;
-    return fail(ctx, GFEM_ERR_HIP, std::string("split GEMM: ") + what);
-  };
-  const hipblasOperation_t opA = HIPBLAS_OP_T, opB = HIPBLAS_OP_N;
-  if(hipblasLtMatmulDescCreate(&P->desc, HIPBLAS_COMPUTE_32F, HIP_R_32F) != HIPBLAS_STATUS_SUCCESS ||
-     hipblasLtMatmulDescSetAttribute(P->desc, HIPBLASLT_MATMUL_DESC_TRANSA, &opA, sizeof(opA)) !=
-         HIPBLAS_STATUS_SUCCESS ||
-     hipblasLtMatmulDescSetAttribute(P->desc, HIPBLASLT_MATMUL_DESC_TRANSB, &opB, sizeof(opB)) !=
-         HIPBLAS_STATUS_SUCCESS)
-    return bad("matmul descriptor");
-  if(hipblasLtMatrixLayoutCreate(&P->la, HIP_R_16BF, (uint64_t)k3, (uint64_t)m, k3) != HIPBLAS_STATUS_SUCCESS ||
-     hipblasLtMatrixLayoutCreate(&P->lb, HIP_R_16BF, (uint64_t)k3, (uint64_t)n, k3) != HIPBLAS_STATUS_SUCCESS ||
-     hipblasLtMatrixLayoutCreate(&P->lc, HIP_R_32F, (uint64_t)m, (uint64_t)n, m) != HIPBLAS_STATUS_SUCCESS)
-    return bad("matrix layouts");
-  hipblasLtMatmulPreference_t pref = nullptr;
-  uint64_t max_ws = uint64_t(64) << 20;
-  hipblasLtMatmulHeuristicResult_t cand[kLtCandidates];
-  int nc = 0;
-  if(hipblasLtMatmulPreferenceCreate(&pref) != HIPBLAS_STATUS_SUCCESS) return bad("preference");
-  const bool ok = hipblasLtMatmulPreferenceSetAttribute(pref, HIPBLASLT_MATMUL_PREF_MAX_WORKSPACE_BYTES, &max_ws,
-                                                        sizeof(max_ws)) == HIPBLAS_STATUS_SUCCESS &&
-                  hipblasLtMatmulAlgoGetHeuristic(h, P->desc, P->la, P->lb, P->lc, P->lc, pref, kLtCandidates, cand, &nc) ==
-                      HIPBLAS_STATUS_SUCCESS;
-  (void)hipblasLtMatmulPreferenceDestroy(pref);
-  if(!ok || nc <= 0) return bad("no kernel for this shape");
-  size_t ws = 0;
-  for(int c = 0; c < nc; ++c) ws = std::max(ws, cand[c].workspaceSize);
-  if(ws && hipMalloc(&P->d_ws, ws) != hipSuccess) return bad("workspace allocation");
-  P->ws = ws;
-  P->candidates = nc;
-  int best = 0;
-  if(sys->autotune && nc > 1) {
-    hipEvent_t a = nullptr, b = nullptr;
-    if(hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return bad("events");
-    const float one = 1.f, zero = 0.f;
-    float best_ms = 0.f;
-    for(int c = 0; c < nc; ++c) {
-      float ms = 0.f;
-      bool run_ok = true;
-      for(int it = 0; it < 3 && run_ok; ++it) {
-        if(it == 1) run_ok = hipEventRecord(a, s) == hipSuccess;
-        run_ok = run_ok && hipblasLtMatmul(h, P->desc, &one, g.d_Rb, P->la, g.d_Ab, P->lb, &zero, g.d_dense32, P->lc,
-                                           g.d_dense32, P->lc, &cand[c].algo, P->d_ws, P->ws, s) ==
-                               HIPBLAS_STATUS_SUCCESS;
-      }
-      run_ok = run_ok && hipEventRecord(b, s) == hipSuccess && hipEventSynchronize(b) == hipSuccess &&
-               hipEventElapsedTime(&ms, a, b) == hipSuccess;
-      if(std::getenv("GFEM_SETUP_TIMING"))
-        std::fprintf(stderr, "gfem: split GEMM candidate %d/%d: %.3f ms%s\n", c, nc, ms / 2, run_ok ? "" : " (failed)");
-      if(run_ok && (c == 0 || ms < best_ms)) {
-        best = c;
-        best_ms = ms;
-      }
-    }
-    (void)hipEventDestroy(a);
-    (void)hipEventDestroy(b);
-  }
-  P->algo = cand[best].algo;
-  g.lt_plan = P;
-  return GFEM_OK;
-}
-
-// element matrices of group g: dense[ne*planes][N*N] = A[ne*planes][7Q] R[7Q][N*N]
-// (row-major; as column-major, dense^T = R^T A^T with no transposes).  The
-// split form is the same product over 3*7Q bf16 columns with FP32 sums.
-static gfem_status element_gemm(gfem_system *sys, Group &g, bool fp32, bool split, int planes, hipStream_t s)
-{
-  gfem_ctx *ctx = sys->ctx;
-  if(!ctx->blas) {
-    rocblas_handle h = nullptr;
-    if(rocblas_create_handle(&h) != rocblas_status_success)
-      return fail(ctx, GFEM_ERR_HIP, "rocblas_create_handle failed");
-    ctx->blas = h;
-  }
-  rocblas_handle h = static_cast<rocblas_handle>(ctx->blas);
-  if(rocblas_set_stream(h, s) != rocblas_status_success) return fail(ctx, GFEM_ERR_HIP, "rocblas_set_stream failed");
-  const int64_t m = (int64_t)g.N * g.N, n = g.ne * planes, k = (int64_t)7 * g.rules[g.gemm_rule].Q;
-  if(n == 0) return GFEM_OK;
-  if(n >= (int64_t(1) << 31)) return fail(ctx, GFEM_ERR_TOO_LARGE, "element GEMM larger than 2^31 rows");
-  rocblas_status r;
-  if(split) {
-    if(3 * k >= (int64_t(1) << 31)) return fail(ctx, GFEM_ERR_TOO_LARGE, "split element GEMM: 3*7Q too large");
-    const gfem_status st = lt_plan(sys, g, m, n, 3 * k, s);
-    if(st != GFEM_OK) return st;
-    const LtPlan *P = static_cast<const LtPlan *>(g.lt_plan);
-    const float one = 1.f, zero = 0.f;
-    // Rb is stored [N*N][3*7Q] (K-contiguous): the transposed operand
-    if(hipblasLtMatmul(static_cast<hipblasLtHandle_t>(ctx->blaslt), P->desc, &one, g.d_Rb, P->la, g.d_Ab, P->lb,
-                       &zero, g.d_dense32, P->lc, g.d_dense32, P->lc, &P->algo, P->d_ws, P->ws, s) !=
-       HIPBLAS_STATUS_SUCCESS)
-      return fail(ctx, GFEM_ERR_HIP, "hipblasLtMatmul failed");
-    return GFEM_OK;
-  }
-  else if(fp32) {
-    const float one = 1.f, zero = 0.f;
-    r = rocblas_sgemm(h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)m, (rocblas_int)n,
-                      (rocblas_int)k, &one, g.d_R32, (rocblas_int)m, g.d_A32, (rocblas_int)k, &zero, g.d_dense32,
-                      (rocblas_int)m);
-  }
-  else {
-    const double one = 1., zero = 0.;
-    r = rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)m, (rocblas_int)n,
-                      (rocblas_int)k, &one, g.d_R64, (rocblas_int)m, g.d_A64, (rocblas_int)k, &zero, g.d_dense64,
-                      (rocblas_int)m);
-  }
-  if(r != rocblas_status_success) return fail(ctx, GFEM_ERR_HIP, "rocblas gemm failed");
+  if(g.ne == 0) return GFEM_OK;
+  if(g.ne * planes >= (int64_t(1) << 31)) return fail(ctx, GFEM_ERR_TOO_LARGE, "element GEMM larger than 2^31 rows");
+  const int mode = gemm_arith(fp32, split);
+  const void *A = fp32 ? (const void *)g.d_A32 : (const void *)g.d_A64;
+  const void *Rt = mode == kMmF64 ? (const void *)g.d_Rt64 : mode == kMmF32 ? (const void *)g.d_Rt32 : (const void *)g.d_Rth;
+  void *out = fp32 ? (void *)g.d_dense32 : (void *)g.d_dense64;
+  GFEM_HIP(ctx, launch_elem_mm(g, mode, generated, planes, generated ? nullptr : A, Rt, g.d_Rtl, out, s));
   return GFEM_OK;
 }
 
@@ -1396,13 +1275,16 @@ static gfem_status assemble_impl(gfem_system *sys, gfem_precision prec, gfem_str
       if(!(all ? g.has_bilinear : g.has_dense)) continue;
       if(g.gemm_rule >= 0 && (all || sys->dense_gemm)) {
         const bool split = fp32 && sys->gemm_mode == GFEM_GEMM_BF16X3;
-        st = ensure_gemm(sys, g, fp32, split);
+        const bool generated = gemm_generated(g, all);
+        st = ensure_gemm(sys, g, fp32, split, generated);
         if(st != GFEM_OK) return st;
-        prof.begin("coef_rows");
-        GFEM_HIP(ctx, launch_coef_rows(g, sys->d_xyz, all || g.curved, planes, fp32, split, s));
-        prof.end();
+        if(!generated) {
+          prof.begin("coef_rows");
+          GFEM_HIP(ctx, launch_coef_rows(g, sys->d_xyz, all || g.curved, planes, fp32, s));
+          prof.end();
+        }
         prof.begin("element_gemm");
-        st = element_gemm(sys, g, fp32, split, planes, s);
+        st = element_gemm(sys, g, fp32, split, generated, planes, s);
         prof.end();
         if(st != GFEM_OK) return st;
       }

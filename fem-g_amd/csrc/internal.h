@@ -143,21 +143,20 @@ struct Group {
   // per-point element matrices, planes layout [ne][planes][N*N]
   double *d_dense64 = nullptr;
   float *d_dense32 = nullptr;
-  // GEMM form of the per-point terms: A rows [ne*planes][7Q] (per element and
-  // point: det G (6 components) times the stiffness coefficient, det times the
-  // mass coefficient) times R [7Q][N*N] (weights times reference products)
+  // GEMM form of the per-point terms (csrc/elemmm.hip): A rows [ne*planes][7Q]
+  // (per element and point: det G (6 components) times the stiffness
+  // coefficient, det times the mass coefficient) times R [7Q][N*N] (weights
+  // times reference products), R kept transposed and zero-padded as
+  // R_T [NNp][Kp]: FP64, FP32, and the bf16 hi / lo parts of the split form
   int gemm_rule = -1;  // the rule all GEMM-path terms share, or -1
-  double *d_R64 = nullptr;
-  float *d_R32 = nullptr;
+  double *d_Rt64 = nullptr;
+  float *d_Rt32 = nullptr;
+  uint16_t *d_Rth = nullptr, *d_Rtl = nullptr;
+  // A rows written by k_coef_rows where the kernel cannot generate them
+  // (expression coefficients, curved elements): FP64, or FP32 for the FP32
+  // and split arithmetics
   double *d_A64 = nullptr;
   float *d_A32 = nullptr;
-  // the split form (bf16 bits): Rb [N*N][3*7Q], row `pair` = [hi(R_pair),
-  // lo(R_pair), hi(R_pair)], and Ab [ne*planes][3*7Q], row c = [hi(A_c),
-  // hi(A_c), lo(A_c)], so one bf16 GEMM over 3*7Q sums hi*hi + lo*hi + hi*lo
-  // with FP32 accumulation
-  uint16_t *d_Rb = nullptr;
-  uint16_t *d_Ab = nullptr;
-  void *lt_plan = nullptr;  // hipBLASLt plan of the split GEMM (gfem_api.cpp)
   double2 *d_be = nullptr;      // this group's element loads inside gfem_system::d_be_all
   int64_t be_off = 0, be_stride = 0;  // offset in d_be_all, distance between right-hand sides
   // maps in storage order on the device (from finalize): numDof-1 and
@@ -181,8 +180,6 @@ struct Group {
 
 struct gfem_ctx {
   int device = 0;
-  void *blas = nullptr;  // rocblas_handle, created at the first GEMM
-  void *blaslt = nullptr;  // hipblasLtHandle_t, created at the first split GEMM
   double *d_xyz = nullptr;
   int64_t n_nodes = 0;
   std::vector<double> h_xyz;
@@ -332,10 +329,13 @@ hipError_t launch_geometry_all(const gfem_system &sys, const double *xyz, bool f
 hipError_t launch_linear(const Group &g, const double *xyz, int nrhs, int eb_shift, bool be_real, hipStream_t s);
 hipError_t launch_dense(const Group &g, const double *xyz, bool all_terms, int planes, hipStream_t s);
 hipError_t launch_dense_to32(const Group &g, int planes, hipStream_t s);
-hipError_t launch_ref_products(const Group &g, hipStream_t s);
-hipError_t launch_coef_rows(const Group &g, const double *xyz, bool all_terms, int planes, bool fp32, bool split,
-                            hipStream_t s);
-hipError_t launch_ref_split(const Group &g, hipStream_t s);
+// element-matrix products on the matrix cores (csrc/elemmm.hip)
+enum { kMmF64 = 0, kMmF32 = 1, kMmBf16x3 = 2 };
+void elem_mm_dims(const Group &g, int64_t &NNp, int64_t &Kp);
+hipError_t launch_ref_products_t(const Group &g, int mode, void *Rt, uint16_t *Rt_lo, hipStream_t s);
+hipError_t launch_elem_mm(const Group &g, int mode, bool generated, int planes, const void *Ain, const void *Rt,
+                          const void *Rt_lo, void *out, hipStream_t s);
+hipError_t launch_coef_rows(const Group &g, const double *xyz, bool all_terms, int planes, bool fp32, hipStream_t s);
 // A host descriptor copied into device memory in stream order: the value
 // travels as a kernel argument (at most 4 KiB), so the host copy may change
 // as soon as the launch returns and nothing waits for the stream.

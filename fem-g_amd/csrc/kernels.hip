@@ -764,89 +764,26 @@ __global__ void __launch_bounds__(256) k_dense_to32(int64_t total, const double 
 //                  det * sum over mass terms of fac c(x_q) for s = 6,
 // the same quantity as the reference's per-point B^T W B
 // (fem-master/src/term/BilinearTerm.cpp:303-343): the element matrices are
-// one GEMM  dense[e*planes + p][pair] = sum_k A[e*planes + p][k] R[k][pair].
-template <typename T>
-__global__ void __launch_bounds__(256) k_ref_products(int N, int Q, const double *__restrict__ w,
-                                                      const double *__restrict__ bf, const double *__restrict__ gbf,
-                                                      T *__restrict__ R)
-{
-  const int NN = N * N;
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if(t >= (int64_t)7 * Q * NN) return;
-  const int k = (int)(t / NN), pair = (int)(t % NN);
-  const int q = k / 7, s = k % 7;
-  const int i = pair / N, j = pair % N;
-  const double *gj = &gbf[(q * N + j) * 3], *gi = &gbf[(q * N + i) * 3];
-  double v;
-  switch(s) {
-  case 0: v = gj[0] * gi[0]; break;
-  case 1: v = gj[1] * gi[1]; break;
-  case 2: v = gj[2] * gi[2]; break;
-  case 3: v = gj[0] * gi[1] + gj[1] * gi[0]; break;
-  case 4: v = gj[0] * gi[2] + gj[2] * gi[0]; break;
-  case 5: v = gj[1] * gi[2] + gj[2] * gi[1]; break;
-  default: v = bf[q * N + j] * bf[q * N + i]; break;
-  }
-  R[t] = (T)(w[q] * v);
-}
-
-// bfloat16 bits of x, rounded to nearest even (finite x)
-__device__ __forceinline__ uint32_t bf16_bits(float x)
-{
-  const uint32_t u = __float_as_uint(x);
-  return (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
-}
-
-// x = hi + lo + O(2^-17 |x|) with hi, lo bfloat16: hi = bf16(x), lo = bf16(x - hi)
-__device__ __forceinline__ void bf16_split(double x, uint16_t &hi, uint16_t &lo)
-{
-  const uint32_t h = bf16_bits((float)x);
-  hi = (uint16_t)h;
-  lo = (uint16_t)bf16_bits((float)(x - (double)__uint_as_float(h << 16)));
-}
-
-// R [K][NN] (K = 7Q) -> Rb [NN][3K], row `pair` = [hi(R_pair), lo(R_pair),
-// hi(R_pair)]: K-contiguous like the rows of A, so both GEMM operands stream
-// along the inner dimension
-__global__ void __launch_bounds__(256) k_ref_split(int64_t total, int64_t NN, const double *__restrict__ R,
-                                                   uint16_t *__restrict__ Rb)
-{
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if(t >= total) return;
-  const int64_t K = total / NN, kk = t / NN, pair = t - kk * NN;
-  uint16_t hi, lo;
-  bf16_split(R[t], hi, lo);
-  uint16_t *row = Rb + pair * 3 * K;
-  row[kk] = hi;
-  row[K + kk] = lo;
-  row[2 * K + kk] = hi;
-}
-
-// one thread per (element, Gauss point): the 7 entries of A for that point,
+// one GEMM  dense[e*planes + p][pair] = sum_k A[e*planes + p][k] R[k][pair],
+// computed on the matrix cores by csrc/elemmm.hip.  k_coef_rows writes the
+// rows of A where that kernel cannot generate them itself.
+//
+// One thread per (element, Gauss point): the 7 entries of A for that point,
 // per plane (real part, then imaginary part for complex systems).  A block's
 // rows are staged in LDS and written back with consecutive lanes on
 // consecutive addresses (a block's points are consecutive entries of each
-// row of A).  SPLIT writes row c of A as [hi(A_c), hi(A_c), lo(A_c)]
-// (bf16 bits) into Ab.  PP: some coefficient is an expression of x (without
-// it the kernel leaves out the point coordinates and the interpreter).
-template <typename T, bool SPLIT> struct CoefStage {
-  using type = T;
-};
-template <typename T> struct CoefStage<T, true> {
-  using type = uint32_t;  // hi | lo << 16
-};
-
+// row of A).  PP: some coefficient is an expression of x (without it the
+// kernel leaves out the point coordinates and the interpreter).
 constexpr int kCoefBlock = 256;
 
-template <typename T, bool SPLIT, bool PP>
+template <typename T, bool PP>
 __global__ void __launch_bounds__(kCoefBlock) k_coef_rows(const GroupDev *__restrict__ gd, int all_terms, int rule,
                                                           int64_t ne, const int32_t *__restrict__ vtx,
                                                           const double *__restrict__ xyz,
                                                           const double *__restrict__ F64, int planes,
-                                                          T *__restrict__ A, uint16_t *__restrict__ Ab = nullptr)
+                                                          T *__restrict__ A)
 {
-  using S = typename CoefStage<T, SPLIT>::type;
-  __shared__ S st[2][kCoefBlock * 7];
+  __shared__ T st[2][kCoefBlock * 7];
   const RuleDev &R = gd->rules[rule];
   const int Q = R.Q;
   const int64_t total = ne * Q, t0 = (int64_t)blockIdx.x * kCoefBlock;
@@ -896,14 +833,7 @@ __global__ void __launch_bounds__(kCoefBlock) k_coef_rows(const GroupDev *__rest
       const double cs = p == 0 ? stiff.x : stiff.y, cm = p == 0 ? mass.x : mass.y;
 #pragma unroll
       for(int s = 0; s < 7; ++s) {
-        const double v = F[s] * (s < 6 ? cs : cm);
-        if constexpr(SPLIT) {
-          uint16_t hi, lo;
-          bf16_split(v, hi, lo);
-          st[p][threadIdx.x * 7 + s] = (uint32_t)hi | (uint32_t)lo << 16;
-        }
-        else
-          st[p][threadIdx.x * 7 + s] = (T)v;
+        st[p][threadIdx.x * 7 + s] = (T)(F[s] * (s < 6 ? cs : cm));
       }
     }
   }
@@ -915,15 +845,7 @@ __global__ void __launch_bounds__(kCoefBlock) k_coef_rows(const GroupDev *__rest
       const uint32_t tl = (uint32_t)j / 7u, r = q0 + tl, de = r / (uint32_t)Q;
       const int64_t e = e0 + de;
       const int64_t col = (int64_t)(r - de * (uint32_t)Q) * 7 + (j - 7 * (int)tl);
-      const S v = st[p][j];
-      if constexpr(SPLIT) {
-        uint16_t *row = Ab + (e * planes + p) * 3 * K + col;
-        row[0] = (uint16_t)(v & 0xffffu);
-        row[K] = (uint16_t)(v & 0xffffu);
-        row[2 * K] = (uint16_t)(v >> 16);
-      }
-      else
-        A[(e * planes + p) * K + col] = v;
+      A[(e * planes + p) * K + col] = st[p][j];
     }
 }
 
@@ -1653,17 +1575,6 @@ hipError_t launch_dense_to32(const Group &g, int planes, hipStream_t s)
   return hipGetLastError();
 }
 
-hipError_t launch_ref_products(const Group &g, hipStream_t s)
-{
-  const Rule &R = g.rules[g.gemm_rule];
-  const int64_t total = (int64_t)7 * R.Q * g.N * g.N;
-  hipLaunchKernelGGL(k_ref_products<double>, dim3(blocks_for(total, 256)), dim3(256), 0, s, g.N, R.Q, R.d_w, R.d_bf,
-                     R.d_gbf, g.d_R64);
-  hipLaunchKernelGGL(k_ref_products<float>, dim3(blocks_for(total, 256)), dim3(256), 0, s, g.N, R.Q, R.d_w, R.d_bf,
-                     R.d_gbf, g.d_R32);
-  return hipGetLastError();
-}
-
 // ------------------------------------------------------------------ descriptor updates
 hipError_t launch_put_term(DevTerm *dst, const DevTerm &v, hipStream_t s)
 {
@@ -1692,38 +1603,26 @@ hipError_t launch_put_table(GroupTable *dst, const GroupTable &v, hipStream_t s)
   return hipGetLastError();
 }
 
-hipError_t launch_ref_split(const Group &g, hipStream_t s)
-{
-  const int64_t total = (int64_t)7 * g.rules[g.gemm_rule].Q * g.N * g.N;
-  hipLaunchKernelGGL(k_ref_split, dim3(blocks_for(total, 256)), dim3(256), 0, s, total, (int64_t)g.N * g.N, g.d_R64,
-                     g.d_Rb);
-  return hipGetLastError();
-}
-
-template <typename T, bool SPLIT>
+template <typename T>
 static void coef_rows_pp(const Group &g, const double *xyz, bool all_terms, int planes, T *A, hipStream_t s)
 {
   const int64_t total = g.ne * (int64_t)g.rules[g.gemm_rule].Q;
   const dim3 grid(blocks_for(total, kCoefBlock)), block(kCoefBlock);
-  uint16_t *Ab = SPLIT ? g.d_Ab : nullptr;
   if(g.has_dense)  // a bilinear term with a coefficient that depends on x
-    hipLaunchKernelGGL((k_coef_rows<T, SPLIT, true>), grid, block, 0, s, g.d_dev, all_terms ? 1 : 0, g.gemm_rule, g.ne,
-                       g.d_vtx, xyz, g.d_F64, planes, A, Ab);
+    hipLaunchKernelGGL((k_coef_rows<T, true>), grid, block, 0, s, g.d_dev, all_terms ? 1 : 0, g.gemm_rule, g.ne,
+                       g.d_vtx, xyz, g.d_F64, planes, A);
   else
-    hipLaunchKernelGGL((k_coef_rows<T, SPLIT, false>), grid, block, 0, s, g.d_dev, all_terms ? 1 : 0, g.gemm_rule,
-                       g.ne, g.d_vtx, xyz, g.d_F64, planes, A, Ab);
+    hipLaunchKernelGGL((k_coef_rows<T, false>), grid, block, 0, s, g.d_dev, all_terms ? 1 : 0, g.gemm_rule, g.ne,
+                       g.d_vtx, xyz, g.d_F64, planes, A);
 }
 
-hipError_t launch_coef_rows(const Group &g, const double *xyz, bool all_terms, int planes, bool fp32, bool split,
-                            hipStream_t s)
+hipError_t launch_coef_rows(const Group &g, const double *xyz, bool all_terms, int planes, bool fp32, hipStream_t s)
 {
   if(g.ne == 0) return hipSuccess;
-  if(split)
-    coef_rows_pp<float, true>(g, xyz, all_terms, planes, nullptr, s);
-  else if(fp32)
-    coef_rows_pp<float, false>(g, xyz, all_terms, planes, g.d_A32, s);
+  if(fp32)
+    coef_rows_pp<float>(g, xyz, all_terms, planes, g.d_A32, s);
   else
-    coef_rows_pp<double, false>(g, xyz, all_terms, planes, g.d_A64, s);
+    coef_rows_pp<double>(g, xyz, all_terms, planes, g.d_A64, s);
   return hipGetLastError();
 }
 

@@ -371,7 +371,8 @@ gfem_status gfem_batch_done(gfem_batch *batch, int max, int64_t *ids, int *count
 /* Solution (n complex128, may be NULL) and info of an ended request; frees it. */
 gfem_status gfem_batch_result(gfem_batch *batch, int64_t id, void *x_host, size_t bytes, gfem_solve_info *info);
 
-/* Arithmetic of the element-matrix GEMM (GFEM_STRATEGY_DENSE, and per-point
+/* Arithmetic of the element-matrix product on the matrix cores
+ * (GFEM_STRATEGY_DENSE, and per-point
  * terms in the fused strategy) under GFEM_PREC_MIXED:
  *   GFEM_GEMM_FP32   : FP32 operands on the FP32 matrix cores (default);
  *   GFEM_GEMM_BF16X3 : each operand x split as hi = bf16(x), lo = bf16(x - hi);

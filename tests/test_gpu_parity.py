@@ -269,6 +269,40 @@ def test_per_point_gemm_matches_direct_kernel(monkeypatch):
     assert_close(F2.values(), P.values(), FP64_RTOL, "direct A")
     F1.assemble(precision="mixed")
     assert_close(F1.values(), P.values(), DENSE_FP32_RTOL, "gemm A (fp32)")
+    # every term as element matrices, with the A rows read from memory
+    F1.assemble(precision="fp64", strategy="dense")
+    assert_close(F1.values(), P.values(), FP64_RTOL, "dense gemm A (fp64, read rows)")
+    F1.set_gemm_mode("bf16x3")
+    F1.assemble(precision="mixed", strategy="dense")
+    assert_close(F1.values(), P.values(), BF16X3_RTOL, "dense gemm A (bf16x3, read rows)")
+
+
+@pytest.mark.parametrize("gemm", ["fp64", "fp32", "bf16x3"])
+def test_generated_and_read_gemm_rows_agree(gemm):
+    """The element-matrix kernel generates A from the element factors when
+    every coefficient is constant and reads k_coef_rows' rows otherwise.  The
+    same complex Helmholtz problem with its mass coefficient written as an
+    expression (-k^2 + 0 x) takes the read path: both agree with the oracle
+    to the arithmetic's bar, and with each other."""
+    from gfem import workloads as W, x
+    print(f"case generated vs read gemm rows {gemm}", flush=True)
+    spec = W.helmholtz3d(n=4, order=2)
+    k2 = spec.terms[1]["coef"]
+    F1, P = _pair(spec)
+    spec_x = W.helmholtz3d(n=4, order=2)
+    spec_x.terms[1]["coef"] = k2 + 0.0 * x()
+    F2 = W.build(spec_x)
+    F2.pre()
+    prec = "fp64" if gemm == "fp64" else "mixed"
+    tol = {"fp64": FP64_RTOL, "fp32": DENSE_FP32_RTOL, "bf16x3": BF16X3_RTOL}[gemm]
+    vals = []
+    for F in (F1, F2):
+        F.assemble(download=False)  # creates the system
+        F.set_gemm_mode("bf16x3" if gemm == "bf16x3" else "fp32")
+        F.assemble(precision=prec, strategy="dense")
+        assert_close(F.values(), P.values(), tol, f"{gemm} A")
+        vals.append(F.values())
+    assert_close(vals[1], vals[0], tol, f"{gemm} read vs generated")
 
 
 def test_autotuned_assembly_matches_untuned(monkeypatch):
