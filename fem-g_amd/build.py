@@ -76,7 +76,8 @@ def build(verbose=False):
     libhost = os.path.join(OUT, "libgfem_host.so")
     run(["g++", "-shared", "-o", libhost, *host_objs, "-L", OUT, "-lgfem", "-Wl,-rpath,$ORIGIN"])
     run(["g++", "-shared", "-o", os.path.join(OUT, "_gfem_host" + ext), py_obj,
-         "-L", OUT, "-lgfem_host", "-lgfem", "-Wl,-rpath,$ORIGIN"])
+         "-L", OUT, "-lgfem_host", "-lgfem", "-L/opt/rocm/lib", "-lamdhip64", "-Wl,-rpath,$ORIGIN",
+         "-Wl,-rpath,/opt/rocm/lib"])
 
 
 def build_oracle():

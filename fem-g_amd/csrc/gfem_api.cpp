@@ -11,6 +11,7 @@
 #include <mutex>
 #include <new>
 #include <string>
+#include <tuple>
 #include <vector>
 
 
@@ -240,6 +241,12 @@ gfem_status gfem_ctx_destroy(gfem_ctx *ctx)
   if(!ctx) return GFEM_OK;
   (void)hipSetDevice(ctx->device);
   dfree(ctx->d_xyz);
+  if(ctx->stage_done) {
+    (void)hipEventSynchronize(ctx->stage_done);
+    (void)hipEventDestroy(ctx->stage_done);
+  }
+  if(ctx->h_stage) (void)hipHostFree(ctx->h_stage);
+  if(ctx->d_stage) (void)hipFree(ctx->d_stage);
   delete ctx;
   return GFEM_OK;
 }
@@ -933,6 +940,11 @@ static gfem_status ensure_reference_maps(gfem_system *sys)
 static gfem_status apply_updates(gfem_system *sys, hipStream_t s)
 {
   gfem_ctx *ctx = sys->ctx;
+  if(sys->terms_stale) {
+    for(auto &g : sys->groups)
+      for(size_t k = 0; k < g.dterms.size(); ++k) GFEM_HIP(ctx, launch_put_term(g.d_terms + k, g.dterms[k], s));
+    sys->terms_stale = false;
+  }
   for(auto &g : sys->groups)
     if(g.tensor_dirty) {
       GFEM_HIP(ctx, launch_tensor(g, s));
@@ -1248,6 +1260,318 @@ gfem_status gfem_assemble_into_noalloc(gfem_system *sys, gfem_precision prec, vo
   return GFEM_OK;
 }
 
+// ------------------------------------------------------------------ many instances
+// Several parameter sets on resident systems in a few launches.  Instances
+// go in waves: the k-th instance of each system joins wave k, so a wave holds
+// at most one instance per system and each instance works in its system's
+// own buffers (tensors, element loads), as a standalone assembly does.  The
+// parameters of every instance are applied on the host in instance order;
+// every item table of every wave is packed on the host and uploaded in one
+// copy; then each wave runs one launch per kernel kind for all its systems
+// (tensor rebuilds; compiled element loads per program; RHS rows and the
+// matrix gather per tile choice), each block doing exactly what it does in
+// the standalone launch of its system.  Element factors depend on the mesh
+// alone and are computed once per system.
+static gfem_status term_param_host(gfem_system *sys, int group, int term, double coef_re, double coef_im,
+                                   int n_consts, const double *consts, bool check_only, bool &be_flip);
+
+extern "C++" {
+namespace {
+
+struct ManyLaunch {
+  int kind;      // 0 tensors, 1 compiled element loads, 2 RHS, 3 gather, 4 system-wide action
+  size_t items;  // byte offset of the items in the staging buffer
+  size_t first;  // byte offset of the first-block table (RHS, gather)
+  int nitems;
+  int64_t total;   // blocks (RHS, gather)
+  int maxN;        // tensors: largest N
+  int64_t max_ne;  // element loads: largest element count
+  hipFunction_t fn;
+  bool cplx, wide;
+  int tile;        // RHS lanes / gather slots
+  // kind 4: per-system work in stream order before the fused launches
+  gfem_system *sys;
+  bool zero_be, put_table, geometry;
+  GroupTable table;
+  std::vector<std::pair<int, int>> fallback_linear;  // (group, eb_shift): interpreted element loads
+};
+
+template <typename T>
+size_t stage_put(std::vector<unsigned char> &buf, const std::vector<T> &v)
+{
+  const size_t off = (buf.size() + 255) & ~size_t(255);
+  buf.resize(off + sizeof(T) * v.size());
+  if(!v.empty()) std::memcpy(buf.data() + off, v.data(), sizeof(T) * v.size());
+  return off;
+}
+
+}  // namespace
+}  // extern "C++"
+
+gfem_status gfem_assemble_many(int n, const gfem_instance *inst, gfem_precision prec, void *stream)
+{
+  if(n < 0 || (n > 0 && !inst)) return fail(nullptr, GFEM_ERR_INVALID_ARG, "assemble_many: bad arguments");
+  if(n == 0) return GFEM_OK;
+  if(!inst[0].sys) return fail(nullptr, GFEM_ERR_INVALID_ARG, "assemble_many: null system");
+  gfem_ctx *ctx = inst[0].sys->ctx;
+  const int p = (int)prec;
+  const bool fp32 = prec == GFEM_PREC_MIXED;
+  // everything is checked before anything changes
+  for(int i = 0; i < n; ++i) {
+    gfem_system *sys = inst[i].sys;
+    if(!sys || sys->ctx != ctx) return fail(ctx, GFEM_ERR_INVALID_ARG, "assemble_many: systems of one context only");
+    gfem_status st = check_assemble(sys, prec, GFEM_STRATEGY_FUSED);
+    if(st != GFEM_OK) return st;
+    if(!inst[i].vals || !inst[i].rhs) return fail(ctx, GFEM_ERR_INVALID_ARG, "assemble_many: null output");
+    if(inst[i].n_params < 0 || (inst[i].n_params > 0 && !inst[i].params))
+      return fail(ctx, GFEM_ERR_INVALID_ARG, "assemble_many: bad parameter list");
+    for(const Group &g : sys->groups)
+      if(g.has_dense)
+        return fail(ctx, GFEM_ERR_UNSUPPORTED,
+                    "assemble_many: per-point element matrices (expression coefficients of bilinear terms, curved "
+                    "elements) are assembled one system at a time");
+    for(int k = 0; k < inst[i].n_params; ++k) {
+      const gfem_term_param &q = inst[i].params[k];
+      bool flip = false;
+      st = term_param_host(sys, q.group, q.term, q.coef_re, q.coef_im, q.n_consts, q.consts, true, flip);
+      if(st != GFEM_OK) return st;
+    }
+  }
+  GFEM_HIP(ctx, hipSetDevice(ctx->device));
+  const hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  // waves
+  std::vector<int> wave(n);
+  int nwaves = 0;
+  {
+    std::vector<std::pair<gfem_system *, int>> seen;
+    for(int i = 0; i < n; ++i) {
+      int c = 0;
+      for(auto &kv : seen)
+        if(kv.first == inst[i].sys) c = ++kv.second;
+      if(c == 0) seen.push_back({inst[i].sys, 0});
+      wave[i] = c;
+      nwaves = std::max(nwaves, c + 1);
+    }
+  }
+  // phase A: parameters on the host, the launch plan and its item tables
+  std::vector<unsigned char> buf;
+  std::vector<std::vector<ManyLaunch>> plan(nwaves);
+  for(int w = 0; w < nwaves; ++w) {
+    std::vector<TensorItem> tens;
+    int maxN = 0;
+    std::vector<std::pair<hipFunction_t, std::vector<JitLinItem>>> lin_rounds[4];  // rounds of element loads
+    std::vector<std::tuple<bool, int, std::vector<RhsItem>, std::vector<int64_t>>> rhs_groups;
+    std::vector<std::tuple<bool, bool, int, std::vector<GatherItem>, std::vector<int64_t>>> gat_groups;
+    for(int i = 0; i < n; ++i) {
+      if(wave[i] != w) continue;
+      gfem_system *sys = inst[i].sys;
+      ManyLaunch act{};
+      act.kind = 4;
+      act.sys = sys;
+      for(int k = 0; k < inst[i].n_params; ++k) {
+        const gfem_term_param &q = inst[i].params[k];
+        bool flip = false;
+        (void)term_param_host(sys, q.group, q.term, q.coef_re, q.coef_im, q.n_consts, q.consts, false, flip);
+        act.zero_be = act.zero_be || flip;
+      }
+      if(inst[i].n_params > 0) sys->terms_stale = true;
+      if(sys->table_dirty) {
+        act.put_table = true;
+        act.table = make_table(sys, false);
+        sys->table_dirty = false;
+      }
+      act.geometry = !sys->geom_valid[p];
+      sys->geom_valid[0] = true;
+      if(fp32) sys->geom_valid[1] = true;
+      for(int gi = 0; gi < (int)sys->groups.size(); ++gi) {
+        Group &g = sys->groups[gi];
+        if(g.tensor_dirty) {
+          tens.push_back(TensorItem{tensor_args(g), g.d_T64, g.d_T32, g.d_T64r, g.d_T32r});
+          maxN = std::max(maxN, g.N);
+          g.tensor_dirty = false;
+        }
+        if(!g.has_linear || g.ne == 0) continue;
+        // compiled programs for every linear term of the group, else the
+        // group's element loads run on their own (interpreted)
+        bool all_jit = !g.curved;
+        for(size_t k = 0; k < g.dterms.size() && all_jit; ++k)
+          if(g.dterms[k].kind == 1 && (k >= g.jit.size() || !g.jit[k] || !jit_linear_many(g.jit[k]))) all_jit = false;
+        if(!all_jit) {
+          act.fallback_linear.push_back({gi, sys->linear_eb_shift[p]});
+          continue;
+        }
+        for(int r = 0; r < sys->nrhs; ++r) {
+          int round = 0;
+          for(size_t k = 0; k < g.dterms.size(); ++k) {
+            const DevTerm &t = g.dterms[k];
+            if(t.kind != 1 || t.rhs_idx != r) continue;
+            if(round >= 4) return fail(ctx, GFEM_ERR_UNSUPPORTED, "assemble_many: more than 4 linear terms on a group");
+            const Rule &R = g.rules[t.rule];
+            JitLinItem it{};
+            for(int q = 0; q < 2 * GFEM_MAX_CONSTS; ++q) it.a.consts[q] = t.expr.consts[q];
+            it.a.fac_re = t.fac_re;
+            it.a.fac_im = t.fac_im;
+            it.a.Q = R.Q;
+            it.a.dim = g.dim;
+            it.a.nv = g.nv;
+            it.a.N = g.N;
+            it.a.accumulate = round ? 1 : 0;
+            it.a.EB = 0;
+            it.a.real_out = sys->be_real ? 1 : 0;
+            it.w = R.d_w;
+            it.pts = R.d_pts;
+            it.bf = R.d_bf;
+            it.ne = g.ne;
+            it.vtx = g.d_vtx;
+            it.xyz = sys->d_xyz;
+            it.F64 = g.d_F64;
+            it.be = sys->be_real ? (void *)(reinterpret_cast<double *>(g.d_be - g.be_off) + (int64_t)r * g.be_stride +
+                                            g.be_off)
+                                 : (void *)(g.d_be + (int64_t)r * g.be_stride);
+            hipFunction_t fm = jit_linear_many(g.jit[k]);
+            auto &rv = lin_rounds[round];
+            auto f = std::find_if(rv.begin(), rv.end(), [&](const auto &e) { return e.first == fm; });
+            if(f == rv.end()) {
+              rv.push_back({fm, {}});
+              f = rv.end() - 1;
+            }
+            f->second.push_back(it);
+            ++round;
+          }
+        }
+      }
+      plan[w].push_back(std::move(act));
+      // RHS rows and matrix gather, grouped by tile choice
+      if(sys->n > 0) {
+        const int lanes = sys->rhs_lanes[p];
+        auto f = std::find_if(rhs_groups.begin(), rhs_groups.end(),
+                              [&](const auto &e) { return std::get<0>(e) == sys->cplx && std::get<1>(e) == lanes; });
+        if(f == rhs_groups.end()) {
+          rhs_groups.push_back({sys->cplx, lanes, {}, {}});
+          f = rhs_groups.end() - 1;
+        }
+        auto &first = std::get<3>(*f);
+        const int64_t b0 = first.empty() ? 0 : first.back();
+        std::get<2>(*f).push_back(rhs_item(*sys, inst[i].rhs));
+        if(first.empty()) first.push_back(0);
+        first.push_back(b0 + rhs_item_blocks(*sys, p));
+      }
+      if(sys->n_chunks > 0) {
+        const bool wide = sys->rec_wide != 0;
+        const int slots = sys->gather_slots[p];
+        auto f = std::find_if(gat_groups.begin(), gat_groups.end(), [&](const auto &e) {
+          return std::get<0>(e) == sys->cplx && std::get<1>(e) == wide && std::get<2>(e) == slots;
+        });
+        if(f == gat_groups.end()) {
+          gat_groups.push_back({sys->cplx, wide, slots, {}, {}});
+          f = gat_groups.end() - 1;
+        }
+        auto &first = std::get<4>(*f);
+        const int64_t b0 = first.empty() ? 0 : first.back();
+        std::get<3>(*f).push_back(gather_item(*sys, inst[i].vals));
+        if(first.empty()) first.push_back(0);
+        first.push_back(b0 + gather_item_blocks(*sys));
+      }
+    }
+    if(!tens.empty()) {
+      ManyLaunch L{};
+      L.kind = 0;
+      L.items = stage_put(buf, tens);
+      L.nitems = (int)tens.size();
+      L.maxN = maxN;
+      plan[w].push_back(L);
+    }
+    for(auto &rounds : lin_rounds)
+      for(auto &fv : rounds) {
+        ManyLaunch L{};
+        L.kind = 1;
+        L.fn = fv.first;
+        L.items = stage_put(buf, fv.second);
+        L.nitems = (int)fv.second.size();
+        for(const auto &it : fv.second) L.max_ne = std::max<int64_t>(L.max_ne, it.ne);
+        plan[w].push_back(L);
+      }
+    for(auto &e : rhs_groups) {
+      ManyLaunch L{};
+      L.kind = 2;
+      L.cplx = std::get<0>(e);
+      L.tile = std::get<1>(e);
+      L.items = stage_put(buf, std::get<2>(e));
+      L.first = stage_put(buf, std::get<3>(e));
+      L.nitems = (int)std::get<2>(e).size();
+      L.total = std::get<3>(e).back();
+      plan[w].push_back(L);
+    }
+    for(auto &e : gat_groups) {
+      ManyLaunch L{};
+      L.kind = 3;
+      L.cplx = std::get<0>(e);
+      L.wide = std::get<1>(e);
+      L.tile = std::get<2>(e);
+      L.items = stage_put(buf, std::get<3>(e));
+      L.first = stage_put(buf, std::get<4>(e));
+      L.nitems = (int)std::get<3>(e).size();
+      L.total = std::get<4>(e).back();
+      plan[w].push_back(L);
+    }
+  }
+  // phase B: one upload of every table (the pinned buffer is reused once its
+  // previous upload has completed)
+  if(!buf.empty()) {
+    if(ctx->stage_done) GFEM_HIP(ctx, hipEventSynchronize(ctx->stage_done));
+    else GFEM_HIP(ctx, hipEventCreateWithFlags(&ctx->stage_done, hipEventDisableTiming));
+    if(ctx->stage_bytes < buf.size()) {
+      if(ctx->h_stage) GFEM_HIP(ctx, hipHostFree(ctx->h_stage));
+      if(ctx->d_stage) GFEM_HIP(ctx, hipFree(ctx->d_stage));
+      ctx->h_stage = ctx->d_stage = nullptr;
+      ctx->stage_bytes = 0;
+      const size_t want = std::max<size_t>(buf.size() * 2, size_t(1) << 20);
+      GFEM_HIP(ctx, hipHostMalloc(&ctx->h_stage, want));
+      GFEM_HIP(ctx, hipMalloc(&ctx->d_stage, want));
+      ctx->stage_bytes = want;
+    }
+    std::memcpy(ctx->h_stage, buf.data(), buf.size());
+    GFEM_HIP(ctx, hipMemcpyAsync(ctx->d_stage, ctx->h_stage, buf.size(), hipMemcpyHostToDevice, s));
+    GFEM_HIP(ctx, hipEventRecord(ctx->stage_done, s));
+  }
+  unsigned char *dev = static_cast<unsigned char *>(ctx->d_stage);
+  // phase C: the launches, wave after wave
+  for(int w = 0; w < nwaves; ++w)
+    for(const ManyLaunch &L : plan[w]) {
+      switch(L.kind) {
+      case 4: {
+        gfem_system *sys = L.sys;
+        if(L.zero_be && sys->d_be_all)
+          GFEM_HIP(ctx, launch_zero(sys->d_be_all, sizeof(double2) * (size_t)(sys->nrhs * sys->be_total), s));
+        if(L.put_table) GFEM_HIP(ctx, launch_put_table(sys->d_table, L.table, s));
+        if(L.geometry) GFEM_HIP(ctx, launch_geometry_all(*sys, sys->d_xyz, fp32, s));
+        for(const auto &gl : L.fallback_linear)
+          GFEM_HIP(ctx, launch_linear(sys->groups[gl.first], sys->d_xyz, sys->nrhs, gl.second, sys->be_real, s));
+        break;
+      }
+      case 0:
+        GFEM_HIP(ctx, launch_tensor_many(reinterpret_cast<const TensorItem *>(dev + L.items), L.nitems, L.maxN, s));
+        break;
+      case 1: {
+        const JitLinItem *items = reinterpret_cast<const JitLinItem *>(dev + L.items);
+        void *params[] = {&items};
+        const unsigned bs = 128, gx = (unsigned)((L.max_ne + bs - 1) / bs);
+        GFEM_HIP(ctx, hipModuleLaunchKernel(L.fn, gx, (unsigned)L.nitems, 1, bs, 1, 1, 0, s, params, nullptr));
+        break;
+      }
+      case 2:
+        GFEM_HIP(ctx, launch_rhs_many(L.cplx, fp32, L.tile, reinterpret_cast<const RhsItem *>(dev + L.items),
+                                      reinterpret_cast<const int64_t *>(dev + L.first), L.nitems, L.total, s));
+        break;
+      case 3:
+        GFEM_HIP(ctx, launch_gather_many(L.cplx, fp32, L.wide, L.tile, reinterpret_cast<const GatherItem *>(dev + L.items),
+                                         reinterpret_cast<const int64_t *>(dev + L.first), L.nitems, L.total, s));
+        break;
+      }
+    }
+  return GFEM_OK;
+}
+
 static gfem_status assemble_impl(gfem_system *sys, gfem_precision prec, gfem_strategy strategy, void *vals_out,
                                  void *rhs_out, hipStream_t s)
 {
@@ -1271,6 +1595,8 @@ static gfem_status assemble_impl(gfem_system *sys, gfem_precision prec, gfem_str
     prof.begin("geometry");
     GFEM_HIP(ctx, launch_geometry_all(*sys, sys->d_xyz, fp32, s));
     prof.end();
+    sys->geom_valid[0] = true;
+    if(fp32) sys->geom_valid[1] = true;
     for(auto &g : sys->groups) {
       if(!(all ? g.has_bilinear : g.has_dense)) continue;
       if(g.gemm_rule >= 0 && (all || sys->dense_gemm)) {
@@ -1341,11 +1667,15 @@ static gfem_status assemble_impl(gfem_system *sys, gfem_precision prec, gfem_str
   return GFEM_OK;
 }
 
-gfem_status gfem_term_update(gfem_system *sys, int group, int term, double coef_re, double coef_im, int n_consts,
-                             const double *consts, void *stream)
+// Host side of a term parameter change: checks it against the term and
+// applies it to the host descriptors (tensor rebuild and group-table flags);
+// be_flip: the element loads change layout (real <-> complex) and must be
+// cleared before the next assembly.  check_only: validate, change nothing.
+static gfem_status term_param_host(gfem_system *sys, int group, int term, double coef_re, double coef_im,
+                                   int n_consts, const double *consts, bool check_only, bool &be_flip)
 {
-  if(!sys) return fail(nullptr, GFEM_ERR_INVALID_ARG, "system is null");
   gfem_ctx *ctx = sys->ctx;
+  be_flip = false;
   if(!sys->finalized) return fail(ctx, GFEM_ERR_STATE, "term_update before finalize");
   if(group < 0 || group >= (int)sys->groups.size()) return fail(ctx, GFEM_ERR_INVALID_ARG, "bad group index");
   Group &g = sys->groups[group];
@@ -1359,23 +1689,20 @@ gfem_status gfem_term_update(gfem_system *sys, int group, int term, double coef_
     for(int q = 0; q < n_consts; ++q)
       if(d.expr.real_only && consts[2 * q + 1] != 0.)
         return fail(ctx, GFEM_ERR_INVALID_ARG, "term_update: a real coefficient program cannot take complex constants");
+    if(check_only) return GFEM_OK;
     for(int q = 0; q < 2 * n_consts; ++q) d.expr.consts[q] = consts[q];
   }
   else {
     if(n_consts != 0) return fail(ctx, GFEM_ERR_INVALID_ARG, "term_update: the term has a constant coefficient");
+    if(check_only) return GFEM_OK;
     d.c_re = coef_re;
     d.c_im = coef_im;
     if(d.kind == 0) g.tensor_dirty = true;
   }
-  GFEM_HIP(ctx, hipSetDevice(ctx->device));
   const bool be_real = linear_terms_real(sys);
   if(be_real != sys->be_real) {
-    // the element loads change layout: clear them, so right-hand sides a
-    // group has no term for read zeros in the new layout too
     sys->be_real = be_real;
-    if(sys->d_be_all)
-      GFEM_HIP(ctx, launch_zero(sys->d_be_all, sizeof(double2) * (size_t)(sys->nrhs * sys->be_total),
-                                reinterpret_cast<hipStream_t>(stream)));
+    be_flip = true;
   }
   bool treal = true;
   for(const DevTerm &e : g.dterms)
@@ -1384,8 +1711,31 @@ gfem_status gfem_term_update(gfem_system *sys, int group, int term, double coef_
     g.treal = treal;
     sys->table_dirty = true;
   }
+  return GFEM_OK;
+}
+
+gfem_status gfem_term_update(gfem_system *sys, int group, int term, double coef_re, double coef_im, int n_consts,
+                             const double *consts, void *stream)
+{
+  if(!sys) return fail(nullptr, GFEM_ERR_INVALID_ARG, "system is null");
+  gfem_ctx *ctx = sys->ctx;
+  bool be_flip = false;
+  gfem_status st = term_param_host(sys, group, term, coef_re, coef_im, n_consts, consts, false, be_flip);
+  if(st != GFEM_OK) return st;
+  GFEM_HIP(ctx, hipSetDevice(ctx->device));
+  const hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  // the element loads change layout: clear them, so right-hand sides a group
+  // has no term for read zeros in the new layout too
+  if(be_flip && sys->d_be_all)
+    GFEM_HIP(ctx, launch_zero(sys->d_be_all, sizeof(double2) * (size_t)(sys->nrhs * sys->be_total), s));
   // in stream order, without waiting: the descriptor travels as a kernel argument
-  GFEM_HIP(ctx, launch_put_term(g.d_terms + term, d, reinterpret_cast<hipStream_t>(stream)));
+  Group &g = sys->groups[group];
+  if(sys->terms_stale) {
+    for(size_t k = 0; k < g.dterms.size(); ++k) GFEM_HIP(ctx, launch_put_term(g.d_terms + k, g.dterms[k], s));
+  }
+  else {
+    GFEM_HIP(ctx, launch_put_term(g.d_terms + term, g.dterms[term], s));
+  }
   return GFEM_OK;
 }
 

@@ -90,6 +90,48 @@ struct GroupTable {
   int64_t ne[kMaxGroups];
 };
 
+// Constant-coefficient reference tensors of a group (k_tensor): per term its
+// forms, rule and full constant factor; per rule its points.
+struct TensorArgs {
+  int N;
+  int nterms;
+  int form_dof[16], form_tf[16], rule[16], active[16];
+  double2 fac[16];  // full constant factor of the term
+  int Q[8];
+  const double *w[8];
+  const double *bf[8];
+  const double *gbf[8];
+};
+
+// Items of the launches over several systems (gfem_assemble_many): each is
+// what the standalone launch of one system takes.
+struct TensorItem {
+  TensorArgs a;
+  double2 *T64;
+  float2 *T32;
+  double *T64r;
+  float *T32r;
+};
+struct GatherItem {
+  const GroupTable *table;
+  const uint32_t *recs;
+  void *out;
+};
+struct RhsItem {
+  const GroupTable *table;
+  int64_t n;
+  int nrhs, be_real;
+  const int64_t *dc_ptr;
+  const uint32_t *dc_words;
+  const int32_t *dc_fix;
+  const double2 *uD;
+  const int64_t *lin_ptr;
+  const uint32_t *lin_words;
+  const double2 *be_all;
+  int64_t be_total;
+  void *out;
+};
+
 // Device-resident descriptor of one group's rules and terms.
 struct RuleDev {
   int Q;
@@ -185,6 +227,11 @@ struct gfem_ctx {
   std::vector<double> h_xyz;
   std::string err;
   void *scratch = nullptr;  // device address of this GPU's no-allocation scratch array
+  // staging of the item tables of gfem_assemble_many: pinned host and device
+  // buffers, reused once the last upload from the pinned one has completed
+  void *h_stage = nullptr, *d_stage = nullptr;
+  size_t stage_bytes = 0;
+  hipEvent_t stage_done = nullptr;
 };
 
 struct gfem_system {
@@ -258,6 +305,11 @@ struct gfem_system {
   hipStream_t sol_cap = nullptr;  // capture stream (default priority)
   // parameters changed by gfem_term_update, applied at the next assembly
   bool table_dirty = false;
+  // the device copies of the term descriptors lag the host's (after
+  // gfem_assemble_many, which reads the parameters from its own tables)
+  bool terms_stale = false;
+  // element factors of the current nodes are in place (FP64, FP32 copy)
+  bool geom_valid[2] = {false, false};
   // profiling
   bool profiling = false;
   std::vector<const char *> prof_names;
@@ -349,6 +401,18 @@ __global__ void __launch_bounds__(64) k_put(T *__restrict__ dst, const T v)
 }
 hipError_t launch_put_term(DevTerm *dst, const DevTerm &v, hipStream_t s);
 hipError_t launch_put_table(GroupTable *dst, const GroupTable &v, hipStream_t s);
+// launches over several systems (gfem_assemble_many); items and first-block
+// tables in device memory
+TensorArgs tensor_args(const Group &g);
+hipError_t launch_tensor_many(const TensorItem *items, int nitems, int maxN, hipStream_t s);
+GatherItem gather_item(const gfem_system &sys, void *out);
+int64_t gather_item_blocks(const gfem_system &sys);
+hipError_t launch_gather_many(bool cplx, bool mixed, bool wide, int slots, const GatherItem *items,
+                              const int64_t *first, int nitems, int64_t total, hipStream_t s);
+RhsItem rhs_item(const gfem_system &sys, void *out);
+int64_t rhs_item_blocks(const gfem_system &sys, int prec);
+hipError_t launch_rhs_many(bool cplx, bool mixed, int lanes, const RhsItem *items, const int64_t *first, int nitems,
+                           int64_t total, hipStream_t s);
 // zero `bytes` (a multiple of 8) at p, in stream order, with a kernel
 hipError_t launch_zero(void *p, size_t bytes, hipStream_t s);
 // tensor = 0: the constant-coefficient tensors are left out (every bilinear
@@ -363,5 +427,23 @@ hipError_t launch_ref_rhs(const gfem_system &sys, const Group &g, hipStream_t s)
 // The kernel is specialised for the group's shape: Q Gauss points, N dofs,
 // nv vertices, dimension dim.
 hipFunction_t jit_linear_kernel(int device, const DevExpr &ex, int Q, int N, int nv, int dim, std::string *log);
+// the entry of the same compiled program over several groups (items
+// JitLinItem, blockIdx.y), or null
+hipFunction_t jit_linear_many(hipFunction_t single);
+// argument block of a compiled element-load kernel (layout of the compiled
+// source's JitLinArgs / JitLinItem)
+struct JitLinArgs {
+  double consts[2 * GFEM_MAX_CONSTS];
+  double fac_re, fac_im;
+  int Q, dim, nv, N, accumulate, EB, real_out;
+};
+struct JitLinItem {
+  JitLinArgs a;
+  const double *w, *pts, *bf;
+  int64_t ne;
+  const int32_t *vtx;
+  const double *xyz, *F64;
+  void *be;
+};
 
 }  // namespace gfem

@@ -68,14 +68,12 @@ struct JitLinArgs {
 // right-hand side).  Rule and basis are read with uniform (scalar) loads;
 // no LDS, no barrier.
 const char *kLinearBody = R"(
-extern "C" __global__ void __launch_bounds__(512)
-gfem_jit_linear(JitLinArgs a, const double *__restrict__ w, const double *__restrict__ pts,
-                const double *__restrict__ bf, i64 ne, const int *__restrict__ vtx,
-                const double *__restrict__ xyz, const double *__restrict__ F64, void *__restrict__ be)
+__device__ __forceinline__ void lin_element(const JitLinArgs &a, const double *__restrict__ w,
+                                            const double *__restrict__ pts, const double *__restrict__ bf, i64 e,
+                                            const int *__restrict__ vtx, const double *__restrict__ xyz,
+                                            const double *__restrict__ F64, void *__restrict__ be)
 {
   constexpr int Q = @Q@, N = @N@, NV = @NV@, DIM = @DIM@;
-  const i64 e = (i64)blockIdx.x * blockDim.x + threadIdx.x;
-  if(e >= ne) return;
   double X[NV][3];
 #pragma unroll
   for(int v = 0; v < NV; ++v) {
@@ -133,6 +131,33 @@ gfem_jit_linear(JitLinArgs a, const double *__restrict__ w, const double *__rest
 #pragma unroll
     for(int i = 0; i < N; ++i) o[i] = acc[i];
   }
+}
+
+extern "C" __global__ void __launch_bounds__(512)
+gfem_jit_linear(JitLinArgs a, const double *__restrict__ w, const double *__restrict__ pts,
+                const double *__restrict__ bf, i64 ne, const int *__restrict__ vtx,
+                const double *__restrict__ xyz, const double *__restrict__ F64, void *__restrict__ be)
+{
+  const i64 e = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if(e >= ne) return;
+  lin_element(a, w, pts, bf, e, vtx, xyz, F64, be);
+}
+
+// the same program over the groups of several systems: item blockIdx.y
+struct JitLinItem {
+  JitLinArgs a;
+  const double *w, *pts, *bf;
+  i64 ne;
+  const int *vtx;
+  const double *xyz, *F64;
+  void *be;
+};
+extern "C" __global__ void __launch_bounds__(512) gfem_jit_linear_many(const JitLinItem *__restrict__ items)
+{
+  const JitLinItem &it = items[blockIdx.y];
+  const i64 e = (i64)blockIdx.x * blockDim.x + threadIdx.x;
+  if(e >= it.ne) return;
+  lin_element(it.a, it.w, it.pts, it.bf, e, it.vtx, it.xyz, it.F64, it.be);
 }
 )";
 
@@ -222,6 +247,7 @@ std::string expr_code(const DevExpr &ex)
 struct Cache {
   std::mutex m;
   std::map<std::pair<int, std::string>, hipFunction_t> fn;  // (device, source) -> kernel
+  std::map<hipFunction_t, hipFunction_t> many;              // kernel -> its entry over several groups
   std::vector<hipModule_t> modules;
 };
 
@@ -280,9 +306,21 @@ hipFunction_t jit_linear_kernel(int device, const DevExpr &ex, int Q, int N, int
   hipFunction_t fn = nullptr;
   if(hipModuleLoadData(&mod, bin.data()) != hipSuccess) return nullptr;
   if(hipModuleGetFunction(&fn, mod, "gfem_jit_linear") != hipSuccess) return nullptr;
+  hipFunction_t fm = nullptr;
+  if(hipModuleGetFunction(&fm, mod, "gfem_jit_linear_many") != hipSuccess) fm = nullptr;
   C.modules.push_back(mod);
   C.fn[key] = fn;
+  C.many[fn] = fm;
   return fn;
+}
+
+hipFunction_t jit_linear_many(hipFunction_t single)
+{
+  if(!single) return nullptr;
+  Cache &C = cache();
+  std::lock_guard<std::mutex> lk(C.m);
+  auto it = C.many.find(single);
+  return it == C.many.end() ? nullptr : it->second;
 }
 
 }  // namespace gfem

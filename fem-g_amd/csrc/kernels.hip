@@ -387,26 +387,13 @@ __global__ void k_basis(int dim, int p, int N, int Q, const int *__restrict__ no
 }
 
 // ------------------------------------------------------------------ k_tensor
-struct TensorArgs {
-  int N;
-  int nterms;
-  int form_dof[16], form_tf[16], rule[16], active[16];
-  double2 fac[16];  // full constant factor of the term
-  int Q[8];
-  const double *w[8];
-  const double *bf[8];
-  const double *gbf[8];
-};
-
 // T[pair][c], pair = i_tf*N + j_dof:
 //   c=0..2  sum_q w dphi_j/du_c dphi_i/du_c      c=3..5 symmetric cross terms
 //   c=6     sum_q w phi_j phi_i
 // weighted by each term's constant factor and summed over terms.
-__global__ void k_tensor(TensorArgs a, double2 *__restrict__ T64, float2 *__restrict__ T32, double *__restrict__ T64r,
-                         float *__restrict__ T32r)
+__device__ __forceinline__ void tensor_pair(int pair, const TensorArgs &a, double2 *__restrict__ T64,
+                                            float2 *__restrict__ T32, double *__restrict__ T64r, float *__restrict__ T32r)
 {
-  const int pair = blockIdx.x * blockDim.x + threadIdx.x;
-  if(pair >= a.N * a.N) return;
   const int i = pair / a.N, j = pair % a.N;
   double2 acc[8];
   for(int c = 0; c < 8; ++c) acc[c] = make_double2(0., 0.);
@@ -438,6 +425,23 @@ __global__ void k_tensor(TensorArgs a, double2 *__restrict__ T64, float2 *__rest
     T64r[pair * 8 + c] = acc[c].x;
     T32r[pair * 8 + c] = (float)acc[c].x;
   }
+}
+
+__global__ void k_tensor(TensorArgs a, double2 *__restrict__ T64, float2 *__restrict__ T32, double *__restrict__ T64r,
+                         float *__restrict__ T32r)
+{
+  const int pair = blockIdx.x * blockDim.x + threadIdx.x;
+  if(pair >= a.N * a.N) return;
+  tensor_pair(pair, a, T64, T32, T64r, T32r);
+}
+
+// the tensors of several groups in one launch: item blockIdx.y
+__global__ void k_tensor_many(const TensorItem *__restrict__ items)
+{
+  const TensorItem &it = items[blockIdx.y];
+  const int pair = blockIdx.x * blockDim.x + threadIdx.x;
+  if(pair >= it.a.N * it.a.N) return;
+  tensor_pair(pair, it.a, it.T64, it.T32, it.T64r, it.T32r);
 }
 
 // ------------------------------------------------------------------ k_geometry
@@ -1104,9 +1108,8 @@ __device__ __forceinline__ void gather_runs(int t, const int *h, const bool *in,
 
 // S slots per thread, kChunk / S threads per block
 template <bool CPLX, bool MIXED, bool WIDE, int S>
-__global__ void __launch_bounds__(kChunk / S, S == 8 ? 5 : 8)
-  k_gather(const GroupTable *__restrict__ gtp, const uint32_t *__restrict__ recs, int64_t nchunks, int tensor,
-           void *__restrict__ out)
+__device__ __forceinline__ void gather_chunk(const GroupTable *__restrict__ gtp, const uint32_t *__restrict__ recs,
+                                             int64_t c, int tensor, void *__restrict__ out)
 {
   using R = typename std::conditional<MIXED, float, double>::type;
   constexpr int B = kChunk / S;
@@ -1127,8 +1130,6 @@ __global__ void __launch_bounds__(kChunk / S, S == 8 ? 5 : 8)
   __shared__ int s_wf[B / 64];
   const GroupTable &gt = *gtp;
   const int t = threadIdx.x;
-  const int64_t c = xcd_remap(blockIdx.x, gridDim.x);
-  if(c >= nchunks) return;  // uniform per block
   // the whole record in one round of loads: words of slots S t .. S t + S - 1
   const uint32_t *rec = recs + c * kStride;
   const uint32_t ns = rec[0], clen = rec[1], nel = rec[2];
@@ -1218,6 +1219,42 @@ __global__ void __launch_bounds__(kChunk / S, S == 8 ? 5 : 8)
   gather_runs<CPLX, R, S, B>(t, h, in, vr, vi, ns, s_wsum, s_wx, s_wy, s_wf, out);
 }
 
+template <bool CPLX, bool MIXED, bool WIDE, int S>
+__global__ void __launch_bounds__(kChunk / S, S == 8 ? 5 : 8)
+  k_gather(const GroupTable *__restrict__ gtp, const uint32_t *__restrict__ recs, int64_t nchunks, int tensor,
+           void *__restrict__ out)
+{
+  const int64_t c = xcd_remap(blockIdx.x, gridDim.x);
+  if(c >= nchunks) return;  // uniform per block
+  gather_chunk<CPLX, MIXED, WIDE, S>(gtp, recs, c, tensor, out);
+}
+
+// item of a block range of a launch over several systems: the global block
+// index of its first block, and what the standalone launch would take
+__device__ __forceinline__ int find_item(const int64_t *__restrict__ first, int nitems, int64_t b)
+{
+  int lo = 0, hi = nitems - 1;
+  while(lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if(first[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// the chunks of several systems (tensor form) in one launch: items
+// [first chunk, table, records, output], grid over all their chunks
+template <bool CPLX, bool MIXED, bool WIDE, int S>
+__global__ void __launch_bounds__(kChunk / S, S == 8 ? 5 : 8)
+  k_gather_many(const GatherItem *__restrict__ items, const int64_t *__restrict__ first, int nitems, int64_t total)
+{
+  const int64_t cg = xcd_remap(blockIdx.x, gridDim.x);
+  if(cg >= total) return;  // uniform per block
+  const int i = find_item(first, nitems, cg);
+  const GatherItem it = items[i];
+  gather_chunk<CPLX, MIXED, WIDE, S>(it.table, it.recs, cg - first[i], 1, it.out);
+}
+
 // ------------------------------------------------------------------ k_rhs
 // L lanes per row (2, 4 or 8, autotuned): Dirichlet lifting
 // A_e(i_dof, j_tf) u_D for fixed columns (transposed local index,
@@ -1238,14 +1275,15 @@ __device__ __forceinline__ double2 lane_sum(double2 v)
 }
 
 template <bool CPLX, bool MIXED, int L>
-__global__ void __launch_bounds__(256) k_rhs(const GroupTable *__restrict__ gtp, int tensor, int64_t n, int nrhs, const int64_t *__restrict__ dc_ptr,
-                                             const uint32_t *__restrict__ dc_words, const int32_t *__restrict__ dc_fix,
-                                             const double2 *__restrict__ uD, const int64_t *__restrict__ lin_ptr,
-                                             const uint32_t *__restrict__ lin_words, const double2 *__restrict__ be_all,
-                                             int64_t be_total, int be_real, void *__restrict__ out)
+__device__ __forceinline__ void rhs_rows(int64_t blk, const GroupTable *__restrict__ gtp, int tensor, int64_t n, int nrhs,
+                                         const int64_t *__restrict__ dc_ptr, const uint32_t *__restrict__ dc_words,
+                                         const int32_t *__restrict__ dc_fix, const double2 *__restrict__ uD,
+                                         const int64_t *__restrict__ lin_ptr, const uint32_t *__restrict__ lin_words,
+                                         const double2 *__restrict__ be_all, int64_t be_total, int be_real,
+                                         void *__restrict__ out)
 {
   const GroupTable &gt = *gtp;
-  const int64_t row = (xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x) / L;
+  const int64_t row = (blk * blockDim.x + threadIdx.x) / L;
   const int l = (int)(threadIdx.x % L);
   if(row >= n) return;  // whole lane groups leave together
   double2 lift = make_double2(0., 0.);
@@ -1312,6 +1350,30 @@ __global__ void __launch_bounds__(256) k_rhs(const GroupTable *__restrict__ gtp,
         reinterpret_cast<double *>(out)[pos] = acc.x;
     }
   }
+}
+
+template <bool CPLX, bool MIXED, int L>
+__global__ void __launch_bounds__(256) k_rhs(const GroupTable *__restrict__ gtp, int tensor, int64_t n, int nrhs, const int64_t *__restrict__ dc_ptr,
+                                             const uint32_t *__restrict__ dc_words, const int32_t *__restrict__ dc_fix,
+                                             const double2 *__restrict__ uD, const int64_t *__restrict__ lin_ptr,
+                                             const uint32_t *__restrict__ lin_words, const double2 *__restrict__ be_all,
+                                             int64_t be_total, int be_real, void *__restrict__ out)
+{
+  rhs_rows<CPLX, MIXED, L>(xcd_remap(blockIdx.x, gridDim.x), gtp, tensor, n, nrhs, dc_ptr, dc_words, dc_fix, uD,
+                           lin_ptr, lin_words, be_all, be_total, be_real, out);
+}
+
+// the right-hand sides of several systems (tensor form) in one launch
+template <bool CPLX, bool MIXED, int L>
+__global__ void __launch_bounds__(256) k_rhs_many(const RhsItem *__restrict__ items, const int64_t *__restrict__ first,
+                                                  int nitems, int64_t total)
+{
+  const int64_t bg = xcd_remap(blockIdx.x, gridDim.x);
+  if(bg >= total) return;
+  const int i = find_item(first, nitems, bg);
+  const RhsItem &a = items[i];
+  rhs_rows<CPLX, MIXED, L>(bg - first[i], a.table, 1, a.n, a.nrhs, a.dc_ptr, a.dc_words, a.dc_fix, a.uD, a.lin_ptr,
+                           a.lin_words, a.be_all, a.be_total, a.be_real, a.out);
 }
 
 // ------------------------------------------------------------------ reference strategy
@@ -1404,6 +1466,21 @@ hipError_t launch_basis(const Group &g, Rule &r, hipStream_t s)
 
 hipError_t launch_tensor(const Group &g, hipStream_t s)
 {
+  const TensorArgs a = tensor_args(g);
+  hipLaunchKernelGGL(k_tensor, dim3(blocks_for(g.N * g.N, 64)), dim3(64), 0, s, a, g.d_T64, g.d_T32, g.d_T64r,
+                     g.d_T32r);
+  return hipGetLastError();
+}
+
+hipError_t launch_tensor_many(const TensorItem *items, int nitems, int maxN, hipStream_t s)
+{
+  if(nitems == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_tensor_many, dim3(blocks_for(maxN * maxN, 64), (unsigned)nitems), dim3(64), 0, s, items);
+  return hipGetLastError();
+}
+
+TensorArgs tensor_args(const Group &g)
+{
   TensorArgs a{};
   a.N = g.N;
   a.nterms = 0;
@@ -1425,9 +1502,7 @@ hipError_t launch_tensor(const Group &g, hipStream_t s)
     a.bf[r] = g.rules[r].d_bf;
     a.gbf[r] = g.rules[r].d_gbf;
   }
-  hipLaunchKernelGGL(k_tensor, dim3(blocks_for(g.N * g.N, 64)), dim3(64), 0, s, a, g.d_T64, g.d_T32, g.d_T64r,
-                     g.d_T32r);
-  return hipGetLastError();
+  return a;
 }
 
 hipError_t launch_geometry_all(const gfem_system &sys, const double *xyz, bool fp32, hipStream_t s)
@@ -1504,11 +1579,7 @@ hipError_t launch_linear(const Group &g, const double *xyz, int nrhs, int eb_shi
       if(jf) {
         // compiled coefficient program (jit.cpp), one thread per element in
         // blocks of 64 << eb_shift threads; argument layout of JitLinArgs
-        struct {
-          double consts[2 * GFEM_MAX_CONSTS];
-          double fac_re, fac_im;
-          int Q, dim, nv, N, accumulate, EB, real_out;
-        } ja{};
+        JitLinArgs ja{};
         for(int k = 0; k < 2 * GFEM_MAX_CONSTS; ++k) ja.consts[k] = t.expr.consts[k];
         ja.fac_re = t.fac_re;
         ja.fac_im = t.fac_im;
@@ -1701,6 +1772,89 @@ hipError_t launch_rhs(const gfem_system &sys, int prec, int tensor, void *out, h
   case 8: rhs_lanes<8>(sys, prec, tensor, out, s); break;
   default: return hipErrorInvalidValue;
   }
+  return hipGetLastError();
+}
+
+GatherItem gather_item(const gfem_system &sys, void *out) { return GatherItem{sys.d_table, sys.d_rec, out}; }
+
+int64_t gather_item_blocks(const gfem_system &sys) { return sys.n_chunks; }
+
+hipError_t launch_gather_many(bool cplx, bool mixed, bool wide, int slots, const GatherItem *items,
+                              const int64_t *first, int nitems, int64_t total, hipStream_t s)
+{
+  if(total == 0) return hipSuccess;
+  const dim3 grid((unsigned)xcd_grid(total));
+#define GFEM_GM(C, M, W, S)                                                                                        \
+  hipLaunchKernelGGL((k_gather_many<C, M, W, S>), grid, dim3(kChunk / S), 0, s, items, first, nitems, total)
+#define GFEM_GM_S(C, M, W)                                                                                         \
+  if(slots == 8)                                                                                                   \
+    GFEM_GM(C, M, W, 8);                                                                                           \
+  else if(slots == 4)                                                                                              \
+    GFEM_GM(C, M, W, 4);                                                                                           \
+  else                                                                                                             \
+    GFEM_GM(C, M, W, 2);
+#define GFEM_GM_W(C, M)                                                                                            \
+  if(wide) {                                                                                                       \
+    GFEM_GM_S(C, M, true)                                                                                          \
+  }                                                                                                                \
+  else {                                                                                                           \
+    GFEM_GM_S(C, M, false)                                                                                         \
+  }
+  if(cplx) {
+    if(mixed) {
+      GFEM_GM_W(true, true)
+    }
+    else {
+      GFEM_GM_W(true, false)
+    }
+  }
+  else {
+    if(mixed) {
+      GFEM_GM_W(false, true)
+    }
+    else {
+      GFEM_GM_W(false, false)
+    }
+  }
+#undef GFEM_GM_W
+#undef GFEM_GM_S
+#undef GFEM_GM
+  return hipGetLastError();
+}
+
+RhsItem rhs_item(const gfem_system &sys, void *out)
+{
+  return RhsItem{sys.d_table, sys.n, sys.nrhs, sys.be_real ? 1 : 0, sys.d_dc_ptr, sys.d_dc_words, sys.d_dc_fix, sys.d_dc,
+                 sys.d_lin_ptr, sys.d_lin_words, sys.d_be_all, sys.be_total, out};
+}
+
+int64_t rhs_item_blocks(const gfem_system &sys, int prec) { return blocks_for(sys.n * sys.rhs_lanes[prec ? 1 : 0], 256); }
+
+hipError_t launch_rhs_many(bool cplx, bool mixed, int lanes, const RhsItem *items, const int64_t *first, int nitems,
+                           int64_t total, hipStream_t s)
+{
+  if(total == 0) return hipSuccess;
+  const dim3 grid((unsigned)xcd_grid(total)), block(256);
+#define GFEM_RM(L)                                                                                                 \
+  if(cplx) {                                                                                                       \
+    if(mixed)                                                                                                      \
+      hipLaunchKernelGGL((k_rhs_many<true, true, L>), grid, block, 0, s, items, first, nitems, total);             \
+    else                                                                                                           \
+      hipLaunchKernelGGL((k_rhs_many<true, false, L>), grid, block, 0, s, items, first, nitems, total);            \
+  }                                                                                                                \
+  else {                                                                                                           \
+    if(mixed)                                                                                                      \
+      hipLaunchKernelGGL((k_rhs_many<false, true, L>), grid, block, 0, s, items, first, nitems, total);            \
+    else                                                                                                           \
+      hipLaunchKernelGGL((k_rhs_many<false, false, L>), grid, block, 0, s, items, first, nitems, total);           \
+  }
+  switch(lanes) {
+  case 2: GFEM_RM(2) break;
+  case 4: GFEM_RM(4) break;
+  case 8: GFEM_RM(8) break;
+  default: return hipErrorInvalidValue;
+  }
+#undef GFEM_RM
   return hipGetLastError();
 }
 

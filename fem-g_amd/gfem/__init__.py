@@ -16,8 +16,8 @@ except ImportError as e:  # pragma: no cover - exercised only on a broken build
         "gfem native extension not built: run `python fem-g_amd/build.py` "
         f"(or __graft_entry__.build()) first ({e})") from e
 
-from ._gfem_host import (Domain, Field, Model, ScalarFunction, Server, Unknown, abs, cos,  # noqa: F401,E402
-                         dof, dt2_dof, dt_dof, exp, from_arrays, gauss_rule, grad,
+from ._gfem_host import (Domain, Field, ManyInstance, Model, ScalarFunction, Server, Unknown, abs,  # noqa: F401,E402
+                         assemble_many, cos, dof, dt2_dof, dt_dof, exp, from_arrays, gauss_rule, grad,
                          hilbert_index, hilbert_key, lattice_nodes, pow, read_msh, set_current_model, set_device,
                          sin, sqrt, tf, unit_cube, unit_square, write_msh, x, y, z)
 

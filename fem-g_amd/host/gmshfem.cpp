@@ -437,6 +437,42 @@ void Formulation<T>::setTermCoefficient(unsigned int tag, const function::Scalar
 }
 
 template <class T>
+void Formulation<T>::instanceParams(const std::vector<std::pair<unsigned int, function::ScalarFunction>> &params,
+                                   std::vector<gfem_term_param> &out, std::vector<std::vector<double>> &store)
+{
+  if(!_sys) throw common::Exception("instanceParams: the system exists after the first assemble()");
+  for(const auto &tf : params) {
+    const unsigned int tag = tf.first;
+    const function::ScalarFunction &f = tf.second;
+    if(tag >= _terms.size()) throw common::Exception("instanceParams: unknown term " + std::to_string(tag));
+    Term &t = _terms[tag];
+    function::ScalarFunction &c = t.bilinear ? t.lhs.coef : t.source;
+    if(c.isConstant() != f.isConstant() || (!f.isConstant() && f.ops() != c.ops()))
+      throw common::Exception("instanceParams: the coefficient must keep its form (constant, or the same program)");
+    c = f;
+    const double *consts = nullptr;
+    if(!f.isConstant()) {
+      store.push_back(f.consts());
+      consts = store.back().data();
+    }
+    for(const auto &gk : _termSlots[tag]) {
+      gfem_term_param q{};
+      q.group = gk.first;
+      q.term = gk.second;
+      if(f.isConstant()) {
+        q.coef_re = f.value().real();
+        q.coef_im = f.value().imag();
+      }
+      else {
+        q.n_consts = (int)f.consts().size() / 2;
+        q.consts = consts;
+      }
+      out.push_back(q);
+    }
+  }
+}
+
+template <class T>
 std::pair<int64_t, int> Formulation<T>::nodeDof(int64_t tag) const
 {
   if(tag < 1 || tag >= (int64_t)_dofNum.size()) return {0, 0};

@@ -202,6 +202,13 @@ class Formulation {
   // device from the next assembly.  (The reference binds coefficients when a
   // term is created; this serves repeated solves on one mesh.)
   void setTermCoefficient(unsigned int tag, const function::ScalarFunction &f);
+  // The parameters of one instance of gfem_assemble_many: the (group, term)
+  // slots of each term tag with its new coefficient, after the checks of
+  // setTermCoefficient.  The formulation's own coefficients take the new
+  // values too (the system keeps its last instance's parameters); expression
+  // constants are held in `store`, which must outlive the call.
+  void instanceParams(const std::vector<std::pair<unsigned int, function::ScalarFunction>> &params,
+                      std::vector<gfem_term_param> &out, std::vector<std::vector<double>> &store);
 
   void setSymmetric(bool s) { _symmetric = s; }
   void setAngularFrequency(double w) { _frequency = w; }

@@ -8,6 +8,9 @@
 #include <pybind11/stl.h>
 
 #include <cstring>
+#include <memory>
+
+#include <hip/hip_runtime_api.h>
 
 #include "../csrc/refelem.h"
 #include "gmshfem.hpp"
@@ -57,6 +60,57 @@ void check(gfem_status s, const char *what)
                              gfem_ctx_last_error(common::deviceContext()));
 }
 
+// One instance of gfem_assemble_many: its system, parameters and device
+// outputs (allocated at the first assembly in a precision).
+struct ManyInstance {
+  gfem_system *sys = nullptr;
+  int64_t nnz = 0, n = 0;
+  int nrhs = 1;
+  std::vector<gfem_term_param> params;
+  std::vector<std::vector<double>> store;  // expression constants the params point to
+  void *vals[2] = {nullptr, nullptr}, *rhs[2] = {nullptr, nullptr};
+  int vb[2] = {0, 0};
+  ManyInstance() = default;
+  ManyInstance(const ManyInstance &) = delete;
+  ~ManyInstance()
+  {
+    for(int p = 0; p < 2; ++p) {
+      if(vals[p]) (void)hipFree(vals[p]);
+      if(rhs[p]) (void)hipFree(rhs[p]);
+    }
+  }
+  void ensure(int p)
+  {
+    if(vals[p]) return;
+    int64_t nz = 0, nn = 0;
+    check(gfem_system_info(sys, (gfem_precision)p, &nz, &nn, &vb[p]), "gfem_system_info");
+    if(hipMalloc(&vals[p], (size_t)std::max<int64_t>(nz, 1) * vb[p]) != hipSuccess ||
+       hipMalloc(&rhs[p], (size_t)std::max<int64_t>(nn * nrhs, 1) * vb[p]) != hipSuccess)
+      throw std::runtime_error("assemble_many: output allocation failed");
+  }
+  // device values of precision p as complex128
+  py::array_t<std::complex<double>> fetch(int p, bool rhs_side, int idx)
+  {
+    if(!vals[p]) throw std::runtime_error("instance not assembled in this precision");
+    const int64_t cnt = rhs_side ? n : nnz;
+    std::vector<unsigned char> h((size_t)cnt * vb[p]);
+    const unsigned char *src = static_cast<const unsigned char *>(rhs_side ? rhs[p] : vals[p]);
+    if(rhs_side) src += (size_t)idx * n * vb[p];
+    if(cnt && hipMemcpy(h.data(), src, h.size(), hipMemcpyDeviceToHost) != hipSuccess)
+      throw std::runtime_error("assemble_many: download failed");
+    py::array_t<std::complex<double>> a((py::ssize_t)cnt);
+    auto *o = a.mutable_data();
+    for(int64_t i = 0; i < cnt; ++i) {
+      const unsigned char *e = h.data() + (size_t)i * vb[p];
+      if(vb[p] == 16) { double v[2]; std::memcpy(v, e, 16); o[i] = {v[0], v[1]}; }
+      else if(vb[p] == 8 && p == 0) { double v; std::memcpy(&v, e, 8); o[i] = {v, 0.}; }
+      else if(vb[p] == 8) { float v[2]; std::memcpy(v, e, 8); o[i] = {v[0], v[1]}; }
+      else { float v; std::memcpy(&v, e, 4); o[i] = {v, 0.f}; }
+    }
+    return a;
+  }
+};
+
 template <class T>
 void bind_formulation(py::module_ &m, const char *name)
 {
@@ -77,6 +131,18 @@ void bind_formulation(py::module_ &m, const char *name)
       py::arg("source"), py::arg("rhs"), py::arg("domain"), py::arg("integration"), py::arg("hermitian") = true,
       py::arg("rhs_idx") = 0)
     .def("set_term_coefficient", &F::setTermCoefficient, py::arg("tag"), py::arg("coefficient"))
+    .def(
+      "many_instance",
+      [](F &f, const std::vector<std::pair<unsigned int, ScalarFunction>> &params) {
+        auto inst = std::make_shared<ManyInstance>();
+        f.instanceParams(params, inst->params, inst->store);
+        inst->sys = f.system();
+        int vb = 0;
+        check(gfem_system_info(inst->sys, GFEM_PREC_FP64, &inst->nnz, &inst->n, &vb), "gfem_system_info");
+        inst->nrhs = (int)f.numRHS();
+        return inst;
+      },
+      py::arg("params") = std::vector<std::pair<unsigned int, ScalarFunction>>{})
     .def("set_symmetric", &F::setSymmetric)
     .def("set_angular_frequency", &F::setAngularFrequency)
     .def("num_rhs", &F::numRHS)
@@ -245,6 +311,28 @@ PYBIND11_MODULE(_gfem_host, m)
 {
   m.doc() = "gfem host API: GmshFEM Formulation over the MI355X assembly engine";
   m.def("set_device", &common::setDevice);
+  py::class_<ManyInstance, std::shared_ptr<ManyInstance>>(m, "ManyInstance")
+    .def("values", [](ManyInstance &i, const std::string &p) { return i.fetch((int)parse_precision(p), false, 0); },
+         py::arg("precision") = "fp64")
+    .def("rhs", [](ManyInstance &i, const std::string &p, int idx) { return i.fetch((int)parse_precision(p), true, idx); },
+         py::arg("precision") = "fp64", py::arg("idx") = 0);
+  m.def(
+    "assemble_many",
+    [](const std::vector<std::shared_ptr<ManyInstance>> &inst, const std::string &prec, bool sync) {
+      const gfem_precision p = parse_precision(prec);
+      std::vector<gfem_instance> v(inst.size());
+      for(size_t k = 0; k < inst.size(); ++k) {
+        inst[k]->ensure((int)p);
+        v[k].sys = inst[k]->sys;
+        v[k].n_params = (int)inst[k]->params.size();
+        v[k].params = inst[k]->params.data();
+        v[k].vals = inst[k]->vals[(int)p];
+        v[k].rhs = inst[k]->rhs[(int)p];
+      }
+      check(gfem_assemble_many((int)v.size(), v.data(), p, nullptr), "gfem_assemble_many");
+      if(sync) check(gfem_stream_sync(common::deviceContext(), nullptr), "gfem_stream_sync");
+    },
+    py::arg("instances"), py::arg("precision") = "fp64", py::arg("sync") = true);
 
   py::class_<model::Model, std::shared_ptr<model::Model>>(m, "Model")
     .def_readonly("name", &model::Model::name)

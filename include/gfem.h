@@ -310,6 +310,32 @@ gfem_status gfem_assemble_into_noalloc(gfem_system *sys, gfem_precision prec, vo
 gfem_status gfem_term_update(gfem_system *sys, int group, int term, double coef_re, double coef_im, int n_consts,
                              const double *consts, void *stream);
 
+/* Several parameter sets of resident systems assembled together (fused
+ * strategy): instance i is what gfem_term_update of its parameters, in list
+ * order, followed by gfem_assemble_into(sys, prec, vals, rhs) would give,
+ * bit for bit, and each system keeps the parameters of its last instance.
+ * The k-th instances of all systems are assembled in one launch per kernel
+ * kind (tensor rebuilds, compiled element loads, RHS rows, matrix gather),
+ * so many small instances cost a few launches instead of a few each; element
+ * factors are computed once per mesh.  GFEM_ERR_UNSUPPORTED for systems with
+ * per-point element matrices (expression coefficients of bilinear terms,
+ * curved elements).  No counterpart in the reference, which assembles one
+ * formulation at a time (fem-master/src/problem/Formulation.cpp:1151-1215);
+ * callers are batched and serving workloads over resident meshes. */
+typedef struct {
+  int group, term;             /* as gfem_term_update */
+  double coef_re, coef_im;
+  int n_consts;
+  const double *consts;
+} gfem_term_param;
+typedef struct {
+  gfem_system *sys;
+  int n_params;                  /* this instance's coefficients (others keep the system's) */
+  const gfem_term_param *params;
+  void *vals, *rhs;              /* device outputs, as gfem_assemble_into */
+} gfem_instance;
+gfem_status gfem_assemble_many(int n, const gfem_instance *inst, gfem_precision prec, void *stream);
+
 /* Wait for the work queued on `stream`. */
 gfem_status gfem_stream_sync(gfem_ctx *ctx, void *stream);
 
