@@ -250,6 +250,29 @@ def poisson2d_leg(args):
             "kernel_ms": {k: round(v, 5) for k, v in prof.items()}}
 
 
+def solver_leg(args):
+    """One Helmholtz request on the largest serving mesh (39^3 x 6 P2 tets,
+    493,039 unknowns, k = 10 pi) with the single-request solver: GMRES(30)
+    in FP32 inside FP64 iterative refinement, to 1e-10.  SpMV bytes per
+    product: nnz (complex64 value + 32-bit column) + row pointers (64-bit) +
+    x read and y written (complex64)."""
+    from gfem import workloads as W
+    F = W.build(W.helmholtz3d(n=39, order=2, k=10 * 3.141592653589793))
+    F.pre()
+    F.assemble(download=False)
+    F.solve(tol=1e-10, restart=30, max_iter=30000, download=False)  # workspace, graphs, SpMV lanes
+    t0 = time.perf_counter()
+    info = F.solve(tol=1e-10, restart=30, max_iter=30000, download=False)
+    dt = time.perf_counter() - t0
+    n, nnz = int(F.n_unknown), int(F.nnz)
+    res = {"mesh": "helmholtz3d 39^3x6 P2, k=10pi", "n_unknown": n, "nnz": nnz, "iterations": info["iterations"],
+           "cycles": info["cycles"], "seconds": dt, "ms_per_iteration": dt / max(info["iterations"], 1) * 1e3,
+           "rel_residual": info["rel_residual"], "spmv_bytes": nnz * 12 + (n + 1) * 8 + 2 * n * 8}
+    log(f"solver: {n} unknowns, {info['iterations']} iterations in {dt:.2f} s "
+        f"({res['ms_per_iteration']:.3f} ms/iteration), rel residual {info['rel_residual']:.1e}")
+    return res
+
+
 def batched(args):
     """Uneven Helmholtz instances solved in one batch: tet meshes log-uniform
     in 2k-200k elements (n^3 x 6 cubes), order from {1, 2, 3}, k ~ U[2pi,
@@ -412,6 +435,7 @@ def main():
     ap.add_argument("--no-highorder", action="store_true", help="skip the order-4 matrix-core comparison")
     ap.add_argument("--ho-n", type=int, default=32)
     ap.add_argument("--no-batched", action="store_true", help="skip the uneven-instance batch")
+    ap.add_argument("--no-solver", action="store_true", help="skip the single-request solve")
     ap.add_argument("--batch-instances", type=int, default=256)
     ap.add_argument("--batch-tol", type=float, default=1e-8)
     ap.add_argument("--batch-hbm-gb", type=int, default=96)
@@ -540,6 +564,10 @@ def main():
     if not args.no_highorder:
         ho = highorder(args)
 
+    sol = None
+    if not args.no_solver:
+        sol = solver_leg(args)
+
     bat = None
     if not args.no_batched:
         bat = batched(args)
@@ -589,6 +617,8 @@ def main():
         line["highorder"] = ho
     if bat is not None:
         line["batched"] = bat
+    if sol is not None:
+        line["solver"] = sol
     print(json.dumps(line), flush=True)
 
 

@@ -27,7 +27,7 @@ namespace {
 constexpr int kRowLanes = 8;       // lanes per CSR row in the SpMV kernels
 constexpr int kRedBlocks = 512;    // partial sums per reduction
 constexpr int kRedThreads = 256;
-constexpr int kDotGroup = 4;       // basis vectors per pass of the dot kernel
+constexpr int kDotGroup = 8;       // basis vectors per pass of the dot kernel
 
 template <bool C>
 struct Sc;
@@ -150,13 +150,11 @@ __global__ void __launch_bounds__(256) k_residual64(int64_t n, const int64_t *__
   }
 }
 
-// block partials of conj(V_k) . w for k in [k0, k0 + kg), FP64 accumulation;
-// V rows are `ld` apart.  part[(k) * kRedBlocks + block]
 template <typename T>
-__global__ void __launch_bounds__(kRedThreads) k_dots(int64_t n, int k0, int kg, const T *__restrict__ V, int64_t ld,
-                                                      const T *__restrict__ w, double2 *__restrict__ part)
+__device__ __forceinline__ void dots_group(int64_t n, int k0, int kg, const T *__restrict__ V, int64_t ld,
+                                           const T *__restrict__ w, double2 *__restrict__ part,
+                                           double2 (*s_red)[kDotGroup])
 {
-  __shared__ double2 s_red[kRedThreads / 64][kDotGroup];
   double2 acc[kDotGroup];
 #pragma unroll
   for(int q = 0; q < kDotGroup; ++q) acc[q] = make_double2(0., 0.);
@@ -188,6 +186,21 @@ __global__ void __launch_bounds__(kRedThreads) k_dots(int64_t n, int k0, int kg,
       s.y += s_red[k][threadIdx.x].y;
     }
     part[(int64_t)(k0 + threadIdx.x) * kRedBlocks + blockIdx.x] = s;
+  }
+  __syncthreads();  // s_red is reused by the next group
+}
+
+// block partials of conj(V_k) . w for k in [0, nk), FP64 accumulation, the
+// vectors kDotGroup at a time (w read once per group) inside one launch; V
+// rows are `ld` apart.  part[(k) * kRedBlocks + block]
+template <typename T>
+__global__ void __launch_bounds__(kRedThreads) k_dots(int64_t n, int nk, const T *__restrict__ V, int64_t ld,
+                                                      const T *__restrict__ w, double2 *__restrict__ part)
+{
+  __shared__ double2 s_red[kRedThreads / 64][kDotGroup];
+  for(int k0 = 0; k0 < nk; k0 += kDotGroup) {
+    const int kg = min(kDotGroup, nk - k0);
+    dots_group<T>(n, k0, kg, V, ld, w, part, s_red);
   }
 }
 
@@ -422,11 +435,7 @@ struct Gmres {
   template <typename T>
   hipError_t dots(const T *V, int64_t ld, int j1, const T *w, double2 *out, int accumulate)
   {
-    for(int k0 = 0; k0 < j1; k0 += kDotGroup) {
-      const int kg = std::min(kDotGroup, j1 - k0);
-      hipLaunchKernelGGL(k_dots<T>, dim3(kRedBlocks), dim3(kRedThreads), 0, s, n, k0, kg, V, ld, w,
-                         sys->d_sol_part);
-    }
+    hipLaunchKernelGGL(k_dots<T>, dim3(kRedBlocks), dim3(kRedThreads), 0, s, n, j1, V, ld, w, sys->d_sol_part);
     hipLaunchKernelGGL(k_dots_final, dim3((unsigned)j1), dim3(kRedThreads), 0, s, sys->d_sol_part, out, accumulate);
     return hipGetLastError();
   }
