@@ -35,8 +35,8 @@ namespace {
 constexpr int kLanes = 8;                       // lanes per CSR row
 constexpr int kRowsPerBlock = 256 / kLanes;      // 32 rows per 256-thread block
 constexpr int kElemsPerBlock = 256;              // vector kernels: one element per thread
-constexpr int kDotSpan = 4096;                   // rows per reduction block (at least)
-constexpr int kPartMax = 256;                    // reduction blocks per request (at most)
+constexpr int kDotSpan = 1024;                   // rows per reduction block (at least)
+constexpr int kPartMax = 1024;                   // reduction blocks per request (at most)
 constexpr int kCheck = 5;                        // iterations between convergence checks
 
 // device view of one request slot
