@@ -861,6 +861,11 @@ gfem_status gfem_system_finalize(gfem_system *sys)
   sys->dense_gemm = !(dg && dg[0] == '0');
   const char *gb = std::getenv("GFEM_GEMM_BF16X3");
   if(gb && gb[0] == '1') sys->gemm_mode = GFEM_GEMM_BF16X3;
+  if(const char *e = std::getenv("GFEM_GATHER_WAVE")) {
+    // 1: the one-wave gather wherever 4 slots are used, 0: never
+    sys->gather_wave_env = e[0] == '1' ? 1 : 0;
+    sys->gather_wave[0] = sys->gather_wave[1] = sys->gather_wave_env == 1;
+  }
   if(const char *e = std::getenv("GFEM_GATHER_SLOTS")) {
     // a fixed choice, not tuned
     const int v = std::atoi(e);
@@ -1098,15 +1103,27 @@ static gfem_status autotune(gfem_system *sys, int p, int tensor, void *vals, voi
     // the rounding, is stable from run to run
     float t[3] = {0.f, 0.f, 0.f};
     const int cand[3] = {4, 2, 8};
+    sys->gather_wave[p] = false;
     for(int c = 0; c < 3 && st == GFEM_OK; ++c) {
       sys->gather_slots[p] = cand[c];
       const hipError_t e = timed([&]() { return launch_gather(*sys, p, tensor, vals, s); }, t[c]);
       if(e != hipSuccess) st = hip_fail(ctx, e, "autotune gather");
     }
+    // the one-wave kernel gives the 4-slot values bit for bit: it is taken
+    // whenever it is faster
+    float tw = 1e30f;
+    if(st == GFEM_OK && tensor && sys->gather_wave_env != 0) {
+      sys->gather_slots[p] = 4;
+      sys->gather_wave[p] = true;
+      const hipError_t e = timed([&]() { return launch_gather(*sys, p, tensor, vals, s); }, tw);
+      if(e != hipSuccess) st = hip_fail(ctx, e, "autotune gather (one wave per chunk)");
+    }
+    const float t4 = std::min(t[0], tw);
     int pick = 0;
     for(int c = 1; c < 3; ++c)
-      if(st == GFEM_OK && t[c] < 0.95f * t[0] && t[c] < t[pick]) pick = c;
+      if(st == GFEM_OK && t[c] < 0.95f * t4 && t[c] < (pick ? t[pick] : t4)) pick = c;
     sys->gather_slots[p] = cand[pick];
+    sys->gather_wave[p] = pick == 0 && (sys->gather_wave_env == 1 || (sys->gather_wave_env < 0 && tw < t[0]));
   }
   if(st == GFEM_OK && sys->n > 0) {
     // lanes per RHS row: also an order of sums, so the default (2) stays
@@ -1748,6 +1765,14 @@ gfem_status gfem_tuning(const gfem_system *sys, gfem_precision prec, int *gather
   if(linear_eb_shift) *linear_eb_shift = sys->linear_eb_shift[p];
   if(spmv_lanes) *spmv_lanes = sys->spmv_lanes;
   if(rhs_lanes) *rhs_lanes = sys->rhs_lanes[p];
+  return GFEM_OK;
+}
+
+gfem_status gfem_gather_one_wave(const gfem_system *sys, gfem_precision prec, int *one_wave)
+{
+  if(!sys || !one_wave) return fail(sys ? sys->ctx : nullptr, GFEM_ERR_INVALID_ARG, "gather_one_wave");
+  const int p = prec == GFEM_PREC_MIXED ? 1 : 0;
+  *one_wave = sys->gather_wave[p] && sys->gather_slots[p] == 4 ? 1 : 0;
   return GFEM_OK;
 }
 

@@ -90,6 +90,18 @@ struct GroupTable {
   int64_t ne[kMaxGroups];
 };
 
+// Reference tensors the one-wave gather (k_gather_wave) stages in LDS, as
+// the launch sized them: per group the byte offset of its rows (-1: read from
+// memory), the LDS row stride, 16-byte pieces per row and whether the rows
+// are real; bytes = all staged rows.
+struct TensorLds {
+  int off[kMaxGroups];
+  int rs[kMaxGroups];
+  int q16[kMaxGroups];
+  int real[kMaxGroups];
+  int bytes;
+};
+
 // Constant-coefficient reference tensors of a group (k_tensor): per term its
 // forms, rule and full constant factor; per rule its points.
 struct TensorArgs {
@@ -267,6 +279,13 @@ struct gfem_system {
   // gfem_api.cpp autotune): gather slots per thread 4 (128-thread blocks) or
   // 2 (256); linear-term elements per block ~ (256 << linear_eb_shift) / Q
   int gather_slots[2] = {4, 4};
+  // 4 slots: the one-wave persistent kernel (k_gather_wave) instead of the
+  // 128-thread block; the same values, so a free choice by speed
+  bool gather_wave[2] = {false, false};
+  int gather_wave_env = -1;  // GFEM_GATHER_WAVE at finalize: 1 always, 0 never, -1 autotuned
+  // its launch, cached per precision for the LDS bytes it was sized for
+  mutable int gw_tbytes[2] = {-1, -1}, gw_nw[2] = {0, 0};
+  mutable int64_t gw_blocks[2] = {0, 0};
   int linear_eb_shift[2] = {2, 2};
   int rhs_lanes[2] = {2, 2};    // k_rhs lanes per row: 2, 4 or 8
   bool tuned[2] = {false, false};
@@ -418,6 +437,10 @@ hipError_t launch_zero(void *p, size_t bytes, hipStream_t s);
 // tensor = 0: the constant-coefficient tensors are left out (every bilinear
 // term is in the per-point element matrices)
 hipError_t launch_gather(const gfem_system &sys, int prec, int tensor, void *vals, hipStream_t s);
+// LDS layout of the reference tensors for k_gather_wave under precision
+// prec (0 FP64, 1 FP32).  Groups are staged by their share of the
+// contributions, largest first, within a per-precision byte budget.
+TensorLds gather_lds_layout(const gfem_system &sys, int prec);
 hipError_t launch_rhs(const gfem_system &sys, int prec, int tensor, void *rhs, hipStream_t s);
 hipError_t launch_ref_scatter(const gfem_system &sys, const Group &g, hipStream_t s);
 hipError_t launch_ref_rhs(const gfem_system &sys, const Group &g, hipStream_t s);

@@ -939,19 +939,74 @@ struct ValTraits<double2> {
 //      once.  No atomics; the order of additions is fixed, so results are
 //      bitwise reproducible.
 
+// F . T[pair] for one tensor row `row` (real rows: 8 components; complex
+// rows: 8 (re, im) pairs), added to (vr, vi).  The row may sit in memory or
+// in LDS: the arithmetic is this one function either way, so both give the
+// same bits.
+template <bool CPLX, bool MIXED>
+__device__ __forceinline__ void tensor_row(const typename std::conditional<MIXED, float, double>::type *F, bool treal,
+                                           const unsigned char *row,
+                                           typename std::conditional<MIXED, float, double>::type &vr,
+                                           typename std::conditional<MIXED, float, double>::type &vi)
+{
+  if(!CPLX || treal) {
+    if(MIXED) {
+      const float4 *T = reinterpret_cast<const float4 *>(row);
+      const float4 a = T[0], b = T[1];
+      vr += F[0] * a.x + F[1] * a.y + F[2] * a.z + F[3] * a.w + F[4] * b.x + F[5] * b.y + F[6] * b.z;
+    }
+    else {
+      const double2 *T = reinterpret_cast<const double2 *>(row);
+      const double2 a = T[0], b = T[1], c = T[2], d = T[3];
+      vr += F[0] * a.x + F[1] * a.y + F[2] * b.x + F[3] * b.y + F[4] * c.x + F[5] * c.y + F[6] * d.x;
+    }
+  }
+  else {
+    if(MIXED) {
+      const float4 *T = reinterpret_cast<const float4 *>(row);
+#pragma unroll
+      for(int q = 0; q < 4; ++q) {
+        const float4 tq = T[q];  // components 2q (x,y) and 2q+1 (z,w)
+        vr += F[2 * q] * tq.x + F[2 * q + 1] * tq.z;
+        vi += F[2 * q] * tq.y + F[2 * q + 1] * tq.w;
+      }
+    }
+    else {
+      const double2 *T = reinterpret_cast<const double2 *>(row);
+#pragma unroll
+      for(int q = 0; q < 7; ++q) {
+        const double2 tq = T[q];
+        vr += F[q] * tq.x;
+        vi += F[q] * tq.y;
+      }
+    }
+  }
+}
+
+// bytes of one tensor row in memory
+template <bool CPLX, bool MIXED>
+__host__ __device__ constexpr int tensor_row_bytes(bool treal)
+{
+  return 8 * (MIXED ? 4 : 8) * ((!CPLX || treal) ? 1 : 2);
+}
+
 // Value of one contribution word: pair (i, j), chunk element le.  The
 // element's factors F (7 per element, structure of arrays) sit in LDS with
 // its group and index; the group's reference tensor row [pair][8] is read
-// from memory (it stays in the caches).  Per-point element matrices (planes
-// [real, imaginary][N*N]) are added first.
-template <bool CPLX, bool MIXED>
+// from memory (it stays in the caches), or, with LDS_T and a staged group
+// (s_toff[g] >= 0), from the block's LDS copy at s_tl + s_toff[g] + pair *
+// s_trs[g].  Per-point element matrices (planes [real, imaginary][N*N]) are
+// added first.
+template <bool CPLX, bool MIXED, bool LDS_T = false>
 __device__ __forceinline__ void gather_eval(uint32_t w, int logP, uint32_t pmask, int tensor,
                                             const typename std::conditional<MIXED, float, double>::type (*s_F)[kChunkElems],
                                             const uint8_t *s_eg, const uint32_t *s_el, const uint8_t *s_gR,
                                             const int *s_gNN, const void *const *s_gT,
                                             const typename std::conditional<MIXED, float, double>::type *const *s_gD,
                                             typename std::conditional<MIXED, float, double>::type &vr,
-                                            typename std::conditional<MIXED, float, double>::type &vi)
+                                            typename std::conditional<MIXED, float, double>::type &vi,
+                                            const unsigned char *s_tl = nullptr, const int *s_toff = nullptr,
+                                            const int *s_trs = nullptr)
 {
   using R = typename std::conditional<MIXED, float, double>::type;
   const int pair = (int)(w & pmask);
@@ -970,38 +1025,25 @@ __device__ __forceinline__ void gather_eval(uint32_t w, int logP, uint32_t pmask
 #pragma unroll
   for(int q = 0; q < 7; ++q) F[q] = s_F[q][le];  // structure-of-arrays: no bank conflicts across elements
   F[7] = 0;
-  if(!CPLX || s_gR[g]) {
-    if(MIXED) {
-      const float4 *T = reinterpret_cast<const float4 *>(s_gT[g]) + (int64_t)pair * 2;
-      const float4 a = T[0], b = T[1];
-      vr += F[0] * a.x + F[1] * a.y + F[2] * a.z + F[3] * a.w + F[4] * b.x + F[5] * b.y + F[6] * b.z;
-    }
-    else {
-      const double2 *T = reinterpret_cast<const double2 *>(s_gT[g]) + (int64_t)pair * 4;
-      const double2 a = T[0], b = T[1], c = T[2], d = T[3];
-      vr += F[0] * a.x + F[1] * a.y + F[2] * b.x + F[3] * b.y + F[4] * c.x + F[5] * c.y + F[6] * d.x;
-    }
-  }
-  else {
-    if(MIXED) {
-      const float4 *T = reinterpret_cast<const float4 *>(s_gT[g]) + (int64_t)pair * 4;
-#pragma unroll
-      for(int q = 0; q < 4; ++q) {
-        const float4 tq = T[q];  // components 2q (x,y) and 2q+1 (z,w)
-        vr += F[2 * q] * tq.x + F[2 * q + 1] * tq.z;
-        vi += F[2 * q] * tq.y + F[2 * q + 1] * tq.w;
-      }
-    }
-    else {
-      const double2 *T = reinterpret_cast<const double2 *>(s_gT[g]) + (int64_t)pair * 8;
-#pragma unroll
-      for(int q = 0; q < 7; ++q) {
-        const double2 tq = T[q];
-        vr += F[q] * tq.x;
-        vi += F[q] * tq.y;
-      }
+  // real and complex rows at separate call sites (a run-time flag inside the
+  // helper keeps the slot arrays out of registers)
+  const bool treal = !CPLX || s_gR[g];
+  if constexpr(LDS_T) {
+    const int off = s_toff[g];
+    if(off >= 0) {
+      const unsigned char *row = s_tl + off + pair * s_trs[g];
+      if(treal)
+        tensor_row<CPLX, MIXED>(F, true, row, vr, vi);
+      else
+        tensor_row<CPLX, MIXED>(F, false, row, vr, vi);
+      return;
     }
   }
+  const unsigned char *T = reinterpret_cast<const unsigned char *>(s_gT[g]);
+  if(treal)
+    tensor_row<CPLX, MIXED>(F, true, T + (int64_t)pair * tensor_row_bytes<CPLX, MIXED>(true), vr, vi);
+  else
+    tensor_row<CPLX, MIXED>(F, false, T + (int64_t)pair * tensor_row_bytes<CPLX, MIXED>(false), vr, vi);
 }
 
 // Sums of the nonzeros' runs of slots and their output positions.  Thread t
@@ -1012,15 +1054,18 @@ __device__ __forceinline__ void gather_eval(uint32_t w, int logP, uint32_t pmask
 // exchange across waves), so long runs cost log steps instead of a serial
 // walk.  The order of additions is fixed: results are bitwise reproducible.
 // Output position of a run = first nonzero of the chunk + rank of its head.
-template <bool CPLX, typename R, int S, int B>
-__device__ __forceinline__ void gather_runs(int t, const int *h, const bool *in, const R *vr, const R *vi,
-                                            uint32_t ns, int *s_wsum, R *s_wx, R *s_wy, int *s_wf,
-                                            void *__restrict__ out)
+//
+// runs_scan: the wave's part.  (xr, xi) = the sum of the slots that continue
+// the run from the left, taken over this lane and the lanes to its right up
+// to the first head (f = a head was met); incl = inclusive prefix count of
+// the heads over the lanes.
+template <bool CPLX, typename R, int S>
+__device__ __forceinline__ void runs_scan(int lane, const int *h, const bool *in, const R *vr, const R *vi, R &xr,
+                                          R &xi, int &f, int &incl)
 {
-  constexpr int kW = B / 64;
-  // part of this thread's slots that continues the run from the left
-  R xr = 0, xi = 0;
-  int f = 0;
+  xr = 0;
+  xi = 0;
+  f = 0;
 #pragma unroll
   for(int k = 0; k < S; ++k) {
     if(h[k]) f = 1;
@@ -1029,7 +1074,6 @@ __device__ __forceinline__ void gather_runs(int t, const int *h, const bool *in,
       xi += vi[k];
     }
   }
-  const int lane = t & 63, wv = t >> 6;
 #pragma unroll
   for(int d = 1; d < 64; d <<= 1) {
     const R yr = __shfl_down(xr, d, 64);
@@ -1045,33 +1089,20 @@ __device__ __forceinline__ void gather_runs(int t, const int *h, const bool *in,
   int cnt = 0;
 #pragma unroll
   for(int k = 0; k < S; ++k) cnt += (in[k] && h[k]) ? 1 : 0;
-  int incl = cnt;
+  incl = cnt;
 #pragma unroll
   for(int d = 1; d < 64; d <<= 1) {
     const int v = __shfl_up(incl, d, 64);
     if(lane >= d) incl += v;
   }
-  if(lane == 63) s_wsum[wv] = incl;
-  if(lane == 0) {
-    s_wx[wv] = xr;
-    s_wy[wv] = xi;
-    s_wf[wv] = f;
-  }
-  __syncthreads();
-  int rank = incl;  // one past this thread's last head
-  for(int k = 0; k < wv; ++k) rank += s_wsum[k];
-  // carry into this wave from the waves to its right
-  R cr = 0, ci = 0;
-  for(int k = kW - 1; k > wv; --k) {
-    if(s_wf[k]) {
-      cr = s_wx[k];
-      ci = s_wy[k];
-    }
-    else {
-      cr += s_wx[k];
-      ci += s_wy[k];
-    }
-  }
+}
+
+// runs_store: with the carry (cr, ci) into the wave from its right and
+// rank = one past this lane's last head, sum each run and write it once
+template <bool CPLX, typename R, int S>
+__device__ __forceinline__ void runs_store(int lane, const int *h, const bool *in, const R *vr, const R *vi, R xr,
+                                           R xi, int f, R cr, R ci, int rank, uint32_t ns, void *__restrict__ out)
+{
   if(!f) {
     xr += cr;
     xi += ci;
@@ -1104,6 +1135,42 @@ __device__ __forceinline__ void gather_runs(int t, const int *h, const bool *in,
       ai = 0;
     }
   }
+}
+
+// the block's runs: each wave scans, the waves exchange their counts and
+// carries through LDS
+template <bool CPLX, typename R, int S, int B>
+__device__ __forceinline__ void gather_runs(int t, const int *h, const bool *in, const R *vr, const R *vi,
+                                            uint32_t ns, int *s_wsum, R *s_wx, R *s_wy, int *s_wf,
+                                            void *__restrict__ out)
+{
+  constexpr int kW = B / 64;
+  const int lane = t & 63, wv = t >> 6;
+  R xr, xi;
+  int f, incl;
+  runs_scan<CPLX, R, S>(lane, h, in, vr, vi, xr, xi, f, incl);
+  if(lane == 63) s_wsum[wv] = incl;
+  if(lane == 0) {
+    s_wx[wv] = xr;
+    s_wy[wv] = xi;
+    s_wf[wv] = f;
+  }
+  __syncthreads();
+  int rank = incl;  // one past this thread's last head
+  for(int k = 0; k < wv; ++k) rank += s_wsum[k];
+  // carry into this wave from the waves to its right
+  R cr = 0, ci = 0;
+  for(int k = kW - 1; k > wv; --k) {
+    if(s_wf[k]) {
+      cr = s_wx[k];
+      ci = s_wy[k];
+    }
+    else {
+      cr += s_wx[k];
+      ci += s_wy[k];
+    }
+  }
+  runs_store<CPLX, R, S>(lane, h, in, vr, vi, xr, xi, f, cr, ci, rank, ns, out);
 }
 
 // S slots per thread, kChunk / S threads per block
@@ -1253,6 +1320,228 @@ __global__ void __launch_bounds__(kChunk / S, S == 8 ? 5 : 8)
   const int i = find_item(first, nitems, cg);
   const GatherItem it = items[i];
   gather_chunk<CPLX, MIXED, WIDE, S>(it.table, it.recs, cg - first[i], 1, it.out);
+}
+
+// ------------------------------------------------------------------ k_gather_wave
+// The 4-slot gather with one wavefront per chunk, persistent.  The 4-slot
+// block above gives a chunk's left half (slots 0..255) to one wave and its
+// right half to another, which meet at a barrier; here one wave does the
+// right half, then the left half with the right half's carry, in registers:
+// the same operations in the same order, so the values are bitwise those of
+// the 4-slot block.  With no barrier per chunk the waves of a block run
+// independently:
+//   - each block stages the reference tensors of the groups that carry most
+//     contributions in LDS once (TensorLds, sized by the launch), so a contribution's
+//     tensor row is four LDS reads instead of four vector-memory loads (the
+//     tensor-row loads were ~40 % of the block kernel's time);
+//   - each wave loads its next chunk's record while it evaluates the current
+//     one, so the record's HBM latency is off the chain.
+// Chunks: each XCD takes a contiguous eighth of them (elements shared by
+// neighbouring chunks stay in that XCD's L2), its waves round robin.
+constexpr int kGwMaxWaves = 16;  // waves per block (at most)
+
+// LDS bytes of one wave: factors [7][kChunkElems], element list, groups
+template <bool MIXED>
+__host__ __device__ constexpr int gather_wave_bytes()
+{
+  return 7 * kChunkElems * (MIXED ? 4 : 8) + kChunkElems * 4 + kChunkElems;
+}
+
+__device__ __forceinline__ void wave_sync()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <bool CPLX, bool MIXED, bool WIDE>
+__global__ void __launch_bounds__(64 * kGwMaxWaves)
+  k_gather_wave(const GroupTable *__restrict__ gtp, const uint32_t *__restrict__ recs, int64_t nchunks,
+                void *__restrict__ out, const TensorLds tl)
+{
+  using R = typename std::conditional<MIXED, float, double>::type;
+  using R2 = typename std::conditional<MIXED, float2, double2>::type;
+  using WV = typename std::conditional<WIDE, uint4, uint2>::type;  // 4 words of one lane and half
+  constexpr int S = 4;
+  constexpr int kStride = WIDE ? kRecWide : kRecNarrow;
+  constexpr int kList = kStride - kChunkElems;
+  extern __shared__ uint4 s_gw[];
+  __shared__ const void *s_gF[kMaxGroups];
+  __shared__ const void *s_gT[kMaxGroups];
+  __shared__ const R *s_gD[kMaxGroups];
+  __shared__ int s_gNN[kMaxGroups];
+  __shared__ int s_toff[kMaxGroups], s_trs[kMaxGroups];
+  __shared__ uint8_t s_gR[kMaxGroups];
+  const GroupTable &gt = *gtp;
+  const int t = threadIdx.x, lane = t & 63;
+  const int wv = __builtin_amdgcn_readfirstlane(t >> 6), nw = (int)(blockDim.x >> 6);
+  const int ng = gt.ng;
+  unsigned char *lds = reinterpret_cast<unsigned char *>(s_gw);
+  if(t < ng) {
+    const bool treal = !CPLX || gt.treal[t];
+    s_gR[t] = treal ? 1 : 0;
+    s_gF[t] = MIXED ? (const void *)gt.F32[t] : (const void *)gt.F64[t];
+    s_gT[t] = MIXED ? (treal ? (const void *)gt.T32r[t] : (const void *)gt.T32[t])
+                    : (treal ? (const void *)gt.T64r[t] : (const void *)gt.T64[t]);
+    s_gD[t] = MIXED ? (const R *)gt.dense32[t] : (const R *)gt.dense64[t];
+    s_gNN[t] = gt.N[t] * gt.N[t];
+    s_toff[t] = tl.off[t];
+    s_trs[t] = tl.rs[t];
+  }
+  // the staged groups' tensor rows, 16 bytes per thread and step, in the
+  // layout the launch sized the LDS for
+  for(int g = 0; g < ng; ++g) {
+    const int off = tl.off[g];
+    if(off < 0) continue;
+    const uint4 *T = reinterpret_cast<const uint4 *>(
+      MIXED ? (tl.real[g] ? (const void *)gt.T32r[g] : (const void *)gt.T32[g])
+            : (tl.real[g] ? (const void *)gt.T64r[g] : (const void *)gt.T64[g]));
+    const int q16 = tl.q16[g], rs = tl.rs[g];
+    const int n16 = gt.N[g] * gt.N[g] * q16;
+    for(int idx = t; idx < n16; idx += (int)blockDim.x) {
+      const int r = idx / q16, p = idx - r * q16;
+      *reinterpret_cast<uint4 *>(lds + off + r * rs + p * 16) = T[idx];
+    }
+  }
+  // this wave's region: factors (structure of arrays), element list
+  unsigned char *wbase = lds + tl.bytes + wv * gather_wave_bytes<MIXED>();
+  R(*s_F)[kChunkElems] = reinterpret_cast<R(*)[kChunkElems]>(wbase);
+  uint32_t *s_el = reinterpret_cast<uint32_t *>(wbase + 7 * kChunkElems * sizeof(R));
+  uint8_t *s_eg = reinterpret_cast<uint8_t *>(s_el + kChunkElems);
+  __syncthreads();
+  const int logP = gt.logP;
+  const uint32_t pmask = (1u << logP) - 1u;
+  const uint32_t hb = WIDE ? 31u : 15u;
+  const int64_t nbx = gridDim.x / kXcds;  // the grid is a multiple of kXcds
+  const int64_t per = (nchunks + kXcds - 1) / kXcds;
+  const int64_t c0 = (int64_t)(blockIdx.x % kXcds) * per, c1 = min(nchunks, c0 + per);
+  const int64_t step = nbx * nw;
+  int64_t c = c0 + (int64_t)(blockIdx.x / kXcds) * nw + wv;
+  WV wq[2], wn[2];  // words of the halves: this chunk, next chunk
+  uint32_t lv[2], ln[2];
+  uint32_t ns = 0, clen = 0, nel = 0, nsn = 0, clenn = 0, neln = 0;
+  auto load = [&](int64_t cc, WV *wd, uint32_t *ld, uint32_t &a, uint32_t &b, uint32_t &e) {
+    const uint32_t *rec = recs + cc * kStride;
+    a = rec[0];
+    b = rec[1];
+    e = rec[2];
+    wd[0] = reinterpret_cast<const WV *>(rec + kRecHead)[lane];
+    wd[1] = reinterpret_cast<const WV *>(rec + kRecHead)[64 + lane];
+    ld[0] = rec[kList + lane];
+    ld[1] = rec[kList + 64 + lane];
+  };
+  auto words = [&](const WV &p, uint32_t *w) {
+    if constexpr(WIDE) {
+      w[0] = p.x; w[1] = p.y; w[2] = p.z; w[3] = p.w;
+    }
+    else {
+      w[0] = p.x & 0xFFFFu; w[1] = p.x >> 16; w[2] = p.y & 0xFFFFu; w[3] = p.y >> 16;
+    }
+  };
+  if(c < c1) load(c, wq, lv, ns, clen, nel);
+  while(c < c1) {
+    const int64_t cn = c + step;
+    // chunk elements: entry j of the list sits in list slot kChunkElems - nel + j
+#pragma unroll
+    for(int k = 0; k < 2; ++k) {
+      const int j = k * 64 + lane - (kChunkElems - (int)nel);
+      if(j >= 0) {
+        s_eg[j] = (uint8_t)(lv[k] >> 27);
+        s_el[j] = lv[k] & 0x7FFFFFFu;
+      }
+    }
+    wave_sync();
+    // element factors: 4 lanes per element, 2 factors each (one 64-byte
+    // row), in two batches of loads (registers); the next record's loads go
+    // out after the first batch and stay in flight through this chunk
+    constexpr int kFB = 2 * kChunkElems / 64;  // loads per lane and batch
+#pragma unroll
+    for(int bt = 0; bt < 2; ++bt) {
+      R2 fv[kFB];
+#pragma unroll
+      for(int k = 0; k < kFB; ++k) {
+        const int idx = (bt * kFB + k) * 64 + lane;
+        if(idx < (int)nel * 4) {
+          const int jj = idx >> 2, q = idx & 3;
+          fv[k] = reinterpret_cast<const R2 *>(reinterpret_cast<const R *>(s_gF[s_eg[jj]]) + (int64_t)s_el[jj] * 8)[q];
+        }
+      }
+      if(bt == 0 && cn < c1) load(cn, wn, ln, nsn, clenn, neln);
+#pragma unroll
+      for(int k = 0; k < kFB; ++k) {
+        const int idx = (bt * kFB + k) * 64 + lane;
+        if(idx < (int)nel * 4) {
+          const int jj = idx >> 2, q = idx & 3;
+          s_F[2 * q][jj] = fv[k].x;
+          if(q < 3) s_F[2 * q + 1][jj] = fv[k].y;
+        }
+      }
+    }
+    wave_sync();
+    // heads of the left half: the right half's ranks start after them
+    uint32_t w[S];
+    int h[S];
+    bool in[S];
+    int cnt0 = 0;
+    words(wq[0], w);
+#pragma unroll
+    for(int k = 0; k < S; ++k)
+      if(S * lane + k < (int)clen && ((w[k] >> hb) & 1u)) ++cnt0;
+#pragma unroll
+    for(int d = 32; d >= 1; d >>= 1) cnt0 += __shfl_xor(cnt0, d, 64);
+    R vr[S], vi[S];
+    R xr, xi, x1r, x1i;
+    int f, incl, f1;
+    // right half (slots 256..511): no carry from its right
+    words(wq[1], w);
+#pragma unroll
+    for(int k = 0; k < S; ++k) {
+      in[k] = S * (64 + lane) + k < (int)clen;
+      h[k] = in[k] ? (int)((w[k] >> hb) & 1u) : 1;  // past the end: closed, not written
+      vr[k] = 0;
+      vi[k] = 0;
+      if(in[k])
+        gather_eval<CPLX, MIXED, true>(w[k], logP, pmask, 1, s_F, s_eg, s_el, s_gR, s_gNN, s_gT, s_gD, vr[k], vi[k],
+                                       lds, s_toff, s_trs);
+    }
+    runs_scan<CPLX, R, S>(lane, h, in, vr, vi, xr, xi, f, incl);
+    x1r = __shfl(xr, 0, 64);
+    x1i = __shfl(xi, 0, 64);
+    f1 = __shfl(f, 0, 64);
+    runs_store<CPLX, R, S>(lane, h, in, vr, vi, xr, xi, f, (R)0, (R)0, incl + cnt0, ns, out);
+    // left half (slots 0..255), carry from the right half
+    words(wq[0], w);
+#pragma unroll
+    for(int k = 0; k < S; ++k) {
+      in[k] = S * lane + k < (int)clen;
+      h[k] = in[k] ? (int)((w[k] >> hb) & 1u) : 1;
+      vr[k] = 0;
+      vi[k] = 0;
+      if(in[k])
+        gather_eval<CPLX, MIXED, true>(w[k], logP, pmask, 1, s_F, s_eg, s_el, s_gR, s_gNN, s_gT, s_gD, vr[k], vi[k],
+                                       lds, s_toff, s_trs);
+    }
+    runs_scan<CPLX, R, S>(lane, h, in, vr, vi, xr, xi, f, incl);
+    R cr = 0, ci = 0;
+    if(f1) {
+      cr = x1r;
+      ci = x1i;
+    }
+    else {
+      cr += x1r;
+      ci += x1i;
+    }
+    runs_store<CPLX, R, S>(lane, h, in, vr, vi, xr, xi, f, cr, ci, incl, ns, out);
+    wave_sync();  // this chunk's list and factors are read before the next one's are written
+    c = cn;
+    wq[0] = wn[0];
+    wq[1] = wn[1];
+    lv[0] = ln[0];
+    lv[1] = ln[1];
+    ns = nsn;
+    clen = clenn;
+    nel = neln;
+  }
 }
 
 // ------------------------------------------------------------------ k_rhs
@@ -1697,9 +1986,114 @@ hipError_t launch_coef_rows(const Group &g, const double *xyz, bool all_terms, i
   return hipGetLastError();
 }
 
+constexpr int kTPad = 16;  // bytes after each tensor row in LDS: rows start on different banks
+
+TensorLds gather_lds_layout(const gfem_system &sys, int prec)
+{
+  TensorLds tl{};
+  const int ng = (int)sys.groups.size();
+  const int budget = prec ? 8192 : 16384;  // bytes of staged rows
+  int64_t total = 0;
+  std::vector<std::pair<int64_t, int>> cand;
+  for(int g = 0; g < ng; ++g) {
+    tl.off[g] = -1;
+    const Group &G = sys.groups[g];
+    const bool treal = !sys.cplx || G.treal;
+    const void *T = prec ? (treal ? (const void *)G.d_T32r : (const void *)G.d_T32)
+                         : (treal ? (const void *)G.d_T64r : (const void *)G.d_T64);
+    if(G.curved || !T || G.ne == 0) continue;
+    const int64_t w = G.ne * G.N * G.N;  // contributions of the group
+    total += w;
+    cand.push_back({w, g});
+  }
+  std::stable_sort(cand.begin(), cand.end(), [](const auto &a, const auto &b) { return a.first > b.first; });
+  int bytes = 0;
+  for(const auto &c : cand) {
+    if(c.first * 32 < total) break;  // groups with < 1/32 of the contributions stay in memory
+    const Group &G = sys.groups[c.second];
+    const bool treal = !sys.cplx || G.treal;
+    const int rb = 8 * (prec ? 4 : 8) * (treal ? 1 : 2);
+    const int need = G.N * G.N * (rb + kTPad);
+    if(bytes + need > budget) continue;
+    tl.off[c.second] = bytes;
+    tl.rs[c.second] = rb + kTPad;
+    tl.q16[c.second] = rb / 16;
+    tl.real[c.second] = treal ? 1 : 0;
+    bytes += need;
+  }
+  tl.bytes = bytes;
+  return tl;
+}
+
+template <bool C, bool M, bool W>
+const void *gather_wave_fn()
+{
+  return reinterpret_cast<const void *>(&k_gather_wave<C, M, W>);
+}
+
+// waves per block and blocks of the one-wave gather: the most resident waves
+// per CU (LDS and registers, from the runtime's occupancy), cached per
+// precision for the staged bytes
+static hipError_t gather_wave_plan(const gfem_system &sys, int prec, const void *fn, int tbytes)
+{
+  if(sys.gw_tbytes[prec] == tbytes) return hipSuccess;
+  const int wb = prec ? gather_wave_bytes<true>() : gather_wave_bytes<false>();
+  int dev = 0, ncu = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if(e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  // one limit for every launch of the function: the LDS of a CU less the
+  // kernel's static arrays
+  constexpr int kMaxDyn = 160 * 1024 - 1024;
+  if(e == hipSuccess) e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, kMaxDyn);
+  int best_nw = 0, best_occ = 0;
+  for(int nw : {4, 6, 8, 10, 12, 16}) {
+    if(e != hipSuccess) break;
+    const size_t lds = (size_t)tbytes + (size_t)nw * wb;
+    if(lds > (size_t)kMaxDyn) continue;
+    int occ = 0;
+    if(e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, fn, 64 * nw, lds);
+    if(e == hipSuccess && occ * nw > best_occ * best_nw) {
+      best_nw = nw;
+      best_occ = occ;
+    }
+  }
+  if(e != hipSuccess) return e;
+  if(best_nw == 0) return hipErrorInvalidConfiguration;
+  int64_t nb = std::min<int64_t>((int64_t)ncu * best_occ, (sys.n_chunks + best_nw - 1) / best_nw);
+  sys.gw_blocks[prec] = xcd_grid(std::max<int64_t>(nb, 1));
+  sys.gw_nw[prec] = best_nw;
+  sys.gw_tbytes[prec] = tbytes;
+  return hipSuccess;
+}
+
+template <bool C, bool M, bool W>
+static hipError_t gather_wave_launch(const gfem_system &sys, int prec, void *out, hipStream_t s)
+{
+  const TensorLds tl = gather_lds_layout(sys, prec);
+  const void *fn = gather_wave_fn<C, M, W>();
+  hipError_t e = gather_wave_plan(sys, prec, fn, tl.bytes);
+  if(e != hipSuccess) return e;
+  const int nw = sys.gw_nw[prec];
+  const size_t lds = (size_t)tl.bytes + (size_t)nw * (M ? gather_wave_bytes<true>() : gather_wave_bytes<false>());
+  hipLaunchKernelGGL((k_gather_wave<C, M, W>), dim3((unsigned)sys.gw_blocks[prec]), dim3(64 * nw), lds, s,
+                     sys.d_table, sys.d_rec, sys.n_chunks, out, tl);
+  return hipGetLastError();
+}
+
 hipError_t launch_gather(const gfem_system &sys, int prec, int tensor, void *out, hipStream_t s)
 {
   if(sys.n_chunks == 0) return hipSuccess;
+  if(tensor && sys.gather_wave[prec] && sys.gather_slots[prec] == 4) {
+    if(sys.cplx)
+      return prec ? (sys.rec_wide ? gather_wave_launch<true, true, true>(sys, prec, out, s)
+                                  : gather_wave_launch<true, true, false>(sys, prec, out, s))
+                  : (sys.rec_wide ? gather_wave_launch<true, false, true>(sys, prec, out, s)
+                                  : gather_wave_launch<true, false, false>(sys, prec, out, s));
+    return prec ? (sys.rec_wide ? gather_wave_launch<false, true, true>(sys, prec, out, s)
+                                : gather_wave_launch<false, true, false>(sys, prec, out, s))
+                : (sys.rec_wide ? gather_wave_launch<false, false, true>(sys, prec, out, s)
+                                : gather_wave_launch<false, false, false>(sys, prec, out, s));
+  }
   const GroupTable *gt = tensor ? sys.d_table : sys.d_table_dense;
   const dim3 grid((unsigned)xcd_grid(sys.n_chunks));
 #define GFEM_GATHER_S(C, M, W, S)                                                                            \

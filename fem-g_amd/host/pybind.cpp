@@ -241,6 +241,9 @@ void bind_formulation(py::module_ &m, const char *name)
            d["linear_eb_shift"] = eb;
            d["spmv_lanes"] = lanes;
            d["rhs_lanes"] = rl;
+           int ow = 0;
+           check(gfem_gather_one_wave(f.system(), parse_precision(prec), &ow), "gfem_gather_one_wave");
+           d["gather_one_wave"] = ow != 0;
            return d;
          },
          py::arg("precision") = "fp64")

@@ -430,6 +430,13 @@ gfem_status gfem_last_kernel_times(const gfem_system *sys, int max, const char *
 gfem_status gfem_tuning(const gfem_system *sys, gfem_precision prec, int *gather_slots, int *linear_eb_shift,
                         int *spmv_lanes, int *rhs_lanes);
 
+/* Whether the matrix gather of `prec` runs as the one-wave persistent kernel
+ * (one wavefront per chunk, the largest groups' reference tensors staged in
+ * LDS; chosen at the first assembly when faster, or by GFEM_GATHER_WAVE=1/0).
+ * Its values are bitwise those of the 4-slot block kernel.  No counterpart in
+ * the reference. */
+gfem_status gfem_gather_one_wave(const gfem_system *sys, gfem_precision prec, int *one_wave);
+
 /* Algorithmic HBM bytes of the fused matrix gather kernel for `prec`
  * (contribution map + element factors + CSR values written). */
 gfem_status gfem_gather_bytes(const gfem_system *sys, gfem_precision prec, double *bytes);
