@@ -24,6 +24,11 @@ for p in (os.path.join(ROOT, "fem-g_amd"), ROOT):
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "gather_traffic.json")
 GATHER_KERNEL = "k_gather<true, false, false, {}>"  # complex, FP64, 16-bit words (P2), slots per thread
+GATHER_WAVE_KERNEL = "k_gather_wave<true, false, false>"  # the one-wave persistent form of 4 slots
+
+
+def gather_kernel_name(args):
+    return GATHER_WAVE_KERNEL if args.gather_slots == 4 and args.gather_wave else GATHER_KERNEL.format(args.gather_slots)
 
 
 def measured_traffic(kernel):
@@ -446,6 +451,8 @@ def main():
     ap.add_argument("--serve-baseline", type=int, default=16, help="requests timed one at a time")
     ap.add_argument("--gather-slots", type=int, choices=[2, 4, 8], default=4,
                     help="contributions per gather thread (4: 128-thread blocks, 2: 256-thread blocks)")
+    ap.add_argument("--gather-wave", type=int, choices=[0, 1], default=1,
+                    help="4 slots: the one-wave persistent gather (1) or the 128-thread block (0); same values")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="BYTES",
                     help="run a device-to-device copy of BYTES first (counter calibration)")
     args = ap.parse_args()
@@ -457,6 +464,7 @@ def main():
     from gfem import workloads as W
 
     os.environ["GFEM_GATHER_SLOTS"] = str(args.gather_slots)
+    os.environ["GFEM_GATHER_WAVE"] = str(args.gather_wave)
     if args.calib_copy:
         copy_calibration(args.calib_copy)
 
@@ -603,8 +611,8 @@ def main():
                    "element_dofs": edofs, "n_unknown": int(F.n_unknown), "nnz": int(F.nnz)},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(GATHER_KERNEL.format(args.gather_slots)) if args.n == 50 and args.order == 2 else None,
-                     "kernel": GATHER_KERNEL.format(args.gather_slots), "kernel_ms": gather_ms,
+                     "traffic": measured_traffic(gather_kernel_name(args)) if args.n == 50 and args.order == 2 else None,
+                     "kernel": gather_kernel_name(args), "kernel_ms": gather_ms,
                      "algorithmic_bytes": gbytes},
         "cpu_baseline": cpu,
         "autotuned": tuning,
