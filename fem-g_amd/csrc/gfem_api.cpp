@@ -16,6 +16,7 @@
 
 
 #include "gfem.h"
+#include "hbasis.h"
 #include "internal.h"
 #include "refelem.h"
 
@@ -357,24 +358,26 @@ gfem_status gfem_system_destroy(gfem_system *sys)
   return GFEM_OK;
 }
 
-gfem_status gfem_group_add(gfem_system *sys, int dim, int type, int64_t ne, const int64_t *conn,
-                           const int64_t *dof_num, const int32_t *dof_type, const int64_t *elem2nnz, int *group_out)
+// The group of one (entity, element type) pass.  family 0: Lagrange, the
+// keys are the element's nodes (conn holds N = npe node tags per element);
+// family 1: hierarchical H1 of `order` (hbasis.h), conn holds the dim + 1
+// vertex tags per element in ascending order and the N keys follow the
+// basis' local order.
+static gfem_status group_add_impl(gfem_system *sys, int dim, int type, int family, int order, int N, int64_t ne,
+                                  const int64_t *conn, const int64_t *dof_num, const int32_t *dof_type,
+                                  const int64_t *elem2nnz, int *group_out)
 {
-  if(!sys) return fail(nullptr, GFEM_ERR_INVALID_ARG, "system is null");
   gfem_ctx *ctx = sys->ctx;
   if(sys->finalized) return fail(ctx, GFEM_ERR_STATE, "group_add after finalize");
-  ElemType et;
-  if(!elem_type_info(type, et) || et.dim != dim || dim < 1)
-    return fail(ctx, GFEM_ERR_UNSUPPORTED, "element type " + std::to_string(type) + " not supported");
   if(ne < 0 || (ne > 0 && (!conn || !dof_num || !dof_type)))
     return fail(ctx, GFEM_ERR_INVALID_ARG, "group_add: bad arguments");
   if((int)sys->groups.size() >= kMaxGroups)
     return fail(ctx, GFEM_ERR_UNSUPPORTED, "more than " + std::to_string(kMaxGroups) + " element groups");
-  if(et.npe > kMaxN) return fail(ctx, GFEM_ERR_UNSUPPORTED, "too many dofs by element");
+  if(N > kMaxN) return fail(ctx, GFEM_ERR_UNSUPPORTED, "too many dofs by element");
   // Global and linked keys (Dof.h:27-35) need the reference's _indicesLC /
   // global-row handling (VectorFactory.cpp:181-208), which this engine does
   // not implement: refuse them rather than drop their contributions.
-  for(int64_t i = 0; i < ne * (int64_t)et.npe; ++i) {
+  for(int64_t i = 0; i < ne * (int64_t)N; ++i) {
     const int32_t t = dof_type[i];
     if(t == GFEM_AT_UNKNOWN_GLOBAL || t == GFEM_AT_LINKED || t == GFEM_AT_LINKED_BUBBLE)
       return fail(ctx, GFEM_ERR_UNSUPPORTED, "assemble type " + std::to_string(t) + " (global / linked) not supported");
@@ -384,23 +387,29 @@ gfem_status gfem_group_add(gfem_system *sys, int dim, int type, int64_t ne, cons
   Group g;
   g.dim = dim;
   g.type = type;
-  g.order = et.order;
-  g.N = et.npe;
+  g.order = order;
+  g.family = family;
+  g.N = N;
   g.nv = dim + 1;
   g.ne = ne;
   g.e_off = sys->groups.empty() ? 0 : sys->groups.back().e_off + sys->groups.back().ne;
-  const int N = g.N;
+  // ascending vertex order can reverse a tetrahedron: its volume measure is |det|
+  g.absdet = family == 1 && dim == 3;
+  const int cs = family == 1 ? g.nv : N;  // node tags per element in conn
   // vertices + affinity of the high-order nodes (a curved element makes the
   // group curved: point-by-point geometry)
-  const std::vector<Lat> lat = lattice_nodes(dim, et.order);
+  const std::vector<Lat> lat = family == 0 ? lattice_nodes(dim, order) : std::vector<Lat>();
   g.h_vtx.resize((size_t)(ne * g.nv));
   const std::vector<double> &X = ctx->h_xyz;
   for(int64_t e = 0; e < ne; ++e) {
-    for(int k = 0; k < N; ++k) {
-      const int64_t tag = conn[e * N + k];
+    for(int k = 0; k < cs; ++k) {
+      const int64_t tag = conn[e * cs + k];
       if(tag < 1 || tag > ctx->n_nodes) return fail(ctx, GFEM_ERR_INVALID_ARG, "node tag out of range");
+      if(family == 1 && k > 0 && tag <= conn[e * cs + k - 1])
+        return fail(ctx, GFEM_ERR_INVALID_ARG, "hierarchical group: vertex tags must ascend within an element");
     }
-    for(int v = 0; v < g.nv; ++v) g.h_vtx[e * g.nv + v] = (int32_t)(conn[e * N + v] - 1);
+    for(int v = 0; v < g.nv; ++v) g.h_vtx[e * g.nv + v] = (int32_t)(conn[e * cs + v] - 1);
+    if(family == 1) continue;
     double h = 0.;
     for(int v = 1; v < g.nv; ++v)
       for(int i = 0; i < 3; ++i) h = std::max(h, std::fabs(X[3 * (conn[e * N + v] - 1) + i] - X[3 * (conn[e * N] - 1) + i]));
@@ -408,7 +417,7 @@ gfem_status gfem_group_add(gfem_system *sys, int dim, int type, int64_t ne, cons
       for(int i = 0; i < 3; ++i) {
         double x = 0.;
         for(int v = 0; v < g.nv; ++v) x += lat[k][v] * X[3 * (conn[e * N + v] - 1) + i];
-        x /= et.order;
+        x /= order;
         if(std::fabs(x - X[3 * (conn[e * N + k] - 1) + i]) > 1e-9 * (h + 1e-300)) g.curved = true;
       }
     }
@@ -452,6 +461,29 @@ gfem_status gfem_group_add(gfem_system *sys, int dim, int type, int64_t ne, cons
   sys->groups.push_back(std::move(g));
   if(group_out) *group_out = (int)sys->groups.size() - 1;
   return GFEM_OK;
+}
+
+gfem_status gfem_group_add(gfem_system *sys, int dim, int type, int64_t ne, const int64_t *conn,
+                           const int64_t *dof_num, const int32_t *dof_type, const int64_t *elem2nnz, int *group_out)
+{
+  if(!sys) return fail(nullptr, GFEM_ERR_INVALID_ARG, "system is null");
+  ElemType et;
+  if(!elem_type_info(type, et) || et.dim != dim || dim < 1)
+    return fail(sys->ctx, GFEM_ERR_UNSUPPORTED, "element type " + std::to_string(type) + " not supported");
+  return group_add_impl(sys, dim, type, 0, et.order, et.npe, ne, conn, dof_num, dof_type, elem2nnz, group_out);
+}
+
+gfem_status gfem_group_add_hierarchical(gfem_system *sys, int dim, int order, int64_t ne, const int64_t *conn,
+                                        const int64_t *dof_num, const int32_t *dof_type, const int64_t *elem2nnz,
+                                        int *group_out)
+{
+  if(!sys) return fail(nullptr, GFEM_ERR_INVALID_ARG, "system is null");
+  if(dim < 1 || dim > 3) return fail(sys->ctx, GFEM_ERR_UNSUPPORTED, "hierarchical groups of dimension 1..3");
+  if(order < 1 || order > GFEM_HIER_MAX_ORDER)
+    return fail(sys->ctx, GFEM_ERR_UNSUPPORTED, "hierarchical order " + std::to_string(order));
+  const int p1_type = dim == 1 ? 1 : (dim == 2 ? 2 : 4);  // gmsh line, triangle, tetrahedron
+  return group_add_impl(sys, dim, p1_type, 1, order, hier::count(dim, order), ne, conn, dof_num, dof_type, elem2nnz,
+                        group_out);
 }
 
 gfem_status gfem_group_add_term(gfem_system *sys, int group, const gfem_term_desc *t)
@@ -810,9 +842,18 @@ gfem_status gfem_system_finalize(gfem_system *sys)
     for(auto &R : g.rules) {
       GFEM_HIP(ctx, dupload(&R.d_w, R.w.data(), R.w.size()));
       GFEM_HIP(ctx, dupload(&R.d_pts, R.pts.data(), R.pts.size()));
-      GFEM_HIP(ctx, dalloc(&R.d_bf, (size_t)(R.Q * g.N)));
-      GFEM_HIP(ctx, dalloc(&R.d_gbf, (size_t)(R.Q * g.N * 3)));
-      GFEM_HIP(ctx, launch_basis(g, R, nullptr));
+      if(g.family == 1) {
+        // hierarchical tables: a few kB per rule, evaluated on the host
+        std::vector<double> bf((size_t)(R.Q * g.N)), gbf((size_t)(R.Q * g.N * 3));
+        hier::eval(g.dim, g.order, R.Q, R.pts.data(), bf.data(), gbf.data());
+        GFEM_HIP(ctx, dupload(&R.d_bf, bf.data(), bf.size()));
+        GFEM_HIP(ctx, dupload(&R.d_gbf, gbf.data(), gbf.size()));
+      }
+      else {
+        GFEM_HIP(ctx, dalloc(&R.d_bf, (size_t)(R.Q * g.N)));
+        GFEM_HIP(ctx, dalloc(&R.d_gbf, (size_t)(R.Q * g.N * 3)));
+        GFEM_HIP(ctx, launch_basis(g, R, nullptr));
+      }
     }
     GFEM_HIP(ctx, launch_tensor(g, nullptr));
     GroupDev gd{};
@@ -825,6 +866,7 @@ gfem_status gfem_system_finalize(gfem_system *sys)
       gd.rules[r] = RuleDev{g.rules[r].Q, g.rules[r].d_w, g.rules[r].d_pts, g.rules[r].d_bf, g.rules[r].d_gbf};
     if(g.curved && !g.d_conn) GFEM_HIP(ctx, dupload(&g.d_conn, g.h_conn.data(), g.h_conn.size()));
     gd.conn = g.curved ? g.d_conn : nullptr;
+    gd.absdet = g.absdet ? 1 : 0;
     GFEM_HIP(ctx, dupload(&g.d_dev, &gd, 1));
   }
   timer.mark("groups (rules, tensors, jit)");

@@ -151,6 +151,7 @@ struct RuleDev {
 };
 struct GroupDev {
   int dim, nv, N, nterms;
+  int absdet;  // volume measure |det| (hierarchical tetrahedra, vertices in ascending order)
   const DevTerm *terms;
   RuleDev rules[8];
   // curved groups: node indices of every element [ne][N] (storage order), for
@@ -169,6 +170,8 @@ struct Rule {
 
 struct Group {
   int dim = 0, type = 0, order = 0, N = 0, nv = 0;
+  int family = 0;       // basis: 0 Lagrange (keys = nodes), 1 hierarchical H1 (hbasis.h)
+  bool absdet = false;  // volume measure |det|: ascending vertex order may reverse a tetrahedron
   int64_t ne = 0, e_off = 0;
   std::vector<int32_t> h_vtx;
   std::vector<int64_t> h_num;

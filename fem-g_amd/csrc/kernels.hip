@@ -445,7 +445,7 @@ __global__ void k_tensor_many(const TensorItem *__restrict__ items)
 }
 
 // ------------------------------------------------------------------ k_geometry
-__device__ __forceinline__ void geometry_one(int dim, int nv, int64_t e, const int32_t *__restrict__ vtx,
+__device__ __forceinline__ void geometry_one(int dim, int nv, int absdet, int64_t e, const int32_t *__restrict__ vtx,
                                              const double *__restrict__ xyz, double *__restrict__ F64,
                                              float *__restrict__ F32)
 {
@@ -453,6 +453,7 @@ __device__ __forceinline__ void geometry_one(int dim, int nv, int64_t e, const i
   load_vertices(vtx, nv, e, xyz, X);
   Geo g;
   element_geometry(dim, X, g);
+  if(absdet) g.det = fabs(g.det);
   double f[8];
   geometry_factors(g, f);
   f[7] = 0.;
@@ -465,12 +466,12 @@ __device__ __forceinline__ void geometry_one(int dim, int nv, int64_t e, const i
   }
 }
 
-__global__ void __launch_bounds__(256) k_geometry(int dim, int nv, int64_t ne, const int32_t *__restrict__ vtx,
-                                                  const double *__restrict__ xyz, double *__restrict__ F64,
-                                                  float *__restrict__ F32)
+__global__ void __launch_bounds__(256) k_geometry(int dim, int nv, int absdet, int64_t ne,
+                                                  const int32_t *__restrict__ vtx, const double *__restrict__ xyz,
+                                                  double *__restrict__ F64, float *__restrict__ F32)
 {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if(e < ne) geometry_one(dim, nv, e, vtx, xyz, F64, F32);
+  if(e < ne) geometry_one(dim, nv, absdet, e, vtx, xyz, F64, F32);
 }
 
 // every group of a system in one launch: thread t takes element t - off[g]
@@ -478,7 +479,7 @@ __global__ void __launch_bounds__(256) k_geometry(int dim, int nv, int64_t ne, c
 // contiguous, so the scan is short and uniform within most waves)
 struct GeomGroups {
   int ng;
-  int dim[kMaxGroups], nv[kMaxGroups];
+  int dim[kMaxGroups], nv[kMaxGroups], absdet[kMaxGroups];
   int64_t off[kMaxGroups + 1];
   const int32_t *vtx[kMaxGroups];
   double *F64[kMaxGroups];
@@ -503,7 +504,7 @@ __global__ void __launch_bounds__(256) k_geometry_all(const GeomGroups gg, const
     if(t >= total) return;
     int g = g0;
     while(g + 1 < gg.ng && t >= gg.off[g + 1]) ++g;
-    geometry_one(gg.dim[g], gg.nv[g], t - gg.off[g], gg.vtx[g], xyz, gg.F64[g], gg.F32[g]);
+    geometry_one(gg.dim[g], gg.nv[g], gg.absdet[g], t - gg.off[g], gg.vtx[g], xyz, gg.F64[g], gg.F32[g]);
     return;
   }
   if(t < tend) {
@@ -512,6 +513,7 @@ __global__ void __launch_bounds__(256) k_geometry_all(const GeomGroups gg, const
     load_vertices(gg.vtx[g0], gg.nv[g0], e, xyz, X);
     Geo g;
     element_geometry(gg.dim[g0], X, g);
+    if(gg.absdet[g0]) g.det = fabs(g.det);
     double f[8];
     geometry_factors(g, f);
     f[7] = 0.;
@@ -697,7 +699,10 @@ __global__ void __launch_bounds__(256) k_dense(const GroupDev *__restrict__ gd, 
   double X[4][3];
   load_vertices(vtx, gd->nv, e, xyz, X);
   Geo g;
-  if(!conn) element_geometry(dim, X, g);
+  if(!conn) {
+    element_geometry(dim, X, g);
+    if(gd->absdet) g.det = fabs(g.det);
+  }
   double2 acc = make_double2(0., 0.);
   const int nterms = __builtin_amdgcn_readfirstlane(gd->nterms);
   for(int k = 0; k < nterms; ++k) {
@@ -1804,6 +1809,7 @@ hipError_t launch_geometry_all(const gfem_system &sys, const double *xyz, bool f
     const Group &G = sys.groups[g];
     gg.dim[g] = G.dim;
     gg.nv[g] = G.nv;
+    gg.absdet[g] = G.absdet ? 1 : 0;
     gg.off[g + 1] = gg.off[g] + G.ne;
     gg.vtx[g] = G.d_vtx;
     gg.F64[g] = G.d_F64;
@@ -1818,7 +1824,7 @@ hipError_t launch_geometry_all(const gfem_system &sys, const double *xyz, bool f
 hipError_t launch_geometry(const Group &g, const double *xyz, hipStream_t s)
 {
   if(g.ne == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_geometry, dim3(blocks_for(g.ne, 256)), dim3(256), 0, s, g.dim, g.nv, g.ne, g.d_vtx, xyz,
+  hipLaunchKernelGGL(k_geometry, dim3(blocks_for(g.ne, 256)), dim3(256), 0, s, g.dim, g.nv, g.absdet ? 1 : 0, g.ne, g.d_vtx, xyz,
                      g.d_F64, nullptr);  // FP64 only
   return hipGetLastError();
 }

@@ -18,7 +18,8 @@ except ImportError as e:  # pragma: no cover - exercised only on a broken build
 
 from ._gfem_host import (Domain, Field, ManyInstance, Model, ScalarFunction, Server, Unknown, abs,  # noqa: F401,E402
                          assemble_many, cos, dof, dt2_dof, dt_dof, exp, from_arrays, gauss_rule, grad,
-                         hilbert_index, hilbert_key, lattice_nodes, pow, read_msh, set_current_model, set_device,
+                         hier_basis, hier_keys, hilbert_index, hilbert_key, lattice_nodes, pow, read_msh,
+                         set_current_model, set_device,
                          sin, sqrt, tf, unit_cube, unit_square, write_msh, x, y, z)
 
 _FORMULATIONS = {

@@ -32,6 +32,7 @@ class Spec:
     frequency: float = 0.0
     dofs_sort: str = "default"
     scalar: str = "f64"
+    basis: str = "lagrange"  # or "hierarchical" (HierarchicalH1 on a first-order mesh)
 
     def element_dofs(self):
         """sum over assembled elements of dofs per element (element-DOFs)."""
@@ -41,23 +42,29 @@ class Spec:
         total = 0
         for b in self.model.blocks():
             if (b["dim"], b["tag"]) in ents and b["dim"] > 0:
-                total += b["conn"].shape[0] * b["conn"].shape[1]
+                npe = b["conn"].shape[1] if self.basis == "lagrange" else _hier_count(b["dim"], self.order)
+                total += b["conn"].shape[0] * npe
         return total
 
 
-def poisson2d(n=64, order=1, jitter=0.2, seed=1234, gauss=None):
-    m = unit_square(n, order, jitter, seed)
+def _hier_count(dim, p):
+    return p + 1 if dim == 1 else ((p + 1) * (p + 2) // 2 if dim == 2 else (p + 1) * (p + 2) * (p + 3) // 6)
+
+
+def poisson2d(n=64, order=1, jitter=0.2, seed=1234, gauss=None, basis="lagrange"):
+    m = unit_square(n, 1 if basis == "hierarchical" else order, jitter, seed)
     g = 2 * order if gauss is None else gauss
     omega = [(2, 1)]
     gamma = [(1, 1), (1, 2), (1, 3), (1, 4)]
     return Spec(
-        name=f"poisson2d_n{n}_p{order}", model=m, order=order, field_domain=omega,
+        name=f"poisson2d_n{n}_p{order}" + ("_h" if basis == "hierarchical" else ""), model=m, order=order,
+        field_domain=omega,
         dirichlet=[(gamma, 0.0)],
         terms=[
             dict(kind="bilinear", domain=omega, gauss=g, form=1),
             dict(kind="linear", domain=omega, gauss=g, source=-1.0),
         ],
-        scalar="f64")
+        scalar="f64", basis=basis)
 
 
 def gaussian_source(center=(0.5, 0.5, 0.5), sigma=0.05):
@@ -67,21 +74,22 @@ def gaussian_source(center=(0.5, 0.5, 0.5), sigma=0.05):
 
 
 def helmholtz3d(n=50, order=2, k=2 * math.pi * 5, jitter=0.15, seed=4321, center=(0.5, 0.5, 0.5), sigma=0.05,
-                scalar="c128", gauss=None):
-    m = unit_cube(n, order, jitter, seed)
+                scalar="c128", gauss=None, basis="lagrange"):
+    m = unit_cube(n, 1 if basis == "hierarchical" else order, jitter, seed)
     g = 2 * order if gauss is None else gauss
     omega = [(3, 1)]
     gamma = [(2, s) for s in range(1, 7)]
     f = gaussian_source(center, sigma)
     return Spec(
-        name=f"helmholtz3d_n{n}_p{order}", model=m, order=order, field_domain=omega,
+        name=f"helmholtz3d_n{n}_p{order}" + ("_h" if basis == "hierarchical" else ""), model=m, order=order,
+        field_domain=omega,
         terms=[
             dict(kind="bilinear", domain=omega, gauss=g, form=1),
             dict(kind="bilinear", domain=omega, gauss=g, form=0, coef=-k * k),
             dict(kind="bilinear", domain=gamma, gauss=g, form=0, coef=-1j * k),
             dict(kind="linear", domain=omega, gauss=g, source=-f),
         ],
-        scalar=scalar)
+        scalar=scalar, basis=basis)
 
 
 # term tags of helmholtz3d (order of its integral() calls)
@@ -119,7 +127,7 @@ def build(spec, scalar=None):
     """Formulation of a spec on the current device, before pre()."""
     set_current_model(spec.model)
     F = Formulation(spec.name, scalar or spec.scalar)
-    u = Field("u", _domain(spec.field_domain), spec.order)
+    u = Field("u", _domain(spec.field_domain), spec.order, spec.basis)
     for ents, value in spec.dirichlet:
         u.add_constraint(_domain(ents), value)
     F._field = u  # keep the field alive with the formulation

@@ -218,6 +218,14 @@ Field::Field(const std::string &name, const domain::Domain &dom, int order)
 {
   if(order < 1 || order > 4) throw common::Exception("Lagrange order must be in 1..4");
 }
+Field::Field(const std::string &name, const domain::Domain &dom, FunctionSpaceTypeForm0 type, int order)
+  : _name(name), _domain(dom), _type(type), _order(order), _tag(g_field_tag++)
+{
+  if(type == FunctionSpaceTypeForm0::Lagrange && (order < 1 || order > 4))
+    throw common::Exception("Lagrange order must be in 1..4");
+  if(type == FunctionSpaceTypeForm0::HierarchicalH1 && (order < 1 || order > GFEM_HIER_MAX_ORDER))
+    throw common::Exception("HierarchicalH1 order must be in 1.." + std::to_string(GFEM_HIER_MAX_ORDER));
+}
 void Field::addConstraint(const domain::Domain &dom, const function::ScalarFunction &value)
 {
   _constraints.emplace_back(dom, value);
@@ -550,13 +558,16 @@ common::Timer Formulation<T>::pre(DofsSort sort)
   _dofNum.assign(nn + 1, 0);
   _dofType.assign(nn + 1, 0);
   _fixedValues.clear();
+  _keyDof.clear();
+  _unknownKeys.clear();
   _nFixed = _nUnknown = 0;
-  for(const auto &ent : _field->domain().entities())
-    for(int type : M.elementTypes(ent.first, ent.second)) {
-      const model::Block *b = M.block(ent.first, ent.second, type);
-      if(b->order != _field->order() && b->dim > 0)
-        throw common::Exception("Lagrange order differs from the mesh order");
-    }
+  if(!_field->hierarchical())
+    for(const auto &ent : _field->domain().entities())
+      for(int type : M.elementTypes(ent.first, ent.second)) {
+        const model::Block *b = M.block(ent.first, ent.second, type);
+        if(b->order != _field->order() && b->dim > 0)
+          throw common::Exception("Lagrange order differs from the mesh order");
+      }
   // assembly passes: every (entity, element type) carrying a term
   for(const auto &ent : M.entities()) {
     std::vector<unsigned> terms;
@@ -578,13 +589,174 @@ common::Timer Formulation<T>::pre(DofsSort sort)
       _passes.push_back(std::move(p));
     }
   }
-  if(_preDevice)
+  if(_field->hierarchical())
+    preHier(sort);  // entity keys: on the host
+  else if(_preDevice)
     preDevice(sort);
   else
     preHost(sort);
   time.tock();
   _preSeconds = time.time();
   return time;
+}
+
+// pre() of a hierarchical H1 field (FunctionSpaceHierarchicalH1 keys,
+// FunctionSpaceInterface.cpp:319-331): a key is (entity, mode), the entity
+// named by its vertex tags in ascending order (csrc/hbasis.h).  Numbering is
+// first touch over the same block sequences as preHost (fixed dofs
+// DofsFactory.cpp:39-74, unknown dofs :406-460), the unknowns renumbered
+// along the Hilbert curve of the key coordinates (an entity's barycentre,
+// DofsManager.cpp:118-239).  Fixed values: at order 1 the constraint at the
+// vertex; above, the L2 projection of the constraint on its domain, solved
+// as a formulation of its own as the reference does (DofsFactory.cpp:104-131).
+template <class T>
+void Formulation<T>::preHier(DofsSort sort)
+{
+  using gfem::hier::Key;
+  const model::Model &M = *_model;
+  const int p = _field->order();
+  auto checkBlock = [](const model::Block *b) {
+    if(b->dim > 0 && b->order != 1)
+      throw common::Exception("hierarchical fields take a first-order (straight-sided) mesh");
+  };
+  auto elemKeys = [&](const model::Block *b, int64_t e, int64_t *vt) {
+    const int nv = b->dim + 1;
+    for(int k = 0; k < nv; ++k) vt[k] = b->conn[e * b->npe + k];
+    std::sort(vt, vt + nv);
+    if(b->dim == 0) return std::vector<Key>{Key{1, {vt[0], 0, 0, 0}, 0}};
+    return gfem::hier::keys(b->dim, p, vt);
+  };
+  auto keyCoord = [&](const Key &k, double *x) {
+    for(int j = 0; j < 3; ++j) {
+      double c = 0.;
+      for(int i = 0; i < k.n; ++i) c += M.xyz[3 * (k.v[i] - 1) + j];
+      x[j] = c / k.n;
+    }
+  };
+  int64_t vt[4];
+  // fixed dofs, per constraint in order; a key keeps its first constraint
+  const auto &cons = _field->constraints();
+  std::vector<std::vector<Key>> fixedKeys(cons.size());
+  for(size_t c = 0; c < cons.size(); ++c)
+    for(const auto &ent : cons[c].first.entities())
+      for(int type : M.elementTypes(ent.first, ent.second)) {
+        const model::Block *b = M.block(ent.first, ent.second, type);
+        checkBlock(b);
+        for(int64_t e = 0; e < b->size(); ++e)
+          for(const Key &k : elemKeys(b, e, vt))
+            if(_keyDof.emplace(k, std::make_pair((int64_t)(_nFixed + 1), (int8_t)2)).second) {
+              ++_nFixed;
+              fixedKeys[c].push_back(k);
+            }
+      }
+  // unknown dofs over the field's domain
+  for(const auto &ent : _field->domain().entities())
+    for(int type : M.elementTypes(ent.first, ent.second)) {
+      const model::Block *b = M.block(ent.first, ent.second, type);
+      checkBlock(b);
+      for(int64_t e = 0; e < b->size(); ++e)
+        for(const Key &k : elemKeys(b, e, vt))
+          if(_keyDof.emplace(k, std::make_pair((int64_t)(_nUnknown + 1), (int8_t)1)).second) {
+            ++_nUnknown;
+            _unknownKeys.push_back(k);
+          }
+    }
+  if(_nUnknown == 0) {
+    _lastError = "no unknown dofs";
+    return;
+  }
+  if(sort != DofsSort::None && M.dim >= 2) {
+    double mn[3], mx[3];
+    M.boundingBox(mn, mx);
+    float fmn[3], fmx[3];
+    for(int j = 0; j < 3; ++j) {
+      fmn[j] = (float)(mn[j] - 0.000001 * std::fabs(mn[j]));
+      fmx[j] = (float)(mx[j] + 0.000001 * std::fabs(mx[j]));
+    }
+    std::vector<float> coords(3 * _unknownKeys.size());
+    for(size_t i = 0; i < _unknownKeys.size(); ++i) {
+      double x[3];
+      keyCoord(_unknownKeys[i], x);
+      for(int j = 0; j < 3; ++j) coords[3 * i + j] = (float)x[j];
+    }
+    const std::vector<int64_t> order = reorder::hilbertOrder(M.dim, coords, fmn, fmx);
+    std::vector<Key> sorted(_unknownKeys.size());
+    for(size_t i = 0; i < order.size(); ++i) {
+      sorted[i] = _unknownKeys[order[i]];
+      _keyDof[sorted[i]].first = (int64_t)i + 1;
+    }
+    _unknownKeys.swap(sorted);
+  }
+  for(const auto &kv : _keyDof)
+    if(kv.first.n == 1) {
+      _dofNum[kv.first.v[0]] = kv.second.first;
+      _dofType[kv.first.v[0]] = kv.second.second;
+    }
+  // fixed values
+  _fixedValues.assign(_nFixed, std::complex<double>(0.));
+  for(size_t c = 0; c < cons.size(); ++c) {
+    if(fixedKeys[c].empty()) continue;
+    if(p == 1) {
+      for(const Key &k : fixedKeys[c]) {
+        double x[3];
+        keyCoord(k, x);
+        _fixedValues[(size_t)_keyDof[k].first - 1] = cons[c].second.evaluate(x[0], x[1], x[2]);
+      }
+      continue;
+    }
+    Formulation<T> proj("boundary");
+    field::Field fcp("fieldCp", cons[c].first, field::FunctionSpaceTypeForm0::HierarchicalH1, p);
+    const std::string gauss = "Gauss" + std::to_string(2 * (p + 1));
+    proj.integral(equation::dof(fcp), equation::tf(fcp), cons[c].first, gauss);
+    proj.integral(-cons[c].second, equation::tf(fcp), cons[c].first, gauss);
+    proj.pre(DofsSort::None);
+    if(!proj.lastError().empty()) throw common::Exception("constraint projection: " + proj.lastError());
+    proj.assemble();
+    if(!proj.lastError().empty()) throw common::Exception("constraint projection: " + proj.lastError());
+    SolveOptions so;
+    so.tol = 1e-13;
+    so.restart = 100;
+    so.maxIter = 20000;
+    proj.solve(so);
+    if(!proj.lastError().empty()) throw common::Exception("constraint projection: " + proj.lastError());
+    const std::vector<T> &x = proj.getSolution();
+    for(const Key &k : fixedKeys[c]) {
+      const auto d = proj.keyDof(k);
+      if(d.second == 1 && d.first >= 1 && (size_t)d.first <= x.size())
+        _fixedValues[(size_t)_keyDof[k].first - 1] = std::complex<double>(x[(size_t)d.first - 1]);
+    }
+  }
+  // element maps of every pass, keys in the basis' local order
+  for(auto &pp : _passes) {
+    const model::Block *b = M.block(pp.dim, pp.tag, pp.type);
+    checkBlock(b);
+    const int nv = pp.dim + 1;
+    const int N = gfem::hier::count(pp.dim, p);
+    const int64_t ne = b->size();
+    pp.N = N;
+    pp.verts.resize((size_t)(ne * nv));
+    pp.num.assign((size_t)(ne * N), 0);
+    pp.atype.assign((size_t)(ne * N), GFEM_AT_UNDEFINED);
+    for(int64_t e = 0; e < ne; ++e) {
+      const std::vector<Key> keys = elemKeys(b, e, vt);
+      for(int k = 0; k < nv; ++k) pp.verts[(size_t)(e * nv + k)] = vt[k];
+      for(int i = 0; i < N; ++i) {
+        const auto it = _keyDof.find(keys[i]);
+        if(it == _keyDof.end()) continue;
+        pp.num[(size_t)(e * N + i)] = it->second.first - 1;
+        pp.atype[(size_t)(e * N + i)] = it->second.second == 1 ? GFEM_AT_UNKNOWN : GFEM_AT_FIXED;
+      }
+    }
+  }
+  buildPattern();
+}
+
+template <class T>
+std::pair<int64_t, int> Formulation<T>::keyDof(const gfem::hier::Key &k) const
+{
+  const auto it = _keyDof.find(k);
+  if(it == _keyDof.end()) return {0, 0};
+  return {it->second.first, it->second.second};
 }
 
 template <class T>
@@ -645,6 +817,7 @@ void Formulation<T>::preHost(DofsSort sort)
   // element maps of every pass (FieldInterface::fillIndices)
   for(auto &p : _passes) {
     const model::Block *b = M.block(p.dim, p.tag, p.type);
+    p.N = b->npe;
     p.num.resize(b->conn.size());
     p.atype.resize(b->conn.size());
     for(size_t k = 0; k < b->conn.size(); ++k) {
@@ -655,13 +828,21 @@ void Formulation<T>::preHost(DofsSort sort)
     }
   }
 
+  buildPattern();
+}
+
+// sparsity pattern and element -> nonzero maps of the passes' element maps
+// (p.num / p.atype, p.N keys per element)
+template <class T>
+void Formulation<T>::buildPattern()
+{
   // sparsity pattern (Pattern::build + MatrixFactory::addPattern/finalizePattern):
   // row = tf dof, col = dof; unknown x unknown; upper triangle if symmetric
   const int64_t n = (int64_t)_nUnknown;
   std::vector<int64_t> cnt(n + 1, 0);
   for(const auto &p : _passes) {
     if(!p.bilinear) continue;
-    const int N = M.block(p.dim, p.tag, p.type)->npe;
+    const int N = p.N;
     const int64_t ne = (int64_t)p.num.size() / N;
     for(int64_t e = 0; e < ne; ++e) {
       int nu = 0;
@@ -676,7 +857,7 @@ void Formulation<T>::preHost(DofsSort sort)
     std::vector<int64_t> fill(cnt.begin(), cnt.end() - 1);
     for(const auto &p : _passes) {
       if(!p.bilinear) continue;
-      const int N = M.block(p.dim, p.tag, p.type)->npe;
+      const int N = p.N;
       const int64_t ne = (int64_t)p.num.size() / N;
       for(int64_t e = 0; e < ne; ++e)
         for(int i = 0; i < N; ++i) {
@@ -702,7 +883,7 @@ void Formulation<T>::preHost(DofsSort sort)
   // element -> nonzero map
   for(auto &p : _passes) {
     if(!p.bilinear) continue;
-    const int N = M.block(p.dim, p.tag, p.type)->npe;
+    const int N = p.N;
     const int64_t ne = (int64_t)p.num.size() / N;
     p.e2n.assign((size_t)(ne * N * N), -1);
     for(int64_t e = 0; e < ne; ++e)
@@ -867,6 +1048,10 @@ void Formulation<T>::upload()
     int k = 0;
     if(_pre)
       st = gfem_group_add_pre(_sys, _pre, (int)(p - _passes.data()), p->dim, p->type, &g);
+    else if(_field->hierarchical())
+      st = gfem_group_add_hierarchical(_sys, p->dim, _field->order(), (int64_t)(p->verts.size() / (p->dim + 1)),
+                                       p->verts.data(), p->num.data(), p->atype.data(),
+                                       p->e2n.empty() ? nullptr : p->e2n.data(), &g);
     else
       st = gfem_group_add(_sys, p->dim, p->type, b->size(), b->conn.data(), p->num.data(), p->atype.data(),
                           p->e2n.empty() ? nullptr : p->e2n.data(), &g);

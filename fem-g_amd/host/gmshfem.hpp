@@ -13,10 +13,12 @@
 #include <set>
 #include <stdexcept>
 #include <string>
+#include <unordered_map>
 #include <utility>
 #include <vector>
 
 #include "gfem.h"
+#include "hbasis.h"
 #include "model.hpp"
 
 namespace gmshfem {
@@ -103,21 +105,30 @@ ScalarFunction pow(const ScalarFunction &a, const ScalarFunction &b);
 }  // namespace function
 
 namespace field {
-// Field<T, Form0> with a Lagrange function space
-// (fem-master/src/field/Field.h:30-44, FunctionSpaceLagrange.cpp:21-50)
+// FunctionSpaceTypeForm0 (fem-master/src/field/FunctionSpace.h): the scalar
+// H1 spaces of the hot path
+enum class FunctionSpaceTypeForm0 { Lagrange, HierarchicalH1 };
+
+// Field<T, Form0> with a Lagrange or hierarchical H1 function space
+// (fem-master/src/field/Field.h:30-44, FunctionSpaceLagrange.cpp:21-50,
+// FunctionSpaceHierarchicalH1.cpp:20-24)
 class Field {
  public:
   Field(const std::string &name, const domain::Domain &dom, int order);
+  Field(const std::string &name, const domain::Domain &dom, FunctionSpaceTypeForm0 type, int order);
   void addConstraint(const domain::Domain &dom, const function::ScalarFunction &value);
   const std::string &name() const { return _name; }
   const domain::Domain &domain() const { return _domain; }
   int order() const { return _order; }
+  FunctionSpaceTypeForm0 type() const { return _type; }
+  bool hierarchical() const { return _type == FunctionSpaceTypeForm0::HierarchicalH1; }
   int tag() const { return _tag; }
   const std::vector<std::pair<domain::Domain, function::ScalarFunction>> &constraints() const { return _constraints; }
 
  private:
   std::string _name;
   domain::Domain _domain;
+  FunctionSpaceTypeForm0 _type = FunctionSpaceTypeForm0::Lagrange;
   int _order;
   int _tag;
   std::vector<std::pair<domain::Domain, function::ScalarFunction>> _constraints;
@@ -231,7 +242,12 @@ class Formulation {
   const std::vector<int64_t> &ai() const { return _ai; }
   const std::vector<int64_t> &aj() const;
   // numDof (1-based) and type (1 unknown, 2 fixed, 0 none) of the dof of a node tag
+  // (hierarchical fields: of the vertex function of that node)
   std::pair<int64_t, int> nodeDof(int64_t tag) const;
+  // hierarchical fields: numDof and type of a key (entity vertex tags
+  // ascending, mode), and the key of every unknown dof in numDof order
+  std::pair<int64_t, int> keyDof(const gfem::hier::Key &k) const;
+  const std::vector<gfem::hier::Key> &unknownKeys() const { return _unknownKeys; }
   // element maps of the block (dim, tag, type): numDof-1 / AssembleType per key
   const std::vector<int64_t> *blockDofs(int dim, int tag, int type) const;
   const std::vector<int32_t> *blockTypes(int dim, int tag, int type) const;
@@ -252,6 +268,8 @@ class Formulation {
  private:
   struct Pass {
     int dim, tag, type;
+    int N = 0;                   // keys per element
+    std::vector<int64_t> verts;  // hierarchical fields: vertex tags per element, ascending
     std::vector<int64_t> num;
     std::vector<int32_t> atype;
     std::vector<int64_t> e2n;
@@ -271,6 +289,25 @@ class Formulation {
   std::vector<int64_t> _dofNum;     // node tag -> numDof (1-based)
   std::vector<int8_t> _dofType;     // node tag -> 1 unknown, 2 fixed
   std::vector<std::complex<double>> _fixedValues;
+  struct KeyHash {
+    size_t operator()(const gfem::hier::Key &k) const
+    {
+      uint64_t h = (uint64_t)k.n * 0x9E3779B97F4A7C15ull ^ (uint64_t)k.mode;
+      for(int i = 0; i < k.n; ++i) h = (h ^ (uint64_t)k.v[i]) * 0x100000001B3ull;
+      return (size_t)h;
+    }
+  };
+  struct KeyEq {
+    bool operator()(const gfem::hier::Key &a, const gfem::hier::Key &b) const
+    {
+      if(a.n != b.n || a.mode != b.mode) return false;
+      for(int i = 0; i < a.n; ++i)
+        if(a.v[i] != b.v[i]) return false;
+      return true;
+    }
+  };
+  std::unordered_map<gfem::hier::Key, std::pair<int64_t, int8_t>, KeyHash, KeyEq> _keyDof;
+  std::vector<gfem::hier::Key> _unknownKeys;
   std::vector<int64_t> _ai;
   mutable std::vector<int64_t> _aj;  // device pre(): copied on request
   mutable std::vector<Pass> _passes;
@@ -290,7 +327,9 @@ class Formulation {
   void releaseSystem();
   void releasePre();
   void preHost(DofsSort sort);
+  void preHier(DofsSort sort);
   void preDevice(DofsSort sort);
+  void buildPattern();
   void upload();
 };
 

@@ -264,6 +264,20 @@ void bind_formulation(py::module_ &m, const char *name)
     .def(
       "rhs", [](const F &f, unsigned i) { return to_numpy(f.getRHS(i)); }, py::arg("idx") = 0)
     .def("node_dof", &F::nodeDof)
+    .def("unknown_keys",
+         [](const F &f) {
+           // hierarchical fields: one row per unknown dof, numDof order:
+           // (entity vertex count, vertex tags ascending (0-padded), mode)
+           const auto &k = f.unknownKeys();
+           py::array_t<int64_t> a({(py::ssize_t)k.size(), (py::ssize_t)6});
+           auto r = a.mutable_unchecked<2>();
+           for(size_t i = 0; i < k.size(); ++i) {
+             r(i, 0) = k[i].n;
+             for(int j = 0; j < 4; ++j) r(i, 1 + j) = k[i].v[j];
+             r(i, 5) = k[i].mode;
+           }
+           return a;
+         })
     .def("system_maps",
          [](const F &f, int which) {
            int64_t n = 0;
@@ -431,10 +445,16 @@ PYBIND11_MODULE(_gfem_host, m)
   m.def("pow", &function::pow);
 
   py::class_<field::Field>(m, "Field")
-    .def(py::init<const std::string &, const domain::Domain &, int>(), py::arg("name"), py::arg("domain"),
-         py::arg("order") = 1)
+    .def(py::init([](const std::string &name, const domain::Domain &dom, int order, const std::string &basis) {
+           if(basis == "lagrange") return new field::Field(name, dom, field::FunctionSpaceTypeForm0::Lagrange, order);
+           if(basis == "hierarchical")
+             return new field::Field(name, dom, field::FunctionSpaceTypeForm0::HierarchicalH1, order);
+           throw std::invalid_argument("basis must be 'lagrange' or 'hierarchical'");
+         }),
+         py::arg("name"), py::arg("domain"), py::arg("order") = 1, py::arg("basis") = "lagrange")
     .def("add_constraint", &field::Field::addConstraint)
-    .def_property_readonly("order", &field::Field::order);
+    .def_property_readonly("order", &field::Field::order)
+    .def_property_readonly("basis", [](const field::Field &f) { return f.hierarchical() ? "hierarchical" : "lagrange"; });
 
   py::class_<equation::Unknown>(m, "Unknown")
     .def("__rmul__", [](const equation::Unknown &u, const ScalarFunction &c) { return c * u; })
@@ -520,5 +540,25 @@ PYBIND11_MODULE(_gfem_host, m)
     std::vector<double> p, w;
     gfem::gauss_rule(dim, deg, p, w);
     return py::make_tuple(p, w);
+  });
+  // hierarchical H1 basis (csrc/hbasis.h): values [npts, N] and reference
+  // gradients [npts, N, 3] at reference points pts [npts, 3]
+  m.def("hier_basis", [](int dim, int order, py::array_t<double, py::array::c_style | py::array::forcecast> pts) {
+    if(dim < 1 || dim > 3 || order < 1 || order > GFEM_HIER_MAX_ORDER) throw std::invalid_argument("dim / order");
+    if(pts.ndim() != 2 || pts.shape(1) != 3) throw std::invalid_argument("pts must be [npts, 3]");
+    const int n = (int)pts.shape(0), N = gfem::hier::count(dim, order);
+    py::array_t<double> v({(py::ssize_t)n, (py::ssize_t)N}), g({(py::ssize_t)n, (py::ssize_t)N, (py::ssize_t)3});
+    gfem::hier::eval(dim, order, n, pts.data(), v.mutable_data(), g.mutable_data());
+    return py::make_tuple(v, g);
+  });
+  // keys of the local functions of an element with ascending vertex tags:
+  // rows (entity vertex count, vertex tags (0-padded), mode)
+  m.def("hier_keys", [](int dim, int order, std::vector<int64_t> vt) {
+    if(dim < 1 || dim > 3 || order < 1 || order > GFEM_HIER_MAX_ORDER || (int)vt.size() != dim + 1)
+      throw std::invalid_argument("dim / order / vertices");
+    std::vector<std::vector<int64_t>> out;
+    for(const auto &k : gfem::hier::keys(dim, order, vt.data()))
+      out.push_back({k.n, k.v[0], k.v[1], k.v[2], k.v[3], k.mode});
+    return out;
   });
 }

@@ -24,6 +24,9 @@
  *                                  (entity, element type) pass,
  *                                  problem/IndiceBucket.h:21, problem/ElementBucket.h:24-25,
  *                                  term/Assembler.cpp:68-155 (assemblyInitialization)
+ *   gfem_group_add_hierarchical    the same pass of a HierarchicalH1 field,
+ *                                  field/FunctionSpaceHierarchicalH1.cpp:83-109,
+ *                                  field/FunctionSpaceInterface.cpp:319-331, 495-519
  *   gfem_group_add_term            Assembler::addBilinearTerm / addLinearTerm,
  *                                  term/Assembler.cpp:38-50; term evaluators
  *                                  term/BilinearTerm.cpp:155-273, term/LinearTerm.cpp:87-159
@@ -196,6 +199,25 @@ gfem_status gfem_system_destroy(gfem_system *sys);
 gfem_status gfem_group_add(gfem_system *sys, int dim, int gmsh_type, int64_t ne, const int64_t *conn,
                            const int64_t *dof_num, const int32_t *dof_type, const int64_t *elem2nnz,
                            int *group_out);
+
+/* The same pass for a field of the hierarchical H1 space of `order`
+ * (FunctionSpaceHierarchicalH1, field/FunctionSpaceHierarchicalH1.cpp:83-109;
+ * keys and orientations field/FunctionSpaceInterface.cpp:319-331, 495-519) on
+ * straight-sided simplices of dimension dim (1 line, 2 triangle, 3
+ * tetrahedron):
+ *   conn[ne*(dim+1)]    vertex tags (1-based) of each element, ascending
+ *   dof_num, dof_type   [ne*N], N = dim P_order, keys in the basis' local order:
+ *                       vertices, edges (a<b), faces (a<b<c), interior, the
+ *                       modes of an entity by total degree
+ *   elem2nnz[ne*N*N]    as for gfem_group_add
+ * The basis is the published hierarchic family of Solin, Segeth and Dolezel
+ * (Legendre-based kernel functions); gmsh's H1Legendre tables are not part of
+ * the reference, so values match the reference's up to the choice of basis
+ * of the same space. */
+#define GFEM_HIER_MAX_ORDER 4
+gfem_status gfem_group_add_hierarchical(gfem_system *sys, int dim, int order, int64_t ne, const int64_t *conn,
+                                        const int64_t *dof_num, const int32_t *dof_type, const int64_t *elem2nnz,
+                                        int *group_out);
 gfem_status gfem_group_add_term(gfem_system *sys, int group, const gfem_term_desc *term);
 
 /* Build the device maps (nonzero-sorted element map, RHS maps, reference
