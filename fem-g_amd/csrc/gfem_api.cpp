@@ -334,6 +334,7 @@ gfem_status gfem_system_destroy(gfem_system *sys)
   dfree(sys->d_sol_V);
   dfree(sys->d_sol_a32);
   dfree(sys->d_sol_state);
+  dfree(sys->d_sol_dcgs);
   dfree(sys->d_sol_y);
   dfree(sys->d_sol_part);
   dfree(sys->d_sol_nrm);

@@ -315,6 +315,8 @@ struct gfem_system {
   int32_t *d_aj = nullptr;
   void *d_sol_x = nullptr, *d_sol_r = nullptr, *d_sol_w = nullptr, *d_sol_V = nullptr, *d_sol_a32 = nullptr;
   double2 *d_sol_state = nullptr, *d_sol_y = nullptr, *d_sol_part = nullptr, *d_sol_nrm = nullptr;
+  // delayed CGS2: unrotated Hessenberg, first-pass column, coefficients, dots
+  double2 *d_sol_dcgs = nullptr;
   double *d_sol_dinv = nullptr;  // right scaling of the Krylov iteration (1 / row norms)
   void *h_sol_est = nullptr;  // pinned
   int sol_m = 0;
@@ -324,6 +326,7 @@ struct gfem_system {
   // above, the restart length sol_graph_m and the SpMV lanes sol_graph_lanes
   std::vector<hipGraphExec_t> sol_graphs;
   int sol_graph_m = 0, sol_graph_lanes = 0;
+  bool sol_graph_dcgs = false;  // the graphs hold delayed-CGS2 steps
   hipStream_t sol_cap = nullptr;  // capture stream (default priority)
   // parameters changed by gfem_term_update, applied at the next assembly
   bool table_dirty = false;

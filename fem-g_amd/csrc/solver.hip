@@ -239,6 +239,108 @@ __global__ void __launch_bounds__(kRedThreads) k_dots_final(const double2 *__res
   }
 }
 
+// Delayed CGS2 (one reduction per Krylov step): block partials of
+// conj(V_k) . u and conj(V_k) . w for k in [0, nk), both targets in one sweep.
+// Block b owns rows [b kD2Span, (b+1) kD2Span); its waves take the basis
+// vectors round robin over those rows (u and w, reused by the four waves,
+// stay in L1), so the basis is read once from HBM and no barrier is needed.
+// A lane sums its rows in order, the wave in a fixed butterfly, the blocks in
+// a fixed order (k_dots2_final): bitwise reproducible.
+// part[(2k + t) * gridDim.x + block], t = 0 for u, 1 for w.
+constexpr int kD2Span = 256;  // rows per block
+inline unsigned dots2_blocks(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + kD2Span - 1) / kD2Span); }
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_dots2(int64_t n, int nk, const T *__restrict__ V, int64_t ld,
+                                               const T *__restrict__ u, const T *__restrict__ w,
+                                               double2 *__restrict__ part)
+{
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int64_t r0 = (int64_t)blockIdx.x * kD2Span;
+  const int64_t r1 = min(n, r0 + kD2Span);
+  for(int k = wv; k < nk; k += 4) {
+    double2 au = make_double2(0., 0.), aw = make_double2(0., 0.);
+#pragma unroll 4
+    for(int64_t i = r0 + lane; i < r1; i += 64) {
+      const double2 v = ld_d(V, (int64_t)k * ld + i);
+      const double2 pa = zconjmul(v, ld_d(u, i)), pb = zconjmul(v, ld_d(w, i));
+      au.x += pa.x;
+      au.y += pa.y;
+      aw.x += pb.x;
+      aw.y += pb.y;
+    }
+#pragma unroll
+    for(int d = 32; d >= 1; d >>= 1) {
+      au.x += __shfl_xor(au.x, d, 64);
+      au.y += __shfl_xor(au.y, d, 64);
+      aw.x += __shfl_xor(aw.x, d, 64);
+      aw.y += __shfl_xor(aw.y, d, 64);
+    }
+    if(lane == 0) {
+      part[(int64_t)(2 * k) * gridDim.x + blockIdx.x] = au;
+      part[(int64_t)(2 * k + 1) * gridDim.x + blockIdx.x] = aw;
+    }
+  }
+}
+
+// out[o] = sum over the nb block partials of part[o][.], fixed order; one block per output
+__global__ void __launch_bounds__(kRedThreads) k_dots2_final(const double2 *__restrict__ part, int nb,
+                                                             double2 *__restrict__ out)
+{
+  __shared__ double2 s_red[kRedThreads / 64];
+  const int o = blockIdx.x;
+  double2 s = make_double2(0., 0.);
+  for(int b = threadIdx.x; b < nb; b += kRedThreads) {
+    s.x += part[(int64_t)o * nb + b].x;
+    s.y += part[(int64_t)o * nb + b].y;
+  }
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+  for(int d = 32; d >= 1; d >>= 1) {
+    s.x += __shfl_xor(s.x, d, 64);
+    s.y += __shfl_xor(s.y, d, 64);
+  }
+  if(lane == 0) s_red[wv] = s;
+  __syncthreads();
+  if(threadIdx.x == 0) {
+    double2 t = make_double2(0., 0.);
+    for(int q = 0; q < kRedThreads / 64; ++q) {
+      t.x += s_red[q].x;
+      t.y += s_red[q].y;
+    }
+    out[o] = t;
+  }
+}
+
+// Delayed CGS2 update of step j (u = V_j, the once-orthogonalised vector, v
+// = A u in w):
+//   q_j     = (u - sum_{k<j} V_k s_k) / r            into V_j (in place)
+//   u_{j+1} = (v - sum_{k<j} V_k c_k - q_j c_j) / r  into V_{j+1}
+// one read of V_0..V_j for both; s, c, 1/r from k_dcgs_coef
+template <typename T>
+__global__ void __launch_bounds__(256) k_dcgs_update(int64_t n, int j, T *__restrict__ V, int64_t ld,
+                                                     const T *__restrict__ v, const double2 *__restrict__ sv,
+                                                     const double2 *__restrict__ cv, const double2 *__restrict__ ir)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(i >= n) return;
+  double2 q = ld_d(V, (int64_t)j * ld + i), un = ld_d(v, i);
+  for(int k = 0; k < j; ++k) {
+    const double2 vk = ld_d(V, (int64_t)k * ld + i);
+    const double2 a = zmul(vk, sv[k]), b = zmul(vk, cv[k]);
+    q.x -= a.x;
+    q.y -= a.y;
+    un.x -= b.x;
+    un.y -= b.y;
+  }
+  const double inv = ir->x;
+  q = make_double2(q.x * inv, q.y * inv);
+  const double2 qc = zmul(q, cv[j]);
+  un = make_double2((un.x - qc.x) * inv, (un.y - qc.y) * inv);
+  st(V, (int64_t)j * ld + i, q);
+  st(V, (int64_t)(j + 1) * ld + i, un);
+}
+
 // w -= sum_{k<=j} V_k h_k  (h FP64, arithmetic in FP64, w FP32)
 template <typename T>
 __global__ void __launch_bounds__(256) k_orth_update(int64_t n, int j1, const T *__restrict__ V, int64_t ld,
@@ -341,16 +443,15 @@ __global__ void k_add_into(int k, const double2 *__restrict__ y, double2 *__rest
 // Column j holds h_{0..j,j} (from the dots) and h_{j+1,j}^2 in slot j+1
 // (squared norm from the norm pass).  Applies the previous rotations, makes
 // rotation j, updates g and the residual estimate |g_{j+1}|.
-__global__ void k_givens(int m, int j, double2 *__restrict__ stt)
+// rotations of column j whose entries 0..j+1 are in place (h_{j+1,j} real)
+__device__ void givens_column(int m, int j, double2 *__restrict__ stt)
 {
-  if(threadIdx.x != 0 || blockIdx.x != 0) return;
   double2 *H = stt;
   double2 *cs = stt + (int64_t)(m + 1) * m;
   double2 *sn = cs + m;
   double2 *g = sn + m;
   double2 *est = g + m + 1;
   double2 *col = H + (int64_t)j * (m + 1);
-  col[j + 1] = make_double2(sqrt(col[j + 1].x), 0.);
   for(int i = 0; i < j; ++i) {
     // [c s; -conj(s) c] with real c
     const double c = cs[i].x;
@@ -391,6 +492,88 @@ __global__ void k_givens(int m, int j, double2 *__restrict__ stt)
   g[j] = make_double2(c * gj.x, c * gj.y);
   g[j + 1] = make_double2(-csg.x, -csg.y);
   est->x = hypot(g[j + 1].x, g[j + 1].y);
+}
+
+__global__ void k_givens(int m, int j, double2 *__restrict__ stt)
+{
+  if(threadIdx.x != 0 || blockIdx.x != 0) return;
+  double2 *col = stt + (int64_t)j * (m + 1);
+  col[j + 1] = make_double2(sqrt(col[j + 1].x), 0.);
+  givens_column(m, j, stt);
+}
+
+// Delayed CGS2 bookkeeping of step j (one wavefront).  D[2k] = V_k^H u,
+// D[2k+1] = V_k^H v for k <= j, with u = V_j and v = A u.  dc = Hu[(m+1) x m]
+// (the unrotated Hessenberg, A Q_{j-1} = Q_j Hu) | h1[m+1] (first-pass
+// column) | s[m] | c[m+1] | 1/r.
+//   s = Q_{j-1}^H u, r = sqrt(|u|^2 - |s|^2) (Pythagoras: u - Q s is
+//   orthogonal to Q), column j-1 = h1 + s with r below it, then rotated;
+//   c = Q_j^H v = [z; (u^H v - s^H z) / r], h1 (column j, first pass) =
+//   (c - Hu s) / r.  tail: the cycle's last column only.
+// Sums over k are lane-strided then combined in a fixed butterfly.
+__global__ void __launch_bounds__(64) k_dcgs_coef(int m, int j, int tail, const double2 *__restrict__ D,
+                                                  double2 *__restrict__ dc, double2 *__restrict__ stt)
+{
+  const int t = threadIdx.x;
+  double2 *Hu = dc;
+  double2 *h1 = Hu + (int64_t)(m + 1) * m;
+  double2 *sv = h1 + m + 1;
+  double2 *cv = sv + m;
+  double2 *ir = cv + m + 1;
+  double ss = 0.;
+  for(int k = t; k < j; k += 64) {
+    const double2 d = D[2 * k];
+    sv[k] = d;
+    ss += d.x * d.x + d.y * d.y;
+  }
+#pragma unroll
+  for(int d = 32; d >= 1; d >>= 1) ss += __shfl_xor(ss, d, 64);
+  const double r2 = D[2 * j].x - ss;
+  const double r = r2 > 0. ? sqrt(r2) : 0.;
+  const double inv = r > 0. ? 1. / r : 0.;
+  if(j >= 1) {
+    double2 *hu = Hu + (int64_t)(j - 1) * (m + 1);
+    double2 *col = stt + (int64_t)(j - 1) * (m + 1);
+    for(int i = t; i < j; i += 64) {  // h1[i], sv[i] of this lane
+      hu[i] = make_double2(h1[i].x + sv[i].x, h1[i].y + sv[i].y);
+      col[i] = hu[i];
+    }
+    if(t == 0) {
+      hu[j] = make_double2(r, 0.);
+      col[j] = hu[j];
+    }
+    __syncthreads();
+    if(t == 0) givens_column(m, j - 1, stt);
+  }
+  if(tail) return;
+  double2 sz = make_double2(0., 0.);
+  for(int k = t; k < j; k += 64) {
+    const double2 z = D[2 * k + 1];
+    cv[k] = z;
+    const double2 p = zconjmul(sv[k], z);
+    sz.x += p.x;
+    sz.y += p.y;
+  }
+#pragma unroll
+  for(int d = 32; d >= 1; d >>= 1) {
+    sz.x += __shfl_xor(sz.x, d, 64);
+    sz.y += __shfl_xor(sz.y, d, 64);
+  }
+  if(t == 0) {
+    const double2 beta = D[2 * j + 1];
+    cv[j] = make_double2((beta.x - sz.x) * inv, (beta.y - sz.y) * inv);
+    ir[0] = make_double2(inv, 0.);
+  }
+  __syncthreads();  // sv, cv and column j-1 of Hu seen by every lane
+  for(int i = t; i <= j; i += 64) {
+    double2 a = cv[i];
+    for(int l = (i > 0 ? i - 1 : 0); l < j; ++l) {  // Hu column l has rows 0..l+1
+      const double2 p = zmul(Hu[(int64_t)l * (m + 1) + i], sv[l]);
+      a.x -= p.x;
+      a.y -= p.y;
+    }
+    h1[i] = make_double2(a.x * inv, a.y * inv);
+  }
 }
 
 // back substitution R y = g over the first k columns (one thread); y -> out
@@ -443,6 +626,10 @@ struct Gmres {
 
 }  // namespace
 
+// delayed CGS2 state (k_dcgs_coef layout) followed by the step's 2(m+1) dot products
+static size_t dcgs_dots_offset(int m) { return (size_t)(m + 1) * m + (size_t)(m + 1) + m + (size_t)(m + 1) + 1; }
+static size_t dcgs_size(int m) { return dcgs_dots_offset(m) + 2 * (size_t)(m + 1); }
+
 template <bool C>
 static gfem_status gmres_ir(gfem_system *sys, int rhs_idx, double tol, int m, int max_iter, hipStream_t s,
                             gfem_solve_info *info)
@@ -459,13 +646,17 @@ static gfem_status gmres_ir(gfem_system *sys, int rhs_idx, double tol, int m, in
     sys->sol_m = 0;
     dfree(sys->d_sol_V);
     dfree(sys->d_sol_state);
+    dfree(sys->d_sol_dcgs);
     dfree(sys->d_sol_y);
     dfree(sys->d_sol_part);
     for(hipGraphExec_t g : sys->sol_graphs) (void)hipGraphExecDestroy(g);  // they hold the old workspace
     sys->sol_graphs.clear();
     if(!chk(hipMalloc(&sys->d_sol_V, need_v))) return fail(ctx, GFEM_ERR_OUT_OF_MEMORY, "solver basis allocation");
-    if(!chk(hipMalloc(&sys->d_sol_part, sizeof(double2) * (size_t)kRedBlocks * (size_t)(m + 2))))
+    if(!chk(hipMalloc(&sys->d_sol_part, sizeof(double2) * (size_t)std::max<int64_t>(kRedBlocks, dots2_blocks(n)) *
+                                          (size_t)(2 * m + 4))))
       return fail(ctx, GFEM_ERR_OUT_OF_MEMORY, "solver reduction allocation");
+    if(!chk(hipMalloc(&sys->d_sol_dcgs, sizeof(double2) * dcgs_size(m))))
+      return fail(ctx, GFEM_ERR_OUT_OF_MEMORY, "solver state allocation");
     if(!chk(hipMalloc(&sys->d_sol_state, sizeof(double2) * ((size_t)(m + 1) * m + 3 * (size_t)m + 2))))
       return fail(ctx, GFEM_ERR_OUT_OF_MEMORY, "solver state allocation");
     if(!chk(hipMalloc(&sys->d_sol_y, sizeof(double2) * (size_t)(m + 1))))
@@ -539,11 +730,40 @@ static gfem_status gmres_ir(gfem_system *sys, int rhs_idx, double tol, int m, in
     }
     sys->spmv_tuned = true;
   }
+  // Gram-Schmidt: delayed CGS2 (default; one reduction and two reads of the
+  // basis per step) or, with GFEM_SOLVER_CGS2=1, classical CGS2 (two
+  // reductions, four reads)
+  const char *cg_env = std::getenv("GFEM_SOLVER_CGS2");
+  const bool dcgs = !(cg_env && cg_env[0] == '1');
+  double2 *dc = sys->d_sol_dcgs;
+  double2 *dcD = dc + dcgs_dots_offset(m);
+  double2 *dcS = dc + (int64_t)(m + 1) * m + (m + 1), *dcC = dcS + m, *dcR = dcC + m + 1;
+  // Delayed CGS2 step j: V_j holds u_j (once orthogonalised; q_0 at j = 0).
+  // w = A u_j; one sweep for Q_{j-1}^H [u_j, w] and u_j^H [u_j, w]; the
+  // bookkeeping finishes column j-1 (rotated: the estimate lags one step)
+  // and the coefficients; one sweep writes q_j into V_j and u_{j+1} into
+  // V_{j+1}.  The tail (j = m) finishes column m-1 without a product.
+  auto dcgs_step = [&](int j, bool tail, hipStream_t st) -> hipError_t {
+    T32 *vj = V + (int64_t)j * n;
+    if(!tail) {
+      const hipError_t e = spmv32<C>(sys->spmv_lanes, n, sys->d_ai, sys->d_aj, a32, vj, w, st);
+      if(e != hipSuccess) return e;
+    }
+    const unsigned nb2 = dots2_blocks(n);
+    hipLaunchKernelGGL(k_dots2<T32>, dim3(nb2), dim3(256), 0, st, n, j + 1, V, n, vj, tail ? vj : w, sys->d_sol_part);
+    hipLaunchKernelGGL(k_dots2_final, dim3((unsigned)(2 * (j + 1))), dim3(kRedThreads), 0, st, sys->d_sol_part,
+                       (int)nb2, dcD);
+    hipLaunchKernelGGL(k_dcgs_coef, dim3(1), dim3(64), 0, st, m, j, tail ? 1 : 0, dcD, dc, stt);
+    if(!tail)
+      hipLaunchKernelGGL(k_dcgs_update<T32>, dim3(vb), dim3(256), 0, st, n, j, V, n, w, dcS, dcC, dcR);
+    return hipGetLastError();
+  };
   // One Krylov iteration j on stream st: w = A v_j, two Gram-Schmidt passes,
   // |w|, v_{j+1}, Givens rotation.  Every argument is fixed for the system's
   // workspace, restart length and SpMV lanes, so a block of iterations can be
   // captured once and replayed.
   auto iteration = [&](int j, hipStream_t st) -> hipError_t {
+    if(dcgs) return dcgs_step(j, false, st);
     Gmres<C> Gs{sys, st, n, m};
     T32 *vj = V + (int64_t)j * n;
     hipError_t e = spmv32<C>(sys->spmv_lanes, n, sys->d_ai, sys->d_aj, a32, vj, w, st);
@@ -566,7 +786,8 @@ static gfem_status gmres_ir(gfem_system *sys, int rhs_idx, double tol, int m, in
   // capture falls back to the direct launches.
   const char *gs_env = std::getenv("GFEM_SOLVER_GRAPHS");
   bool use_graphs = !(gs_env && gs_env[0] == '0');
-  if(use_graphs && (sys->sol_graphs.empty() || sys->sol_graph_m != m || sys->sol_graph_lanes != sys->spmv_lanes)) {
+  if(use_graphs && (sys->sol_graphs.empty() || sys->sol_graph_m != m || sys->sol_graph_lanes != sys->spmv_lanes ||
+                    sys->sol_graph_dcgs != dcgs)) {
     for(hipGraphExec_t g : sys->sol_graphs) (void)hipGraphExecDestroy(g);
     sys->sol_graphs.clear();
     hipError_t e = sys->sol_cap ? hipSuccess : hipStreamCreateWithFlags(&sys->sol_cap, hipStreamNonBlocking);
@@ -592,6 +813,7 @@ static gfem_status gmres_ir(gfem_system *sys, int rhs_idx, double tol, int m, in
     else {
       sys->sol_graph_m = m;
       sys->sol_graph_lanes = sys->spmv_lanes;
+      sys->sol_graph_dcgs = dcgs;
     }
   }
   // ||b||
@@ -637,7 +859,9 @@ static gfem_status gmres_ir(gfem_system *sys, int rhs_idx, double tol, int m, in
         ++iters;
         ++j;
       }
-      k = j;
+      if(dcgs && j == m) GFEM_HIP(ctx, dcgs_step(m, true, s));  // the cycle's last column
+      // finished columns: delayed CGS2 finishes column j-1 at step j
+      k = dcgs ? (j == m ? m : j - 1) : j;
       if(j % kSolGraphBlock == 0 || j == m) {
         GFEM_HIP(ctx, hipMemcpyAsync(hest, est, sizeof(double2), hipMemcpyDeviceToHost, s));
         GFEM_HIP(ctx, hipStreamSynchronize(s));
