@@ -27,6 +27,7 @@
 #include <vector>
 
 #include "internal.h"
+#include "krylov.h"
 
 namespace gfem {
 
@@ -53,11 +54,18 @@ struct BItem {
   double2 *part;     // (m+2) rows of kPartMax partial sums
   const double2 *dinv;  // right preconditioner: inverse diagonal of A (ones without it)
   int nparts;        // reduction blocks of this request (a function of n only)
+  double2 *dc;       // delayed CGS2 state (krylov.h layout, its dot products at the end)
+  double2 *part2;    // 2(m+1) rows of nparts2 partial sums of the delayed-CGS2 sweep
+  int nparts2;       // blocks of kD2Span rows of this request
 };
 
 // rows per reduction block of a request of n rows: kDotSpan, or more so that
 // a request has at most kPartMax blocks
 inline int64_t dot_span(int64_t n) { return std::max<int64_t>(kDotSpan, (n + kPartMax - 1) / kPartMax); }
+
+// the delayed-CGS2 sweep: one block per kD2Span rows of a request
+constexpr int kD2Span = 256;
+inline int64_t d2_parts(int64_t n) { return std::max<int64_t>(1, (n + kD2Span - 1) / kD2Span); }
 
 __device__ __forceinline__ double2 zmul(double2 a, double2 b)
 {
@@ -266,6 +274,114 @@ __global__ void __launch_bounds__(256) bk_orth(const BItem *__restrict__ items, 
     s.y -= p.y;
   }
   I.w[i] = make_float2((float)s.x, (float)s.y);
+}
+
+// Delayed CGS2 (krylov.h), one global reduction per Krylov step.
+// Block partials of conj(V_k) . u and conj(V_k) . w, k <= j, u = V_j, over
+// the block's kD2Span rows of its request (map: slot, part, first row, end
+// row); the four waves take the basis vectors round robin, u and w stay in
+// L1.  tail: w = u (the cycle's last column needs V_k^H u only).
+__global__ void __launch_bounds__(256) bk_dots2(const BItem *__restrict__ items, const int *__restrict__ act,
+                                                const int4 *__restrict__ d2blk, int j, const int *__restrict__ dj,
+                                                int tail)
+{
+  if(dj) j = *dj;
+  const int4 db = d2blk[blockIdx.x];
+  if(!act[db.x]) return;
+  const BItem &I = items[db.x];
+  const float2 *u = I.V + (int64_t)j * I.n;
+  const float2 *w = tail ? u : I.w;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for(int k = wv; k <= j; k += 4) {
+    const float2 *vk = I.V + (int64_t)k * I.n;
+    double2 au = make_double2(0., 0.), aw = make_double2(0., 0.);
+#pragma unroll 4
+    for(int64_t i = (int64_t)db.z + lane; i < (int64_t)db.w; i += 64) {
+      const double2 v = f2d(vk[i]);
+      const double2 pa = zconjmul(v, f2d(u[i])), pb = zconjmul(v, f2d(w[i]));
+      au.x += pa.x;
+      au.y += pa.y;
+      aw.x += pb.x;
+      aw.y += pb.y;
+    }
+#pragma unroll
+    for(int d = 32; d >= 1; d >>= 1) {
+      au.x += __shfl_xor(au.x, d, 64);
+      au.y += __shfl_xor(au.y, d, 64);
+      aw.x += __shfl_xor(aw.x, d, 64);
+      aw.y += __shfl_xor(aw.y, d, 64);
+    }
+    if(lane == 0) {
+      I.part2[(int64_t)(2 * k) * I.nparts2 + db.y] = au;
+      I.part2[(int64_t)(2 * k + 1) * I.nparts2 + db.y] = aw;
+    }
+  }
+}
+
+// the sweep's 2(j+1) sums of each request in a fixed order; grid (2(m+1), slots)
+__global__ void __launch_bounds__(64) bk_dots2_final(const BItem *__restrict__ items, const int *__restrict__ act,
+                                                     int m, int j, const int *__restrict__ dj)
+{
+  if(dj) j = *dj;
+  const int it = blockIdx.y, o = blockIdx.x;
+  if(!act[it] || o >= 2 * (j + 1)) return;
+  const BItem &I = items[it];
+  double2 s = make_double2(0., 0.);
+  for(int p = threadIdx.x; p < I.nparts2; p += 64) {
+    const double2 v = I.part2[(int64_t)o * I.nparts2 + p];
+    s.x += v.x;
+    s.y += v.y;
+  }
+#pragma unroll
+  for(int d = 32; d >= 1; d >>= 1) {
+    s.x += __shfl_xor(s.x, d, 64);
+    s.y += __shfl_xor(s.y, d, 64);
+  }
+  if(threadIdx.x == 0) I.dc[krylov::dcgs_dots_offset(m) + o] = s;
+}
+
+// the bookkeeping of step j per request (krylov::dcgs_coef): columns finished
+// -> kcnt; grid (slots), 64 threads
+__global__ void __launch_bounds__(64) bk_dcgs_coef(const BItem *__restrict__ items, const int *__restrict__ act,
+                                                   int *__restrict__ kcnt, int m, int j, const int *__restrict__ dj,
+                                                   int tail)
+{
+  if(dj) j = *dj;
+  const int it = blockIdx.x;
+  if(!act[it]) return;
+  const BItem &I = items[it];
+  krylov::dcgs_coef(m, j, tail, I.dc, I.st);
+  if(threadIdx.x == 0) kcnt[it] = tail ? m : j;
+}
+
+// q_j = (u - sum_{k<j} V_k s_k) / r into V_j, u_{j+1} = (w - sum_{k<j} V_k c_k
+// - q_j c_j) / r into V_{j+1}: one read of the basis (map: slot, first row)
+__global__ void __launch_bounds__(256) bk_dcgs_update(const BItem *__restrict__ items, const int *__restrict__ act,
+                                                      const int2 *__restrict__ vblk, int m, int j,
+                                                      const int *__restrict__ dj)
+{
+  if(dj) j = *dj;
+  const int2 bb = vblk[blockIdx.x];
+  if(!act[bb.x]) return;
+  const BItem &I = items[bb.x];
+  const int64_t i = (int64_t)bb.y + threadIdx.x;
+  if(i >= I.n) return;
+  const double2 *sv = I.dc + krylov::dcgs_s_offset(m), *cv = I.dc + krylov::dcgs_c_offset(m);
+  const double inv = I.dc[krylov::dcgs_r_offset(m)].x;
+  double2 q = f2d(I.V[(int64_t)j * I.n + i]), un = f2d(I.w[i]);
+  for(int k = 0; k < j; ++k) {
+    const double2 vk = f2d(I.V[(int64_t)k * I.n + i]);
+    const double2 a = zmul(vk, sv[k]), b = zmul(vk, cv[k]);
+    q.x -= a.x;
+    q.y -= a.y;
+    un.x -= b.x;
+    un.y -= b.y;
+  }
+  q = make_double2(q.x * inv, q.y * inv);
+  const double2 qc = zmul(q, cv[j]);
+  un = make_double2((un.x - qc.x) * inv, (un.y - qc.y) * inv);
+  I.V[(int64_t)j * I.n + i] = make_float2((float)q.x, (float)q.y);
+  I.V[(int64_t)(j + 1) * I.n + i] = make_float2((float)un.x, (float)un.y);
 }
 
 // H column j += y (second Gram-Schmidt pass), grid (slots)
@@ -533,7 +649,8 @@ size_t request_bytes(const gfem_system *sys, int m)
   const size_t n = (size_t)sys->n, nnz = (size_t)std::max<int64_t>(sys->nnz, 1);
   const size_t st = (size_t)(m + 1) * m + 3 * (size_t)m + 2 + (size_t)(m + 1) + 2;
   return align256(16 * nnz) + align256(8 * nnz) + align256(16 * n * (size_t)sys->nrhs) + 3 * align256(16 * n) +
-         align256(8 * n) + align256(8 * n * (size_t)(m + 1)) + align256(16 * st) + align256(16 * (size_t)(m + 2) * kPartMax);
+         align256(8 * n) + align256(8 * n * (size_t)(m + 1)) + align256(16 * st) + align256(16 * (size_t)(m + 2) * kPartMax) +
+         align256(16 * krylov::dcgs_size(m)) + align256(16 * 2 * (size_t)(m + 1) * (size_t)d2_parts(sys->n));
 }
 
 }  // namespace
@@ -561,8 +678,8 @@ struct gfem_batch {
   BItem *d_items = nullptr;
   int *d_act = nullptr, *d_live = nullptr, *d_kcnt = nullptr;
   int2 *d_rblk = nullptr, *d_vblk = nullptr;
-  int4 *d_dblk = nullptr;
-  int64_t cap_rblk = 0, cap_vblk = 0, cap_dblk = 0;
+  int4 *d_dblk = nullptr, *d_d2blk = nullptr;
+  int64_t cap_rblk = 0, cap_vblk = 0, cap_dblk = 0, cap_d2blk = 0;
   double *d_scal = nullptr;
   int *h_flags = nullptr;   // pinned: act | live
   double *h_scal = nullptr; // pinned: 3 per slot
@@ -570,11 +687,12 @@ struct gfem_batch {
   std::vector<int64_t> done;          // completed, not yet fetched
   std::map<int64_t, int> where;       // id -> slot
   bool maps_dirty = true;
-  int64_t n_rblk = 0, n_vblk = 0, n_dblk = 0;
+  int64_t n_rblk = 0, n_vblk = 0, n_dblk = 0, n_d2blk = 0;
   // Krylov iterations as HIP graphs (1 and kCheck iterations), captured on
   // cap_stream with the Krylov index read from d_j; valid until the block
   // maps change.  GFEM_BATCH_GRAPH=0 launches the kernels one by one.
   bool use_graph = true;
+  bool dcgs = true;    // delayed CGS2 (GFEM_BATCH_CGS2=1: classical CGS2)
   bool jacobi = true;  // right scaling by row norms (GFEM_KRYLOV_SCALE=0: none)
   int *d_j = nullptr;
   hipStream_t cap_stream = nullptr;
@@ -594,7 +712,7 @@ void drop_graphs(gfem_batch *B)
 gfem_status build_maps(gfem_batch *B, hipStream_t s)
 {
   std::vector<int2> rb, vb;
-  std::vector<int4> db;
+  std::vector<int4> db, d2;
   for(size_t i = 0; i < B->slots.size(); ++i) {
     const auto &S = B->slots[i];
     if(!S.live) continue;
@@ -603,11 +721,20 @@ gfem_status build_maps(gfem_batch *B, hipStream_t s)
     const int64_t span = dot_span(S.item.n);
     for(int64_t r = 0, p = 0; r < S.item.n; r += span, ++p)
       db.push_back(make_int4((int)i, (int)p, (int)r, (int)std::min<int64_t>(S.item.n, r + span)));
+    for(int64_t r = 0, p = 0; r < S.item.n; r += kD2Span, ++p)
+      d2.push_back(make_int4((int)i, (int)p, (int)r, (int)std::min<int64_t>(S.item.n, r + kD2Span)));
   }
   if(rb.empty()) rb.push_back(make_int2(0, 0));
   if(vb.empty()) vb.push_back(make_int2(0, 0));
   if(db.empty()) db.push_back(make_int4(0, 0, 0, 0));
+  if(d2.empty()) d2.push_back(make_int4(0, 0, 0, 0));
   gfem_ctx *ctx = B->ctx;
+  if((int64_t)d2.size() > B->cap_d2blk) {
+    dfree(B->d_d2blk);
+    B->cap_d2blk = (int64_t)d2.size() * 2;
+    GFEM_HIP(ctx, dalloc(&B->d_d2blk, (size_t)B->cap_d2blk));
+  }
+  GFEM_HIP(ctx, hipMemcpyAsync(B->d_d2blk, d2.data(), sizeof(int4) * d2.size(), hipMemcpyHostToDevice, s));
   if((int64_t)db.size() > B->cap_dblk) {
     dfree(B->d_dblk);
     B->cap_dblk = (int64_t)db.size() * 2;
@@ -630,6 +757,7 @@ gfem_status build_maps(gfem_batch *B, hipStream_t s)
   B->n_rblk = (int64_t)rb.size();
   B->n_vblk = (int64_t)vb.size();
   B->n_dblk = (int64_t)db.size();
+  B->n_d2blk = (int64_t)d2.size();
   B->maps_dirty = false;
   drop_graphs(B);
   return GFEM_OK;
@@ -653,8 +781,32 @@ gfem_status push_flags(gfem_batch *B, const std::vector<int> &act, hipStream_t s
 // One Krylov iteration j of every active request: w = A V_j, two
 // Gram-Schmidt passes, |w|, V_{j+1}, Givens rotation.  With dj the index is
 // read from device memory (captured graphs), else j is used.
+// Delayed CGS2 step j of every active request (krylov.h): w = A V_j, one
+// sweep for the dot products, their sums, the bookkeeping (column j-1 and
+// its rotation), one sweep writing q_j and u_{j+1}.  tail (j = m): column m-1
+// only, without a product.
+void launch_dcgs(gfem_batch *B, hipStream_t s, int j, const int *dj, bool tail)
+{
+  const int ns = (int)B->slots.size(), m = B->m;
+  if(!tail)
+    hipLaunchKernelGGL(bk_spmv32, dim3((unsigned)B->n_rblk), dim3(256), 0, s, B->d_items, B->d_act, B->d_rblk, j, dj);
+  hipLaunchKernelGGL(bk_dots2, dim3((unsigned)B->n_d2blk), dim3(256), 0, s, B->d_items, B->d_act, B->d_d2blk, j, dj,
+                     tail ? 1 : 0);
+  hipLaunchKernelGGL(bk_dots2_final, dim3((unsigned)(2 * (m + 1)), (unsigned)ns), dim3(64), 0, s, B->d_items, B->d_act,
+                     m, j, dj);
+  hipLaunchKernelGGL(bk_dcgs_coef, dim3((unsigned)ns), dim3(64), 0, s, B->d_items, B->d_act, B->d_kcnt, m, j, dj,
+                     tail ? 1 : 0);
+  if(!tail)
+    hipLaunchKernelGGL(bk_dcgs_update, dim3((unsigned)B->n_vblk), dim3(256), 0, s, B->d_items, B->d_act, B->d_vblk, m,
+                       j, dj);
+}
+
 void launch_iteration(gfem_batch *B, hipStream_t s, int j, const int *dj)
 {
+  if(B->dcgs) {
+    launch_dcgs(B, s, j, dj, false);
+    return;
+  }
   const int ns = (int)B->slots.size(), m = B->m;
   const dim3 red((unsigned)B->n_dblk), fin((unsigned)(m + 2), (unsigned)ns);
   hipLaunchKernelGGL(bk_spmv32, dim3((unsigned)B->n_rblk), dim3(256), 0, s, B->d_items, B->d_act, B->d_rblk, j, dj);
@@ -783,6 +935,8 @@ gfem_status gfem_batch_create(gfem_ctx *ctx, int restart, size_t hbm_bytes, int 
   if(ok) ok = hipMemset(B->d_items, 0, sizeof(BItem) * (size_t)max_items) == hipSuccess;
   const char *ge = std::getenv("GFEM_BATCH_GRAPH");
   B->use_graph = !(ge && ge[0] == '0');
+  const char *ce = std::getenv("GFEM_BATCH_CGS2");
+  B->dcgs = !(ce && ce[0] == '1');
   const char *je = std::getenv("GFEM_KRYLOV_SCALE");
   B->jacobi = !(je && je[0] == '0');
   if(!ok) {
@@ -806,6 +960,7 @@ gfem_status gfem_batch_destroy(gfem_batch *B)
   dfree(B->d_rblk);
   dfree(B->d_vblk);
   dfree(B->d_dblk);
+  dfree(B->d_d2blk);
   drop_graphs(B);
   dfree(B->d_j);
   if(B->cap_stream) (void)hipStreamDestroy(B->cap_stream);
@@ -873,6 +1028,9 @@ gfem_status gfem_batch_submit(gfem_batch *B, gfem_system *sys, double tol, int m
   I.st = reinterpret_cast<double2 *>(take(16 * ((size_t)(m + 1) * m + 3 * (size_t)m + 2 + (size_t)(m + 1) + 2)));
   I.part = reinterpret_cast<double2 *>(take(16 * (size_t)(m + 2) * kPartMax));
   I.nparts = (int)((sys->n + dot_span(sys->n) - 1) / dot_span(sys->n));
+  I.dc = reinterpret_cast<double2 *>(take(16 * krylov::dcgs_size(m)));
+  I.nparts2 = (int)d2_parts(sys->n);
+  I.part2 = reinterpret_cast<double2 *>(take(16 * 2 * (size_t)(m + 1) * (size_t)I.nparts2));
   B->where[S.id] = slot;
   *id = S.id;
   // assemble this request's system with the system's current parameters
@@ -977,6 +1135,15 @@ gfem_status gfem_batch_step(gfem_batch *B, void *stream, int *n_live)
         st = push_flags(B, act, s);
         if(st != GFEM_OK) return st;
       }
+    }
+  }
+  if(B->dcgs) {
+    // requests that ran the whole cycle: its last column
+    int running = 0;
+    for(int i = 0; i < ns; ++i) running += act[i];
+    if(running > 0) {
+      launch_dcgs(B, s, m, nullptr, true);
+      GFEM_HIP(ctx, hipGetLastError());
     }
   }
   for(int i = 0; i < ns; ++i)

@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "internal.h"
+#include "krylov.h"
 
 namespace gfem {
 
@@ -443,137 +444,19 @@ __global__ void k_add_into(int k, const double2 *__restrict__ y, double2 *__rest
 // Column j holds h_{0..j,j} (from the dots) and h_{j+1,j}^2 in slot j+1
 // (squared norm from the norm pass).  Applies the previous rotations, makes
 // rotation j, updates g and the residual estimate |g_{j+1}|.
-// rotations of column j whose entries 0..j+1 are in place (h_{j+1,j} real)
-__device__ void givens_column(int m, int j, double2 *__restrict__ stt)
-{
-  double2 *H = stt;
-  double2 *cs = stt + (int64_t)(m + 1) * m;
-  double2 *sn = cs + m;
-  double2 *g = sn + m;
-  double2 *est = g + m + 1;
-  double2 *col = H + (int64_t)j * (m + 1);
-  for(int i = 0; i < j; ++i) {
-    // [c s; -conj(s) c] with real c
-    const double c = cs[i].x;
-    const double2 s = sn[i];
-    const double2 a = col[i], b = col[i + 1];
-    const double2 sb = zmul(s, b);
-    const double2 csa = zconjmul(s, a);  // conj(s) a
-    col[i] = make_double2(c * a.x + sb.x, c * a.y + sb.y);
-    col[i + 1] = make_double2(-csa.x + c * b.x, -csa.y + c * b.y);
-  }
-  const double2 a = col[j], b = col[j + 1];
-  const double na = hypot(a.x, a.y), nb = hypot(b.x, b.y);
-  double c;
-  double2 s;
-  if(nb == 0.) {
-    c = 1.;
-    s = make_double2(0., 0.);
-  }
-  else if(na == 0.) {
-    c = 0.;
-    s = make_double2(b.x / nb, -b.y / nb);  // conj(b)/|b|
-    // s . b = |b|
-  }
-  else {
-    const double t = hypot(na, nb);
-    c = na / t;
-    // s = (a/|a|) conj(b) / t
-    const double2 ua = make_double2(a.x / na, a.y / na);
-    s = make_double2((ua.x * b.x + ua.y * b.y) / t, (ua.y * b.x - ua.x * b.y) / t);
-  }
-  cs[j] = make_double2(c, 0.);
-  sn[j] = s;
-  const double2 sb = zmul(s, b);
-  col[j] = make_double2(c * a.x + sb.x, c * a.y + sb.y);
-  col[j + 1] = make_double2(0., 0.);
-  const double2 gj = g[j];
-  const double2 csg = zconjmul(s, gj);
-  g[j] = make_double2(c * gj.x, c * gj.y);
-  g[j + 1] = make_double2(-csg.x, -csg.y);
-  est->x = hypot(g[j + 1].x, g[j + 1].y);
-}
-
 __global__ void k_givens(int m, int j, double2 *__restrict__ stt)
 {
   if(threadIdx.x != 0 || blockIdx.x != 0) return;
   double2 *col = stt + (int64_t)j * (m + 1);
   col[j + 1] = make_double2(sqrt(col[j + 1].x), 0.);
-  givens_column(m, j, stt);
+  krylov::givens_column(m, j, stt);
 }
 
-// Delayed CGS2 bookkeeping of step j (one wavefront).  D[2k] = V_k^H u,
-// D[2k+1] = V_k^H v for k <= j, with u = V_j and v = A u.  dc = Hu[(m+1) x m]
-// (the unrotated Hessenberg, A Q_{j-1} = Q_j Hu) | h1[m+1] (first-pass
-// column) | s[m] | c[m+1] | 1/r.
-//   s = Q_{j-1}^H u, r = sqrt(|u|^2 - |s|^2) (Pythagoras: u - Q s is
-//   orthogonal to Q), column j-1 = h1 + s with r below it, then rotated;
-//   c = Q_j^H v = [z; (u^H v - s^H z) / r], h1 (column j, first pass) =
-//   (c - Hu s) / r.  tail: the cycle's last column only.
-// Sums over k are lane-strided then combined in a fixed butterfly.
-__global__ void __launch_bounds__(64) k_dcgs_coef(int m, int j, int tail, const double2 *__restrict__ D,
-                                                  double2 *__restrict__ dc, double2 *__restrict__ stt)
+// Delayed CGS2 bookkeeping of step j (krylov.h), one wavefront
+__global__ void __launch_bounds__(64) k_dcgs_coef(int m, int j, int tail, double2 *__restrict__ dc,
+                                                  double2 *__restrict__ stt)
 {
-  const int t = threadIdx.x;
-  double2 *Hu = dc;
-  double2 *h1 = Hu + (int64_t)(m + 1) * m;
-  double2 *sv = h1 + m + 1;
-  double2 *cv = sv + m;
-  double2 *ir = cv + m + 1;
-  double ss = 0.;
-  for(int k = t; k < j; k += 64) {
-    const double2 d = D[2 * k];
-    sv[k] = d;
-    ss += d.x * d.x + d.y * d.y;
-  }
-#pragma unroll
-  for(int d = 32; d >= 1; d >>= 1) ss += __shfl_xor(ss, d, 64);
-  const double r2 = D[2 * j].x - ss;
-  const double r = r2 > 0. ? sqrt(r2) : 0.;
-  const double inv = r > 0. ? 1. / r : 0.;
-  if(j >= 1) {
-    double2 *hu = Hu + (int64_t)(j - 1) * (m + 1);
-    double2 *col = stt + (int64_t)(j - 1) * (m + 1);
-    for(int i = t; i < j; i += 64) {  // h1[i], sv[i] of this lane
-      hu[i] = make_double2(h1[i].x + sv[i].x, h1[i].y + sv[i].y);
-      col[i] = hu[i];
-    }
-    if(t == 0) {
-      hu[j] = make_double2(r, 0.);
-      col[j] = hu[j];
-    }
-    __syncthreads();
-    if(t == 0) givens_column(m, j - 1, stt);
-  }
-  if(tail) return;
-  double2 sz = make_double2(0., 0.);
-  for(int k = t; k < j; k += 64) {
-    const double2 z = D[2 * k + 1];
-    cv[k] = z;
-    const double2 p = zconjmul(sv[k], z);
-    sz.x += p.x;
-    sz.y += p.y;
-  }
-#pragma unroll
-  for(int d = 32; d >= 1; d >>= 1) {
-    sz.x += __shfl_xor(sz.x, d, 64);
-    sz.y += __shfl_xor(sz.y, d, 64);
-  }
-  if(t == 0) {
-    const double2 beta = D[2 * j + 1];
-    cv[j] = make_double2((beta.x - sz.x) * inv, (beta.y - sz.y) * inv);
-    ir[0] = make_double2(inv, 0.);
-  }
-  __syncthreads();  // sv, cv and column j-1 of Hu seen by every lane
-  for(int i = t; i <= j; i += 64) {
-    double2 a = cv[i];
-    for(int l = (i > 0 ? i - 1 : 0); l < j; ++l) {  // Hu column l has rows 0..l+1
-      const double2 p = zmul(Hu[(int64_t)l * (m + 1) + i], sv[l]);
-      a.x -= p.x;
-      a.y -= p.y;
-    }
-    h1[i] = make_double2(a.x * inv, a.y * inv);
-  }
+  krylov::dcgs_coef(m, j, tail, dc, stt);
 }
 
 // back substitution R y = g over the first k columns (one thread); y -> out
@@ -626,10 +509,6 @@ struct Gmres {
 
 }  // namespace
 
-// delayed CGS2 state (k_dcgs_coef layout) followed by the step's 2(m+1) dot products
-static size_t dcgs_dots_offset(int m) { return (size_t)(m + 1) * m + (size_t)(m + 1) + m + (size_t)(m + 1) + 1; }
-static size_t dcgs_size(int m) { return dcgs_dots_offset(m) + 2 * (size_t)(m + 1); }
-
 template <bool C>
 static gfem_status gmres_ir(gfem_system *sys, int rhs_idx, double tol, int m, int max_iter, hipStream_t s,
                             gfem_solve_info *info)
@@ -655,7 +534,7 @@ static gfem_status gmres_ir(gfem_system *sys, int rhs_idx, double tol, int m, in
     if(!chk(hipMalloc(&sys->d_sol_part, sizeof(double2) * (size_t)std::max<int64_t>(kRedBlocks, dots2_blocks(n)) *
                                           (size_t)(2 * m + 4))))
       return fail(ctx, GFEM_ERR_OUT_OF_MEMORY, "solver reduction allocation");
-    if(!chk(hipMalloc(&sys->d_sol_dcgs, sizeof(double2) * dcgs_size(m))))
+    if(!chk(hipMalloc(&sys->d_sol_dcgs, sizeof(double2) * krylov::dcgs_size(m))))
       return fail(ctx, GFEM_ERR_OUT_OF_MEMORY, "solver state allocation");
     if(!chk(hipMalloc(&sys->d_sol_state, sizeof(double2) * ((size_t)(m + 1) * m + 3 * (size_t)m + 2))))
       return fail(ctx, GFEM_ERR_OUT_OF_MEMORY, "solver state allocation");
@@ -736,8 +615,8 @@ static gfem_status gmres_ir(gfem_system *sys, int rhs_idx, double tol, int m, in
   const char *cg_env = std::getenv("GFEM_SOLVER_CGS2");
   const bool dcgs = !(cg_env && cg_env[0] == '1');
   double2 *dc = sys->d_sol_dcgs;
-  double2 *dcD = dc + dcgs_dots_offset(m);
-  double2 *dcS = dc + (int64_t)(m + 1) * m + (m + 1), *dcC = dcS + m, *dcR = dcC + m + 1;
+  double2 *dcD = dc + krylov::dcgs_dots_offset(m);
+  double2 *dcS = dc + krylov::dcgs_s_offset(m), *dcC = dc + krylov::dcgs_c_offset(m), *dcR = dc + krylov::dcgs_r_offset(m);
   // Delayed CGS2 step j: V_j holds u_j (once orthogonalised; q_0 at j = 0).
   // w = A u_j; one sweep for Q_{j-1}^H [u_j, w] and u_j^H [u_j, w]; the
   // bookkeeping finishes column j-1 (rotated: the estimate lags one step)
@@ -753,7 +632,7 @@ static gfem_status gmres_ir(gfem_system *sys, int rhs_idx, double tol, int m, in
     hipLaunchKernelGGL(k_dots2<T32>, dim3(nb2), dim3(256), 0, st, n, j + 1, V, n, vj, tail ? vj : w, sys->d_sol_part);
     hipLaunchKernelGGL(k_dots2_final, dim3((unsigned)(2 * (j + 1))), dim3(kRedThreads), 0, st, sys->d_sol_part,
                        (int)nb2, dcD);
-    hipLaunchKernelGGL(k_dcgs_coef, dim3(1), dim3(64), 0, st, m, j, tail ? 1 : 0, dcD, dc, stt);
+    hipLaunchKernelGGL(k_dcgs_coef, dim3(1), dim3(64), 0, st, m, j, tail ? 1 : 0, dc, stt);
     if(!tail)
       hipLaunchKernelGGL(k_dcgs_update<T32>, dim3(vb), dim3(256), 0, st, n, j, V, n, w, dcS, dcC, dcR);
     return hipGetLastError();
