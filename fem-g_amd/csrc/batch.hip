@@ -34,6 +34,7 @@ namespace gfem {
 namespace {
 
 constexpr int kLanes = 8;                       // lanes per CSR row
+constexpr int kXcds = 8;                        // XCDs of an MI355X
 constexpr int kRowsPerBlock = 256 / kLanes;      // 32 rows per 256-thread block
 constexpr int kElemsPerBlock = 256;              // vector kernels: one element per thread
 constexpr int kDotSpan = 1024;                   // rows per reduction block (at least)
@@ -100,10 +101,16 @@ __device__ __forceinline__ double2 lanes_sum(double2 v)
 
 // w = A32 V_j (FP32 products, lanes combined in a fixed butterfly)
 __global__ void __launch_bounds__(256) bk_spmv32(const BItem *__restrict__ items, const int *__restrict__ act,
-                                                 const int2 *__restrict__ rblk, int j, const int *__restrict__ dj)
+                                                 const int2 *__restrict__ rblk, int nrblk, int j,
+                                                 const int *__restrict__ dj)
 {
   if(dj) j = *dj;
-  const int2 bb = rblk[blockIdx.x];
+  // each XCD takes a contiguous run of the row blocks (the grid is a
+  // multiple of kXcds): its L2 holds one segment of each request's x
+  const int per = (int)(gridDim.x / kXcds);
+  const int b = (int)(blockIdx.x % kXcds) * per + (int)(blockIdx.x / kXcds);
+  if(b >= nrblk) return;
+  const int2 bb = rblk[b];
   if(!act[bb.x]) return;
   const BItem &I = items[bb.x];
   const int64_t row = (int64_t)bb.y + threadIdx.x / kLanes;
@@ -789,7 +796,8 @@ void launch_dcgs(gfem_batch *B, hipStream_t s, int j, const int *dj, bool tail)
 {
   const int ns = (int)B->slots.size(), m = B->m;
   if(!tail)
-    hipLaunchKernelGGL(bk_spmv32, dim3((unsigned)B->n_rblk), dim3(256), 0, s, B->d_items, B->d_act, B->d_rblk, j, dj);
+    hipLaunchKernelGGL(bk_spmv32, dim3((unsigned)((B->n_rblk + kXcds - 1) / kXcds * kXcds)), dim3(256), 0, s, B->d_items,
+                       B->d_act, B->d_rblk, (int)B->n_rblk, j, dj);
   hipLaunchKernelGGL(bk_dots2, dim3((unsigned)B->n_d2blk), dim3(256), 0, s, B->d_items, B->d_act, B->d_d2blk, j, dj,
                      tail ? 1 : 0);
   hipLaunchKernelGGL(bk_dots2_final, dim3((unsigned)(2 * (m + 1)), (unsigned)ns), dim3(64), 0, s, B->d_items, B->d_act,
@@ -809,7 +817,8 @@ void launch_iteration(gfem_batch *B, hipStream_t s, int j, const int *dj)
   }
   const int ns = (int)B->slots.size(), m = B->m;
   const dim3 red((unsigned)B->n_dblk), fin((unsigned)(m + 2), (unsigned)ns);
-  hipLaunchKernelGGL(bk_spmv32, dim3((unsigned)B->n_rblk), dim3(256), 0, s, B->d_items, B->d_act, B->d_rblk, j, dj);
+  hipLaunchKernelGGL(bk_spmv32, dim3((unsigned)((B->n_rblk + kXcds - 1) / kXcds * kXcds)), dim3(256), 0, s, B->d_items,
+                       B->d_act, B->d_rblk, (int)B->n_rblk, j, dj);
   for(int pass = 0; pass < 2; ++pass) {
     hipLaunchKernelGGL(bk_dots, red, dim3(256), 0, s, B->d_items, B->d_act, B->d_dblk, 0, j, dj);
     hipLaunchKernelGGL(bk_dots_final, fin, dim3(64), 0, s, B->d_items, B->d_act, m, 0, j, pass, dj);

@@ -29,6 +29,7 @@ constexpr int kRowLanes = 8;       // lanes per CSR row in the SpMV kernels
 constexpr int kRedBlocks = 512;    // partial sums per reduction
 constexpr int kRedThreads = 256;
 constexpr int kDotGroup = 8;       // basis vectors per pass of the dot kernel
+constexpr int kSpmvXcds = 8;       // XCDs of an MI355X: the SpMV gives each a contiguous run of rows
 
 template <bool C>
 struct Sc;
@@ -82,7 +83,12 @@ __global__ void __launch_bounds__(256) k_spmv32(int64_t n, const int64_t *__rest
                                                 const typename Sc<C>::f32 *__restrict__ x,
                                                 typename Sc<C>::f32 *__restrict__ y)
 {
-  const int64_t row = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / L;
+  // each XCD takes a contiguous run of blocks: its rows are one segment of
+  // the (Hilbert-ordered) unknowns, so its L2 holds that segment of x and a
+  // halo instead of all of x (the grid is a multiple of kSpmvXcds)
+  const int64_t per = gridDim.x / kSpmvXcds;
+  const int64_t blk = (int64_t)(blockIdx.x % kSpmvXcds) * per + blockIdx.x / kSpmvXcds;
+  const int64_t row = (blk * blockDim.x + threadIdx.x) / L;
   const int l = (int)(threadIdx.x % L);
   if(row >= n) return;  // whole lane groups leave together
   float sr = 0.f, si = 0.f;
@@ -116,7 +122,7 @@ template <bool C>
 hipError_t spmv32(int lanes, int64_t n, const int64_t *ai, const int32_t *aj, const typename Sc<C>::f32 *a,
                   const typename Sc<C>::f32 *x, typename Sc<C>::f32 *y, hipStream_t s)
 {
-  const unsigned nb = (unsigned)((n * lanes + 255) / 256);
+  const unsigned nb = (unsigned)(((n * lanes + 255) / 256 + kSpmvXcds - 1) / kSpmvXcds * kSpmvXcds);
   if(lanes == 4)
     hipLaunchKernelGGL((k_spmv32<C, 4>), dim3(nb), dim3(256), 0, s, n, ai, aj, a, x, y);
   else if(lanes == 16)
