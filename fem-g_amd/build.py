@@ -48,7 +48,7 @@ def build(verbose=False):
 
     dev_objs = [os.path.join(OBJ, "kernels.o"), os.path.join(OBJ, "gfem_api.o"), os.path.join(OBJ, "jit.o"),
                 os.path.join(OBJ, "solver.o"), os.path.join(OBJ, "batch.o"), os.path.join(OBJ, "setup.o"),
-                os.path.join(OBJ, "elemmm.o")]
+                os.path.join(OBJ, "elemmm.o"), os.path.join(OBJ, "mg.o")]
     host_srcs = ["model.cpp", "msh.cpp", "gmshfem.cpp", "serve.cpp"]
     host_objs = [os.path.join(OBJ, s.replace(".cpp", ".o")) for s in host_srcs]
     py_obj = os.path.join(OBJ, "pybind.o")
@@ -61,6 +61,7 @@ def build(verbose=False):
         [HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "batch.hip"), "-o", dev_objs[4]],
         [HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "setup.hip"), "-o", dev_objs[5]],
         [HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "elemmm.hip"), "-o", dev_objs[6]],
+        [HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, "mg.hip"), "-o", dev_objs[7]],
     ]
     jobs += [["g++", *CXX_FLAGS, "-c", os.path.join(HOST, s), "-o", o] for s, o in zip(host_srcs, host_objs)]
     jobs.append(["g++", *CXX_FLAGS, "-fvisibility=hidden", "-I", pybind11.get_include(), "-I", py_inc,

@@ -28,6 +28,7 @@
 
 #include "internal.h"
 #include "krylov.h"
+#include "mg.h"
 
 namespace gfem {
 
@@ -58,6 +59,11 @@ struct BItem {
   double2 *dc;       // delayed CGS2 state (krylov.h layout, its dot products at the end)
   double2 *part2;    // 2(m+1) rows of nparts2 partial sums of the delayed-CGS2 sweep
   int nparts2;       // blocks of kD2Span rows of this request
+  // shifted-Laplacian multigrid (mg.h), or null: the Krylov vector goes
+  // through one V-cycle into z before the product, and a cycle's correction
+  // V y (in u) through one more
+  const MgReq *mg;
+  float2 *z, *u;
 };
 
 // rows per reduction block of a request of n rows: kDotSpan, or more so that
@@ -116,7 +122,7 @@ __global__ void __launch_bounds__(256) bk_spmv32(const BItem *__restrict__ items
   const int64_t row = (int64_t)bb.y + threadIdx.x / kLanes;
   const int l = threadIdx.x % kLanes;
   if(row >= I.n) return;
-  const float2 *x = I.V + (int64_t)j * I.n;
+  const float2 *x = I.mg ? I.z : I.V + (int64_t)j * I.n;
   float sr = 0.f, si = 0.f;
   // kU terms of the lane at a time (columns and values, then the x entries,
   // in flight together), summed in term order
@@ -527,8 +533,238 @@ __global__ void __launch_bounds__(256) bk_update_x(const BItem *__restrict__ ite
     s.x += p.x;
     s.y += p.y;
   }
+  if(I.mg) {  // u = V y, then x += M^-1 u (bk_mg_addz)
+    I.u[i] = make_float2((float)s.x, (float)s.y);
+    return;
+  }
   const double2 d = zmul(I.dinv[i], s);  // x += D^-1 V y (right preconditioning)
   I.x[i] = make_double2(I.x[i].x + d.x, I.x[i].y + d.y);
+}
+
+// ------------------------------------------------------------ multigrid V-cycle
+// One V-cycle of every active request with a hierarchy (mg.h): levels below
+// lsmall row-parallel over per-level block maps (slot, first row), the rest
+// of the cycle in one block per request.  Level 0's right-hand side is the
+// Krylov vector V_j (src 0) or the cycle's correction u (src 1); its result
+// goes to z.  The same operations in the same order whatever else is in the
+// batch.
+
+__device__ __forceinline__ float2 cmulf(float2 a, float2 b)
+{
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+__device__ __forceinline__ const float2 *mg_rhs(const BItem &I, int l, int src, int j)
+{
+  return l > 0 ? I.mg->lev[l].b : (src ? I.u : I.V + (int64_t)j * I.n);
+}
+
+__device__ __forceinline__ int64_t mg_row0(const MgLevelReq &L, int64_t r)
+{
+  return L.ai64 ? L.ai64[r] : (int64_t)L.ai[r];
+}
+
+// sum over row r of S times v (v = D b when db: the pre-smoothed iterate
+// computed on the fly), lanes of `width` terms apart starting at `lane`
+__device__ __forceinline__ float2 mg_row_dot(const MgLevelReq &L, int64_t r, const float2 *v, bool db, int lane,
+                                             int width)
+{
+  float sr = 0.f, si = 0.f;
+  const int64_t k1 = mg_row0(L, r + 1);
+  for(int64_t k = mg_row0(L, r) + lane; k < k1; k += width) {
+    const int32_t c = L.aj[k];
+    const float2 x = db ? cmulf(L.dinv[c], v[c]) : v[c];
+    const float2 p = cmulf(L.a[k], x);
+    sr += p.x;
+    si += p.y;
+  }
+  return make_float2(sr, si);
+}
+
+__device__ __forceinline__ float2 lanes_sumf(float2 v)
+{
+  const double2 d = lanes_sum(make_double2(v.x, v.y));
+  return make_float2((float)d.x, (float)d.y);
+}
+
+// x = D b, r = b - S x
+__global__ void __launch_bounds__(256) bk_mg_down(const BItem *__restrict__ items, const int *__restrict__ act,
+                                                  const int2 *__restrict__ blk, int l, int src, int j,
+                                                  const int *__restrict__ dj)
+{
+  if(dj) j = *dj;
+  const int2 bb = blk[blockIdx.x];
+  if(!act[bb.x]) return;
+  const BItem &I = items[bb.x];
+  const MgLevelReq &L = I.mg->lev[l];
+  const int64_t row = (int64_t)bb.y + threadIdx.x / kLanes;
+  const int lane = threadIdx.x % kLanes;
+  if(row >= L.n) return;
+  const float2 *b = mg_rhs(I, l, src, j);
+  const float2 s = lanes_sumf(mg_row_dot(L, row, b, true, lane, kLanes));
+  if(lane == 0) {
+    const float2 bi = b[row];
+    L.x[row] = cmulf(L.dinv[row], bi);
+    L.r[row] = make_float2(bi.x - s.x, bi.y - s.y);
+  }
+}
+
+// b_{l+1} = P^T r_l, rows of level l + 1
+__global__ void __launch_bounds__(256) bk_mg_restrict(const BItem *__restrict__ items, const int *__restrict__ act,
+                                                      const int2 *__restrict__ blk, int l)
+{
+  const int2 bb = blk[blockIdx.x];
+  if(!act[bb.x]) return;
+  const BItem &I = items[bb.x];
+  const MgLevelReq &L = I.mg->lev[l], &C = I.mg->lev[l + 1];
+  const int64_t row = (int64_t)bb.y + threadIdx.x / kLanes;
+  const int lane = threadIdx.x % kLanes;
+  if(row >= C.n) return;
+  float sr = 0.f, si = 0.f;
+  for(int32_t k = L.ri[row] + lane; k < L.ri[row + 1]; k += kLanes) {
+    const float w = L.rw[k];
+    const float2 v = L.r[L.rj[k]];
+    sr += w * v.x;
+    si += w * v.y;
+  }
+  const float2 s = lanes_sumf(make_float2(sr, si));
+  if(lane == 0) C.b[row] = s;
+}
+
+__device__ __forceinline__ const float2 *mg_coarse_x(const MgReq &M, int l)
+{
+  return l + 1 == M.nlev - 1 ? M.lev[l + 1].x : M.lev[l + 1].y;
+}
+
+// x_l += P xhat_{l+1}
+__global__ void __launch_bounds__(256) bk_mg_prolong(const BItem *__restrict__ items, const int *__restrict__ act,
+                                                     const int2 *__restrict__ blk, int l)
+{
+  const int2 bb = blk[blockIdx.x];
+  if(!act[bb.x]) return;
+  const BItem &I = items[bb.x];
+  const MgLevelReq &L = I.mg->lev[l];
+  const int64_t row = (int64_t)bb.y + threadIdx.x / kLanes;
+  const int lane = threadIdx.x % kLanes;
+  if(row >= L.n) return;
+  const float2 *xc = mg_coarse_x(*I.mg, l);
+  float sr = 0.f, si = 0.f;
+  for(int32_t k = L.pi[row] + lane; k < L.pi[row + 1]; k += kLanes) {
+    const float w = L.pw[k];
+    const float2 v = xc[L.pj[k]];
+    sr += w * v.x;
+    si += w * v.y;
+  }
+  const float2 s = lanes_sumf(make_float2(sr, si));
+  if(lane == 0) L.x[row] = make_float2(L.x[row].x + s.x, L.x[row].y + s.y);
+}
+
+// y_l = x_l + D (b_l - S x_l); level 0 writes z
+__global__ void __launch_bounds__(256) bk_mg_post(const BItem *__restrict__ items, const int *__restrict__ act,
+                                                  const int2 *__restrict__ blk, int l, int src, int j,
+                                                  const int *__restrict__ dj)
+{
+  if(dj) j = *dj;
+  const int2 bb = blk[blockIdx.x];
+  if(!act[bb.x]) return;
+  const BItem &I = items[bb.x];
+  const MgLevelReq &L = I.mg->lev[l];
+  const int64_t row = (int64_t)bb.y + threadIdx.x / kLanes;
+  const int lane = threadIdx.x % kLanes;
+  if(row >= L.n) return;
+  const float2 *b = mg_rhs(I, l, src, j);
+  const float2 s = lanes_sumf(mg_row_dot(L, row, L.x, false, lane, kLanes));
+  if(lane == 0) {
+    const float2 bi = b[row], xi = L.x[row];
+    const float2 d = cmulf(L.dinv[row], make_float2(bi.x - s.x, bi.y - s.y));
+    float2 *out = l == 0 ? I.z : L.y;
+    out[row] = make_float2(xi.x + d.x, xi.y + d.y);
+  }
+}
+
+// the levels from lsmall down, the coarsest exactly, and back up, in one
+// block per request (a thread per row, the same sums as the row-parallel
+// kernels take in one lane)
+__global__ void __launch_bounds__(256) bk_mg_small(const BItem *__restrict__ items, const int *__restrict__ act,
+                                                   int src, int j, const int *__restrict__ dj)
+{
+  if(dj) j = *dj;
+  const int it = blockIdx.x;
+  if(!act[it] || !items[it].mg) return;
+  const BItem &I = items[it];
+  const MgReq &M = *I.mg;
+  const int t = threadIdx.x, nt = blockDim.x;
+  for(int l = M.lsmall; l < M.nlev - 1; ++l) {
+    const MgLevelReq &L = M.lev[l], &C = M.lev[l + 1];
+    const float2 *b = mg_rhs(I, l, src, j);
+    for(int64_t r = t; r < L.n; r += nt) {
+      const float2 s = mg_row_dot(L, r, b, true, 0, 1);
+      L.x[r] = cmulf(L.dinv[r], b[r]);
+      L.r[r] = make_float2(b[r].x - s.x, b[r].y - s.y);
+    }
+    __syncthreads();
+    for(int64_t r = t; r < C.n; r += nt) {
+      float sr = 0.f, si = 0.f;
+      for(int32_t k = L.ri[r]; k < L.ri[r + 1]; ++k) {
+        const float w = L.rw[k];
+        const float2 v = L.r[L.rj[k]];
+        sr += w * v.x;
+        si += w * v.y;
+      }
+      C.b[r] = make_float2(sr, si);
+    }
+    __syncthreads();
+  }
+  {
+    const MgLevelReq &C = M.lev[M.nlev - 1];
+    const int n = C.n;
+    for(int r = t; r < n; r += nt) {
+      float sr = 0.f, si = 0.f;
+      for(int c = 0; c < n; ++c) {
+        const float2 p = cmulf(M.ainv[(int64_t)r * n + c], C.b[c]);
+        sr += p.x;
+        si += p.y;
+      }
+      C.x[r] = make_float2(sr, si);
+    }
+    __syncthreads();
+  }
+  for(int l = M.nlev - 2; l >= M.lsmall; --l) {
+    const MgLevelReq &L = M.lev[l];
+    const float2 *xc = mg_coarse_x(M, l);
+    for(int64_t r = t; r < L.n; r += nt) {
+      float sr = 0.f, si = 0.f;
+      for(int32_t k = L.pi[r]; k < L.pi[r + 1]; ++k) {
+        const float w = L.pw[k];
+        const float2 v = xc[L.pj[k]];
+        sr += w * v.x;
+        si += w * v.y;
+      }
+      L.x[r] = make_float2(L.x[r].x + sr, L.x[r].y + si);
+    }
+    __syncthreads();
+    const float2 *b = mg_rhs(I, l, src, j);
+    float2 *out = l == 0 ? I.z : L.y;
+    for(int64_t r = t; r < L.n; r += nt) {
+      const float2 s = mg_row_dot(L, r, L.x, false, 0, 1);
+      const float2 d = cmulf(L.dinv[r], make_float2(b[r].x - s.x, b[r].y - s.y));
+      out[r] = make_float2(L.x[r].x + d.x, L.x[r].y + d.y);
+    }
+    __syncthreads();
+  }
+}
+
+// x += z (the V-cycle of the cycle's correction)
+__global__ void __launch_bounds__(256) bk_mg_addz(const BItem *__restrict__ items, const int *__restrict__ live,
+                                                  const int2 *__restrict__ vblk)
+{
+  const int2 bb = vblk[blockIdx.x];
+  if(!live[bb.x]) return;
+  const BItem &I = items[bb.x];
+  if(!I.mg) return;
+  const int64_t i = (int64_t)bb.y + threadIdx.x;
+  if(i >= I.n) return;
+  I.x[i] = make_double2(I.x[i].x + (double)I.z[i].x, I.x[i].y + (double)I.z[i].y);
 }
 
 // per-slot scalars to one array: est, |r|^2, |b|^2
@@ -655,9 +891,12 @@ size_t request_bytes(const gfem_system *sys, int m)
 {
   const size_t n = (size_t)sys->n, nnz = (size_t)std::max<int64_t>(sys->nnz, 1);
   const size_t st = (size_t)(m + 1) * m + 3 * (size_t)m + 2 + (size_t)(m + 1) + 2;
-  return align256(16 * nnz) + align256(8 * nnz) + align256(16 * n * (size_t)sys->nrhs) + 3 * align256(16 * n) +
-         align256(8 * n) + align256(8 * n * (size_t)(m + 1)) + align256(16 * st) + align256(16 * (size_t)(m + 2) * kPartMax) +
-         align256(16 * krylov::dcgs_size(m)) + align256(16 * 2 * (size_t)(m + 1) * (size_t)d2_parts(sys->n));
+  size_t b = align256(16 * nnz) + align256(8 * nnz) + align256(16 * n * (size_t)sys->nrhs) + 3 * align256(16 * n) +
+             align256(8 * n) + align256(8 * n * (size_t)(m + 1)) + align256(16 * st) +
+             align256(16 * (size_t)(m + 2) * kPartMax) + align256(16 * krylov::dcgs_size(m)) +
+             align256(16 * 2 * (size_t)(m + 1) * (size_t)d2_parts(sys->n));
+  if(sys->mg) b += 2 * align256(8 * n) + mg_request_bytes(sys);  // z, u, the cycle's state
+  return b;
 }
 
 }  // namespace
@@ -704,6 +943,15 @@ struct gfem_batch {
   int *d_j = nullptr;
   hipStream_t cap_stream = nullptr;
   hipGraphExec_t g1 = nullptr, g5 = nullptr;
+  // multigrid: per level l, blocks over its rows (d_mgblk) and over the rows
+  // of level l + 1 (d_mgcblk) of the live requests that run level l row-
+  // parallel (l < lsmall); mg_big = the most such levels, n_mg = live
+  // requests with a hierarchy
+  int2 *d_mgblk[kMgMaxLevels] = {}, *d_mgcblk[kMgMaxLevels] = {};
+  int64_t n_mgblk[kMgMaxLevels] = {}, n_mgcblk[kMgMaxLevels] = {};
+  int64_t cap_mgblk[kMgMaxLevels] = {}, cap_mgcblk[kMgMaxLevels] = {};
+  int mg_big = 0, n_mg = 0;
+  std::vector<MgReq> mg_host;  // per slot: the request's cycle layout
 };
 
 namespace {
@@ -760,6 +1008,36 @@ gfem_status build_maps(gfem_batch *B, hipStream_t s)
   }
   GFEM_HIP(ctx, hipMemcpyAsync(B->d_rblk, rb.data(), sizeof(int2) * rb.size(), hipMemcpyHostToDevice, s));
   GFEM_HIP(ctx, hipMemcpyAsync(B->d_vblk, vb.data(), sizeof(int2) * vb.size(), hipMemcpyHostToDevice, s));
+  // multigrid level maps
+  std::vector<int2> mb[kMgMaxLevels], mc[kMgMaxLevels];
+  B->mg_big = 0;
+  B->n_mg = 0;
+  for(size_t i = 0; i < B->slots.size(); ++i) {
+    const auto &S = B->slots[i];
+    if(!S.live || !S.item.mg) continue;
+    const MgReq &M = B->mg_host[i];
+    ++B->n_mg;
+    B->mg_big = std::max(B->mg_big, M.lsmall);
+    for(int l = 0; l < M.lsmall; ++l) {
+      for(int r = 0; r < M.lev[l].n; r += kRowsPerBlock) mb[l].push_back(make_int2((int)i, r));
+      for(int r = 0; r < M.lev[l + 1].n; r += kRowsPerBlock) mc[l].push_back(make_int2((int)i, r));
+    }
+  }
+  for(int l = 0; l < kMgMaxLevels; ++l) {
+    for(int q = 0; q < 2; ++q) {
+      std::vector<int2> &v = q ? mc[l] : mb[l];
+      int2 *&d = q ? B->d_mgcblk[l] : B->d_mgblk[l];
+      int64_t &cap = q ? B->cap_mgcblk[l] : B->cap_mgblk[l];
+      (q ? B->n_mgcblk[l] : B->n_mgblk[l]) = (int64_t)v.size();
+      if(v.empty()) continue;
+      if((int64_t)v.size() > cap) {
+        dfree(d);
+        cap = (int64_t)v.size() * 2;
+        GFEM_HIP(ctx, dalloc(&d, (size_t)cap));
+      }
+      GFEM_HIP(ctx, hipMemcpyAsync(d, v.data(), sizeof(int2) * v.size(), hipMemcpyHostToDevice, s));
+    }
+  }
   GFEM_HIP(ctx, hipStreamSynchronize(s));  // the sources are local vectors
   B->n_rblk = (int64_t)rb.size();
   B->n_vblk = (int64_t)vb.size();
@@ -785,6 +1063,30 @@ gfem_status push_flags(gfem_batch *B, const std::vector<int> &act, hipStream_t s
   return GFEM_OK;
 }
 
+// One V-cycle of the requests with a hierarchy flagged in act: right-hand
+// side V_j (src 0, j or dj) or u (src 1), result in z
+void launch_mg(gfem_batch *B, hipStream_t s, const int *act, int src, int j, const int *dj)
+{
+  if(B->n_mg == 0) return;
+  const int ns = (int)B->slots.size();
+  for(int l = 0; l < B->mg_big; ++l) {
+    if(B->n_mgblk[l])
+      hipLaunchKernelGGL(bk_mg_down, dim3((unsigned)B->n_mgblk[l]), dim3(256), 0, s, B->d_items, act, B->d_mgblk[l], l,
+                         src, j, dj);
+    if(B->n_mgcblk[l])
+      hipLaunchKernelGGL(bk_mg_restrict, dim3((unsigned)B->n_mgcblk[l]), dim3(256), 0, s, B->d_items, act,
+                         B->d_mgcblk[l], l);
+  }
+  hipLaunchKernelGGL(bk_mg_small, dim3((unsigned)ns), dim3(256), 0, s, B->d_items, act, src, j, dj);
+  for(int l = B->mg_big - 1; l >= 0; --l) {
+    if(!B->n_mgblk[l]) continue;
+    hipLaunchKernelGGL(bk_mg_prolong, dim3((unsigned)B->n_mgblk[l]), dim3(256), 0, s, B->d_items, act, B->d_mgblk[l],
+                       l);
+    hipLaunchKernelGGL(bk_mg_post, dim3((unsigned)B->n_mgblk[l]), dim3(256), 0, s, B->d_items, act, B->d_mgblk[l], l,
+                       src, j, dj);
+  }
+}
+
 // One Krylov iteration j of every active request: w = A V_j, two
 // Gram-Schmidt passes, |w|, V_{j+1}, Givens rotation.  With dj the index is
 // read from device memory (captured graphs), else j is used.
@@ -795,6 +1097,7 @@ gfem_status push_flags(gfem_batch *B, const std::vector<int> &act, hipStream_t s
 void launch_dcgs(gfem_batch *B, hipStream_t s, int j, const int *dj, bool tail)
 {
   const int ns = (int)B->slots.size(), m = B->m;
+  if(!tail) launch_mg(B, s, B->d_act, 0, j, dj);
   if(!tail)
     hipLaunchKernelGGL(bk_spmv32, dim3((unsigned)((B->n_rblk + kXcds - 1) / kXcds * kXcds)), dim3(256), 0, s, B->d_items,
                        B->d_act, B->d_rblk, (int)B->n_rblk, j, dj);
@@ -817,6 +1120,7 @@ void launch_iteration(gfem_batch *B, hipStream_t s, int j, const int *dj)
   }
   const int ns = (int)B->slots.size(), m = B->m;
   const dim3 red((unsigned)B->n_dblk), fin((unsigned)(m + 2), (unsigned)ns);
+  launch_mg(B, s, B->d_act, 0, j, dj);
   hipLaunchKernelGGL(bk_spmv32, dim3((unsigned)((B->n_rblk + kXcds - 1) / kXcds * kXcds)), dim3(256), 0, s, B->d_items,
                        B->d_act, B->d_rblk, (int)B->n_rblk, j, dj);
   for(int pass = 0; pass < 2; ++pass) {
@@ -934,6 +1238,7 @@ gfem_status gfem_batch_create(gfem_ctx *ctx, int restart, size_t hbm_bytes, int 
   B->m = restart;
   B->max_items = max_items;
   B->slots.resize((size_t)max_items);
+  B->mg_host.resize((size_t)max_items);
   bool ok = B->arena.init(hbm_bytes) && dalloc(&B->d_items, (size_t)max_items) == hipSuccess &&
             dalloc(&B->d_act, (size_t)max_items) == hipSuccess && dalloc(&B->d_live, (size_t)max_items) == hipSuccess &&
             dalloc(&B->d_kcnt, (size_t)max_items) == hipSuccess && dalloc(&B->d_j, 1) == hipSuccess &&
@@ -970,6 +1275,10 @@ gfem_status gfem_batch_destroy(gfem_batch *B)
   dfree(B->d_vblk);
   dfree(B->d_dblk);
   dfree(B->d_d2blk);
+  for(int l = 0; l < kMgMaxLevels; ++l) {
+    dfree(B->d_mgblk[l]);
+    dfree(B->d_mgcblk[l]);
+  }
   drop_graphs(B);
   dfree(B->d_j);
   if(B->cap_stream) (void)hipStreamDestroy(B->cap_stream);
@@ -1040,6 +1349,15 @@ gfem_status gfem_batch_submit(gfem_batch *B, gfem_system *sys, double tol, int m
   I.dc = reinterpret_cast<double2 *>(take(16 * krylov::dcgs_size(m)));
   I.nparts2 = (int)d2_parts(sys->n);
   I.part2 = reinterpret_cast<double2 *>(take(16 * 2 * (size_t)(m + 1) * (size_t)I.nparts2));
+  I.mg = nullptr;
+  I.z = I.u = nullptr;
+  if(sys->mg) {
+    I.z = reinterpret_cast<float2 *>(take(8 * n));
+    I.u = reinterpret_cast<float2 *>(take(8 * n));
+    char *mb = take(mg_request_bytes(sys));
+    mg_bind(sys, mb, &B->mg_host[slot]);
+    I.mg = reinterpret_cast<const MgReq *>(mb);  // the descriptor sits at the start of its block
+  }
   B->where[S.id] = slot;
   *id = S.id;
   // assemble this request's system with the system's current parameters
@@ -1050,11 +1368,17 @@ gfem_status gfem_batch_submit(gfem_batch *B, gfem_system *sys, double tol, int m
     S = gfem_batch::Slot();
     return st;
   }
+  // right scaling by row norms, unless the multigrid cycle preconditions
   hipLaunchKernelGGL(bk_diag_inv, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, (int64_t)n, sys->d_ai, sys->d_aj,
-                     a64, B->jacobi ? 1 : 0, dinv);
+                     a64, B->jacobi && !sys->mg ? 1 : 0, dinv);
   hipLaunchKernelGGL(bk_round_cols, dim3((unsigned)((nnz + 255) / 256)), dim3(256), 0, s, (int64_t)sys->nnz, a64,
                      sys->d_aj, dinv, a32);
   hipLaunchKernelGGL(bk_zero, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, (int64_t)n, I.x);
+  if(sys->mg) {
+    // the cycle's descriptor and level operators, in stream order
+    hipLaunchKernelGGL(k_put<MgReq>, dim3(1), dim3(64), 0, s, const_cast<MgReq *>(I.mg), B->mg_host[slot]);
+    GFEM_HIP(ctx, mg_prepare(sys, B->mg_host[slot], a64, s));
+  }
   // this slot's descriptor only, in stream order (no wait)
   hipLaunchKernelGGL(k_put<BItem>, dim3(1), dim3(64), 0, s, B->d_items + slot, S.item);
   GFEM_HIP(ctx, hipGetLastError());
@@ -1160,6 +1484,10 @@ gfem_status gfem_batch_step(gfem_batch *B, void *stream, int *n_live)
   hipLaunchKernelGGL(bk_backsolve, dim3((unsigned)ns), dim3(64), 0, s, B->d_items, B->d_live, B->d_kcnt, m);
   hipLaunchKernelGGL(bk_update_x, dim3((unsigned)B->n_vblk), dim3(256), 0, s, B->d_items, B->d_live, B->d_kcnt,
                      B->d_vblk, m);
+  if(B->n_mg) {  // x += M^-1 V y
+    launch_mg(B, s, B->d_live, 1, 0, nullptr);
+    hipLaunchKernelGGL(bk_mg_addz, dim3((unsigned)B->n_vblk), dim3(256), 0, s, B->d_items, B->d_live, B->d_vblk);
+  }
   GFEM_HIP(ctx, hipGetLastError());
   for(int i = 0; i < ns; ++i) act[i] = B->slots[i].live ? 1 : 0;
   st = residual_and_prepare(B, act, s, false);

@@ -342,6 +342,8 @@ gfem_status gfem_system_destroy(gfem_system *sys)
   if(sys->h_sol_est) (void)hipHostFree(sys->h_sol_est);
   for(hipGraphExec_t g : sys->sol_graphs) (void)hipGraphExecDestroy(g);
   if(sys->sol_cap) (void)hipStreamDestroy(sys->sol_cap);
+  mg_free(sys->mg);
+  sys->mg = nullptr;
   dfree(sys->d_table);
   dfree(sys->d_table_dense);
   dfree(sys->d_rec);

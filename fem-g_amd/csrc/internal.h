@@ -33,6 +33,9 @@
 
 namespace gfem {
 
+struct MgSystem;  // multigrid hierarchy of a system (mg.h)
+void mg_free(MgSystem *m);
+
 constexpr int kMaxGroups = 16;
 constexpr int kSolGraphBlock = 5;  // Krylov iterations between convergence checks (one graph each)
 constexpr int kMaxN = 64;
@@ -306,6 +309,8 @@ struct gfem_system {
   int32_t *d_dc_fix = nullptr;
   int64_t *d_lin_ptr = nullptr, *d_dc_ptr = nullptr;
   int64_t n_lin = 0, n_dcw = 0;
+  // shifted-Laplacian multigrid hierarchy (gfem_mg_setup), or null
+  gfem::MgSystem *mg = nullptr;
   // outputs per precision (0 FP64, 1 MIXED)
   void *d_vals[2] = {nullptr, nullptr};
   void *d_rhs[2] = {nullptr, nullptr};

@@ -445,6 +445,23 @@ void Formulation<T>::setTermCoefficient(unsigned int tag, const function::Scalar
 }
 
 template <class T>
+std::vector<int> Formulation<T>::setShiftedLaplacian(unsigned int tag, double beta)
+{
+  if(!_sys) throw common::Exception("setShiftedLaplacian: the system exists after the first assemble()");
+  if(tag >= _terms.size() || !_terms[tag].bilinear)
+    throw common::Exception("setShiftedLaplacian: unknown bilinear term " + std::to_string(tag));
+  if(_termSlots[tag].size() != 1)
+    throw common::Exception("setShiftedLaplacian: the shifted term must sit on one element type and entity");
+  const int g = _termSlots[tag][0].first, t = _termSlots[tag][0].second;
+  gfem_ctx *ctx = common::deviceContext();
+  if(gfem_mg_setup(_sys, 1, &g, &t, beta) != GFEM_OK)
+    throw common::Exception(std::string("gfem_mg_setup: ") + gfem_ctx_last_error(ctx));
+  int levels = 0, rows[8] = {0};
+  gfem_mg_info(_sys, &levels, rows, nullptr);
+  return std::vector<int>(rows, rows + levels);
+}
+
+template <class T>
 void Formulation<T>::instanceParams(const std::vector<std::pair<unsigned int, function::ScalarFunction>> &params,
                                    std::vector<gfem_term_param> &out, std::vector<std::vector<double>> &store)
 {

@@ -419,6 +419,28 @@ gfem_status gfem_batch_done(gfem_batch *batch, int max, int64_t *ids, int *count
 /* Solution (n complex128, may be NULL) and info of an ended request; frees it. */
 gfem_status gfem_batch_result(gfem_batch *batch, int64_t id, void *x_host, size_t bytes, gfem_solve_info *info);
 
+/* Shifted-Laplacian multigrid preconditioner for the batched solves of a
+ * finalized complex system.  It has no counterpart in the reference, which
+ * factors A exactly (fem-master/src/system/Solver.cpp:106-115, MUMPS LU): it
+ * serves the iterative replacement of that solve.  One V-cycle of
+ *   S = A + i beta c_s X_s
+ * preconditions GMRES on the right, where (group, term) names the shifted
+ * constant bilinear term with coefficient c_s and unit matrix X_s.  For the
+ * mass term of a Helmholtz formulation S = K - (1 + i beta) k^2 M - i k R.
+ * The hierarchy is built once, on the host from the device maps:
+ *   - geometric levels on the DOF positions: trilinear interpolation from
+ *     uniform grids of twice, four times, ... the mean DOF spacing, down to
+ *     at most 6 points per axis;
+ *   - Galerkin products of every constant bilinear term's unit matrix.
+ * Each request combines the hierarchy with its own coefficients at
+ * submission.  Requests submitted afterwards use it; gfem_solve does not.
+ * Lagrange fields with constant bilinear coefficients only (else
+ * GFEM_ERR_UNSUPPORTED).  n_shift must be 1. */
+gfem_status gfem_mg_setup(gfem_system *sys, int n_shift, const int *shift_group, const int *shift_term, double beta);
+/* Levels of the hierarchy (0 without one), rows and stored entries per
+ * level (arrays of at least 8; the coarsest level is dense). */
+gfem_status gfem_mg_info(const gfem_system *sys, int *levels, int *rows, int64_t *nnz);
+
 /* Arithmetic of the element-matrix product on the matrix cores
  * (GFEM_STRATEGY_DENSE, and per-point
  * terms in the fused strategy) under GFEM_PREC_MIXED:
