@@ -361,6 +361,14 @@ def batched(args):
     sv = Server(restart=50, hbm_bytes=args.batch_hbm_gb << 30, max_requests=256)
     for m in keys:
         sv.add_mesh(Fs[m])
+    if args.batch_mg:
+        t1 = time.perf_counter()
+        for m in keys:
+            Fs[m].set_shifted_laplacian(W.HELMHOLTZ_MASS, args.serve_mg_beta)
+        res["multigrid_setup_s"] = time.perf_counter() - t1
+        log(f"batched: multigrid hierarchies of {len(keys)} meshes in {res['multigrid_setup_s']:.1f} s")
+    res["preconditioner"] = ("shifted-Laplacian multigrid V(1,1), one V-cycle per Krylov step" if args.batch_mg
+                             else "row-norm scaling")
     sched = [dict(mesh=mesh_ids[m], arrival=0.0, params=W.helmholtz_params(k, c), tol=args.batch_tol,
                   max_iter=60000) for m, k, c in inst]
     out = sv.run(sched)
@@ -463,6 +471,8 @@ def main():
     ap.add_argument("--serve-mg", type=int, choices=[0, 1], default=1,
                     help="shifted-Laplacian multigrid preconditioner on the serving meshes (1) or row scaling (0)")
     ap.add_argument("--serve-mg-beta", type=float, default=0.5)
+    ap.add_argument("--batch-mg", type=int, choices=[0, 1], default=1,
+                    help="shifted-Laplacian multigrid preconditioner on the uneven-instance batch")
     ap.add_argument("--gather-slots", type=int, choices=[2, 4, 8], default=4,
                     help="contributions per gather thread (4: 128-thread blocks, 2: 256-thread blocks)")
     ap.add_argument("--gather-wave", type=int, choices=[0, 1], default=1,

@@ -683,45 +683,55 @@ __global__ void __launch_bounds__(256) bk_mg_post(const BItem *__restrict__ item
 }
 
 // the levels from lsmall down, the coarsest exactly, and back up, in one
-// block per request (a thread per row, the same sums as the row-parallel
-// kernels take in one lane)
-__global__ void __launch_bounds__(256) bk_mg_small(const BItem *__restrict__ items, const int *__restrict__ act,
-                                                   int src, int j, const int *__restrict__ dj)
+// block per request: rows kLanes lanes each, as the row-parallel kernels
+// (the same sums in the same order), stages separated by block barriers
+__global__ void __launch_bounds__(kMgSmallThreads) bk_mg_small(const BItem *__restrict__ items,
+                                                              const int *__restrict__ act, int src, int j,
+                                                              const int *__restrict__ dj)
 {
   if(dj) j = *dj;
   const int it = blockIdx.x;
   if(!act[it] || !items[it].mg) return;
   const BItem &I = items[it];
   const MgReq &M = *I.mg;
-  const int t = threadIdx.x, nt = blockDim.x;
+  const int lane = threadIdx.x % kLanes;
+  const int r0 = threadIdx.x / kLanes;
+  constexpr int kRows = kMgSmallThreads / kLanes;  // rows per pass
   for(int l = M.lsmall; l < M.nlev - 1; ++l) {
     const MgLevelReq &L = M.lev[l], &C = M.lev[l + 1];
     const float2 *b = mg_rhs(I, l, src, j);
-    for(int64_t r = t; r < L.n; r += nt) {
-      const float2 s = mg_row_dot(L, r, b, true, 0, 1);
-      L.x[r] = cmulf(L.dinv[r], b[r]);
-      L.r[r] = make_float2(b[r].x - s.x, b[r].y - s.y);
+    for(int64_t base = 0; base < L.n; base += kRows) {
+      const int64_t r = base + r0;
+      const float2 s = lanes_sumf(r < L.n ? mg_row_dot(L, r, b, true, lane, kLanes) : make_float2(0.f, 0.f));
+      if(r < L.n && lane == 0) {
+        L.x[r] = cmulf(L.dinv[r], b[r]);
+        L.r[r] = make_float2(b[r].x - s.x, b[r].y - s.y);
+      }
     }
     __syncthreads();
-    for(int64_t r = t; r < C.n; r += nt) {
+    for(int64_t base = 0; base < C.n; base += kRows) {
+      const int64_t r = base + r0;
       float sr = 0.f, si = 0.f;
-      for(int32_t k = L.ri[r]; k < L.ri[r + 1]; ++k) {
-        const float w = L.rw[k];
-        const float2 v = L.r[L.rj[k]];
-        sr += w * v.x;
-        si += w * v.y;
-      }
-      C.b[r] = make_float2(sr, si);
+      if(r < C.n)
+        for(int32_t k = L.ri[r] + lane; k < L.ri[r + 1]; k += kLanes) {
+          const float w = L.rw[k];
+          const float2 v = L.r[L.rj[k]];
+          sr += w * v.x;
+          si += w * v.y;
+        }
+      const float2 s = lanes_sumf(make_float2(sr, si));
+      if(r < C.n && lane == 0) C.b[r] = s;
     }
     __syncthreads();
   }
   {
+    // coarsest: x = S^-1 b, the inverse column-major
     const MgLevelReq &C = M.lev[M.nlev - 1];
     const int n = C.n;
-    for(int r = t; r < n; r += nt) {
+    for(int r = threadIdx.x; r < n; r += kMgSmallThreads) {
       float sr = 0.f, si = 0.f;
       for(int c = 0; c < n; ++c) {
-        const float2 p = cmulf(M.ainv[(int64_t)r * n + c], C.b[c]);
+        const float2 p = cmulf(M.ainv[(int64_t)c * n + r], C.b[c]);
         sr += p.x;
         si += p.y;
       }
@@ -732,23 +742,29 @@ __global__ void __launch_bounds__(256) bk_mg_small(const BItem *__restrict__ ite
   for(int l = M.nlev - 2; l >= M.lsmall; --l) {
     const MgLevelReq &L = M.lev[l];
     const float2 *xc = mg_coarse_x(M, l);
-    for(int64_t r = t; r < L.n; r += nt) {
+    for(int64_t base = 0; base < L.n; base += kRows) {
+      const int64_t r = base + r0;
       float sr = 0.f, si = 0.f;
-      for(int32_t k = L.pi[r]; k < L.pi[r + 1]; ++k) {
-        const float w = L.pw[k];
-        const float2 v = xc[L.pj[k]];
-        sr += w * v.x;
-        si += w * v.y;
-      }
-      L.x[r] = make_float2(L.x[r].x + sr, L.x[r].y + si);
+      if(r < L.n)
+        for(int32_t k = L.pi[r] + lane; k < L.pi[r + 1]; k += kLanes) {
+          const float w = L.pw[k];
+          const float2 v = xc[L.pj[k]];
+          sr += w * v.x;
+          si += w * v.y;
+        }
+      const float2 s = lanes_sumf(make_float2(sr, si));
+      if(r < L.n && lane == 0) L.x[r] = make_float2(L.x[r].x + s.x, L.x[r].y + s.y);
     }
     __syncthreads();
     const float2 *b = mg_rhs(I, l, src, j);
     float2 *out = l == 0 ? I.z : L.y;
-    for(int64_t r = t; r < L.n; r += nt) {
-      const float2 s = mg_row_dot(L, r, L.x, false, 0, 1);
-      const float2 d = cmulf(L.dinv[r], make_float2(b[r].x - s.x, b[r].y - s.y));
-      out[r] = make_float2(L.x[r].x + d.x, L.x[r].y + d.y);
+    for(int64_t base = 0; base < L.n; base += kRows) {
+      const int64_t r = base + r0;
+      const float2 s = lanes_sumf(r < L.n ? mg_row_dot(L, r, L.x, false, lane, kLanes) : make_float2(0.f, 0.f));
+      if(r < L.n && lane == 0) {
+        const float2 d = cmulf(L.dinv[r], make_float2(b[r].x - s.x, b[r].y - s.y));
+        out[r] = make_float2(L.x[r].x + d.x, L.x[r].y + d.y);
+      }
     }
     __syncthreads();
   }
@@ -1077,7 +1093,7 @@ void launch_mg(gfem_batch *B, hipStream_t s, const int *act, int src, int j, con
       hipLaunchKernelGGL(bk_mg_restrict, dim3((unsigned)B->n_mgcblk[l]), dim3(256), 0, s, B->d_items, act,
                          B->d_mgcblk[l], l);
   }
-  hipLaunchKernelGGL(bk_mg_small, dim3((unsigned)ns), dim3(256), 0, s, B->d_items, act, src, j, dj);
+  hipLaunchKernelGGL(bk_mg_small, dim3((unsigned)ns), dim3(kMgSmallThreads), 0, s, B->d_items, act, src, j, dj);
   for(int l = B->mg_big - 1; l >= 0; --l) {
     if(!B->n_mgblk[l]) continue;
     hipLaunchKernelGGL(bk_mg_prolong, dim3((unsigned)B->n_mgblk[l]), dim3(256), 0, s, B->d_items, act, B->d_mgblk[l],

@@ -9,7 +9,8 @@
 //   level 0  the system's DOFs, S assembled per request from A and X_s;
 //   level 1  a uniform grid over the DOFs' bounding box with twice the mean DOF
 //            spacing, P_0 = trilinear interpolation of grid values at the DOF
-//            positions;
+//            positions snapped to the lattice of half that spacing (weights
+//            1, 1/2, 1/4, 1/8: short rows, small coarse stencils);
 //   level l+1  every other grid line of level l (and the last), P_l trilinear;
 //   coarsest  at most kMgCoarsest points per axis, solved exactly (dense
 //            inverse per request).
@@ -25,8 +26,9 @@ namespace gfem {
 
 constexpr int kMgMaxLevels = 8;  // levels including the fine one
 constexpr int kMgMaxTerms = 8;   // constant bilinear terms combined per level
-constexpr int kMgCoarsest = 6;   // grid points per axis of the coarsest level, at most
-constexpr int kMgSmall = 4096;   // levels of at most this many rows run inside one block per request
+constexpr int kMgCoarsest = 5;   // grid points per axis of the coarsest level, at most
+constexpr int kMgSmall = 512;    // levels of at most this many rows run inside one block per request
+constexpr int kMgSmallThreads = 1024;
 constexpr float kMgOmega = 0.5f;
 
 // One level of a request's cycle (device pointers; level 0: the system's
@@ -50,7 +52,7 @@ struct MgLevelReq {
 struct MgReq {
   int nlev;    // levels, the last one solved exactly
   int lsmall;  // first level run by the one-block kernel (levels lsmall .. nlev - 1)
-  const float2 *ainv;  // dense inverse of S on the coarsest level, [n][n]
+  const float2 *ainv;  // dense inverse of S on the coarsest level, column-major [n][n]
   MgLevelReq lev[kMgMaxLevels];
 };
 

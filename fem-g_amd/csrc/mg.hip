@@ -8,6 +8,7 @@
 #include <cmath>
 #include <complex>
 #include <cstring>
+#include <memory>
 #include <thread>
 #include <vector>
 
@@ -135,12 +136,18 @@ Csr transpose(const Csr &A)
   return T;
 }
 
-// a pattern with nt complex values per entry
+// complex FP32 value of the host products (the coarse operators are complex64)
+struct CF {
+  float re, im;
+};
+
+// a pattern with nt complex values per entry, [nnz][nt], left uninitialised
+// at allocation (the unit matrices of large meshes take gigabytes)
 struct MultiCsr {
   int64_t rows = 0;
   std::vector<int64_t> ptr;
   std::vector<int32_t> idx;
-  std::vector<std::complex<double>> v;  // [nnz][nt]
+  std::unique_ptr<CF[]> v;
 };
 
 // C = P^T X P for every value plane of X, rows of C in parallel; each row's
@@ -150,31 +157,35 @@ MultiCsr galerkin(const MultiCsr &X, const Csr &P, const Csr &Pt, int nt)
   const int64_t nc = P.cols;
   unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
   std::vector<std::vector<int32_t>> cols(nth);
-  std::vector<std::vector<std::complex<double>>> vals(nth);
+  std::vector<std::vector<CF>> vals(nth);
   std::vector<std::vector<int64_t>> rowlen(nth);
   std::vector<std::thread> th;
   auto work = [&](unsigned w) {
     const int64_t r0 = nc * w / nth, r1 = nc * (w + 1) / nth;
     std::vector<int64_t> mark((size_t)nc, -1);
-    std::vector<std::complex<double>> acc((size_t)nc * nt);
+    std::vector<CF> acc((size_t)nc * nt);
     std::vector<int32_t> list;
     for(int64_t I = r0; I < r1; ++I) {
       list.clear();
       for(int64_t a = Pt.ptr[I]; a < Pt.ptr[I + 1]; ++a) {
         const int64_t i = Pt.idx[a];
-        const double wi = Pt.w[a];
+        const float wi = (float)Pt.w[a];
         for(int64_t k = X.ptr[i]; k < X.ptr[i + 1]; ++k) {
           const int64_t j = X.idx[k];
-          const std::complex<double> *xv = &X.v[(size_t)k * nt];
+          const CF *xv = &X.v[(size_t)k * nt];
           for(int64_t b = P.ptr[j]; b < P.ptr[j + 1]; ++b) {
             const int32_t J = P.idx[b];
-            const double f = wi * P.w[b];
+            const float f = wi * (float)P.w[b];
+            CF *ac = &acc[(size_t)J * nt];
             if(mark[J] != I) {
               mark[J] = I;
               list.push_back(J);
-              for(int t = 0; t < nt; ++t) acc[(size_t)J * nt + t] = 0.;
+              for(int t = 0; t < nt; ++t) ac[t] = CF{0.f, 0.f};
             }
-            for(int t = 0; t < nt; ++t) acc[(size_t)J * nt + t] += f * xv[t];
+            for(int t = 0; t < nt; ++t) {
+              ac[t].re += f * xv[t].re;
+              ac[t].im += f * xv[t].im;
+            }
           }
         }
       }
@@ -192,13 +203,18 @@ MultiCsr galerkin(const MultiCsr &X, const Csr &P, const Csr &Pt, int nt)
   C.rows = nc;
   C.ptr.assign((size_t)nc + 1, 0);
   int64_t r = 0;
+  size_t nvals = 0;
+  for(unsigned w = 0; w < nth; ++w) nvals += vals[w].size();
+  C.v.reset(new CF[std::max<size_t>(nvals, 1)]);
+  size_t at = 0;
   for(unsigned w = 0; w < nth; ++w) {
     for(int64_t len : rowlen[w]) {
       C.ptr[r + 1] = C.ptr[r] + len;
       ++r;
     }
     C.idx.insert(C.idx.end(), cols[w].begin(), cols[w].end());
-    C.v.insert(C.v.end(), vals[w].begin(), vals[w].end());
+    std::copy(vals[w].begin(), vals[w].end(), C.v.get() + at);
+    at += vals[w].size();
   }
   return C;
 }
@@ -257,88 +273,98 @@ __global__ void k_mg_dinv(int64_t n, const P *__restrict__ ai, const int32_t *__
   dinv[r] = m > 0.f ? make_float2(kMgOmega * d.x / m, -kMgOmega * d.y / m) : make_float2(0.f, 0.f);
 }
 
-// Inverse of the coarsest level's S = sum_t c_t G_t (dense, n <= 216), one
-// block: Gauss-Jordan with partial pivoting in FP64 on w = [S | I], then the
-// right half rounded to complex64.
-__global__ void __launch_bounds__(256) k_mg_invert(int n, const float2 *__restrict__ gd, Coefs cf,
-                                                   double2 *__restrict__ w, float2 *__restrict__ ainv)
+// Inverse of the coarsest level's S = sum_t c_t G_t (dense, n <= kMgCoarsest^3
+// = 125), one block: in-place Gauss-Jordan with partial pivoting on the
+// complex64 matrix held in LDS (row swaps recorded, columns swapped back at
+// the end), written column-major so the cycle's coarse product reads it with
+// consecutive lanes on consecutive rows.
+constexpr int kMgInvMax = kMgCoarsest * kMgCoarsest * kMgCoarsest;
+constexpr int kMgInvThreads = 1024;
+
+__global__ void __launch_bounds__(kMgInvThreads) k_mg_invert(int n, const float2 *__restrict__ gd, Coefs cf,
+                                                             float2 *__restrict__ ainv)
 {
-  const int t = threadIdx.x, n2 = 2 * n;
-  __shared__ double s_val[256];
-  __shared__ int s_idx[256];
-  __shared__ int s_piv;
-  for(int e = t; e < n * n2; e += blockDim.x) {
-    const int r = e / n2, c = e % n2;
-    double2 v = make_double2(0., 0.);
-    if(c < n) {
-      for(int k = 0; k < cf.nt; ++k) {
-        const float2 g = gd[((int64_t)k * n + r) * n + c], q = cf.c[k];
-        v.x += (double)q.x * g.x - (double)q.y * g.y;
-        v.y += (double)q.x * g.y + (double)q.y * g.x;
-      }
+  __shared__ float2 A[kMgInvMax * kMgInvMax];
+  __shared__ int s_perm[kMgInvMax];
+  __shared__ float s_val[64];
+  __shared__ int s_idx[64];
+  const int t = threadIdx.x;
+  for(int e = t; e < n * n; e += kMgInvThreads) {
+    float re = 0.f, im = 0.f;
+    for(int k = 0; k < cf.nt; ++k) {
+      const float2 g = gd[(int64_t)k * n * n + e], q = cf.c[k];
+      re += q.x * g.x - q.y * g.y;
+      im += q.x * g.y + q.y * g.x;
     }
-    else if(c - n == r)
-      v.x = 1.;
-    w[e] = v;
+    A[e] = make_float2(re, im);
   }
   __syncthreads();
   for(int c = 0; c < n; ++c) {
-    // pivot row: largest |w[r][c]|, r >= c
-    double best = -1.;
-    int bi = c;
-    for(int r = c + t; r < n; r += blockDim.x) {
-      const double2 v = w[(int64_t)r * n2 + c];
-      const double m = v.x * v.x + v.y * v.y;
-      if(m > best) {
-        best = m;
-        bi = r;
+    // pivot: largest |A[r][c]|, r >= c (one wave, smallest index on ties)
+    if(t < 64) {
+      float best = -1.f;
+      int bi = c;
+      for(int r = c + t; r < n; r += 64) {
+        const float2 v = A[r * n + c];
+        const float m = v.x * v.x + v.y * v.y;
+        if(m > best) {
+          best = m;
+          bi = r;
+        }
       }
-    }
-    s_val[t] = best;
-    s_idx[t] = bi;
-    __syncthreads();
-    for(int d = blockDim.x / 2; d > 0; d >>= 1) {
-      if(t < d && (s_val[t + d] > s_val[t] || (s_val[t + d] == s_val[t] && s_idx[t + d] < s_idx[t]))) {
-        s_val[t] = s_val[t + d];
-        s_idx[t] = s_idx[t + d];
+      for(int d = 32; d > 0; d >>= 1) {
+        const float ob = __shfl_down(best, d, 64);
+        const int oi = __shfl_down(bi, d, 64);
+        if(t + d < 64 && (ob > best || (ob == best && oi < bi))) {
+          best = ob;
+          bi = oi;
+        }
       }
-      __syncthreads();
+      if(t == 0) s_perm[c] = bi;
     }
-    if(t == 0) s_piv = s_idx[0];
     __syncthreads();
-    const int p = s_piv;
+    const int p = s_perm[c];
     if(p != c)
-      for(int k = t; k < n2; k += blockDim.x) {
-        const double2 a = w[(int64_t)c * n2 + k];
-        w[(int64_t)c * n2 + k] = w[(int64_t)p * n2 + k];
-        w[(int64_t)p * n2 + k] = a;
+      for(int k = t; k < n; k += kMgInvThreads) {
+        const float2 x = A[c * n + k];
+        A[c * n + k] = A[p * n + k];
+        A[p * n + k] = x;
       }
     __syncthreads();
-    const double2 d = w[(int64_t)c * n2 + c];
-    const double dm = d.x * d.x + d.y * d.y;
-    const double2 inv = dm > 0. ? make_double2(d.x / dm, -d.y / dm) : make_double2(0., 0.);
+    const float2 d = A[c * n + c];
+    const float dm = d.x * d.x + d.y * d.y;
+    const float2 inv = dm > 0.f ? make_float2(d.x / dm, -d.y / dm) : make_float2(0.f, 0.f);
     __syncthreads();
-    for(int k = t; k < n2; k += blockDim.x) {
-      const double2 a = w[(int64_t)c * n2 + k];
-      w[(int64_t)c * n2 + k] = make_double2(a.x * inv.x - a.y * inv.y, a.x * inv.y + a.y * inv.x);
+    // pivot row: A[c][c] = 1, then scaled by 1 / d
+    for(int k = t; k < n; k += kMgInvThreads) {
+      const float2 x = k == c ? make_float2(1.f, 0.f) : A[c * n + k];
+      A[c * n + k] = make_float2(x.x * inv.x - x.y * inv.y, x.x * inv.y + x.y * inv.x);
     }
     __syncthreads();
-    for(int e = t; e < n * n2; e += blockDim.x) {
-      const int r = e / n2, k = e % n2;
+    // every other row: f = A[r][c], A[r][c] = 0, A[r][:] -= f A[c][:]
+    for(int e = t; e < n * n; e += kMgInvThreads) {
+      const int r = e / n, k = e - r * n;
       if(r == c) continue;
-      const double2 f = w[(int64_t)r * n2 + c], a = w[(int64_t)c * n2 + k];
-      if(k == c) continue;  // column c is cleared below, after every row has read it
-      w[e] = make_double2(w[e].x - (f.x * a.x - f.y * a.y), w[e].y - (f.x * a.y + f.y * a.x));
+      const float2 f = A[r * n + c], y = A[c * n + k];
+      const float2 x = k == c ? make_float2(0.f, 0.f) : A[e];
+      A[e] = make_float2(x.x - (f.x * y.x - f.y * y.y), x.y - (f.x * y.y + f.y * y.x));
     }
-    __syncthreads();
-    for(int r = t; r < n; r += blockDim.x)
-      if(r != c) w[(int64_t)r * n2 + c] = make_double2(0., 0.);
     __syncthreads();
   }
-  for(int e = t; e < n * n; e += blockDim.x) {
-    const int r = e / n, c = e % n;
-    const double2 v = w[(int64_t)r * n2 + n + c];
-    ainv[e] = make_float2((float)v.x, (float)v.y);
+  // undo the row swaps as column swaps, last first, and write column-major
+  for(int c = n - 1; c >= 0; --c) {
+    const int p = s_perm[c];
+    if(p != c)
+      for(int r = t; r < n; r += kMgInvThreads) {
+        const float2 x = A[r * n + c];
+        A[r * n + c] = A[r * n + p];
+        A[r * n + p] = x;
+      }
+    __syncthreads();
+  }
+  for(int e = t; e < n * n; e += kMgInvThreads) {
+    const int r = e / n, c = e - r * n;
+    ainv[(int64_t)c * n + r] = A[e];
   }
 }
 
@@ -376,7 +402,7 @@ size_t mg_request_bytes(const gfem_system *sys)
     if(l < m->nlev - 1) b += a256(8 * (size_t)m->nnz[l]);  // level values (level 0: S, the system pattern)
   }
   const size_t nc = (size_t)m->n[m->nlev - 1];
-  b += a256(8 * nc * nc) + a256(16 * nc * 2 * nc);  // inverse, Gauss-Jordan workspace
+  b += a256(8 * nc * nc);  // coarsest inverse, column-major
   return b;
 }
 
@@ -454,9 +480,8 @@ hipError_t mg_prepare(const gfem_system *sys, const MgReq &req, const double2 *a
                        const_cast<float2 *>(L.dinv));
   }
   const int nc = m->n[m->nlev - 1];
-  double2 *work = reinterpret_cast<double2 *>(reinterpret_cast<char *>(const_cast<float2 *>(req.ainv)) +
-                                              a256(8 * (size_t)nc * nc));
-  hipLaunchKernelGGL(k_mg_invert, dim3(1), dim3(256), 0, s, nc, m->gdense, cf, work, const_cast<float2 *>(req.ainv));
+  hipLaunchKernelGGL(k_mg_invert, dim3(1), dim3(kMgInvThreads), 0, s, nc, m->gdense, cf,
+                     const_cast<float2 *>(req.ainv));
   return hipGetLastError();
 }
 
@@ -498,6 +523,7 @@ extern "C" gfem_status gfem_mg_setup(gfem_system *sys, int n_shift, const int *s
   const int64_t n = sys->n, nnz = sys->nnz;
   if(n == 0) return fail(ctx, GFEM_ERR_INVALID_ARG, "mg_setup: no unknowns");
 
+  SetupTimer timer;
   // DOF positions
   std::vector<double> xyz((size_t)(3 * n), 0.);
   {
@@ -524,12 +550,13 @@ extern "C" gfem_status gfem_mg_setup(gfem_system *sys, int n_shift, const int *s
     if(e != hipSuccess) return hip_fail(ctx, e, "mg_setup: DOF positions");
   }
 
+  timer.mark("mg: DOF positions");
   // unit matrices of the terms: each assembled alone with coefficient 1
   MultiCsr X;
   X.rows = n;
   X.ptr.resize((size_t)n + 1);
   X.idx.resize((size_t)nnz);
-  X.v.assign((size_t)nnz * nt, 0.);
+  X.v.reset(new CF[(size_t)std::max<int64_t>(nnz, 1) * nt]);
   {
     std::vector<int64_t> ai((size_t)n + 1);
     GFEM_HIP(ctx, hipMemcpy(ai.data(), sys->d_ai, sizeof(int64_t) * ai.size(), hipMemcpyDeviceToHost));
@@ -540,21 +567,21 @@ extern "C" gfem_status gfem_mg_setup(gfem_system *sys, int n_shift, const int *s
       const DevTerm &d = sys->groups[terms[t].first].dterms[terms[t].second];
       saved[t] = {d.c_re, d.c_im};
     }
-    double2 *vals = nullptr, *rhs = nullptr;
+    double2 *vals = nullptr, *rhs = nullptr, *h = nullptr;  // h: pinned staging of one unit matrix
     GFEM_HIP(ctx, dalloc(&vals, (size_t)nnz));
     hipError_t e = dalloc(&rhs, (size_t)(n * sys->nrhs));
-    std::vector<double2> h((size_t)nnz);
+    if(e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void **>(&h), sizeof(double2) * (size_t)nnz);
     gfem_status st = e == hipSuccess ? GFEM_OK : hip_fail(ctx, e, "mg_setup: buffers");
     for(int t = 0; t < nt && st == GFEM_OK; ++t) {
       for(int u = 0; u < nt && st == GFEM_OK; ++u)
         st = gfem_term_update(sys, terms[u].first, terms[u].second, u == t ? 1. : 0., 0., 0, nullptr, nullptr);
       if(st == GFEM_OK) st = gfem_assemble_into(sys, GFEM_PREC_FP64, vals, rhs, nullptr);
       if(st == GFEM_OK) {
-        e = hipMemcpy(h.data(), vals, sizeof(double2) * (size_t)nnz, hipMemcpyDeviceToHost);
+        e = hipMemcpy(h, vals, sizeof(double2) * (size_t)nnz, hipMemcpyDeviceToHost);
         if(e != hipSuccess) st = hip_fail(ctx, e, "mg_setup: unit matrix");
       }
       if(st == GFEM_OK)
-        for(int64_t k = 0; k < nnz; ++k) X.v[(size_t)k * nt + t] = {h[k].x, h[k].y};
+        for(int64_t k = 0; k < nnz; ++k) X.v[(size_t)k * nt + t] = CF{(float)h[k].x, (float)h[k].y};
     }
     for(int u = 0; u < nt; ++u) {
       const gfem_status r = gfem_term_update(sys, terms[u].first, terms[u].second, saved[u].real(), saved[u].imag(), 0,
@@ -563,9 +590,11 @@ extern "C" gfem_status gfem_mg_setup(gfem_system *sys, int n_shift, const int *s
     }
     dfree(vals);
     dfree(rhs);
+    if(h) (void)hipHostFree(h);
     if(st != GFEM_OK) return st;
   }
 
+  timer.mark("mg: unit matrices");
   // grids: twice the mean DOF spacing, halved per level down to kMgCoarsest
   double lo[3], hi[3];
   for(int a = 0; a < 3; ++a) {
@@ -593,10 +622,22 @@ extern "C" gfem_status gfem_mg_setup(gfem_system *sys, int n_shift, const int *s
     grid[a].resize(G);
     for(int i = 0; i < G; ++i) grid[a][i] = G == 1 ? lo[a] : lo[a] + ext * i / (G - 1);
   }
+  // P_0 interpolates at the DOF positions snapped to the lattice of half the
+  // level-1 spacing: weights 1, 1/2, 1/4 or 1/8 (on average 3.4 per row
+  // instead of 8), which keeps P_0^T X P_0 and the level-1 stencil small
+  std::vector<double> snap(xyz);
+  for(int64_t i = 0; i < n; ++i)
+    for(int a = 0; a < 3; ++a) {
+      const int G = (int)grid[a].size();
+      if(G < 2) continue;
+      const double half = 0.5 * (grid[a][G - 1] - grid[a][0]) / (G - 1);
+      const double q = std::round((xyz[3 * i + a] - grid[a][0]) / half);
+      snap[3 * i + a] = std::min(std::max(grid[a][0] + q * half, grid[a][0]), grid[a][G - 1]);
+    }
   std::vector<Csr> Ps;
   std::vector<MultiCsr> Ls;
   Ls.push_back(std::move(X));
-  Ps.push_back(trilinear(xyz, grid));
+  Ps.push_back(trilinear(snap, grid));
   auto coarser = [](const std::vector<double> &g) {
     std::vector<double> c;
     for(size_t i = 0; i < g.size(); i += 2) c.push_back(g[i]);
@@ -623,9 +664,10 @@ extern "C" gfem_status gfem_mg_setup(gfem_system *sys, int n_shift, const int *s
     Ps.push_back(trilinear(pts, cg));
     for(int a = 0; a < 3; ++a) grid[a] = cg[a];
   }
+  timer.mark("mg: transfers and Galerkin products");
   const int nlev = (int)Ls.size();
   const int64_t ncoarse = Ls.back().rows;
-  if(ncoarse > 256) return fail(ctx, GFEM_ERR_UNSUPPORTED, "mg_setup: coarsest level too large");
+  if(ncoarse > kMgInvMax) return fail(ctx, GFEM_ERR_UNSUPPORTED, "mg_setup: coarsest level too large");
 
   MgSystem *m = new MgSystem();
   m->nlev = nlev;
@@ -649,7 +691,7 @@ extern "C" gfem_status gfem_mg_setup(gfem_system *sys, int n_shift, const int *s
       std::vector<float2> g((size_t)nt * L.idx.size());
       for(int t = 0; t < nt; ++t)
         for(size_t k = 0; k < L.idx.size(); ++k)
-          g[(size_t)t * L.idx.size() + k] = make_float2((float)L.v[k * nt + t].real(), (float)L.v[k * nt + t].imag());
+          g[(size_t)t * L.idx.size() + k] = make_float2(L.v[k * nt + t].re, L.v[k * nt + t].im);
       e = up(&m->ai[l], ai.data(), ai.size());
       if(e == hipSuccess) e = up(&m->aj[l], L.idx.data(), L.idx.size());
       if(e == hipSuccess) e = up(&m->g[l], g.data(), g.size());
@@ -676,7 +718,7 @@ extern "C" gfem_status gfem_mg_setup(gfem_system *sys, int n_shift, const int *s
       for(int64_t k = L.ptr[r]; k < L.ptr[r + 1]; ++k)
         for(int t = 0; t < nt; ++t)
           gd[((size_t)t * nc + r) * nc + L.idx[k]] =
-            make_float2((float)L.v[k * nt + t].real(), (float)L.v[k * nt + t].imag());
+            make_float2(L.v[k * nt + t].re, L.v[k * nt + t].im);
     e = up(&m->gdense, gd.data(), gd.size());
   }
   if(e == hipSuccess) {
@@ -684,7 +726,7 @@ extern "C" gfem_status gfem_mg_setup(gfem_system *sys, int n_shift, const int *s
     const MultiCsr &L = Ls[0];
     std::vector<float2> xs((size_t)nnz);
     for(int64_t k = 0; k < nnz; ++k)
-      xs[k] = make_float2((float)L.v[(size_t)k * nt + shift].real(), (float)L.v[(size_t)k * nt + shift].imag());
+      xs[k] = make_float2(L.v[(size_t)k * nt + shift].re, L.v[(size_t)k * nt + shift].im);
     e = up(&m->xs, xs.data(), xs.size());
   }
   if(e != hipSuccess) {
@@ -693,6 +735,7 @@ extern "C" gfem_status gfem_mg_setup(gfem_system *sys, int n_shift, const int *s
   }
   mg_free(sys->mg);
   sys->mg = m;
+  timer.mark("mg: upload");
   return GFEM_OK;
 }
 

@@ -26,10 +26,10 @@ def _server(meshes, restart=30, mg=True):
 
 def test_mg_hierarchy_levels():
     """rows per level: the DOFs, then uniform grids of twice the mean DOF
-    spacing halved per level, down to at most 6 points per axis"""
+    spacing halved per level, down to at most 5 points per axis"""
     print("case mg levels", flush=True)
     _, Fs, levels = _server([(8, 2), (5, 1)])
-    assert levels[0] == [17 ** 3, 10 ** 3, 6 ** 3], levels
+    assert levels[0] == [17 ** 3, 10 ** 3, 6 ** 3, 4 ** 3], levels
     assert levels[1] == [6 ** 3, 4 ** 3], levels
     for lv, F in zip(levels, Fs):
         assert lv[0] == F.n_unknown
