@@ -564,21 +564,50 @@ __device__ __forceinline__ int64_t mg_row0(const MgLevelReq &L, int64_t r)
   return L.ai64 ? L.ai64[r] : (int64_t)L.ai[r];
 }
 
-// sum over row r of S times v (v = D b when db: the pre-smoothed iterate
-// computed on the fly), lanes of `width` terms apart starting at `lane`
-__device__ __forceinline__ float2 mg_row_dot(const MgLevelReq &L, int64_t r, const float2 *v, bool db, int lane,
-                                             int width)
+// sum over row r of S times v (v = D b when DB: the pre-smoothed iterate
+// computed on the fly), lanes of kLanes terms apart starting at `lane`; kU
+// terms of the lane at a time (columns and values, then the v and D entries,
+// in flight together), summed in term order
+template <bool DB>
+__device__ __forceinline__ float2 mg_row_dot(const MgLevelReq &L, int64_t r, const float2 *v, int lane)
 {
+  constexpr int kU = 4;
   float sr = 0.f, si = 0.f;
   const int64_t k1 = mg_row0(L, r + 1);
-  for(int64_t k = mg_row0(L, r) + lane; k < k1; k += width) {
-    const int32_t c = L.aj[k];
-    const float2 x = db ? cmulf(L.dinv[c], v[c]) : v[c];
-    const float2 p = cmulf(L.a[k], x);
-    sr += p.x;
-    si += p.y;
+  for(int64_t k0 = mg_row0(L, r) + lane; k0 < k1; k0 += kU * kLanes) {
+    int32_t c[kU];
+    float2 a[kU], x[kU], d[kU];
+#pragma unroll
+    for(int u = 0; u < kU; ++u) {
+      const bool in = k0 + u * kLanes < k1;
+      c[u] = in ? L.aj[k0 + u * kLanes] : 0;
+      a[u] = in ? L.a[k0 + u * kLanes] : make_float2(0.f, 0.f);
+    }
+#pragma unroll
+    for(int u = 0; u < kU; ++u) {
+      const bool in = k0 + u * kLanes < k1;
+      x[u] = in ? v[c[u]] : make_float2(0.f, 0.f);
+      if(DB) d[u] = in ? L.dinv[c[u]] : make_float2(0.f, 0.f);
+    }
+#pragma unroll
+    for(int u = 0; u < kU; ++u)
+      if(k0 + u * kLanes < k1) {
+        const float2 p = cmulf(a[u], DB ? cmulf(d[u], x[u]) : x[u]);
+        sr += p.x;
+        si += p.y;
+      }
   }
   return make_float2(sr, si);
+}
+
+// block b of a per-level map: each XCD takes a contiguous run of the blocks
+// (the grid is a multiple of kXcds), so its L2 holds one segment of each
+// request's vectors; false past the map
+__device__ __forceinline__ bool mg_block(int nblk, int &b)
+{
+  const int per = (int)(gridDim.x / kXcds);
+  b = (int)(blockIdx.x % kXcds) * per + (int)(blockIdx.x / kXcds);
+  return b < nblk;
 }
 
 __device__ __forceinline__ float2 lanes_sumf(float2 v)
@@ -589,11 +618,13 @@ __device__ __forceinline__ float2 lanes_sumf(float2 v)
 
 // x = D b, r = b - S x
 __global__ void __launch_bounds__(256) bk_mg_down(const BItem *__restrict__ items, const int *__restrict__ act,
-                                                  const int2 *__restrict__ blk, int l, int src, int j,
+                                                  const int2 *__restrict__ blk, int nblk, int l, int src, int j,
                                                   const int *__restrict__ dj)
 {
   if(dj) j = *dj;
-  const int2 bb = blk[blockIdx.x];
+  int bi;
+  if(!mg_block(nblk, bi)) return;
+  const int2 bb = blk[bi];
   if(!act[bb.x]) return;
   const BItem &I = items[bb.x];
   const MgLevelReq &L = I.mg->lev[l];
@@ -601,19 +632,21 @@ __global__ void __launch_bounds__(256) bk_mg_down(const BItem *__restrict__ item
   const int lane = threadIdx.x % kLanes;
   if(row >= L.n) return;
   const float2 *b = mg_rhs(I, l, src, j);
-  const float2 s = lanes_sumf(mg_row_dot(L, row, b, true, lane, kLanes));
+  const float2 s = lanes_sumf(mg_row_dot<true>(L, row, b, lane));
   if(lane == 0) {
-    const float2 bi = b[row];
-    L.x[row] = cmulf(L.dinv[row], bi);
-    L.r[row] = make_float2(bi.x - s.x, bi.y - s.y);
+    const float2 br = b[row];
+    L.x[row] = cmulf(L.dinv[row], br);
+    L.r[row] = make_float2(br.x - s.x, br.y - s.y);
   }
 }
 
 // b_{l+1} = P^T r_l, rows of level l + 1
 __global__ void __launch_bounds__(256) bk_mg_restrict(const BItem *__restrict__ items, const int *__restrict__ act,
-                                                      const int2 *__restrict__ blk, int l)
+                                                      const int2 *__restrict__ blk, int nblk, int l)
 {
-  const int2 bb = blk[blockIdx.x];
+  int bi;
+  if(!mg_block(nblk, bi)) return;
+  const int2 bb = blk[bi];
   if(!act[bb.x]) return;
   const BItem &I = items[bb.x];
   const MgLevelReq &L = I.mg->lev[l], &C = I.mg->lev[l + 1];
@@ -638,9 +671,11 @@ __device__ __forceinline__ const float2 *mg_coarse_x(const MgReq &M, int l)
 
 // x_l += P xhat_{l+1}
 __global__ void __launch_bounds__(256) bk_mg_prolong(const BItem *__restrict__ items, const int *__restrict__ act,
-                                                     const int2 *__restrict__ blk, int l)
+                                                     const int2 *__restrict__ blk, int nblk, int l)
 {
-  const int2 bb = blk[blockIdx.x];
+  int bi;
+  if(!mg_block(nblk, bi)) return;
+  const int2 bb = blk[bi];
   if(!act[bb.x]) return;
   const BItem &I = items[bb.x];
   const MgLevelReq &L = I.mg->lev[l];
@@ -661,11 +696,13 @@ __global__ void __launch_bounds__(256) bk_mg_prolong(const BItem *__restrict__ i
 
 // y_l = x_l + D (b_l - S x_l); level 0 writes z
 __global__ void __launch_bounds__(256) bk_mg_post(const BItem *__restrict__ items, const int *__restrict__ act,
-                                                  const int2 *__restrict__ blk, int l, int src, int j,
+                                                  const int2 *__restrict__ blk, int nblk, int l, int src, int j,
                                                   const int *__restrict__ dj)
 {
   if(dj) j = *dj;
-  const int2 bb = blk[blockIdx.x];
+  int bi;
+  if(!mg_block(nblk, bi)) return;
+  const int2 bb = blk[bi];
   if(!act[bb.x]) return;
   const BItem &I = items[bb.x];
   const MgLevelReq &L = I.mg->lev[l];
@@ -673,10 +710,10 @@ __global__ void __launch_bounds__(256) bk_mg_post(const BItem *__restrict__ item
   const int lane = threadIdx.x % kLanes;
   if(row >= L.n) return;
   const float2 *b = mg_rhs(I, l, src, j);
-  const float2 s = lanes_sumf(mg_row_dot(L, row, L.x, false, lane, kLanes));
+  const float2 s = lanes_sumf(mg_row_dot<false>(L, row, L.x, lane));
   if(lane == 0) {
-    const float2 bi = b[row], xi = L.x[row];
-    const float2 d = cmulf(L.dinv[row], make_float2(bi.x - s.x, bi.y - s.y));
+    const float2 br = b[row], xi = L.x[row];
+    const float2 d = cmulf(L.dinv[row], make_float2(br.x - s.x, br.y - s.y));
     float2 *out = l == 0 ? I.z : L.y;
     out[row] = make_float2(xi.x + d.x, xi.y + d.y);
   }
@@ -702,7 +739,7 @@ __global__ void __launch_bounds__(kMgSmallThreads) bk_mg_small(const BItem *__re
     const float2 *b = mg_rhs(I, l, src, j);
     for(int64_t base = 0; base < L.n; base += kRows) {
       const int64_t r = base + r0;
-      const float2 s = lanes_sumf(r < L.n ? mg_row_dot(L, r, b, true, lane, kLanes) : make_float2(0.f, 0.f));
+      const float2 s = lanes_sumf(r < L.n ? mg_row_dot<true>(L, r, b, lane) : make_float2(0.f, 0.f));
       if(r < L.n && lane == 0) {
         L.x[r] = cmulf(L.dinv[r], b[r]);
         L.r[r] = make_float2(b[r].x - s.x, b[r].y - s.y);
@@ -760,7 +797,7 @@ __global__ void __launch_bounds__(kMgSmallThreads) bk_mg_small(const BItem *__re
     float2 *out = l == 0 ? I.z : L.y;
     for(int64_t base = 0; base < L.n; base += kRows) {
       const int64_t r = base + r0;
-      const float2 s = lanes_sumf(r < L.n ? mg_row_dot(L, r, L.x, false, lane, kLanes) : make_float2(0.f, 0.f));
+      const float2 s = lanes_sumf(r < L.n ? mg_row_dot<false>(L, r, L.x, lane) : make_float2(0.f, 0.f));
       if(r < L.n && lane == 0) {
         const float2 d = cmulf(L.dinv[r], make_float2(b[r].x - s.x, b[r].y - s.y));
         out[r] = make_float2(L.x[r].x + d.x, L.x[r].y + d.y);
@@ -1081,25 +1118,31 @@ gfem_status push_flags(gfem_batch *B, const std::vector<int> &act, hipStream_t s
 
 // One V-cycle of the requests with a hierarchy flagged in act: right-hand
 // side V_j (src 0, j or dj) or u (src 1), result in z
+// blocks of a launch over nblk map entries: a multiple of kXcds
+unsigned xcd_grid(int64_t nblk)
+{
+  return (unsigned)((nblk + kXcds - 1) / kXcds * kXcds);
+}
+
 void launch_mg(gfem_batch *B, hipStream_t s, const int *act, int src, int j, const int *dj)
 {
   if(B->n_mg == 0) return;
   const int ns = (int)B->slots.size();
   for(int l = 0; l < B->mg_big; ++l) {
     if(B->n_mgblk[l])
-      hipLaunchKernelGGL(bk_mg_down, dim3((unsigned)B->n_mgblk[l]), dim3(256), 0, s, B->d_items, act, B->d_mgblk[l], l,
-                         src, j, dj);
+      hipLaunchKernelGGL(bk_mg_down, dim3(xcd_grid(B->n_mgblk[l])), dim3(256), 0, s, B->d_items, act, B->d_mgblk[l],
+                         (int)B->n_mgblk[l], l, src, j, dj);
     if(B->n_mgcblk[l])
-      hipLaunchKernelGGL(bk_mg_restrict, dim3((unsigned)B->n_mgcblk[l]), dim3(256), 0, s, B->d_items, act,
-                         B->d_mgcblk[l], l);
+      hipLaunchKernelGGL(bk_mg_restrict, dim3(xcd_grid(B->n_mgcblk[l])), dim3(256), 0, s, B->d_items, act,
+                         B->d_mgcblk[l], (int)B->n_mgcblk[l], l);
   }
   hipLaunchKernelGGL(bk_mg_small, dim3((unsigned)ns), dim3(kMgSmallThreads), 0, s, B->d_items, act, src, j, dj);
   for(int l = B->mg_big - 1; l >= 0; --l) {
     if(!B->n_mgblk[l]) continue;
-    hipLaunchKernelGGL(bk_mg_prolong, dim3((unsigned)B->n_mgblk[l]), dim3(256), 0, s, B->d_items, act, B->d_mgblk[l],
-                       l);
-    hipLaunchKernelGGL(bk_mg_post, dim3((unsigned)B->n_mgblk[l]), dim3(256), 0, s, B->d_items, act, B->d_mgblk[l], l,
-                       src, j, dj);
+    hipLaunchKernelGGL(bk_mg_prolong, dim3(xcd_grid(B->n_mgblk[l])), dim3(256), 0, s, B->d_items, act, B->d_mgblk[l],
+                       (int)B->n_mgblk[l], l);
+    hipLaunchKernelGGL(bk_mg_post, dim3(xcd_grid(B->n_mgblk[l])), dim3(256), 0, s, B->d_items, act, B->d_mgblk[l],
+                       (int)B->n_mgblk[l], l, src, j, dj);
   }
 }
 
@@ -1233,6 +1276,165 @@ gfem_status residual_and_prepare(gfem_batch *B, std::vector<int> &act, hipStream
 
 }  // namespace
 
+namespace gfem {
+
+// gfem_batch_submit, or (a64_src, rhs_src non-null) a request whose FP64
+// matrix and right-hand side are copied from those device buffers instead of
+// assembled (gfem_solve of a system with a multigrid hierarchy)
+gfem_status submit_request(gfem_batch *B, gfem_system *sys, double tol, int max_iter, void *stream, int64_t *id,
+                           const double2 *a64_src, const double2 *rhs_src)
+{
+  if(!B || !sys || !id) return fail(B ? B->ctx : nullptr, GFEM_ERR_INVALID_ARG, "batch_submit: null argument");
+  gfem_ctx *ctx = B->ctx;
+  if(sys->ctx != ctx) return fail(ctx, GFEM_ERR_INVALID_ARG, "batch_submit: system of another context");
+  if(!sys->finalized) return fail(ctx, GFEM_ERR_STATE, "batch_submit: system not finalized");
+  if(!sys->cplx) return fail(ctx, GFEM_ERR_UNSUPPORTED, "batch_submit: complex systems only");
+  if(sys->symmetric) return fail(ctx, GFEM_ERR_UNSUPPORTED, "batch_submit: symmetric storage");
+  if(sys->n >= (int64_t(1) << 31)) return fail(ctx, GFEM_ERR_TOO_LARGE, "batch_submit: system too large");
+  if(!(tol > 0.) || max_iter < 1) return fail(ctx, GFEM_ERR_INVALID_ARG, "batch_submit: tol > 0, max_iter >= 1");
+  int slot = -1;
+  for(int i = 0; i < (int)B->slots.size(); ++i)
+    if(!B->slots[i].used) {
+      slot = i;
+      break;
+    }
+  if(slot < 0) return fail(ctx, GFEM_ERR_BUSY, "batch_submit: every request slot is taken");
+  const int m = B->m;
+  const int64_t off = B->arena.alloc(request_bytes(sys, m));
+  if(off < 0) return fail(ctx, GFEM_ERR_BUSY, "batch_submit: the HBM budget is taken");
+  GFEM_HIP(ctx, hipSetDevice(ctx->device));
+  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+  auto &S = B->slots[slot];
+  S = gfem_batch::Slot();
+  S.used = true;
+  S.live = true;
+  S.id = B->next_id++;
+  S.sys = sys;
+  S.off = off;
+  S.tol = tol;
+  S.max_iter = max_iter;
+  const size_t n = (size_t)sys->n, nnz = (size_t)std::max<int64_t>(sys->nnz, 1);
+  char *p = B->arena.at(off);
+  auto take = [&](size_t bytes) {
+    char *q = p;
+    p += align256(bytes);
+    return q;
+  };
+  BItem &I = S.item;
+  I.n = sys->n;
+  I.nnz = sys->nnz;
+  I.ai = sys->d_ai;
+  I.aj = sys->d_aj;
+  double2 *a64 = reinterpret_cast<double2 *>(take(16 * nnz));
+  float2 *a32 = reinterpret_cast<float2 *>(take(8 * nnz));
+  double2 *rhs = reinterpret_cast<double2 *>(take(16 * n * (size_t)sys->nrhs));
+  I.a64 = a64;
+  I.a32 = a32;
+  I.b = rhs;
+  I.x = reinterpret_cast<double2 *>(take(16 * n));
+  I.r = reinterpret_cast<double2 *>(take(16 * n));
+  double2 *dinv = reinterpret_cast<double2 *>(take(16 * n));
+  I.dinv = dinv;
+  I.w = reinterpret_cast<float2 *>(take(8 * n));
+  I.V = reinterpret_cast<float2 *>(take(8 * n * (size_t)(m + 1)));
+  I.st = reinterpret_cast<double2 *>(take(16 * ((size_t)(m + 1) * m + 3 * (size_t)m + 2 + (size_t)(m + 1) + 2)));
+  I.part = reinterpret_cast<double2 *>(take(16 * (size_t)(m + 2) * kPartMax));
+  I.nparts = (int)((sys->n + dot_span(sys->n) - 1) / dot_span(sys->n));
+  I.dc = reinterpret_cast<double2 *>(take(16 * krylov::dcgs_size(m)));
+  I.nparts2 = (int)d2_parts(sys->n);
+  I.part2 = reinterpret_cast<double2 *>(take(16 * 2 * (size_t)(m + 1) * (size_t)I.nparts2));
+  I.mg = nullptr;
+  I.z = I.u = nullptr;
+  if(sys->mg) {
+    I.z = reinterpret_cast<float2 *>(take(8 * n));
+    I.u = reinterpret_cast<float2 *>(take(8 * n));
+    char *mb = take(mg_request_bytes(sys));
+    mg_bind(sys, mb, &B->mg_host[slot]);
+    I.mg = reinterpret_cast<const MgReq *>(mb);  // the descriptor sits at the start of its block
+  }
+  B->where[S.id] = slot;
+  *id = S.id;
+  // assemble this request's system with the system's current parameters (or
+  // take the given matrix and right-hand side)
+  gfem_status st = GFEM_OK;
+  if(a64_src) {
+    const hipError_t e = hipMemcpyAsync(a64, a64_src, 16 * (size_t)sys->nnz, hipMemcpyDeviceToDevice, s);
+    const hipError_t e2 =
+      e == hipSuccess ? hipMemcpyAsync(rhs, rhs_src, 16 * n, hipMemcpyDeviceToDevice, s) : e;
+    if(e2 != hipSuccess) st = hip_fail(ctx, e2, "batch_submit: copy of the assembled system");
+  }
+  else
+    st = gfem_assemble_into(sys, GFEM_PREC_FP64, a64, rhs, stream);
+  if(st != GFEM_OK) {
+    B->where.erase(S.id);
+    B->arena.release(off);
+    S = gfem_batch::Slot();
+    return st;
+  }
+  // right scaling by row norms, unless the multigrid cycle preconditions
+  hipLaunchKernelGGL(bk_diag_inv, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, (int64_t)n, sys->d_ai, sys->d_aj,
+                     a64, B->jacobi && !sys->mg ? 1 : 0, dinv);
+  hipLaunchKernelGGL(bk_round_cols, dim3((unsigned)((nnz + 255) / 256)), dim3(256), 0, s, (int64_t)sys->nnz, a64,
+                     sys->d_aj, dinv, a32);
+  hipLaunchKernelGGL(bk_zero, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, (int64_t)n, I.x);
+  if(sys->mg) {
+    // the cycle's descriptor and level operators, in stream order
+    hipLaunchKernelGGL(k_put<MgReq>, dim3(1), dim3(64), 0, s, const_cast<MgReq *>(I.mg), B->mg_host[slot]);
+    GFEM_HIP(ctx, mg_prepare(sys, B->mg_host[slot], a64, s));
+  }
+  // this slot's descriptor only, in stream order (no wait)
+  hipLaunchKernelGGL(k_put<BItem>, dim3(1), dim3(64), 0, s, B->d_items + slot, S.item);
+  GFEM_HIP(ctx, hipGetLastError());
+  // block maps, initial residual (= -b) and |b| at the next step, for every
+  // request submitted until then at once
+  S.pending = true;
+  B->maps_dirty = true;
+  return GFEM_OK;
+}
+
+// gfem_solve of a system with a multigrid hierarchy: the last FP64 assembly
+// as the only request of a batch kept by the system (one slot, its restart
+// length), iterated to the end; x is copied to the system's solution
+gfem_status solve_mg(gfem_system *sys, int rhs_idx, double tol, int restart, int max_iter, hipStream_t s,
+                     gfem_solve_info *info)
+{
+  gfem_ctx *ctx = sys->ctx;
+  const int64_t n = sys->n;
+  const size_t bytes = request_bytes(sys, restart);
+  if(sys->mg_batch && (sys->mg_batch_m != restart || sys->mg_batch_bytes < bytes)) {
+    gfem_batch_destroy(sys->mg_batch);
+    sys->mg_batch = nullptr;
+  }
+  if(!sys->mg_batch) {
+    const gfem_status st = gfem_batch_create(ctx, restart, bytes, 1, &sys->mg_batch);
+    if(st != GFEM_OK) return st;
+    sys->mg_batch_m = restart;
+    sys->mg_batch_bytes = bytes;
+  }
+  gfem_batch *B = sys->mg_batch;
+  if(!sys->d_sol_x) GFEM_HIP(ctx, dalloc(reinterpret_cast<double2 **>(&sys->d_sol_x), (size_t)std::max<int64_t>(n, 1)));
+  int64_t id = -1;
+  gfem_status st = submit_request(B, sys, tol, max_iter, s, &id, reinterpret_cast<const double2 *>(sys->d_vals[0]),
+                                  reinterpret_cast<const double2 *>(sys->d_rhs[0]) + (int64_t)rhs_idx * n);
+  if(st != GFEM_OK) return st;
+  for(int live = 1; live > 0 && st == GFEM_OK;) st = gfem_batch_step(B, s, &live);
+  const auto w = B->where.find(id);
+  if(st == GFEM_OK && w != B->where.end()) {
+    const hipError_t e = hipMemcpyAsync(sys->d_sol_x, B->slots[w->second].item.x, 16 * (size_t)n,
+                                        hipMemcpyDeviceToDevice, s);
+    const hipError_t e2 = e == hipSuccess ? hipStreamSynchronize(s) : e;
+    if(e2 != hipSuccess) st = hip_fail(ctx, e2, "solve: solution copy");
+  }
+  // frees the slot also after an error
+  const gfem_status r = gfem_batch_result(B, id, nullptr, 0, info);
+  if(st != GFEM_OK) return st;
+  if(r != GFEM_OK) return r;
+  sys->sol_valid = true;
+  return GFEM_OK;
+}
+
+}  // namespace gfem
+
 extern "C" {
 
 gfem_status gfem_batch_request_bytes(const gfem_system *sys, int restart, size_t *bytes)
@@ -1306,103 +1508,7 @@ gfem_status gfem_batch_destroy(gfem_batch *B)
 
 gfem_status gfem_batch_submit(gfem_batch *B, gfem_system *sys, double tol, int max_iter, void *stream, int64_t *id)
 {
-  if(!B || !sys || !id) return fail(B ? B->ctx : nullptr, GFEM_ERR_INVALID_ARG, "batch_submit: null argument");
-  gfem_ctx *ctx = B->ctx;
-  if(sys->ctx != ctx) return fail(ctx, GFEM_ERR_INVALID_ARG, "batch_submit: system of another context");
-  if(!sys->finalized) return fail(ctx, GFEM_ERR_STATE, "batch_submit: system not finalized");
-  if(!sys->cplx) return fail(ctx, GFEM_ERR_UNSUPPORTED, "batch_submit: complex systems only");
-  if(sys->symmetric) return fail(ctx, GFEM_ERR_UNSUPPORTED, "batch_submit: symmetric storage");
-  if(sys->n >= (int64_t(1) << 31)) return fail(ctx, GFEM_ERR_TOO_LARGE, "batch_submit: system too large");
-  if(!(tol > 0.) || max_iter < 1) return fail(ctx, GFEM_ERR_INVALID_ARG, "batch_submit: tol > 0, max_iter >= 1");
-  int slot = -1;
-  for(int i = 0; i < (int)B->slots.size(); ++i)
-    if(!B->slots[i].used) {
-      slot = i;
-      break;
-    }
-  if(slot < 0) return fail(ctx, GFEM_ERR_BUSY, "batch_submit: every request slot is taken");
-  const int m = B->m;
-  const int64_t off = B->arena.alloc(request_bytes(sys, m));
-  if(off < 0) return fail(ctx, GFEM_ERR_BUSY, "batch_submit: the HBM budget is taken");
-  GFEM_HIP(ctx, hipSetDevice(ctx->device));
-  hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-  auto &S = B->slots[slot];
-  S = gfem_batch::Slot();
-  S.used = true;
-  S.live = true;
-  S.id = B->next_id++;
-  S.sys = sys;
-  S.off = off;
-  S.tol = tol;
-  S.max_iter = max_iter;
-  const size_t n = (size_t)sys->n, nnz = (size_t)std::max<int64_t>(sys->nnz, 1);
-  char *p = B->arena.at(off);
-  auto take = [&](size_t bytes) {
-    char *q = p;
-    p += align256(bytes);
-    return q;
-  };
-  BItem &I = S.item;
-  I.n = sys->n;
-  I.nnz = sys->nnz;
-  I.ai = sys->d_ai;
-  I.aj = sys->d_aj;
-  double2 *a64 = reinterpret_cast<double2 *>(take(16 * nnz));
-  float2 *a32 = reinterpret_cast<float2 *>(take(8 * nnz));
-  double2 *rhs = reinterpret_cast<double2 *>(take(16 * n * (size_t)sys->nrhs));
-  I.a64 = a64;
-  I.a32 = a32;
-  I.b = rhs;
-  I.x = reinterpret_cast<double2 *>(take(16 * n));
-  I.r = reinterpret_cast<double2 *>(take(16 * n));
-  double2 *dinv = reinterpret_cast<double2 *>(take(16 * n));
-  I.dinv = dinv;
-  I.w = reinterpret_cast<float2 *>(take(8 * n));
-  I.V = reinterpret_cast<float2 *>(take(8 * n * (size_t)(m + 1)));
-  I.st = reinterpret_cast<double2 *>(take(16 * ((size_t)(m + 1) * m + 3 * (size_t)m + 2 + (size_t)(m + 1) + 2)));
-  I.part = reinterpret_cast<double2 *>(take(16 * (size_t)(m + 2) * kPartMax));
-  I.nparts = (int)((sys->n + dot_span(sys->n) - 1) / dot_span(sys->n));
-  I.dc = reinterpret_cast<double2 *>(take(16 * krylov::dcgs_size(m)));
-  I.nparts2 = (int)d2_parts(sys->n);
-  I.part2 = reinterpret_cast<double2 *>(take(16 * 2 * (size_t)(m + 1) * (size_t)I.nparts2));
-  I.mg = nullptr;
-  I.z = I.u = nullptr;
-  if(sys->mg) {
-    I.z = reinterpret_cast<float2 *>(take(8 * n));
-    I.u = reinterpret_cast<float2 *>(take(8 * n));
-    char *mb = take(mg_request_bytes(sys));
-    mg_bind(sys, mb, &B->mg_host[slot]);
-    I.mg = reinterpret_cast<const MgReq *>(mb);  // the descriptor sits at the start of its block
-  }
-  B->where[S.id] = slot;
-  *id = S.id;
-  // assemble this request's system with the system's current parameters
-  gfem_status st = gfem_assemble_into(sys, GFEM_PREC_FP64, a64, rhs, stream);
-  if(st != GFEM_OK) {
-    B->where.erase(S.id);
-    B->arena.release(off);
-    S = gfem_batch::Slot();
-    return st;
-  }
-  // right scaling by row norms, unless the multigrid cycle preconditions
-  hipLaunchKernelGGL(bk_diag_inv, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, (int64_t)n, sys->d_ai, sys->d_aj,
-                     a64, B->jacobi && !sys->mg ? 1 : 0, dinv);
-  hipLaunchKernelGGL(bk_round_cols, dim3((unsigned)((nnz + 255) / 256)), dim3(256), 0, s, (int64_t)sys->nnz, a64,
-                     sys->d_aj, dinv, a32);
-  hipLaunchKernelGGL(bk_zero, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, (int64_t)n, I.x);
-  if(sys->mg) {
-    // the cycle's descriptor and level operators, in stream order
-    hipLaunchKernelGGL(k_put<MgReq>, dim3(1), dim3(64), 0, s, const_cast<MgReq *>(I.mg), B->mg_host[slot]);
-    GFEM_HIP(ctx, mg_prepare(sys, B->mg_host[slot], a64, s));
-  }
-  // this slot's descriptor only, in stream order (no wait)
-  hipLaunchKernelGGL(k_put<BItem>, dim3(1), dim3(64), 0, s, B->d_items + slot, S.item);
-  GFEM_HIP(ctx, hipGetLastError());
-  // block maps, initial residual (= -b) and |b| at the next step, for every
-  // request submitted until then at once
-  S.pending = true;
-  B->maps_dirty = true;
-  return GFEM_OK;
+  return gfem::submit_request(B, sys, tol, max_iter, stream, id, nullptr, nullptr);
 }
 
 gfem_status gfem_batch_step(gfem_batch *B, void *stream, int *n_live)

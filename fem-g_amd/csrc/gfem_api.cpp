@@ -328,6 +328,7 @@ gfem_status gfem_system_destroy(gfem_system *sys)
   dfree(sys->d_ai);
   dfree(sys->d_aj);
   dfree(sys->d_xyz);
+  if(sys->mg_batch) gfem_batch_destroy(sys->mg_batch);
   dfree(sys->d_sol_x);
   dfree(sys->d_sol_r);
   dfree(sys->d_sol_w);
@@ -1832,6 +1833,7 @@ gfem_status gfem_solve(gfem_system *sys, int rhs_idx, double tol, int restart, i
     return fail(ctx, GFEM_ERR_INVALID_ARG, "solve: tol > 0, 1 <= restart <= 1000, max_iter >= 1");
   if(sys->symmetric) return fail(ctx, GFEM_ERR_UNSUPPORTED, "solve: symmetric storage");
   GFEM_HIP(ctx, hipSetDevice(ctx->device));
+  if(sys->mg) return solve_mg(sys, rhs_idx, tol, restart, max_iter, reinterpret_cast<hipStream_t>(stream), info);
   return solve_gmres_ir(sys, rhs_idx, tol, restart, max_iter, reinterpret_cast<hipStream_t>(stream), info);
 }
 

@@ -309,8 +309,12 @@ struct gfem_system {
   int32_t *d_dc_fix = nullptr;
   int64_t *d_lin_ptr = nullptr, *d_dc_ptr = nullptr;
   int64_t n_lin = 0, n_dcw = 0;
-  // shifted-Laplacian multigrid hierarchy (gfem_mg_setup), or null
+  // shifted-Laplacian multigrid hierarchy (gfem_mg_setup), or null; gfem_solve
+  // then runs the system as the only request of mg_batch (restart mg_batch_m)
   gfem::MgSystem *mg = nullptr;
+  gfem_batch *mg_batch = nullptr;
+  int mg_batch_m = 0;
+  size_t mg_batch_bytes = 0;
   // outputs per precision (0 FP64, 1 MIXED)
   void *d_vals[2] = {nullptr, nullptr};
   void *d_rhs[2] = {nullptr, nullptr};
@@ -399,6 +403,10 @@ struct SetupTimer {
 // GMRES with FP64 iterative refinement on the last FP64 assembly (solver.hip)
 gfem_status solve_gmres_ir(gfem_system *sys, int rhs_idx, double tol, int restart, int max_iter, hipStream_t s,
                            gfem_solve_info *info);
+// the same solve preconditioned by the system's multigrid cycle, through the
+// batched engine (batch.hip)
+gfem_status solve_mg(gfem_system *sys, int rhs_idx, double tol, int restart, int max_iter, hipStream_t s,
+                     gfem_solve_info *info);
 
 // Launchers implemented in kernels.hip.  All are asynchronous on `s`.
 hipError_t launch_basis(const Group &g, Rule &r, hipStream_t s);

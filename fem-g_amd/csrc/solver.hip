@@ -548,8 +548,9 @@ static gfem_status gmres_ir(gfem_system *sys, int rhs_idx, double tol, int m, in
       return fail(ctx, GFEM_ERR_OUT_OF_MEMORY, "solver state allocation");
     sys->sol_m = m;
   }
-  if(!sys->d_sol_x) {
-    if(!chk(hipMalloc(&sys->d_sol_x, sizeof(T64) * (size_t)n)) || !chk(hipMalloc(&sys->d_sol_r, sizeof(T64) * (size_t)n)) ||
+  if(!sys->d_sol_r) {  // (x alone may exist: a preconditioned solve allocates only it)
+    if((!sys->d_sol_x && !chk(hipMalloc(&sys->d_sol_x, sizeof(T64) * (size_t)n))) ||
+       !chk(hipMalloc(&sys->d_sol_r, sizeof(T64) * (size_t)n)) ||
        !chk(hipMalloc(&sys->d_sol_w, sizeof(T32) * (size_t)n)) ||
        !chk(hipMalloc(&sys->d_sol_a32, sizeof(T32) * (size_t)std::max<int64_t>(nnz, 1))) ||
        !chk(hipMalloc(&sys->d_sol_nrm, sizeof(double2) * 2)) ||

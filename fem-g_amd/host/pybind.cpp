@@ -476,6 +476,7 @@ PYBIND11_MODULE(_gfem_host, m)
          py::arg("max_requests") = 256)
     .def("add_mesh", &serve::Server::addMesh, py::arg("formulation"), py::keep_alive<1, 2>())
     .def("request_bytes", &serve::Server::requestBytes)
+    .def("set_progress", &serve::Server::setProgress, py::arg("seconds"))
     .def(
       "run",
       [](serve::Server &sv, const py::list &schedule, bool keep) {
@@ -523,6 +524,7 @@ PYBIND11_MODULE(_gfem_host, m)
       d["steps"] = sv.stats().steps;
       d["max_live"] = sv.stats().maxLive;
       d["peak_bytes"] = sv.stats().peakBytes;
+      d["max_waiting"] = sv.stats().maxWaiting;
       return d;
     });
 

@@ -1,5 +1,7 @@
 #include "serve.hpp"
 
+#include <cstdio>
+
 #include <algorithm>
 #include <chrono>
 #include <deque>
@@ -61,10 +63,18 @@ std::vector<Completed> Server::run(const std::vector<Request> &schedule, bool ke
   std::deque<size_t> waiting;          // arrived, not admitted
   std::map<int64_t, size_t> running;   // batch id -> request
   std::map<int64_t, double> admitted;
-  size_t held = 0;
+  size_t held = 0, nadmitted = 0;
+  double report = _progress;
   while(out.size() < schedule.size()) {
     const double t = now();
     while(next < schedule.size() && schedule[next].arrival <= t) waiting.push_back(next++);
+    _stats.maxWaiting = std::max(_stats.maxWaiting, waiting.size());
+    if(_progress > 0. && t >= report) {
+      std::fprintf(stderr, "serve: %.0f s  arrived %zu  admitted %zu  completed %zu  iterating %zu  waiting %zu\n", t,
+                   next, nadmitted, out.size(), running.size(), waiting.size());
+      std::fflush(stderr);
+      while(report <= t) report += _progress;
+    }
     // admit in arrival order while a slot and memory are free
     while(!waiting.empty()) {
       const size_t ri = waiting.front();
@@ -80,6 +90,7 @@ std::vector<Completed> Server::run(const std::vector<Request> &schedule, bool ke
       }
       check(st, "gfem_batch_submit");
       waiting.pop_front();
+      ++nadmitted;
       running[id] = ri;
       admitted[id] = now();
       held += bytes[ri];

@@ -43,6 +43,7 @@ struct RunStats {
   int steps = 0;         // batch steps (restart cycles)
   int maxLive = 0;       // most requests iterating together
   size_t peakBytes = 0;  // most HBM held by admitted requests
+  size_t maxWaiting = 0; // most requests arrived and not yet admitted
 };
 
 class Server {
@@ -61,6 +62,9 @@ class Server {
   // per request, in completion order.
   std::vector<Completed> run(const std::vector<Request> &schedule, bool keepSolutions = false);
   const RunStats &stats() const { return _stats; }
+  // Print a line to stderr every `seconds` of a run (0: never): time,
+  // requests arrived / admitted / completed, iterating and waiting.
+  void setProgress(double seconds) { _progress = seconds; }
 
  private:
   gfem_batch *_batch = nullptr;
@@ -68,6 +72,7 @@ class Server {
   size_t _budget;
   std::vector<problem::Formulation<std::complex<double>> *> _meshes;
   RunStats _stats;
+  double _progress = 0.;
 };
 
 }  // namespace serve

@@ -385,7 +385,8 @@ typedef struct gfem_solve_info {
  * Restarted GMRES(restart) in FP32 inside FP64 iterative refinement: each
  * cycle starts from the FP64 residual; stops at rel_residual <= tol or after
  * max_iter inner iterations.  x stays on the device (gfem_solution_*).
- * General (non-symmetric) storage only. */
+ * General (non-symmetric) storage only.  A system with a multigrid hierarchy
+ * (gfem_mg_setup) is preconditioned by it. */
 gfem_status gfem_solve(gfem_system *sys, int rhs_idx, double tol, int restart, int max_iter, void *stream,
                        gfem_solve_info *info);
 /* Copy the last solution (n scalars of the system's FP64 type) to the host. */
@@ -430,10 +431,14 @@ gfem_status gfem_batch_result(gfem_batch *batch, int64_t id, void *x_host, size_
  * The hierarchy is built once, on the host from the device maps:
  *   - geometric levels on the DOF positions: trilinear interpolation from
  *     uniform grids of twice, four times, ... the mean DOF spacing, down to
- *     at most 6 points per axis;
+ *     at most 5 points per axis (the coarsest level, at most 125 unknowns,
+ *     is inverted per request);
  *   - Galerkin products of every constant bilinear term's unit matrix.
  * Each request combines the hierarchy with its own coefficients at
- * submission.  Requests submitted afterwards use it; gfem_solve does not.
+ * submission.  Requests submitted afterwards use it, and so does gfem_solve
+ * of the system: it then runs the last FP64 assembly as the only request of
+ * a one-slot batch that the system keeps (same algorithm and result as
+ * gfem_batch_submit of that assembly).
  * Lagrange fields with constant bilinear coefficients only (else
  * GFEM_ERR_UNSUPPORTED).  n_shift must be 1. */
 gfem_status gfem_mg_setup(gfem_system *sys, int n_shift, const int *shift_group, const int *shift_term, double beta);
