@@ -47,6 +47,22 @@ class Spec:
         return total
 
 
+    def scatter_map_bytes(self):
+        """bytes of the reference-format element maps that the matrix scatter
+        reads, over the elements of bilinear terms: element->DOF (4 B per local
+        DOF) and element->nonzero (4 B per local pair), E (4 N + 4 N^2)."""
+        ents = set()
+        for t in self.terms:
+            if t["kind"] == "bilinear":
+                ents.update(t["domain"])
+        total = 0
+        for b in self.model.blocks():
+            if (b["dim"], b["tag"]) in ents and b["dim"] > 0:
+                npe = b["conn"].shape[1] if self.basis == "lagrange" else _hier_count(b["dim"], self.order)
+                total += b["conn"].shape[0] * (4 * npe + 4 * npe * npe)
+        return total
+
+
 def _hier_count(dim, p):
     return p + 1 if dim == 1 else ((p + 1) * (p + 2) // 2 if dim == 2 else (p + 1) * (p + 2) * (p + 3) // 6)
 
