@@ -298,7 +298,7 @@ def solver_leg(args):
     if args.solver_mg:
         # the same solve preconditioned by one shifted-Laplacian V-cycle per step
         t0 = time.perf_counter()
-        levels = F.set_shifted_laplacian(W.HELMHOLTZ_MASS, 0.5)
+        levels = F.set_shifted_laplacian(W.HELMHOLTZ_MASS, args.serve_mg_beta)
         t_setup = time.perf_counter() - t0
         F.solve(tol=1e-10, restart=30, max_iter=30000, download=False)  # the system's one-request batch
         t0 = time.perf_counter()
@@ -495,7 +495,7 @@ def main():
     ap.add_argument("--serve-baseline", type=int, default=16, help="requests timed one at a time")
     ap.add_argument("--serve-mg", type=int, choices=[0, 1], default=1,
                     help="shifted-Laplacian multigrid preconditioner on the serving meshes (1) or row scaling (0)")
-    ap.add_argument("--serve-mg-beta", type=float, default=0.5)
+    ap.add_argument("--serve-mg-beta", type=float, default=1.0)
     ap.add_argument("--solver-mg", type=int, choices=[0, 1], default=1,
                     help="also time the single solve preconditioned by the shifted-Laplacian multigrid")
     ap.add_argument("--batch-mg", type=int, choices=[0, 1], default=1,

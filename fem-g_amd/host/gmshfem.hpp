@@ -216,9 +216,10 @@ class Formulation {
   // Shifted-Laplacian multigrid preconditioner of this system's batched
   // solves (gfem_mg_setup): the term `tag` (the mass term of a Helmholtz
   // formulation) has its coefficient multiplied by (1 + i beta) in the
-  // preconditioned operator.  Requests submitted afterwards use it.  Needs
-  // the system (after the first assemble()).  Returns the rows per level.
-  std::vector<int> setShiftedLaplacian(unsigned int tag, double beta = 0.5);
+  // preconditioned operator.  Requests submitted afterwards use it, and so
+  // does solve().  Needs the system (after the first assemble()).  Returns
+  // the rows per level.
+  std::vector<int> setShiftedLaplacian(unsigned int tag, double beta = 1.0);
   // The parameters of one instance of gfem_assemble_many: the (group, term)
   // slots of each term tag with its new coefficient, after the checks of
   // setTermCoefficient.  The formulation's own coefficients take the new

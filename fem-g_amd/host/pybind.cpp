@@ -131,7 +131,7 @@ void bind_formulation(py::module_ &m, const char *name)
       py::arg("source"), py::arg("rhs"), py::arg("domain"), py::arg("integration"), py::arg("hermitian") = true,
       py::arg("rhs_idx") = 0)
     .def("set_term_coefficient", &F::setTermCoefficient, py::arg("tag"), py::arg("coefficient"))
-    .def("set_shifted_laplacian", &F::setShiftedLaplacian, py::arg("tag"), py::arg("beta") = 0.5)
+    .def("set_shifted_laplacian", &F::setShiftedLaplacian, py::arg("tag"), py::arg("beta") = 1.0)
     .def(
       "many_instance",
       [](F &f, const std::vector<std::pair<unsigned int, ScalarFunction>> &params) {

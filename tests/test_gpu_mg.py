@@ -20,7 +20,7 @@ def _server(meshes, restart=30, mg=True):
     for F in Fs:
         sv.add_mesh(F)
         if mg:
-            levels.append(F.set_shifted_laplacian(W.HELMHOLTZ_MASS, 0.5))
+            levels.append(F.set_shifted_laplacian(W.HELMHOLTZ_MASS))  # beta 1
     return sv, Fs, levels
 
 
@@ -78,7 +78,7 @@ def test_mg_request_is_the_same_alone_and_in_a_mix():
     sv = Server(restart=25, hbm_bytes=1 << 31, max_requests=16)
     for F in Fs:
         sv.add_mesh(F)
-    Fs[0].set_shifted_laplacian(W.HELMHOLTZ_MASS, 0.5)
+    Fs[0].set_shifted_laplacian(W.HELMHOLTZ_MASS, 0.5)  # a smaller shift here
     Fs[1].set_shifted_laplacian(W.HELMHOLTZ_MASS, 0.5)  # mesh 2 stays row-scaled
     probe = (1, 20.0, (0.4, 0.55, 0.5))
     alone = sv.run(_schedule([probe]), keep_solutions=True)[0]
@@ -166,7 +166,7 @@ def test_mg_single_solve():
         F.set_term_coefficient(tag, f)
     F.assemble(download=False)
     plain = F.solve(tol=1e-11, restart=30, max_iter=40000)
-    F.set_shifted_laplacian(W.HELMHOLTZ_MASS, 0.5)
+    F.set_shifted_laplacian(W.HELMHOLTZ_MASS)
     F.assemble(download=False)
     r = F.solve(tol=1e-11, restart=30, max_iter=40000)
     assert r["converged"], r
