@@ -65,7 +65,33 @@ __global__ void k_mg_dof_xyz(const int32_t *__restrict__ num, const int32_t *__r
   for(int c = 0; c < 3; ++c) out[3 * (int64_t)d + c] = X[c];
 }
 
+// FP64 complex values to complex64
+__global__ void k_mg_to_f2(int64_t n, const double2 *__restrict__ a, float2 *__restrict__ out)
+{
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if(i < n) out[i] = make_float2((float)a[i].x, (float)a[i].y);
+}
+
 // ------------------------------------------------------------ host sparse algebra
+
+unsigned host_threads()
+{
+  return std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+}
+
+// fn(begin, end) over contiguous ranges of [0, n), one per host thread
+template <class Fn>
+void host_parallel(int64_t n, Fn fn)
+{
+  const unsigned nth = (unsigned)std::max<int64_t>(1, std::min<int64_t>(host_threads(), n / 65536 + 1));
+  if(nth == 1) {
+    fn(int64_t(0), n);
+    return;
+  }
+  std::vector<std::thread> th;
+  for(unsigned w = 0; w < nth; ++w) th.emplace_back(fn, n * w / nth, n * (w + 1) / nth);
+  for(auto &t : th) t.join();
+}
 
 struct Csr {
   int64_t rows = 0, cols = 0;
@@ -155,7 +181,7 @@ struct MultiCsr {
 MultiCsr galerkin(const MultiCsr &X, const Csr &P, const Csr &Pt, int nt)
 {
   const int64_t nc = P.cols;
-  unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  unsigned nth = host_threads();
   std::vector<std::vector<int32_t>> cols(nth);
   std::vector<std::vector<CF>> vals(nth);
   std::vector<std::vector<int64_t>> rowlen(nth);
@@ -524,6 +550,12 @@ extern "C" gfem_status gfem_mg_setup(gfem_system *sys, int n_shift, const int *s
   if(n == 0) return fail(ctx, GFEM_ERR_INVALID_ARG, "mg_setup: no unknowns");
 
   SetupTimer timer;
+  // the shifted term's unit matrix on level 0, complex64, handed to the
+  // hierarchy at the end (freed here on every earlier return)
+  struct DevBuf {
+    float2 *p = nullptr;
+    ~DevBuf() { dfree(p); }
+  } xs_dev;
   // DOF positions
   std::vector<double> xyz((size_t)(3 * n), 0.);
   {
@@ -571,6 +603,7 @@ extern "C" gfem_status gfem_mg_setup(gfem_system *sys, int n_shift, const int *s
     GFEM_HIP(ctx, dalloc(&vals, (size_t)nnz));
     hipError_t e = dalloc(&rhs, (size_t)(n * sys->nrhs));
     if(e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void **>(&h), sizeof(double2) * (size_t)nnz);
+    if(e == hipSuccess) e = dalloc(&xs_dev.p, (size_t)std::max<int64_t>(nnz, 1));
     gfem_status st = e == hipSuccess ? GFEM_OK : hip_fail(ctx, e, "mg_setup: buffers");
     for(int t = 0; t < nt && st == GFEM_OK; ++t) {
       for(int u = 0; u < nt && st == GFEM_OK; ++u)
@@ -578,10 +611,19 @@ extern "C" gfem_status gfem_mg_setup(gfem_system *sys, int n_shift, const int *s
       if(st == GFEM_OK) st = gfem_assemble_into(sys, GFEM_PREC_FP64, vals, rhs, nullptr);
       if(st == GFEM_OK) {
         e = hipMemcpy(h, vals, sizeof(double2) * (size_t)nnz, hipMemcpyDeviceToHost);
+        if(e == hipSuccess && t == shift) {
+          // the shifted term's unit matrix stays on the device (level 0 of each request)
+          hipLaunchKernelGGL(k_mg_to_f2, dim3(blocks(nnz)), dim3(kTB), 0, nullptr, nnz, vals, xs_dev.p);
+          e = hipGetLastError();
+        }
         if(e != hipSuccess) st = hip_fail(ctx, e, "mg_setup: unit matrix");
       }
+      timer.mark("mg: unit matrix assembled and copied");
       if(st == GFEM_OK)
-        for(int64_t k = 0; k < nnz; ++k) X.v[(size_t)k * nt + t] = CF{(float)h[k].x, (float)h[k].y};
+        host_parallel(nnz, [&](int64_t b, int64_t end) {
+          for(int64_t k = b; k < end; ++k) X.v[(size_t)k * nt + t] = CF{(float)h[k].x, (float)h[k].y};
+        });
+      timer.mark("mg: unit matrix converted");
     }
     for(int u = 0; u < nt; ++u) {
       const gfem_status r = gfem_term_update(sys, terms[u].first, terms[u].second, saved[u].real(), saved[u].imag(), 0,
@@ -689,9 +731,12 @@ extern "C" gfem_status gfem_mg_setup(gfem_system *sys, int n_shift, const int *s
     if(l > 0 && l < nlev - 1) {
       std::vector<int32_t> ai(L.ptr.begin(), L.ptr.end());
       std::vector<float2> g((size_t)nt * L.idx.size());
-      for(int t = 0; t < nt; ++t)
-        for(size_t k = 0; k < L.idx.size(); ++k)
-          g[(size_t)t * L.idx.size() + k] = make_float2(L.v[k * nt + t].re, L.v[k * nt + t].im);
+      const int64_t lnnz = (int64_t)L.idx.size();
+      host_parallel(lnnz, [&](int64_t b, int64_t end) {
+        for(int t = 0; t < nt; ++t)
+          for(int64_t k = b; k < end; ++k)
+            g[(size_t)t * lnnz + k] = make_float2(L.v[k * nt + t].re, L.v[k * nt + t].im);
+      });
       e = up(&m->ai[l], ai.data(), ai.size());
       if(e == hipSuccess) e = up(&m->aj[l], L.idx.data(), L.idx.size());
       if(e == hipSuccess) e = up(&m->g[l], g.data(), g.size());
@@ -723,11 +768,8 @@ extern "C" gfem_status gfem_mg_setup(gfem_system *sys, int n_shift, const int *s
   }
   if(e == hipSuccess) {
     // the shifted term's unit matrix on the fine level
-    const MultiCsr &L = Ls[0];
-    std::vector<float2> xs((size_t)nnz);
-    for(int64_t k = 0; k < nnz; ++k)
-      xs[k] = make_float2(L.v[(size_t)k * nt + shift].re, L.v[(size_t)k * nt + shift].im);
-    e = up(&m->xs, xs.data(), xs.size());
+    m->xs = xs_dev.p;
+    xs_dev.p = nullptr;
   }
   if(e != hipSuccess) {
     mg_free(m);
