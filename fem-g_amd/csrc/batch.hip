@@ -1416,7 +1416,6 @@ gfem_status solve_mg(gfem_system *sys, int rhs_idx, double tol, int restart, int
   int64_t id = -1;
   gfem_status st = submit_request(B, sys, tol, max_iter, s, &id, reinterpret_cast<const double2 *>(sys->d_vals[0]),
                                   reinterpret_cast<const double2 *>(sys->d_rhs[0]) + (int64_t)rhs_idx * n);
-  if(st != GFEM_OK) return st;
   for(int live = 1; live > 0 && st == GFEM_OK;) st = gfem_batch_step(B, s, &live);
   const auto w = B->where.find(id);
   if(st == GFEM_OK && w != B->where.end()) {
@@ -1425,10 +1424,13 @@ gfem_status solve_mg(gfem_system *sys, int rhs_idx, double tol, int restart, int
     const hipError_t e2 = e == hipSuccess ? hipStreamSynchronize(s) : e;
     if(e2 != hipSuccess) st = hip_fail(ctx, e2, "solve: solution copy");
   }
-  // frees the slot also after an error
-  const gfem_status r = gfem_batch_result(B, id, nullptr, 0, info);
-  if(st != GFEM_OK) return st;
-  if(r != GFEM_OK) return r;
+  if(st == GFEM_OK) st = gfem_batch_result(B, id, nullptr, 0, info);  // frees the slot
+  if(st != GFEM_OK) {
+    // a request may be left in the slot: the next solve starts from a new batch
+    gfem_batch_destroy(B);
+    sys->mg_batch = nullptr;
+    return st;
+  }
   sys->sol_valid = true;
   return GFEM_OK;
 }

@@ -14,8 +14,9 @@
 //   level l+1  every other grid line of level l (and the last), P_l trilinear;
 //   coarsest  at most kMgCoarsest points per axis, solved exactly (dense
 //            inverse per request).
-// Coarse operators are Galerkin products P^T X_t P of every term, computed
-// once; a request combines them with its own coefficients.  Smoothing is one
+// Coarse operators are Galerkin products P^T X_t P of every term class
+// (terms of equal coefficient summed), computed once; a request combines them
+// with its own coefficients.  Smoothing is one
 // damped Jacobi sweep (weight kMgOmega) before and after the coarse
 // correction.  Vectors and operators of the cycle are complex64.
 #pragma once
@@ -25,7 +26,7 @@
 namespace gfem {
 
 constexpr int kMgMaxLevels = 8;  // levels including the fine one
-constexpr int kMgMaxTerms = 8;   // constant bilinear terms combined per level
+constexpr int kMgMaxTerms = 8;   // classes of constant bilinear terms combined per level
 constexpr int kMgCoarsest = 5;   // grid points per axis of the coarsest level, at most
 constexpr int kMgSmall = 512;    // levels of at most this many rows run inside one block per request
 constexpr int kMgSmallThreads = 1024;
@@ -59,7 +60,8 @@ struct MgReq {
 // Hierarchy of a system, device-resident (mg.hip).
 struct MgSystem {
   int nlev = 0, lsmall = 0;
-  int nterms = 0, shift = -1;           // terms combined, index of the shifted one among them
+  int nterms = 0, shift = -1;           // term classes combined, index of the shifted one among them
+  // (group, term) of each class's first member: the class's coefficient
   double beta = 0.5;
   int term_group[kMgMaxTerms], term_index[kMgMaxTerms];
   int n[kMgMaxLevels];

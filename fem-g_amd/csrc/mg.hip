@@ -523,15 +523,23 @@ extern "C" gfem_status gfem_mg_setup(gfem_system *sys, int n_shift, const int *s
   if(!sys->finalized) return fail(ctx, GFEM_ERR_STATE, "mg_setup: system not finalized");
   if(!sys->cplx) return fail(ctx, GFEM_ERR_UNSUPPORTED, "mg_setup: complex systems only");
   if(sys->symmetric) return fail(ctx, GFEM_ERR_UNSUPPORTED, "mg_setup: symmetric storage");
-  if(n_shift != 1 || !shift_group || !shift_term)
-    return fail(ctx, GFEM_ERR_INVALID_ARG, "mg_setup: exactly one shifted term (group, term)");
+  if(n_shift < 1 || !shift_group || !shift_term)
+    return fail(ctx, GFEM_ERR_INVALID_ARG, "mg_setup: at least one shifted (group, term)");
   if(!(beta > 0.)) return fail(ctx, GFEM_ERR_INVALID_ARG, "mg_setup: beta > 0");
   if(sys->n >= (int64_t(1) << 31) || sys->nnz >= (int64_t(1) << 31))
     return fail(ctx, GFEM_ERR_TOO_LARGE, "mg_setup: system too large");
   GFEM_HIP(ctx, hipSetDevice(ctx->device));
-  // the constant-coefficient bilinear terms and the shifted one among them
-  std::vector<std::pair<int, int>> terms;
-  int shift = -1;
+  // the constant-coefficient bilinear terms, in classes of equal
+  // coefficient, form and test-function factor (one integral() over several
+  // element groups: its terms keep equal coefficients, so the class has one
+  // unit matrix, the sum of theirs); the shifted slots must form one class
+  std::vector<std::pair<int, int>> all;  // every member
+  std::vector<std::vector<std::pair<int, int>>> cls;
+  auto same = [&](const DevTerm &a, const DevTerm &b) {
+    return a.c_re == b.c_re && a.c_im == b.c_im && a.form_dof == b.form_dof && a.form_tf == b.form_tf &&
+           a.fac_re == b.fac_re && a.fac_im == b.fac_im;
+  };
+  auto term = [&](const std::pair<int, int> &s) -> const DevTerm & { return sys->groups[s.first].dterms[s.second]; };
   for(int g = 0; g < (int)sys->groups.size(); ++g) {
     const Group &G = sys->groups[g];
     if(G.family != 0) return fail(ctx, GFEM_ERR_UNSUPPORTED, "mg_setup: Lagrange fields only");
@@ -539,13 +547,26 @@ extern "C" gfem_status gfem_mg_setup(gfem_system *sys, int n_shift, const int *s
       const DevTerm &d = G.dterms[t];
       if(d.kind != 0) continue;
       if(d.per_point) return fail(ctx, GFEM_ERR_UNSUPPORTED, "mg_setup: expression coefficients of bilinear terms");
-      if(g == shift_group[0] && t == shift_term[0]) shift = (int)terms.size();
-      terms.push_back({g, t});
+      all.push_back({g, t});
+      size_t c = 0;
+      while(c < cls.size() && !same(term(cls[c][0]), d)) ++c;
+      if(c == cls.size()) cls.emplace_back();
+      cls[c].push_back({g, t});
     }
   }
-  if(shift < 0) return fail(ctx, GFEM_ERR_INVALID_ARG, "mg_setup: the shifted term is not a bilinear term");
-  if((int)terms.size() > kMgMaxTerms) return fail(ctx, GFEM_ERR_UNSUPPORTED, "mg_setup: too many bilinear terms");
-  const int nt = (int)terms.size();
+  int shift = -1;
+  for(int i = 0; i < n_shift; ++i) {
+    int c = -1;
+    for(int u = 0; u < (int)cls.size() && c < 0; ++u)
+      for(const auto &s : cls[u])
+        if(s.first == shift_group[i] && s.second == shift_term[i]) c = u;
+    if(c < 0) return fail(ctx, GFEM_ERR_INVALID_ARG, "mg_setup: a shifted (group, term) is not a bilinear term");
+    if(shift >= 0 && c != shift)
+      return fail(ctx, GFEM_ERR_INVALID_ARG, "mg_setup: the shifted terms have different coefficients or forms");
+    shift = c;
+  }
+  if((int)cls.size() > kMgMaxTerms) return fail(ctx, GFEM_ERR_UNSUPPORTED, "mg_setup: too many bilinear terms");
+  const int nt = (int)cls.size();
   const int64_t n = sys->n, nnz = sys->nnz;
   if(n == 0) return fail(ctx, GFEM_ERR_INVALID_ARG, "mg_setup: no unknowns");
 
@@ -594,11 +615,8 @@ extern "C" gfem_status gfem_mg_setup(gfem_system *sys, int n_shift, const int *s
     GFEM_HIP(ctx, hipMemcpy(ai.data(), sys->d_ai, sizeof(int64_t) * ai.size(), hipMemcpyDeviceToHost));
     GFEM_HIP(ctx, hipMemcpy(X.idx.data(), sys->d_aj, sizeof(int32_t) * (size_t)nnz, hipMemcpyDeviceToHost));
     for(size_t i = 0; i < ai.size(); ++i) X.ptr[i] = ai[i];
-    std::vector<std::complex<double>> saved(nt);
-    for(int t = 0; t < nt; ++t) {
-      const DevTerm &d = sys->groups[terms[t].first].dterms[terms[t].second];
-      saved[t] = {d.c_re, d.c_im};
-    }
+    std::vector<std::complex<double>> saved(all.size());
+    for(size_t i = 0; i < all.size(); ++i) saved[i] = {term(all[i]).c_re, term(all[i]).c_im};
     double2 *vals = nullptr, *rhs = nullptr, *h = nullptr;  // h: pinned staging of one unit matrix
     GFEM_HIP(ctx, dalloc(&vals, (size_t)nnz));
     hipError_t e = dalloc(&rhs, (size_t)(n * sys->nrhs));
@@ -606,8 +624,9 @@ extern "C" gfem_status gfem_mg_setup(gfem_system *sys, int n_shift, const int *s
     if(e == hipSuccess) e = dalloc(&xs_dev.p, (size_t)std::max<int64_t>(nnz, 1));
     gfem_status st = e == hipSuccess ? GFEM_OK : hip_fail(ctx, e, "mg_setup: buffers");
     for(int t = 0; t < nt && st == GFEM_OK; ++t) {
-      for(int u = 0; u < nt && st == GFEM_OK; ++u)
-        st = gfem_term_update(sys, terms[u].first, terms[u].second, u == t ? 1. : 0., 0., 0, nullptr, nullptr);
+      for(int u = 0; u < nt; ++u)
+        for(const auto &s : cls[u])
+          if(st == GFEM_OK) st = gfem_term_update(sys, s.first, s.second, u == t ? 1. : 0., 0., 0, nullptr, nullptr);
       if(st == GFEM_OK) st = gfem_assemble_into(sys, GFEM_PREC_FP64, vals, rhs, nullptr);
       if(st == GFEM_OK) {
         e = hipMemcpy(h, vals, sizeof(double2) * (size_t)nnz, hipMemcpyDeviceToHost);
@@ -625,9 +644,9 @@ extern "C" gfem_status gfem_mg_setup(gfem_system *sys, int n_shift, const int *s
         });
       timer.mark("mg: unit matrix converted");
     }
-    for(int u = 0; u < nt; ++u) {
-      const gfem_status r = gfem_term_update(sys, terms[u].first, terms[u].second, saved[u].real(), saved[u].imag(), 0,
-                                             nullptr, nullptr);
+    for(size_t i = 0; i < all.size(); ++i) {
+      const gfem_status r =
+        gfem_term_update(sys, all[i].first, all[i].second, saved[i].real(), saved[i].imag(), 0, nullptr, nullptr);
       if(st == GFEM_OK) st = r;
     }
     dfree(vals);
@@ -716,9 +735,9 @@ extern "C" gfem_status gfem_mg_setup(gfem_system *sys, int n_shift, const int *s
   m->nterms = nt;
   m->shift = shift;
   m->beta = beta;
-  for(int t = 0; t < nt; ++t) {
-    m->term_group[t] = terms[t].first;
-    m->term_index[t] = terms[t].second;
+  for(int t = 0; t < nt; ++t) {  // a class's coefficient: its first member's
+    m->term_group[t] = cls[t][0].first;
+    m->term_index[t] = cls[t][0].second;
   }
   m->lsmall = nlev - 1;
   for(int l = nlev - 2; l >= 0; --l)

@@ -450,11 +450,14 @@ std::vector<int> Formulation<T>::setShiftedLaplacian(unsigned int tag, double be
   if(!_sys) throw common::Exception("setShiftedLaplacian: the system exists after the first assemble()");
   if(tag >= _terms.size() || !_terms[tag].bilinear)
     throw common::Exception("setShiftedLaplacian: unknown bilinear term " + std::to_string(tag));
-  if(_termSlots[tag].size() != 1)
-    throw common::Exception("setShiftedLaplacian: the shifted term must sit on one element type and entity");
-  const int g = _termSlots[tag][0].first, t = _termSlots[tag][0].second;
+  if(_termSlots[tag].empty()) throw common::Exception("setShiftedLaplacian: the term has no elements");
+  std::vector<int> g, t;  // every (group, term) slot of the tag: one class of equal coefficients
+  for(const auto &s : _termSlots[tag]) {
+    g.push_back(s.first);
+    t.push_back(s.second);
+  }
   gfem_ctx *ctx = common::deviceContext();
-  if(gfem_mg_setup(_sys, 1, &g, &t, beta) != GFEM_OK)
+  if(gfem_mg_setup(_sys, (int)g.size(), g.data(), t.data(), beta) != GFEM_OK)
     throw common::Exception(std::string("gfem_mg_setup: ") + gfem_ctx_last_error(ctx));
   int levels = 0, rows[8] = {0};
   gfem_mg_info(_sys, &levels, rows, nullptr);

@@ -386,7 +386,9 @@ typedef struct gfem_solve_info {
  * cycle starts from the FP64 residual; stops at rel_residual <= tol or after
  * max_iter inner iterations.  x stays on the device (gfem_solution_*).
  * General (non-symmetric) storage only.  A system with a multigrid hierarchy
- * (gfem_mg_setup) is preconditioned by it. */
+ * (gfem_mg_setup) is preconditioned by it; the cycle's levels take the
+ * system's current term coefficients, those of the last assembly unless
+ * gfem_term_update was called since. */
 gfem_status gfem_solve(gfem_system *sys, int rhs_idx, double tol, int restart, int max_iter, void *stream,
                        gfem_solve_info *info);
 /* Copy the last solution (n scalars of the system's FP64 type) to the host. */
@@ -425,9 +427,15 @@ gfem_status gfem_batch_result(gfem_batch *batch, int64_t id, void *x_host, size_
  * factors A exactly (fem-master/src/system/Solver.cpp:106-115, MUMPS LU): it
  * serves the iterative replacement of that solve.  One V-cycle of
  *   S = A + i beta c_s X_s
- * preconditions GMRES on the right, where (group, term) names the shifted
- * constant bilinear term with coefficient c_s and unit matrix X_s.  For the
- * mass term of a Helmholtz formulation S = K - (1 + i beta) k^2 M - i k R.
+ * preconditions GMRES on the right, where the n_shift (group, term) slots
+ * name the shifted constant bilinear term with coefficient c_s and unit
+ * matrix X_s.  For the mass term of a Helmholtz formulation
+ * S = K - (1 + i beta) k^2 M - i k R.  Constant bilinear terms with equal
+ * coefficient, forms and test-function factor at set-up form one class
+ * with one unit matrix (an integral() over several element groups); each
+ * request takes a class's coefficient from its first term, so the terms of
+ * a class must keep equal coefficients (a different one only weakens the
+ * preconditioner).  The shifted slots must form one class.
  * The hierarchy is built once, on the host from the device maps:
  *   - geometric levels on the DOF positions: trilinear interpolation from
  *     uniform grids of twice, four times, ... the mean DOF spacing, down to
@@ -439,8 +447,8 @@ gfem_status gfem_batch_result(gfem_batch *batch, int64_t id, void *x_host, size_
  * of the system: it then runs the last FP64 assembly as the only request of
  * a one-slot batch that the system keeps (same algorithm and result as
  * gfem_batch_submit of that assembly).
- * Lagrange fields with constant bilinear coefficients only (else
- * GFEM_ERR_UNSUPPORTED).  n_shift must be 1. */
+ * Lagrange fields with constant bilinear coefficients only, at most 8
+ * classes (else GFEM_ERR_UNSUPPORTED). */
 gfem_status gfem_mg_setup(gfem_system *sys, int n_shift, const int *shift_group, const int *shift_term, double beta);
 /* Levels of the hierarchy (0 without one), rows and stored entries per
  * level (arrays of at least 8; the coarsest level is dense). */
