@@ -1310,6 +1310,7 @@ gfem_status submit_request(gfem_batch *B, gfem_system *sys, double tol, int max_
   S.live = true;
   S.id = B->next_id++;
   S.sys = sys;
+  ++sys->batch_refs;
   S.off = off;
   S.tol = tol;
   S.max_iter = max_iter;
@@ -1368,6 +1369,7 @@ gfem_status submit_request(gfem_batch *B, gfem_system *sys, double tol, int max_
   if(st != GFEM_OK) {
     B->where.erase(S.id);
     B->arena.release(off);
+    --sys->batch_refs;
     S = gfem_batch::Slot();
     return st;
   }
@@ -1486,6 +1488,8 @@ gfem_status gfem_batch_destroy(gfem_batch *B)
   if(!B) return GFEM_OK;
   (void)hipSetDevice(B->ctx->device);
   (void)hipDeviceSynchronize();
+  for(auto &S : B->slots)  // requests never fetched release their system
+    if(S.used && S.sys) --S.sys->batch_refs;
   dfree(B->d_items);
   dfree(B->d_act);
   dfree(B->d_live);
@@ -1664,6 +1668,7 @@ gfem_status gfem_batch_result(gfem_batch *B, int64_t id, void *x_host, size_t by
   GFEM_HIP(ctx, hipStreamSynchronize(nullptr));
   B->arena.release(S.off);
   B->where.erase(w);
+  if(S.sys) --S.sys->batch_refs;
   S = gfem_batch::Slot();
   return GFEM_OK;
 }

@@ -312,6 +312,7 @@ struct gfem_system {
   // shifted-Laplacian multigrid hierarchy (gfem_mg_setup), or null; gfem_solve
   // then runs the system as the only request of mg_batch (restart mg_batch_m)
   gfem::MgSystem *mg = nullptr;
+  int batch_refs = 0;  // requests of this system holding a batch slot (they read mg)
   gfem_batch *mg_batch = nullptr;
   int mg_batch_m = 0;
   size_t mg_batch_bytes = 0;

@@ -30,7 +30,7 @@ constexpr int kMgMaxTerms = 8;   // classes of constant bilinear terms combined 
 constexpr int kMgCoarsest = 5;   // grid points per axis of the coarsest level, at most
 constexpr int kMgSmall = 512;    // levels of at most this many rows run inside one block per request
 constexpr int kMgSmallThreads = 1024;
-constexpr float kMgOmega = 0.5f;
+constexpr float kMgOmega = 0.5f;  // Jacobi damping
 
 // One level of a request's cycle (device pointers; level 0: the system's
 // pattern, its values and right-hand side come from the Krylov vector).

@@ -526,6 +526,8 @@ extern "C" gfem_status gfem_mg_setup(gfem_system *sys, int n_shift, const int *s
   if(n_shift < 1 || !shift_group || !shift_term)
     return fail(ctx, GFEM_ERR_INVALID_ARG, "mg_setup: at least one shifted (group, term)");
   if(!(beta > 0.)) return fail(ctx, GFEM_ERR_INVALID_ARG, "mg_setup: beta > 0");
+  if(sys->batch_refs > 0)  // their cycles read the hierarchy this call replaces
+    return fail(ctx, GFEM_ERR_STATE, "mg_setup: requests of this system are in a batch");
   if(sys->n >= (int64_t(1) << 31) || sys->nnz >= (int64_t(1) << 31))
     return fail(ctx, GFEM_ERR_TOO_LARGE, "mg_setup: system too large");
   GFEM_HIP(ctx, hipSetDevice(ctx->device));

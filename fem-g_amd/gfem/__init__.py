@@ -19,7 +19,7 @@ except ImportError as e:  # pragma: no cover - exercised only on a broken build
 from ._gfem_host import (Domain, Field, ManyInstance, Model, ScalarFunction, Server, Unknown, abs,  # noqa: F401,E402
                          assemble_many, cos, dof, dt2_dof, dt_dof, exp, from_arrays, gauss_rule, grad,
                          hier_basis, hier_keys, hilbert_index, hilbert_key, lattice_nodes, pow, read_msh,
-                         set_current_model, set_device,
+                         set_current_model, set_device, device_context_handle,
                          sin, sqrt, tf, unit_cube, unit_square, write_msh, x, y, z)
 
 _FORMULATIONS = {

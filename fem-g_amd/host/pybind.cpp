@@ -329,6 +329,10 @@ PYBIND11_MODULE(_gfem_host, m)
 {
   m.doc() = "gfem host API: GmshFEM Formulation over the MI355X assembly engine";
   m.def("set_device", &common::setDevice);
+  m.def("device_context_handle", []() {
+    // the gfem_ctx* the formulations use, for calls through the C ABI
+    return reinterpret_cast<uintptr_t>(common::deviceContext());
+  });
   py::class_<ManyInstance, std::shared_ptr<ManyInstance>>(m, "ManyInstance")
     .def("values", [](ManyInstance &i, const std::string &p) { return i.fetch((int)parse_precision(p), false, 0); },
          py::arg("precision") = "fp64")

@@ -448,7 +448,10 @@ gfem_status gfem_batch_result(gfem_batch *batch, int64_t id, void *x_host, size_
  * a one-slot batch that the system keeps (same algorithm and result as
  * gfem_batch_submit of that assembly).
  * Lagrange fields with constant bilinear coefficients only, at most 8
- * classes (else GFEM_ERR_UNSUPPORTED). */
+ * classes (else GFEM_ERR_UNSUPPORTED).  GFEM_ERR_STATE while requests of the
+ * system hold batch slots (fetch their results first): their cycles read
+ * the hierarchy a new set-up replaces.  A batch must be destroyed before
+ * the systems whose requests it still holds. */
 gfem_status gfem_mg_setup(gfem_system *sys, int n_shift, const int *shift_group, const int *shift_term, double beta);
 /* Levels of the hierarchy (0 without one), rows and stored entries per
  * level (arrays of at least 8; the coarsest level is dense). */

@@ -182,3 +182,45 @@ def test_mg_single_solve():
     b = sv.run(_schedule([(0, k, c)]), keep_solutions=True)[0]
     assert b["iterations"] == r["iterations"]
     assert np.array_equal(b["x"], x)
+
+
+def test_mg_setup_waits_for_batched_requests():
+    """a new hierarchy is refused while a request of the system holds a batch
+    slot (its cycle reads the hierarchy), and accepted once the result is
+    fetched; the request itself converges"""
+    import ctypes
+
+    import gfem
+    from gfem import workloads as W
+    print("case mg set-up with a request in flight", flush=True)
+    lib = gfem.load_c_abi()
+    F = _resident((4, 2))
+    for tag, f in W.helmholtz_params(9.0, (0.5, 0.5, 0.5)):
+        F.set_term_coefficient(tag, f)
+    F.assemble(download=False)
+    F.set_shifted_laplacian(W.HELMHOLTZ_MASS)
+    ctx = ctypes.c_void_p(gfem.device_context_handle())
+    sysh = ctypes.c_void_p(F.system_handle())
+    B = ctypes.c_void_p()
+    assert lib.gfem_batch_create(ctx, 30, ctypes.c_size_t(1 << 30), 2, ctypes.byref(B)) == 0
+    try:
+        rid = ctypes.c_int64()
+        assert lib.gfem_batch_submit(B, sysh, ctypes.c_double(1e-10), 10000, None, ctypes.byref(rid)) == 0
+        with pytest.raises(Exception):
+            F.set_shifted_laplacian(W.HELMHOLTZ_MASS)
+        live = ctypes.c_int(1)
+        steps = 0
+        while live.value and steps < 1000:
+            assert lib.gfem_batch_step(B, None, ctypes.byref(live)) == 0
+            steps += 1
+        assert live.value == 0
+
+        class Info(ctypes.Structure):
+            _fields_ = [("iterations", ctypes.c_int), ("cycles", ctypes.c_int), ("rel_residual", ctypes.c_double),
+                        ("converged", ctypes.c_int)]
+        info = Info()
+        assert lib.gfem_batch_result(B, rid, None, ctypes.c_size_t(0), ctypes.byref(info)) == 0
+        assert info.converged == 1 and info.rel_residual <= 1e-10
+        F.set_shifted_laplacian(W.HELMHOLTZ_MASS)  # nothing in flight now
+    finally:
+        lib.gfem_batch_destroy(B)
