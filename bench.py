@@ -143,7 +143,7 @@ def serving(args):
         F = W.build(W.helmholtz3d(n=n, order=order, k=10.0))
         F.pre()
         Fs.append(F)
-    sv = Server(restart=30, hbm_bytes=args.serve_hbm_gb << 30, max_requests=256)
+    sv = Server(restart=args.serve_restart, hbm_bytes=args.serve_hbm_gb << 30, max_requests=256)
     for F in Fs:
         sv.add_mesh(F)
     log(f"serving: {len(Fs)} meshes resident ({time.perf_counter() - t0:.1f} s)")
@@ -175,6 +175,7 @@ def serving(args):
     st = sv.stats
     res = {
         "requests": len(out), "arrival_rate_per_s": args.serve_rate, "tol": args.serve_tol,
+        "restart": args.serve_restart,
         "mesh_dofs": [int(F.n_unknown) for F in Fs],
         "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
         "solves_per_s": len(out) / st["seconds"], "seconds": st["seconds"],
@@ -225,7 +226,7 @@ def serving(args):
         for tag, f in s["params"]:
             F.set_term_coefficient(tag, f)
         F.assemble(download=False)
-        F.solve(tol=args.serve_tol, restart=30, max_iter=30000, download=False)
+        F.solve(tol=args.serve_tol, restart=args.serve_restart, max_iter=30000, download=False)
     dt = time.perf_counter() - t1
     res["one_at_a_time_solves_per_s"] = nb / dt
     res["one_at_a_time_sample"] = nb
@@ -237,7 +238,7 @@ def serving(args):
     import scipy.sparse.linalg as sla
     small = min(range(len(Fs)), key=lambda i: Fs[i].n_unknown)
     F = Fs[small]
-    chk = Server(restart=30, hbm_bytes=1 << 30, max_requests=4)
+    chk = Server(restart=args.serve_restart, hbm_bytes=1 << 30, max_requests=4)
     chk.add_mesh(F)
     errs = []
     for s in [s for s in sched if s["mesh"] == small][:3]:
@@ -492,6 +493,7 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-tol", type=float, default=1e-10)
     ap.add_argument("--serve-hbm-gb", type=int, default=64)
+    ap.add_argument("--serve-restart", type=int, default=30, help="GMRES restart length of the serving stream")
     ap.add_argument("--serve-baseline", type=int, default=16, help="requests timed one at a time")
     ap.add_argument("--serve-mg", type=int, choices=[0, 1], default=1,
                     help="shifted-Laplacian multigrid preconditioner on the serving meshes (1) or row scaling (0)")
