@@ -35,6 +35,7 @@ namespace gfem {
 namespace {
 
 constexpr int kLanes = 8;                       // lanes per CSR row
+constexpr int kTerms = 4;                       // terms of a lane in flight together (row kernels)
 constexpr int kXcds = 8;                        // XCDs of an MI355X
 constexpr int kRowsPerBlock = 256 / kLanes;      // 32 rows per 256-thread block
 constexpr int kElemsPerBlock = 256;              // vector kernels: one element per thread
@@ -126,7 +127,7 @@ __global__ void __launch_bounds__(256) bk_spmv32(const BItem *__restrict__ items
   float sr = 0.f, si = 0.f;
   // kU terms of the lane at a time (columns and values, then the x entries,
   // in flight together), summed in term order
-  constexpr int kU = 4;
+  constexpr int kU = kTerms;
   const int64_t k1 = I.ai[row + 1];
   for(int64_t k0 = I.ai[row] + l; k0 < k1; k0 += kU * kLanes) {
     int32_t c[kU];
@@ -161,10 +162,27 @@ __global__ void __launch_bounds__(256) bk_residual(const BItem *__restrict__ ite
   const int l = threadIdx.x % kLanes;
   if(row >= I.n) return;
   double2 s = make_double2(0., 0.);
-  for(int64_t k = I.ai[row] + l; k < I.ai[row + 1]; k += kLanes) {
-    const double2 p = zmul(I.a64[k], I.x[I.aj[k]]);
-    s.x += p.x;
-    s.y += p.y;
+  // kU terms of the lane at a time, summed in term order
+  constexpr int kU = kTerms;
+  const int64_t k1 = I.ai[row + 1];
+  for(int64_t k0 = I.ai[row] + l; k0 < k1; k0 += kU * kLanes) {
+    int32_t c[kU];
+    double2 a[kU], v[kU];
+#pragma unroll
+    for(int u = 0; u < kU; ++u) {
+      const bool in = k0 + u * kLanes < k1;
+      c[u] = in ? I.aj[k0 + u * kLanes] : 0;
+      a[u] = in ? I.a64[k0 + u * kLanes] : make_double2(0., 0.);
+    }
+#pragma unroll
+    for(int u = 0; u < kU; ++u) v[u] = k0 + u * kLanes < k1 ? I.x[c[u]] : make_double2(0., 0.);
+#pragma unroll
+    for(int u = 0; u < kU; ++u)
+      if(k0 + u * kLanes < k1) {
+        const double2 p = zmul(a[u], v[u]);
+        s.x += p.x;
+        s.y += p.y;
+      }
   }
   s = lanes_sum(s);
   if(l == 0) I.r[row] = make_double2(-I.b[row].x - s.x, -I.b[row].y - s.y);
@@ -571,7 +589,7 @@ __device__ __forceinline__ int64_t mg_row0(const MgLevelReq &L, int64_t r)
 template <bool DB>
 __device__ __forceinline__ float2 mg_row_dot(const MgLevelReq &L, int64_t r, const float2 *v, int lane)
 {
-  constexpr int kU = 4;
+  constexpr int kU = kTerms;
   float sr = 0.f, si = 0.f;
   const int64_t k1 = mg_row0(L, r + 1);
   for(int64_t k0 = mg_row0(L, r) + lane; k0 < k1; k0 += kU * kLanes) {
@@ -595,6 +613,37 @@ __device__ __forceinline__ float2 mg_row_dot(const MgLevelReq &L, int64_t r, con
         const float2 p = cmulf(a[u], DB ? cmulf(d[u], x[u]) : x[u]);
         sr += p.x;
         si += p.y;
+      }
+  }
+  return make_float2(sr, si);
+}
+
+// sum over row r of a transfer operator (P or its transpose: pointers, columns,
+// real weights) times v, lanes of kLanes terms apart starting at `lane`,
+// kTerms terms of the lane at a time, summed in term order
+__device__ __forceinline__ float2 mg_transfer_dot(const int32_t *ptr, const int32_t *idx, const float *w,
+                                                  const float2 *v, int64_t r, int lane)
+{
+  constexpr int kU = kTerms;
+  float sr = 0.f, si = 0.f;
+  const int32_t k1 = ptr[r + 1];
+  for(int32_t k0 = ptr[r] + lane; k0 < k1; k0 += kU * kLanes) {
+    int32_t c[kU];
+    float wt[kU];
+    float2 x[kU];
+#pragma unroll
+    for(int u = 0; u < kU; ++u) {
+      const bool in = k0 + u * kLanes < k1;
+      c[u] = in ? idx[k0 + u * kLanes] : 0;
+      wt[u] = in ? w[k0 + u * kLanes] : 0.f;
+    }
+#pragma unroll
+    for(int u = 0; u < kU; ++u) x[u] = k0 + u * kLanes < k1 ? v[c[u]] : make_float2(0.f, 0.f);
+#pragma unroll
+    for(int u = 0; u < kU; ++u)
+      if(k0 + u * kLanes < k1) {
+        sr += wt[u] * x[u].x;
+        si += wt[u] * x[u].y;
       }
   }
   return make_float2(sr, si);
@@ -653,14 +702,7 @@ __global__ void __launch_bounds__(256) bk_mg_restrict(const BItem *__restrict__ 
   const int64_t row = (int64_t)bb.y + threadIdx.x / kLanes;
   const int lane = threadIdx.x % kLanes;
   if(row >= C.n) return;
-  float sr = 0.f, si = 0.f;
-  for(int32_t k = L.ri[row] + lane; k < L.ri[row + 1]; k += kLanes) {
-    const float w = L.rw[k];
-    const float2 v = L.r[L.rj[k]];
-    sr += w * v.x;
-    si += w * v.y;
-  }
-  const float2 s = lanes_sumf(make_float2(sr, si));
+  const float2 s = lanes_sumf(mg_transfer_dot(L.ri, L.rj, L.rw, L.r, row, lane));
   if(lane == 0) C.b[row] = s;
 }
 
@@ -683,14 +725,7 @@ __global__ void __launch_bounds__(256) bk_mg_prolong(const BItem *__restrict__ i
   const int lane = threadIdx.x % kLanes;
   if(row >= L.n) return;
   const float2 *xc = mg_coarse_x(*I.mg, l);
-  float sr = 0.f, si = 0.f;
-  for(int32_t k = L.pi[row] + lane; k < L.pi[row + 1]; k += kLanes) {
-    const float w = L.pw[k];
-    const float2 v = xc[L.pj[k]];
-    sr += w * v.x;
-    si += w * v.y;
-  }
-  const float2 s = lanes_sumf(make_float2(sr, si));
+  const float2 s = lanes_sumf(mg_transfer_dot(L.pi, L.pj, L.pw, xc, row, lane));
   if(lane == 0) L.x[row] = make_float2(L.x[row].x + s.x, L.x[row].y + s.y);
 }
 
@@ -748,15 +783,8 @@ __global__ void __launch_bounds__(kMgSmallThreads) bk_mg_small(const BItem *__re
     __syncthreads();
     for(int64_t base = 0; base < C.n; base += kRows) {
       const int64_t r = base + r0;
-      float sr = 0.f, si = 0.f;
-      if(r < C.n)
-        for(int32_t k = L.ri[r] + lane; k < L.ri[r + 1]; k += kLanes) {
-          const float w = L.rw[k];
-          const float2 v = L.r[L.rj[k]];
-          sr += w * v.x;
-          si += w * v.y;
-        }
-      const float2 s = lanes_sumf(make_float2(sr, si));
+      const float2 s =
+        lanes_sumf(r < C.n ? mg_transfer_dot(L.ri, L.rj, L.rw, L.r, r, lane) : make_float2(0.f, 0.f));
       if(r < C.n && lane == 0) C.b[r] = s;
     }
     __syncthreads();
@@ -781,15 +809,7 @@ __global__ void __launch_bounds__(kMgSmallThreads) bk_mg_small(const BItem *__re
     const float2 *xc = mg_coarse_x(M, l);
     for(int64_t base = 0; base < L.n; base += kRows) {
       const int64_t r = base + r0;
-      float sr = 0.f, si = 0.f;
-      if(r < L.n)
-        for(int32_t k = L.pi[r] + lane; k < L.pi[r + 1]; k += kLanes) {
-          const float w = L.pw[k];
-          const float2 v = xc[L.pj[k]];
-          sr += w * v.x;
-          si += w * v.y;
-        }
-      const float2 s = lanes_sumf(make_float2(sr, si));
+      const float2 s = lanes_sumf(r < L.n ? mg_transfer_dot(L.pi, L.pj, L.pw, xc, r, lane) : make_float2(0.f, 0.f));
       if(r < L.n && lane == 0) L.x[r] = make_float2(L.x[r].x + s.x, L.x[r].y + s.y);
     }
     __syncthreads();
