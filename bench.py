@@ -155,7 +155,7 @@ def serving(args):
         t_mg = time.perf_counter() - t1
         log(f"serving: multigrid hierarchies in {t_mg:.1f} s, levels {[len(l) for l in mg_levels]}")
     rng = np.random.default_rng(2024)
-    if args.serve_stated:
+    if args.serve_requests is None:
         # the whole stated stream: every arrival of 60 s at the rate
         arrivals = np.cumsum(rng.exponential(1.0 / args.serve_rate, int(args.serve_rate * 66) + 100))
         arrivals = arrivals[arrivals < 60.0]
@@ -188,8 +188,9 @@ def serving(args):
     if args.serve_mg:
         res["multigrid"] = {"setup_s": t_mg, "rows_per_level": mg_levels}
     # the stated stream is Poisson arrivals at 200/s for 60 s (about 12,000
-    # requests); this leg serves its first serve_requests arrivals, a reduced
-    # configuration, and reports the steady-state rate and the queue it built
+    # requests), served whole by default (or its first serve_requests
+    # arrivals, a reduced configuration); report the steady-state rate and
+    # the queue it built
     arr = np.array([r["arrival"] for r in out])
     adm = np.array([r["admitted"] for r in out])
     done = np.array([r["arrival"] + r["latency"] for r in out])
@@ -202,15 +203,15 @@ def serving(args):
     in_system = (n_by(arr) - n_by(done)).tolist()
     stated = args.serve_rate * 60.0
     res.update({
-        "config": "stated" if args.serve_stated else "reduced",
+        "config": "stated" if args.serve_requests is None else "reduced",
         "stated_config": {"arrival_rate_per_s": args.serve_rate, "seconds": 60.0, "requests": int(stated),
-                          "measured": bool(args.serve_stated),
+                          "measured": args.serve_requests is None,
                           "projected_seconds_at_sustained_rate": None},
         "sustained_solves_per_s": float(inside.sum() / max(t_hi - t_lo, 1e-9)),
         "sustained_window_s": [float(t_lo), float(t_hi)],
         "queue_depth_max": max(waiting), "queue_depth_mean": float(np.mean(waiting)),
         "in_system_max": max(in_system),
-        "what": (f"{'all' if args.serve_stated else 'first'} {len(out)} requests of the stated stream (Poisson "
+        "what": (f"{'all' if args.serve_requests is None else 'first'} {len(out)} requests of the stated stream (Poisson "
                  f"arrivals at {args.serve_rate:g}/s for 60 s); sustained rate = completions between the 10th and "
                  "90th percentile completion times"),
     })
@@ -487,9 +488,9 @@ def main():
     ap.add_argument("--batch-instances", type=int, default=256)
     ap.add_argument("--batch-tol", type=float, default=1e-8)
     ap.add_argument("--batch-hbm-gb", type=int, default=96)
-    ap.add_argument("--serve-requests", type=int, default=256)
-    ap.add_argument("--serve-stated", action="store_true",
-                    help="serve the whole stated stream (60 s of arrivals) instead of --serve-requests")
+    ap.add_argument("--serve-requests", type=int, default=None,
+                    help="serve only the first N arrivals (a reduced stream); default: the whole stated stream, "
+                         "60 s of arrivals")
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-tol", type=float, default=1e-10)
     ap.add_argument("--serve-hbm-gb", type=int, default=64)
