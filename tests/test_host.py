@@ -196,3 +196,19 @@ def test_term_coefficient_keeps_its_form():
         F.set_term_coefficient(tag, f)
     with pytest.raises(Exception, match="unknown term"):
         F.set_term_coefficient(17, gfem.ScalarFunction(1.0))
+
+
+@pytest.mark.parametrize("n,order", [(3, 1), (2, 2), (4, 2)])
+def test_scatter_map_bytes_counts_the_element_maps(n, order):
+    """the roofline's algorithmic bytes of the matrix scatter: 4 B per local
+    DOF and 4 B per local pair of every element of a bilinear term, the
+    Helmholtz cube having n^3 x 6 tetrahedra and 6 n^2 x 2 Robin triangles"""
+    spec = W.helmholtz3d(n=n, order=order, k=5.0)
+    Nt = (order + 1) * (order + 2) * (order + 3) // 6  # DOFs of a tetrahedron
+    Nf = (order + 1) * (order + 2) // 2                 # of a triangle
+    tets, tris = 6 * n ** 3, 12 * n ** 2
+    assert spec.scatter_map_bytes() == tets * (4 * Nt + 4 * Nt * Nt) + tris * (4 * Nf + 4 * Nf * Nf)
+    # the linear-only domains add nothing: the same mesh with the source term alone
+    only_src = W.helmholtz3d(n=n, order=order, k=5.0)
+    only_src.terms = [t for t in only_src.terms if t["kind"] == "linear"]
+    assert only_src.scatter_map_bytes() == 0
