@@ -36,6 +36,7 @@ namespace {
 
 constexpr int kLanes = 8;                       // lanes per CSR row
 constexpr int kTerms = 4;                       // terms of a lane in flight together (row kernels)
+constexpr int kSmallTerms = 8;                  // the same in the one-block V-cycle kernel (same sums)
 constexpr int kXcds = 8;                        // XCDs of an MI355X
 constexpr int kRowsPerBlock = 256 / kLanes;      // 32 rows per 256-thread block
 constexpr int kElemsPerBlock = 256;              // vector kernels: one element per thread
@@ -586,10 +587,9 @@ __device__ __forceinline__ int64_t mg_row0(const MgLevelReq &L, int64_t r)
 // computed on the fly), lanes of kLanes terms apart starting at `lane`; kU
 // terms of the lane at a time (columns and values, then the v and D entries,
 // in flight together), summed in term order
-template <bool DB>
+template <bool DB, int kU = kTerms>
 __device__ __forceinline__ float2 mg_row_dot(const MgLevelReq &L, int64_t r, const float2 *v, int lane)
 {
-  constexpr int kU = kTerms;
   float sr = 0.f, si = 0.f;
   const int64_t k1 = mg_row0(L, r + 1);
   for(int64_t k0 = mg_row0(L, r) + lane; k0 < k1; k0 += kU * kLanes) {
@@ -620,11 +620,12 @@ __device__ __forceinline__ float2 mg_row_dot(const MgLevelReq &L, int64_t r, con
 
 // sum over row r of a transfer operator (P or its transpose: pointers, columns,
 // real weights) times v, lanes of kLanes terms apart starting at `lane`,
-// kTerms terms of the lane at a time, summed in term order
+// kU terms of the lane at a time, summed in term order (the sum does not
+// depend on kU)
+template <int kU = kTerms>
 __device__ __forceinline__ float2 mg_transfer_dot(const int32_t *ptr, const int32_t *idx, const float *w,
                                                   const float2 *v, int64_t r, int lane)
 {
-  constexpr int kU = kTerms;
   float sr = 0.f, si = 0.f;
   const int32_t k1 = ptr[r + 1];
   for(int32_t k0 = ptr[r] + lane; k0 < k1; k0 += kU * kLanes) {
@@ -774,7 +775,7 @@ __global__ void __launch_bounds__(kMgSmallThreads) bk_mg_small(const BItem *__re
     const float2 *b = mg_rhs(I, l, src, j);
     for(int64_t base = 0; base < L.n; base += kRows) {
       const int64_t r = base + r0;
-      const float2 s = lanes_sumf(r < L.n ? mg_row_dot<true>(L, r, b, lane) : make_float2(0.f, 0.f));
+      const float2 s = lanes_sumf(r < L.n ? mg_row_dot<true, kSmallTerms>(L, r, b, lane) : make_float2(0.f, 0.f));
       if(r < L.n && lane == 0) {
         L.x[r] = cmulf(L.dinv[r], b[r]);
         L.r[r] = make_float2(b[r].x - s.x, b[r].y - s.y);
@@ -783,24 +784,40 @@ __global__ void __launch_bounds__(kMgSmallThreads) bk_mg_small(const BItem *__re
     __syncthreads();
     for(int64_t base = 0; base < C.n; base += kRows) {
       const int64_t r = base + r0;
-      const float2 s =
-        lanes_sumf(r < C.n ? mg_transfer_dot(L.ri, L.rj, L.rw, L.r, r, lane) : make_float2(0.f, 0.f));
+      const float2 s = lanes_sumf(r < C.n ? mg_transfer_dot<kSmallTerms>(L.ri, L.rj, L.rw, L.r, r, lane)
+                                          : make_float2(0.f, 0.f));
       if(r < C.n && lane == 0) C.b[r] = s;
     }
     __syncthreads();
   }
   {
-    // coarsest: x = S^-1 b, the inverse column-major
+    // coarsest: x = S^-1 b with the inverse column-major; each row's sum in
+    // kCg column groups (columns g, g + kCg, ...) whose loads are in flight
+    // together, the groups' partial sums added in group order
     const MgLevelReq &C = M.lev[M.nlev - 1];
     const int n = C.n;
-    for(int r = threadIdx.x; r < n; r += kMgSmallThreads) {
-      float sr = 0.f, si = 0.f;
-      for(int c = 0; c < n; ++c) {
+    constexpr int kRowSlots = 128, kCg = kMgSmallThreads / kRowSlots;
+    static_assert(kMgCoarsest * kMgCoarsest * kMgCoarsest <= kRowSlots, "coarsest rows per column group");
+    __shared__ float2 s_part[kMgSmallThreads];
+    const int r = threadIdx.x % kRowSlots, g = threadIdx.x / kRowSlots;
+    float sr = 0.f, si = 0.f;
+    if(r < n) {
+#pragma unroll 4
+      for(int c = g; c < n; c += kCg) {
         const float2 p = cmulf(M.ainv[(int64_t)c * n + r], C.b[c]);
         sr += p.x;
         si += p.y;
       }
-      C.x[r] = make_float2(sr, si);
+    }
+    s_part[threadIdx.x] = make_float2(sr, si);
+    __syncthreads();
+    if(g == 0 && r < n) {
+      float2 t = s_part[r];
+      for(int q = 1; q < kCg; ++q) {
+        t.x += s_part[q * kRowSlots + r].x;
+        t.y += s_part[q * kRowSlots + r].y;
+      }
+      C.x[r] = t;
     }
     __syncthreads();
   }
@@ -809,7 +826,8 @@ __global__ void __launch_bounds__(kMgSmallThreads) bk_mg_small(const BItem *__re
     const float2 *xc = mg_coarse_x(M, l);
     for(int64_t base = 0; base < L.n; base += kRows) {
       const int64_t r = base + r0;
-      const float2 s = lanes_sumf(r < L.n ? mg_transfer_dot(L.pi, L.pj, L.pw, xc, r, lane) : make_float2(0.f, 0.f));
+      const float2 s = lanes_sumf(r < L.n ? mg_transfer_dot<kSmallTerms>(L.pi, L.pj, L.pw, xc, r, lane)
+                                          : make_float2(0.f, 0.f));
       if(r < L.n && lane == 0) L.x[r] = make_float2(L.x[r].x + s.x, L.x[r].y + s.y);
     }
     __syncthreads();
@@ -817,7 +835,7 @@ __global__ void __launch_bounds__(kMgSmallThreads) bk_mg_small(const BItem *__re
     float2 *out = l == 0 ? I.z : L.y;
     for(int64_t base = 0; base < L.n; base += kRows) {
       const int64_t r = base + r0;
-      const float2 s = lanes_sumf(r < L.n ? mg_row_dot<false>(L, r, L.x, lane) : make_float2(0.f, 0.f));
+      const float2 s = lanes_sumf(r < L.n ? mg_row_dot<false, kSmallTerms>(L, r, L.x, lane) : make_float2(0.f, 0.f));
       if(r < L.n && lane == 0) {
         const float2 d = cmulf(L.dinv[r], make_float2(b[r].x - s.x, b[r].y - s.y));
         out[r] = make_float2(L.x[r].x + d.x, L.x[r].y + d.y);
