@@ -131,6 +131,28 @@ def test_admission_respects_the_hbm_budget():
     assert max(r["admitted"] for r in out) >= first_done
 
 
+def test_serving_reports_progress_and_queue(capfd):
+    """a run with progress lines on: stderr names arrivals, admissions and
+    completions; five arrivals at once on a budget of two requests leave
+    five waiting before the first admission"""
+    from gfem import Server
+    print("case serving progress and queue", flush=True)
+    mesh = (4, 2)
+    F = _resident(mesh)
+    probe = Server(restart=20, hbm_bytes=1 << 26, max_requests=8)
+    probe.add_mesh(F)
+    per = probe.request_bytes(0)
+    sv = Server(restart=20, hbm_bytes=2 * per + 4096, max_requests=8)
+    sv.add_mesh(F)
+    sv.set_progress(1e-6)  # a line at every pass of the loop
+    out = sv.run(_schedule([(0, 5.0 + i, (0.5, 0.5, 0.5)) for i in range(5)]), keep_solutions=False)
+    assert len(out) == 5 and all(r["converged"] for r in out)
+    assert sv.stats["max_waiting"] == 5
+    lines = [ln for ln in capfd.readouterr().err.splitlines() if ln.startswith("serve:")]
+    assert lines and "arrived 5" in lines[-1]
+    assert any("waiting 3" in ln for ln in lines)  # two admitted, three held back by the budget
+
+
 def test_graph_and_eager_batches_agree(monkeypatch):
     """The batched Krylov iterations replayed as HIP graphs (default) and
     launched one kernel at a time (GFEM_BATCH_GRAPH=0) give bitwise equal
