@@ -666,7 +666,24 @@ __device__ __forceinline__ float2 lanes_sumf(float2 v)
   return make_float2((float)d.x, (float)d.y);
 }
 
-// x = D b, r = b - S x
+// level 0's pre-smoothed iterate x = D b, one thread per row (requests that
+// run level 0 row-parallel), so that bk_mg_down gathers x alone
+__global__ void __launch_bounds__(256) bk_mg_x0(const BItem *__restrict__ items, const int *__restrict__ act,
+                                                const int2 *__restrict__ vblk, int src, int j,
+                                                const int *__restrict__ dj)
+{
+  if(dj) j = *dj;
+  const int2 bb = vblk[blockIdx.x];
+  if(!act[bb.x]) return;
+  const BItem &I = items[bb.x];
+  if(!I.mg || I.mg->lsmall == 0) return;
+  const int64_t i = (int64_t)bb.y + threadIdx.x;
+  if(i >= I.n) return;
+  const MgLevelReq &L = I.mg->lev[0];
+  L.x[i] = cmulf(L.dinv[i], mg_rhs(I, 0, src, j)[i]);
+}
+
+// x = D b, r = b - S x (level 0: x formed by bk_mg_x0)
 __global__ void __launch_bounds__(256) bk_mg_down(const BItem *__restrict__ items, const int *__restrict__ act,
                                                   const int2 *__restrict__ blk, int nblk, int l, int src, int j,
                                                   const int *__restrict__ dj)
@@ -682,6 +699,11 @@ __global__ void __launch_bounds__(256) bk_mg_down(const BItem *__restrict__ item
   const int lane = threadIdx.x % kLanes;
   if(row >= L.n) return;
   const float2 *b = mg_rhs(I, l, src, j);
+  if(l == 0) {  // one gather per term
+    const float2 s = lanes_sumf(mg_row_dot<false>(L, row, L.x, lane));
+    if(lane == 0) L.r[row] = make_float2(b[row].x - s.x, b[row].y - s.y);
+    return;
+  }
   const float2 s = lanes_sumf(mg_row_dot<true>(L, row, b, lane));
   if(lane == 0) {
     const float2 br = b[row];
@@ -1166,6 +1188,8 @@ void launch_mg(gfem_batch *B, hipStream_t s, const int *act, int src, int j, con
 {
   if(B->n_mg == 0) return;
   const int ns = (int)B->slots.size();
+  if(B->mg_big > 0)
+    hipLaunchKernelGGL(bk_mg_x0, dim3((unsigned)B->n_vblk), dim3(256), 0, s, B->d_items, act, B->d_vblk, src, j, dj);
   for(int l = 0; l < B->mg_big; ++l) {
     if(B->n_mgblk[l])
       hipLaunchKernelGGL(bk_mg_down, dim3(xcd_grid(B->n_mgblk[l])), dim3(256), 0, s, B->d_items, act, B->d_mgblk[l],
