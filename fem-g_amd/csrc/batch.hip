@@ -867,6 +867,106 @@ __global__ void __launch_bounds__(kMgSmallThreads) bk_mg_small(const BItem *__re
   }
 }
 
+// Inverse of the coarsest level's S = sum_t c_t G_t (dense, n <= kMgInvMax),
+// one block per request admitted since the last step (act): in-place
+// Gauss-Jordan with partial pivoting on the complex64 matrix held in LDS (row
+// swaps recorded, columns swapped back at the end), written column-major so
+// the cycle's coarse product reads it with consecutive lanes on consecutive
+// rows.
+__global__ void __launch_bounds__(kMgInvThreads) bk_mg_invert(const BItem *__restrict__ items,
+                                                              const int *__restrict__ act)
+{
+  const int it = blockIdx.x;
+  if(!act[it] || !items[it].mg) return;
+  const MgReq &M = *items[it].mg;
+  const int n = M.lev[M.nlev - 1].n;
+  const float2 *gd = M.gdense;
+  float2 *ainv = M.ainv;
+  __shared__ float2 A[kMgInvMax * kMgInvMax];
+  __shared__ float2 s_f[kMgInvMax];  // the multipliers of the column being eliminated
+  __shared__ int s_perm[kMgInvMax];
+  const int t = threadIdx.x;
+  for(int e = t; e < n * n; e += kMgInvThreads) {
+    float re = 0.f, im = 0.f;
+    for(int k = 0; k < M.cf.nt; ++k) {
+      const float2 g = gd[(int64_t)k * n * n + e], q = M.cf.c[k];
+      re += q.x * g.x - q.y * g.y;
+      im += q.x * g.y + q.y * g.x;
+    }
+    A[e] = make_float2(re, im);
+  }
+  __syncthreads();
+  for(int c = 0; c < n; ++c) {
+    // pivot: largest |A[r][c]|, r >= c (one wave, smallest index on ties)
+    if(t < 64) {
+      float best = -1.f;
+      int bi = c;
+      for(int r = c + t; r < n; r += 64) {
+        const float2 v = A[r * n + c];
+        const float m = v.x * v.x + v.y * v.y;
+        if(m > best) {
+          best = m;
+          bi = r;
+        }
+      }
+      for(int d = 32; d > 0; d >>= 1) {
+        const float ob = __shfl_down(best, d, 64);
+        const int oi = __shfl_down(bi, d, 64);
+        if(t + d < 64 && (ob > best || (ob == best && oi < bi))) {
+          best = ob;
+          bi = oi;
+        }
+      }
+      if(t == 0) s_perm[c] = bi;
+    }
+    __syncthreads();
+    const int p = s_perm[c];
+    if(p != c)
+      for(int k = t; k < n; k += kMgInvThreads) {
+        const float2 x = A[c * n + k];
+        A[c * n + k] = A[p * n + k];
+        A[p * n + k] = x;
+      }
+    __syncthreads();
+    const float2 d = A[c * n + c];
+    const float dm = d.x * d.x + d.y * d.y;
+    const float2 inv = dm > 0.f ? make_float2(d.x / dm, -d.y / dm) : make_float2(0.f, 0.f);
+    // the other rows' multipliers, read before any of those rows changes
+    for(int r = t; r < n; r += kMgInvThreads) s_f[r] = A[r * n + c];
+    __syncthreads();
+    // pivot row: A[c][c] = 1, then scaled by 1 / d
+    for(int k = t; k < n; k += kMgInvThreads) {
+      const float2 x = k == c ? make_float2(1.f, 0.f) : A[c * n + k];
+      A[c * n + k] = make_float2(x.x * inv.x - x.y * inv.y, x.x * inv.y + x.y * inv.x);
+    }
+    __syncthreads();
+    // every other row: A[r][c] = 0, A[r][:] -= f_r A[c][:]
+    for(int e = t; e < n * n; e += kMgInvThreads) {
+      const int r = e / n, k = e - r * n;
+      if(r == c) continue;
+      const float2 f = s_f[r], y = A[c * n + k];
+      const float2 x = k == c ? make_float2(0.f, 0.f) : A[e];
+      A[e] = make_float2(x.x - (f.x * y.x - f.y * y.y), x.y - (f.x * y.y + f.y * y.x));
+    }
+    __syncthreads();
+  }
+  // undo the row swaps as column swaps, last first, and write column-major
+  for(int c = n - 1; c >= 0; --c) {
+    const int p = s_perm[c];
+    if(p != c)
+      for(int r = t; r < n; r += kMgInvThreads) {
+        const float2 x = A[r * n + c];
+        A[r * n + c] = A[r * n + p];
+        A[r * n + p] = x;
+      }
+    __syncthreads();
+  }
+  for(int e = t; e < n * n; e += kMgInvThreads) {
+    const int r = e / n, c = e - r * n;
+    ainv[(int64_t)c * n + r] = A[e];
+  }
+}
+
 // x += z (the V-cycle of the cycle's correction)
 __global__ void __launch_bounds__(256) bk_mg_addz(const BItem *__restrict__ items, const int *__restrict__ live,
                                                   const int2 *__restrict__ vblk)
@@ -1442,9 +1542,10 @@ gfem_status submit_request(gfem_batch *B, gfem_system *sys, double tol, int max_
                      sys->d_aj, dinv, a32);
   hipLaunchKernelGGL(bk_zero, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, (int64_t)n, I.x);
   if(sys->mg) {
-    // the cycle's descriptor and level operators, in stream order
-    hipLaunchKernelGGL(k_put<MgReq>, dim3(1), dim3(64), 0, s, const_cast<MgReq *>(I.mg), B->mg_host[slot]);
+    // the cycle's level operators, then its descriptor (with the request's
+    // coefficients), in stream order
     GFEM_HIP(ctx, mg_prepare(sys, B->mg_host[slot], a64, s));
+    hipLaunchKernelGGL(k_put<MgReq>, dim3(1), dim3(64), 0, s, const_cast<MgReq *>(I.mg), B->mg_host[slot]);
   }
   // this slot's descriptor only, in stream order (no wait)
   hipLaunchKernelGGL(k_put<BItem>, dim3(1), dim3(64), 0, s, B->d_items + slot, S.item);
@@ -1604,6 +1705,14 @@ gfem_status gfem_batch_step(gfem_batch *B, void *stream, int *n_live)
     }
     st = residual_and_prepare(B, act, s, true);
     if(st != GFEM_OK) return st;
+    // coarsest-level inverses of the fresh requests with a hierarchy, one
+    // block each, all in one launch (act is on the device)
+    bool fresh_mg = false;
+    for(int i = 0; i < ns; ++i) fresh_mg |= act[i] && B->slots[i].item.mg;
+    if(fresh_mg) {
+      hipLaunchKernelGGL(bk_mg_invert, dim3((unsigned)ns), dim3(kMgInvThreads), 0, s, B->d_items, B->d_act);
+      GFEM_HIP(ctx, hipGetLastError());
+    }
   }
   int live = 0;
   for(int i = 0; i < ns; ++i) {

@@ -50,12 +50,26 @@ struct MgLevelReq {
   const float *rw;
 };
 
+// coefficient of each term class of a request, the shifted one times (1 + i beta)
+struct Coefs {
+  int nt;
+  float2 c[kMgMaxTerms];
+};
+
 struct MgReq {
   int nlev;    // levels, the last one solved exactly
   int lsmall;  // first level run by the one-block kernel (levels lsmall .. nlev - 1)
-  const float2 *ainv;  // dense inverse of S on the coarsest level, column-major [n][n]
+  float2 *ainv;  // dense inverse of S on the coarsest level, column-major [n][n]
+  // what the inversion at the request's first batch step reads: the
+  // system's per-class dense coarsest matrices [nterms][n][n] and this
+  // request's coefficients
+  const float2 *gdense;
+  Coefs cf;
   MgLevelReq lev[kMgMaxLevels];
 };
+
+constexpr int kMgInvMax = kMgCoarsest * kMgCoarsest * kMgCoarsest;  // coarsest rows, at most
+constexpr int kMgInvThreads = 1024;
 
 // Hierarchy of a system, device-resident (mg.hip).
 struct MgSystem {
@@ -80,8 +94,11 @@ void mg_free(MgSystem *m);
 size_t mg_request_bytes(const gfem_system *sys);
 // lay out a request's cycle state at `base` (mg_request_bytes bytes) and
 // fill *req; then mg_prepare computes its level operators from the system's
-// current coefficients and the request's assembled FP64 matrix a64
+// current coefficients and the request's assembled FP64 matrix a64 and
+// stores the coefficients in *req (the coarsest inverse is formed at the
+// request's first batch step, for all requests admitted since the last step
+// in one launch)
 void mg_bind(const gfem_system *sys, char *base, MgReq *req);
-hipError_t mg_prepare(const gfem_system *sys, const MgReq &req, const double2 *a64, hipStream_t s);
+hipError_t mg_prepare(const gfem_system *sys, MgReq &req, const double2 *a64, hipStream_t s);
 
 }  // namespace gfem

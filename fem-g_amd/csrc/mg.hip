@@ -253,11 +253,6 @@ hipError_t up(T **d, const T *h, size_t n)
 
 // ------------------------------------------------------------ per-request operators
 
-struct Coefs {
-  int nt;
-  float2 c[kMgMaxTerms];  // coefficient of each term, the shifted one times (1 + i beta)
-};
-
 // level 0: S = A + i beta sum_s c_s X_s (the system's pattern)
 __global__ void k_mg_level0(int64_t nnz, const double2 *__restrict__ a64, const float2 *__restrict__ xs, float2 cs,
                             float2 *__restrict__ out)
@@ -297,101 +292,6 @@ __global__ void k_mg_dinv(int64_t n, const P *__restrict__ ai, const int32_t *__
     }
   const float m = d.x * d.x + d.y * d.y;
   dinv[r] = m > 0.f ? make_float2(kMgOmega * d.x / m, -kMgOmega * d.y / m) : make_float2(0.f, 0.f);
-}
-
-// Inverse of the coarsest level's S = sum_t c_t G_t (dense, n <= kMgCoarsest^3
-// = 125), one block: in-place Gauss-Jordan with partial pivoting on the
-// complex64 matrix held in LDS (row swaps recorded, columns swapped back at
-// the end), written column-major so the cycle's coarse product reads it with
-// consecutive lanes on consecutive rows.
-constexpr int kMgInvMax = kMgCoarsest * kMgCoarsest * kMgCoarsest;
-constexpr int kMgInvThreads = 1024;
-
-__global__ void __launch_bounds__(kMgInvThreads) k_mg_invert(int n, const float2 *__restrict__ gd, Coefs cf,
-                                                             float2 *__restrict__ ainv)
-{
-  __shared__ float2 A[kMgInvMax * kMgInvMax];
-  __shared__ int s_perm[kMgInvMax];
-  __shared__ float s_val[64];
-  __shared__ int s_idx[64];
-  const int t = threadIdx.x;
-  for(int e = t; e < n * n; e += kMgInvThreads) {
-    float re = 0.f, im = 0.f;
-    for(int k = 0; k < cf.nt; ++k) {
-      const float2 g = gd[(int64_t)k * n * n + e], q = cf.c[k];
-      re += q.x * g.x - q.y * g.y;
-      im += q.x * g.y + q.y * g.x;
-    }
-    A[e] = make_float2(re, im);
-  }
-  __syncthreads();
-  for(int c = 0; c < n; ++c) {
-    // pivot: largest |A[r][c]|, r >= c (one wave, smallest index on ties)
-    if(t < 64) {
-      float best = -1.f;
-      int bi = c;
-      for(int r = c + t; r < n; r += 64) {
-        const float2 v = A[r * n + c];
-        const float m = v.x * v.x + v.y * v.y;
-        if(m > best) {
-          best = m;
-          bi = r;
-        }
-      }
-      for(int d = 32; d > 0; d >>= 1) {
-        const float ob = __shfl_down(best, d, 64);
-        const int oi = __shfl_down(bi, d, 64);
-        if(t + d < 64 && (ob > best || (ob == best && oi < bi))) {
-          best = ob;
-          bi = oi;
-        }
-      }
-      if(t == 0) s_perm[c] = bi;
-    }
-    __syncthreads();
-    const int p = s_perm[c];
-    if(p != c)
-      for(int k = t; k < n; k += kMgInvThreads) {
-        const float2 x = A[c * n + k];
-        A[c * n + k] = A[p * n + k];
-        A[p * n + k] = x;
-      }
-    __syncthreads();
-    const float2 d = A[c * n + c];
-    const float dm = d.x * d.x + d.y * d.y;
-    const float2 inv = dm > 0.f ? make_float2(d.x / dm, -d.y / dm) : make_float2(0.f, 0.f);
-    __syncthreads();
-    // pivot row: A[c][c] = 1, then scaled by 1 / d
-    for(int k = t; k < n; k += kMgInvThreads) {
-      const float2 x = k == c ? make_float2(1.f, 0.f) : A[c * n + k];
-      A[c * n + k] = make_float2(x.x * inv.x - x.y * inv.y, x.x * inv.y + x.y * inv.x);
-    }
-    __syncthreads();
-    // every other row: f = A[r][c], A[r][c] = 0, A[r][:] -= f A[c][:]
-    for(int e = t; e < n * n; e += kMgInvThreads) {
-      const int r = e / n, k = e - r * n;
-      if(r == c) continue;
-      const float2 f = A[r * n + c], y = A[c * n + k];
-      const float2 x = k == c ? make_float2(0.f, 0.f) : A[e];
-      A[e] = make_float2(x.x - (f.x * y.x - f.y * y.y), x.y - (f.x * y.y + f.y * y.x));
-    }
-    __syncthreads();
-  }
-  // undo the row swaps as column swaps, last first, and write column-major
-  for(int c = n - 1; c >= 0; --c) {
-    const int p = s_perm[c];
-    if(p != c)
-      for(int r = t; r < n; r += kMgInvThreads) {
-        const float2 x = A[r * n + c];
-        A[r * n + c] = A[r * n + p];
-        A[r * n + p] = x;
-      }
-    __syncthreads();
-  }
-  for(int e = t; e < n * n; e += kMgInvThreads) {
-    const int r = e / n, c = e - r * n;
-    ainv[(int64_t)c * n + r] = A[e];
-  }
 }
 
 size_t a256(size_t b) { return (b + 255) / 256 * 256; }
@@ -476,7 +376,7 @@ void mg_bind(const gfem_system *sys, char *base, MgReq *req)
   req->ainv = reinterpret_cast<float2 *>(take(8 * nc * nc));
 }
 
-hipError_t mg_prepare(const gfem_system *sys, const MgReq &req, const double2 *a64, hipStream_t s)
+hipError_t mg_prepare(const gfem_system *sys, MgReq &req, const double2 *a64, hipStream_t s)
 {
   const MgSystem *m = sys->mg;
   Coefs cf{};
@@ -505,9 +405,8 @@ hipError_t mg_prepare(const gfem_system *sys, const MgReq &req, const double2 *a
     hipLaunchKernelGGL(k_mg_dinv<int32_t>, dim3(blocks(L.n)), dim3(kTB), 0, s, (int64_t)L.n, m->ai[l], m->aj[l], L.a,
                        const_cast<float2 *>(L.dinv));
   }
-  const int nc = m->n[m->nlev - 1];
-  hipLaunchKernelGGL(k_mg_invert, dim3(1), dim3(kMgInvThreads), 0, s, nc, m->gdense, cf,
-                     const_cast<float2 *>(req.ainv));
+  req.gdense = m->gdense;
+  req.cf = cf;
   return hipGetLastError();
 }
 
