@@ -73,7 +73,7 @@ std::vector<Completed> Server::run(const std::vector<Request> &schedule, bool ke
       std::fprintf(stderr, "serve: %.0f s  arrived %zu  admitted %zu  completed %zu  iterating %zu  waiting %zu\n", t,
                    next, nadmitted, out.size(), running.size(), waiting.size());
       std::fflush(stderr);
-      while(report <= t) report += _progress;
+      report = t + _progress;
     }
     // admit in arrival order while a slot and memory are free
     while(!waiting.empty()) {
