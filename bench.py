@@ -469,7 +469,7 @@ def highorder(args):
     return out
 
 
-def main():
+def arg_parser():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
@@ -509,7 +509,11 @@ def main():
                     help="4 slots: the one-wave persistent gather (1) or the 128-thread block (0); same values")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="BYTES",
                     help="run a device-to-device copy of BYTES first (counter calibration)")
-    args = ap.parse_args()
+    return ap
+
+
+def main():
+    args = arg_parser().parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (this build runs on a single "
               f"MI355X; multi-GPU sharding is a design note only)", file=sys.stderr)

@@ -212,3 +212,20 @@ def test_scatter_map_bytes_counts_the_element_maps(n, order):
     only_src = W.helmholtz3d(n=n, order=order, k=5.0)
     only_src.terms = [t for t in only_src.terms if t["kind"] == "linear"]
     assert only_src.scatter_map_bytes() == 0
+
+
+def test_bench_defaults_measure_the_stated_workloads():
+    """bench.py with no flags: one GPU, 20 timed assemblies after 3 warm-up
+    steps, and the serving leg on the whole stated stream (no request cap,
+    200 arrivals/s) with the multigrid preconditioner at shift 1, GMRES(30)"""
+    import importlib.util
+    import os
+    spec = importlib.util.spec_from_file_location("bench_module", os.path.join(os.path.dirname(__file__), "..",
+                                                                               "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    a = bench.arg_parser().parse_args([])
+    assert (a.gpus, a.steps, a.warmup) == (1, 20, 3)
+    assert a.serve_requests is None and a.serve_rate == 200.0
+    assert a.serve_mg == 1 and a.serve_mg_beta == 1.0 and a.serve_restart == 30
+    assert a.batch_instances == 256 and a.batch_tol == 1e-8
