@@ -143,7 +143,7 @@ def serving(args):
         F = W.build(W.helmholtz3d(n=n, order=order, k=10.0))
         F.pre()
         Fs.append(F)
-    sv = Server(restart=args.serve_restart, hbm_bytes=args.serve_hbm_gb << 30, max_requests=256)
+    sv = Server(restart=args.serve_restart, hbm_bytes=args.serve_hbm_gb << 30, max_requests=args.serve_slots)
     for F in Fs:
         sv.add_mesh(F)
     log(f"serving: {len(Fs)} meshes resident ({time.perf_counter() - t0:.1f} s)")
@@ -175,7 +175,7 @@ def serving(args):
     st = sv.stats
     res = {
         "requests": len(out), "arrival_rate_per_s": args.serve_rate, "tol": args.serve_tol,
-        "restart": args.serve_restart,
+        "restart": args.serve_restart, "slots": args.serve_slots, "hbm_budget_GB": args.serve_hbm_gb,
         "mesh_dofs": [int(F.n_unknown) for F in Fs],
         "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
         "solves_per_s": len(out) / st["seconds"], "seconds": st["seconds"],
@@ -495,6 +495,7 @@ def arg_parser():
     ap.add_argument("--serve-tol", type=float, default=1e-10)
     ap.add_argument("--serve-hbm-gb", type=int, default=64)
     ap.add_argument("--serve-restart", type=int, default=30, help="GMRES restart length of the serving stream")
+    ap.add_argument("--serve-slots", type=int, default=256, help="requests iterating together, at most")
     ap.add_argument("--serve-baseline", type=int, default=16, help="requests timed one at a time")
     ap.add_argument("--serve-mg", type=int, choices=[0, 1], default=1,
                     help="shifted-Laplacian multigrid preconditioner on the serving meshes (1) or row scaling (0)")
